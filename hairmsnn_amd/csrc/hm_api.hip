@@ -1,0 +1,657 @@
+// C ABI (include/hmsnn.h): library plumbing, scene contexts, path tracing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+
+#include "../../include/hmsnn.h"
+#include "hm_api_util.h"
+#include "hm_context.h"
+
+namespace hm {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int bsdf_eval_launch(const HairConsts& hc, const void* q, int n, float* f, float* pdf, hipStream_t s,
+                     std::string& err);
+int bsdf_sample_launch(const HairConsts& hc, const void* q, const float* u4, int n, float* wi, float* f,
+                       float* pdf, int* lobe, hipStream_t s, std::string& err);
+
+static HairConsts consts_from(const hmsnn_hair_params* p) {
+  return make_hair_consts(p->beta_m, p->beta_n, p->alpha, p->gain);
+}
+
+// Pinhole camera (render_path_tracing.cu:767-799): the image plane spans
+// cos_fovy vertically at unit distance; pixel row 0 is at the bottom.
+void camera_vectors(const Scene& scene, V3& pos, V3& d00_out, V3& du, V3& dv) {
+  const SceneCamera& cam = scene.camera;
+  pos = cam.from;
+  V3 d00 = normalize(cam.at - cam.from);
+  const float aspect = (float)scene.width / (float)scene.height;
+  du = cam.cos_fovy * aspect * normalize(cross(d00, cam.up));
+  dv = cam.cos_fovy * normalize(cross(du, d00));
+  d00 = d00 - 0.5f * du;
+  d00 = d00 - 0.5f * dv;
+  d00_out = d00;
+}
+
+// Scene-derived fields of hmsnn_info (no generated content).
+static void fill_scene_info(const Scene& s, hmsnn_info* info) {
+  info->width = s.width;
+  info->height = s.height;
+  info->spp = s.spp;
+  info->path_v1 = s.path_v1;
+  info->path_v2 = s.path_v2;
+  info->num_dir_lights = (int)s.dlight_from.size();
+  info->has_env = s.has_env;
+  info->has_sphere = s.has_surface && s.surface_sphere;
+  info->mis = s.mis;
+  info->env_pdf = s.env_pdf;
+  info->sigma_a[0] = s.sig_a.x;
+  info->sigma_a[1] = s.sig_a.y;
+  info->sigma_a[2] = s.sig_a.z;
+  info->beta_m = s.beta_m;
+  info->beta_n = s.beta_n;
+  info->alpha = s.alpha;
+  info->gain[0] = s.R_G;
+  info->gain[1] = s.TT_G;
+  info->gain[2] = s.TRT_G;
+  info->gain[3] = s.TRRT_G;
+  info->env_scale = s.env_scale;
+  info->env_rot_phi = s.env_rot_phi;
+  info->sphere[0] = s.sphere_center.x;
+  info->sphere[1] = s.sphere_center.y;
+  info->sphere[2] = s.sphere_center.z;
+  info->sphere[3] = s.sphere_radius;
+  info->sphere_albedo[0] = s.surface_albedo.x;
+  info->sphere_albedo[1] = s.surface_albedo.y;
+  info->sphere_albedo[2] = s.surface_albedo.z;
+  const float a = std::min(std::max(s.surface_alpha, 0.01f), 1.f);
+  info->sphere_alpha = a * a;
+  V3 cam[4];
+  camera_vectors(s, cam[0], cam[1], cam[2], cam[3]);
+  for (int k = 0; k < 4; ++k) {
+    info->camera[3 * k] = cam[k].x;
+    info->camera[3 * k + 1] = cam[k].y;
+    info->camera[3 * k + 2] = cam[k].z;
+  }
+  for (int i = 0; i < (int)s.dlight_from.size() && i < 8; ++i) {
+    info->dir_from[i][0] = s.dlight_from[i].x;
+    info->dir_from[i][1] = s.dlight_from[i].y;
+    info->dir_from[i][2] = s.dlight_from[i].z;
+    info->dir_emit[i][0] = s.dlight_emit[i].x;
+    info->dir_emit[i][1] = s.dlight_emit[i].y;
+    info->dir_emit[i][2] = s.dlight_emit[i].z;
+  }
+  if (s.has_hair && s.hair_procedural) {
+    info->num_strands = s.hair_gen.strands;
+    info->num_segments = s.hair_gen.strands * s.hair_gen.segments;
+  }
+}
+
+Context::~Context() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  msnn_release(msnn);
+  msnn = nullptr;
+  free_path_buffers();
+  dev_free(seg_dev);
+  dev_free(strand_sigma_dev);
+  dev_free(stats_dev);
+  bvh.release();
+  env.release();
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Context::free_path_buffers() {
+  for (void* p : pb_allocs) dev_free(p);
+  pb_allocs.clear();
+  pb = PathBuffers{};
+  dev_free(accum);
+  dev_free(average);
+  accum = average = nullptr;
+}
+
+int Context::ensure_path_buffers(bool msnn_mode, std::string& err) {
+  if (pb.ray_o && (!msnn_mode || pb_msnn)) return kOk;
+  free_path_buffers();
+  const size_t n = (size_t)scene.width * scene.height;
+  auto A = [&](void** p, size_t bytes) -> int {
+    HM_TRY(dev_alloc(p, bytes, err));
+    pb_allocs.push_back(*p);
+    return kOk;
+  };
+  const size_t f4 = sizeof(float4) * n;
+  HM_TRY(A((void**)&pb.ray_o, f4));
+  HM_TRY(A((void**)&pb.ray_d, f4));
+  HM_TRY(A((void**)&pb.beta, f4));
+  HM_TRY(A((void**)&pb.rng, sizeof(uint32_t) * n));
+  HM_TRY(A((void**)&pb.L, f4));
+  HM_TRY(A((void**)&pb.flags, sizeof(uint32_t) * n));
+  HM_TRY(A((void**)&pb.hit_t, sizeof(float) * n));
+  HM_TRY(A((void**)&pb.hit_prim, sizeof(int) * n));
+  HM_TRY(A((void**)&pb.hit_u, sizeof(float) * n));
+  HM_TRY(A((void**)&pb.sh_o1, f4));
+  HM_TRY(A((void**)&pb.sh_d1, f4));
+  HM_TRY(A((void**)&pb.sh_c1, f4));
+  HM_TRY(A((void**)&pb.sh_o2, f4));
+  HM_TRY(A((void**)&pb.sh_d2, f4));
+  HM_TRY(A((void**)&pb.sh_c2, f4));
+  HM_TRY(A((void**)&pb.sh_beta, f4));
+  HM_TRY(A((void**)&pb.queue[0], sizeof(int) * n));
+  HM_TRY(A((void**)&pb.queue[1], sizeof(int) * n));
+  HM_TRY(A((void**)&pb.shadow_queue, sizeof(int) * n));
+  HM_TRY(A((void**)&pb.counts, sizeof(int) * 16));
+  if (msnn_mode) {
+    HM_TRY(A((void**)&pb.beta_short, f4));
+    HM_TRY(A((void**)&pb.L_short, f4));
+    HM_TRY(A((void**)&pb.sh_beta_short, f4));
+  }
+  pb_msnn = msnn_mode;
+  // traversal stack spill: levels beyond the LDS part, one column per thread
+  spill_levels = std::max(1, bvh.max_depth - kStackLds + 2);
+  pb.spill_stride = trace_blocks * kTraceBlock;
+  pb.spill_levels = spill_levels;
+  HM_TRY(A((void**)&pb.spill, sizeof(int) * (size_t)spill_levels * pb.spill_stride));
+  HM_TRY(dev_alloc((void**)&accum, f4, err));
+  HM_TRY(dev_alloc((void**)&average, f4, err));
+  HM_HIP(hipMemsetAsync(pb.counts, 0, sizeof(int) * 16, stream), err);
+  HM_HIP(hipMemsetAsync(accum, 0, f4, stream), err);
+  HM_HIP(hipMemsetAsync(average, 0, f4, stream), err);
+  accum_id = 0;
+  return kOk;
+}
+
+PassConfig Context::pt_pass_config() const {
+  PassConfig c;
+  c.accum_id = accum_id;
+  const int v1 = scene.path_v1 - 1, v2 = scene.path_v2 - 1;
+  c.dl_zero = (v1 == 0);
+  c.dl_from = v1;
+  c.max_vertex = v2;
+  if (v2 < v1) {  // pathTrace returns black (optix_common.cuh:365-366)
+    c.dl_zero = 0;
+    c.dl_from = 1 << 30;
+  }
+  c.trace_blocks = trace_blocks;
+  c.shade_blocks = trace_blocks;
+  return c;
+}
+
+int Context::init(std::string& err) {
+  HM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), err);
+  int dev = 0;
+  HM_HIP(hipGetDevice(&dev), err);
+  hipDeviceProp_t prop;
+  HM_HIP(hipGetDeviceProperties(&prop, dev), err);
+  num_cus = prop.multiProcessorCount;
+  trace_blocks = std::max(64, num_cus * 8);
+
+  // ---- geometry ----
+  if (scene.has_surface && !scene.surface_sphere) {
+    err = "surface.geometry '" + scene.surface_geometry +
+          "': OBJ meshes are not supported; use surface.sphere {center, radius}";
+    return kEUnsupported;
+  }
+  if (scene.has_hair && !scene.hair_procedural) {
+    err = "hair.geometry '" + scene.hair_geometry +
+          "': .hair files are not supported; use hair.procedural {...}";
+    return kEUnsupported;
+  }
+  if (scene.has_env && !scene.env_constant && !scene.env_procedural) {
+    err = "lights.environment.exr '" + scene.env_exr +
+          "': EXR maps are not supported; use environment.constant or environment.procedural";
+    return kEUnsupported;
+  }
+  if ((int)scene.dlight_from.size() > kMaxDirLights) {
+    err = "at most " + std::to_string(kMaxDirLights) + " directional lights are supported";
+    return kEUnsupported;
+  }
+  if (!scene.has_env && scene.dlight_from.empty()) {
+    err = "the scene has no light: sampling would read an undefined environment";
+    return kEInval;
+  }
+  bmin = v3(1e30f);
+  bmax = v3(-1e30f);
+  float sscale = 0.f;
+  if (scene.has_surface) {
+    const V3 c = scene.sphere_center;
+    const float r = scene.sphere_radius;
+    bmin = vmin(bmin, c - v3(r));
+    bmax = vmax(bmax, c + v3(r));
+    sscale = length(bmax - bmin);
+  }
+  if (scene.has_hair) {
+    generate_hair(scene.hair_gen, hair);
+    bmin = vmin(bmin, hair.bmin);
+    bmax = vmax(bmax, hair.bmax);
+    sscale = std::max(sscale, length(hair.bmax - hair.bmin));  // render_path_tracing.cu:212-214
+    const size_t bytes = sizeof(float) * hair.seg.size();
+    HM_TRY(dev_alloc((void**)&seg_dev, bytes, err));
+    HM_HIP(hipMemcpyAsync(seg_dev, hair.seg.data(), bytes, hipMemcpyHostToDevice, stream), err);
+    HM_TRY(bvh_build(seg_dev, (int)(hair.seg.size() / 8), bvh, stream, err));
+  }
+  scene_scale = sscale > 0.f ? sscale : 1.f;
+
+  // ---- environment ----
+  if (scene.has_env) {
+    generate_env(scene, env_host);
+    env.width = env_host.width;
+    env.height = env_host.height;
+    const size_t bytes = sizeof(float) * env_host.rgba.size();
+    HM_TRY(dev_alloc((void**)&env.tex, bytes, err));
+    HM_HIP(hipMemcpyAsync(env.tex, env_host.rgba.data(), bytes, hipMemcpyHostToDevice, stream), err);
+    HM_TRY(env_build_tables(env, stream, err));
+  }
+
+  camera_vectors(scene, cam_pos, cam_d00, cam_du, cam_dv);
+
+  // ---- view ----
+  SceneView& v = view;
+  v = SceneView{};
+  v.bvh.nodes = bvh.nodes;
+  v.bvh.segs = bvh.segs;
+  v.bvh.num_segs = bvh.num_segs;
+  v.seg_by_id = seg_dev;
+  v.strand_sigma = nullptr;
+  v.sphere.enabled = scene.has_surface ? 1 : 0;
+  v.sphere.cx = scene.sphere_center.x;
+  v.sphere.cy = scene.sphere_center.y;
+  v.sphere.cz = scene.sphere_center.z;
+  v.sphere.r = scene.sphere_radius;
+  v.sphere_albedo = scene.surface_albedo;
+  const float a = std::min(std::max(scene.surface_alpha, 0.01f), 1.f);
+  v.sphere_alpha = a * a;
+  v.env.enabled = scene.has_env ? 1 : 0;
+  v.env.pdf_sampling = scene.env_pdf ? 1 : 0;
+  v.env.width = env.width;
+  v.env.height = env.height;
+  v.env.scale = scene.env_scale;
+  v.env.rot_phi = scene.env_rot_phi;
+  v.env.tex = env.tex;
+  v.env.c_pdf = env.c_pdf;
+  v.env.c_cdf = env.c_cdf;
+  v.env.m_pdf = env.m_pdf;
+  v.env.m_cdf = env.m_cdf;
+  v.num_dlights = (int)scene.dlight_from.size();
+  v.num_total_lights = v.num_dlights + (scene.has_env ? 1 : 0);
+  for (int i = 0; i < v.num_dlights; ++i) {
+    v.dl_from[i] = scene.dlight_from[i];
+    v.dl_emit[i] = scene.dlight_emit[i];
+  }
+  v.cam_pos = cam_pos;
+  v.cam_d00 = cam_d00;
+  v.cam_du = cam_du;
+  v.cam_dv = cam_dv;
+  v.width = scene.width;
+  v.height = scene.height;
+  const float gains[4] = {scene.R_G, scene.TT_G, scene.TRT_G, scene.TRRT_G};
+  v.hc = make_hair_consts(scene.beta_m, scene.beta_n, scene.alpha, gains);
+  v.sigma_a = scene.sig_a;
+  v.mis = scene.mis ? 1 : 0;
+  HM_TRY(dev_alloc((void**)&stats_dev, sizeof(unsigned long long) * 8, err));
+  HM_HIP(hipStreamSynchronize(stream), err);
+  return kOk;
+}
+
+}  // namespace hm
+
+using namespace hm;
+
+struct hmsnn_ctx {
+  Context c;
+};
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* hmsnn_last_error(void) { return g_last_error.c_str(); }
+
+int hmsnn_version(void) { return 10000; }
+
+int hmsnn_device_count(int* count, char* name, int name_len) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) {
+    if (count) *count = 0;
+    return fail(HMSNN_EDEVICE, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+  }
+  if (count) *count = n;
+  if (name && name_len > 0) {
+    name[0] = 0;
+    if (n > 0) {
+      hipDeviceProp_t p;
+      if (hipGetDeviceProperties(&p, 0) == hipSuccess) {
+        std::snprintf(name, (size_t)name_len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+      }
+    }
+  }
+  return HMSNN_OK;
+}
+
+int hmsnn_dev_alloc(size_t bytes, void** ptr) {
+  if (!ptr) return fail(HMSNN_EINVAL, "ptr is NULL");
+  std::string err;
+  int rc = dev_alloc(ptr, bytes, err);
+  return rc ? fail(rc, err) : HMSNN_OK;
+}
+int hmsnn_dev_free(void* ptr) {
+  if (ptr) {
+    hipError_t e = hipFree(ptr);
+    if (e != hipSuccess) return fail(HMSNN_EDEVICE, std::string("hipFree: ") + hipGetErrorString(e));
+  }
+  return HMSNN_OK;
+}
+int hmsnn_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  return e == hipSuccess ? HMSNN_OK : fail(HMSNN_EDEVICE, std::string("hipMemcpy H2D: ") + hipGetErrorString(e));
+}
+int hmsnn_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? HMSNN_OK : fail(HMSNN_EDEVICE, std::string("hipMemcpy D2H: ") + hipGetErrorString(e));
+}
+int hmsnn_dev_synchronize(void) {
+  hipError_t e = hipDeviceSynchronize();
+  return e == hipSuccess ? HMSNN_OK : fail(HMSNN_EDEVICE, std::string("hipDeviceSynchronize: ") + hipGetErrorString(e));
+}
+
+int hmsnn_hair_bsdf_eval_dev(const hmsnn_hair_params* params, const hmsnn_bsdf_query* q, int n, float* f,
+                             float* pdf, void* stream) {
+  if (!params || n < 0 || (n > 0 && (!q || !f || !pdf))) return fail(HMSNN_EINVAL, "bsdf eval: bad arguments");
+  std::string err;
+  int rc = bsdf_eval_launch(consts_from(params), q, n, f, pdf, (hipStream_t)stream, err);
+  return rc ? fail(rc, err) : HMSNN_OK;
+}
+
+int hmsnn_hair_bsdf_sample_dev(const hmsnn_hair_params* params, const hmsnn_bsdf_query* q, const float* u4, int n,
+                               float* wi, float* f, float* pdf, int* lobe, void* stream) {
+  if (!params || n < 0 || (n > 0 && (!q || !u4 || !wi || !f || !pdf)))
+    return fail(HMSNN_EINVAL, "bsdf sample: bad arguments");
+  std::string err;
+  int rc = bsdf_sample_launch(consts_from(params), q, u4, n, wi, f, pdf, lobe, (hipStream_t)stream, err);
+  return rc ? fail(rc, err) : HMSNN_OK;
+}
+
+static int create_common(Scene&& scene, hmsnn_ctx** out) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(HMSNN_EDEVICE, "no GPU visible");
+  hmsnn_ctx* c = new hmsnn_ctx();
+  c->c.scene = std::move(scene);
+  std::string err;
+  int rc = c->c.init(err);
+  if (rc) {
+    delete c;
+    return fail(rc, err);
+  }
+  *out = c;
+  return HMSNN_OK;
+}
+
+int hmsnn_create(const char* path, hmsnn_ctx** out) {
+  if (!path || !out) return fail(HMSNN_EINVAL, "hmsnn_create: NULL argument");
+  *out = nullptr;
+  Scene s;
+  std::string err;
+  if (!parse_scene_file(path, s, err)) {
+    const bool io = err.find("cannot open") != std::string::npos;
+    return fail(io ? HMSNN_EIO : HMSNN_EINVAL, err);
+  }
+  return create_common(std::move(s), out);
+}
+
+int hmsnn_create_from_json(const char* text, const char* base_dir, hmsnn_ctx** out) {
+  if (!text || !out) return fail(HMSNN_EINVAL, "hmsnn_create_from_json: NULL argument");
+  *out = nullptr;
+  Scene s;
+  std::string err;
+  if (!parse_scene_text(text, base_dir ? base_dir : ".", s, err)) return fail(HMSNN_EINVAL, err);
+  return create_common(std::move(s), out);
+}
+
+int hmsnn_destroy(hmsnn_ctx* ctx) {
+  delete ctx;
+  return HMSNN_OK;
+}
+
+int hmsnn_get_info(const hmsnn_ctx* ctx, hmsnn_info* info) {
+  if (!ctx || !info) return fail(HMSNN_EINVAL, "hmsnn_get_info: NULL argument");
+  const Context& c = ctx->c;
+  std::memset(info, 0, sizeof(*info));
+  fill_scene_info(c.scene, info);
+  info->num_segments = c.bvh.num_segs;
+  info->num_strands = c.hair.num_strands;
+  info->bvh_nodes = c.bvh.num_nodes;
+  info->bvh_depth = c.bvh.max_depth;
+  info->env_width = c.env.width;
+  info->env_height = c.env.height;
+  info->scene_scale = c.scene_scale;
+  return HMSNN_OK;
+}
+
+int hmsnn_parse_scene(const char* text, hmsnn_info* info) {
+  if (!text || !info) return fail(HMSNN_EINVAL, "hmsnn_parse_scene: NULL argument");
+  Scene s;
+  std::string err;
+  if (!parse_scene_text(text, ".", s, err)) return fail(HMSNN_EINVAL, err);
+  std::memset(info, 0, sizeof(*info));
+  fill_scene_info(s, info);
+  return HMSNN_OK;
+}
+
+int hmsnn_export_segments(const hmsnn_ctx* ctx, float* seg_host) {
+  if (!ctx || !seg_host) return fail(HMSNN_EINVAL, "hmsnn_export_segments: NULL argument");
+  std::memcpy(seg_host, ctx->c.hair.seg.data(), sizeof(float) * ctx->c.hair.seg.size());
+  return HMSNN_OK;
+}
+
+int hmsnn_export_env(const hmsnn_ctx* ctx, float* rgba, float* c_pdf, float* c_cdf, float* m_pdf, float* m_cdf) {
+  if (!ctx) return fail(HMSNN_EINVAL, "hmsnn_export_env: NULL context");
+  const Context& c = ctx->c;
+  if (!c.scene.has_env) return fail(HMSNN_EINVAL, "scene has no environment light");
+  const size_t W = c.env.width, H = c.env.height;
+  auto cp = [&](float* dst, const float* src, size_t n) -> int {
+    if (!dst) return HMSNN_OK;
+    hipError_t e = hipMemcpy(dst, src, sizeof(float) * n, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? HMSNN_OK : fail(HMSNN_EDEVICE, std::string("export env: ") + hipGetErrorString(e));
+  };
+  HM_TRY(cp(rgba, (const float*)c.env.tex, W * H * 4));
+  HM_TRY(cp(c_pdf, c.env.c_pdf, (W + 1) * H));
+  HM_TRY(cp(c_cdf, c.env.c_cdf, (W + 1) * H));
+  HM_TRY(cp(m_pdf, c.env.m_pdf, H + 1));
+  HM_TRY(cp(m_cdf, c.env.m_cdf, H + 1));
+  return HMSNN_OK;
+}
+
+int hmsnn_set_strand_sigma_a(hmsnn_ctx* ctx, const float* sigma, int num_strands) {
+  if (!ctx) return fail(HMSNN_EINVAL, "hmsnn_set_strand_sigma_a: NULL context");
+  Context& c = ctx->c;
+  if (!sigma) {
+    c.view.strand_sigma = nullptr;
+    return HMSNN_OK;
+  }
+  if (num_strands != c.hair.num_strands)
+    return fail(HMSNN_EINVAL, "sigma_a array must hold num_strands x 3 floats");
+  std::string err;
+  if (!c.strand_sigma_dev) {
+    int rc = dev_alloc((void**)&c.strand_sigma_dev, sizeof(float) * 3 * (size_t)num_strands, err);
+    if (rc) return fail(rc, err);
+  }
+  hipError_t e = hipMemcpyAsync(c.strand_sigma_dev, sigma, sizeof(float) * 3 * (size_t)num_strands,
+                                hipMemcpyHostToDevice, c.stream);
+  if (e != hipSuccess) return fail(HMSNN_EDEVICE, hipGetErrorString(e));
+  (void)hipStreamSynchronize(c.stream);
+  c.view.strand_sigma = c.strand_sigma_dev;
+  return HMSNN_OK;
+}
+
+int hmsnn_pt_reset(hmsnn_ctx* ctx) {
+  if (!ctx) return fail(HMSNN_EINVAL, "NULL context");
+  ctx->c.accum_id = 0;
+  return HMSNN_OK;
+}
+
+static int run_pt_passes(Context& c, int passes, std::string& err) {
+  HM_TRY(c.ensure_path_buffers(false, err));
+  SceneView v = c.view;
+  v.bvh.nodes = c.bvh.nodes;
+  c.pb.spill_stride = c.trace_blocks * kTraceBlock;
+  const int n = c.scene.width * c.scene.height;
+  for (int k = 0; k < passes; ++k) {
+    PassConfig cfg = c.pt_pass_config();
+    HM_TRY(pt_run_pass(v, c.pb, cfg, c.stream, err));
+    HM_TRY(pt_accumulate(c.pb.L, c.accum, c.average, n, c.accum_id, c.stream, err));
+    c.accum_id++;
+  }
+  return kOk;
+}
+
+int hmsnn_pt_passes(hmsnn_ctx* ctx, int passes) {
+  if (!ctx || passes < 0) return fail(HMSNN_EINVAL, "hmsnn_pt_passes: bad arguments");
+  std::string err;
+  int rc = run_pt_passes(ctx->c, passes, err);
+  return rc ? fail(rc, err) : HMSNN_OK;
+}
+
+int hmsnn_ctx_synchronize(hmsnn_ctx* ctx) {
+  if (!ctx) return fail(HMSNN_EINVAL, "NULL context");
+  hipError_t e = hipStreamSynchronize(ctx->c.stream);
+  return e == hipSuccess ? HMSNN_OK : fail(HMSNN_EDEVICE, std::string("stream sync: ") + hipGetErrorString(e));
+}
+
+int hmsnn_pt_accum_count(const hmsnn_ctx* ctx, int* count) {
+  if (!ctx || !count) return fail(HMSNN_EINVAL, "NULL argument");
+  *count = ctx->c.accum_id;
+  return HMSNN_OK;
+}
+
+int hmsnn_get_image(hmsnn_ctx* ctx, int which, float* rgba) {
+  if (!ctx || !rgba || which < 0 || which > 1) return fail(HMSNN_EINVAL, "hmsnn_get_image: bad arguments");
+  Context& c = ctx->c;
+  if (!c.accum) return fail(HMSNN_EINVAL, "nothing rendered yet");
+  const size_t bytes = sizeof(float4) * (size_t)c.scene.width * c.scene.height;
+  hipError_t e = hipMemcpyAsync(rgba, which == 0 ? c.average : c.accum, bytes, hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  return e == hipSuccess ? HMSNN_OK : fail(HMSNN_EDEVICE, std::string("get image: ") + hipGetErrorString(e));
+}
+
+int hmsnn_get_last_pass(hmsnn_ctx* ctx, float* rgba) {
+  if (!ctx || !rgba) return fail(HMSNN_EINVAL, "hmsnn_get_last_pass: bad arguments");
+  Context& c = ctx->c;
+  if (!c.pb.L) return fail(HMSNN_EINVAL, "nothing rendered yet");
+  const size_t bytes = sizeof(float4) * (size_t)c.scene.width * c.scene.height;
+  hipError_t e = hipMemcpyAsync(rgba, c.pb.L, bytes, hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  return e == hipSuccess ? HMSNN_OK : fail(HMSNN_EDEVICE, std::string("get last pass: ") + hipGetErrorString(e));
+}
+
+int hmsnn_render_path_tracing(hmsnn_ctx* ctx, int spp, float* rgba) {
+  if (!ctx || spp <= 0) return fail(HMSNN_EINVAL, "hmsnn_render_path_tracing: bad arguments");
+  ctx->c.accum_id = 0;
+  std::string err;
+  int rc = run_pt_passes(ctx->c, spp, err);
+  if (rc) return fail(rc, err);
+  if (rgba) return hmsnn_get_image(ctx, 0, rgba);
+  return hmsnn_ctx_synchronize(ctx);
+}
+
+int hmsnn_pt_trace_stats(hmsnn_ctx* ctx, int passes, double out[6]) {
+  if (!ctx || passes <= 0 || !out) return fail(HMSNN_EINVAL, "hmsnn_pt_trace_stats: bad arguments");
+  Context& c = ctx->c;
+  std::string err;
+  int rc = c.ensure_path_buffers(false, err);
+  if (rc) return fail(rc, err);
+  hipError_t e = hipMemsetAsync(c.stats_dev, 0, sizeof(unsigned long long) * 8, c.stream);
+  if (e != hipSuccess) return fail(HMSNN_EDEVICE, hipGetErrorString(e));
+  for (int p = 0; p < passes; ++p) {
+    PassConfig cfg = c.pt_pass_config();
+    cfg.accum_id = c.accum_id + p;
+    cfg.stats = c.stats_dev;
+    rc = pt_run_pass(c.view, c.pb, cfg, c.stream, err);
+    if (rc) return fail(rc, err);
+  }
+  unsigned long long h[8];
+  e = hipMemcpyAsync(h, c.stats_dev, sizeof(h), hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  if (e != hipSuccess) return fail(HMSNN_EDEVICE, hipGetErrorString(e));
+  out[0] = (double)h[4];
+  out[1] = (double)h[0];
+  out[2] = (double)h[1];
+  out[3] = (double)h[5];
+  out[4] = (double)h[2];
+  out[5] = (double)h[3];
+  return HMSNN_OK;
+}
+
+int hmsnn_pt_time_passes(hmsnn_ctx* ctx, int passes, double out[5]) {
+  if (!ctx || passes <= 0 || !out) return fail(HMSNN_EINVAL, "hmsnn_pt_time_passes: bad arguments");
+  Context& c = ctx->c;
+  std::string err;
+  int rc = c.ensure_path_buffers(false, err);
+  if (rc) return fail(rc, err);
+  const int per_pass = 3 * (c.scene.path_v2 + 1);
+  PassTimer tm;
+  tm.capacity = per_pass * passes;
+  std::vector<hipEvent_t> ev(2 * (size_t)tm.capacity);
+  std::vector<int> kinds((size_t)tm.capacity);
+  for (auto& x : ev) (void)hipEventCreate(&x);
+  tm.events = ev.data();
+  tm.kinds = kinds.data();
+  hipEvent_t t0, t1;
+  (void)hipEventCreate(&t0);
+  (void)hipEventCreate(&t1);
+  (void)hipEventRecord(t0, c.stream);
+  const int n = c.scene.width * c.scene.height;
+  for (int p = 0; p < passes && rc == 0; ++p) {
+    PassConfig cfg = c.pt_pass_config();
+    cfg.timer = &tm;
+    rc = pt_run_pass(c.view, c.pb, cfg, c.stream, err);
+    if (rc == 0) rc = pt_accumulate(c.pb.L, c.accum, c.average, n, c.accum_id, c.stream, err);
+    c.accum_id++;
+  }
+  (void)hipEventRecord(t1, c.stream);
+  hipError_t e = hipEventSynchronize(t1);
+  double sums[3] = {0, 0, 0};
+  int launches = 0;
+  if (rc == 0 && e == hipSuccess) {
+    for (int k = 0; k < tm.used; ++k) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]);
+      if (kinds[k] >= 0 && kinds[k] < 3) sums[kinds[k]] += ms;
+      if (kinds[k] == 0) launches++;
+    }
+    float tot = 0.f;
+    (void)hipEventElapsedTime(&tot, t0, t1);
+    out[0] = tot;
+    out[1] = sums[0];
+    out[2] = sums[1];
+    out[3] = sums[2];
+    out[4] = launches;
+  }
+  for (auto& x : ev) (void)hipEventDestroy(x);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  if (rc) return fail(rc, err);
+  if (e != hipSuccess) return fail(HMSNN_EDEVICE, hipGetErrorString(e));
+  return HMSNN_OK;
+}
+
+int hmsnn_trace_rays_dev(hmsnn_ctx* ctx, const float* rays, int n, int any_hit, int* prim, float* t, float* u) {
+  if (!ctx || n < 0 || (n > 0 && (!rays || !prim))) return fail(HMSNN_EINVAL, "hmsnn_trace_rays_dev: bad arguments");
+  Context& c = ctx->c;
+  std::string err;
+  int rc = c.ensure_path_buffers(false, err);
+  if (rc) return fail(rc, err);
+  rc = trace_rays(c.view, rays, n, any_hit, prim, t, u, c.pb.spill, c.pb.spill_stride, c.pb.spill_levels,
+                  c.trace_blocks, nullptr, c.stream, err);
+  if (rc) return fail(rc, err);
+  hipError_t e = hipStreamSynchronize(c.stream);
+  return e == hipSuccess ? HMSNN_OK : fail(HMSNN_EDEVICE, std::string("trace rays: ") + hipGetErrorString(e));
+}
+
+}  // extern "C"

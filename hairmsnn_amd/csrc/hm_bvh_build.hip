@@ -1,0 +1,304 @@
+// LBVH build over hair segments, on the GPU:
+//   1. segment boxes and centroid bounds
+//   2. 30-bit Morton codes of the centroids, radix sort (rocPRIM)
+//   3. Karras (2012) radix-tree hierarchy, one thread per internal node
+//   4. bottom-up box refit with per-node arrival counters
+//   5. emit 64-byte traversal nodes and Morton-ordered segment records
+// This replaces the OptiX curve acceleration structure the reference builds
+// with owlCurvesGeomGroupCreate + owlGroupBuildAccel
+// (render_path_tracing.cu:236-256, render_hair_msnn.cu:370-413).
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "hm_bvh.h"
+#include "hm_internal.h"
+
+namespace hm {
+
+namespace {
+
+__device__ __forceinline__ unsigned expand_bits10(unsigned v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+// order-preserving float <-> int mapping for atomic min/max
+__device__ __forceinline__ int f2ord(float f) {
+  int i = __float_as_int(f);
+  return i >= 0 ? i : i ^ 0x7FFFFFFF;
+}
+__device__ __forceinline__ float ord2f(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7FFFFFFF); }
+
+struct BuildBuffers {
+  const float* seg_in;  // 8 floats per segment
+  int n;
+  float* leaf_box;      // 6 floats per segment, padded
+  int* cbounds;         // 6 ordered ints: min xyz, max xyz
+  unsigned* keys;
+  int* vals;
+  unsigned* keys_sorted;
+  int* vals_sorted;
+  int* child;           // 2 per internal node (encoded)
+  int* parent_internal; // per internal node
+  int* parent_leaf;     // per leaf (sorted position)
+  float* node_box;      // 6 floats per internal node
+  int* arrive;          // per internal node
+  int* max_depth;
+};
+
+__global__ void k_leaf_boxes(BuildBuffers b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int lmin[3] = {INT_MAX, INT_MAX, INT_MAX}, lmax[3] = {INT_MIN, INT_MIN, INT_MIN};
+  if (i < b.n) {
+    const float* s = b.seg_in + 8 * (size_t)i;
+    const float r = s[3];
+    float lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+      const float a = s[k], c = s[4 + k];
+      const float pad = r + 1e-6f * (fabsf(a) + fabsf(c) + r) + 1e-7f;
+      lo[k] = fminf(a, c) - pad;
+      hi[k] = fmaxf(a, c) + pad;
+      b.leaf_box[6 * (size_t)i + k] = lo[k];
+      b.leaf_box[6 * (size_t)i + 3 + k] = hi[k];
+      const float cen = 0.5f * (a + c);
+      lmin[k] = f2ord(cen);
+      lmax[k] = f2ord(cen);
+    }
+  }
+  // wave reduction then one atomic per wave
+  for (int k = 0; k < 3; ++k) {
+    int vmin = lmin[k], vmax = lmax[k];
+    for (int off = 32; off > 0; off >>= 1) {
+      vmin = min(vmin, __shfl_xor(vmin, off));
+      vmax = max(vmax, __shfl_xor(vmax, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicMin(&b.cbounds[k], vmin);
+      atomicMax(&b.cbounds[3 + k], vmax);
+    }
+  }
+}
+
+__global__ void k_morton(BuildBuffers b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.n) return;
+  const float* s = b.seg_in + 8 * (size_t)i;
+  unsigned code = 0;
+  for (int k = 0; k < 3; ++k) {
+    const float lo = ord2f(b.cbounds[k]), hi = ord2f(b.cbounds[3 + k]);
+    const float ext = hi - lo;
+    const float cen = 0.5f * (s[k] + s[4 + k]);
+    float q = ext > 0.f ? (cen - lo) / ext : 0.5f;
+    unsigned qi = (unsigned)fminf(fmaxf(q * 1024.f, 0.f), 1023.f);
+    code |= expand_bits10(qi) << (2 - k);
+  }
+  b.keys[i] = code;
+  b.vals[i] = i;
+}
+
+__device__ __forceinline__ int delta(const unsigned* keys, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  const unsigned ki = keys[i], kj = keys[j];
+  if (ki == kj) return 32 + __clz((unsigned)(i ^ j));
+  return __clz(ki ^ kj);
+}
+
+__global__ void k_karras(BuildBuffers b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = b.n;
+  if (i >= n - 1) return;
+  const unsigned* keys = b.keys_sorted;
+  const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = delta(keys, n, i, i - d);
+  int lmax = 2;
+  while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+  int l = 0;
+  for (int t = lmax >> 1; t >= 1; t >>= 1)
+    if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+  const int j = i + l * d;
+  const int dnode = delta(keys, n, i, j);
+  int s = 0, step = l;
+  do {
+    step = (step + 1) >> 1;
+    if (delta(keys, n, i, i + (s + step) * d) > dnode) s += step;
+  } while (step > 1);
+  const int gamma = i + s * d + min(d, 0);
+  const int lo = min(i, j), hi = max(i, j);
+  int left, right;
+  if (lo == gamma) {
+    left = ~gamma;
+    b.parent_leaf[gamma] = i;
+  } else {
+    left = gamma;
+    b.parent_internal[gamma] = i;
+  }
+  if (hi == gamma + 1) {
+    right = ~(gamma + 1);
+    b.parent_leaf[gamma + 1] = i;
+  } else {
+    right = gamma + 1;
+    b.parent_internal[gamma + 1] = i;
+  }
+  b.child[2 * i] = left;
+  b.child[2 * i + 1] = right;
+  if (i == 0) b.parent_internal[0] = -1;
+}
+
+__device__ __forceinline__ void child_box(const BuildBuffers& b, int c, float box[6]) {
+  if (c < 0) {
+    const int leaf = ~c;
+    const int orig = b.vals_sorted[leaf];
+    for (int k = 0; k < 6; ++k) box[k] = b.leaf_box[6 * (size_t)orig + k];
+  } else {
+    for (int k = 0; k < 6; ++k)
+      box[k] = __hip_atomic_load(&b.node_box[6 * (size_t)c + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__global__ void k_refit(BuildBuffers b) {
+  const int leaf = blockIdx.x * blockDim.x + threadIdx.x;
+  if (leaf >= b.n) return;
+  int node = b.parent_leaf[leaf];
+  for (int guard = 0; node >= 0 && guard < b.n; ++guard) {
+    __threadfence();
+    const int prev = atomicAdd(&b.arrive[node], 1);
+    if (prev == 0) return;  // the sibling subtree is not done yet
+    __threadfence();
+    float bl[6], br[6];
+    child_box(b, b.child[2 * node], bl);
+    child_box(b, b.child[2 * node + 1], br);
+    for (int k = 0; k < 3; ++k) {
+      __hip_atomic_store(&b.node_box[6 * (size_t)node + k], fminf(bl[k], br[k]), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&b.node_box[6 * (size_t)node + 3 + k], fmaxf(bl[3 + k], br[3 + k]),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    node = b.parent_internal[node];
+  }
+}
+
+// Number of internal ancestors of each leaf; the maximum sizes the stack spill.
+__global__ void k_depth(BuildBuffers b) {
+  const int leaf = blockIdx.x * blockDim.x + threadIdx.x;
+  if (leaf >= b.n) return;
+  int depth = 0;
+  for (int node = b.parent_leaf[leaf]; node >= 0 && depth < b.n; node = b.parent_internal[node]) ++depth;
+  atomicMax(b.max_depth, depth);
+}
+
+__global__ void k_emit(BuildBuffers b, BvhNode* nodes, float4* segs_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < b.n - 1) {
+    float bl[6], br[6];
+    const int c0 = b.child[2 * i], c1 = b.child[2 * i + 1];
+    child_box(b, c0, bl);
+    child_box(b, c1, br);
+    BvhNode nd;
+    nd.a = make_float4(bl[0], bl[1], bl[2], bl[3]);
+    nd.b = make_float4(bl[4], bl[5], br[0], br[1]);
+    nd.c = make_float4(br[2], br[3], br[4], br[5]);
+    nd.d = make_int4(c0, c1, 0, 0);
+    nodes[i] = nd;
+  }
+  if (i < b.n) {
+    const int orig = b.vals_sorted[i];
+    const float* s = b.seg_in + 8 * (size_t)orig;
+    segs_out[2 * (size_t)i] = make_float4(s[0], s[1], s[2], s[3]);
+    segs_out[2 * (size_t)i + 1] = make_float4(s[4], s[5], s[6], __int_as_float(orig));
+  }
+}
+
+// Single-segment tree: one node whose two children are the same leaf.
+__global__ void k_emit_single(const float* seg_in, BvhNode* nodes, float4* segs_out) {
+  const float* s = seg_in;
+  const float r = s[3];
+  float lo[3], hi[3];
+  for (int k = 0; k < 3; ++k) {
+    const float pad = r + 1e-6f * (fabsf(s[k]) + fabsf(s[4 + k]) + r) + 1e-7f;
+    lo[k] = fminf(s[k], s[4 + k]) - pad;
+    hi[k] = fmaxf(s[k], s[4 + k]) + pad;
+  }
+  BvhNode nd;
+  nd.a = make_float4(lo[0], lo[1], lo[2], hi[0]);
+  nd.b = make_float4(hi[1], hi[2], 1.f, 1.f);
+  nd.c = make_float4(1.f, -1.f, -1.f, -1.f);  // empty second box
+  nd.d = make_int4(~0, ~0, 0, 0);
+  nodes[0] = nd;
+  segs_out[0] = make_float4(s[0], s[1], s[2], s[3]);
+  segs_out[1] = make_float4(s[4], s[5], s[6], __int_as_float(0));
+}
+
+}  // namespace
+
+// Builds the BVH from device segments (8 floats each). All scratch is
+// released before returning; the result owns nodes and segment records.
+int bvh_build(const float* seg_dev, int n, DeviceBvh& out, hipStream_t stream, std::string& err) {
+  out.release();
+  out.num_segs = n;
+  if (n <= 0) {
+    out.max_depth = 0;
+    return 0;
+  }
+  const int num_nodes = n > 1 ? n - 1 : 1;
+  HM_TRY(dev_alloc((void**)&out.nodes, sizeof(BvhNode) * (size_t)num_nodes, err));
+  HM_TRY(dev_alloc((void**)&out.segs, sizeof(float4) * 2 * (size_t)n, err));
+  out.num_nodes = num_nodes;
+  if (n == 1) {
+    hipLaunchKernelGGL(k_emit_single, dim3(1), dim3(1), 0, stream, seg_dev, out.nodes, out.segs);
+    out.max_depth = 1;
+    return check_launch("bvh single", err);
+  }
+  DeviceArena arena;
+  BuildBuffers b{};
+  b.seg_in = seg_dev;
+  b.n = n;
+  HM_TRY(arena.alloc((void**)&b.leaf_box, sizeof(float) * 6 * (size_t)n, err));
+  HM_TRY(arena.alloc((void**)&b.cbounds, sizeof(int) * 8, err));
+  HM_TRY(arena.alloc((void**)&b.keys, sizeof(unsigned) * (size_t)n, err));
+  HM_TRY(arena.alloc((void**)&b.vals, sizeof(int) * (size_t)n, err));
+  HM_TRY(arena.alloc((void**)&b.keys_sorted, sizeof(unsigned) * (size_t)n, err));
+  HM_TRY(arena.alloc((void**)&b.vals_sorted, sizeof(int) * (size_t)n, err));
+  HM_TRY(arena.alloc((void**)&b.child, sizeof(int) * 2 * (size_t)n, err));
+  HM_TRY(arena.alloc((void**)&b.parent_internal, sizeof(int) * (size_t)n, err));
+  HM_TRY(arena.alloc((void**)&b.parent_leaf, sizeof(int) * (size_t)n, err));
+  HM_TRY(arena.alloc((void**)&b.node_box, sizeof(float) * 6 * (size_t)n, err));
+  HM_TRY(arena.alloc((void**)&b.arrive, sizeof(int) * (size_t)n, err));
+  HM_TRY(arena.alloc((void**)&b.max_depth, sizeof(int) * 4, err));
+  const int init_b[8] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN, 0, 0};
+  HM_HIP(hipMemcpyAsync(b.cbounds, init_b, sizeof(init_b), hipMemcpyHostToDevice, stream), err);
+  HM_HIP(hipMemsetAsync(b.arrive, 0, sizeof(int) * (size_t)n, stream), err);
+  HM_HIP(hipMemsetAsync(b.max_depth, 0, sizeof(int) * 4, stream), err);
+  const int bs = 256, gs = (n + bs - 1) / bs;
+  hipLaunchKernelGGL(k_leaf_boxes, dim3(gs), dim3(bs), 0, stream, b);
+  HM_TRY(check_launch("bvh leaf boxes", err));
+  hipLaunchKernelGGL(k_morton, dim3(gs), dim3(bs), 0, stream, b);
+  HM_TRY(check_launch("bvh morton", err));
+  size_t tmp_bytes = 0;
+  HM_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, b.keys, b.keys_sorted, b.vals, b.vals_sorted, n, 0,
+                                   30, stream),
+         err);
+  void* tmp = nullptr;
+  HM_TRY(arena.alloc(&tmp, tmp_bytes, err));
+  HM_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, b.keys, b.keys_sorted, b.vals, b.vals_sorted, n, 0, 30,
+                                   stream),
+         err);
+  hipLaunchKernelGGL(k_karras, dim3(gs), dim3(bs), 0, stream, b);
+  HM_TRY(check_launch("bvh karras", err));
+  hipLaunchKernelGGL(k_refit, dim3(gs), dim3(bs), 0, stream, b);
+  HM_TRY(check_launch("bvh refit", err));
+  hipLaunchKernelGGL(k_emit, dim3(gs), dim3(bs), 0, stream, b, out.nodes, out.segs);
+  HM_TRY(check_launch("bvh emit", err));
+  hipLaunchKernelGGL(k_depth, dim3(gs), dim3(bs), 0, stream, b);
+  HM_TRY(check_launch("bvh depth", err));
+  int depth = 0;
+  HM_HIP(hipMemcpyAsync(&depth, b.max_depth, sizeof(int), hipMemcpyDeviceToHost, stream), err);
+  HM_HIP(hipStreamSynchronize(stream), err);
+  out.max_depth = depth;
+  return 0;
+}
+
+}  // namespace hm

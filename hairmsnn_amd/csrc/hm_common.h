@@ -1,0 +1,96 @@
+// Shared host/device vector math for the HairMSNN MI355X path tracer.
+//
+// Everything here compiles under hipcc (device + host) and under plain g++
+// (host-only tools: scene parser, procedural generators, CLIs).
+#pragma once
+
+#include <cstdint>
+#include <cmath>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define HM_HD __host__ __device__ __forceinline__
+#define HM_D __device__ __forceinline__
+#else
+#define HM_HD inline
+#define HM_D inline
+#endif
+
+namespace hm {
+
+// The reference defines Pi as 3.1415926f and TWO_Pi as the unparenthesised
+// `2.f * 3.14159f` (cuda_headers/utils.cuh:10-11); both constants feed the
+// environment-map and BSDF arithmetic, so the same values are used here.
+constexpr float kPi = 3.1415926f;
+constexpr float kTwoPiEnv = 2.f * 3.14159f;  // env wrap constant (utils.cuh:11)
+constexpr float kHostPi = 3.14159f;          // host-side constant (scene.cpp:207, :363)
+
+struct V3 {
+  float x, y, z;
+};
+
+HM_HD V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+HM_HD V3 v3(float s) { return V3{s, s, s}; }
+HM_HD V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+HM_HD V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+HM_HD V3 operator-(V3 a) { return V3{-a.x, -a.y, -a.z}; }
+HM_HD V3 operator*(V3 a, V3 b) { return V3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+HM_HD V3 operator*(float s, V3 a) { return V3{s * a.x, s * a.y, s * a.z}; }
+HM_HD V3 operator*(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
+HM_HD V3 operator/(V3 a, float s) { return V3{a.x / s, a.y / s, a.z / s}; }
+HM_HD V3 operator/(V3 a, V3 b) { return V3{a.x / b.x, a.y / b.y, a.z / b.z}; }
+HM_HD V3& operator+=(V3& a, V3 b) { a = a + b; return a; }
+HM_HD V3& operator*=(V3& a, V3 b) { a = a * b; return a; }
+HM_HD float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+HM_HD V3 cross(V3 a, V3 b) {
+  return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+HM_HD float length(V3 a) { return sqrtf(dot(a, a)); }
+HM_HD V3 normalize(V3 a) {
+  float inv = 1.f / sqrtf(dot(a, a));
+  return a * inv;
+}
+HM_HD V3 vmin(V3 a, V3 b) { return V3{fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)}; }
+HM_HD V3 vmax(V3 a, V3 b) { return V3{fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}; }
+HM_HD bool any_nan(V3 a) { return isnan(a.x) || isnan(a.y) || isnan(a.z); }
+HM_HD bool any_inf(V3 a) { return isinf(a.x) || isinf(a.y) || isinf(a.z); }
+HM_HD bool is_zero(V3 a) { return a.x == 0.f && a.y == 0.f && a.z == 0.f; }
+HM_HD float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+HM_HD float sqr(float x) { return x * x; }
+HM_HD float safe_sqrt(float x) { return sqrtf(fmaxf(0.f, x)); }
+
+// Rec. 709 luminance (utils.cuh:79-83).
+HM_HD float luminance(V3 c) { return 0.212671f * c.x + 0.715160f * c.y + 0.072169f * c.z; }
+
+// Power heuristic with one sample per strategy (utils.cuh:290-294).
+HM_HD float power_heuristic(float f, float g) {
+  float ff = f * f, gg = g * g;
+  return ff / (ff + gg);
+}
+
+// Row-vector frame: local = (dot(r0,v), dot(r1,v), dot(r2,v)).
+struct Frame {
+  V3 r0, r1, r2;
+};
+HM_HD V3 to_local(const Frame& f, V3 v) { return V3{dot(f.r0, v), dot(f.r1, v), dot(f.r2, v)}; }
+// The transpose applied to a local vector (the frames are orthonormal).
+HM_HD V3 to_world(const Frame& f, V3 l) { return f.r0 * l.x + f.r1 * l.y + f.r2 * l.z; }
+
+// Tangent frame around a surface normal (utils.cuh:232-259): rows are
+// (c1, c2, n). The reference promotes `1./(1.+n.z)` to double; on the GPU
+// everything here stays FP32 (the frame is orthonormal either way).
+HM_HD Frame basis_from_normal(V3 n) {
+  V3 c1, c2;
+  if (n.z < -0.999999f) {
+    c1 = v3(0.f, -1.f, 0.f);
+    c2 = v3(-1.f, 0.f, 0.f);
+  } else {
+    float a = 1.f / (1.f + n.z);
+    float b = -n.x * n.y * a;
+    c1 = normalize(v3(1.f - n.x * n.x * a, b, -n.x));
+    c2 = normalize(v3(b, 1.f - n.y * n.y * a, -n.y));
+  }
+  return Frame{c1, c2, n};
+}
+
+}  // namespace hm

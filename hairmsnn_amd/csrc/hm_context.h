@@ -1,0 +1,62 @@
+// Scene context: owns everything a render needs on the GPU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "hm_internal.h"
+#include "hm_pt_kernels.h"
+#include "hm_scene.h"
+
+namespace hm {
+
+struct MlpState;  // radiance-cache network (hm_mlp.hip)
+struct MsnnState;  // HairMSNN frame state (hm_msnn.hip)
+void msnn_release(MsnnState* m);
+
+struct Context {
+  Scene scene;
+  hipStream_t stream = nullptr;
+  int num_cus = 256;
+
+  // geometry
+  HairGeometry hair;
+  float* seg_dev = nullptr;  // 8 floats per segment, original order
+  float* strand_sigma_dev = nullptr;
+  DeviceBvh bvh;
+  float scene_scale = 1.f;
+  V3 bmin{0, 0, 0}, bmax{0, 0, 0};
+
+  // environment
+  DeviceEnv env;
+  EnvImage env_host;
+
+  // camera (render_path_tracing.cu:767-799)
+  V3 cam_pos, cam_d00, cam_du, cam_dv;
+
+  SceneView view{};
+
+  // path-tracer state
+  PathBuffers pb{};
+  std::vector<void*> pb_allocs;
+  bool pb_msnn = false;
+  float4* accum = nullptr;
+  float4* average = nullptr;
+  int accum_id = 0;
+  int trace_blocks = 2048;
+  int spill_levels = 0;
+  unsigned long long* stats_dev = nullptr;
+
+  MsnnState* msnn = nullptr;
+
+  ~Context();
+  int init(std::string& err);
+  int ensure_path_buffers(bool msnn_mode, std::string& err);
+  void free_path_buffers();
+  PassConfig pt_pass_config() const;
+};
+
+}  // namespace hm

@@ -1,0 +1,116 @@
+// Library-internal helpers: error plumbing, device allocations, and the
+// device-resident objects (BVH, environment tables) shared by the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "hm_common.h"
+
+namespace hm {
+
+// Status codes (mirrors include/hmsnn.h).
+enum Status { kOk = 0, kEInval = 1, kEIO = 2, kEDevice = 3, kENoMem = 4, kEUnsupported = 5 };
+
+#define HM_TRY(expr)                \
+  do {                              \
+    int hm_rc_ = (expr);            \
+    if (hm_rc_ != 0) return hm_rc_; \
+  } while (0)
+
+#define HM_HIP(expr, errstr)                                                          \
+  do {                                                                                \
+    hipError_t hm_e_ = (expr);                                                        \
+    if (hm_e_ != hipSuccess) {                                                        \
+      (errstr) = std::string(#expr) + ": " + hipGetErrorString(hm_e_);              \
+      return hm_e_ == hipErrorOutOfMemory ? ::hm::kENoMem : ::hm::kEDevice;           \
+    }                                                                                 \
+  } while (0)
+
+inline int check_launch(const char* what, std::string& err) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    err = std::string("kernel launch failed (") + what + "): " + hipGetErrorString(e);
+    return kEDevice;
+  }
+  return kOk;
+}
+
+inline int dev_alloc(void** p, size_t bytes, std::string& err) {
+  *p = nullptr;
+  if (bytes == 0) return kOk;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    err = "hipMalloc(" + std::to_string(bytes) + " bytes): " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? kENoMem : kEDevice;
+  }
+  return kOk;
+}
+
+inline void dev_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+// Scope-bound list of device allocations (build scratch).
+struct DeviceArena {
+  std::vector<void*> ptrs;
+  int alloc(void** p, size_t bytes, std::string& err) {
+    HM_TRY(dev_alloc(p, bytes, err));
+    if (*p) ptrs.push_back(*p);
+    return kOk;
+  }
+  ~DeviceArena() {
+    (void)hipDeviceSynchronize();
+    for (void* p : ptrs) dev_free(p);
+  }
+};
+
+struct BvhNode;
+
+struct DeviceBvh {
+  BvhNode* nodes = nullptr;
+  float4* segs = nullptr;  // 2 per segment, Morton order
+  int num_segs = 0;
+  int num_nodes = 0;
+  int max_depth = 0;
+  void release() {
+    dev_free(nodes);
+    dev_free(segs);
+    nodes = nullptr;
+    segs = nullptr;
+    num_segs = num_nodes = max_depth = 0;
+  }
+  ~DeviceBvh() { release(); }
+};
+
+int bvh_build(const float* seg_dev, int n, DeviceBvh& out, hipStream_t stream, std::string& err);
+
+// Environment map and its importance-sampling tables (RGBA32F map, FP32
+// tables produced from FP64 prefix sums).
+struct DeviceEnv {
+  int width = 0, height = 0;
+  float4* tex = nullptr;   // width*height
+  float* c_pdf = nullptr;  // (width+1)*height
+  float* c_cdf = nullptr;
+  float* m_pdf = nullptr;  // height+1
+  float* m_cdf = nullptr;
+  void release() {
+    dev_free(tex);
+    dev_free(c_pdf);
+    dev_free(c_cdf);
+    dev_free(m_pdf);
+    dev_free(m_cdf);
+    tex = nullptr;
+    c_pdf = c_cdf = m_pdf = m_cdf = nullptr;
+    width = height = 0;
+  }
+  ~DeviceEnv() { release(); }
+};
+
+// Builds the tables on the GPU from an RGBA32F map already in device memory.
+int env_build_tables(DeviceEnv& env, hipStream_t stream, std::string& err);
+
+}  // namespace hm

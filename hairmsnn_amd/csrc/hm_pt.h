@@ -1,0 +1,273 @@
+// Wavefront path tracer: scene view, per-path state layout and the shading
+// routines shared by the plain path tracer and the HairMSNN passes.
+//
+// Integrator semantics follow cuda_headers/optix_common.cuh:151-409
+// (sampleBSDF, sampleLights, directLighting, nextPathVertex, pathTrace) and
+// cuda/hair_msnn.cu:17-185 (truncated and training paths), with the
+// reference's random-number consumption order per vertex:
+//   light pick (1) [+ env sample (2)] [+ MIS BSDF sample (2, +2 on hair)]
+//   [Russian roulette (1), from the second vertex on]
+//   continuation sample (2, +2 on hair)
+// Shadow rays are not traced inside the shading kernel: their unoccluded
+// contributions are stored with the ray and resolved by the shadow kernel,
+// which applies the reference's NaN scrub to the sum of the visible parts.
+#pragma once
+
+#include "hm_bvh.h"
+#include "hm_env.h"
+#include "hm_ggx.h"
+#include "hm_hair_bsdf.h"
+#include "hm_rng.h"
+
+namespace hm {
+
+constexpr int kMaxDirLights = 8;
+constexpr int kPrimMiss = -1;
+constexpr int kPrimSphere = -2;
+
+struct SceneView {
+  BvhView bvh;
+  const float* seg_by_id;     // 8 floats per segment, original order
+  const float* strand_sigma;  // per-strand sigma_a (3 floats) or null
+  SphereView sphere;
+  V3 sphere_albedo;
+  float sphere_alpha;  // clamped to [0.01, 1] and squared (optix_common.cuh:551-552)
+  EnvView env;
+  int num_dlights, num_total_lights;
+  V3 dl_from[kMaxDirLights];  // normalised
+  V3 dl_emit[kMaxDirLights];
+  V3 cam_pos, cam_d00, cam_du, cam_dv;
+  int width, height;
+  HairConsts hc;
+  V3 sigma_a;
+  int mis;
+};
+
+// Per-slot path state (structure of arrays in HBM). A slot is one pixel of
+// the current sample pass.
+struct PathBuffers {
+  float4* ray_o;
+  float4* ray_d;
+  float4* beta;
+  float4* beta_short;  // HairMSNN training paths only
+  uint32_t* rng;
+  float4* L;
+  float4* L_short;     // HairMSNN training paths only
+  uint32_t* flags;     // bit 0: training pixel
+  float* hit_t;
+  int* hit_prim;
+  float* hit_u;
+  // pending direct-lighting records
+  float4* sh_o1;  // xyz origin, w = 1 if the ray exists
+  float4* sh_d1;
+  float4* sh_c1;  // unoccluded contribution (before path throughput)
+  float4* sh_o2;
+  float4* sh_d2;
+  float4* sh_c2;
+  float4* sh_beta;        // path throughput at the vertex
+  float4* sh_beta_short;  // HairMSNN training paths only
+  int* queue[2];
+  int* shadow_queue;
+  int* counts;  // [0], [1] continuation queues, [2] shadow queue, [3..] stats
+  int* spill;   // traversal stack spill
+  int spill_stride;
+  int spill_levels;
+};
+
+struct ShadeParams {
+  int depth;       // vertex index of the hits being shaded (0 = camera hit)
+  int dl_zero;     // direct lighting at the camera vertex
+  int dl_from;     // direct lighting at vertices >= dl_from (for depth >= 1)
+  int max_vertex;  // continuation allowed while depth + 1 <= max_vertex
+  int trunc_beta;  // HairMSNN: terminate after the vertex when depth > trunc_beta
+  int msnn;        // 1: HairMSNN G-buffer pass (training paths enabled)
+  int cur;         // current continuation queue index
+};
+
+#if defined(__HIPCC__)
+
+// Shading point after a closest hit.
+struct Hit {
+  int surface;
+  V3 p, n, wo, wo_local;
+  Frame fr;
+  float radius;
+  HairWo hw;
+  V3 albedo;
+  float alpha;
+};
+
+HM_D float draw(uint32_t& s) { return lcg_uniform(s); }
+
+// Closest-hit shading setup (optix_common.cuh:527-599).
+HM_D void setup_hit(const SceneView& sc, V3 o, V3 d, float t, int prim, float u, Hit& h) {
+  h.wo = -1.f * d;
+  V3 p = o + t * d;
+  if (prim == kPrimSphere) {
+    const V3 c = v3(sc.sphere.cx, sc.sphere.cy, sc.sphere.cz);
+    h.n = normalize(p - c);
+    h.p = c + sc.sphere.r * h.n;
+    h.fr = basis_from_normal(h.n);
+    h.wo_local = normalize(to_local(h.fr, h.wo));
+    h.surface = 1;
+    h.albedo = sc.sphere_albedo;
+    h.alpha = sc.sphere_alpha;
+    h.radius = 0.f;
+    return;
+  }
+  const float* s = sc.seg_by_id + 8 * (size_t)prim;
+  const V3 p0 = v3(s[0], s[1], s[2]), p1 = v3(s[4], s[5], s[6]);
+  V3 n, tg;
+  segment_hit_geometry(p0, p1, s[3], u, &p, &n, &tg);
+  h.p = p;
+  h.n = n;
+  h.radius = s[3];
+  h.fr = hair_frame(tg, h.wo);
+  h.wo_local = normalize(to_local(h.fr, h.wo));
+  h.surface = 0;
+  V3 sig = sc.sigma_a;
+  if (sc.strand_sigma) {
+    const int strand = (int)s[7];
+    sig = v3(sc.strand_sigma[3 * strand], sc.strand_sigma[3 * strand + 1], sc.strand_sigma[3 * strand + 2]);
+  }
+  const float hh = dot(h.fr.r1, n);
+  h.hw = hair_prepare(sc.hc, h.wo_local, hh, sig);
+}
+
+HM_D V3 eval_bsdf(const SceneView& sc, const Hit& h, V3 wi_local, float* pdf) {
+  if (h.surface) {
+    *pdf = surface_pdf(h.alpha, h.wo_local, normalize(h.wo_local + wi_local));
+    return surface_eval(h.wo_local, wi_local, h.albedo, h.alpha);
+  }
+  return hair_eval(sc.hc, h.hw, wi_local, pdf);
+}
+
+// BSDF sample with the reference's draw order: two uniforms, then two more
+// on hair (optix_common.cuh:154-170, :314-329).
+HM_D V3 sample_bsdf(const SceneView& sc, const Hit& h, uint32_t& rng, V3* wi_local, V3* wi, float* pdf) {
+  const float r0 = draw(rng), r1 = draw(rng);
+  V3 f;
+  if (!h.surface) {
+    const float r2 = draw(rng), r3 = draw(rng);
+    int lobe;
+    *wi_local = hair_sample_dir(sc.hc, h.hw, r0, r1, r2, r3, &lobe);
+    *wi = normalize(to_world(h.fr, *wi_local));
+    f = hair_eval(sc.hc, h.hw, *wi_local, pdf);
+  } else {
+    *wi_local = surface_sample(h.alpha, h.wo_local, r0, r1, pdf);
+    *wi = normalize(to_world(h.fr, *wi_local));
+    f = surface_eval(h.wo_local, *wi_local, h.albedo, h.alpha);
+  }
+  return f;
+}
+
+struct DirectOut {
+  int has1, has2;
+  V3 o1, d1, c1, o2, d2, c2;
+};
+
+// directLighting with MIS (optix_common.cuh:205-309) up to the shadow tests.
+HM_D void direct_lighting(const SceneView& sc, const Hit& h, uint32_t& rng, DirectOut& out) {
+  out.has1 = out.has2 = 0;
+  out.c1 = out.c2 = v3(0.f);
+  V3 p = h.p;  // the push-through below is shared by both samples (reference quirk)
+  const float inv_n = 1.f / (float)sc.num_total_lights;
+  // ---- light sample (sampleLights) ----
+  const float rsel = draw(rng);
+  int sel = (int)floorf(rsel * (float)sc.num_total_lights);
+  // A draw of exactly 1.0 selects index N; without an environment light that
+  // index does not exist (undefined in the reference), so it maps to the last
+  // directional light.
+  if (!sc.env.enabled && sel >= sc.num_dlights) sel = sc.num_dlights - 1;
+  bool is_delta = false;
+  V3 emit, wi;
+  float light_pdf = 1.f * inv_n;
+  if (sel >= sc.num_dlights) {
+    const float ux = draw(rng), uy = draw(rng);
+    float env_p;
+    emit = env_sample(sc.env, ux, uy, &wi, &env_p);
+    light_pdf = light_pdf * env_p;
+  } else {
+    emit = sc.dl_emit[sel];
+    wi = normalize(sc.dl_from[sel]);
+    is_delta = true;
+  }
+  {
+    const V3 wi_local = normalize(to_local(h.fr, wi));
+    const bool lower = dot(wi, h.n) < 0.f;
+    V3 nd = h.n;
+    bool skip = false;
+    if (h.surface && lower) {
+      skip = true;
+    } else if (!h.surface && lower) {
+      nd = -1.f * h.n;
+      p = p + (2.f * h.radius) * nd;
+    }
+    if (!skip) {
+      float bpdf;
+      const V3 f = eval_bsdf(sc, h, wi_local, &bpdf);
+      V3 c;
+      if (sc.mis && !is_delta) c = f * emit * power_heuristic(light_pdf, bpdf) / light_pdf;
+      else c = f * emit / light_pdf;
+      out.has1 = 1;
+      out.o1 = p + 1e-3f * nd;
+      out.d1 = wi;
+      out.c1 = c;
+    }
+  }
+  if (!sc.mis || is_delta) return;
+  // ---- BSDF sample toward the environment (sampleBSDF) ----
+  V3 wl, wb;
+  float bpdf;
+  const V3 f = sample_bsdf(sc, h, rng, &wl, &wb, &bpdf);
+  const bool lower = dot(wb, h.n) < 0.f;
+  V3 nd = h.n;
+  if (h.surface && lower) return;
+  if (!h.surface && lower) {
+    nd = -1.f * h.n;
+    p = p + (2.f * h.radius) * nd;
+  }
+  const V3 e = env_radiance(sc.env, wb);
+  const float lp = env_pdf(sc.env, wb) * 1.f * inv_n;
+  out.has2 = 1;
+  out.o2 = p + 1e-3f * nd;
+  out.d2 = wb;
+  out.c2 = f * e * power_heuristic(bpdf, lp) / bpdf;
+}
+
+// Closest hit against the head sphere and the hair BVH. The sphere is tested
+// first so its distance prunes the hair traversal; equal distances go to hair.
+template <bool kCount>
+HM_D void closest_hit(const SceneView& sc, V3 o, V3 d, TraceStack& st, float& t, int& prim, float& u,
+                      TraceCounters& cnt) {
+  t = __builtin_inff();
+  prim = kPrimMiss;
+  u = 0.f;
+  bool sphere_hit = false;
+  if (sc.sphere.enabled) {
+    float ts;
+    if (intersect_sphere(o, d, v3(sc.sphere.cx, sc.sphere.cy, sc.sphere.cz), sc.sphere.r, &ts)) {
+      t = ts;
+      sphere_hit = true;
+    }
+  }
+  int id = 0x7fffffff;
+  const RayInv r = make_ray(o.x, o.y, o.z, d.x, d.y, d.z);
+  trace_closest<kCount>(sc.bvh, r, st, t, id, u, cnt);
+  if (id != 0x7fffffff) prim = id;
+  else if (sphere_hit) prim = kPrimSphere;
+}
+
+template <bool kCount>
+HM_D bool occluded(const SceneView& sc, V3 o, V3 d, TraceStack& st, TraceCounters& cnt) {
+  if (sc.sphere.enabled) {
+    float ts;
+    if (intersect_sphere(o, d, v3(sc.sphere.cx, sc.sphere.cy, sc.sphere.cz), sc.sphere.r, &ts)) return true;
+  }
+  const RayInv r = make_ray(o.x, o.y, o.z, d.x, d.y, d.z);
+  return trace_any<kCount>(sc.bvh, r, st, cnt);
+}
+
+#endif  // __HIPCC__
+
+}  // namespace hm

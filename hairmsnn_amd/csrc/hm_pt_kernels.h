@@ -1,0 +1,93 @@
+// Host-callable launchers of the wavefront kernels and the G-buffer writers
+// used by the HairMSNN G_BUFFER pass.
+#pragma once
+
+#include <string>
+
+#include "hm_internal.h"
+#include "hm_pt.h"
+
+namespace hm {
+
+// Per-pixel outputs of the HairMSNN G_BUFFER pass at the primary vertex
+// (hair_msnn.cu:290-311): the 12-float network input record and hit flags.
+// Record layout: position / sceneScale (3), wo (3), tangent (3), sigma_a (3).
+// The reference writes channels 0-8 and leaves 9-11 uninitialised; here they
+// carry the hit's absorption coefficient.
+struct GBufferOut {
+  int enabled;
+  float* nn_input;     // 12 floats per pixel
+  uint8_t* hit_flags;  // bit 0: hit, bit 1: surface
+  float scene_scale;
+};
+
+constexpr int kNnInputCh = 12;
+
+#if defined(__HIPCC__)
+HM_D void gbuffer_write_hit(const GBufferOut& gb, const SceneView& sc, int slot, const Hit& h, int prim) {
+  float* r = gb.nn_input + (size_t)slot * kNnInputCh;
+  const V3 p = h.p / gb.scene_scale;
+  r[0] = p.x; r[1] = p.y; r[2] = p.z;
+  r[3] = h.wo.x; r[4] = h.wo.y; r[5] = h.wo.z;
+  r[6] = h.fr.r0.x; r[7] = h.fr.r0.y; r[8] = h.fr.r0.z;
+  V3 sig = sc.sigma_a;
+  if (!h.surface && sc.strand_sigma) {
+    const int strand = (int)sc.seg_by_id[8 * (size_t)prim + 7];
+    sig = v3(sc.strand_sigma[3 * strand], sc.strand_sigma[3 * strand + 1], sc.strand_sigma[3 * strand + 2]);
+  }
+  r[9] = sig.x; r[10] = sig.y; r[11] = sig.z;
+  gb.hit_flags[slot] = (uint8_t)(1u | (h.surface ? 2u : 0u));
+}
+
+HM_D void gbuffer_write_miss(const GBufferOut& gb, int slot, V3 dir) {
+  float* r = gb.nn_input + (size_t)slot * kNnInputCh;
+  r[0] = 0.f; r[1] = 0.f; r[2] = 0.f;
+  r[3] = -dir.x; r[4] = -dir.y; r[5] = -dir.z;
+  for (int k = 6; k < kNnInputCh; ++k) r[k] = 0.f;
+  gb.hit_flags[slot] = 0;
+}
+#endif
+
+// Optional per-kernel timing: event pairs recorded around each launch.
+struct PassTimer {
+  hipEvent_t* events = nullptr;  // 2 per record
+  int* kinds = nullptr;          // 0 extend, 1 shade, 2 shadow, 3 other
+  int capacity = 0;              // records
+  int used = 0;
+};
+
+struct PassConfig {
+  int accum_id = 0;
+  int dl_zero = 1;        // direct lighting at the camera vertex
+  int dl_from = 0;        // direct lighting at vertices >= dl_from (>= 1)
+  int max_vertex = 39;    // path_v2 - 1
+  int trunc_beta = 1 << 30;
+  int msnn = 0;
+  const int* train_idx = nullptr;
+  int every_nth = 1;
+  GBufferOut gb{};
+  // non-null: counting traversal. [0] closest-hit nodes, [1] closest-hit
+  // segments, [2] any-hit nodes, [3] any-hit segments, [4] closest-hit rays,
+  // [5] any-hit rays
+  unsigned long long* stats = nullptr;
+  PassTimer* timer = nullptr;
+  int trace_blocks = 2048;
+  int shade_blocks = 2048;
+};
+
+// Runs one full sample pass (camera -> all vertices). Leaves per-pixel
+// radiance in pb.L (and pb.L_short for training pixels).
+int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hipStream_t stream,
+                std::string& err);
+
+// Adds pb.L into the running sum and mean buffers (accumulation id `accum_id`).
+int pt_accumulate(const float4* L, float4* accum, float4* average, int n, int accum_id, hipStream_t stream,
+                  std::string& err);
+
+// Batched closest-hit / any-hit over an explicit ray list (8 floats per ray:
+// origin xyz, pad, direction xyz, pad).
+int trace_rays(const SceneView& sc, const float* rays, int n, int any_hit, int* prim, float* t, float* u,
+               int* spill, int spill_stride, int spill_levels, int blocks, unsigned long long* stats,
+               hipStream_t stream, std::string& err);
+
+}  // namespace hm

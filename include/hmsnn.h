@@ -1,0 +1,230 @@
+/*
+ * hmsnn.h - C ABI of the MI355X-native HairMSNN hair path tracer.
+ *
+ * The reference exposes its hot path as three executables driven by one
+ * scene-config JSON (render_path_tracing.cu:817-841, render_hair_msnn.cu:
+ * 1146-1175, render_nrc.cu:1080-1104); inside, the boundary is an
+ * owlLaunch2D of a ray-generation program per pass and the tiny-cuda-nn
+ * calls (neural_network.cuh:33-50). This header is the library behind
+ * drop-in versions of those executables: every entry point is plain C,
+ * takes plain pointers and sizes, and returns an hmsnn_status.
+ *
+ * Conventions
+ *  - Status: 0 on success, otherwise an HMSNN_E* code; hmsnn_last_error()
+ *    returns a thread-local message for the last failure on this thread.
+ *  - Ownership: the caller owns every buffer it passes. A context owns its
+ *    geometry, BVH, environment tables, path state and network.
+ *  - Threading: one context per host thread; calls on a context are
+ *    serialised. Functions named *_dev take device pointers and run on the
+ *    given HIP stream (NULL = the library's stream) without synchronising;
+ *    everything else is synchronous.
+ *  - Angles are radians unless a name says _deg.
+ */
+#ifndef HMSNN_H
+#define HMSNN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum hmsnn_status {
+  HMSNN_OK = 0,
+  HMSNN_EINVAL = 1,      /* bad argument or malformed scene */
+  HMSNN_EIO = 2,         /* file could not be read or written */
+  HMSNN_EDEVICE = 3,     /* HIP runtime or kernel failure, or no GPU */
+  HMSNN_ENOMEM = 4,      /* device allocation failed */
+  HMSNN_EUNSUPPORTED = 5 /* input the library does not implement */
+} hmsnn_status;
+
+typedef struct hmsnn_ctx hmsnn_ctx;
+
+/* ---- library ------------------------------------------------------------ */
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char* hmsnn_last_error(void);
+/* ABI version: major * 10000 + minor * 100 + patch. */
+int hmsnn_version(void);
+/* Number of visible GPUs; writes the device name of device 0. */
+int hmsnn_device_count(int* count, char* name, int name_len);
+
+/* Device memory helpers for harnesses that have no allocator of their own. */
+int hmsnn_dev_alloc(size_t bytes, void** ptr);
+int hmsnn_dev_free(void* ptr);
+int hmsnn_memcpy_h2d(void* dst_dev, const void* src_host, size_t bytes);
+int hmsnn_memcpy_d2h(void* dst_host, const void* src_dev, size_t bytes);
+int hmsnn_dev_synchronize(void);
+
+/* ---- hair BSDF (disney_hair.cuh:25-386) --------------------------------- */
+
+/* Global hair material, as parsed by parseScene (scene.cpp:193-227):
+ * alpha in radians (the JSON holds degrees), gains R, TT, TRT, TRRT. */
+typedef struct hmsnn_hair_params {
+  float beta_m, beta_n, alpha;
+  float gain[4];
+} hmsnn_hair_params;
+
+/* One BSDF query in the hair frame: x = tangent, wo.y = 0 in renderer use.
+ * h = dot(frame y, surface normal) in [-1, 1]. */
+typedef struct hmsnn_bsdf_query {
+  float wo[3];
+  float wi[3];
+  float h;
+  float sigma_a[3];
+} hmsnn_bsdf_query;
+
+/* f * |cos theta_i| (n x RGB) and the lobe-mixture pdf (n) at wi
+ * (replaces disney_hair, disney_hair.cuh:185-266). */
+int hmsnn_hair_bsdf_eval_dev(const hmsnn_hair_params* params, const hmsnn_bsdf_query* q_dev, int n,
+                             float* f_rgb_dev, float* pdf_dev, void* stream);
+/* Importance-samples wi from four uniforms per query (u4_dev: n x 4) and
+ * returns wi (n x 3), f * |cos| (n x 3), pdf (n) and the chosen lobe (n)
+ * (replaces sample_disney_hair, disney_hair.cuh:276-386; q.wi is ignored). */
+int hmsnn_hair_bsdf_sample_dev(const hmsnn_hair_params* params, const hmsnn_bsdf_query* q_dev,
+                               const float* u4_dev, int n, float* wi_dev, float* f_rgb_dev, float* pdf_dev,
+                               int* lobe_dev, void* stream);
+
+/* ---- scene context ------------------------------------------------------- */
+
+/* Parses a scene JSON (the reference's schema, scene.cpp:119-339, plus the
+ * procedural keys hair.procedural, surface.sphere,
+ * lights.environment.constant / .procedural and the hairmsnn block),
+ * generates the geometry and environment, builds the BVH and the
+ * environment tables on the GPU. Fails like parseScene fails. */
+int hmsnn_create(const char* scene_json_path, hmsnn_ctx** out);
+int hmsnn_create_from_json(const char* json_text, const char* base_dir, hmsnn_ctx** out);
+int hmsnn_destroy(hmsnn_ctx* ctx);
+
+typedef struct hmsnn_info {
+  int width, height, spp, path_v1, path_v2;
+  int num_segments, num_strands;
+  int bvh_nodes, bvh_depth;
+  int env_width, env_height;
+  int num_dir_lights, has_env, has_sphere, mis, env_pdf;
+  float scene_scale;
+  float sigma_a[3];
+  float beta_m, beta_n, alpha;
+  float gain[4];
+  float env_scale, env_rot_phi;
+  float sphere[4];        /* centre xyz, radius */
+  float sphere_albedo[3];
+  float sphere_alpha;     /* after clamp to [0.01, 1] and squaring */
+  float camera[12];       /* pos, dir_00, dir_du, dir_dv (render_path_tracing.cu:767-799) */
+  float dir_from[8][3];
+  float dir_emit[8][3];
+} hmsnn_info;
+int hmsnn_get_info(const hmsnn_ctx* ctx, hmsnn_info* info);
+/* Parses a scene JSON on the host only (no GPU needed) and reports what a
+ * context would be created with; fields that need generated content
+ * (BVH, environment size) are left 0 except num_segments/num_strands of
+ * procedural hair. Same failure rules as hmsnn_create (parseScene,
+ * scene.cpp:119-339). */
+int hmsnn_parse_scene(const char* json_text, hmsnn_info* info);
+
+/* Copies generated content to the host: segments as 8 floats each
+ * (p0 xyz, radius, p1 xyz, strand id); the RGBA32F environment map
+ * (env_width x env_height x 4) and its tables (conditional pdf/cdf of
+ * (env_width + 1) x env_height, marginal pdf/cdf of env_height + 1). Any
+ * pointer may be NULL to skip that array. */
+int hmsnn_export_segments(const hmsnn_ctx* ctx, float* seg_host);
+int hmsnn_export_env(const hmsnn_ctx* ctx, float* rgba_host, float* c_pdf, float* c_cdf, float* m_pdf,
+                     float* m_cdf);
+/* Per-strand absorption (num_strands x 3); NULL restores the global value. */
+int hmsnn_set_strand_sigma_a(hmsnn_ctx* ctx, const float* sigma_host, int num_strands);
+
+/* ---- path tracing (render_path_tracing: render_path_tracing.cu:404-423,
+ *      cuda/path_tracing.cu:19-67) --------------------------------------- */
+
+/* Clears the accumulation (accumId = 0), like cameraChanged does. */
+int hmsnn_pt_reset(hmsnn_ctx* ctx);
+/* Traces `passes` more samples per pixel into the accumulation buffers.
+ * Asynchronous on the context's stream; hmsnn_ctx_synchronize waits. */
+int hmsnn_pt_passes(hmsnn_ctx* ctx, int passes);
+int hmsnn_ctx_synchronize(hmsnn_ctx* ctx);
+/* Samples accumulated so far (the reference's accumId). */
+int hmsnn_pt_accum_count(const hmsnn_ctx* ctx, int* count);
+/* Copies width x height x 4 floats: which = 0 mean, 1 running sum. Row 0 is
+ * the bottom row of the image (the reference's framebuffer order). */
+int hmsnn_get_image(hmsnn_ctx* ctx, int which, float* rgba_host);
+/* Reset, trace spp samples per pixel, copy the mean image. */
+int hmsnn_render_path_tracing(hmsnn_ctx* ctx, int spp, float* rgba_host);
+/* Per-pixel radiance of the last pass before accumulation (w x h x 4). */
+int hmsnn_get_last_pass(hmsnn_ctx* ctx, float* rgba_host);
+
+/* Traversal statistics of `passes` passes (not accumulated into the image):
+ * out[0] closest-hit rays, out[1] nodes visited by them, out[2] segments
+ * tested by them, out[3] any-hit rays, out[4] nodes, out[5] segments. */
+int hmsnn_pt_trace_stats(hmsnn_ctx* ctx, int passes, double out[6]);
+/* Times `passes` accumulated passes with HIP events on the context's
+ * stream: ms_total, and per-kernel-class averages (ms per launch):
+ * out[0] total ms, out[1] extend, out[2] shade, out[3] shadow,
+ * out[4] launches of extend. */
+int hmsnn_pt_time_passes(hmsnn_ctx* ctx, int passes, double out[5]);
+
+/* Traces an explicit ray list against the scene (rays_dev: n x 8 floats,
+ * origin xyz, pad, direction xyz, pad). Closest hit writes prim (segment id,
+ * -1 miss, -2 head sphere), t and u; any hit writes prim = 1 if occluded. */
+int hmsnn_trace_rays_dev(hmsnn_ctx* ctx, const float* rays_dev, int n, int any_hit, int* prim_dev,
+                         float* t_dev, float* u_dev);
+
+/* ---- HairMSNN (render_hair_msnn: render_hair_msnn.cu:98-771,
+ *      cuda/hair_msnn.cu:17-357) ------------------------------------------ */
+
+typedef struct hmsnn_msnn_stats {
+  double gbuffer_ms, train_ms, infer_ms, render_ms, total_ms;
+  double loss;
+  int frames, train_steps;
+} hmsnn_msnn_stats;
+
+/* Sets BETA exactly as the CLI argument (kernel beta = BETA - 1,
+ * render_hair_msnn.cu:1170) and runs the fixed-length warm-up training. */
+int hmsnn_msnn_begin(hmsnn_ctx* ctx, int BETA);
+/* Renders `frames` frames (shuffle, G_BUFFER, one training step, inference,
+ * RENDER composite), accumulating. */
+int hmsnn_msnn_frames(hmsnn_ctx* ctx, int frames, hmsnn_msnn_stats* stats);
+/* which = 0 final mean, 1 short-path mean, 2 network mean (w x h x 4). */
+int hmsnn_msnn_get_image(hmsnn_ctx* ctx, int which, float* rgba_host);
+/* Reset, warm up, render spp frames, copy the final mean. */
+int hmsnn_render_hair_msnn(hmsnn_ctx* ctx, int BETA, int spp, float* rgba_host, hmsnn_msnn_stats* stats);
+
+/* ---- radiance-cache MLP (replaces TINY_MLP, neural_network.cuh:33-50) -- */
+
+typedef struct hmsnn_mlp hmsnn_mlp;
+typedef enum hmsnn_precision { HMSNN_FP16 = 0, HMSNN_BF16 = 1, HMSNN_FP8 = 2, HMSNN_FP32 = 3 } hmsnn_precision;
+typedef enum hmsnn_loss { HMSNN_LOSS_RELATIVE_L2_LUMINANCE = 0, HMSNN_LOSS_RELATIVE_L2 = 1, HMSNN_LOSS_L2 = 2 } hmsnn_loss;
+
+/* 12-float input records -> 64 encoded features -> 5 x 64 ReLU -> 3, exp. */
+int hmsnn_mlp_create(int precision, uint64_t seed, hmsnn_mlp** out);
+int hmsnn_mlp_destroy(hmsnn_mlp* mlp);
+/* Weights as FP32 host arrays: 5 x (64 x 64) hidden then 3 x 64 output,
+ * each row-major [out][in]. n = 20672. */
+int hmsnn_mlp_get_weights(const hmsnn_mlp* mlp, float* w_host, int n);
+int hmsnn_mlp_set_weights(hmsnn_mlp* mlp, const float* w_host, int n);
+/* Encodes 12-float records into 64 features (n x 64 FP32), for tests. */
+int hmsnn_mlp_encode_dev(const float* in_dev, int n, float* feat_dev, void* stream);
+/* Inference: in n x 12 FP32, out n x 3 FP32 (TINY_MLP::inference). */
+int hmsnn_mlp_forward_dev(hmsnn_mlp* mlp, const float* in_dev, int n, float* out_dev, void* stream);
+/* Loss and gradient w.r.t. the weights (FP32, 20672) for one batch; does not
+ * update the weights. */
+int hmsnn_mlp_backward_dev(hmsnn_mlp* mlp, const float* in_dev, const float* target_dev, int n, int loss,
+                           float* grad_dev, float* loss_out, void* stream);
+/* One Adam step (trainer->training_step): forward, backward, update. */
+int hmsnn_mlp_train_step_dev(hmsnn_mlp* mlp, const float* in_dev, const float* target_dev, int n, int loss,
+                             float lr, float* loss_out, void* stream);
+
+/* ---- differentiable appearance (path-replay adjoint) --------------------- */
+
+/* Gradient of L = 0.5 * sum over pixels and channels of
+ * (mean image - target)^2 ... see the implementation notes. dL/dimage is
+ * given as width x height x 4 floats (w ignored). Writes per-strand
+ * d/d sigma_a (num_strands x 3, FP64) and d/d (beta_m, beta_n, alpha). */
+int hmsnn_adjoint_render(hmsnn_ctx* ctx, int spp, const float* dl_dimage_host, double* grad_sigma_host,
+                         double grad_global[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HMSNN_H */

@@ -1,0 +1,262 @@
+// ORACLE - test infrastructure only (see orc_math.h).
+//
+// extern "C" surface of the oracle for the Python test harness (ctypes).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "orc_mlp.h"
+#include "orc_scene.h"
+
+using namespace orc;
+
+namespace {
+
+template <class R>
+HairParams<R> hp_from(const float* p) {
+  return HairParams<R>{(R)p[0], (R)p[1], (R)p[2], (R)p[3], (R)p[4], (R)p[5], (R)p[6]};
+}
+
+template <class R>
+void eval_batch(const float* prm, int n, const float* q, double* f, double* pdf) {
+  const HairParams<R> hp = hp_from<R>(prm);
+  const HairShading<R> hs = setup_hair(hp);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    const float* x = q + 10 * (size_t)i;
+    HairQuery<R> hq{mk((R)x[0], (R)x[1], (R)x[2]), mk((R)x[3], (R)x[4], (R)x[5]), (R)x[6],
+                    mk((R)x[7], (R)x[8], (R)x[9])};
+    R pd;
+    const Vec<R> ff = hair_eval(hp, hs, hq, &pd);
+    f[3 * (size_t)i] = (double)ff.x;
+    f[3 * (size_t)i + 1] = (double)ff.y;
+    f[3 * (size_t)i + 2] = (double)ff.z;
+    pdf[i] = (double)pd;
+  }
+}
+
+template <class R>
+void sample_batch(const float* prm, int n, const float* q, const float* u4, double* wi, double* f, double* pdf,
+                  int* lobe) {
+  const HairParams<R> hp = hp_from<R>(prm);
+  const HairShading<R> hs = setup_hair(hp);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    const float* x = q + 10 * (size_t)i;
+    const float* u = u4 + 4 * (size_t)i;
+    const Vec<R> wo = mk((R)x[0], (R)x[1], (R)x[2]);
+    const Vec<R> sig = mk((R)x[7], (R)x[8], (R)x[9]);
+    int p;
+    const Vec<R> w = hair_sample(hs, wo, (R)x[6], sig, (R)u[0], (R)u[1], (R)u[2], (R)u[3], &p);
+    HairQuery<R> hq{wo, w, (R)x[6], sig};
+    R pd;
+    const Vec<R> ff = hair_eval(hp, hs, hq, &pd);
+    wi[3 * (size_t)i] = (double)w.x;
+    wi[3 * (size_t)i + 1] = (double)w.y;
+    wi[3 * (size_t)i + 2] = (double)w.z;
+    f[3 * (size_t)i] = (double)ff.x;
+    f[3 * (size_t)i + 1] = (double)ff.y;
+    f[3 * (size_t)i + 2] = (double)ff.z;
+    pdf[i] = (double)pd;
+    lobe[i] = p;
+  }
+}
+
+}  // namespace
+
+// Mirror of the Python-side scene description.
+struct orc_scene {
+  const float* seg;
+  int nseg;
+  int has_sphere;
+  float sphere[4];
+  float sphere_albedo[3];
+  float sphere_alpha;
+  int has_env, env_pdf;
+  float env_scale, env_rot;
+  int env_w, env_h;
+  const float* env_rgba;
+  const float* c_pdf;
+  const float* c_cdf;
+  const float* m_pdf;
+  const float* m_cdf;
+  int ndl;
+  float dl_from[8][3], dl_emit[8][3];
+  float cam[12];
+  int width, height;
+  float beta_m, beta_n, alpha, gains[4];
+  float sigma_a[3];
+  const float* strand_sigma;
+  int mis, path_v1, path_v2;
+};
+
+extern "C" {
+
+uint32_t orc_path_seed(uint32_t frame, uint32_t x, uint32_t y, uint32_t w) { return get_rng(frame, x, y, w); }
+
+void orc_lcg_stream(uint32_t seed, int n, uint32_t* out_u32, float* out_f32) {
+  Lcg r{seed};
+  for (int i = 0; i < n; ++i) {
+    const uint32_t s = r.next_u32();
+    if (out_u32) out_u32[i] = s;
+    if (out_f32) out_f32[i] = std::ldexp((float)s, -32);
+  }
+}
+
+// params: beta_m, beta_n, alpha (rad), gain R, TT, TRT, TRRT
+// query: wo(3), wi(3), h, sigma_a(3) as 10 floats. prec: 0 = FP64, 1 = FP32.
+void orc_hair_eval(const float* params, int n, const float* q, int prec, double* f, double* pdf) {
+  if (prec == 0) eval_batch<double>(params, n, q, f, pdf);
+  else eval_batch<float>(params, n, q, f, pdf);
+}
+
+void orc_hair_sample(const float* params, int n, const float* q, const float* u4, int prec, double* wi, double* f,
+                     double* pdf, int* lobe) {
+  if (prec == 0) sample_batch<double>(params, n, q, u4, wi, f, pdf, lobe);
+  else sample_batch<float>(params, n, q, u4, wi, f, pdf, lobe);
+}
+
+// v[3], s, sin2k[3], cos2k[3] in FP64.
+void orc_hair_setup(const float* params, double* out) {
+  const HairShading<double> hs = setup_hair(hp_from<double>(params));
+  for (int i = 0; i < 3; ++i) out[i] = hs.v[i];
+  out[3] = hs.s;
+  for (int i = 0; i < 3; ++i) out[4 + i] = hs.sin2k[i];
+  for (int i = 0; i < 3; ++i) out[7 + i] = hs.cos2k[i];
+}
+
+double orc_bessel_i0(double x) { return I0<double>(x); }
+
+double orc_mp(double ci, double co, double si, double so, double v) { return Mp<double>(ci, co, si, so, v); }
+
+// mode 0: FP64 running sums; 1: sequential FP32 (reference order).
+void orc_env_tables(int W, int H, const float* rgba, int mode, float* c_pdf, float* c_cdf, float* m_pdf,
+                    float* m_cdf) {
+  EnvTables e;
+  e.W = W;
+  e.H = H;
+  e.rgba.assign(rgba, rgba + (size_t)W * H * 4);
+  env_tables(e, mode);
+  std::memcpy(c_pdf, e.cpdf.data(), sizeof(float) * e.cpdf.size());
+  std::memcpy(c_cdf, e.ccdf.data(), sizeof(float) * e.ccdf.size());
+  std::memcpy(m_pdf, e.mpdf.data(), sizeof(float) * e.mpdf.size());
+  std::memcpy(m_cdf, e.mcdf.data(), sizeof(float) * e.mcdf.size());
+}
+
+// Brute force (use_bvh = 0) or CPU BVH (1). Rays: 8 floats (o, pad, d, pad).
+void orc_intersect(int nseg, const float* seg, int nray, const float* rays, int any_hit, int use_bvh, int* prim,
+                   float* t, float* u) {
+  Bvh bvh;
+  if (use_bvh) bvh.build(seg, nseg);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < nray; ++i) {
+    const float* r = rays + 8 * (size_t)i;
+    const float o[3] = {r[0], r[1], r[2]}, d[3] = {r[4], r[5], r[6]};
+    if (any_hit) {
+      bool occ = false;
+      if (use_bvh) {
+        occ = bvh.any(o, d);
+      } else {
+        for (int s = 0; s < nseg && !occ; ++s) {
+          float tt, uu;
+          occ = seg_intersect(o, d, seg + 8 * (size_t)s, &tt, &uu);
+        }
+      }
+      prim[i] = occ ? 1 : 0;
+      continue;
+    }
+    float bt = 0.f, bu = 0.f;
+    int bid = -1;
+    if (use_bvh) {
+      if (!bvh.closest(o, d, &bt, &bid, &bu)) bid = -1;
+    } else {
+      for (int s = 0; s < nseg; ++s) {
+        float tt, uu;
+        if (seg_intersect(o, d, seg + 8 * (size_t)s, &tt, &uu)) {
+          if (bid < 0 || tt < bt) {  // ascending ids: ties keep the lower id
+            bt = tt;
+            bu = uu;
+            bid = s;
+          }
+        }
+      }
+    }
+    prim[i] = bid;
+    if (t) t[i] = bid >= 0 ? bt : 0.f;
+    if (u) u[i] = bid >= 0 ? bu : 0.f;
+  }
+}
+
+void* orc_bvh_create(const float* seg, int n) {
+  Bvh* b = new Bvh();
+  b->build(seg, n);
+  return b;
+}
+
+void orc_bvh_destroy(void* b) { delete (Bvh*)b; }
+
+// Renders a crop of the image. mode 0: pathTrace, 1: msnnPathTrace with
+// kernel beta kbeta, 2: msnnTrainingPath. prec 0 FP64, 1 FP32. Outputs the
+// per-pixel mean over spp passes starting at accum_id0 (RGB doubles).
+int orc_render(const orc_scene* sc, void* bvh_handle, int mode, int kbeta, int accum_id0, int spp, int x0, int y0,
+               int w, int h, int threads, int prec, double* out_full, double* out_short, uint8_t* out_hit) {
+  if (!sc || !bvh_handle || spp <= 0) return 1;
+  EnvTables env;
+  SceneDesc s{};
+  s.seg = sc->seg;
+  s.nseg = sc->nseg;
+  s.has_sphere = sc->has_sphere;
+  std::memcpy(s.sphere, sc->sphere, sizeof(s.sphere));
+  std::memcpy(s.sphere_albedo, sc->sphere_albedo, sizeof(s.sphere_albedo));
+  s.sphere_alpha = sc->sphere_alpha;
+  s.has_env = sc->has_env;
+  s.env_pdf = sc->env_pdf;
+  s.env_scale = sc->env_scale;
+  s.env_rot = sc->env_rot;
+  if (sc->has_env) {
+    env.W = sc->env_w;
+    env.H = sc->env_h;
+    const size_t np = (size_t)env.W * env.H;
+    env.rgba.assign(sc->env_rgba, sc->env_rgba + np * 4);
+    env.cpdf.assign(sc->c_pdf, sc->c_pdf + (size_t)(env.W + 1) * env.H);
+    env.ccdf.assign(sc->c_cdf, sc->c_cdf + (size_t)(env.W + 1) * env.H);
+    env.mpdf.assign(sc->m_pdf, sc->m_pdf + env.H + 1);
+    env.mcdf.assign(sc->m_cdf, sc->m_cdf + env.H + 1);
+  }
+  s.env = &env;
+  s.ndl = sc->ndl;
+  std::memcpy(s.dl_from, sc->dl_from, sizeof(s.dl_from));
+  std::memcpy(s.dl_emit, sc->dl_emit, sizeof(s.dl_emit));
+  std::memcpy(s.cam, sc->cam, sizeof(s.cam));
+  s.width = sc->width;
+  s.height = sc->height;
+  s.beta_m = sc->beta_m;
+  s.beta_n = sc->beta_n;
+  s.alpha = sc->alpha;
+  std::memcpy(s.gains, sc->gains, sizeof(s.gains));
+  std::memcpy(s.sigma_a, sc->sigma_a, sizeof(s.sigma_a));
+  s.strand_sigma = sc->strand_sigma;
+  s.mis = sc->mis;
+  s.path_v1 = sc->path_v1;
+  s.path_v2 = sc->path_v2;
+  const Bvh& bvh = *(const Bvh*)bvh_handle;
+  if (prec == 0)
+    render_pixels<double>(s, bvh, mode, kbeta, accum_id0, spp, x0, y0, w, h, threads, out_full, out_short, out_hit);
+  else
+    render_pixels<float>(s, bvh, mode, kbeta, accum_id0, spp, x0, y0, w, h, threads, out_full, out_short, out_hit);
+  return 0;
+}
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+}  // extern "C"

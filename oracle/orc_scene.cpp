@@ -1,0 +1,764 @@
+// ORACLE - test infrastructure only (see orc_math.h).
+#include "orc_scene.h"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// scene.cpp:349-425
+void env_tables(EnvTables& e, int mode) {
+  const int W = e.W, H = e.H, cw = W + 1;
+  e.cpdf.assign((size_t)cw * H, 0.f);
+  e.ccdf.assign((size_t)cw * H, 0.f);
+  e.mpdf.assign(H + 1, 0.f);
+  e.mcdf.assign(H + 1, 0.f);
+  for (int y = 0; y < H; ++y) {
+    const float sinTheta = sinf(3.14159f * (y + 0.5f) / H);
+    float* pdf = &e.cpdf[(size_t)y * cw];
+    float* cdf = &e.ccdf[(size_t)y * cw];
+    for (int x = 0; x < W; ++x) {
+      const float* t = &e.rgba[4 * ((size_t)y * W + x)];
+      pdf[x] = (t[0] + t[1] + t[2]) * (1.0f / 3.0f) * sinTheta;
+    }
+    if (mode == 1) {
+      cdf[0] = 0.f;
+      for (int x = 1; x < W; ++x) cdf[x] = cdf[x - 1] + pdf[x - 1] / W;
+      const float total = cdf[W - 1] + pdf[W - 1] / W;
+      pdf[W] = total;
+      if (total > 0.f) {
+        const float inv = 1.0f / total;
+        for (int x = 1; x < W; ++x) cdf[x] *= inv;
+      }
+    } else {
+      std::vector<double> run(W);
+      double acc = 0.0;
+      for (int x = 0; x < W; ++x) {
+        run[x] = acc;
+        acc += (double)pdf[x] / (double)W;
+      }
+      const double total = acc;
+      pdf[W] = (float)total;
+      cdf[0] = 0.f;
+      for (int x = 1; x < W; ++x) cdf[x] = (float)(total > 0.0 ? run[x] / total : run[x]);
+    }
+    cdf[W] = 1.f;
+  }
+  float* mp = e.mpdf.data();
+  float* mc = e.mcdf.data();
+  for (int y = 0; y < H; ++y) mp[y] = e.cpdf[(size_t)y * cw + W];
+  if (mode == 1) {
+    mc[0] = 0.f;
+    for (int i = 1; i < H; ++i) mc[i] = mc[i - 1] + mp[i - 1] / H;
+    const float total = mc[H - 1] + mp[H - 1] / H;
+    mp[H] = total;
+    if (total > 0.f)
+      for (int i = 1; i < H; ++i) mc[i] /= total;
+  } else {
+    std::vector<double> run(H);
+    double acc = 0.0;
+    for (int y = 0; y < H; ++y) {
+      run[y] = acc;
+      acc += (double)mp[y] / (double)H;
+    }
+    mp[H] = (float)acc;
+    mc[0] = 0.f;
+    for (int y = 1; y < H; ++y) mc[y] = (float)(acc > 0.0 ? run[y] / acc : run[y]);
+  }
+  mc[H] = 1.f;
+}
+
+// ---------------------------------------------------------------------------
+// Intersection primitives (FP32, compiled with -ffp-contract=off).
+bool seg_intersect(const float o[3], const float d[3], const float* s, float* t_out, float* u_out) {
+  const float p0x = s[0], p0y = s[1], p0z = s[2], r = s[3];
+  const float p1x = s[4], p1y = s[5], p1z = s[6];
+  const float ax = p1x - p0x, ay = p1y - p0y, az = p1z - p0z;
+  const float aa = ax * ax + ay * ay + az * az;
+  const float dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  const float cx = (p0x + p1x) * 0.5f, cy = (p0y + p1y) * 0.5f, cz = (p0z + p1z) * 0.5f;
+  const float tc = ((cx - o[0]) * d[0] + (cy - o[1]) * d[1] + (cz - o[2]) * d[2]) / dd;
+  const float qx = (o[0] + tc * d[0]) - p0x, qy = (o[1] + tc * d[1]) - p0y, qz = (o[2] + tc * d[2]) - p0z;
+  const float da = d[0] * ax + d[1] * ay + d[2] * az;
+  const float qa = qx * ax + qy * ay + qz * az;
+  const float A = dd * aa - da * da;
+  const float B = (d[0] * qx + d[1] * qy + d[2] * qz) * aa - da * qa;
+  const float C = (qx * qx + qy * qy + qz * qz) * aa - qa * qa - r * r * aa;
+  if (!(A > 0.f)) return false;
+  const float disc = B * B - A * C;
+  if (!(disc >= 0.f)) return false;
+  const float sq = sqrtf(disc);
+  const float qq = (B >= 0.f) ? -(B + sq) : -(B - sq);
+  float s0 = qq / A;
+  float s1 = (qq != 0.f) ? C / qq : s0;
+  if (s1 < s0) std::swap(s0, s1);
+  const float inv_aa = 1.f / aa;
+  const float roots[2] = {s0, s1};
+  for (float sr : roots) {
+    const float t = tc + sr;
+    const float u = (qa + sr * da) * inv_aa;
+    if (t > 0.f && u >= 0.f && u <= 1.f) {
+      *t_out = t;
+      *u_out = u;
+      return true;
+    }
+  }
+  return false;
+}
+
+bool sphere_intersect(const float o[3], const float d[3], const float c[3], float radius, float* t_out) {
+  const float dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  const float tc = ((c[0] - o[0]) * d[0] + (c[1] - o[1]) * d[1] + (c[2] - o[2]) * d[2]) / dd;
+  const float qx = (o[0] + tc * d[0]) - c[0], qy = (o[1] + tc * d[1]) - c[1], qz = (o[2] + tc * d[2]) - c[2];
+  const float C = (qx * qx + qy * qy + qz * qz) - radius * radius;
+  const float disc = -dd * C;
+  if (!(disc >= 0.f)) return false;
+  const float s = sqrtf(disc) / dd;
+  if (tc - s > 0.f) {
+    *t_out = tc - s;
+    return true;
+  }
+  if (tc + s > 0.f) {
+    *t_out = tc + s;
+    return true;
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// CPU BVH (median split, leaves of up to 4 segments). Boxes are conservative
+// (padded) and pruning keeps every candidate with entry distance <= best t,
+// so the result equals the brute-force minimum over (t, id).
+void Bvh::build(const float* s, int n) {
+  seg = s;
+  nseg = n;
+  nodes.clear();
+  idx.resize(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  if (n == 0) return;
+  std::vector<float> cen(3 * (size_t)n);
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) cen[3 * (size_t)i + k] = 0.5f * (s[8 * (size_t)i + k] + s[8 * (size_t)i + 4 + k]);
+  struct Item {
+    int node, first, count;
+  };
+  std::vector<Item> work;
+  nodes.push_back(Node{});
+  work.push_back({0, 0, n});
+  while (!work.empty()) {
+    Item it = work.back();
+    work.pop_back();
+    Node nd{};
+    for (int k = 0; k < 3; ++k) {
+      nd.lo[k] = std::numeric_limits<float>::infinity();
+      nd.hi[k] = -std::numeric_limits<float>::infinity();
+    }
+    float clo[3] = {1e30f, 1e30f, 1e30f}, chi[3] = {-1e30f, -1e30f, -1e30f};
+    for (int j = it.first; j < it.first + it.count; ++j) {
+      const float* q = s + 8 * (size_t)idx[j];
+      const float r = q[3];
+      for (int k = 0; k < 3; ++k) {
+        const float pad = r + 2e-6f * (std::fabs(q[k]) + std::fabs(q[4 + k]) + r) + 1e-6f;
+        nd.lo[k] = std::min(nd.lo[k], std::min(q[k], q[4 + k]) - pad);
+        nd.hi[k] = std::max(nd.hi[k], std::max(q[k], q[4 + k]) + pad);
+        clo[k] = std::min(clo[k], cen[3 * (size_t)idx[j] + k]);
+        chi[k] = std::max(chi[k], cen[3 * (size_t)idx[j] + k]);
+      }
+    }
+    if (it.count <= 4) {
+      nd.first = it.first;
+      nd.count = it.count;
+      nd.left = nd.right = -1;
+      nodes[it.node] = nd;
+      continue;
+    }
+    int axis = 0;
+    for (int k = 1; k < 3; ++k)
+      if (chi[k] - clo[k] > chi[axis] - clo[axis]) axis = k;
+    const int mid = it.first + it.count / 2;
+    std::nth_element(idx.begin() + it.first, idx.begin() + mid, idx.begin() + it.first + it.count,
+                     [&](int a, int b) { return cen[3 * (size_t)a + axis] < cen[3 * (size_t)b + axis]; });
+    const int l = (int)nodes.size();
+    nodes.push_back(Node{});
+    const int r = (int)nodes.size();
+    nodes.push_back(Node{});
+    nd.left = l;
+    nd.right = r;
+    nd.count = 0;
+    nodes[it.node] = nd;
+    work.push_back({l, it.first, mid - it.first});
+    work.push_back({r, mid, it.first + it.count - mid});
+  }
+}
+
+static inline bool box_hit(const Bvh::Node& n, const double o[3], const double inv[3], double tmax, double* tn) {
+  double t0 = 0.0, t1 = tmax;
+  for (int k = 0; k < 3; ++k) {
+    double a = (n.lo[k] - o[k]) * inv[k], b = (n.hi[k] - o[k]) * inv[k];
+    if (a > b) std::swap(a, b);
+    t0 = std::max(t0, a);
+    t1 = std::min(t1, b);
+  }
+  *tn = t0;
+  return t0 <= t1 * (1.0 + 1e-9) + 1e-12;
+}
+
+bool Bvh::closest(const float o[3], const float d[3], float* t, int* id, float* u) const {
+  if (nseg == 0) return false;
+  double od[3] = {o[0], o[1], o[2]}, inv[3];
+  for (int k = 0; k < 3; ++k) inv[k] = 1.0 / (d[k] != 0.f ? (double)d[k] : 1e-300);
+  float best_t = std::numeric_limits<float>::infinity(), best_u = 0.f;
+  int best_id = -1;
+  int stack[128];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp) {
+    const Node& n = nodes[stack[--sp]];
+    double tn;
+    if (!box_hit(n, od, inv, std::isinf(best_t) ? 1e300 : (double)best_t * (1.0 + 1e-6) + 1e-6, &tn)) continue;
+    if (n.count > 0) {
+      for (int j = n.first; j < n.first + n.count; ++j) {
+        const int sid = idx[j];
+        float tt, uu;
+        if (seg_intersect(o, d, seg + 8 * (size_t)sid, &tt, &uu)) {
+          if (tt < best_t || (tt == best_t && sid < best_id)) {
+            best_t = tt;
+            best_id = sid;
+            best_u = uu;
+          }
+        }
+      }
+    } else {
+      stack[sp++] = n.left;
+      stack[sp++] = n.right;
+    }
+  }
+  if (best_id < 0) return false;
+  *t = best_t;
+  *id = best_id;
+  *u = best_u;
+  return true;
+}
+
+bool Bvh::any(const float o[3], const float d[3]) const {
+  if (nseg == 0) return false;
+  double od[3] = {o[0], o[1], o[2]}, inv[3];
+  for (int k = 0; k < 3; ++k) inv[k] = 1.0 / (d[k] != 0.f ? (double)d[k] : 1e-300);
+  int stack[128];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp) {
+    const Node& n = nodes[stack[--sp]];
+    double tn;
+    if (!box_hit(n, od, inv, 1e300, &tn)) continue;
+    if (n.count > 0) {
+      for (int j = n.first; j < n.first + n.count; ++j) {
+        float tt, uu;
+        if (seg_intersect(o, d, seg + 8 * (size_t)idx[j], &tt, &uu)) return true;
+      }
+    } else {
+      stack[sp++] = n.left;
+      stack[sp++] = n.right;
+    }
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// Integrator
+template <class R>
+struct Inter {
+  bool hit = false, surface = false;
+  Vec<R> p, n, wo, wi, wo_local, wi_local;
+  Vec<R> to_local[3];
+  R radius = 0;
+  Vec<R> color;  // sigma_a (hair) or albedo (surface)
+  R alpha = 0;
+  Vec<R> Le;
+};
+
+template <class R>
+struct Ctx {
+  const SceneDesc& s;
+  const Bvh& bvh;
+  HairParams<R> hp;
+  HairShading<R> hs;
+};
+
+template <class R> inline Vec<R> apply_mat(const Vec<R> m[3], Vec<R> v) { return mk(dot(m[0], v), dot(m[1], v), dot(m[2], v)); }
+template <class R> inline Vec<R> apply_T(const Vec<R> m[3], Vec<R> v) {  // to_world = transpose
+  return mk(m[0].x * v.x + m[1].x * v.y + m[2].x * v.z, m[0].y * v.x + m[1].y * v.y + m[2].y * v.z,
+            m[0].z * v.x + m[1].z * v.y + m[2].z * v.z);
+}
+
+// optix_common.cuh:10-24 (bilinear, clamp)
+template <class R>
+Vec<R> env_radiance(const SceneDesc& s, Vec<R> dir) {
+  const EnvTables& e = *s.env;
+  const R theta = std::acos(std::fmin((R)1, std::fmax((R)-1, dir.z)));
+  R phi = std::atan2(dir.y, dir.x);
+  if (phi < 0) phi += (R)2 * Pi<R>();
+  phi += (R)s.env_rot;
+  if (phi > TwoPiEnv<R>()) phi -= TwoPiEnv<R>();
+  const R x = phi / ((R)2 * Pi<R>()), y = theta / Pi<R>();
+  const R xb = x * (R)e.W - (R)0.5, yb = y * (R)e.H - (R)0.5;
+  const R fx = std::floor(xb), fy = std::floor(yb);
+  const R a = xb - fx, b = yb - fy;
+  const int x0 = (int)fx, y0 = (int)fy;
+  auto cl = [](int v, int hi) { return std::min(std::max(v, 0), hi - 1); };
+  const int xa = cl(x0, e.W), xb1 = cl(x0 + 1, e.W), ya = cl(y0, e.H), yb1 = cl(y0 + 1, e.H);
+  auto T = [&](int xx, int yy, int c) { return (R)e.rgba[4 * ((size_t)yy * e.W + xx) + c]; };
+  Vec<R> r;
+  R* rc[3] = {&r.x, &r.y, &r.z};
+  for (int c = 0; c < 3; ++c)
+    *rc[c] = ((R)1 - a) * ((R)1 - b) * T(xa, ya, c) + a * ((R)1 - b) * T(xb1, ya, c) + ((R)1 - a) * b * T(xa, yb1, c) +
+             a * b * T(xb1, yb1, c);
+  return (R)s.env_scale * r;
+}
+
+// optix_common.cuh:26-64
+template <class R>
+R env_pdf(const SceneDesc& s, Vec<R> wi) {
+  if (!s.env_pdf) return (R)1 / ((R)4 * Pi<R>());
+  const EnvTables& e = *s.env;
+  const R theta = std::acos(std::fmin((R)1, std::fmax((R)-1, wi.z)));
+  R phi = std::atan2(wi.y, wi.x);
+  if (phi < 0) phi += (R)2 * Pi<R>();
+  phi += (R)s.env_rot;
+  if (phi > TwoPiEnv<R>()) phi -= TwoPiEnv<R>();
+  const R u = phi / ((R)2 * Pi<R>()), v = theta / Pi<R>();
+  const int iu = std::min(std::max((int)(u * e.W), 0), e.W - 1);
+  const int iv = std::min(std::max((int)(v * e.H), 0), e.H - 1);
+  const int cw = e.W + 1;
+  const R last_u = e.cpdf[(size_t)iv * cw + e.W], last_v = e.mpdf[e.H];
+  const R denom = ((R)2 * Pi<R>() * Pi<R>() * std::sin(theta)) * last_u * last_v;
+  const R cu = e.cpdf[(size_t)iv * cw + iu], cv = e.mpdf[iv];
+  return denom != 0 ? (cu * cv) / denom : (R)0;
+}
+
+static int lower_bound_m1(const float* cdf, int count, float u) {
+  int first = 0;
+  while (count > 0) {
+    const int step = count >> 1, mid = first + step;
+    if (cdf[mid] < u) {
+      first = mid + 1;
+      count -= step + 1;
+    } else {
+      count = step;
+    }
+  }
+  return std::max(0, first - 1);
+}
+
+// optix_common.cuh:66-149
+template <class R>
+Vec<R> env_sample(const SceneDesc& s, Lcg& rng, Vec<R>* wi, R* pdf) {
+  const float ux = rng.next(), uy = rng.next();
+  if (!s.env_pdf) {
+    const R z = (R)1 - (R)2 * (R)ux;
+    const R r = std::sqrt(std::fmax((R)0, (R)1 - z * z));
+    const R ph = (R)2 * Pi<R>() * (R)uy;
+    *wi = nrm(mk(r * std::cos(ph), r * std::sin(ph), z));
+    *pdf = (R)1 / ((R)4 * Pi<R>());
+    return env_radiance(s, *wi);
+  }
+  const EnvTables& e = *s.env;
+  const int cw = e.W + 1;
+  const int iv = lower_bound_m1(e.mcdf.data(), e.H, uy);
+  const R cv = e.mcdf[iv], cnv = e.mcdf[iv + 1];
+  const R v = ((R)iv + (cnv - (R)uy) / (cnv - cv)) / (R)e.H;
+  const float* row = &e.ccdf[(size_t)iv * cw];
+  const int iu = lower_bound_m1(row, e.W, ux);
+  const R cu = row[iu], cnu = row[iu + 1];
+  const R u = ((R)iu + (cnu - (R)ux) / (cnu - cu)) / (R)e.W;
+  // radiance at the sampled texture coordinate (u, v): reuse env_radiance's
+  // filter by building the direction's texture coordinates directly
+  const R xb = u * (R)e.W - (R)0.5, yb = v * (R)e.H - (R)0.5;
+  const R fx = std::floor(xb), fy = std::floor(yb);
+  const R a = xb - fx, b = yb - fy;
+  auto cl = [](int vv, int hi) { return std::min(std::max(vv, 0), hi - 1); };
+  const int xa = cl((int)fx, e.W), xb1 = cl((int)fx + 1, e.W), ya = cl((int)fy, e.H), yb1 = cl((int)fy + 1, e.H);
+  auto T = [&](int xx, int yy, int c) { return (R)e.rgba[4 * ((size_t)yy * e.W + xx) + c]; };
+  Vec<R> L;
+  R* lc[3] = {&L.x, &L.y, &L.z};
+  for (int c = 0; c < 3; ++c)
+    *lc[c] = ((R)1 - a) * ((R)1 - b) * T(xa, ya, c) + a * ((R)1 - b) * T(xb1, ya, c) + ((R)1 - a) * b * T(xa, yb1, c) +
+             a * b * T(xb1, yb1, c);
+  L = (R)s.env_scale * L;
+  const R last_u = e.cpdf[(size_t)iv * cw + e.W], last_v = e.mpdf[e.H];
+  const R pu = e.cpdf[(size_t)iv * cw + iu], pv = e.mpdf[iv];
+  const R theta = Pi<R>() * v;
+  const R st = std::sin(theta);
+  const R denom = ((R)2 * Pi<R>() * Pi<R>() * st) * last_u * last_v;
+  *pdf = denom != 0 ? (pu * pv) / denom : (R)0;
+  R phi = (R)2 * Pi<R>() * u - (R)s.env_rot;
+  if (phi < 0) phi = TwoPiEnv<R>() + phi;
+  *wi = mk(st * std::cos(phi), st * std::sin(phi), std::cos(theta));
+  return L;
+}
+
+template <class R>
+void setup_hit(const Ctx<R>& c, Vec<R> o, Vec<R> d, float t, int prim, float u, Inter<R>& si) {
+  const SceneDesc& s = c.s;
+  si.hit = true;
+  si.wo = (R)-1 * d;
+  Vec<R> p = o + (R)t * d;
+  if (prim == -2) {  // head sphere (triangleMeshCH analogue, optix_common.cuh:527-556)
+    const Vec<R> cc = mk((R)s.sphere[0], (R)s.sphere[1], (R)s.sphere[2]);
+    si.n = nrm(p - cc);
+    si.p = cc + (R)s.sphere[3] * si.n;
+    const Vec<R> n = si.n;
+    Vec<R> c1, c2;
+    if (n.z < (R)-0.999999f) {  // orthonormalBasis, utils.cuh:232-259
+      c1 = mk((R)0, (R)-1, (R)0);
+      c2 = mk((R)-1, (R)0, (R)0);
+    } else {
+      const R a = (R)1 / ((R)1 + n.z);
+      const R b = -n.x * n.y * a;
+      c1 = nrm(mk((R)1 - n.x * n.x * a, b, -n.x));
+      c2 = nrm(mk(b, (R)1 - n.y * n.y * a, -n.y));
+    }
+    si.to_local[0] = c1;
+    si.to_local[1] = c2;
+    si.to_local[2] = n;
+    si.wo_local = nrm(apply_mat(si.to_local, si.wo));
+    si.color = mk((R)s.sphere_albedo[0], (R)s.sphere_albedo[1], (R)s.sphere_albedo[2]);
+    si.alpha = (R)s.sphere_alpha;
+    si.surface = true;
+    si.radius = 0;
+    return;
+  }
+  // hairCH + computeCurveIntersection, linear branch (optix_common.cuh:558-599,
+  // curve_utils.cuh:242-255, surfaceNormal :159-197)
+  const float* q = s.seg + 8 * (size_t)prim;
+  const Vec<R> p0 = mk((R)q[0], (R)q[1], (R)q[2]), p1 = mk((R)q[4], (R)q[5], (R)q[6]);
+  const Vec<R> dv = p1 - p0;
+  const R r = (R)q[3];
+  Vec<R> n;
+  if (u == 0.f) {
+    n = nrm((R)-1 * dv);
+  } else if (u == 1.f) {
+    n = nrm(dv);
+  } else {
+    const Vec<R> pa = p0 + (R)u * dv;
+    const R dd = dot(dv, dv);
+    Vec<R> o1 = p - pa;
+    o1 = o1 - (dot(o1, dv) / dd) * dv;
+    o1 = (r / len(o1)) * o1;
+    p = pa + o1;
+    n = nrm(dd * o1);
+  }
+  const Vec<R> X = nrm(dv);
+  const Vec<R> Y = nrm(cross(si.wo, X));
+  const Vec<R> Z = nrm(cross(X, Y));
+  si.p = p;
+  si.n = n;
+  si.to_local[0] = X;
+  si.to_local[1] = Y;
+  si.to_local[2] = Z;
+  si.wo_local = nrm(apply_mat(si.to_local, si.wo));
+  si.radius = r;
+  if (s.strand_sigma) {
+    const int strand = (int)q[7];
+    si.color = mk((R)s.strand_sigma[3 * strand], (R)s.strand_sigma[3 * strand + 1], (R)s.strand_sigma[3 * strand + 2]);
+  } else {
+    si.color = mk((R)s.sigma_a[0], (R)s.sigma_a[1], (R)s.sigma_a[2]);
+  }
+  si.surface = false;
+}
+
+template <class R>
+Vec<R> bsdf_eval(const Ctx<R>& c, const Inter<R>& si, R* pdf) {
+  if (si.surface) {
+    *pdf = ggx_pdf(si.alpha, si.wo_local, nrm(si.wo_local + si.wi_local));
+    return frostbite(si.wo_local, si.wi_local, si.color, si.alpha);
+  }
+  HairQuery<R> q{si.wo_local, si.wi_local, dot(si.to_local[1], si.n), si.color};
+  return hair_eval(c.hp, c.hs, q, pdf);
+}
+
+// sample_disney_hair / sample_GGX with the draw order of
+// optix_common.cuh:154-170 and :314-329.
+template <class R>
+Vec<R> bsdf_sample(const Ctx<R>& c, Inter<R>& si, Lcg& rng, R* pdf) {
+  const float r0 = rng.next(), r1 = rng.next();
+  if (!si.surface) {
+    const float r2 = rng.next(), r3 = rng.next();
+    int lobe;
+    si.wi_local = hair_sample(c.hs, si.wo_local, dot(si.to_local[1], si.n), si.color, (R)r0, (R)r1, (R)r2, (R)r3, &lobe);
+    si.wi = nrm(apply_T(si.to_local, si.wi_local));
+    HairQuery<R> q{si.wo_local, si.wi_local, dot(si.to_local[1], si.n), si.color};
+    return hair_eval(c.hp, c.hs, q, pdf);
+  }
+  si.wi_local = ggx_sample(si.alpha, si.wo_local, (R)r0, (R)r1, pdf);
+  si.wi = nrm(apply_T(si.to_local, si.wi_local));
+  return frostbite(si.wo_local, si.wi_local, si.color, si.alpha);
+}
+
+template <class R>
+bool occluded(const Ctx<R>& c, Vec<R> o, Vec<R> d) {
+  const float of[3] = {(float)o.x, (float)o.y, (float)o.z}, df[3] = {(float)d.x, (float)d.y, (float)d.z};
+  if (c.s.has_sphere) {
+    float t;
+    if (sphere_intersect(of, df, c.s.sphere, c.s.sphere[3], &t)) return true;
+  }
+  return c.bvh.any(of, df);
+}
+
+template <class R>
+void trace(const Ctx<R>& c, Vec<R> o, Vec<R> d, Inter<R>& si) {
+  const float of[3] = {(float)o.x, (float)o.y, (float)o.z}, df[3] = {(float)d.x, (float)d.y, (float)d.z};
+  float best = std::numeric_limits<float>::infinity();
+  int prim = -1;
+  float u = 0.f;
+  if (c.s.has_sphere) {
+    float t;
+    if (sphere_intersect(of, df, c.s.sphere, c.s.sphere[3], &t)) {
+      best = t;
+      prim = -2;
+    }
+  }
+  float th, uh;
+  int id;
+  if (c.bvh.closest(of, df, &th, &id, &uh) && th <= best) {
+    best = th;
+    prim = id;
+    u = uh;
+  }
+  if (prim == -1) {
+    si.hit = false;
+    si.Le = c.s.has_env ? env_radiance<R>(c.s, d) : mk((R)0, (R)0, (R)0);
+    return;
+  }
+  setup_hit(c, o, d, best, prim, u, si);
+}
+
+// sampleBSDF (optix_common.cuh:151-203); si is the direct-lighting copy.
+template <class R>
+Vec<R> sample_bsdf_light(const Ctx<R>& c, Inter<R>& si, Lcg& rng) {
+  R bsdfPdf = 1;
+  const Vec<R> bsdf = bsdf_sample(c, si, rng, &bsdfPdf);
+  const bool lower = dot(si.wi, si.n) < 0;
+  Vec<R> nd = si.n;
+  if (si.surface && lower) return mk((R)0, (R)0, (R)0);
+  if (!si.surface && lower) {
+    nd = (R)-1 * si.n;
+    si.p = si.p + ((R)2 * si.radius) * nd;
+  }
+  if (occluded(c, si.p + (R)1e-3f * nd, si.wi)) return mk((R)0, (R)0, (R)0);
+  const Vec<R> emit = env_radiance<R>(c.s, si.wi);
+  const R lightPdf = env_pdf<R>(c.s, si.wi) * (R)1 / (R)(c.s.ndl + (c.s.has_env ? 1 : 0));
+  const R f2 = bsdfPdf * bsdfPdf, g2 = lightPdf * lightPdf;
+  if (c.s.mis) return (f2 / (f2 + g2) / bsdfPdf) * (bsdf * emit);
+  return (bsdf * emit) / bsdfPdf;
+}
+
+// sampleLights (optix_common.cuh:205-277)
+template <class R>
+Vec<R> sample_lights(const Ctx<R>& c, Inter<R>& si, Lcg& rng, bool& isDelta) {
+  isDelta = false;
+  const int ntot = c.s.ndl + (c.s.has_env ? 1 : 0);
+  const float rsel = rng.next();
+  int sel = (int)std::floor(rsel * (float)ntot);
+  if (!c.s.has_env && sel >= c.s.ndl) sel = c.s.ndl - 1;
+  R lightPdf = (R)1 * (R)1 / (R)ntot;
+  Vec<R> emit;
+  if (sel >= c.s.ndl) {
+    R envPdf = 1;
+    Vec<R> wi;
+    emit = env_sample<R>(c.s, rng, &wi, &envPdf);
+    si.wi = wi;
+    si.wi_local = nrm(apply_mat(si.to_local, si.wi));
+    lightPdf = lightPdf * envPdf;
+  } else {
+    emit = mk((R)c.s.dl_emit[sel][0], (R)c.s.dl_emit[sel][1], (R)c.s.dl_emit[sel][2]);
+    si.wi = nrm(mk((R)c.s.dl_from[sel][0], (R)c.s.dl_from[sel][1], (R)c.s.dl_from[sel][2]));
+    si.wi_local = nrm(apply_mat(si.to_local, si.wi));
+    isDelta = true;
+  }
+  const bool lower = dot(si.wi, si.n) < 0;
+  Vec<R> nd = si.n;
+  if (si.surface && lower) return mk((R)0, (R)0, (R)0);
+  if (!si.surface && lower) {
+    nd = (R)-1 * si.n;
+    si.p = si.p + ((R)2 * si.radius) * nd;
+  }
+  if (occluded(c, si.p + (R)1e-3f * nd, si.wi)) return mk((R)0, (R)0, (R)0);
+  R bsdfPdf = 1;
+  const Vec<R> bsdf = bsdf_eval(c, si, &bsdfPdf);
+  if (c.s.mis && !isDelta) {
+    const R f2 = lightPdf * lightPdf, g2 = bsdfPdf * bsdfPdf;
+    return (f2 / (f2 + g2) / lightPdf) * (bsdf * emit);
+  }
+  return (bsdf * emit) / lightPdf;
+}
+
+// directLighting (optix_common.cuh:279-309): si by value.
+template <class R>
+Vec<R> direct_lighting(const Ctx<R>& c, Inter<R> si, Lcg& rng) {
+  Vec<R> out;
+  if (c.s.mis) {
+    bool isDelta = false;
+    const Vec<R> ls = sample_lights(c, si, rng, isDelta);
+    Vec<R> bs = mk((R)0, (R)0, (R)0);
+    if (!isDelta) bs = sample_bsdf_light(c, si, rng);
+    out = ls + bs;
+  } else {
+    bool isDelta = false;
+    out = sample_lights(c, si, rng, isDelta);
+  }
+  if (has_nan(out)) out = mk((R)0, (R)0, (R)0);
+  return out;
+}
+
+// nextPathVertex (optix_common.cuh:311-360); returns mulFac.
+template <class R>
+Vec<R> next_vertex(const Ctx<R>& c, Inter<R>& si, Lcg& rng) {
+  R pdf = 1;
+  const Vec<R> f = bsdf_sample(c, si, rng, &pdf);
+  Vec<R> mul = pdf == 0 ? f : f / pdf;
+  if (has_nan(mul)) mul = mk((R)1, (R)1, (R)1);
+  const Vec<R> wi = si.wi;
+  Vec<R> nd = si.n;
+  if (dot(wi, si.n) < 0 && !si.surface) {
+    nd = (R)-1 * si.n;
+    si.p = si.p + ((R)2 * si.radius) * nd;
+  }
+  const Vec<R> o = si.p + (R)1e-3f * nd;
+  Inter<R> nx;
+  nx.wi = wi;
+  trace(c, o, wi, nx);
+  si = nx;
+  return mul;
+}
+
+template <class R> inline R lum(Vec<R> v) { return (R)0.212671f * v.x + (R)0.715160f * v.y + (R)0.072169f * v.z; }
+
+// pathTrace (optix_common.cuh:362-409)
+template <class R>
+Vec<R> path_trace(const Ctx<R>& c, Inter<R> si, Lcg& rng, int v1, int v2) {
+  if (v2 < v1) return mk((R)0, (R)0, (R)0);
+  Vec<R> beta = mk((R)1, (R)1, (R)1), col = mk((R)0, (R)0, (R)0);
+  if (v1 == 0) col = direct_lighting(c, si, rng);
+  for (int b = 1; b <= v2; ++b) {
+    beta = beta * next_vertex(c, si, rng);
+    if (!si.hit) break;
+    if (b >= v1) col = col + beta * direct_lighting(c, si, rng);
+    const R q = std::fmax((R)0.05f, (R)1 - lum(beta));
+    const float eps = rng.next();
+    if ((R)eps < q) break;
+    beta = beta / ((R)1 - q);
+  }
+  return col;
+}
+
+// msnnPathTrace (hair_msnn.cu:134-185)
+template <class R>
+Vec<R> msnn_path(const Ctx<R>& c, Inter<R> si, Lcg& rng, int v2, int kbeta) {
+  Vec<R> beta = mk((R)1, (R)1, (R)1);
+  Vec<R> col = direct_lighting(c, si, rng);
+  for (int b = 1; b <= v2; ++b) {
+    beta = beta * next_vertex(c, si, rng);
+    if (!si.hit) break;
+    col = col + beta * direct_lighting(c, si, rng);
+    const R q = std::fmax((R)0.05f, (R)1 - lum(beta));
+    const float eps = rng.next();
+    if ((R)eps < q || b > kbeta) break;
+    beta = beta / ((R)1 - q);
+  }
+  return col;
+}
+
+// msnnTrainingPath (hair_msnn.cu:67-132)
+template <class R>
+Vec<R> msnn_training(const Ctx<R>& c, Inter<R> si, Lcg& rng, int v2, int kbeta, Vec<R>& shortc) {
+  Vec<R> beta = mk((R)1, (R)1, (R)1), bs = beta;
+  Vec<R> col = direct_lighting(c, si, rng);
+  shortc = col;
+  for (int b = 1; b <= v2; ++b) {
+    const Vec<R> mul = next_vertex(c, si, rng);
+    beta = beta * mul;
+    bs = bs * mul;
+    if (!si.hit) break;
+    const Vec<R> dl = direct_lighting(c, si, rng);
+    col = col + beta * dl;
+    shortc = shortc + bs * dl;
+    const R q = std::fmax((R)0.05f, (R)1 - lum(beta));
+    const R qs = std::fmax((R)0.05f, (R)1 - lum(bs));
+    const float eps = rng.next();
+    if ((R)eps < qs || b > kbeta) bs = mk((R)0, (R)0, (R)0);
+    if ((R)eps < q) break;
+    beta = beta / ((R)1 - q);
+    if (!(bs.x == 0 && bs.y == 0 && bs.z == 0)) bs = bs / ((R)1 - qs);
+  }
+  return col;
+}
+
+template <class R>
+void render_pixels(const SceneDesc& s, const Bvh& bvh, int mode, int kbeta, int accum_id0, int spp, int x0, int y0,
+                   int w, int h, int threads, double* out_full, double* out_short, uint8_t* out_hit) {
+  Ctx<R> c{s, bvh, {}, {}};
+  c.hp = HairParams<R>{(R)s.beta_m, (R)s.beta_n, (R)s.alpha, (R)s.gains[0], (R)s.gains[1], (R)s.gains[2], (R)s.gains[3]};
+  c.hs = setup_hair(c.hp);
+  const Vec<R> cp = mk((R)s.cam[0], (R)s.cam[1], (R)s.cam[2]);
+  const Vec<R> d00 = mk((R)s.cam[3], (R)s.cam[4], (R)s.cam[5]);
+  const Vec<R> du = mk((R)s.cam[6], (R)s.cam[7], (R)s.cam[8]);
+  const Vec<R> dv = mk((R)s.cam[9], (R)s.cam[10], (R)s.cam[11]);
+  const int v1 = s.path_v1 - 1, v2 = s.path_v2 - 1;
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (int yy = 0; yy < h; ++yy) {
+    for (int xx = 0; xx < w; ++xx) {
+      const int px = x0 + xx, py = y0 + yy;
+      double acc[3] = {0, 0, 0}, accs[3] = {0, 0, 0};
+      uint8_t hitflag = 0;
+      for (int k = 0; k < spp; ++k) {
+        Lcg rng{get_rng((uint32_t)(accum_id0 + k + 10007), (uint32_t)px, (uint32_t)py, (uint32_t)s.width)};
+        const float jx = rng.next(), jy = rng.next();
+        const float sx = ((float)px + jx) / (float)s.width, sy = ((float)py + jy) / (float)s.height;
+        const Vec<R> d = nrm(d00 + (R)sx * du + (R)sy * dv);
+        Inter<R> si;
+        si.wi = d;
+        trace(c, cp, d, si);
+        Vec<R> col = mk((R)0, (R)0, (R)0), sc = col;
+        if (!si.hit) {
+          col = si.Le;
+          sc = col;
+        } else {
+          hitflag = (uint8_t)(1 | (si.surface ? 2 : 0));
+          if (mode == 0) col = path_trace(c, si, rng, v1, v2);
+          else if (mode == 1) col = msnn_path(c, si, rng, v2, kbeta);
+          else col = msnn_training(c, si, rng, v2, kbeta, sc);
+        }
+        if (has_nan(col)) col = mk((R)0, (R)0, (R)0);  // writePixel scrub (utils.cuh:20-21)
+        acc[0] += (double)col.x;
+        acc[1] += (double)col.y;
+        acc[2] += (double)col.z;
+        accs[0] += (double)sc.x;
+        accs[1] += (double)sc.y;
+        accs[2] += (double)sc.z;
+      }
+      const size_t o = (size_t)yy * w + xx;
+      for (int k = 0; k < 3; ++k) {
+        out_full[3 * o + k] = acc[k] / spp;
+        if (out_short) out_short[3 * o + k] = accs[k] / spp;
+      }
+      if (out_hit) out_hit[o] = hitflag;
+    }
+  }
+}
+
+template void render_pixels<float>(const SceneDesc&, const Bvh&, int, int, int, int, int, int, int, int, int, double*,
+                                   double*, uint8_t*);
+template void render_pixels<double>(const SceneDesc&, const Bvh&, int, int, int, int, int, int, int, int, int, double*,
+                                    double*, uint8_t*);
+
+}  // namespace orc

@@ -1,0 +1,89 @@
+// ORACLE - test infrastructure only (see orc_math.h).
+//
+// Environment tables and sampling, hair-segment / sphere intersection, a CPU
+// BVH, and the path integrators, restating:
+//   scene.cpp:349-425           generateEnvSamplingTables
+//   optix_common.cuh:10-149     environment radiance / pdf / sampling
+//   optix_common.cuh:151-409    sampleBSDF, sampleLights, directLighting,
+//                               nextPathVertex, pathTrace
+//   optix_common.cuh:527-599    closest-hit programs (hit frames)
+//   curve_utils.cuh:159-197     hit-point refinement and normal (linear)
+//   cuda/path_tracing.cu:19-67  camera ray and per-pixel seed
+//   cuda/hair_msnn.cu:17-185    truncated and training paths
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "orc_math.h"
+
+namespace orc {
+
+// ---- environment ----------------------------------------------------------
+struct EnvTables {
+  int W = 0, H = 0;
+  std::vector<float> rgba;               // W*H*4
+  std::vector<float> cpdf, ccdf;         // (W+1)*H
+  std::vector<float> mpdf, mcdf;         // H+1
+};
+// mode 0: running sums in FP64, rounded once (the GPU build's scheme);
+// mode 1: the reference's sequential FP32 sums (scene.cpp:349-425).
+void env_tables(EnvTables& e, int mode);
+
+// ---- geometry ---------------------------------------------------------------
+// Ray / finite open cylinder around p0->p1 (constant radius r): nearest root
+// t > 0 with axial u in [0,1]. IEEE FP32, no contraction (build flag), with
+// the root solved around the ray point nearest the segment centre.
+bool seg_intersect(const float o[3], const float d[3], const float* seg, float* t, float* u);
+bool sphere_intersect(const float o[3], const float d[3], const float c[3], float r, float* t);
+
+struct Bvh {
+  struct Node {
+    float lo[3], hi[3];
+    int left, right;  // children when count == 0
+    int first, count; // leaf range into idx
+  };
+  std::vector<Node> nodes;
+  std::vector<int> idx;
+  const float* seg = nullptr;  // 8 floats per segment
+  int nseg = 0;
+  void build(const float* seg, int n);
+  // minimum (t, id) over all segments, like a brute-force scan
+  bool closest(const float o[3], const float d[3], float* t, int* id, float* u) const;
+  bool any(const float o[3], const float d[3]) const;
+};
+
+// ---- scene and integrators --------------------------------------------------
+struct SceneDesc {
+  // geometry
+  const float* seg;  // 8 floats per segment
+  int nseg;
+  int has_sphere;
+  float sphere[4];
+  float sphere_albedo[3];
+  float sphere_alpha;  // already clamped and squared
+  // environment
+  int has_env, env_pdf;
+  float env_scale, env_rot;
+  const EnvTables* env;
+  // lights
+  int ndl;
+  float dl_from[8][3], dl_emit[8][3];
+  // camera
+  float cam[12];
+  int width, height;
+  // material and integrator
+  float beta_m, beta_n, alpha, gains[4];
+  float sigma_a[3];
+  const float* strand_sigma;  // optional, 3 per strand
+  int mis;
+  int path_v1, path_v2;
+};
+
+// mode 0: pathTrace; mode 1: msnnPathTrace truncated at kernel beta;
+// mode 2: msnnTrainingPath (full and short colours).
+template <class R>
+void render_pixels(const SceneDesc& s, const Bvh& bvh, int mode, int kbeta, int accum_id0, int spp, int x0,
+                   int y0, int w, int h, int threads, double* out_full, double* out_short, uint8_t* out_hit);
+
+}  // namespace orc

@@ -1,0 +1,175 @@
+"""ctypes loader for the CPU oracle (oracle/_build/liborc.so).
+
+The oracle is test infrastructure: a C++ restatement of the reference's
+kernels used only as the checker and as the CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+ORC_DIR = ROOT / "oracle"
+ORC_LIB = ORC_DIR / "_build" / "liborc.so"
+
+_P = C.c_void_p
+_I = C.c_int
+_F = C.c_float
+_D = C.c_double
+
+
+class OrcScene(C.Structure):
+    _fields_ = [
+        ("seg", _P), ("nseg", _I), ("has_sphere", _I), ("sphere", _F * 4), ("sphere_albedo", _F * 3),
+        ("sphere_alpha", _F), ("has_env", _I), ("env_pdf", _I), ("env_scale", _F), ("env_rot", _F),
+        ("env_w", _I), ("env_h", _I), ("env_rgba", _P), ("c_pdf", _P), ("c_cdf", _P), ("m_pdf", _P),
+        ("m_cdf", _P), ("ndl", _I), ("dl_from", (_F * 3) * 8), ("dl_emit", (_F * 3) * 8), ("cam", _F * 12),
+        ("width", _I), ("height", _I), ("beta_m", _F), ("beta_n", _F), ("alpha", _F), ("gains", _F * 4),
+        ("sigma_a", _F * 3), ("strand_sigma", _P), ("mis", _I), ("path_v1", _I), ("path_v2", _I),
+    ]
+
+
+_SIGS = {
+    "orc_path_seed": (C.c_uint32, [C.c_uint32] * 4),
+    "orc_lcg_stream": (None, [C.c_uint32, _I, _P, _P]),
+    "orc_hair_eval": (None, [_P, _I, _P, _I, _P, _P]),
+    "orc_hair_sample": (None, [_P, _I, _P, _P, _I, _P, _P, _P, _P]),
+    "orc_hair_setup": (None, [_P, _P]),
+    "orc_bessel_i0": (_D, [_D]),
+    "orc_mp": (_D, [_D, _D, _D, _D, _D]),
+    "orc_env_tables": (None, [_I, _I, _P, _I, _P, _P, _P, _P]),
+    "orc_intersect": (None, [_I, _P, _I, _P, _I, _I, _P, _P, _P]),
+    "orc_bvh_create": (_P, [_P, _I]),
+    "orc_bvh_destroy": (None, [_P]),
+    "orc_render": (_I, [C.POINTER(OrcScene), _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P]),
+    "orc_max_threads": (_I, []),
+}
+
+_lib = None
+
+
+def build_oracle() -> None:
+    subprocess.run(["make", "-s", "-C", str(ORC_DIR)], check=True)
+
+
+def load_oracle() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not ORC_LIB.exists():
+        build_oracle()
+    lib = C.CDLL(str(ORC_LIB))
+    for n, (r, a) in _SIGS.items():
+        f = getattr(lib, n)
+        f.restype = r
+        f.argtypes = a
+    _lib = lib
+    return lib
+
+
+def ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+def hair_params(beta_m=0.3, beta_n=0.3, alpha=0.0, gains=(1, 1, 1, 1)) -> np.ndarray:
+    return np.array([beta_m, beta_n, alpha, *gains], dtype=np.float32)
+
+
+def hair_eval(q: np.ndarray, params: np.ndarray, prec: int = 0):
+    lib = load_oracle()
+    q = np.ascontiguousarray(q, np.float32)
+    n = q.shape[0]
+    f = np.empty((n, 3)), np.empty(n)
+    lib.orc_hair_eval(ptr(params), n, ptr(q), prec, ptr(f[0]), ptr(f[1]))
+    return f
+
+
+def hair_sample(q: np.ndarray, u4: np.ndarray, params: np.ndarray, prec: int = 0):
+    lib = load_oracle()
+    q = np.ascontiguousarray(q, np.float32)
+    u4 = np.ascontiguousarray(u4, np.float32)
+    n = q.shape[0]
+    wi, f, pdf, lobe = np.empty((n, 3)), np.empty((n, 3)), np.empty(n), np.empty(n, np.int32)
+    lib.orc_hair_sample(ptr(params), n, ptr(q), ptr(u4), prec, ptr(wi), ptr(f), ptr(pdf), ptr(lobe))
+    return wi, f, pdf, lobe
+
+
+def env_tables(rgba: np.ndarray, mode: int = 0):
+    lib = load_oracle()
+    H, W = rgba.shape[:2]
+    rgba = np.ascontiguousarray(rgba, np.float32)
+    cp, cc = np.empty((H, W + 1), np.float32), np.empty((H, W + 1), np.float32)
+    mp, mc = np.empty(H + 1, np.float32), np.empty(H + 1, np.float32)
+    lib.orc_env_tables(W, H, ptr(rgba), mode, ptr(cp), ptr(cc), ptr(mp), ptr(mc))
+    return {"c_pdf": cp, "c_cdf": cc, "m_pdf": mp, "m_cdf": mc}
+
+
+def intersect(seg: np.ndarray, rays: np.ndarray, any_hit=False, use_bvh=True):
+    lib = load_oracle()
+    seg = np.ascontiguousarray(seg, np.float32)
+    rays = np.ascontiguousarray(rays, np.float32)
+    n = rays.shape[0]
+    prim, t, u = np.empty(n, np.int32), np.empty(n, np.float32), np.empty(n, np.float32)
+    lib.orc_intersect(seg.shape[0], ptr(seg), n, ptr(rays), int(any_hit), int(use_bvh), ptr(prim), ptr(t), ptr(u))
+    return prim if any_hit else (prim, t, u)
+
+
+class OracleScene:
+    """Scene description for the oracle integrators, filled from a context's exports."""
+
+    def __init__(self, info: dict, seg: np.ndarray, env: dict | None, strand_sigma: np.ndarray | None = None):
+        self.lib = load_oracle()
+        self.seg = np.ascontiguousarray(seg, np.float32)
+        self.env = {k: np.ascontiguousarray(v, np.float32) for k, v in env.items()} if env else None
+        self.strand_sigma = None if strand_sigma is None else np.ascontiguousarray(strand_sigma, np.float32)
+        s = OrcScene()
+        s.seg = ptr(self.seg)
+        s.nseg = self.seg.shape[0]
+        s.has_sphere = int(info["has_sphere"])
+        for i in range(4):
+            s.sphere[i] = float(info["sphere"][i])
+        for i in range(3):
+            s.sphere_albedo[i] = float(info["sphere_albedo"][i])
+            s.sigma_a[i] = float(info["sigma_a"][i])
+        s.sphere_alpha = float(info["sphere_alpha"])
+        s.has_env = int(info["has_env"])
+        s.env_pdf = int(info["env_pdf"])
+        s.env_scale = float(info["env_scale"])
+        s.env_rot = float(info["env_rot_phi"])
+        if self.env:
+            s.env_h, s.env_w = self.env["rgba"].shape[:2]
+            s.env_rgba, s.c_pdf, s.c_cdf = ptr(self.env["rgba"]), ptr(self.env["c_pdf"]), ptr(self.env["c_cdf"])
+            s.m_pdf, s.m_cdf = ptr(self.env["m_pdf"]), ptr(self.env["m_cdf"])
+        s.ndl = int(info["num_dir_lights"])
+        for i in range(s.ndl):
+            for k in range(3):
+                s.dl_from[i][k] = float(info["dir_from"][i][k])
+                s.dl_emit[i][k] = float(info["dir_emit"][i][k])
+        for i in range(12):
+            s.cam[i] = float(info["camera"][i])
+        s.width, s.height = int(info["width"]), int(info["height"])
+        s.beta_m, s.beta_n, s.alpha = float(info["beta_m"]), float(info["beta_n"]), float(info["alpha"])
+        for i in range(4):
+            s.gains[i] = float(info["gain"][i])
+        s.strand_sigma = ptr(self.strand_sigma) if self.strand_sigma is not None else None
+        s.mis, s.path_v1, s.path_v2 = int(info["mis"]), int(info["path_v1"]), int(info["path_v2"])
+        self.s = s
+        self.bvh = self.lib.orc_bvh_create(ptr(self.seg), self.seg.shape[0])
+
+    def __del__(self):
+        try:
+            self.lib.orc_bvh_destroy(self.bvh)
+        except Exception:
+            pass
+
+    def render(self, x0, y0, w, h, spp=1, accum_id0=0, mode=0, kbeta=1 << 30, prec=0, threads=0):
+        full = np.zeros((h, w, 3))
+        short = np.zeros((h, w, 3))
+        hit = np.zeros((h, w), np.uint8)
+        rc = self.lib.orc_render(C.byref(self.s), self.bvh, mode, kbeta, accum_id0, spp, x0, y0, w, h, threads, prec,
+                                 ptr(full), ptr(short), ptr(hit))
+        assert rc == 0
+        return full, short, hit

@@ -1,0 +1,155 @@
+"""C-ABI library: loads, exports every declared entry point, and parses scene
+configs with the reference's defaults and failure rules (scene.cpp:119-339).
+Host-only: nothing here launches GPU work."""
+import ctypes as C
+import json
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def header_symbols():
+    text = (ROOT / "include" / "hmsnn.h").read_text()
+    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(hmsnn_[a-z0-9_]+)\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    import hairmsnn_amd as hm
+    lib = hm._lib.load()
+    syms = header_symbols()
+    assert len(syms) >= 40
+    for s in syms:
+        assert hasattr(lib, s), s
+    # the Python bindings cover exactly the declared entry points
+    assert sorted(hm.declared_symbols()) == syms
+
+
+def test_version_and_error_string():
+    import hairmsnn_amd as hm
+    lib = hm._lib.load()
+    assert lib.hmsnn_version() >= 10000
+    with pytest.raises(hm.HmsnnError) as e:
+        hm.parse_scene("{not json")
+    assert "json" in str(e.value).lower()
+
+
+def test_parse_reference_scene_config():
+    import hairmsnn_amd as hm
+    cfg = json.loads((GOLDEN / "ref_scene_curly.json").read_text())
+    info = hm.parse_scene(json.dumps(cfg))
+    assert info["width"] == 1024 and info["height"] == 1024 and info["spp"] == 500
+    assert info["path_v1"] == 1 and info["path_v2"] == 40
+    assert info["mis"] == 1 and info["env_pdf"] == 1
+    np.testing.assert_allclose(info["sigma_a"], [0.06, 0.1, 0.2], rtol=1e-7)
+    assert info["beta_m"] == pytest.approx(0.3) and info["beta_n"] == pytest.approx(0.3)
+    # alpha = 3.14159f * deg / 180 (scene.cpp:207)
+    assert info["alpha"] == pytest.approx(np.float32(np.float32(3.14159) * np.float32(2.0)) / np.float32(180.0), rel=1e-7)
+    np.testing.assert_allclose(info["gain"], [1, 1, 1, 1])
+    assert info["has_env"] == 1 and info["env_scale"] == pytest.approx(1.0)
+    assert info["num_dir_lights"] == 1
+    d = np.array([3, 3, 3]) / math.sqrt(27)
+    np.testing.assert_allclose(info["dir_from"][0], d, rtol=1e-6)
+    np.testing.assert_allclose(info["dir_emit"][0], [1, 1, 1])
+    assert info["has_sphere"] == 0  # an OBJ surface, not the sphere proxy
+
+
+def test_camera_vectors_match_closed_form():
+    """render_path_tracing.cu:767-799 camera basis."""
+    import hairmsnn_amd as hm
+    cfg = json.loads((GOLDEN / "ref_scene_curly.json").read_text())
+    info = hm.parse_scene(json.dumps(cfg))
+    frm = np.array(cfg["camera"]["from"], np.float64)
+    to = np.array(cfg["camera"]["to"], np.float64)
+    up = np.array(cfg["camera"]["up"], np.float64)
+    cf = cfg["camera"]["cos_fovy"]
+    d00 = (to - frm) / np.linalg.norm(to - frm)
+    c = np.cross(d00, up)
+    du = cf * 1.0 * c / np.linalg.norm(c)
+    c2 = np.cross(du, d00)
+    dv = cf * c2 / np.linalg.norm(c2)
+    d00 = d00 - 0.5 * du - 0.5 * dv
+    cam = info["camera"]
+    np.testing.assert_allclose(cam[0:3], frm, rtol=1e-6)
+    np.testing.assert_allclose(cam[3:6], d00, atol=2e-6)
+    np.testing.assert_allclose(cam[6:9], du, atol=2e-6)
+    np.testing.assert_allclose(cam[9:12], dv, atol=2e-6)
+
+
+def base_cfg():
+    return json.loads((GOLDEN / "ref_scene_curly.json").read_text())
+
+
+def test_missing_hair_type_keeps_struct_defaults():
+    """Without "type" the hair block is abandoned: alpha stays 0.03 rad (scene.h:42)."""
+    import hairmsnn_amd as hm
+    cfg = base_cfg()
+    del cfg["hair"]["type"]
+    cfg["hair"]["beta_m"] = 0.7
+    info = hm.parse_scene(json.dumps(cfg))
+    assert info["alpha"] == pytest.approx(0.03)
+    assert info["beta_m"] == pytest.approx(0.3)
+
+
+def test_missing_alpha_defaults_to_zero():
+    import hairmsnn_amd as hm
+    cfg = base_cfg()
+    del cfg["hair"]["alpha"]
+    del cfg["hair"]["sigma_a"]
+    info = hm.parse_scene(json.dumps(cfg))
+    assert info["alpha"] == 0.0
+    np.testing.assert_allclose(info["sigma_a"], [0.06, 0.1, 0.2], rtol=1e-7)
+
+
+@pytest.mark.parametrize("section", ["camera", "integrator"])
+def test_required_sections(section):
+    import hairmsnn_amd as hm
+    cfg = base_cfg()
+    del cfg[section]
+    with pytest.raises(hm.HmsnnError, match="must be defined"):
+        hm.parse_scene(json.dumps(cfg))
+
+
+def test_geometry_required():
+    import hairmsnn_amd as hm
+    cfg = base_cfg()
+    del cfg["hair"]["geometry"]
+    del cfg["surface"]
+    with pytest.raises(hm.HmsnnError, match="hair or surface"):
+        hm.parse_scene(json.dumps(cfg))
+
+
+@pytest.mark.parametrize("w,h,ok", [(1024, 1024, True), (2048, 2048, True), (2048, 2049, False), (100, 100, False),
+                                    (128, 1, True)])
+def test_image_size_guard(w, h, ok):
+    """W*H <= 2048^2 and W*H % 128 == 0 (scene.cpp:300-306)."""
+    import hairmsnn_amd as hm
+    cfg = base_cfg()
+    cfg["integrator"]["width"], cfg["integrator"]["height"] = w, h
+    if ok:
+        assert hm.parse_scene(json.dumps(cfg))["width"] == w
+    else:
+        with pytest.raises(hm.HmsnnError, match="hard limit"):
+            hm.parse_scene(json.dumps(cfg))
+
+
+def test_env_scale_missing_leaves_zero_and_lights_optional():
+    import hairmsnn_amd as hm
+    cfg = base_cfg()
+    del cfg["lights"]["environment"]["scale"]
+    del cfg["lights"]["directional"]
+    info = hm.parse_scene(json.dumps(cfg))
+    assert info["has_env"] == 1 and info["env_scale"] == 0.0
+    assert info["num_dir_lights"] == 0
+
+
+def test_procedural_configs_parse():
+    import hairmsnn_amd as hm
+    for name, segs in [("c0_hairball.json", 2000 * 16), ("c1_curly.json", 50000 * 64)]:
+        info = hm.parse_scene((ROOT / "hairmsnn_amd" / "scenes" / name).read_text())
+        assert info["num_segments"] == segs

@@ -1,0 +1,127 @@
+"""Pins the oracle's hair BSDF (restating disney_hair.cuh) against closed
+forms: the scene constants, the Bessel series, lobe normalisation, the white
+furnace identity, and a chi-square test of its sampling routine."""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate, special
+
+from tests import oracle_lib as ol
+from tests.gen import ALPHA_2DEG, bsdf_queries, uniforms
+
+
+def test_shading_constants_closed_form(orc):
+    """Values at beta_m = beta_n = 0.3, alpha = 2 deg computed independently
+    from the formulas of disney_hair.cuh:28-40."""
+    out = np.empty(10)
+    orc.orc_hair_setup(ol.ptr(ol.hair_params(0.3, 0.3, ALPHA_2DEG)), ol.ptr(out))
+    v0 = (0.726 * 0.3 + 0.812 * 0.09 + 3.7 * 0.3 ** 20) ** 2
+    np.testing.assert_allclose(out[:3], [v0, v0 / 4, 4 * v0], rtol=1e-6)
+    np.testing.assert_allclose(out[:3], [0.084611185, 0.021152796, 0.33844474], rtol=1e-6)
+    assert out[3] == pytest.approx(0.11715981, rel=1e-6)
+    a = np.float64(np.float32(ALPHA_2DEG))
+    np.testing.assert_allclose(out[4:7], [math.sin(a), math.sin(2 * a), math.sin(4 * a)], rtol=1e-9)
+    np.testing.assert_allclose(out[7:10], [math.cos(a), math.cos(2 * a), math.cos(4 * a)], rtol=1e-9)
+
+
+def test_bessel_series_against_scipy(orc):
+    for x in np.linspace(0, 3, 31):
+        assert orc.orc_bessel_i0(float(x)) == pytest.approx(special.i0(x), rel=1e-10)
+    assert orc.orc_bessel_i0(4.0) == pytest.approx(special.i0(4.0), rel=1e-8)
+    # ten terms truncate the tail: at x = 12 the series is ~1-2 % below I0
+    r = orc.orc_bessel_i0(12.0) / special.i0(12.0)
+    assert 0.97 < r < 0.999
+
+
+@pytest.mark.parametrize("v", [0.3, 0.5, 1.0])
+def test_mp_normalised_direct_branch(orc, v):
+    """int M_p(theta_i) cos(theta_i) d theta_i = 1 (d'Eon et al.)."""
+    for so in (-0.7, 0.0, 0.4):
+        co = math.sqrt(1 - so * so)
+        val, _ = integrate.quad(lambda t: orc.orc_mp(math.cos(t), co, math.sin(t), so, v) * math.cos(t),
+                                -math.pi / 2, math.pi / 2, limit=200)
+        assert val == pytest.approx(1.0, rel=1e-6)
+
+
+def test_mp_log_branch_matches_exact_normalisation(orc):
+    """The v <= 0.1 branch (log domain, asymptotic log I0 above 12) integrates
+    to 1 up to the reference's truncated series and ln 2 ~ 0.6931."""
+    v = 0.1
+    for so in (-0.5, 0.0, 0.3):
+        co = math.sqrt(1 - so * so)
+        val, _ = integrate.quad(lambda t: orc.orc_mp(math.cos(t), co, math.sin(t), so, v) * math.cos(t),
+                                -math.pi / 2, math.pi / 2, limit=400, points=[-math.asin(so)])
+        assert val == pytest.approx(1.0, abs=0.01)
+
+
+def sphere_integral(fun, n_theta=384, n_phi=768):
+    """Midpoint rule over the sphere with x = sin(theta_i) (hair frame)."""
+    st = (np.arange(n_theta) + 0.5) / n_theta * np.pi - np.pi / 2  # theta in (-pi/2, pi/2)
+    ph = (np.arange(n_phi) + 0.5) / n_phi * 2 * np.pi - np.pi
+    T, P = np.meshgrid(st, ph, indexing="ij")
+    wi = np.stack([np.sin(T), np.cos(T) * np.cos(P), np.cos(T) * np.sin(P)], -1).reshape(-1, 3)
+    w = (np.cos(T) * (np.pi / n_theta) * (2 * np.pi / n_phi)).reshape(-1)
+    return fun(wi), w
+
+
+def eval_with(q0, wi, params):
+    q = np.repeat(q0[None, :], wi.shape[0], 0)
+    q[:, 3:6] = wi.astype(np.float32)
+    return ol.hair_eval(q, params, prec=0)
+
+
+@pytest.mark.parametrize("so,h", [(0.0, 0.0), (0.5, 0.3), (-0.8, -0.9)])
+def test_pdf_integrates_to_one_and_white_furnace(orc, so, h):
+    params = ol.hair_params(0.6, 0.5, ALPHA_2DEG)
+    q0 = np.array([so, 0, math.sqrt(1 - so * so), 0, 0, 1, h, 0, 0, 0], np.float32)
+    (f, pdf), w = sphere_integral(lambda wi: eval_with(q0, wi, params))
+    assert np.sum(pdf * w) == pytest.approx(1.0, abs=2e-3)
+    # sigma_a = 0: sum_p A_p = 1, so the directional albedo is 1
+    np.testing.assert_allclose(np.sum(f * w[:, None], 0), 1.0, atol=2e-3)
+
+
+def test_pdf_at_config_parameters_up_to_series_truncation(orc):
+    params = ol.hair_params(0.3, 0.3, ALPHA_2DEG)
+    q0 = np.array([0.2, 0, math.sqrt(1 - 0.04), 0, 0, 1, 0.1, 0.06, 0.1, 0.2], np.float32)
+    (f, pdf), w = sphere_integral(lambda wi: eval_with(q0, wi, params), 1024, 2048)
+    assert np.sum(pdf * w) == pytest.approx(1.0, abs=0.03)
+
+
+def test_fp32_twin_close_to_fp64(orc):
+    q = bsdf_queries(20000, 11)
+    params = ol.hair_params(0.3, 0.3, ALPHA_2DEG)
+    f64, p64 = ol.hair_eval(q, params, 0)
+    f32, p32 = ol.hair_eval(q, params, 1)
+    m = f64 > 1e-6
+    assert np.quantile(np.abs(f32[m] - f64[m]) / f64[m], 0.99) < 1e-4
+
+
+def test_sampling_chi_square(orc):
+    """Sampled directions follow the returned pdf (binned chi-square)."""
+    params = ol.hair_params(0.6, 0.5, ALPHA_2DEG)
+    so, h = 0.3, 0.4
+    q0 = np.array([so, 0, math.sqrt(1 - so * so), 0, 0, 1, h, 0.1, 0.2, 0.3], np.float32)
+    n = 400000
+    q = np.repeat(q0[None], n, 0)
+    wi, f, pdf, lobe = ol.hair_sample(q, uniforms(n, 4, 5), params)
+    # bins in (sin theta_i, phi_i)
+    nb_t, nb_p = 16, 32
+    ti = np.clip(((wi[:, 0] + 1) / 2 * nb_t).astype(int), 0, nb_t - 1)
+    pi_ = np.clip(((np.arctan2(wi[:, 2], wi[:, 1]) + np.pi) / (2 * np.pi) * nb_p).astype(int), 0, nb_p - 1)
+    obs = np.bincount(ti * nb_p + pi_, minlength=nb_t * nb_p).astype(np.float64)
+    # expected mass per bin from the pdf (uniform in sin theta and phi: dw = d(sin) d(phi))
+    m = 8
+    s = ((np.arange(nb_t * m) + 0.5) / (nb_t * m)) * 2 - 1
+    p = ((np.arange(nb_p * m) + 0.5) / (nb_p * m)) * 2 * np.pi - np.pi
+    S, P = np.meshgrid(s, p, indexing="ij")
+    c = np.sqrt(1 - S * S)
+    dirs = np.stack([S, c * np.cos(P), c * np.sin(P)], -1).reshape(-1, 3)
+    _, pe = eval_with(q0, dirs, params)
+    pe = pe.reshape(nb_t * m, nb_p * m) * (2 / (nb_t * m)) * (2 * np.pi / (nb_p * m))
+    exp = pe.reshape(nb_t, m, nb_p, m).sum((1, 3)).reshape(-1) * n
+    keep = exp > 20
+    chi2 = np.sum((obs[keep] - exp[keep]) ** 2 / exp[keep])
+    dof = keep.sum() - 1
+    assert chi2 < dof + 5 * math.sqrt(2 * dof), (chi2, dof)
+    assert obs[~keep].sum() < 0.002 * n
