@@ -1,0 +1,671 @@
+// Radiance-cache MLP on the MFMA cores: encoding, 5 x 64 ReLU hidden layers,
+// 3 exp outputs. Replaces the tiny-cuda-nn network the reference drives
+// through TINY_MLP (neural_network.cuh:33-50, render_hair_msnn.cu:667-697,
+// 740-752).
+//
+// Layout on the matrix cores (v_mfma_f32_32x32x16_{f16,bf16}): a wave holds 32
+// queries as the N dimension and computes Z^T = W H^T. Lane (g, n) (g = lane /
+// 32, n = lane % 32) receives output features i = 32 ob + 8 (r / 4) + 4 g + r % 4
+// of query n in accumulator register r of output block ob. The next layer
+// contracts over exactly those features in that register order (the
+// contraction order is free as long as the weight fragment uses the same
+// one), so activations never leave registers between layers. The weight
+// fragments are pre-permuted on the device after each update and staged in
+// LDS by every block.
+//
+// Training splits into: forward + backward per 32-query tile (activations
+// and pre-activation gradients written feature-major in FP16), a weight-
+// gradient GEMM over K = queries on the MFMA cores, and a reduce + Adam
+// kernel in FP32.
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/hmsnn.h"
+#include "hm_api_util.h"
+#include "hm_internal.h"
+#include "hm_mlp.h"
+
+namespace hm {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+namespace {
+
+// Input feature held by lane group g at element v of contraction step s.
+__host__ __device__ constexpr int feat_of(int s, int g, int v) {
+  return 32 * (s / 2) + 8 * (2 * (s % 2) + v / 4) + 4 * g + (v % 4);
+}
+
+// Fragment buffer (in elements of 8 values, i.e. one lane's operand):
+//   forward hidden  [l 0..4][ob 0..1][s 0..3][lane 64]
+//   forward output  [s 0..3][lane 64]                   (rows 0..2 valid)
+//   backward hidden [l 0..4][ib 0..1][s 0..3][lane 64]  (W^T)
+//   backward output [ib 0..1][lane 64]                  (W5^T, K = 3 of 16)
+constexpr int kFragFwd = 5 * 2 * 4 * 64;
+constexpr int kFragOut = 4 * 64;
+constexpr int kFragBwd = 5 * 2 * 4 * 64;
+constexpr int kFragBwdOut = 2 * 64;
+constexpr int kFragAll = kFragFwd + kFragOut + kFragBwd + kFragBwdOut;
+constexpr int kFragFwdTotal = kFragFwd + kFragOut;  // inference needs only these
+
+template <class T>
+struct MfmaTraits;
+template <>
+struct MfmaTraits<_Float16> {
+  typedef f16x8 vec;
+  static __device__ __forceinline__ f32x16 mfma(vec a, vec b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ _Float16 cvt(float x) { return (_Float16)x; }
+  static __device__ __forceinline__ float back(_Float16 x) { return (float)x; }
+};
+template <>
+struct MfmaTraits<__bf16> {
+  typedef bf16x8 vec;
+  static __device__ __forceinline__ f32x16 mfma(vec a, vec b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ __bf16 cvt(float x) { return (__bf16)x; }
+  static __device__ __forceinline__ float back(__bf16 x) { return (float)x; }
+};
+
+// Builds the permuted fragments from the FP32 master weights.
+template <class T>
+__global__ void k_mlp_permute(const float* w, T* frag) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;  // element index
+  if (e >= kFragAll * 8) return;
+  const int v = e & 7;
+  int f = e >> 3;  // fragment-lane index
+  float val = 0.f;
+  if (f < kFragFwd) {
+    const int lane = f & 63, s = (f >> 6) & 3, ob = (f >> 8) & 1, l = f >> 9;
+    const int g = lane >> 5, m = lane & 31;
+    val = w[l * 4096 + (32 * ob + m) * 64 + feat_of(s, g, v)];
+  } else if ((f -= kFragFwd) < kFragOut) {
+    const int lane = f & 63, s = f >> 6;
+    const int g = lane >> 5, m = lane & 31;
+    val = m < 3 ? w[5 * 4096 + m * 64 + feat_of(s, g, v)] : 0.f;
+  } else if ((f -= kFragOut) < kFragBwd) {
+    const int lane = f & 63, s = (f >> 6) & 3, ib = (f >> 8) & 1, l = f >> 9;
+    const int g = lane >> 5, m = lane & 31;
+    val = w[l * 4096 + feat_of(s, g, v) * 64 + (32 * ib + m)];
+  } else {
+    f -= kFragBwd;
+    const int lane = f & 63, ib = f >> 6;
+    const int g = lane >> 5, m = lane & 31;
+    const int o = 8 * g + v;
+    val = o < 3 ? w[5 * 4096 + o * 64 + 32 * ib + m] : 0.f;
+  }
+  frag[e] = (T)val;
+}
+
+// Encoded features of one 12-float record: frequency encoding of the
+// position, spherical angles of wo and tangent, sigma_a, a constant 1.
+struct Encoded {
+  float f[44];
+};
+
+__device__ __forceinline__ Encoded encode(const float* rec) {
+  Encoded e;
+  const float kPiF = 3.14159265358979f;
+  for (int c = 0; c < 3; ++c) {
+    float s, co;
+    sincospif(rec[c], &s, &co);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      e.f[12 * c + 2 * k] = s;
+      e.f[12 * c + 2 * k + 1] = co;
+      const float s2 = 2.f * s * co, c2 = co * co - s * s;  // double angle
+      s = s2;
+      co = c2;
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 2; ++v) {
+    const float x = rec[3 + 3 * v], y = rec[4 + 3 * v], z = rec[5 + 3 * v];
+    e.f[36 + 2 * v] = acosf(fminf(1.f, fmaxf(-1.f, z))) / kPiF;
+    e.f[37 + 2 * v] = atan2f(y, x) / (2.f * kPiF) + 0.5f;
+  }
+  e.f[40] = rec[9];
+  e.f[41] = rec[10];
+  e.f[42] = rec[11];
+  e.f[43] = 1.f;
+  return e;
+}
+
+template <class T>
+__device__ __forceinline__ typename MfmaTraits<T>::vec encoded_operand(const Encoded& e, int s, int g) {
+  typename MfmaTraits<T>::vec b;
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    const int f0 = feat_of(s, 0, v), f1 = feat_of(s, 1, v);
+    const float a0 = f0 < 44 ? e.f[f0] : 0.f;
+    const float a1 = f1 < 44 ? e.f[f1] : 0.f;
+    b[v] = MfmaTraits<T>::cvt(g ? a1 : a0);
+  }
+  return b;
+}
+
+template <class T>
+__device__ __forceinline__ void stage_fragments(const T* frag, typename MfmaTraits<T>::vec* lds, int nfrag) {
+  const typename MfmaTraits<T>::vec* src = reinterpret_cast<const typename MfmaTraits<T>::vec*>(frag);
+  for (int i = threadIdx.x; i < nfrag; i += blockDim.x) lds[i] = src[i];
+  __syncthreads();
+}
+
+// One hidden layer: acc[ob] = W_l (rows 32 ob..) x B, then ReLU into B.
+template <class T>
+__device__ __forceinline__ void hidden_layer(const typename MfmaTraits<T>::vec* lds, int l, int lane,
+                                             typename MfmaTraits<T>::vec B[4]) {
+  typedef MfmaTraits<T> M;
+  f32x16 acc[2];
+  acc[0] = f32x16{};
+  acc[1] = f32x16{};
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+#pragma unroll
+    for (int ob = 0; ob < 2; ++ob) acc[ob] = M::mfma(lds[((l * 2 + ob) * 4 + s) * 64 + lane], B[s], acc[ob]);
+  }
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int v = 0; v < 8; ++v) B[s][v] = M::cvt(fmaxf(acc[s >> 1][8 * (s & 1) + v], 0.f));
+}
+
+template <class T>
+__device__ __forceinline__ f32x16 output_layer(const typename MfmaTraits<T>::vec* lds, int lane,
+                                               const typename MfmaTraits<T>::vec B[4]) {
+  f32x16 acc = f32x16{};
+#pragma unroll
+  for (int s = 0; s < 4; ++s) acc = MfmaTraits<T>::mfma(lds[kFragFwd + s * 64 + lane], B[s], acc);
+  return acc;
+}
+
+constexpr int kMlpBlock = 256;
+
+template <class T>
+__global__ void __launch_bounds__(kMlpBlock) k_mlp_infer(const T* frag, const float* in, int n, float* out) {
+  typedef typename MfmaTraits<T>::vec V;
+  __shared__ V lds[kFragFwdTotal];
+  stage_fragments<T>(frag, lds, kFragFwdTotal);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 5, col = lane & 31;
+  const int waves = gridDim.x * (kMlpBlock / 64);
+  for (int tile = blockIdx.x * (kMlpBlock / 64) + wave; tile * 32 < n; tile += waves) {
+    const int q = tile * 32 + col;
+    float rec[12];
+    if (q < n) {
+#pragma unroll
+      for (int k = 0; k < 12; ++k) rec[k] = in[(size_t)q * 12 + k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 12; ++k) rec[k] = 0.f;
+      rec[5] = 1.f;
+      rec[8] = 1.f;
+    }
+    const Encoded e = encode(rec);
+    V B[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) B[s] = encoded_operand<T>(e, s, g);
+#pragma unroll 1
+    for (int l = 0; l < 5; ++l) hidden_layer<T>(lds, l, lane, B);
+    const f32x16 z = output_layer<T>(lds, lane, B);
+    if (g == 0 && q < n) {
+      out[(size_t)q * 3 + 0] = expf(z[0]);
+      out[(size_t)q * 3 + 1] = expf(z[1]);
+      out[(size_t)q * 3 + 2] = expf(z[2]);
+    }
+  }
+}
+
+// Feature-major activation store for layer l: buf[(l * 64 + f) * npad + q].
+// The lane-dependent part (4 g rows and the query) is folded into one
+// pointer; the per-feature offsets are wave-uniform.
+template <class T>
+__device__ __forceinline__ void store_fm(T* buf, int l, int npad, int q, int g, const typename MfmaTraits<T>::vec B[4]) {
+  T* p = buf + (size_t)(4 * g) * npad + q;
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int v = 0; v < 8; ++v) p[(size_t)(l * 64 + feat_of(s, 0, v)) * npad] = B[s][v];
+}
+
+// Forward + backward for 32-query tiles. Writes H_l (inputs of layer l,
+// l = 0..5) and dZ_l (gradients of layer l's pre-activations, scaled by
+// grad_scale) feature-major, and adds the batch loss.
+template <class T>
+__global__ void __launch_bounds__(kMlpBlock) k_mlp_fwd_bwd(const T* frag, const float* in, const float* target, int n,
+                                                           int npad, int loss, float grad_scale, T* H, T* dZ,
+                                                           float* loss_sum) {
+  typedef MfmaTraits<T> M;
+  typedef typename M::vec V;
+  extern __shared__ char smem[];
+  V* lds = reinterpret_cast<V*>(smem);
+  stage_fragments<T>(frag, lds, kFragAll);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 5, col = lane & 31;
+  float lsum = 0.f;
+  // one 32-query tile per wave (the grid covers the padded batch)
+  {
+    const int tile = blockIdx.x * (kMlpBlock / 64) + wave;
+    const int q = tile * 32 + col;
+    const bool valid = q < n;
+    float rec[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) rec[k] = valid ? in[(size_t)q * 12 + k] : 0.f;
+    if (!valid) {
+      rec[5] = 1.f;
+      rec[8] = 1.f;
+    }
+    const Encoded e = encode(rec);
+    V B[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) B[s] = encoded_operand<T>(e, s, g);
+    store_fm<T>(H, 0, npad, q, g, B);
+    // forward; keep one ReLU bit per held feature for the backward pass
+    uint32_t mask[5];
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+      hidden_layer<T>(lds, l, lane, B);
+      uint32_t m = 0;
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int v = 0; v < 8; ++v) m |= (M::back(B[s][v]) > 0.f ? 1u : 0u) << (8 * s + v);
+      mask[l] = m;
+      store_fm<T>(H, l + 1, npad, q, g, B);
+    }
+    const f32x16 z = output_layer<T>(lds, lane, B);
+    // ---- loss and output gradient (lanes of group 0 own rows 0..2) ----
+    float dz[3] = {0.f, 0.f, 0.f};
+    if (g == 0 && valid) {
+      float y[3], t[3];
+#pragma unroll
+      for (int o = 0; o < 3; ++o) {
+        y[o] = expf(z[o]);
+        t[o] = target[(size_t)q * 3 + o];
+      }
+      const float lum = 0.212671f * y[0] + 0.715160f * y[1] + 0.072169f * y[2];
+#pragma unroll
+      for (int o = 0; o < 3; ++o) {
+        const float d = y[o] - t[o];
+        float sc = 1.f;
+        if (loss == HMSNN_LOSS_RELATIVE_L2_LUMINANCE) sc = 1.f / (lum * lum + 0.01f);
+        else if (loss == HMSNN_LOSS_RELATIVE_L2) sc = 1.f / (y[o] * y[o] + 0.01f);
+        lsum += d * d * sc;
+        const float gval = 2.f * d * sc * y[o] * grad_scale;
+        dz[o] = fminf(fmaxf(gval, -60000.f), 60000.f);
+      }
+    }
+    // dZ_5: rows 0..2 from the lanes of group 0
+    if (g == 0) {
+#pragma unroll
+      for (int o = 0; o < 3; ++o) dZ[((size_t)5 * 64 + o) * npad + q] = M::cvt(dz[o]);
+    }
+    // ---- backward: dH_5 = W5^T dz ----
+    V Bdz;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) Bdz[v] = M::cvt(v < 3 ? dz[v] : 0.f);  // group 1 holds zeros
+    f32x16 acc[2];
+#pragma unroll
+    for (int ib = 0; ib < 2; ++ib)
+      acc[ib] = M::mfma(lds[kFragFwd + kFragOut + kFragBwd + ib * 64 + lane], Bdz, f32x16{});
+#pragma unroll
+    for (int l = 4; l >= 0; --l) {
+      // dZ_l = dH_{l+1} * [H_{l+1} > 0]
+      V D[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int v = 0; v < 8; ++v)
+          D[s][v] = M::cvt(((mask[l] >> (8 * s + v)) & 1u) ? acc[s >> 1][8 * (s & 1) + v] : 0.f);
+      store_fm<T>(dZ, l, npad, q, g, D);
+      if (l == 0) break;
+      // dH_l = W_l^T dZ_l
+      acc[0] = f32x16{};
+      acc[1] = f32x16{};
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int ib = 0; ib < 2; ++ib)
+          acc[ib] = M::mfma(lds[kFragFwd + kFragOut + ((l * 2 + ib) * 4 + s) * 64 + lane], D[s], acc[ib]);
+    }
+  }
+  // wave-reduce the loss, one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
+  if (lane == 0) atomicAdd(loss_sum, lsum);
+}
+
+// dW_l partial over a chunk of queries: 64 x 64 on four waves (2 x 2 blocks of
+// 32 x 32), K = queries in steps of 16. partial[(chunk * 6 + l) * 4096 + o * 64 + i].
+template <class T>
+__global__ void __launch_bounds__(kMlpBlock) k_mlp_dw(const T* H, const T* dZ, int npad, int chunk, float* partial) {
+  typedef MfmaTraits<T> M;
+  typedef typename M::vec V;
+  const int l = blockIdx.y, c = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int mb = wave >> 1, nb = wave & 1;
+  const int g = lane >> 5, r = lane & 31;
+  if (l == 5 && mb == 1) return;  // output layer: 3 rows
+  const int q0 = c * chunk, q1 = min(q0 + chunk, npad);
+  const T* A = dZ + ((size_t)l * 64 + 32 * mb + r) * npad;
+  const T* Bp = H + ((size_t)l * 64 + 32 * nb + r) * npad;
+  f32x16 acc = f32x16{};
+  for (int q = q0; q < q1; q += 16) {
+    const V a = *reinterpret_cast<const V*>(A + q + 8 * g);
+    const V b = *reinterpret_cast<const V*>(Bp + q + 8 * g);
+    acc = M::mfma(a, b, acc);
+  }
+  float* out = partial + ((size_t)c * 6 + l) * 4096;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int i = 32 * mb + 8 * (k / 4) + 4 * g + (k % 4);  // dW row (output feature)
+    const int j = 32 * nb + r;                              // dW column (input feature)
+    out[i * 64 + j] = acc[k];
+  }
+}
+
+// Sums the chunk partials into the gradient of the flat weight vector, then
+// optionally applies one Adam step to the FP32 master weights.
+__global__ void k_mlp_reduce_adam(const float* partial, int nchunks, float inv_scale, float* grad, float* w,
+                                  float* m, float* v, int adam, float lr, float b1, float b2, float eps, float l2,
+                                  float bc1, float bc2) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= kMlpWeightCount) return;
+  const int l = k / 4096, rem = k % 4096;
+  float s = 0.f;
+  for (int c = 0; c < nchunks; ++c) s += partial[((size_t)c * 6 + l) * 4096 + rem];
+  float gk = s * inv_scale;
+  if (grad) grad[k] = gk;
+  if (!adam) return;
+  gk += l2 * w[k];
+  const float mk = b1 * m[k] + (1.f - b1) * gk;
+  const float vk = b2 * v[k] + (1.f - b2) * gk * gk;
+  m[k] = mk;
+  v[k] = vk;
+  w[k] -= lr * (mk / bc1) / (sqrtf(vk / bc2) + eps);
+}
+
+}  // namespace
+
+struct MlpState {
+  int precision = HMSNN_FP16;
+  float* w = nullptr;
+  float* m = nullptr;
+  float* v = nullptr;
+  void* frag = nullptr;
+  float* loss_sum = nullptr;
+  void* H = nullptr;
+  void* dZ = nullptr;
+  float* partial = nullptr;
+  int cap = 0;  // padded batch capacity of H / dZ
+  int step = 0;
+  hipStream_t stream = nullptr;
+  ~MlpState() {
+    dev_free(w);
+    dev_free(m);
+    dev_free(v);
+    dev_free(frag);
+    dev_free(loss_sum);
+    dev_free(H);
+    dev_free(dZ);
+    dev_free(partial);
+  }
+  size_t esize() const { return 2; }
+};
+
+static int mlp_refresh(MlpState& s, hipStream_t st, std::string& err) {
+  const int total = kFragAll * 8;
+  if (s.precision == HMSNN_BF16)
+    hipLaunchKernelGGL(k_mlp_permute<__bf16>, dim3((total + 255) / 256), dim3(256), 0, st, s.w, (__bf16*)s.frag);
+  else
+    hipLaunchKernelGGL(k_mlp_permute<_Float16>, dim3((total + 255) / 256), dim3(256), 0, st, s.w,
+                       (_Float16*)s.frag);
+  return check_launch("mlp permute", err);
+}
+
+int mlp_create(int precision, uint64_t seed, MlpState** out, std::string& err) {
+  if (precision != HMSNN_FP16 && precision != HMSNN_BF16) {
+    err = "MLP precision must be FP16 or BF16";
+    return kEUnsupported;
+  }
+  MlpState* s = new MlpState();
+  s->precision = precision;
+  int rc = dev_alloc((void**)&s->w, sizeof(float) * kMlpWeightCount, err);
+  if (!rc) rc = dev_alloc((void**)&s->m, sizeof(float) * kMlpWeightCount, err);
+  if (!rc) rc = dev_alloc((void**)&s->v, sizeof(float) * kMlpWeightCount, err);
+  if (!rc) rc = dev_alloc(&s->frag, s->esize() * kFragAll * 8, err);
+  if (!rc) rc = dev_alloc((void**)&s->loss_sum, sizeof(float) * 4, err);
+  if (rc) {
+    delete s;
+    return rc;
+  }
+  // Uniform init with the fan-in/fan-out bound, from a seeded splitmix stream.
+  std::vector<float> w(kMlpWeightCount);
+  uint64_t st = seed * 0x9E3779B97F4A7C15ull + 0x51ull;
+  auto next = [&]() {
+    uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return (double)((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+  };
+  for (int k = 0; k < kMlpWeightCount; ++k) {
+    const bool out_layer = k >= 5 * 4096;
+    const double lim = std::sqrt(6.0 / (64.0 + (out_layer ? 3.0 : 64.0)));
+    w[k] = (float)((2.0 * next() - 1.0) * lim);
+  }
+  HM_HIP(hipMemcpy(s->w, w.data(), sizeof(float) * kMlpWeightCount, hipMemcpyHostToDevice), err);
+  HM_HIP(hipMemset(s->m, 0, sizeof(float) * kMlpWeightCount), err);
+  HM_HIP(hipMemset(s->v, 0, sizeof(float) * kMlpWeightCount), err);
+  HM_TRY(mlp_refresh(*s, nullptr, err));
+  HM_HIP(hipDeviceSynchronize(), err);
+  *out = s;
+  return kOk;
+}
+
+void mlp_destroy(MlpState* s) { delete s; }
+
+int mlp_set_weights(MlpState& s, const float* w_host, std::string& err) {
+  HM_HIP(hipMemcpy(s.w, w_host, sizeof(float) * kMlpWeightCount, hipMemcpyHostToDevice), err);
+  HM_HIP(hipMemset(s.m, 0, sizeof(float) * kMlpWeightCount), err);
+  HM_HIP(hipMemset(s.v, 0, sizeof(float) * kMlpWeightCount), err);
+  s.step = 0;
+  HM_TRY(mlp_refresh(s, nullptr, err));
+  HM_HIP(hipDeviceSynchronize(), err);
+  return kOk;
+}
+
+int mlp_get_weights(const MlpState& s, float* w_host, std::string& err) {
+  HM_HIP(hipMemcpy(w_host, s.w, sizeof(float) * kMlpWeightCount, hipMemcpyDeviceToHost), err);
+  return kOk;
+}
+
+static int mlp_blocks(int n) {
+  const int tiles = (n + 31) / 32;
+  return std::max(1, std::min(1024, (tiles + 3) / 4));
+}
+
+int mlp_forward(MlpState& s, const float* in, int n, float* out, hipStream_t st, std::string& err) {
+  if (n <= 0) return kOk;
+  if (s.precision == HMSNN_BF16)
+    hipLaunchKernelGGL(k_mlp_infer<__bf16>, dim3(mlp_blocks(n)), dim3(kMlpBlock), 0, st, (const __bf16*)s.frag, in,
+                       n, out);
+  else
+    hipLaunchKernelGGL(k_mlp_infer<_Float16>, dim3(mlp_blocks(n)), dim3(kMlpBlock), 0, st,
+                       (const _Float16*)s.frag, in, n, out);
+  return check_launch("mlp infer", err);
+}
+
+// Forward + backward + weight-gradient GEMM + reduce (+ Adam when lr > 0).
+int mlp_loss_grad_step(MlpState& s, const float* in, const float* target, int n, int loss, float* grad_out,
+                       float* loss_out, bool adam, float lr, hipStream_t st, std::string& err) {
+  if (n <= 0) {
+    err = "empty training batch";
+    return kEInval;
+  }
+  const int npad = (n + 31) / 32 * 32;
+  const int chunk = 2048;
+  const int nchunks = (npad + chunk - 1) / chunk;
+  if (npad > s.cap) {
+    dev_free(s.H);
+    dev_free(s.dZ);
+    dev_free(s.partial);
+    s.H = s.dZ = nullptr;
+    s.partial = nullptr;
+    s.cap = 0;
+    HM_TRY(dev_alloc(&s.H, s.esize() * 6 * 64 * (size_t)npad, err));
+    HM_TRY(dev_alloc(&s.dZ, s.esize() * 6 * 64 * (size_t)npad, err));
+    HM_TRY(dev_alloc((void**)&s.partial, sizeof(float) * 6 * 4096 * (size_t)nchunks, err));
+    // rows 3..63 of dZ_5 are never written: clear once
+    HM_HIP(hipMemsetAsync(s.dZ, 0, s.esize() * 6 * 64 * (size_t)npad, st), err);
+    s.cap = npad;
+  }
+  // the layouts below index with the current npad: clear dZ_5 rows 3..63
+  HM_HIP(hipMemsetAsync((char*)s.dZ + s.esize() * ((size_t)5 * 64 + 3) * npad, 0, s.esize() * 61 * (size_t)npad, st),
+         err);
+  HM_HIP(hipMemsetAsync(s.loss_sum, 0, sizeof(float), st), err);
+  const float grad_scale = 1.f / 64.f;
+  const size_t smem = sizeof(f16x8) * kFragAll;  // 88 KB of weight fragments
+  static bool attr_set = false;
+  if (!attr_set) {
+    HM_HIP(hipFuncSetAttribute((const void*)k_mlp_fwd_bwd<_Float16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)smem),
+           err);
+    HM_HIP(hipFuncSetAttribute((const void*)k_mlp_fwd_bwd<__bf16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)smem),
+           err);
+    attr_set = true;
+  }
+  const int blocks = npad / 128 + (npad % 128 ? 1 : 0);
+  if (s.precision == HMSNN_BF16) {
+    hipLaunchKernelGGL(k_mlp_fwd_bwd<__bf16>, dim3(blocks), dim3(kMlpBlock), smem, st, (const __bf16*)s.frag, in,
+                       target, n, npad, loss, grad_scale, (__bf16*)s.H, (__bf16*)s.dZ, s.loss_sum);
+    HM_TRY(check_launch("mlp fwd/bwd", err));
+    hipLaunchKernelGGL(k_mlp_dw<__bf16>, dim3(nchunks, 6), dim3(kMlpBlock), 0, st, (const __bf16*)s.H,
+                       (const __bf16*)s.dZ, npad, chunk, s.partial);
+  } else {
+    hipLaunchKernelGGL(k_mlp_fwd_bwd<_Float16>, dim3(blocks), dim3(kMlpBlock), smem, st, (const _Float16*)s.frag,
+                       in, target, n, npad, loss, grad_scale, (_Float16*)s.H, (_Float16*)s.dZ, s.loss_sum);
+    HM_TRY(check_launch("mlp fwd/bwd", err));
+    hipLaunchKernelGGL(k_mlp_dw<_Float16>, dim3(nchunks, 6), dim3(kMlpBlock), 0, st, (const _Float16*)s.H,
+                       (const _Float16*)s.dZ, npad, chunk, s.partial);
+  }
+  HM_TRY(check_launch("mlp dW", err));
+  const float inv_scale = 1.f / (grad_scale * 3.f * (float)n);
+  float bc1 = 1.f, bc2 = 1.f;
+  if (adam) {
+    s.step++;
+    bc1 = 1.f - std::pow(0.9f, (float)s.step);
+    bc2 = 1.f - std::pow(0.99f, (float)s.step);
+  }
+  hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((kMlpWeightCount + 255) / 256), dim3(256), 0, st, s.partial, nchunks,
+                     inv_scale, grad_out, s.w, s.m, s.v, adam ? 1 : 0, lr, 0.9f, 0.99f, 1e-15f, 1e-6f, bc1, bc2);
+  HM_TRY(check_launch("mlp reduce/adam", err));
+  if (adam) HM_TRY(mlp_refresh(s, st, err));
+  if (loss_out) {
+    float h = 0.f;
+    HM_HIP(hipMemcpyAsync(&h, s.loss_sum, sizeof(float), hipMemcpyDeviceToHost, st), err);
+    HM_HIP(hipStreamSynchronize(st), err);
+    *loss_out = h / (3.f * (float)n);
+  }
+  return kOk;
+}
+
+// Encoded features (FP32) for tests.
+__global__ void k_mlp_encode(const float* in, int n, float* feat) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  float rec[12];
+  for (int k = 0; k < 12; ++k) rec[k] = in[(size_t)q * 12 + k];
+  const Encoded e = encode(rec);
+  for (int k = 0; k < 64; ++k) feat[(size_t)q * 64 + k] = k < 44 ? e.f[k] : 0.f;
+}
+
+int mlp_encode(const float* in, int n, float* feat, hipStream_t st, std::string& err) {
+  if (n <= 0) return kOk;
+  hipLaunchKernelGGL(k_mlp_encode, dim3((n + 255) / 256), dim3(256), 0, st, in, n, feat);
+  return check_launch("mlp encode", err);
+}
+
+}  // namespace hm
+
+using namespace hm;
+
+struct hmsnn_mlp {
+  MlpState* s;
+};
+
+extern "C" {
+
+int hmsnn_mlp_create(int precision, uint64_t seed, hmsnn_mlp** out) {
+  if (!out) return fail(HMSNN_EINVAL, "hmsnn_mlp_create: NULL out");
+  std::string err;
+  MlpState* s = nullptr;
+  int rc = mlp_create(precision, seed, &s, err);
+  if (rc) return fail(rc, err);
+  *out = new hmsnn_mlp{s};
+  return HMSNN_OK;
+}
+
+int hmsnn_mlp_destroy(hmsnn_mlp* m) {
+  if (m) {
+    (void)hipDeviceSynchronize();
+    mlp_destroy(m->s);
+    delete m;
+  }
+  return HMSNN_OK;
+}
+
+int hmsnn_mlp_get_weights(const hmsnn_mlp* m, float* w, int n) {
+  if (!m || !w || n != kMlpWeightCount) return fail(HMSNN_EINVAL, "hmsnn_mlp_get_weights: expects 20672 floats");
+  std::string err;
+  int rc = mlp_get_weights(*m->s, w, err);
+  return rc ? fail(rc, err) : HMSNN_OK;
+}
+
+int hmsnn_mlp_set_weights(hmsnn_mlp* m, const float* w, int n) {
+  if (!m || !w || n != kMlpWeightCount) return fail(HMSNN_EINVAL, "hmsnn_mlp_set_weights: expects 20672 floats");
+  std::string err;
+  int rc = mlp_set_weights(*m->s, w, err);
+  return rc ? fail(rc, err) : HMSNN_OK;
+}
+
+int hmsnn_mlp_encode_dev(const float* in, int n, float* feat, void* stream) {
+  if (n < 0 || (n > 0 && (!in || !feat))) return fail(HMSNN_EINVAL, "hmsnn_mlp_encode_dev: bad arguments");
+  std::string err;
+  int rc = mlp_encode(in, n, feat, (hipStream_t)stream, err);
+  return rc ? fail(rc, err) : HMSNN_OK;
+}
+
+int hmsnn_mlp_forward_dev(hmsnn_mlp* m, const float* in, int n, float* out, void* stream) {
+  if (!m || n < 0 || (n > 0 && (!in || !out))) return fail(HMSNN_EINVAL, "hmsnn_mlp_forward_dev: bad arguments");
+  std::string err;
+  int rc = mlp_forward(*m->s, in, n, out, (hipStream_t)stream, err);
+  return rc ? fail(rc, err) : HMSNN_OK;
+}
+
+int hmsnn_mlp_backward_dev(hmsnn_mlp* m, const float* in, const float* target, int n, int loss, float* grad,
+                           float* loss_out, void* stream) {
+  if (!m || n <= 0 || !in || !target || !grad || loss < 0 || loss > 2)
+    return fail(HMSNN_EINVAL, "hmsnn_mlp_backward_dev: bad arguments");
+  std::string err;
+  int rc = mlp_loss_grad_step(*m->s, in, target, n, loss, grad, loss_out, false, 0.f, (hipStream_t)stream, err);
+  return rc ? fail(rc, err) : HMSNN_OK;
+}
+
+int hmsnn_mlp_train_step_dev(hmsnn_mlp* m, const float* in, const float* target, int n, int loss, float lr,
+                             float* loss_out, void* stream) {
+  if (!m || n <= 0 || !in || !target || loss < 0 || loss > 2 || !(lr > 0.f))
+    return fail(HMSNN_EINVAL, "hmsnn_mlp_train_step_dev: bad arguments");
+  std::string err;
+  int rc = mlp_loss_grad_step(*m->s, in, target, n, loss, nullptr, loss_out, true, lr, (hipStream_t)stream, err);
+  return rc ? fail(rc, err) : HMSNN_OK;
+}
+
+}  // extern "C"

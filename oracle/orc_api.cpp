@@ -251,6 +251,30 @@ int orc_render(const orc_scene* sc, void* bvh_handle, int mode, int kbeta, int a
   return 0;
 }
 
+// Radiance-cache network (orc_mlp.h). prec 0 = FP64, 1 = FP32.
+void orc_mlp_encode(const float* in, int n, double* feat) {
+  for (int s = 0; s < n; ++s) mlp_encode<double>(in + (size_t)s * kMlpIn, feat + (size_t)s * kMlpFeat);
+}
+
+void orc_mlp_forward(const float* w, const float* in, int n, int prec, double* out) {
+  if (prec == 0) {
+    mlp_forward<double>(w, in, n, out);
+  } else {
+    std::vector<float> o((size_t)n * kMlpOut);
+    mlp_forward<float>(w, in, n, o.data());
+    for (size_t i = 0; i < o.size(); ++i) out[i] = o[i];
+  }
+}
+
+double orc_mlp_loss_grad(const float* w, const float* in, const float* target, int n, int loss, int prec,
+                         double* grad) {
+  if (prec == 0) return mlp_loss_grad<double>(w, in, target, n, loss, grad);
+  std::vector<float> g(kMlpWeights);
+  const double l = mlp_loss_grad<float>(w, in, target, n, loss, g.data());
+  for (int i = 0; i < kMlpWeights; ++i) grad[i] = g[i];
+  return l;
+}
+
 int orc_max_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

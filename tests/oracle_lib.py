@@ -173,3 +173,46 @@ class OracleScene:
                                  ptr(full), ptr(short), ptr(hit))
         assert rc == 0
         return full, short, hit
+
+
+_SIGS_MLP = {
+    "orc_mlp_encode": (None, [_P, _I, _P]),
+    "orc_mlp_forward": (None, [_P, _P, _I, _I, _P]),
+    "orc_mlp_loss_grad": (_D, [_P, _P, _P, _I, _I, _I, _P]),
+}
+
+
+def _mlp_lib():
+    lib = load_oracle()
+    for n, (r, a) in _SIGS_MLP.items():
+        f = getattr(lib, n)
+        f.restype = r
+        f.argtypes = a
+    return lib
+
+
+MLP_WEIGHTS = 5 * 64 * 64 + 3 * 64
+
+
+def mlp_encode(inp: np.ndarray) -> np.ndarray:
+    inp = np.ascontiguousarray(inp, np.float32)
+    out = np.empty((inp.shape[0], 64))
+    _mlp_lib().orc_mlp_encode(ptr(inp), inp.shape[0], ptr(out))
+    return out
+
+
+def mlp_forward(w: np.ndarray, inp: np.ndarray, prec: int = 0) -> np.ndarray:
+    w = np.ascontiguousarray(w, np.float32)
+    inp = np.ascontiguousarray(inp, np.float32)
+    out = np.empty((inp.shape[0], 3))
+    _mlp_lib().orc_mlp_forward(ptr(w), ptr(inp), inp.shape[0], prec, ptr(out))
+    return out
+
+
+def mlp_loss_grad(w: np.ndarray, inp: np.ndarray, target: np.ndarray, loss: int = 0, prec: int = 0):
+    w = np.ascontiguousarray(w, np.float32)
+    inp = np.ascontiguousarray(inp, np.float32)
+    target = np.ascontiguousarray(target, np.float32)
+    g = np.empty(MLP_WEIGHTS)
+    l = _mlp_lib().orc_mlp_loss_grad(ptr(w), ptr(inp), ptr(target), inp.shape[0], loss, prec, ptr(g))
+    return l, g
