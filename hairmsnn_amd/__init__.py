@@ -132,6 +132,22 @@ def parse_scene(json_text: str) -> dict:
     return _info_dict(info)
 
 
+def generate_host(json_text: str | dict) -> tuple[np.ndarray, np.ndarray | None]:
+    """Procedural segments (n x 8) and environment map (H x W x 4) on the host."""
+    if isinstance(json_text, dict):
+        json_text = json.dumps(json_text)
+    lib = load()
+    ns, w, h = C.c_int(), C.c_int(), C.c_int()
+    t = json_text.encode()
+    check(lib.hmsnn_generate_host(t, None, 0, C.byref(ns), None, 0, C.byref(w), C.byref(h)))
+    seg = np.empty((ns.value, 8), np.float32)
+    env = np.empty((h.value, w.value, 4), np.float32) if w.value else None
+    check(lib.hmsnn_generate_host(t, _ptr(seg) if seg.size else None, seg.size, C.byref(ns),
+                                  _ptr(env) if env is not None else None, 0 if env is None else env.size,
+                                  C.byref(w), C.byref(h)))
+    return seg, env
+
+
 class Scene:
     """A scene context: geometry, BVH, environment tables and render state on the GPU."""
 

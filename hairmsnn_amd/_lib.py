@@ -69,6 +69,8 @@ _SIGS = {
     "hmsnn_hair_bsdf_eval_dev": (_I, [C.POINTER(HairParams), _P, _I, _P, _P, _P]),
     "hmsnn_hair_bsdf_sample_dev": (_I, [C.POINTER(HairParams), _P, _P, _I, _P, _P, _P, _P, _P]),
     "hmsnn_parse_scene": (_I, [C.c_char_p, C.POINTER(Info)]),
+    "hmsnn_generate_host": (_I, [C.c_char_p, _P, C.c_size_t, C.POINTER(_I), _P, C.c_size_t, C.POINTER(_I),
+                                 C.POINTER(_I)]),
     "hmsnn_create": (_I, [C.c_char_p, C.POINTER(_P)]),
     "hmsnn_create_from_json": (_I, [C.c_char_p, C.c_char_p, C.POINTER(_P)]),
     "hmsnn_destroy": (_I, [_P]),

@@ -118,10 +118,12 @@ void Context::free_path_buffers() {
   accum = average = nullptr;
 }
 
-int Context::ensure_path_buffers(bool msnn_mode, std::string& err) {
-  if (pb.ray_o && (!msnn_mode || pb_msnn)) return kOk;
+int Context::ensure_path_buffers(bool msnn_mode, std::string& err, int min_slots) {
+  const int want = std::max(scene.width * scene.height, min_slots);
+  if (pb.ray_o && (!msnn_mode || pb_msnn) && pb_slots >= want) return kOk;
   free_path_buffers();
-  const size_t n = (size_t)scene.width * scene.height;
+  const size_t n = (size_t)want;
+  pb_slots = want;
   auto A = [&](void** p, size_t bytes) -> int {
     HM_TRY(dev_alloc(p, bytes, err));
     pb_allocs.push_back(*p);
@@ -441,6 +443,31 @@ int hmsnn_parse_scene(const char* text, hmsnn_info* info) {
   if (!parse_scene_text(text, ".", s, err)) return fail(HMSNN_EINVAL, err);
   std::memset(info, 0, sizeof(*info));
   fill_scene_info(s, info);
+  return HMSNN_OK;
+}
+
+int hmsnn_generate_host(const char* text, float* seg_host, size_t seg_cap, int* num_segments, float* env_rgba,
+                        size_t env_cap, int* env_w, int* env_h) {
+  if (!text) return fail(HMSNN_EINVAL, "hmsnn_generate_host: NULL json");
+  Scene s;
+  std::string err;
+  if (!parse_scene_text(text, ".", s, err)) return fail(HMSNN_EINVAL, err);
+  HairGeometry hg;
+  if (s.has_hair && s.hair_procedural) generate_hair(s.hair_gen, hg);
+  const size_t nseg = hg.seg.size() / 8;
+  if (num_segments) *num_segments = (int)nseg;
+  if (seg_host) {
+    if (seg_cap < hg.seg.size()) return fail(HMSNN_EINVAL, "segment buffer too small");
+    std::memcpy(seg_host, hg.seg.data(), sizeof(float) * hg.seg.size());
+  }
+  EnvImage env;
+  if (s.has_env && (s.env_constant || s.env_procedural)) generate_env(s, env);
+  if (env_w) *env_w = env.width;
+  if (env_h) *env_h = env.height;
+  if (env_rgba) {
+    if (env_cap < env.rgba.size()) return fail(HMSNN_EINVAL, "environment buffer too small");
+    std::memcpy(env_rgba, env.rgba.data(), sizeof(float) * env.rgba.size());
+  }
   return HMSNN_OK;
 }
 

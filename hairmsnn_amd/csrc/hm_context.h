@@ -54,7 +54,10 @@ struct Context {
 
   ~Context();
   int init(std::string& err);
-  int ensure_path_buffers(bool msnn_mode, std::string& err);
+  // Path state for max(width * height, min_slots) slots; msnn_mode adds the
+  // training-path arrays.
+  int ensure_path_buffers(bool msnn_mode, std::string& err, int min_slots = 0);
+  int pb_slots = 0;
   void free_path_buffers();
   PassConfig pt_pass_config() const;
 };

@@ -36,6 +36,12 @@ constexpr float kHalfLog2Pi = 0.9189385091729444f;  // 0.5*log(2*Pi), Pi = 3.141
 struct HairConsts {
   float v[4];          // variance per lobe: R, TT, TRT, TRRT (TRRT uses TRT's)
   float inv_v[4];
+  // Variance used when *sampling* lobe p. For p = 3 the reference reads
+  // si.hair.v[3] past the end of float v[3] (disney_hair.cuh:329), which is the
+  // next member of HairInteraction, the logistic scale s (common.cuh:38-39);
+  // the pdf keeps v[2] for that lobe (disney_hair.cuh:252).
+  float vs[4];
+  float inv_vs[4];
   float mp_log_c[4];   // ln2_ref + log(1/(2v)), the v <= 0.1 branch constant
   float mp_sinh_c[4];  // 1 / (sinh(1/v) * 2v), the v > 0.1 branch constant
   float s, inv_s;      // logistic scale and its inverse
@@ -66,6 +72,11 @@ inline HairConsts make_hair_consts(float beta_m, float beta_n, float alpha, cons
   const double s = 0.626657069 * (0.265 * bn + 1.194 * bn * bn + 5.372 * std::pow(bn, 22.0));
   c.s = (float)s;
   c.inv_s = (float)(1.0 / s);
+  for (int p = 0; p < 4; ++p) {
+    const double vs = p < 3 ? vv[p] : s;
+    c.vs[p] = (float)vs;
+    c.inv_vs[p] = (float)(1.0 / vs);
+  }
   const double pi = (double)kPi;
   const double cdf_hi = 1.0 / (1.0 + std::exp(-pi / s));
   const double cdf_lo = 1.0 / (1.0 + std::exp(pi / s));
@@ -163,7 +174,7 @@ struct HairWo {
 HM_HD HairWo hair_prepare(const HairConsts& hc, V3 wo, float h, V3 sigma_a) {
   HairWo w;
   w.so = wo.x;
-  w.co = safe_sqrt(1.f - w.so * w.so);
+  w.co = safe_sqrt((1.f - w.so) * (1.f + w.so));
   w.phi_o = atan2f(wo.z, wo.y);
   // Scale tilt: lobe 0 rotates by -2 alpha, lobe 1 by +alpha, lobe 2 by +4 alpha
   // (disney_hair.cuh:228-246).
@@ -175,12 +186,14 @@ HM_HD HairWo hair_prepare(const HairConsts& hc, V3 wo, float h, V3 sigma_a) {
   w.co_t[2] = w.co * hc.cos2k[2] - w.so * hc.sin2k[2];
   w.h = h;
   w.gamma_o = asinf(h);
-  const float cos_gamma_o = safe_sqrt(1.f - h * h);  // = cos(asin(h))
+  // 1 - x^2 is formed as (1 - x)(1 + x), which keeps its relative accuracy
+  // as |x| -> 1 (grazing directions, silhouette hits)
+  const float cos_gamma_o = safe_sqrt((1.f - h) * (1.f + h));  // = cos(asin(h))
   const float sin_theta_t = w.so / hc.eta;
-  const float cos_theta_t = safe_sqrt(1.f - sin_theta_t * sin_theta_t);
-  const float etap = sqrtf(hc.eta * hc.eta - w.so * w.so) / w.co;
+  const float cos_theta_t = safe_sqrt((1.f - sin_theta_t) * (1.f + sin_theta_t));
+  const float etap = sqrtf((hc.eta - w.so) * (hc.eta + w.so)) / w.co;
   const float sin_gamma_t = h / etap;
-  const float cos_gamma_t = safe_sqrt(1.f - sin_gamma_t * sin_gamma_t);
+  const float cos_gamma_t = safe_sqrt((1.f - sin_gamma_t) * (1.f + sin_gamma_t));
   w.gamma_t = asinf(clampf(sin_gamma_t, -1.f, 1.f));
   const float fac = 2.f * cos_gamma_t / cos_theta_t;
   const V3 T = v3(expf(-sigma_a.x * fac), expf(-sigma_a.y * fac), expf(-sigma_a.z * fac));
@@ -208,7 +221,7 @@ HM_HD HairWo hair_prepare(const HairConsts& hc, V3 wo, float h, V3 sigma_a) {
 // in the hair frame (disney_hair.cuh:185-266).
 HM_HD V3 hair_eval(const HairConsts& hc, const HairWo& w, V3 wi, float* pdf) {
   const float si = wi.x;
-  const float ci = safe_sqrt(1.f - si * si);
+  const float ci = safe_sqrt((1.f - si) * (1.f + si));
   const float phi = atan2f(wi.z, wi.y) - w.phi_o;
   V3 f = v3(0.f);
   float pd = 0.f;
@@ -246,13 +259,13 @@ HM_HD V3 hair_sample_dir(const HairConsts& hc, const HairWo& w, float u0, float 
     so_t = w.so;
     co_t = w.co;
   }
-  const float v = hc.v[p];
+  const float v = hc.vs[p];
   const float e2 = fmaxf(u1, 1e-5f);
-  const float cos_t = 1.f + v * logf(e2 + (1.f - e2) * expf(-2.f * hc.inv_v[p]));
-  const float sin_t = safe_sqrt(1.f - cos_t * cos_t);
+  const float cos_t = 1.f + v * logf(e2 + (1.f - e2) * expf(-2.f * hc.inv_vs[p]));
+  const float sin_t = safe_sqrt((1.f - cos_t) * (1.f + cos_t));
   const float cos_phi = cosf(2.f * kPi * u2);
   const float si = -cos_t * so_t + sin_t * cos_phi * co_t;
-  const float ci = safe_sqrt(1.f - si * si);
+  const float ci = safe_sqrt((1.f - si) * (1.f + si));
   float dphi;
   if (p < 3) {
     // Trimmed-logistic inversion (disney_hair.cuh:268-274).

@@ -70,7 +70,7 @@ __global__ void k_reset_counts(int* counts, int a, int b) {
 // Primary rays for every pixel of pass `accum_id` (path_tracing.cu:29-48).
 // HairMSNN training pixels (hair_msnn.cu:206-209) are flagged here.
 __global__ void k_camera(SceneView sc, PathBuffers pb, int accum_id, int msnn, const int* train_idx,
-                         int every_nth) {
+                         int every_nth, int n_train) {
   const int n = sc.width * sc.height;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) pb.counts[0] = n;
@@ -90,12 +90,33 @@ __global__ void k_camera(SceneView sc, PathBuffers pb, int accum_id, int msnn, c
   uint32_t flags = 0;
   if (msnn) {
     const int tr = i / every_nth;
-    const int tidx = train_idx[tr] % every_nth;
-    if (i % every_nth == tidx) flags |= 1u;
+    if (tr < n_train && i % every_nth == train_idx[tr] % every_nth) flags |= 1u;
     pb.beta_short[i] = make_float4(1.f, 1.f, 1.f, 0.f);
     pb.L_short[i] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
   pb.flags[i] = flags;
+}
+
+// Training-data rays: camera position toward sampled hair points; every slot
+// is a training path (cuda/hair_msnn.cu:222-233). Seeds use the launch index
+// of a grid_w-wide 2D launch.
+__global__ void k_camera_points(SceneView sc, PathBuffers pb, int accum_id, int n, int grid_w, const float* pts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) pb.counts[0] = n;
+  if (i >= n) return;
+  const int x = i % grid_w, y = i / grid_w;
+  const uint32_t rng = path_seed((uint32_t)(accum_id + 10007), (uint32_t)x, (uint32_t)y, (uint32_t)grid_w);
+  const V3 p = v3(pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]);
+  const V3 d = normalize(p - sc.cam_pos);
+  pb.ray_o[i] = make_float4(sc.cam_pos.x, sc.cam_pos.y, sc.cam_pos.z, 0.f);
+  pb.ray_d[i] = make_float4(d.x, d.y, d.z, 0.f);
+  pb.beta[i] = make_float4(1.f, 1.f, 1.f, 0.f);
+  pb.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  pb.beta_short[i] = make_float4(1.f, 1.f, 1.f, 0.f);
+  pb.L_short[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  pb.rng[i] = rng;
+  pb.queue[0][i] = i;
+  pb.flags[i] = 1u;
 }
 
 template <bool kCount>
@@ -325,9 +346,15 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_rays(SceneView sc, const 
 
 int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hipStream_t stream,
                 std::string& err) {
-  const int n = sc.width * sc.height;
-  hipLaunchKernelGGL(k_camera, dim3((n + 255) / 256), dim3(256), 0, stream, sc, pb, cfg.accum_id, cfg.msnn,
-                     cfg.train_idx, cfg.every_nth);
+  if (cfg.camera_mode == 1) {
+    const int n = cfg.n_slots;
+    hipLaunchKernelGGL(k_camera_points, dim3((n + 255) / 256), dim3(256), 0, stream, sc, pb, cfg.accum_id, n,
+                       cfg.grid_w, cfg.points);
+  } else {
+    const int n = sc.width * sc.height;
+    hipLaunchKernelGGL(k_camera, dim3((n + 255) / 256), dim3(256), 0, stream, sc, pb, cfg.accum_id, cfg.msnn,
+                       cfg.train_idx, cfg.every_nth, cfg.n_slots);
+  }
   HM_TRY(check_launch("camera", err));
   const bool count = cfg.stats != nullptr;
   PassTimer* tm = cfg.timer;

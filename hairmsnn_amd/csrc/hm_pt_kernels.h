@@ -73,6 +73,13 @@ struct PassConfig {
   PassTimer* timer = nullptr;
   int trace_blocks = 2048;
   int shade_blocks = 2048;
+  // camera_mode 0: one jittered camera ray per pixel (cuda/path_tracing.cu:32-42)
+  // camera_mode 1: rays from the camera to sampled hair points, all slots
+  //                training paths (TRAIN_DATA_GEN, cuda/hair_msnn.cu:222-233)
+  int camera_mode = 0;
+  int n_slots = 0;             // camera_mode 1: number of records
+  int grid_w = 0;              // camera_mode 1: launch width used for seeding
+  const float* points = nullptr;  // camera_mode 1: 3 floats per record
 };
 
 // Runs one full sample pass (camera -> all vertices). Leaves per-pixel

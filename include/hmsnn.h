@@ -122,6 +122,12 @@ int hmsnn_get_info(const hmsnn_ctx* ctx, hmsnn_info* info);
  * procedural hair. Same failure rules as hmsnn_create (parseScene,
  * scene.cpp:119-339). */
 int hmsnn_parse_scene(const char* json_text, hmsnn_info* info);
+/* Generates a scene's procedural content on the host only (no GPU): hair
+ * segments (8 floats each, as hmsnn_export_segments) and the RGBA32F
+ * environment map. Capacities are in floats; *num_segments / *env_w /
+ * *env_h report the sizes (call with NULL buffers to query). */
+int hmsnn_generate_host(const char* json_text, float* seg_host, size_t seg_cap, int* num_segments,
+                        float* env_rgba_host, size_t env_cap, int* env_w, int* env_h);
 
 /* Copies generated content to the host: segments as 8 floats each
  * (p0 xyz, radius, p1 xyz, strand id); the RGBA32F environment map

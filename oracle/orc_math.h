@@ -253,7 +253,10 @@ Vec<R> hair_sample(const HairShading<R>& hs, Vec<R> wo, R h, Vec<R> sig, R u0, R
     sOp = sinThetaO;
     cOp = cosThetaO;
   }
-  const R v = p < 3 ? hs.v[p] : hs.v[2];
+  // p = 3 reads si.hair.v[3], one past float v[3]: the next member of
+  // HairInteraction is the logistic scale s (common.cuh:38-39), so the
+  // residual lobe is sampled with variance s while its pdf uses v[2].
+  const R v = p < 3 ? hs.v[p] : hs.s;
   const R e2 = std::fmax(u1, (R)1e-5f);
   const R cosTheta = (R)1 + v * std::log(e2 + ((R)1 - e2) * std::exp((R)-2 / v));
   const R sinTheta = SafeSqrt((R)1 - Sqr(cosTheta));

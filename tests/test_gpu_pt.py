@@ -29,18 +29,28 @@ def test_env_tables_match_oracle(gpu):
     with gpu.Scene(json_text=cfg) as sc:
         e = sc.env()
     o = ol.env_tables(e["rgba"], mode=0)
-    np.testing.assert_array_equal(e["c_pdf"][:, :-1], o["c_pdf"][:, :-1])
+    # the GPU's sinf may differ from the host's by one ulp, which moves the
+    # pdf entries by ~1e-7 relative; the FP64 running sums do not add error
     for k in ("c_pdf", "c_cdf", "m_pdf", "m_cdf"):
-        np.testing.assert_allclose(e[k], o[k], rtol=3e-7, atol=1e-30, err_msg=k)
+        np.testing.assert_allclose(e[k], o[k], rtol=1e-6, atol=1e-12, err_msg=k)
+    assert np.all(np.diff(e["c_cdf"], axis=1) >= 0) and np.all(np.diff(e["m_cdf"]) >= 0)
+    assert np.all(e["c_cdf"][:, -1] == 1.0) and e["m_cdf"][-1] == 1.0
     # the reference's sequential FP32 sums agree to FP32 accumulation error
     r = ol.env_tables(e["rgba"], mode=1)
     np.testing.assert_allclose(e["c_cdf"], r["c_cdf"], atol=2e-5)
     np.testing.assert_allclose(e["m_cdf"], r["m_cdf"], atol=2e-5)
 
 
-@pytest.mark.parametrize("name", ["c0_hairball.json", "c1_curly.json"])
-def test_single_pass_matches_fp32_oracle_per_pixel(gpu, name):
-    cfg = scene_cfg(name, 64, 64)
+# Light transport in hair is chaotic: each scattering off a curved fibre
+# multiplies a direction difference by ~distance / radius, so one-ulp
+# differences between GPU and host maths decorrelate long paths. Paths cut
+# after the first bounces must agree almost everywhere; full-depth paths are
+# compared per pixel only loosely here and statistically in
+# test_converges_to_fp64_oracle.
+@pytest.mark.parametrize("name,path_v2,need", [("c0_hairball.json", 2, 0.99), ("c1_curly.json", 2, 0.98),
+                                               ("c0_hairball.json", 40, 0.85), ("c1_curly.json", 40, 0.75)])
+def test_single_pass_matches_fp32_oracle_per_pixel(gpu, name, path_v2, need):
+    cfg = scene_cfg(name, 64, 64, path_v2=path_v2)
     if name == "c1_curly.json":
         cfg["hair"]["procedural"]["strands"] = 4000
     with gpu.Scene(json_text=cfg) as sc:
@@ -51,21 +61,29 @@ def test_single_pass_matches_fp32_oracle_per_pixel(gpu, name):
     o, _, hit = orc.render(0, 0, 64, 64, spp=1, accum_id0=0, prec=1)
     close = np.abs(g - o) <= 1e-3 * np.maximum(np.abs(o), 1e-2)
     frac = close.all(-1).mean()
-    print(f"{name}: per-pixel agreement {frac:.4f}, hair pixels {np.mean(hit == 1):.3f}")
-    assert frac >= 0.95
+    print(f"{name} depth {path_v2}: per-pixel agreement {frac:.4f}, hair pixels {np.mean(hit == 1):.3f}")
+    assert frac >= need
 
 
 def test_white_furnace(gpu):
-    """Env-only lighting of 1, sigma_a = 0, lossless lobes: every pixel is 1
-    (MIS with one light is unbiased; escaped paths end on NEE)."""
+    """Env-only lighting of 1, sigma_a = 0, lossless lobes. The reference's
+    estimator loses ~1 % here (the light sample's push-through position is
+    reused by the MIS BSDF sample, optix_common.cuh:247-250 and :179-182), so
+    the GPU is held to the FP64 oracle's value rather than to 1."""
     cfg = scene_cfg("c0_hairball.json", 64, 64, spp=64)
     cfg["hair"].update({"sigma_a": [0, 0, 0], "beta_m": 0.6, "beta_n": 0.5})
     cfg["lights"] = {"environment": {"constant": [1, 1, 1], "resolution": [32, 16], "scale": 1.0}}
     with gpu.Scene(json_text=cfg) as sc:
-        img = sc.render_path_tracing(128)
-    hair = img[..., :3]
-    print(f"furnace mean {hair.mean():.5f} min {hair.min():.4f} max {hair.max():.4f}")
-    assert hair.mean() == pytest.approx(1.0, abs=0.01)
+        img = sc.render_path_tracing(128)[..., :3].astype(np.float64)
+        orc = oracle_for(sc)
+    o, _, hit = orc.render(0, 0, 64, 64, spp=128, accum_id0=100_000, prec=0)
+    miss = hit == 0
+    np.testing.assert_allclose(img[miss], 1.0, rtol=1e-6)  # escaped camera rays see the env
+    g, r = img[~miss].mean(), o[~miss].mean()
+    se = np.sqrt(img[~miss].var() + o[~miss].var()) / np.sqrt(128 * (~miss).sum())
+    print(f"furnace hair mean: GPU {g:.5f} oracle {r:.5f} (se {se:.5f})")
+    assert abs(g - r) < 5 * se + 1e-4
+    assert 0.97 < r < 1.0
 
 
 def test_converges_to_fp64_oracle(gpu):

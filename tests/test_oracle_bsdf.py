@@ -97,11 +97,23 @@ def test_fp32_twin_close_to_fp64(orc):
     assert np.quantile(np.abs(f32[m] - f64[m]) / f64[m], 0.99) < 1e-4
 
 
-def test_sampling_chi_square(orc):
+def matched_beta_m(beta_n):
+    """beta_m whose TRT variance v[2] equals the logistic scale s(beta_n): the
+    residual lobe is sampled with s (reference quirk) but evaluated with v[2],
+    so at this beta_m sampling and pdf agree exactly."""
+    from scipy.optimize import brentq
+    s = 0.626657069 * (0.265 * beta_n + 1.194 * beta_n ** 2 + 5.372 * beta_n ** 22)
+    return brentq(lambda b: 4 * (0.726 * b + 0.812 * b * b + 3.7 * b ** 20) ** 2 - s, 0.05, 0.99)
+
+
+@pytest.mark.parametrize("beta_m,beta_n,sig", [(0.6, 0.5, (0.1, 0.2, 0.3)), (None, 0.9, (0.0, 0.0, 0.0))])
+def test_sampling_chi_square(orc, beta_m, beta_n, sig):
     """Sampled directions follow the returned pdf (binned chi-square)."""
-    params = ol.hair_params(0.6, 0.5, ALPHA_2DEG)
+    if beta_m is None:
+        beta_m = matched_beta_m(beta_n)
+    params = ol.hair_params(beta_m, beta_n, ALPHA_2DEG)
     so, h = 0.3, 0.4
-    q0 = np.array([so, 0, math.sqrt(1 - so * so), 0, 0, 1, h, 0.1, 0.2, 0.3], np.float32)
+    q0 = np.array([so, 0, math.sqrt(1 - so * so), 0, 0, 1, h, *sig], np.float32)
     n = 400000
     q = np.repeat(q0[None], n, 0)
     wi, f, pdf, lobe = ol.hair_sample(q, uniforms(n, 4, 5), params)
