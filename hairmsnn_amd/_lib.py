@@ -69,6 +69,7 @@ _SIGS = {
     "hmsnn_hair_bsdf_eval_dev": (_I, [C.POINTER(HairParams), _P, _I, _P, _P, _P]),
     "hmsnn_hair_bsdf_sample_dev": (_I, [C.POINTER(HairParams), _P, _P, _I, _P, _P, _P, _P, _P]),
     "hmsnn_parse_scene": (_I, [C.c_char_p, C.POINTER(Info)]),
+    "hmsnn_get_string": (_I, [_P, _I, C.c_char_p, _I]),
     "hmsnn_generate_host": (_I, [C.c_char_p, _P, C.c_size_t, C.POINTER(_I), _P, C.c_size_t, C.POINTER(_I),
                                  C.POINTER(_I)]),
     "hmsnn_create": (_I, [C.c_char_p, C.POINTER(_P)]),
@@ -100,6 +101,7 @@ _SIGS = {
     "hmsnn_mlp_forward_dev": (_I, [_P, _P, _I, _P, _P]),
     "hmsnn_mlp_backward_dev": (_I, [_P, _P, _P, _I, _I, _P, C.POINTER(_F), _P]),
     "hmsnn_mlp_train_step_dev": (_I, [_P, _P, _P, _I, _I, _F, C.POINTER(_F), _P]),
+    "hmsnn_save_exr": (_I, [C.c_char_p, _P, _I, _I]),
     "hmsnn_adjoint_render": (_I, [_P, _I, _P, _P, C.POINTER(_D)]),
 }
 

@@ -436,6 +436,16 @@ int hmsnn_get_info(const hmsnn_ctx* ctx, hmsnn_info* info) {
   return HMSNN_OK;
 }
 
+int hmsnn_get_string(const hmsnn_ctx* ctx, int which, char* buf, int len) {
+  if (!ctx || !buf || len <= 0 || which < 0 || which > 2) return fail(HMSNN_EINVAL, "hmsnn_get_string: bad arguments");
+  const Scene& s = ctx->c.scene;
+  std::string v = which == 0 ? s.image_output : which == 1 ? s.stats_output : s.json_path;
+  if (which < 2 && !v.empty() && v[0] != '/' && !(v.size() > 1 && v[1] == ':')) v = s.base_dir + "/" + v;
+  if ((int)v.size() >= len) return fail(HMSNN_EINVAL, "hmsnn_get_string: buffer too small");
+  std::memcpy(buf, v.c_str(), v.size() + 1);
+  return HMSNN_OK;
+}
+
 int hmsnn_parse_scene(const char* text, hmsnn_info* info) {
   if (!text || !info) return fail(HMSNN_EINVAL, "hmsnn_parse_scene: NULL argument");
   Scene s;

@@ -116,6 +116,10 @@ typedef struct hmsnn_info {
   float dir_emit[8][3];
 } hmsnn_info;
 int hmsnn_get_info(const hmsnn_ctx* ctx, hmsnn_info* info);
+/* Strings of the scene: which = 0 integrator.image_output, 1
+ * integrator.stats_output (both resolved against the scene file's
+ * directory when relative), 2 the scene file path. */
+int hmsnn_get_string(const hmsnn_ctx* ctx, int which, char* buf, int len);
 /* Parses a scene JSON on the host only (no GPU needed) and reports what a
  * context would be created with; fields that need generated content
  * (BVH, environment size) are left 0 except num_segments/num_strands of
@@ -219,6 +223,13 @@ int hmsnn_mlp_backward_dev(hmsnn_mlp* mlp, const float* in_dev, const float* tar
 /* One Adam step (trainer->training_step): forward, backward, update. */
 int hmsnn_mlp_train_step_dev(hmsnn_mlp* mlp, const float* in_dev, const float* target_dev, int n, int loss,
                              float lr, float* loss_out, void* stream);
+
+/* ---- image output ------------------------------------------------------ */
+
+/* Writes width x height x 4 floats (render-buffer order, bottom row first) as
+ * an uncompressed RGBA FP32 OpenEXR file, flipped like saveEXR
+ * (model.cpp:366-383). */
+int hmsnn_save_exr(const char* path, const float* rgba, int width, int height);
 
 /* ---- differentiable appearance (path-replay adjoint) --------------------- */
 
