@@ -1,0 +1,294 @@
+// Path-replay adjoint of the path tracer for differentiable hair appearance:
+// gradients of L(mean image) w.r.t. per-strand sigma_a and global beta_m,
+// beta_n, alpha, given dL/d(image).
+//
+// Each (pixel, pass) path is traced twice from the same seed (the reference's
+// per-pixel streams, cuda/path_tracing.cu:29): a primal pass that sums the
+// path's radiance, then a replay that differentiates every factor that
+// depends on the parameters at each hair vertex, with adj = dL/dI / spp:
+//   direct-lighting sample k:  c_k = f_k * w_k * (parameter-free factor), so
+//     dc_k = (c_k / f_k) * df_k + c_k * dw_k / w_k, where w_k is the
+//     power-heuristic MIS weight through the BSDF pdf (1 for delta lights);
+//   continuation: the radiance still to come after the vertex is
+//     proportional to f, so it contributes rem * df / f.
+// Sampled directions, the sampling pdfs in the estimators' denominators and
+// Russian roulette are held fixed (detached): the estimator stays unbiased
+// for the derivative of the expected image without storing a tape.
+// Per-strand gradients are reduced with FP64 atomics.
+#include <hip/hip_runtime.h>
+
+#include "../../include/hmsnn.h"
+#include "hm_api_util.h"
+#include "hm_context.h"
+#include "hm_hair_ad.h"
+
+namespace hm {
+
+namespace {
+
+struct AdjArgs {
+  HairConstsAD hcd;
+  const float4* dl_dimg;  // w*h
+  float inv_spp;
+  int pass0, npass;
+  int dl_zero, dl_from, max_vertex;
+  double* g_sigma;  // num_strands * 3
+  double* g_glob;   // 3
+  float* g_px;      // optional: w*h*3 per-pixel d/d(beta_m, beta_n, alpha)
+  int* spill;
+  int spill_stride, spill_levels;
+};
+
+HM_D int strand_of(const SceneView& sc, int prim) { return (int)sc.seg_by_id[8 * (size_t)prim + 7]; }
+
+HM_D V3 strand_sigma_of(const SceneView& sc, int prim) {
+  if (!sc.strand_sigma) return sc.sigma_a;
+  const int s = strand_of(sc, prim);
+  return v3(sc.strand_sigma[3 * s], sc.strand_sigma[3 * s + 1], sc.strand_sigma[3 * s + 2]);
+}
+
+// g[k] = sum_c wk_c * f_c'[k] + wc * r[k]; the global part goes to gg, the
+// sigma part to the strand's FP64 slots.
+HM_D void add_grad(const SceneView& sc, const AdjArgs& a, int prim, const Dual f[3], V3 wk, float wc, const float r[kAdParams],
+                   float* gg) {
+  float g[kAdParams];
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < kAdParams; ++k) {
+    g[k] = wk.x * f[0].d[k] + wk.y * f[1].d[k] + wk.z * f[2].d[k] + wc * r[k];
+    ok = ok && isfinite(g[k]);
+  }
+  if (!ok) return;
+  gg[0] += g[0];
+  gg[1] += g[1];
+  gg[2] += g[2];
+  const int strand = strand_of(sc, prim);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    if (g[3 + c] != 0.f) atomicAdd(&a.g_sigma[3 * (size_t)strand + c], (double)g[3 + c]);
+}
+
+HM_D V3 safe_ratio(V3 c, V3 f) {
+  return v3(f.x != 0.f ? c.x / f.x : 0.f, f.y != 0.f ? c.y / f.y : 0.f, f.z != 0.f ? c.z / f.z : 0.f);
+}
+
+// One path of the integrator (the order of k_camera / k_shade / k_shadow):
+// kReplay false returns the path's radiance; kReplay true also adds the
+// gradient given the primal radiance L_total.
+template <bool kReplay>
+HM_D V3 adjoint_path(const SceneView& sc, const AdjArgs& a, TraceStack& st, int x, int y, int pass, V3 L_total,
+                     V3 adj, float* gg) {
+  TraceCounters cnt{0, 0, 0};
+  uint32_t rng = path_seed((uint32_t)(pass + 10007), (uint32_t)x, (uint32_t)y, (uint32_t)sc.width);
+  const float jx = draw(rng), jy = draw(rng);
+  const float sx = ((float)x + jx) / (float)sc.width, sy = ((float)y + jy) / (float)sc.height;
+  V3 d = normalize(sc.cam_d00 + sx * sc.cam_du + sy * sc.cam_dv);
+  V3 o = sc.cam_pos;
+  float t, u;
+  int prim;
+  closest_hit<false>(sc, o, d, st, t, prim, u, cnt);
+  if (prim == kPrimMiss) return sc.env.enabled ? env_radiance(sc.env, d) : v3(0.f);
+  V3 beta = v3(1.f), L = v3(0.f);
+  for (int depth = 0; depth <= a.max_vertex; ++depth) {
+    Hit h;
+    setup_hit(sc, o, d, t, prim, u, h);
+    const bool hair = !h.surface;
+    if (depth == 0 ? a.dl_zero != 0 : depth >= a.dl_from) {
+      DirectOut dl;
+      direct_lighting(sc, h, rng, dl);
+      const bool v1 = dl.has1 && !occluded<false>(sc, dl.o1, dl.d1, st, cnt);
+      const bool v2 = dl.has2 && !occluded<false>(sc, dl.o2, dl.d2, st, cnt);
+      V3 sum = v3(0.f);
+      if (v1) sum = sum + dl.c1;
+      if (v2) sum = sum + dl.c2;
+      const bool bad = any_nan(sum);
+      if (bad) sum = v3(0.f);
+      if (kReplay && hair && !bad) {
+        const V3 sigma = strand_sigma_of(sc, prim);
+        const V3 W = adj * beta;  // dLoss / dc per channel
+        float r[kAdParams];
+        if (v1) {
+          Dual f[3], pd;
+          hair_eval_dual(a.hcd, h.wo_local, dl.wl1, h.hw.h, sigma, f, &pd);
+          const float pb = dl.bp1, pl = dl.lp1;
+#pragma unroll
+          for (int k = 0; k < kAdParams; ++k) r[k] = dl.mis1 ? -2.f * pb * pd.d[k] / (pl * pl + pb * pb) : 0.f;
+          add_grad(sc, a, prim, f, W * safe_ratio(dl.c1, dl.f1), dot(W, dl.c1), r, gg);
+        }
+        if (v2) {
+          Dual f[3], pd;
+          hair_eval_dual(a.hcd, h.wo_local, dl.wl2, h.hw.h, sigma, f, &pd);
+          const float pb = dl.bp2, pl = dl.lp2;
+#pragma unroll
+          for (int k = 0; k < kAdParams; ++k) r[k] = pb > 0.f ? 2.f * pl * pl * pd.d[k] / (pb * (pb * pb + pl * pl)) : 0.f;
+          add_grad(sc, a, prim, f, W * safe_ratio(dl.c2, dl.f2), dot(W, dl.c2), r, gg);
+        }
+      }
+      L = L + beta * sum;
+    }
+    if (depth >= 1) {
+      const float q = fmaxf(0.05f, 1.f - luminance(beta));
+      if (draw(rng) < q) break;
+      beta = beta / (1.f - q);
+    }
+    if (depth + 1 > a.max_vertex) break;
+    const BsdfSample bs = sample_bsdf(sc, h, rng);
+    const V3 wl = bs.wl, wi = bs.wi, f = bs.f;
+    const float pdf = bs.pdf;
+    V3 mul = (pdf == 0.f) ? f : f / pdf;
+    const bool mul_bad = any_nan(mul);
+    if (mul_bad) mul = v3(1.f);
+    if (kReplay && hair && !mul_bad) {
+      const V3 rem = L_total - L;  // radiance gathered after this vertex
+      Dual fd[3], pd;
+      hair_eval_dual(a.hcd, h.wo_local, wl, h.hw.h, strand_sigma_of(sc, prim), fd, &pd);
+      float r[kAdParams];
+#pragma unroll
+      for (int k = 0; k < kAdParams; ++k) r[k] = 0.f;
+      add_grad(sc, a, prim, fd, adj * safe_ratio(rem, f), 0.f, r, gg);
+    }
+    beta = beta * mul;
+    V3 p = h.p, nd = h.n;
+    if (dot(wi, h.n) < 0.f && hair) {
+      nd = -1.f * h.n;
+      p = p + (2.f * h.radius) * nd;
+    }
+    o = p + 1e-3f * nd;
+    d = wi;
+    closest_hit<false>(sc, o, d, st, t, prim, u, cnt);
+    if (prim == kPrimMiss) break;
+  }
+  return L;
+}
+
+__global__ void __launch_bounds__(kTraceBlock) k_adjoint(SceneView sc, AdjArgs a) {
+  __shared__ int lds_stack[kStackLds * kTraceBlock];
+  TraceStack st;
+  st.lds = lds_stack + threadIdx.x;
+  st.spill = a.spill + blockIdx.x * blockDim.x + threadIdx.x;
+  st.spill_stride = a.spill_stride;
+  st.spill_levels = a.spill_levels;
+  st.sp = 0;
+  const int npx = sc.width * sc.height;
+  const long long total = (long long)npx * a.npass;
+  float gg[3] = {0.f, 0.f, 0.f};
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    const int pix = (int)(i % npx), pass = a.pass0 + (int)(i / npx);
+    const int x = pix % sc.width, y = pix / sc.width;
+    const float4 g4 = a.dl_dimg[pix];
+    const V3 adj = a.inv_spp * v3(g4.x, g4.y, g4.z);
+    if (adj.x == 0.f && adj.y == 0.f && adj.z == 0.f) continue;
+    const V3 Lt = adjoint_path<false>(sc, a, st, x, y, pass, v3(0.f), adj, gg);
+    if (any_nan(Lt)) continue;  // writePixel drops NaN samples (utils.cuh:20-21)
+    float gp[3] = {0.f, 0.f, 0.f};
+    adjoint_path<true>(sc, a, st, x, y, pass, Lt, adj, gp);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      gg[k] += gp[k];
+      if (a.g_px && gp[k] != 0.f) atomicAdd(&a.g_px[3 * (size_t)pix + k], gp[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float v = gg[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (__lane_id() == 0 && v != 0.f) atomicAdd(&a.g_glob[k], (double)v);
+  }
+}
+
+}  // namespace
+
+}  // namespace hm
+
+using namespace hm;
+
+struct hmsnn_ctx {
+  Context c;
+};
+
+extern "C" {
+
+int hmsnn_set_hair_params(hmsnn_ctx* ctx, float beta_m, float beta_n, float alpha) {
+  if (!ctx || !(beta_m > 0.f) || !(beta_n > 0.f)) return fail(HMSNN_EINVAL, "hmsnn_set_hair_params: bad arguments");
+  Context& c = ctx->c;
+  c.scene.beta_m = beta_m;
+  c.scene.beta_n = beta_n;
+  c.scene.alpha = alpha;
+  const float gains[4] = {c.scene.R_G, c.scene.TT_G, c.scene.TRT_G, c.scene.TRRT_G};
+  c.view.hc = make_hair_consts(beta_m, beta_n, alpha, gains);
+  return HMSNN_OK;
+}
+
+int hmsnn_adjoint_render(hmsnn_ctx* ctx, int spp, const float* dl_dimg, double* grad_sigma, double grad_glob[3],
+                         float* grad_px) {
+  if (!ctx || spp <= 0 || !dl_dimg || !grad_glob) return fail(HMSNN_EINVAL, "hmsnn_adjoint_render: bad arguments");
+  Context& c = ctx->c;
+  if (!c.scene.has_hair) return fail(HMSNN_EINVAL, "scene has no hair");
+  std::string err;
+  int rc = c.ensure_path_buffers(false, err);
+  if (rc) return fail(rc, err);
+  const size_t npx = (size_t)c.scene.width * c.scene.height;
+  const size_t ns = (size_t)c.hair.num_strands;
+  float4* d_adj = nullptr;
+  double* d_gs = nullptr;
+  double* d_gg = nullptr;
+  float* d_px = nullptr;
+  rc = dev_alloc((void**)&d_adj, sizeof(float4) * npx, err);
+  if (!rc) rc = dev_alloc((void**)&d_gs, sizeof(double) * 3 * ns, err);
+  if (!rc) rc = dev_alloc((void**)&d_gg, sizeof(double) * 4, err);
+  if (!rc && grad_px) rc = dev_alloc((void**)&d_px, sizeof(float) * 3 * npx, err);
+  auto cleanup = [&]() {
+    (void)hipStreamSynchronize(c.stream);
+    dev_free(d_adj);
+    dev_free(d_gs);
+    dev_free(d_gg);
+    dev_free(d_px);
+  };
+  if (rc) {
+    cleanup();
+    return fail(rc, err);
+  }
+  hipError_t e = hipMemcpyAsync(d_adj, dl_dimg, sizeof(float4) * npx, hipMemcpyHostToDevice, c.stream);
+  if (e == hipSuccess) e = hipMemsetAsync(d_gs, 0, sizeof(double) * 3 * ns, c.stream);
+  if (e == hipSuccess) e = hipMemsetAsync(d_gg, 0, sizeof(double) * 4, c.stream);
+  if (e == hipSuccess && d_px) e = hipMemsetAsync(d_px, 0, sizeof(float) * 3 * npx, c.stream);
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(HMSNN_EDEVICE, hipGetErrorString(e));
+  }
+  AdjArgs a;
+  const float gains[4] = {c.scene.R_G, c.scene.TT_G, c.scene.TRT_G, c.scene.TRRT_G};
+  a.hcd = make_hair_consts_ad(c.scene.beta_m, c.scene.beta_n, c.scene.alpha, gains);
+  a.dl_dimg = d_adj;
+  a.inv_spp = 1.f / (float)spp;
+  a.pass0 = 0;
+  a.npass = spp;
+  const PassConfig pc = c.pt_pass_config();
+  a.dl_zero = pc.dl_zero;
+  a.dl_from = pc.dl_from;
+  a.max_vertex = pc.max_vertex;
+  a.g_sigma = d_gs;
+  a.g_glob = d_gg;
+  a.g_px = d_px;
+  a.spill = c.pb.spill;
+  a.spill_stride = c.pb.spill_stride;
+  a.spill_levels = c.pb.spill_levels;
+  hipLaunchKernelGGL(k_adjoint, dim3(c.trace_blocks), dim3(kTraceBlock), 0, c.stream, c.view, a);
+  rc = check_launch("adjoint", err);
+  if (!rc) {
+    double gg[4];
+    e = hipMemcpyAsync(gg, d_gg, sizeof(gg), hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess && grad_sigma)
+      e = hipMemcpyAsync(grad_sigma, d_gs, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess && grad_px)
+      e = hipMemcpyAsync(grad_px, d_px, sizeof(float) * 3 * npx, hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess) rc = HMSNN_EDEVICE, err = hipGetErrorString(e);
+    else for (int k = 0; k < 3; ++k) grad_glob[k] = gg[k];
+  }
+  cleanup();
+  return rc ? fail(rc, err) : HMSNN_OK;
+}
+
+}  // extern "C"

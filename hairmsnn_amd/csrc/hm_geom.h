@@ -107,22 +107,29 @@ HM_HD bool intersect_sphere(V3 o, V3 d, V3 c, float radius, float* t_out) {
 // for the linear evaluator): the point is projected onto the radius-r
 // surface, the normal is radial (flat end caps at exactly u = 0 or 1), the
 // tangent is normalize(p1 - p0) (curve_utils.cuh:253-254).
-HM_HD void segment_hit_geometry(V3 p0, V3 p1, float r, float u, V3* ps, V3* n, V3* tangent) {
+struct SegHitGeom {
+  V3 p, n, tangent;
+};
+
+HM_HD SegHitGeom segment_hit_geometry(V3 p0, V3 p1, float r, float u, V3 ps) {
   const V3 dv = p1 - p0;
+  SegHitGeom g;
+  g.p = ps;
   if (u == 0.f) {
-    *n = normalize(-dv);
+    g.n = normalize(-dv);
   } else if (u == 1.f) {
-    *n = normalize(dv);
+    g.n = normalize(dv);
   } else {
     const V3 pa = p0 + u * dv;
     const float dd = dot(dv, dv);
-    V3 o1 = *ps - pa;
+    V3 o1 = ps - pa;
     o1 = o1 - (dot(o1, dv) / dd) * dv;
     o1 = o1 * (r / length(o1));
-    *ps = pa + o1;
-    *n = normalize(dd * o1);
+    g.p = pa + o1;
+    g.n = normalize(dd * o1);
   }
-  *tangent = normalize(dv);
+  g.tangent = normalize(dv);
+  return g;
 }
 
 }  // namespace hm

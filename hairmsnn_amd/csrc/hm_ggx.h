@@ -67,11 +67,19 @@ HM_HD V3 ggx_sample_vndf(float a, V3 v, float u1, float u2) {
 HM_HD float surface_pdf(float a, V3 v, V3 ne) { return ggx_dv(a, v, ne) / (4.f * dot(v, ne)); }
 
 // Reflect about a sampled visible normal (frostbite_anisotropic.cuh:143-157).
-HM_HD V3 surface_sample(float a, V3 v, float u1, float u2, float* pdf) {
+struct DirPdf {
+  V3 dir;
+  float pdf;
+};
+HM_HD DirPdf surface_sample_dp(float a, V3 v, float u1, float u2) {
   const V3 n = ggx_sample_vndf(a, v, u1, u2);
   const V3 l = -v + (2.f * dot(v, n)) * n;
-  *pdf = surface_pdf(a, v, n);
-  return normalize(l);
+  return DirPdf{normalize(l), surface_pdf(a, v, n)};
+}
+HM_HD V3 surface_sample(float a, V3 v, float u1, float u2, float* pdf) {
+  const DirPdf r = surface_sample_dp(a, v, u1, u2);
+  *pdf = r.pdf;
+  return r.dir;
 }
 
 }  // namespace hm

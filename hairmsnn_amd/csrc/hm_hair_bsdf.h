@@ -218,8 +218,15 @@ HM_HD HairWo hair_prepare(const HairConsts& hc, V3 wo, float h, V3 sigma_a) {
 }
 
 // f * |cos theta_i| (RGB) and the lobe-mixture pdf for direction wi, both
-// in the hair frame (disney_hair.cuh:185-266).
-HM_HD V3 hair_eval(const HairConsts& hc, const HairWo& w, V3 wi, float* pdf) {
+// in the hair frame (disney_hair.cuh:185-266). Returned by value: output
+// pointers into caller locals defeat register promotion when the optimiser
+// merges call sites.
+struct FPdf {
+  V3 f;
+  float pdf;
+};
+
+HM_HD FPdf hair_eval_fp(const HairConsts& hc, const HairWo& w, V3 wi) {
   const float si = wi.x;
   const float ci = safe_sqrt((1.f - si) * (1.f + si));
   const float phi = atan2f(wi.z, wi.y) - w.phi_o;
@@ -237,31 +244,50 @@ HM_HD V3 hair_eval(const HairConsts& hc, const HairWo& w, V3 wi, float* pdf) {
   const float np3 = 1.f / (2.f * kPi);
   f += (hc.gain[3] * mp3 * np3) * w.ap[3];
   pd += mp3 * w.ap_pdf[3] * np3;
-  *pdf = pd;
-  return f;
+  return FPdf{f, pd};
+}
+
+HM_HD V3 hair_eval(const HairConsts& hc, const HairWo& w, V3 wi, float* pdf) {
+  const FPdf r = hair_eval_fp(hc, w, wi);
+  *pdf = r.pdf;
+  return r.f;
+}
+
+HM_HD float sel4(int p, float a0, float a1, float a2, float a3) {
+  float r = a3;
+  r = p == 2 ? a2 : r;
+  r = p == 1 ? a1 : r;
+  r = p == 0 ? a0 : r;
+  return r;
 }
 
 // Importance-sample wi from four uniforms (disney_hair.cuh:276-386). Returns
 // the sampled direction in the hair frame and writes the chosen lobe.
 HM_HD V3 hair_sample_dir(const HairConsts& hc, const HairWo& w, float u0, float u1, float u2,
                          float u3, int* lobe_out) {
-  int p = 0;
+  // lobe choice (disney_hair.cuh:300-305), written without a loop so every
+  // array index is a constant
+  int p;
   float e1 = u0;
-  for (p = 0; p < 3; ++p) {
-    if (e1 < w.ap_pdf[p]) break;
-    e1 -= w.ap_pdf[p];
-  }
-  float so_t, co_t;
-  if (p < 3) {
-    so_t = w.so_t[p];
-    co_t = w.co_t[p];  // sampling uses the signed tilted cosine
+  if (e1 < w.ap_pdf[0]) {
+    p = 0;
   } else {
-    so_t = w.so;
-    co_t = w.co;
+    e1 -= w.ap_pdf[0];
+    if (e1 < w.ap_pdf[1]) {
+      p = 1;
+    } else {
+      e1 -= w.ap_pdf[1];
+      p = e1 < w.ap_pdf[2] ? 2 : 3;
+    }
   }
-  const float v = hc.vs[p];
+  // value selects (not lvalue conditionals, not indexing) keep the per-lobe
+  // values in registers; sampling uses the signed tilted cosine
+  const float so_t = sel4(p, w.so_t[0], w.so_t[1], w.so_t[2], w.so);
+  const float co_t = sel4(p, w.co_t[0], w.co_t[1], w.co_t[2], w.co);
+  const float v = sel4(p, hc.vs[0], hc.vs[1], hc.vs[2], hc.vs[3]);
+  const float inv_v = sel4(p, hc.inv_vs[0], hc.inv_vs[1], hc.inv_vs[2], hc.inv_vs[3]);
   const float e2 = fmaxf(u1, 1e-5f);
-  const float cos_t = 1.f + v * logf(e2 + (1.f - e2) * expf(-2.f * hc.inv_vs[p]));
+  const float cos_t = 1.f + v * logf(e2 + (1.f - e2) * expf(-2.f * inv_v));
   const float sin_t = safe_sqrt((1.f - cos_t) * (1.f + cos_t));
   const float cos_phi = cosf(2.f * kPi * u2);
   const float si = -cos_t * so_t + sin_t * cos_phi * co_t;

@@ -100,9 +100,13 @@ struct Hit {
 HM_D float draw(uint32_t& s) { return lcg_uniform(s); }
 
 // Closest-hit shading setup (optix_common.cuh:527-599).
+// Every field is written on both branches so the optimiser can keep the
+// whole record in registers.
 HM_D void setup_hit(const SceneView& sc, V3 o, V3 d, float t, int prim, float u, Hit& h) {
   h.wo = -1.f * d;
-  V3 p = o + t * d;
+  const V3 p = o + t * d;
+  h.albedo = sc.sphere_albedo;
+  h.alpha = sc.sphere_alpha;
   if (prim == kPrimSphere) {
     const V3 c = v3(sc.sphere.cx, sc.sphere.cy, sc.sphere.cz);
     h.n = normalize(p - c);
@@ -110,60 +114,74 @@ HM_D void setup_hit(const SceneView& sc, V3 o, V3 d, float t, int prim, float u,
     h.fr = basis_from_normal(h.n);
     h.wo_local = normalize(to_local(h.fr, h.wo));
     h.surface = 1;
-    h.albedo = sc.sphere_albedo;
-    h.alpha = sc.sphere_alpha;
     h.radius = 0.f;
-    return;
+    h.hw = HairWo{};
+  } else {
+    const float* s = sc.seg_by_id + 8 * (size_t)prim;
+    const V3 p0 = v3(s[0], s[1], s[2]), p1 = v3(s[4], s[5], s[6]);
+    const SegHitGeom g = segment_hit_geometry(p0, p1, s[3], u, p);
+    h.p = g.p;
+    h.n = g.n;
+    h.radius = s[3];
+    h.fr = hair_frame(g.tangent, h.wo);
+    h.wo_local = normalize(to_local(h.fr, h.wo));
+    h.surface = 0;
+    V3 sig = sc.sigma_a;
+    if (sc.strand_sigma) {
+      const int strand = (int)s[7];
+      sig = v3(sc.strand_sigma[3 * strand], sc.strand_sigma[3 * strand + 1], sc.strand_sigma[3 * strand + 2]);
+    }
+    const float hh = dot(h.fr.r1, g.n);
+    h.hw = hair_prepare(sc.hc, h.wo_local, hh, sig);
   }
-  const float* s = sc.seg_by_id + 8 * (size_t)prim;
-  const V3 p0 = v3(s[0], s[1], s[2]), p1 = v3(s[4], s[5], s[6]);
-  V3 n, tg;
-  segment_hit_geometry(p0, p1, s[3], u, &p, &n, &tg);
-  h.p = p;
-  h.n = n;
-  h.radius = s[3];
-  h.fr = hair_frame(tg, h.wo);
-  h.wo_local = normalize(to_local(h.fr, h.wo));
-  h.surface = 0;
-  V3 sig = sc.sigma_a;
-  if (sc.strand_sigma) {
-    const int strand = (int)s[7];
-    sig = v3(sc.strand_sigma[3 * strand], sc.strand_sigma[3 * strand + 1], sc.strand_sigma[3 * strand + 2]);
-  }
-  const float hh = dot(h.fr.r1, n);
-  h.hw = hair_prepare(sc.hc, h.wo_local, hh, sig);
 }
 
-HM_D V3 eval_bsdf(const SceneView& sc, const Hit& h, V3 wi_local, float* pdf) {
+HM_D FPdf eval_bsdf(const SceneView& sc, const Hit& h, V3 wi_local) {
   if (h.surface) {
-    *pdf = surface_pdf(h.alpha, h.wo_local, normalize(h.wo_local + wi_local));
-    return surface_eval(h.wo_local, wi_local, h.albedo, h.alpha);
+    return FPdf{surface_eval(h.wo_local, wi_local, h.albedo, h.alpha),
+                surface_pdf(h.alpha, h.wo_local, normalize(h.wo_local + wi_local))};
   }
-  return hair_eval(sc.hc, h.hw, wi_local, pdf);
+  return hair_eval_fp(sc.hc, h.hw, wi_local);
 }
+
+// A BSDF sample: local and world direction, f * |cos|, pdf.
+struct BsdfSample {
+  V3 wl, wi, f;
+  float pdf;
+};
 
 // BSDF sample with the reference's draw order: two uniforms, then two more
 // on hair (optix_common.cuh:154-170, :314-329).
-HM_D V3 sample_bsdf(const SceneView& sc, const Hit& h, uint32_t& rng, V3* wi_local, V3* wi, float* pdf) {
+HM_D BsdfSample sample_bsdf(const SceneView& sc, const Hit& h, uint32_t& rng) {
   const float r0 = draw(rng), r1 = draw(rng);
-  V3 f;
+  BsdfSample s;
   if (!h.surface) {
     const float r2 = draw(rng), r3 = draw(rng);
     int lobe;
-    *wi_local = hair_sample_dir(sc.hc, h.hw, r0, r1, r2, r3, &lobe);
-    *wi = normalize(to_world(h.fr, *wi_local));
-    f = hair_eval(sc.hc, h.hw, *wi_local, pdf);
+    s.wl = hair_sample_dir(sc.hc, h.hw, r0, r1, r2, r3, &lobe);
+    s.wi = normalize(to_world(h.fr, s.wl));
+    const FPdf e = hair_eval_fp(sc.hc, h.hw, s.wl);
+    s.f = e.f;
+    s.pdf = e.pdf;
   } else {
-    *wi_local = surface_sample(h.alpha, h.wo_local, r0, r1, pdf);
-    *wi = normalize(to_world(h.fr, *wi_local));
-    f = surface_eval(h.wo_local, *wi_local, h.albedo, h.alpha);
+    const DirPdf d = surface_sample_dp(h.alpha, h.wo_local, r0, r1);
+    s.wl = d.dir;
+    s.pdf = d.pdf;
+    s.wi = normalize(to_world(h.fr, s.wl));
+    s.f = surface_eval(h.wo_local, s.wl, h.albedo, h.alpha);
   }
-  return f;
+  return s;
 }
 
 struct DirectOut {
   int has1, has2;
   V3 o1, d1, c1, o2, d2, c2;
+  // for the adjoint: the BSDF values (f * |cos|) and local directions behind
+  // c1 and c2, and the pdfs of their MIS weights (mis1: c1 carries the
+  // weight lp1^2 / (lp1^2 + bp1^2); c2 always carries bp2^2 / (bp2^2 + lp2^2))
+  V3 f1, f2, wl1, wl2;
+  int mis1;
+  float lp1, bp1, lp2, bp2;
 };
 
 // directLighting with MIS (optix_common.cuh:205-309) up to the shadow tests.
@@ -188,8 +206,13 @@ HM_D void direct_lighting(const SceneView& sc, const Hit& h, uint32_t& rng, Dire
     emit = env_sample(sc.env, ux, uy, &wi, &env_p);
     light_pdf = light_pdf * env_p;
   } else {
-    emit = sc.dl_emit[sel];
-    wi = normalize(sc.dl_from[sel]);
+#pragma unroll
+    for (int k = 0; k < kMaxDirLights; ++k) {
+      if (k == sel) {  // unrolled select: no dynamic indexing into the kernel argument
+        emit = sc.dl_emit[k];
+        wi = normalize(sc.dl_from[k]);
+      }
+    }
     is_delta = true;
   }
   {
@@ -204,8 +227,9 @@ HM_D void direct_lighting(const SceneView& sc, const Hit& h, uint32_t& rng, Dire
       p = p + (2.f * h.radius) * nd;
     }
     if (!skip) {
-      float bpdf;
-      const V3 f = eval_bsdf(sc, h, wi_local, &bpdf);
+      const FPdf ev = eval_bsdf(sc, h, wi_local);
+      const V3 f = ev.f;
+      const float bpdf = ev.pdf;
       V3 c;
       if (sc.mis && !is_delta) c = f * emit * power_heuristic(light_pdf, bpdf) / light_pdf;
       else c = f * emit / light_pdf;
@@ -213,13 +237,18 @@ HM_D void direct_lighting(const SceneView& sc, const Hit& h, uint32_t& rng, Dire
       out.o1 = p + 1e-3f * nd;
       out.d1 = wi;
       out.c1 = c;
+      out.f1 = f;
+      out.wl1 = wi_local;
+      out.mis1 = sc.mis && !is_delta;
+      out.lp1 = light_pdf;
+      out.bp1 = bpdf;
     }
   }
   if (!sc.mis || is_delta) return;
   // ---- BSDF sample toward the environment (sampleBSDF) ----
-  V3 wl, wb;
-  float bpdf;
-  const V3 f = sample_bsdf(sc, h, rng, &wl, &wb, &bpdf);
+  const BsdfSample bs = sample_bsdf(sc, h, rng);
+  const V3 wl = bs.wl, wb = bs.wi, f = bs.f;
+  const float bpdf = bs.pdf;
   const bool lower = dot(wb, h.n) < 0.f;
   V3 nd = h.n;
   if (h.surface && lower) return;
@@ -233,6 +262,10 @@ HM_D void direct_lighting(const SceneView& sc, const Hit& h, uint32_t& rng, Dire
   out.o2 = p + 1e-3f * nd;
   out.d2 = wb;
   out.c2 = f * e * power_heuristic(bpdf, lp) / bpdf;
+  out.f2 = f;
+  out.wl2 = wl;
+  out.lp2 = lp;
+  out.bp2 = bpdf;
 }
 
 // Closest hit against the head sphere and the hair BVH. The sphere is tested

@@ -215,9 +215,9 @@ __global__ void __launch_bounds__(256) k_shade(SceneView sc, PathBuffers pb, Sha
         }
         // ---- continuation (nextPathVertex, optix_common.cuh:311-360) ----
         if (alive && sp.depth + 1 <= sp.max_vertex) {
-          V3 wl, wi;
-          float pdf;
-          const V3 f = sample_bsdf(sc, h, rng, &wl, &wi, &pdf);
+          const BsdfSample bs = sample_bsdf(sc, h, rng);
+          const V3 wl = bs.wl, wi = bs.wi, f = bs.f;
+          const float pdf = bs.pdf;
           V3 mul = (pdf == 0.f) ? f : f / pdf;
           if (any_nan(mul)) mul = v3(1.f);
           beta = beta * mul;

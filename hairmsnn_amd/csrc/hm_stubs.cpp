@@ -7,8 +7,5 @@
 
 extern "C" {
 
-int hmsnn_adjoint_render(hmsnn_ctx*, int, const float*, double*, double[3]) {
-  HM_UNSUPPORTED("hmsnn_adjoint_render");
-}
 
 }  // extern "C"

@@ -233,12 +233,20 @@ int hmsnn_save_exr(const char* path, const float* rgba, int width, int height);
 
 /* ---- differentiable appearance (path-replay adjoint) --------------------- */
 
-/* Gradient of L = 0.5 * sum over pixels and channels of
- * (mean image - target)^2 ... see the implementation notes. dL/dimage is
- * given as width x height x 4 floats (w ignored). Writes per-strand
- * d/d sigma_a (num_strands x 3, FP64) and d/d (beta_m, beta_n, alpha). */
+/* Sets the global hair roughness and scale tilt (the values parseScene reads,
+ * scene.cpp:201-208; alpha in radians). */
+int hmsnn_set_hair_params(hmsnn_ctx* ctx, float beta_m, float beta_n, float alpha);
+/* Gradient of a loss of the spp-sample mean image, given dL/d(image) as
+ * width x height x 4 floats (w ignored): per-strand d/d sigma_a
+ * (num_strands x 3, FP64, NULL to skip), d/d (beta_m, beta_n, alpha), and
+ * optionally each pixel's share of the latter (width x height x 3 floats,
+ * NULL to skip). Paths are replayed from the passes 0..spp-1 of the path
+ * tracer's streams (the image hmsnn_render_path_tracing(ctx, spp) makes);
+ * sampled directions, sampling pdfs and Russian roulette are held fixed,
+ * while f and the MIS weights are differentiated. No reference counterpart:
+ * the reference renders only. */
 int hmsnn_adjoint_render(hmsnn_ctx* ctx, int spp, const float* dl_dimage_host, double* grad_sigma_host,
-                         double grad_global[3]);
+                         double grad_global[3], float* grad_pixel_host);
 
 #ifdef __cplusplus
 }

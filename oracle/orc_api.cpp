@@ -94,6 +94,49 @@ struct orc_scene {
   int mis, path_v1, path_v2;
 };
 
+namespace {
+SceneDesc scene_desc(const orc_scene* sc, EnvTables& env) {
+  SceneDesc s{};
+  s.seg = sc->seg;
+  s.nseg = sc->nseg;
+  s.has_sphere = sc->has_sphere;
+  std::memcpy(s.sphere, sc->sphere, sizeof(s.sphere));
+  std::memcpy(s.sphere_albedo, sc->sphere_albedo, sizeof(s.sphere_albedo));
+  s.sphere_alpha = sc->sphere_alpha;
+  s.has_env = sc->has_env;
+  s.env_pdf = sc->env_pdf;
+  s.env_scale = sc->env_scale;
+  s.env_rot = sc->env_rot;
+  if (sc->has_env) {
+    env.W = sc->env_w;
+    env.H = sc->env_h;
+    const size_t np = (size_t)env.W * env.H;
+    env.rgba.assign(sc->env_rgba, sc->env_rgba + np * 4);
+    env.cpdf.assign(sc->c_pdf, sc->c_pdf + (size_t)(env.W + 1) * env.H);
+    env.ccdf.assign(sc->c_cdf, sc->c_cdf + (size_t)(env.W + 1) * env.H);
+    env.mpdf.assign(sc->m_pdf, sc->m_pdf + env.H + 1);
+    env.mcdf.assign(sc->m_cdf, sc->m_cdf + env.H + 1);
+  }
+  s.env = &env;
+  s.ndl = sc->ndl;
+  std::memcpy(s.dl_from, sc->dl_from, sizeof(s.dl_from));
+  std::memcpy(s.dl_emit, sc->dl_emit, sizeof(s.dl_emit));
+  std::memcpy(s.cam, sc->cam, sizeof(s.cam));
+  s.width = sc->width;
+  s.height = sc->height;
+  s.beta_m = sc->beta_m;
+  s.beta_n = sc->beta_n;
+  s.alpha = sc->alpha;
+  std::memcpy(s.gains, sc->gains, sizeof(s.gains));
+  std::memcpy(s.sigma_a, sc->sigma_a, sizeof(s.sigma_a));
+  s.strand_sigma = sc->strand_sigma;
+  s.mis = sc->mis;
+  s.path_v1 = sc->path_v1;
+  s.path_v2 = sc->path_v2;
+  return s;
+}
+}  // namespace
+
 extern "C" {
 
 uint32_t orc_path_seed(uint32_t frame, uint32_t x, uint32_t y, uint32_t w) { return get_rng(frame, x, y, w); }
@@ -206,48 +249,22 @@ int orc_render(const orc_scene* sc, void* bvh_handle, int mode, int kbeta, int a
                int w, int h, int threads, int prec, double* out_full, double* out_short, uint8_t* out_hit) {
   if (!sc || !bvh_handle || spp <= 0) return 1;
   EnvTables env;
-  SceneDesc s{};
-  s.seg = sc->seg;
-  s.nseg = sc->nseg;
-  s.has_sphere = sc->has_sphere;
-  std::memcpy(s.sphere, sc->sphere, sizeof(s.sphere));
-  std::memcpy(s.sphere_albedo, sc->sphere_albedo, sizeof(s.sphere_albedo));
-  s.sphere_alpha = sc->sphere_alpha;
-  s.has_env = sc->has_env;
-  s.env_pdf = sc->env_pdf;
-  s.env_scale = sc->env_scale;
-  s.env_rot = sc->env_rot;
-  if (sc->has_env) {
-    env.W = sc->env_w;
-    env.H = sc->env_h;
-    const size_t np = (size_t)env.W * env.H;
-    env.rgba.assign(sc->env_rgba, sc->env_rgba + np * 4);
-    env.cpdf.assign(sc->c_pdf, sc->c_pdf + (size_t)(env.W + 1) * env.H);
-    env.ccdf.assign(sc->c_cdf, sc->c_cdf + (size_t)(env.W + 1) * env.H);
-    env.mpdf.assign(sc->m_pdf, sc->m_pdf + env.H + 1);
-    env.mcdf.assign(sc->m_cdf, sc->m_cdf + env.H + 1);
-  }
-  s.env = &env;
-  s.ndl = sc->ndl;
-  std::memcpy(s.dl_from, sc->dl_from, sizeof(s.dl_from));
-  std::memcpy(s.dl_emit, sc->dl_emit, sizeof(s.dl_emit));
-  std::memcpy(s.cam, sc->cam, sizeof(s.cam));
-  s.width = sc->width;
-  s.height = sc->height;
-  s.beta_m = sc->beta_m;
-  s.beta_n = sc->beta_n;
-  s.alpha = sc->alpha;
-  std::memcpy(s.gains, sc->gains, sizeof(s.gains));
-  std::memcpy(s.sigma_a, sc->sigma_a, sizeof(s.sigma_a));
-  s.strand_sigma = sc->strand_sigma;
-  s.mis = sc->mis;
-  s.path_v1 = sc->path_v1;
-  s.path_v2 = sc->path_v2;
+  const SceneDesc s = scene_desc(sc, env);
   const Bvh& bvh = *(const Bvh*)bvh_handle;
   if (prec == 0)
     render_pixels<double>(s, bvh, mode, kbeta, accum_id0, spp, x0, y0, w, h, threads, out_full, out_short, out_hit);
   else
     render_pixels<float>(s, bvh, mode, kbeta, accum_id0, spp, x0, y0, w, h, threads, out_full, out_short, out_hit);
+  return 0;
+}
+
+// FP64 path-replay adjoint of the full image (orc_scene.h adjoint_render).
+int orc_adjoint(const orc_scene* sc, void* bvh_handle, int spp, const float* dl_dimg, int num_strands, int threads,
+                double fd_h, double* grad_sigma, double* grad_glob, double* grad_px) {
+  if (!sc || !bvh_handle || spp <= 0 || !dl_dimg || num_strands <= 0) return 1;
+  EnvTables env;
+  const SceneDesc s = scene_desc(sc, env);
+  adjoint_render(s, *(const Bvh*)bvh_handle, spp, dl_dimg, num_strands, threads, fd_h, grad_sigma, grad_glob, grad_px);
   return 0;
 }
 

@@ -281,6 +281,7 @@ struct Inter {
   Vec<R> color;  // sigma_a (hair) or albedo (surface)
   R alpha = 0;
   Vec<R> Le;
+  int prim = -1;  // segment id, -2 for the head sphere
 };
 
 template <class R>
@@ -407,6 +408,7 @@ template <class R>
 void setup_hit(const Ctx<R>& c, Vec<R> o, Vec<R> d, float t, int prim, float u, Inter<R>& si) {
   const SceneDesc& s = c.s;
   si.hit = true;
+  si.prim = prim;
   si.wo = (R)-1 * d;
   Vec<R> p = o + (R)t * d;
   if (prim == -2) {  // head sphere (triangleMeshCH analogue, optix_common.cuh:527-556)
@@ -539,9 +541,18 @@ void trace(const Ctx<R>& c, Vec<R> o, Vec<R> d, Inter<R>& si) {
   setup_hit(c, o, d, best, prim, u, si);
 }
 
+// What the adjoint needs of one direct-lighting sample that reached a light:
+// the direction, the radiance, both pdfs and whether the MIS weight applies.
+template <class R>
+struct DlRec {
+  bool valid = false, mis = false;
+  Vec<R> wi_local, emit;
+  R light_pdf = 0, bsdf_pdf = 0;
+};
+
 // sampleBSDF (optix_common.cuh:151-203); si is the direct-lighting copy.
 template <class R>
-Vec<R> sample_bsdf_light(const Ctx<R>& c, Inter<R>& si, Lcg& rng) {
+Vec<R> sample_bsdf_light(const Ctx<R>& c, Inter<R>& si, Lcg& rng, DlRec<R>* rec = nullptr) {
   R bsdfPdf = 1;
   const Vec<R> bsdf = bsdf_sample(c, si, rng, &bsdfPdf);
   const bool lower = dot(si.wi, si.n) < 0;
@@ -554,6 +565,7 @@ Vec<R> sample_bsdf_light(const Ctx<R>& c, Inter<R>& si, Lcg& rng) {
   if (occluded(c, si.p + (R)1e-3f * nd, si.wi)) return mk((R)0, (R)0, (R)0);
   const Vec<R> emit = env_radiance<R>(c.s, si.wi);
   const R lightPdf = env_pdf<R>(c.s, si.wi) * (R)1 / (R)(c.s.ndl + (c.s.has_env ? 1 : 0));
+  if (rec) *rec = DlRec<R>{true, c.s.mis != 0, si.wi_local, emit, lightPdf, bsdfPdf};
   const R f2 = bsdfPdf * bsdfPdf, g2 = lightPdf * lightPdf;
   if (c.s.mis) return (f2 / (f2 + g2) / bsdfPdf) * (bsdf * emit);
   return (bsdf * emit) / bsdfPdf;
@@ -561,7 +573,7 @@ Vec<R> sample_bsdf_light(const Ctx<R>& c, Inter<R>& si, Lcg& rng) {
 
 // sampleLights (optix_common.cuh:205-277)
 template <class R>
-Vec<R> sample_lights(const Ctx<R>& c, Inter<R>& si, Lcg& rng, bool& isDelta) {
+Vec<R> sample_lights(const Ctx<R>& c, Inter<R>& si, Lcg& rng, bool& isDelta, DlRec<R>* rec = nullptr) {
   isDelta = false;
   const int ntot = c.s.ndl + (c.s.has_env ? 1 : 0);
   const float rsel = rng.next();
@@ -592,6 +604,7 @@ Vec<R> sample_lights(const Ctx<R>& c, Inter<R>& si, Lcg& rng, bool& isDelta) {
   if (occluded(c, si.p + (R)1e-3f * nd, si.wi)) return mk((R)0, (R)0, (R)0);
   R bsdfPdf = 1;
   const Vec<R> bsdf = bsdf_eval(c, si, &bsdfPdf);
+  if (rec) *rec = DlRec<R>{true, c.s.mis && !isDelta, si.wi_local, emit, lightPdf, bsdfPdf};
   if (c.s.mis && !isDelta) {
     const R f2 = lightPdf * lightPdf, g2 = bsdfPdf * bsdfPdf;
     return (f2 / (f2 + g2) / lightPdf) * (bsdf * emit);
@@ -601,28 +614,48 @@ Vec<R> sample_lights(const Ctx<R>& c, Inter<R>& si, Lcg& rng, bool& isDelta) {
 
 // directLighting (optix_common.cuh:279-309): si by value.
 template <class R>
-Vec<R> direct_lighting(const Ctx<R>& c, Inter<R> si, Lcg& rng) {
+Vec<R> direct_lighting(const Ctx<R>& c, Inter<R> si, Lcg& rng, DlRec<R>* r1 = nullptr, DlRec<R>* r2 = nullptr) {
   Vec<R> out;
   if (c.s.mis) {
     bool isDelta = false;
-    const Vec<R> ls = sample_lights(c, si, rng, isDelta);
+    const Vec<R> ls = sample_lights(c, si, rng, isDelta, r1);
     Vec<R> bs = mk((R)0, (R)0, (R)0);
-    if (!isDelta) bs = sample_bsdf_light(c, si, rng);
+    if (!isDelta) bs = sample_bsdf_light(c, si, rng, r2);
     out = ls + bs;
   } else {
     bool isDelta = false;
-    out = sample_lights(c, si, rng, isDelta);
+    out = sample_lights(c, si, rng, isDelta, r1);
   }
-  if (has_nan(out)) out = mk((R)0, (R)0, (R)0);
+  if (has_nan(out)) {
+    out = mk((R)0, (R)0, (R)0);
+    if (r1) r1->valid = false;
+    if (r2) r2->valid = false;
+  }
   return out;
 }
 
+// The continuation's hair query, BSDF value and vertex, for the adjoint.
+template <class R>
+struct NvRec {
+  bool hair = false, mul_bad = false;
+  int prim = -1;
+  HairQuery<R> q;
+  Vec<R> f;
+};
+
 // nextPathVertex (optix_common.cuh:311-360); returns mulFac.
 template <class R>
-Vec<R> next_vertex(const Ctx<R>& c, Inter<R>& si, Lcg& rng) {
+Vec<R> next_vertex(const Ctx<R>& c, Inter<R>& si, Lcg& rng, NvRec<R>* rec = nullptr) {
   R pdf = 1;
   const Vec<R> f = bsdf_sample(c, si, rng, &pdf);
   Vec<R> mul = pdf == 0 ? f : f / pdf;
+  if (rec) {
+    rec->hair = !si.surface;
+    rec->mul_bad = has_nan(mul);
+    rec->prim = si.prim;
+    rec->q = HairQuery<R>{si.wo_local, si.wi_local, dot(si.to_local[1], si.n), si.color};
+    rec->f = f;
+  }
   if (has_nan(mul)) mul = mk((R)1, (R)1, (R)1);
   const Vec<R> wi = si.wi;
   Vec<R> nd = si.n;
@@ -752,6 +785,189 @@ void render_pixels(const SceneDesc& s, const Bvh& bvh, int mode, int kbeta, int 
         if (out_short) out_short[3 * o + k] = accs[k] / spp;
       }
       if (out_hit) out_hit[o] = hitflag;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Path-replay adjoint in FP64 with the BSDF derivatives taken by central
+// differences of the FP64 hair BSDF (an independent route to the GPU's dual
+// numbers). Each (pixel, pass) path of pathTrace is run twice from its seed:
+// first for its radiance, then again accumulating, at every hair vertex, the
+// parameter derivative of
+//   the light sample      f * Le * w(light_pdf, bsdf_pdf(theta)) / light_pdf,
+//   the BSDF env sample   f * Le * w(bsdf_pdf(theta), light_pdf) / bsdf_pdf0,
+//   the continuation      rem * f(theta) / f0 (rem: radiance still to come),
+// with the sampled directions, the sampling pdfs and Russian roulette fixed.
+namespace {
+
+struct AdjFd {
+  HairParams<double> hp[7];  // base, then -/+ for beta_m, beta_n, alpha
+  HairShading<double> hs[7];
+  double h;
+};
+
+inline double pw(double a, double b) { return a * a / (a * a + b * b); }
+
+// g[k] += d/d theta_k of sum_c W_c * C_c(f(theta), pdf(theta)) by central
+// differences; parameters 3..5 are the query's sigma_a channels.
+template <class F>
+void fd_grad(const AdjFd& a, const HairQuery<double>& q, Vec<double> W, F contrib, double g[6]) {
+  auto S = [&](int j, const HairQuery<double>& qq) {
+    double pdf;
+    const Vec<double> f = hair_eval(a.hp[j], a.hs[j], qq, &pdf);
+    const Vec<double> C = contrib(f, pdf);
+    return W.x * C.x + W.y * C.y + W.z * C.z;
+  };
+  for (int k = 0; k < 3; ++k) g[k] += (S(2 + 2 * k, q) - S(1 + 2 * k, q)) / (2 * a.h);
+  for (int ch = 0; ch < 3; ++ch) {
+    HairQuery<double> qp = q, qm = q;
+    double* cp = ch == 0 ? &qp.sigma_a.x : ch == 1 ? &qp.sigma_a.y : &qp.sigma_a.z;
+    double* cm = ch == 0 ? &qm.sigma_a.x : ch == 1 ? &qm.sigma_a.y : &qm.sigma_a.z;
+    *cp += a.h;
+    *cm -= a.h;
+    g[3 + ch] += (S(0, qp) - S(0, qm)) / (2 * a.h);
+  }
+}
+
+struct AdjAcc {
+  double glob[3] = {0, 0, 0};
+  std::vector<double> sigma;
+};
+
+void add_vertex_grad(const SceneDesc& s, int prim, const double g[6], AdjAcc& acc, double* px) {
+  for (int k = 0; k < 6; ++k)
+    if (!std::isfinite(g[k])) return;
+  for (int k = 0; k < 3; ++k) {
+    acc.glob[k] += g[k];
+    px[k] += g[k];
+  }
+  const int strand = (int)s.seg[8 * (size_t)prim + 7];
+  for (int ch = 0; ch < 3; ++ch) acc.sigma[3 * (size_t)strand + ch] += g[3 + ch];
+}
+
+// pathTrace (optix_common.cuh:362-409) with the adjoint records.
+Vec<double> path_adjoint(const Ctx<double>& c, const AdjFd& a, Inter<double> si, Lcg& rng, int v1, int v2,
+                         bool replay, Vec<double> Ltot, Vec<double> adj, AdjAcc& acc, double* px) {
+  const Vec<double> zero = mk(0.0, 0.0, 0.0);
+  if (v2 < v1) return zero;
+  Vec<double> beta = mk(1.0, 1.0, 1.0), col = zero;
+  auto dl_at = [&](const Inter<double>& v) {
+    DlRec<double> r1, r2;
+    const Vec<double> out = direct_lighting(c, v, rng, &r1, &r2);
+    if (replay && !v.surface) {
+      const Vec<double> W = adj * beta;
+      const double hh = dot(v.to_local[1], v.n);
+      double g[6] = {0, 0, 0, 0, 0, 0};
+      if (r1.valid) {
+        const HairQuery<double> q{v.wo_local, r1.wi_local, hh, v.color};
+        fd_grad(a, q, W, [&](Vec<double> f, double pdf) {
+          const double w = r1.mis ? pw(r1.light_pdf, pdf) : 1.0;
+          return (w / r1.light_pdf) * (f * r1.emit);
+        }, g);
+      }
+      if (r2.valid) {
+        const HairQuery<double> q{v.wo_local, r2.wi_local, hh, v.color};
+        const double pb0 = r2.bsdf_pdf;
+        fd_grad(a, q, W, [&](Vec<double> f, double pdf) { return (pw(pdf, r2.light_pdf) / pb0) * (f * r2.emit); }, g);
+      }
+      if (r1.valid || r2.valid) add_vertex_grad(c.s, v.prim, g, acc, px);
+    }
+    return out;
+  };
+  if (v1 == 0) col = dl_at(si);
+  for (int b = 1; b <= v2; ++b) {
+    NvRec<double> nv;
+    const Vec<double> mul = next_vertex(c, si, rng, &nv);
+    if (replay && nv.hair && !nv.mul_bad) {
+      const Vec<double> rem = Ltot - col;
+      const Vec<double> f0 = nv.f;
+      double g[6] = {0, 0, 0, 0, 0, 0};
+      fd_grad(a, nv.q, adj, [&](Vec<double> f, double) {
+        return mk(f0.x != 0 ? rem.x * f.x / f0.x : 0.0, f0.y != 0 ? rem.y * f.y / f0.y : 0.0,
+                  f0.z != 0 ? rem.z * f.z / f0.z : 0.0);
+      }, g);
+      add_vertex_grad(c.s, nv.prim, g, acc, px);
+    }
+    beta = beta * mul;
+    if (!si.hit) break;
+    if (b >= v1) col = col + beta * dl_at(si);
+    const double q = std::fmax((double)0.05f, 1.0 - lum(beta));
+    const float eps = rng.next();
+    if ((double)eps < q) break;
+    beta = beta / (1.0 - q);
+  }
+  return col;
+}
+
+}  // namespace
+
+void adjoint_render(const SceneDesc& s, const Bvh& bvh, int spp, const float* dl_dimg, int num_strands, int threads,
+                    double fd_h, double* grad_sigma, double* grad_glob, double* grad_px) {
+  Ctx<double> c{s, bvh, {}, {}};
+  c.hp = HairParams<double>{(double)s.beta_m, (double)s.beta_n, (double)s.alpha, (double)s.gains[0],
+                            (double)s.gains[1], (double)s.gains[2], (double)s.gains[3]};
+  c.hs = setup_hair(c.hp);
+  AdjFd a;
+  a.h = fd_h;
+  for (int j = 0; j < 7; ++j) {
+    a.hp[j] = c.hp;
+    if (j > 0) {
+      double* t = j <= 2 ? &a.hp[j].beta_m : j <= 4 ? &a.hp[j].beta_n : &a.hp[j].alpha;
+      *t += (j % 2 == 0) ? fd_h : -fd_h;
+    }
+    a.hs[j] = setup_hair(a.hp[j]);
+  }
+  const Vec<double> cp = mk((double)s.cam[0], (double)s.cam[1], (double)s.cam[2]);
+  const Vec<double> d00 = mk((double)s.cam[3], (double)s.cam[4], (double)s.cam[5]);
+  const Vec<double> du = mk((double)s.cam[6], (double)s.cam[7], (double)s.cam[8]);
+  const Vec<double> dv = mk((double)s.cam[9], (double)s.cam[10], (double)s.cam[11]);
+  const int v1 = s.path_v1 - 1, v2 = s.path_v2 - 1;
+  const int W = s.width, H = s.height;
+  for (int k = 0; k < 3; ++k) grad_glob[k] = 0;
+  std::fill(grad_sigma, grad_sigma + 3 * (size_t)num_strands, 0.0);
+  if (grad_px) std::fill(grad_px, grad_px + 3 * (size_t)W * H, 0.0);
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel
+  {
+    AdjAcc acc;
+    acc.sigma.assign(3 * (size_t)num_strands, 0.0);
+#pragma omp for schedule(dynamic, 1)
+    for (int py = 0; py < H; ++py) {
+      for (int px = 0; px < W; ++px) {
+        const float* g4 = dl_dimg + 4 * ((size_t)py * W + px);
+        const Vec<double> adj = (1.0 / spp) * mk((double)g4[0], (double)g4[1], (double)g4[2]);
+        if (adj.x == 0 && adj.y == 0 && adj.z == 0) continue;
+        double pxg[3] = {0, 0, 0};
+        for (int k = 0; k < spp; ++k) {
+          Vec<double> Lt = mk(0.0, 0.0, 0.0);
+          for (int pass = 0; pass < 2; ++pass) {
+            Lcg rng{get_rng((uint32_t)(k + 10007), (uint32_t)px, (uint32_t)py, (uint32_t)W)};
+            const float jx = rng.next(), jy = rng.next();
+            const float sx = ((float)px + jx) / (float)W, sy = ((float)py + jy) / (float)H;
+            const Vec<double> d = nrm(d00 + (double)sx * du + (double)sy * dv);
+            Inter<double> si;
+            si.wi = d;
+            trace(c, cp, d, si);
+            if (!si.hit) break;
+            if (pass == 0) {
+              Lt = path_adjoint(c, a, si, rng, v1, v2, false, Lt, adj, acc, pxg);
+              if (has_nan(Lt)) break;
+            } else {
+              path_adjoint(c, a, si, rng, v1, v2, true, Lt, adj, acc, pxg);
+            }
+          }
+        }
+        if (grad_px)
+          for (int k = 0; k < 3; ++k) grad_px[3 * ((size_t)py * W + px) + k] = pxg[k];
+      }
+    }
+#pragma omp critical
+    {
+      for (int k = 0; k < 3; ++k) grad_glob[k] += acc.glob[k];
+      for (size_t i = 0; i < acc.sigma.size(); ++i) grad_sigma[i] += acc.sigma[i];
     }
   }
 }

@@ -86,4 +86,11 @@ template <class R>
 void render_pixels(const SceneDesc& s, const Bvh& bvh, int mode, int kbeta, int accum_id0, int spp, int x0,
                    int y0, int w, int h, int threads, double* out_full, double* out_short, uint8_t* out_hit);
 
+// Path-replay adjoint of pathTrace in FP64 (BSDF derivatives by central
+// differences of step fd_h): per-strand d/d sigma_a, d/d (beta_m, beta_n,
+// alpha) and each pixel's share of the latter, for dL/d(image) dl_dimg
+// (W x H x 4) of the mean of passes 0..spp-1.
+void adjoint_render(const SceneDesc& s, const Bvh& bvh, int spp, const float* dl_dimg, int num_strands, int threads,
+                    double fd_h, double* grad_sigma, double* grad_glob, double* grad_px);
+
 }  // namespace orc

