@@ -214,6 +214,32 @@ class Scene:
         s = np.ascontiguousarray(sigma, dtype=np.float32)
         check(self.lib.hmsnn_set_strand_sigma_a(self.h, _ptr(s), s.shape[0]))
 
+    def set_hair_params(self, beta_m: float, beta_n: float, alpha: float) -> None:
+        """Global hair roughness and scale tilt (alpha in radians)."""
+        check(self.lib.hmsnn_set_hair_params(self.h, float(beta_m), float(beta_n), float(alpha)))
+        self.info = self._info()
+
+    @property
+    def num_strands(self) -> int:
+        return int(self.info["num_strands"])
+
+    def adjoint(self, dl_dimage: np.ndarray, spp: int, per_pixel: bool = False):
+        """Path-replay gradient of a loss of the spp-pass mean image given
+        dL/d(image) (H x W x 3 or 4): returns (d/d sigma_a per strand
+        (num_strands x 3), d/d (beta_m, beta_n, alpha), and optionally each
+        pixel's share of the latter (H x W x 3))."""
+        H, W = self.height, self.width
+        g = np.zeros((H, W, 4), np.float32)
+        d = np.asarray(dl_dimage, np.float32)
+        g[..., : d.shape[-1]] = d[..., :4]
+        gs = np.zeros((self.num_strands, 3))
+        gg = (C.c_double * 3)()
+        gp = np.zeros((H, W, 3), np.float32) if per_pixel else None
+        check(self.lib.hmsnn_adjoint_render(self.h, int(spp), _ptr(g), _ptr(gs), gg,
+                                            _ptr(gp) if per_pixel else None))
+        out = (gs, np.array(list(gg)))
+        return out + (gp,) if per_pixel else out
+
     # ---- path tracing ----
     def pt_reset(self) -> None:
         check(self.lib.hmsnn_pt_reset(self.h))

@@ -17,6 +17,16 @@
 // and pre-activation gradients written feature-major in FP16), a weight-
 // gradient GEMM over K = queries on the MFMA cores, and a reduce + Adam
 // kernel in FP32.
+//
+// FP8 inference (E4M3, v_mfma_scale_f32_32x32x64_f8f6f4): one instruction
+// contracts a whole 64-wide layer. Lane (g, n) holds the 32 activations of
+// its 32 output features as 32 FP8 bytes in the same register order, so the
+// chaining trick above carries over with byte e <-> feature feat_of(e/8, g,
+// e%8). Scales are per layer and powers of two: weights by their amax, layer
+// inputs by the amax of a calibration pass (BF16 forward over the first
+// queries of the batch after each weight update, with a 2x margin); both are
+// handed to the instruction as E8M0 block scales, so the FP32 accumulators
+// come out dequantised. Training in this mode runs in BF16.
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
@@ -74,6 +84,30 @@ struct MfmaTraits<__bf16> {
   static __device__ __forceinline__ __bf16 cvt(float x) { return (__bf16)x; }
   static __device__ __forceinline__ float back(__bf16 x) { return (float)x; }
 };
+
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+
+// FP8 fragments: hidden [l 0..4][ob 0..1][lane 64], output [lane 64]; one
+// i32x8 (32 E4M3 bytes) each.
+constexpr int kFrag8Hidden = 5 * 2 * 64;
+constexpr int kFrag8Total = kFrag8Hidden + 64;
+constexpr float kE4M3Max = 448.f;
+
+// Power-of-two exponent e with amax / 2^e < 448 (0 for an all-zero layer).
+__host__ __device__ inline int fp8_exponent(float amax) {
+  if (!(amax > 0.f) || !(amax < 3.0e38f)) return 0;
+  int e;
+  (void)frexpf(amax / kE4M3Max, &e);
+  return e < -100 ? -100 : (e > 100 ? 100 : e);
+}
+
+__device__ __forceinline__ float fp8_clamp(float x) { return fminf(fmaxf(x, -kE4M3Max), kE4M3Max); }
+
+// Four floats -> four E4M3 bytes (round to nearest even) in one word.
+__device__ __forceinline__ int fp8_pack4(float a, float b, float c, float d) {
+  const int w = __builtin_amdgcn_cvt_pk_fp8_f32(fp8_clamp(a), fp8_clamp(b), 0, false);
+  return __builtin_amdgcn_cvt_pk_fp8_f32(fp8_clamp(c), fp8_clamp(d), w, true);
+}
 
 // Builds the permuted fragments from the FP32 master weights.
 template <class T>
@@ -216,6 +250,170 @@ __global__ void __launch_bounds__(kMlpBlock) k_mlp_infer(const T* frag, const fl
 #pragma unroll 1
     for (int l = 0; l < 5; ++l) hidden_layer<T>(lds, l, lane, B);
     const f32x16 z = output_layer<T>(lds, lane, B);
+    if (g == 0 && q < n) {
+      out[(size_t)q * 3 + 0] = expf(z[0]);
+      out[(size_t)q * 3 + 1] = expf(z[1]);
+      out[(size_t)q * 3 + 2] = expf(z[2]);
+    }
+  }
+}
+
+// ---- FP8 inference ----------------------------------------------------------
+
+// Per-layer weight amax -> scales[0..5] (exponents). One block per layer.
+__global__ void k_mlp_w_scales(const float* w, int* scales) {
+  __shared__ float red[256];
+  const int l = blockIdx.x;
+  const int cnt = l < 5 ? 4096 : 3 * 64;
+  float a = 0.f;
+  for (int i = threadIdx.x; i < cnt; i += blockDim.x) a = fmaxf(a, fabsf(w[l * 4096 + i]));
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) scales[l] = fp8_exponent(red[0]);
+}
+
+// FP8 fragments: byte e of lane (g, m) holds W[32 ob + m][feat_of(e / 8, g, e % 8)] / 2^scales[l].
+__global__ void k_mlp_permute_fp8(const float* w, const int* scales, i32x8* frag) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= kFrag8Total) return;
+  const bool out_layer = f >= kFrag8Hidden;
+  const int lane = f & 63;
+  const int ob = out_layer ? 0 : (f >> 6) & 1;
+  const int l = out_layer ? 5 : f >> 7;
+  const int g = lane >> 5, m = lane & 31;
+  const float sc = ldexpf(1.f, -scales[l]);
+  float v[32];
+#pragma unroll
+  for (int e = 0; e < 32; ++e) {
+    const int i = feat_of(e >> 3, g, e & 7);
+    v[e] = out_layer ? (m < 3 ? w[5 * 4096 + m * 64 + i] * sc : 0.f) : w[l * 4096 + (32 * ob + m) * 64 + i] * sc;
+  }
+  i32x8 r;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r[k] = fp8_pack4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+  frag[f] = r;
+}
+
+// Calibration: amax of every layer's input (the encoding, then each hidden
+// ReLU output) over the valid queries, from the BF16 / FP16 network.
+template <class T>
+__global__ void __launch_bounds__(kMlpBlock) k_mlp_act_amax(const T* frag, const float* in, int n, unsigned* amax) {
+  typedef typename MfmaTraits<T>::vec V;
+  __shared__ V lds[kFragFwdTotal];
+  stage_fragments<T>(frag, lds, kFragFwdTotal);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 5, col = lane & 31;
+  const int waves = gridDim.x * (kMlpBlock / 64);
+  float mx[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int tile = blockIdx.x * (kMlpBlock / 64) + wave; tile * 32 < n; tile += waves) {
+    const int q = tile * 32 + col;
+    const bool valid = q < n;
+    float rec[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) rec[k] = valid ? in[(size_t)q * 12 + k] : 0.f;
+    if (!valid) rec[5] = rec[8] = 1.f;
+    const Encoded e = encode(rec);
+    if (valid)
+#pragma unroll
+      for (int k = 0; k < 44; ++k) mx[0] = fmaxf(mx[0], fabsf(e.f[k]));
+    V B[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) B[s] = encoded_operand<T>(e, s, g);
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+      f32x16 acc[2];
+      acc[0] = f32x16{};
+      acc[1] = f32x16{};
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int ob = 0; ob < 2; ++ob)
+          acc[ob] = MfmaTraits<T>::mfma(lds[((l * 2 + ob) * 4 + s) * 64 + lane], B[s], acc[ob]);
+      float m = 0.f;
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+          const float x = fmaxf(acc[s >> 1][8 * (s & 1) + v], 0.f);
+          m = fmaxf(m, x);
+          B[s][v] = MfmaTraits<T>::cvt(x);
+        }
+      if (valid) mx[l + 1] = fmaxf(mx[l + 1], m);
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < 6; ++l) {
+    float v = mx[l];
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+    if (lane == 0) atomicMax(&amax[l], __float_as_uint(v));  // non-negative floats order as integers
+  }
+}
+
+// Layer-input exponents scales[6..11] from the calibration amax, 2x margin.
+__global__ void k_mlp_act_scales(const unsigned* amax, int* scales) {
+  const int l = threadIdx.x;
+  if (l < 6) scales[6 + l] = fp8_exponent(2.f * __uint_as_float(amax[l]));
+}
+
+__device__ __forceinline__ f32x16 mfma_fp8(const i32x8& a, const i32x8& b, f32x16 c, int ea, int eb) {
+  return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, 127 + ea, 0, 127 + eb);
+}
+
+// scales: [0..5] weight exponents, [6..11] layer-input exponents.
+__global__ void __launch_bounds__(kMlpBlock) k_mlp_infer_fp8(const i32x8* frag, const int* scales, const float* in,
+                                                             int n, float* out) {
+  __shared__ i32x8 lds[kFrag8Total];
+  for (int i = threadIdx.x; i < kFrag8Total; i += blockDim.x) lds[i] = frag[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 5, col = lane & 31;
+  const int waves = gridDim.x * (kMlpBlock / 64);
+  for (int tile = blockIdx.x * (kMlpBlock / 64) + wave; tile * 32 < n; tile += waves) {
+    const int q = tile * 32 + col;
+    float rec[12];
+    if (q < n) {
+#pragma unroll
+      for (int k = 0; k < 12; ++k) rec[k] = in[(size_t)q * 12 + k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 12; ++k) rec[k] = 0.f;
+      rec[5] = 1.f;
+      rec[8] = 1.f;
+    }
+    const Encoded e = encode(rec);
+    i32x8 B;
+    {
+      const float sc = ldexpf(1.f, -scales[6]);
+      float x[32];
+#pragma unroll
+      for (int b = 0; b < 32; ++b) {
+        const int f0 = feat_of(b >> 3, 0, b & 7), f1 = feat_of(b >> 3, 1, b & 7);
+        const float a0 = f0 < 44 ? e.f[f0] : 0.f, a1 = f1 < 44 ? e.f[f1] : 0.f;
+        x[b] = (g ? a1 : a0) * sc;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) B[k] = fp8_pack4(x[4 * k], x[4 * k + 1], x[4 * k + 2], x[4 * k + 3]);
+    }
+#pragma unroll 1
+    for (int l = 0; l < 5; ++l) {
+      const int ew = scales[l], ex = scales[6 + l];
+      const f32x16 a0 = mfma_fp8(lds[(l * 2 + 0) * 64 + lane], B, f32x16{}, ew, ex);
+      const f32x16 a1 = mfma_fp8(lds[(l * 2 + 1) * 64 + lane], B, f32x16{}, ew, ex);
+      const float sc = ldexpf(1.f, -scales[7 + l]);
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        B[k] = fp8_pack4(fmaxf(a0[4 * k], 0.f) * sc, fmaxf(a0[4 * k + 1], 0.f) * sc, fmaxf(a0[4 * k + 2], 0.f) * sc,
+                         fmaxf(a0[4 * k + 3], 0.f) * sc);
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        B[4 + k] = fp8_pack4(fmaxf(a1[4 * k], 0.f) * sc, fmaxf(a1[4 * k + 1], 0.f) * sc,
+                             fmaxf(a1[4 * k + 2], 0.f) * sc, fmaxf(a1[4 * k + 3], 0.f) * sc);
+    }
+    const f32x16 z = mfma_fp8(lds[kFrag8Hidden + lane], B, f32x16{}, scales[5], scales[11]);
     if (g == 0 && q < n) {
       out[(size_t)q * 3 + 0] = expf(z[0]);
       out[(size_t)q * 3 + 1] = expf(z[1]);
@@ -407,7 +605,17 @@ struct MlpState {
   int cap = 0;  // padded batch capacity of H / dZ
   int step = 0;
   hipStream_t stream = nullptr;
+  // FP8 inference state
+  void* frag8 = nullptr;
+  int* scales = nullptr;      // [0..5] weight exponents, [6..11] layer-input exponents
+  unsigned* amax = nullptr;   // calibration amax per layer input
+  bool calib_dirty = true;    // weights changed since the last calibration
+  bool fp8() const { return precision == HMSNN_FP8; }
+  bool bf16() const { return precision == HMSNN_BF16 || precision == HMSNN_FP8; }
   ~MlpState() {
+    dev_free(frag8);
+    dev_free(scales);
+    dev_free(amax);
     dev_free(w);
     dev_free(m);
     dev_free(v);
@@ -422,17 +630,25 @@ struct MlpState {
 
 static int mlp_refresh(MlpState& s, hipStream_t st, std::string& err) {
   const int total = kFragAll * 8;
-  if (s.precision == HMSNN_BF16)
+  if (s.bf16())
     hipLaunchKernelGGL(k_mlp_permute<__bf16>, dim3((total + 255) / 256), dim3(256), 0, st, s.w, (__bf16*)s.frag);
   else
     hipLaunchKernelGGL(k_mlp_permute<_Float16>, dim3((total + 255) / 256), dim3(256), 0, st, s.w,
                        (_Float16*)s.frag);
-  return check_launch("mlp permute", err);
+  HM_TRY(check_launch("mlp permute", err));
+  if (s.fp8()) {
+    hipLaunchKernelGGL(k_mlp_w_scales, dim3(6), dim3(256), 0, st, s.w, s.scales);
+    hipLaunchKernelGGL(k_mlp_permute_fp8, dim3((kFrag8Total + 127) / 128), dim3(128), 0, st, s.w, s.scales,
+                       (i32x8*)s.frag8);
+    HM_TRY(check_launch("mlp permute fp8", err));
+    s.calib_dirty = true;
+  }
+  return kOk;
 }
 
 int mlp_create(int precision, uint64_t seed, MlpState** out, std::string& err) {
-  if (precision != HMSNN_FP16 && precision != HMSNN_BF16) {
-    err = "MLP precision must be FP16 or BF16";
+  if (precision != HMSNN_FP16 && precision != HMSNN_BF16 && precision != HMSNN_FP8) {
+    err = "MLP precision must be FP16, BF16 or FP8";
     return kEUnsupported;
   }
   MlpState* s = new MlpState();
@@ -442,6 +658,9 @@ int mlp_create(int precision, uint64_t seed, MlpState** out, std::string& err) {
   if (!rc) rc = dev_alloc((void**)&s->v, sizeof(float) * kMlpWeightCount, err);
   if (!rc) rc = dev_alloc(&s->frag, s->esize() * kFragAll * 8, err);
   if (!rc) rc = dev_alloc((void**)&s->loss_sum, sizeof(float) * 4, err);
+  if (!rc && s->fp8()) rc = dev_alloc(&s->frag8, sizeof(i32x8) * kFrag8Total, err);
+  if (!rc && s->fp8()) rc = dev_alloc((void**)&s->scales, sizeof(int) * 12, err);
+  if (!rc && s->fp8()) rc = dev_alloc((void**)&s->amax, sizeof(unsigned) * 6, err);
   if (rc) {
     delete s;
     return rc;
@@ -491,8 +710,25 @@ static int mlp_blocks(int n) {
   return std::max(1, std::min(1024, (tiles + 3) / 4));
 }
 
+// Queries of the batch used to calibrate the FP8 activation scales.
+constexpr int kFp8CalibQueries = 1 << 16;
+
 int mlp_forward(MlpState& s, const float* in, int n, float* out, hipStream_t st, std::string& err) {
   if (n <= 0) return kOk;
+  if (s.fp8()) {
+    if (s.calib_dirty) {
+      const int nc = std::min(n, kFp8CalibQueries);
+      HM_HIP(hipMemsetAsync(s.amax, 0, sizeof(unsigned) * 6, st), err);
+      hipLaunchKernelGGL(k_mlp_act_amax<__bf16>, dim3(mlp_blocks(nc)), dim3(kMlpBlock), 0, st, (const __bf16*)s.frag,
+                         in, nc, s.amax);
+      hipLaunchKernelGGL(k_mlp_act_scales, dim3(1), dim3(64), 0, st, s.amax, s.scales);
+      HM_TRY(check_launch("mlp fp8 calibration", err));
+      s.calib_dirty = false;
+    }
+    hipLaunchKernelGGL(k_mlp_infer_fp8, dim3(mlp_blocks(n)), dim3(kMlpBlock), 0, st, (const i32x8*)s.frag8, s.scales,
+                       in, n, out);
+    return check_launch("mlp infer fp8", err);
+  }
   if (s.precision == HMSNN_BF16)
     hipLaunchKernelGGL(k_mlp_infer<__bf16>, dim3(mlp_blocks(n)), dim3(kMlpBlock), 0, st, (const __bf16*)s.frag, in,
                        n, out);
@@ -543,7 +779,7 @@ int mlp_loss_grad_step(MlpState& s, const float* in, const float* target, int n,
     attr_set = true;
   }
   const int blocks = npad / 128 + (npad % 128 ? 1 : 0);
-  if (s.precision == HMSNN_BF16) {
+  if (s.bf16()) {
     hipLaunchKernelGGL(k_mlp_fwd_bwd<__bf16>, dim3(blocks), dim3(kMlpBlock), smem, st, (const __bf16*)s.frag, in,
                        target, n, npad, loss, grad_scale, (__bf16*)s.H, (__bf16*)s.dZ, s.loss_sum);
     HM_TRY(check_launch("mlp fwd/bwd", err));

@@ -187,7 +187,7 @@ int msnn_setup(Context& c, int BETA, std::string& err) {
             : s.msnn_loss == "L2"        ? HMSNN_LOSS_L2
                                          : HMSNN_LOSS_RELATIVE_L2_LUMINANCE;
   m->rng_state = s.msnn_seed * 0x2545F4914F6CDD1Dull + 1;
-  const int prec = s.msnn_precision == "bf16" ? HMSNN_BF16 : HMSNN_FP16;
+  const int prec = s.msnn_precision == "bf16" ? HMSNN_BF16 : s.msnn_precision == "fp8" ? HMSNN_FP8 : HMSNN_FP16;
   HM_TRY(mlp_create(prec, s.msnn_seed, &m->mlp, err));
   HM_TRY(dev_alloc((void**)&m->nn_input, sizeof(float) * 12 * (size_t)npx, err));
   HM_TRY(dev_alloc((void**)&m->hit_flags, (size_t)npx, err));

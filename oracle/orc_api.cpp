@@ -92,6 +92,9 @@ struct orc_scene {
   float sigma_a[3];
   const float* strand_sigma;
   int mis, path_v1, path_v2;
+  int frozen;
+  float beta_m0, beta_n0, alpha0;
+  const float* strand_sigma0;
 };
 
 namespace {
@@ -133,6 +136,11 @@ SceneDesc scene_desc(const orc_scene* sc, EnvTables& env) {
   s.mis = sc->mis;
   s.path_v1 = sc->path_v1;
   s.path_v2 = sc->path_v2;
+  s.frozen = sc->frozen;
+  s.beta_m0 = sc->beta_m0;
+  s.beta_n0 = sc->beta_n0;
+  s.alpha0 = sc->alpha0;
+  s.strand_sigma0 = sc->strand_sigma0;
   return s;
 }
 }  // namespace

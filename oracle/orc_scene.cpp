@@ -282,6 +282,7 @@ struct Inter {
   R alpha = 0;
   Vec<R> Le;
   int prim = -1;  // segment id, -2 for the head sphere
+  Vec<R> color0;  // sigma_a used for sampling (frozen mode), else = color
 };
 
 template <class R>
@@ -290,6 +291,9 @@ struct Ctx {
   const Bvh& bvh;
   HairParams<R> hp;
   HairShading<R> hs;
+  bool frozen = false;  // sampling uses hp0 / hs0 / color0
+  HairParams<R> hp0;
+  HairShading<R> hs0;
 };
 
 template <class R> inline Vec<R> apply_mat(const Vec<R> m[3], Vec<R> v) { return mk(dot(m[0], v), dot(m[1], v), dot(m[2], v)); }
@@ -431,6 +435,7 @@ void setup_hit(const Ctx<R>& c, Vec<R> o, Vec<R> d, float t, int prim, float u, 
     si.to_local[2] = n;
     si.wo_local = nrm(apply_mat(si.to_local, si.wo));
     si.color = mk((R)s.sphere_albedo[0], (R)s.sphere_albedo[1], (R)s.sphere_albedo[2]);
+    si.color0 = si.color;
     si.alpha = (R)s.sphere_alpha;
     si.surface = true;
     si.radius = 0;
@@ -472,6 +477,11 @@ void setup_hit(const Ctx<R>& c, Vec<R> o, Vec<R> d, float t, int prim, float u, 
   } else {
     si.color = mk((R)s.sigma_a[0], (R)s.sigma_a[1], (R)s.sigma_a[2]);
   }
+  si.color0 = si.color;
+  if (c.frozen && s.strand_sigma0) {
+    const int strand = (int)q[7];
+    si.color0 = mk((R)s.strand_sigma0[3 * strand], (R)s.strand_sigma0[3 * strand + 1], (R)s.strand_sigma0[3 * strand + 2]);
+  }
   si.surface = false;
 }
 
@@ -486,21 +496,39 @@ Vec<R> bsdf_eval(const Ctx<R>& c, const Inter<R>& si, R* pdf) {
 }
 
 // sample_disney_hair / sample_GGX with the draw order of
-// optix_common.cuh:154-170 and :314-329.
+// optix_common.cuh:154-170 and :314-329. Returns f; *pdf is the sampling
+// pdf. In frozen mode the direction and *pdf come from the sampling
+// parameters, *pdf_eval is the pdf under the evaluation parameters and *f0
+// the BSDF value under the sampling ones; otherwise all coincide.
 template <class R>
-Vec<R> bsdf_sample(const Ctx<R>& c, Inter<R>& si, Lcg& rng, R* pdf) {
+Vec<R> bsdf_sample(const Ctx<R>& c, Inter<R>& si, Lcg& rng, R* pdf, R* pdf_eval = nullptr, Vec<R>* f0 = nullptr) {
   const float r0 = rng.next(), r1 = rng.next();
   if (!si.surface) {
     const float r2 = rng.next(), r3 = rng.next();
     int lobe;
-    si.wi_local = hair_sample(c.hs, si.wo_local, dot(si.to_local[1], si.n), si.color, (R)r0, (R)r1, (R)r2, (R)r3, &lobe);
+    const R hh = dot(si.to_local[1], si.n);
+    si.wi_local = hair_sample(c.frozen ? c.hs0 : c.hs, si.wo_local, hh, c.frozen ? si.color0 : si.color, (R)r0, (R)r1,
+                              (R)r2, (R)r3, &lobe);
     si.wi = nrm(apply_T(si.to_local, si.wi_local));
-    HairQuery<R> q{si.wo_local, si.wi_local, dot(si.to_local[1], si.n), si.color};
-    return hair_eval(c.hp, c.hs, q, pdf);
+    HairQuery<R> q{si.wo_local, si.wi_local, hh, si.color};
+    R pe;
+    const Vec<R> f = hair_eval(c.hp, c.hs, q, &pe);
+    *pdf = pe;
+    if (pdf_eval) *pdf_eval = pe;
+    if (f0) *f0 = f;
+    if (c.frozen) {
+      HairQuery<R> q0{si.wo_local, si.wi_local, hh, si.color0};
+      const Vec<R> fs = hair_eval(c.hp0, c.hs0, q0, pdf);
+      if (f0) *f0 = fs;
+    }
+    return f;
   }
   si.wi_local = ggx_sample(si.alpha, si.wo_local, (R)r0, (R)r1, pdf);
   si.wi = nrm(apply_T(si.to_local, si.wi_local));
-  return frostbite(si.wo_local, si.wi_local, si.color, si.alpha);
+  if (pdf_eval) *pdf_eval = *pdf;
+  const Vec<R> f = frostbite(si.wo_local, si.wi_local, si.color, si.alpha);
+  if (f0) *f0 = f;
+  return f;
 }
 
 template <class R>
@@ -553,8 +581,8 @@ struct DlRec {
 // sampleBSDF (optix_common.cuh:151-203); si is the direct-lighting copy.
 template <class R>
 Vec<R> sample_bsdf_light(const Ctx<R>& c, Inter<R>& si, Lcg& rng, DlRec<R>* rec = nullptr) {
-  R bsdfPdf = 1;
-  const Vec<R> bsdf = bsdf_sample(c, si, rng, &bsdfPdf);
+  R bsdfPdf = 1, pdfEval = 1;
+  const Vec<R> bsdf = bsdf_sample(c, si, rng, &bsdfPdf, &pdfEval);
   const bool lower = dot(si.wi, si.n) < 0;
   Vec<R> nd = si.n;
   if (si.surface && lower) return mk((R)0, (R)0, (R)0);
@@ -566,7 +594,7 @@ Vec<R> sample_bsdf_light(const Ctx<R>& c, Inter<R>& si, Lcg& rng, DlRec<R>* rec 
   const Vec<R> emit = env_radiance<R>(c.s, si.wi);
   const R lightPdf = env_pdf<R>(c.s, si.wi) * (R)1 / (R)(c.s.ndl + (c.s.has_env ? 1 : 0));
   if (rec) *rec = DlRec<R>{true, c.s.mis != 0, si.wi_local, emit, lightPdf, bsdfPdf};
-  const R f2 = bsdfPdf * bsdfPdf, g2 = lightPdf * lightPdf;
+  const R f2 = pdfEval * pdfEval, g2 = lightPdf * lightPdf;
   if (c.s.mis) return (f2 / (f2 + g2) / bsdfPdf) * (bsdf * emit);
   return (bsdf * emit) / bsdfPdf;
 }
@@ -643,12 +671,18 @@ struct NvRec {
   Vec<R> f;
 };
 
-// nextPathVertex (optix_common.cuh:311-360); returns mulFac.
+// nextPathVertex (optix_common.cuh:311-360); returns mulFac, and in *mul0
+// the factor under the sampling parameters (frozen mode; else the same).
 template <class R>
-Vec<R> next_vertex(const Ctx<R>& c, Inter<R>& si, Lcg& rng, NvRec<R>* rec = nullptr) {
+Vec<R> next_vertex(const Ctx<R>& c, Inter<R>& si, Lcg& rng, NvRec<R>* rec = nullptr, Vec<R>* mul0 = nullptr) {
   R pdf = 1;
-  const Vec<R> f = bsdf_sample(c, si, rng, &pdf);
+  Vec<R> fs;
+  const Vec<R> f = bsdf_sample(c, si, rng, &pdf, (R*)nullptr, &fs);
   Vec<R> mul = pdf == 0 ? f : f / pdf;
+  if (mul0) {
+    *mul0 = pdf == 0 ? fs : fs / pdf;
+    if (has_nan(*mul0)) *mul0 = mk((R)1, (R)1, (R)1);
+  }
   if (rec) {
     rec->hair = !si.surface;
     rec->mul_bad = has_nan(mul);
@@ -674,19 +708,24 @@ Vec<R> next_vertex(const Ctx<R>& c, Inter<R>& si, Lcg& rng, NvRec<R>* rec = null
 template <class R> inline R lum(Vec<R> v) { return (R)0.212671f * v.x + (R)0.715160f * v.y + (R)0.072169f * v.z; }
 
 // pathTrace (optix_common.cuh:362-409)
+// In frozen mode the roulette follows the throughput beta0 under the
+// sampling parameters; otherwise beta0 == beta.
 template <class R>
 Vec<R> path_trace(const Ctx<R>& c, Inter<R> si, Lcg& rng, int v1, int v2) {
   if (v2 < v1) return mk((R)0, (R)0, (R)0);
-  Vec<R> beta = mk((R)1, (R)1, (R)1), col = mk((R)0, (R)0, (R)0);
+  Vec<R> beta = mk((R)1, (R)1, (R)1), beta0 = beta, col = mk((R)0, (R)0, (R)0);
   if (v1 == 0) col = direct_lighting(c, si, rng);
   for (int b = 1; b <= v2; ++b) {
-    beta = beta * next_vertex(c, si, rng);
+    Vec<R> mul0;
+    beta = beta * next_vertex(c, si, rng, (NvRec<R>*)nullptr, &mul0);
+    beta0 = beta0 * mul0;
     if (!si.hit) break;
     if (b >= v1) col = col + beta * direct_lighting(c, si, rng);
-    const R q = std::fmax((R)0.05f, (R)1 - lum(beta));
+    const R q = std::fmax((R)0.05f, (R)1 - lum(beta0));
     const float eps = rng.next();
     if ((R)eps < q) break;
     beta = beta / ((R)1 - q);
+    beta0 = beta0 / ((R)1 - q);
   }
   return col;
 }
@@ -739,6 +778,14 @@ void render_pixels(const SceneDesc& s, const Bvh& bvh, int mode, int kbeta, int 
   Ctx<R> c{s, bvh, {}, {}};
   c.hp = HairParams<R>{(R)s.beta_m, (R)s.beta_n, (R)s.alpha, (R)s.gains[0], (R)s.gains[1], (R)s.gains[2], (R)s.gains[3]};
   c.hs = setup_hair(c.hp);
+  c.frozen = s.frozen != 0 && mode == 0;
+  c.hp0 = c.hp;
+  if (c.frozen) {
+    c.hp0.beta_m = (R)s.beta_m0;
+    c.hp0.beta_n = (R)s.beta_n0;
+    c.hp0.alpha = (R)s.alpha0;
+  }
+  c.hs0 = setup_hair(c.hp0);
   const Vec<R> cp = mk((R)s.cam[0], (R)s.cam[1], (R)s.cam[2]);
   const Vec<R> d00 = mk((R)s.cam[3], (R)s.cam[4], (R)s.cam[5]);
   const Vec<R> du = mk((R)s.cam[6], (R)s.cam[7], (R)s.cam[8]);

@@ -78,6 +78,13 @@ struct SceneDesc {
   const float* strand_sigma;  // optional, 3 per strand
   int mis;
   int path_v1, path_v2;
+  // Frozen sampling, a test aid for the adjoint: when set, directions, the
+  // sampling pdfs in the estimators' denominators and Russian roulette use
+  // (beta_m0, beta_n0, alpha0, strand_sigma0 or sigma_a), while f and the MIS
+  // weights use the parameters above. Only pathTrace (mode 0) honours it.
+  int frozen;
+  float beta_m0, beta_n0, alpha0;
+  const float* strand_sigma0;
 };
 
 // mode 0: pathTrace; mode 1: msnnPathTrace truncated at kernel beta;

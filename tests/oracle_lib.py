@@ -29,6 +29,7 @@ class OrcScene(C.Structure):
         ("m_cdf", _P), ("ndl", _I), ("dl_from", (_F * 3) * 8), ("dl_emit", (_F * 3) * 8), ("cam", _F * 12),
         ("width", _I), ("height", _I), ("beta_m", _F), ("beta_n", _F), ("alpha", _F), ("gains", _F * 4),
         ("sigma_a", _F * 3), ("strand_sigma", _P), ("mis", _I), ("path_v1", _I), ("path_v2", _I),
+        ("frozen", _I), ("beta_m0", _F), ("beta_n0", _F), ("alpha0", _F), ("strand_sigma0", _P),
     ]
 
 
@@ -46,6 +47,7 @@ _SIGS = {
     "orc_bvh_destroy": (None, [_P]),
     "orc_render": (_I, [C.POINTER(OrcScene), _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P]),
     "orc_max_threads": (_I, []),
+    "orc_adjoint": (_I, [C.POINTER(OrcScene), _P, _I, _P, _I, _I, _D, _P, _P, _P]),
 }
 
 _lib = None
@@ -173,6 +175,20 @@ class OracleScene:
                                  ptr(full), ptr(short), ptr(hit))
         assert rc == 0
         return full, short, hit
+
+    def adjoint(self, dl_dimg: np.ndarray, spp: int, num_strands: int, fd_h: float = 1e-6, threads: int = 0):
+        """FP64 path-replay adjoint of the mean of passes 0..spp-1: per-strand
+        d/d sigma_a (num_strands x 3), d/d (beta_m, beta_n, alpha), and each
+        pixel's share of the latter (H x W x 3)."""
+        W, H = int(self.s.width), int(self.s.height)
+        g = np.ascontiguousarray(dl_dimg, np.float32).reshape(H, W, 4)
+        gs = np.zeros((num_strands, 3))
+        gg = np.zeros(3)
+        gp = np.zeros((H, W, 3))
+        rc = self.lib.orc_adjoint(C.byref(self.s), self.bvh, spp, ptr(g), num_strands, threads, fd_h, ptr(gs),
+                                  ptr(gg), ptr(gp))
+        assert rc == 0
+        return gs, gg, gp
 
 
 _SIGS_MLP = {

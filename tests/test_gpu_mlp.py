@@ -93,6 +93,30 @@ def test_forward_matches_oracle(gpu, precision, tol):
     assert np.all(np.isfinite(y))
 
 
+def test_fp8_inference_against_fp64_and_fp16(gpu):
+    """FP8-E4M3 inference (per-layer power-of-two scales, BF16 calibration)
+    on 2^20 seeded inputs against the FP64 oracle, beside FP16 on the same
+    weights."""
+    n = 1 << 20
+    r = records(n, 31)
+    w = weights(32)
+    y64 = ol.mlp_forward(w, r, 0)
+    out = {}
+    for prec in (0, 2):
+        m = Mlp(gpu, prec)
+        m.set_weights(w)
+        out[prec] = m.forward(r).astype(np.float64)
+        assert np.all(np.isfinite(out[prec]))
+    e8 = np.abs(out[2] - y64) / np.abs(y64)
+    e16 = np.abs(out[0] - y64) / np.abs(y64)
+    e8_16 = np.abs(out[2] - out[0]) / np.abs(out[0])
+    print(f"FP8 vs FP64: median {np.median(e8):.2e} p99 {np.quantile(e8, 0.99):.2e} max {e8.max():.2e}; "
+          f"FP16 vs FP64: median {np.median(e16):.2e} p99 {np.quantile(e16, 0.99):.2e}; "
+          f"FP8 vs FP16 p99 {np.quantile(e8_16, 0.99):.2e}")
+    assert np.median(e8) <= 2e-2
+    assert np.quantile(e8, 0.99) <= 1e-1
+
+
 @pytest.mark.parametrize("loss", [0, 2])
 def test_gradient_matches_oracle(gpu, loss):
     n = 1 << 14
