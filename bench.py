@@ -69,18 +69,107 @@ def cpu_baseline(sc, budget_s: float = 15.0) -> dict:
     }
 
 
+def msnn_vs_pt(sc, pt_ms_per_pass: float, ref_spp: int, frames: int, beta: int = 1) -> dict:
+    """HairMSNN at BETA vs plain path tracing: time to reach equal RMSE
+    against a ref_spp path-traced reference rendered on independent random
+    streams. PT: MSE(k) = a / k + b fitted over k = 1..frames passes (b is the
+    reference's own noise), time per pass from the timed region. HairMSNN:
+    warm-up training plus `frames` frames, wall clock; its MSE has b removed.
+    Returns the speed-up T_PT(MSE_MSNN) / T_MSNN and both curves."""
+    W, H = sc.width, sc.height
+    sc.set_seed_offset(1 << 22)
+    sc.pt_reset()
+    t0 = time.perf_counter()
+    done = 0
+    while done < ref_spp:
+        k = min(64, ref_spp - done)
+        sc.pt_passes(k)
+        done += k
+        log(f"reference: {done}/{ref_spp} passes ({time.perf_counter() - t0:.1f} s)")
+    ref = sc.image(0)[..., :3].astype(np.float64)
+    sc.set_seed_offset(0)
+
+    def mse(img):
+        return float(np.mean((img[..., :3].astype(np.float64) - ref) ** 2))
+
+    sc.pt_reset()
+    ks, pt_mse = [], []
+    for k in range(1, frames + 1):
+        sc.pt_passes(1)
+        if k & (k - 1) == 0 or k == frames:
+            ks.append(k)
+            pt_mse.append(mse(sc.image(0)))
+    ks_a, m_a = np.array(ks, float), np.array(pt_mse)
+    A = np.stack([1.0 / ks_a, np.ones_like(ks_a)], 1)
+    (a, b), *_ = np.linalg.lstsq(A, m_a, rcond=None)
+    b = max(b, 0.0)
+    # HairMSNN: warm-up + frames, timed on the wall clock (synchronous API)
+    sc.synchronize()
+    t0 = time.perf_counter()
+    sc.msnn_begin(beta)
+    t_warm = time.perf_counter() - t0
+    fr, ms_mse, ms_t = [], [], []
+    t_frames = 0.0
+    done = 0
+    for f in sorted({1, 2, 4, 8, 16, 32, 64, frames}):
+        if f > frames:
+            continue
+        t0 = time.perf_counter()
+        sc.msnn_frames(f - done)
+        sc.synchronize()
+        t_frames += time.perf_counter() - t0
+        done = f
+        fr.append(f)
+        ms_t.append(t_warm + t_frames)
+        ms_mse.append(mse(sc.msnn_image(0)))
+    m_final = max(ms_mse[-1] - b, 1e-30)
+    k_eq = a / m_final
+    t_pt = k_eq * pt_ms_per_pass * 1e-3
+    t_ms = ms_t[-1]
+    log(f"msnn vs pt: PT a={a:.4g} b={b:.3g}; MSNN {frames} frames MSE {ms_mse[-1]:.4g} in {t_ms:.2f} s "
+        f"(warm-up {t_warm:.2f} s) = PT at {k_eq:.1f} spp, {t_pt:.2f} s -> speed-up {t_pt / t_ms:.2f}x")
+    return {
+        "beta": beta, "frames": frames, "reference_spp": ref_spp,
+        "speedup": t_pt / t_ms, "t_msnn_s": t_ms, "t_warmup_s": t_warm, "t_pt_equal_rmse_s": t_pt,
+        "pt_spp_equal_rmse": k_eq, "rmse_msnn": float(np.sqrt(m_final)),
+        "pt_fit": {"a": float(a), "b_reference_noise": float(b)},
+        "pt_curve": {"spp": ks, "mse": pt_mse},
+        "msnn_curve": {"frames": fr, "t_s": ms_t, "mse": ms_mse},
+    }
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-msnn", action="store_true", help="skip the HairMSNN vs PT time-to-equal-RMSE leg")
+    ap.add_argument("--ref-spp", type=int, default=1024, help="passes of the RMSE reference image")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="instead of the benchmark, run 8 device-to-device copies of this many MiB (the known-size "
+                         "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--scene", default=str(SCENE))
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the path tracer shards by image tiles; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_mib > 0:
+        import torch
+        nbytes = args.copy_mib << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(8):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / 8
+        print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": dt * 1e3, "GBps_read_plus_write": 2 * nbytes / dt / 1e9}),
+              flush=True)
+        return
 
     import hairmsnn_amd as hm
     n, name = hm.device_info()
@@ -160,6 +249,8 @@ def main() -> None:
         },
         "rays_per_pass": {"closest": stats["closest_rays"], "any": stats["any_rays"]},
     }
+    if not args.no_msnn:
+        line["msnn_vs_pt"] = msnn_vs_pt(sc, wall / args.steps * 1e3, args.ref_spp, 64, beta=1)
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(sc)

@@ -240,6 +240,10 @@ class Scene:
         out = (gs, np.array(list(gg)))
         return out + (gp,) if per_pixel else out
 
+    def set_seed_offset(self, offset: int) -> None:
+        """Offsets the pass index behind every pixel's random stream (0 = the reference's)."""
+        check(self.lib.hmsnn_set_seed_offset(self.h, int(offset)))
+
     # ---- path tracing ----
     def pt_reset(self) -> None:
         check(self.lib.hmsnn_pt_reset(self.h))

@@ -262,7 +262,7 @@ int hmsnn_adjoint_render(hmsnn_ctx* ctx, int spp, const float* dl_dimg, double* 
   a.hcd = make_hair_consts_ad(c.scene.beta_m, c.scene.beta_n, c.scene.alpha, gains);
   a.dl_dimg = d_adj;
   a.inv_spp = 1.f / (float)spp;
-  a.pass0 = 0;
+  a.pass0 = c.seed_offset;
   a.npass = spp;
   const PassConfig pc = c.pt_pass_config();
   a.dl_zero = pc.dl_zero;

@@ -173,6 +173,7 @@ int Context::ensure_path_buffers(bool msnn_mode, std::string& err, int min_slots
 PassConfig Context::pt_pass_config() const {
   PassConfig c;
   c.accum_id = accum_id;
+  c.seed_offset = seed_offset;
   const int v1 = scene.path_v1 - 1, v2 = scene.path_v2 - 1;
   c.dl_zero = (v1 == 0);
   c.dl_from = v1;
@@ -524,6 +525,12 @@ int hmsnn_set_strand_sigma_a(hmsnn_ctx* ctx, const float* sigma, int num_strands
   if (e != hipSuccess) return fail(HMSNN_EDEVICE, hipGetErrorString(e));
   (void)hipStreamSynchronize(c.stream);
   c.view.strand_sigma = c.strand_sigma_dev;
+  return HMSNN_OK;
+}
+
+int hmsnn_set_seed_offset(hmsnn_ctx* ctx, int offset) {
+  if (!ctx || offset < 0) return fail(HMSNN_EINVAL, "hmsnn_set_seed_offset: bad arguments");
+  ctx->c.seed_offset = offset;
   return HMSNN_OK;
 }
 

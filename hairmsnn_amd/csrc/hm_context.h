@@ -46,6 +46,7 @@ struct Context {
   float4* accum = nullptr;
   float4* average = nullptr;
   int accum_id = 0;
+  int seed_offset = 0;  // hmsnn_set_seed_offset
   int trace_blocks = 2048;
   int spill_levels = 0;
   unsigned long long* stats_dev = nullptr;

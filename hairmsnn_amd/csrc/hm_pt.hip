@@ -348,12 +348,12 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
                 std::string& err) {
   if (cfg.camera_mode == 1) {
     const int n = cfg.n_slots;
-    hipLaunchKernelGGL(k_camera_points, dim3((n + 255) / 256), dim3(256), 0, stream, sc, pb, cfg.accum_id, n,
-                       cfg.grid_w, cfg.points);
+    hipLaunchKernelGGL(k_camera_points, dim3((n + 255) / 256), dim3(256), 0, stream, sc, pb,
+                       cfg.accum_id + cfg.seed_offset, n, cfg.grid_w, cfg.points);
   } else {
     const int n = sc.width * sc.height;
-    hipLaunchKernelGGL(k_camera, dim3((n + 255) / 256), dim3(256), 0, stream, sc, pb, cfg.accum_id, cfg.msnn,
-                       cfg.train_idx, cfg.every_nth, cfg.n_slots);
+    hipLaunchKernelGGL(k_camera, dim3((n + 255) / 256), dim3(256), 0, stream, sc, pb, cfg.accum_id + cfg.seed_offset,
+                       cfg.msnn, cfg.train_idx, cfg.every_nth, cfg.n_slots);
   }
   HM_TRY(check_launch("camera", err));
   const bool count = cfg.stats != nullptr;

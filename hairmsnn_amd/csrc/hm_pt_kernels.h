@@ -58,6 +58,7 @@ struct PassTimer {
 
 struct PassConfig {
   int accum_id = 0;
+  int seed_offset = 0;  // added to accum_id for the per-pixel seeds only
   int dl_zero = 1;        // direct lighting at the camera vertex
   int dl_from = 0;        // direct lighting at vertices >= dl_from (>= 1)
   int max_vertex = 39;    // path_v2 - 1

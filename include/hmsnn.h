@@ -147,6 +147,13 @@ int hmsnn_set_strand_sigma_a(hmsnn_ctx* ctx, const float* sigma_host, int num_st
 /* ---- path tracing (render_path_tracing: render_path_tracing.cu:404-423,
  *      cuda/path_tracing.cu:19-67) --------------------------------------- */
 
+/* Adds offset to the pass index that seeds each pixel's random stream
+ * (path_tracing.cu:29 seeds with accumId + 10007), for every later pass of
+ * this context (path tracing, HairMSNN frames and the adjoint's replay).
+ * Default 0 reproduces the reference's streams. No reference counterpart:
+ * it lets a reference image and a test render use independent samples. */
+int hmsnn_set_seed_offset(hmsnn_ctx* ctx, int offset);
+
 /* Clears the accumulation (accumId = 0), like cameraChanged does. */
 int hmsnn_pt_reset(hmsnn_ctx* ctx);
 /* Traces `passes` more samples per pixel into the accumulation buffers.

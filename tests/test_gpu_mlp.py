@@ -106,7 +106,12 @@ def test_fp8_inference_against_fp64_and_fp16(gpu):
         m = Mlp(gpu, prec)
         m.set_weights(w)
         out[prec] = m.forward(r).astype(np.float64)
-        assert np.all(np.isfinite(out[prec]))
+        bad = ~np.isfinite(out[prec]).all(-1)
+        if bad.any():
+            i = np.flatnonzero(bad)[:4]
+            print(f"precision {prec}: {bad.sum()} non-finite outputs, e.g. rows {i}: {out[prec][i]} "
+                  f"(FP64 {y64[i]}) inputs {r[i]}")
+    assert all(np.isfinite(out[p]).all() for p in out)
     e8 = np.abs(out[2] - y64) / np.abs(y64)
     e16 = np.abs(out[0] - y64) / np.abs(y64)
     e8_16 = np.abs(out[2] - out[0]) / np.abs(out[0])
