@@ -48,6 +48,7 @@ class MsnnStats(C.Structure):
     _fields_ = [
         ("gbuffer_ms", C.c_double), ("train_ms", C.c_double), ("infer_ms", C.c_double), ("render_ms", C.c_double),
         ("total_ms", C.c_double), ("loss", C.c_double), ("frames", C.c_int), ("train_steps", C.c_int),
+        ("setup_ms", C.c_double), ("warmup_ms", C.c_double),
     ]
 
 

@@ -101,6 +101,7 @@ Context::~Context() {
   msnn_release(msnn);
   msnn = nullptr;
   free_path_buffers();
+  dev_free(scene_points);
   dev_free(seg_dev);
   dev_free(strand_sigma_dev);
   dev_free(stats_dev);

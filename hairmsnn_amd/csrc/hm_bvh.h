@@ -94,104 +94,77 @@ struct TraceStack {
   }
 };
 
-// Traversal runs in two phases per loop iteration so the expensive segment
-// test executes with many lanes at once instead of inside every node visit:
-//   inner step: a lane with an inner node to visit fetches it and tests both
-//     child boxes; a hit leaf child becomes the lane's pending leaf (one
-//     slot) or, if the slot is taken, goes on the stack as ~leaf; inner
-//     children continue near-first;
-//   leaf step: when at least kLeafBatch lanes hold a pending leaf, or no lane
-//     has an inner node left, every lane with a pending leaf tests it.
-// Stack entries are inner nodes (>= 0) or leaves (~index < 0). A lane never
-// sits with an empty node, an empty leaf slot and a non-empty stack, so it is
-// done exactly when both are empty; every iteration advances at least one
-// lane, and the iteration cap bounds the loop for a malformed tree only.
-constexpr int kLeafBatch = 16;
+// Traversal: the current entry is an inner node (>= 0) or a leaf (~index <
+// 0); each iteration either visits one node (nearer hit child next, the other
+// pushed, leaves included) or tests one segment. Keeping the two steps apart
+// bounds the per-iteration code a wave executes to one node visit plus one
+// segment test (testing leaf children inside the node visit costs up to two
+// segment tests per iteration under divergence). The cap bounds the loop for
+// a malformed tree only: a valid tree visits each node and leaf at most once.
 constexpr int kNone = 0x7fffffff;
-
-// Takes the stack top into node or leaf (the leaf slot must be empty).
-HM_D void refill(TraceStack& st, int& node, int& leaf) {
-  if (st.sp == 0) return;
-  const int e = st.pop();
-  if (e < 0) leaf = ~e;
-  else node = e;
-}
-
-HM_D int wave_popc(bool p) { return __popcll(__ballot(p)); }
-
-// Closest hit over hair segments: minimum (t, id) lexicographically.
 template <bool kCount>
-HM_D void trace_closest(const BvhView& bvh, const RayInv& r, TraceStack& st, float& best_t,
-                        int& best_id, float& best_u, TraceCounters& cnt) {
+HM_D void trace_closest(const BvhView& bvh, const RayInv& r, TraceStack& st, float& best_t, int& best_id,
+                            float& best_u, TraceCounters& cnt) {
   if (bvh.num_segs <= 0) return;
   st.sp = 0;
-  int node = 0, leaf = kNone;
-  const long long cap = 128ll * bvh.num_segs + 256;
-  for (long long iter = 0; iter < cap; ++iter) {
-    if (node != kNone) {
-      const BvhNode n = bvh.nodes[node];
+  int cur = 0;
+  for (int iter = 0; iter < 2 * bvh.num_segs + 64 && cur != kNone; ++iter) {
+    if (cur < 0) {
+      const int leaf = ~cur;
+      const float4 s0 = bvh.segs[2 * leaf];
+      const float4 s1 = bvh.segs[2 * leaf + 1];
+      if (kCount) cnt.segs++;
+      float t, u;
+      if (intersect_segment(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, s0.x, s0.y, s0.z, s1.x, s1.y, s1.z, s0.w, &t, &u)) {
+        const int id = __float_as_int(s1.w);
+        if (t < best_t || (t == best_t && id < best_id)) {
+          best_t = t;
+          best_id = id;
+          best_u = u;
+        }
+      }
+      cur = st.sp ? st.pop() : kNone;
+    } else {
+      const BvhNode n = bvh.nodes[cur];
       if (kCount) cnt.nodes++;
       const float t0 = box_enter(r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, best_t);
       const float t1 = box_enter(r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, best_t);
-      const bool h0 = t0 <= best_t, h1 = t1 <= best_t;  // false on a miss (NaN)
+      const bool h0 = t0 <= best_t, h1 = t1 <= best_t;
       const bool sw = h0 && h1 && t1 < t0;
       const int ca = sw ? n.d.y : n.d.x, cb = sw ? n.d.x : n.d.y;
       const bool ha = sw ? h1 : h0, hb = sw ? h0 : h1;
-      int next = kNone;
-      if (ha) {
-        if (ca >= 0) next = ca;
-        else if (leaf == kNone) leaf = ~ca;
-        else st.push(ca);
-      }
-      if (hb) {
-        if (cb >= 0) {
-          if (next == kNone) next = cb;
-          else st.push(cb);
-        } else if (leaf == kNone) {
-          leaf = ~cb;
-        } else {
-          st.push(cb);
-        }
-      }
-      node = next;
-      if (node == kNone && leaf == kNone) refill(st, node, leaf);
-    }
-    const int pending = wave_popc(leaf != kNone);
-    if (pending >= kLeafBatch || (pending > 0 && wave_popc(node != kNone) == 0)) {
-      if (leaf != kNone) {
-        const float4 s0 = bvh.segs[2 * leaf];
-        const float4 s1 = bvh.segs[2 * leaf + 1];
-        if (kCount) cnt.segs++;
-        float t, u;
-        if (intersect_segment(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, s0.x, s0.y, s0.z, s1.x, s1.y, s1.z, s0.w, &t,
-                              &u)) {
-          const int id = __float_as_int(s1.w);
-          if (t < best_t || (t == best_t && id < best_id)) {
-            best_t = t;
-            best_id = id;
-            best_u = u;
-          }
-        }
-        leaf = kNone;
-        if (node == kNone) refill(st, node, leaf);
+      if (ha && hb) {
+        st.push(cb);
+        cur = ca;
+      } else if (ha) {
+        cur = ca;
+      } else if (hb) {
+        cur = cb;
+      } else {
+        cur = st.sp ? st.pop() : kNone;
       }
     }
-    if (wave_popc(node != kNone || leaf != kNone) == 0) break;
   }
 }
 
-// Any hit with t > 0 over hair segments (binary visibility).
 template <bool kCount>
 HM_D bool trace_any(const BvhView& bvh, const RayInv& r, TraceStack& st, TraceCounters& cnt) {
   if (bvh.num_segs <= 0) return false;
   const float tinf = __builtin_inff();
   st.sp = 0;
-  int node = 0, leaf = kNone;
-  bool hit = false;
-  const long long cap = 128ll * bvh.num_segs + 256;
-  for (long long iter = 0; iter < cap; ++iter) {
-    if (node != kNone) {
-      const BvhNode n = bvh.nodes[node];
+  int cur = 0;
+  for (int iter = 0; iter < 2 * bvh.num_segs + 64 && cur != kNone; ++iter) {
+    if (cur < 0) {
+      const int leaf = ~cur;
+      const float4 s0 = bvh.segs[2 * leaf];
+      const float4 s1 = bvh.segs[2 * leaf + 1];
+      if (kCount) cnt.segs++;
+      float t, u;
+      if (intersect_segment(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, s0.x, s0.y, s0.z, s1.x, s1.y, s1.z, s0.w, &t, &u))
+        return true;
+      cur = st.sp ? st.pop() : kNone;
+    } else {
+      const BvhNode n = bvh.nodes[cur];
       if (kCount) cnt.nodes++;
       const float t0 = box_enter(r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, tinf);
       const float t1 = box_enter(r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, tinf);
@@ -199,46 +172,19 @@ HM_D bool trace_any(const BvhView& bvh, const RayInv& r, TraceStack& st, TraceCo
       const bool sw = h0 && h1 && t1 < t0;
       const int ca = sw ? n.d.y : n.d.x, cb = sw ? n.d.x : n.d.y;
       const bool ha = sw ? h1 : h0, hb = sw ? h0 : h1;
-      int next = kNone;
-      if (ha) {
-        if (ca >= 0) next = ca;
-        else if (leaf == kNone) leaf = ~ca;
-        else st.push(ca);
-      }
-      if (hb) {
-        if (cb >= 0) {
-          if (next == kNone) next = cb;
-          else st.push(cb);
-        } else if (leaf == kNone) {
-          leaf = ~cb;
-        } else {
-          st.push(cb);
-        }
-      }
-      node = next;
-      if (node == kNone && leaf == kNone) refill(st, node, leaf);
-    }
-    const int pending = wave_popc(leaf != kNone);
-    if (pending >= kLeafBatch || (pending > 0 && wave_popc(node != kNone) == 0)) {
-      if (leaf != kNone) {
-        const float4 s0 = bvh.segs[2 * leaf];
-        const float4 s1 = bvh.segs[2 * leaf + 1];
-        if (kCount) cnt.segs++;
-        float t, u;
-        leaf = kNone;
-        if (intersect_segment(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, s0.x, s0.y, s0.z, s1.x, s1.y, s1.z, s0.w, &t,
-                              &u)) {
-          hit = true;
-          node = kNone;
-          st.sp = 0;
-        } else if (node == kNone) {
-          refill(st, node, leaf);
-        }
+      if (ha && hb) {
+        st.push(cb);
+        cur = ca;
+      } else if (ha) {
+        cur = ca;
+      } else if (hb) {
+        cur = cb;
+      } else {
+        cur = st.sp ? st.pop() : kNone;
       }
     }
-    if (wave_popc(node != kNone || leaf != kNone) == 0) break;
   }
-  return hit;
+  return false;
 }
 
 #endif  // __HIPCC__

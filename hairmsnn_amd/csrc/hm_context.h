@@ -52,6 +52,9 @@ struct Context {
   unsigned long long* stats_dev = nullptr;
 
   MsnnState* msnn = nullptr;
+  // HairMSNN scene samples on the hair (3 floats each), made once per scene
+  float* scene_points = nullptr;
+  int num_scene_points = 0;
 
   ~Context();
   int init(std::string& err);

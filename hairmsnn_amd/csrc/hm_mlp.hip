@@ -42,6 +42,11 @@
 namespace hm {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// Materialises a matrix-instruction result at this point of the program for
+// the whole wave (an empty asm that reads and writes it), so the instruction
+// cannot be moved into a following lane-divergent branch.
+__device__ __forceinline__ void pin_wave(f32x16& v) { asm volatile("" : "+v"(v)); }
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
@@ -249,7 +254,8 @@ __global__ void __launch_bounds__(kMlpBlock) k_mlp_infer(const T* frag, const fl
     for (int s = 0; s < 4; ++s) B[s] = encoded_operand<T>(e, s, g);
 #pragma unroll 1
     for (int l = 0; l < 5; ++l) hidden_layer<T>(lds, l, lane, B);
-    const f32x16 z = output_layer<T>(lds, lane, B);
+    f32x16 z = output_layer<T>(lds, lane, B);
+    pin_wave(z);  // see k_mlp_infer_fp8
     if (g == 0 && q < n) {
       out[(size_t)q * 3 + 0] = expf(z[0]);
       out[(size_t)q * 3 + 1] = expf(z[1]);
@@ -413,7 +419,11 @@ __global__ void __launch_bounds__(kMlpBlock) k_mlp_infer_fp8(const i32x8* frag, 
         B[4 + k] = fp8_pack4(fmaxf(a1[4 * k], 0.f) * sc, fmaxf(a1[4 * k + 1], 0.f) * sc,
                              fmaxf(a1[4 * k + 2], 0.f) * sc, fmaxf(a1[4 * k + 3], 0.f) * sc);
     }
-    const f32x16 z = mfma_fp8(lds[kFrag8Hidden + lane], B, f32x16{}, scales[5], scales[11]);
+    f32x16 z = mfma_fp8(lds[kFrag8Hidden + lane], B, f32x16{}, scales[5], scales[11]);
+    // Keep the matrix instruction (and the copies that assemble its operands)
+    // ahead of the lane-divergent store below: it reads all 64 lanes' operands,
+    // and sunk into the branch its operand copies would run for half the wave.
+    pin_wave(z);
     if (g == 0 && q < n) {
       out[(size_t)q * 3 + 0] = expf(z[0]);
       out[(size_t)q * 3 + 1] = expf(z[1]);

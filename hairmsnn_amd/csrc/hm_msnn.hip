@@ -39,8 +39,7 @@ struct MsnnState {
   int frame = 0;       // accumulated frames (the reference's accumId)
   int gen_id = 0;      // accumId used by the warm-up passes
   uint64_t rng_state = 0;
-  std::vector<int> order;  // host permutation buffer
-  std::vector<float> points_host;
+  double setup_ms = 0.0, warmup_ms = 0.0;  // host wall clock of msnn_begin's two parts
   // device buffers
   float* nn_input = nullptr;     // W*H*12
   uint8_t* hit_flags = nullptr;  // W*H
@@ -50,9 +49,7 @@ struct MsnnState {
   float* train_target = nullptr; // n_train*3
   uint8_t* train_hit = nullptr;  // n_train
   int* train_idx = nullptr;      // n_train
-  float* points = nullptr;       // sampled hair points, 3 per sample
   float* gen_points = nullptr;   // points of one training-data pass, n_train*3
-  int num_points = 0;
   float4* acc[3] = {nullptr, nullptr, nullptr};  // short, network, final
   float4* avg[3] = {nullptr, nullptr, nullptr};
   double loss_last = 0.0;
@@ -66,7 +63,6 @@ struct MsnnState {
     dev_free(train_target);
     dev_free(train_hit);
     dev_free(train_idx);
-    dev_free(points);
     dev_free(gen_points);
     for (int k = 0; k < 3; ++k) {
       dev_free(acc[k]);
@@ -165,6 +161,49 @@ float lr_at(float base, int step) {
   return base * std::pow(0.33f, (float)((step - 4000) / 4000 + 1));
 }
 
+// Scene samples on the hair (fetchSceneSamples, render_hair_msnn.cu:34-97):
+// about one million points, per segment in proportion to its length and at
+// least one per segment, uniform along the segment. Made once per scene and
+// kept on the device.
+int ensure_scene_points(Context& c, uint64_t seed, std::string& err) {
+  if (c.scene_points) return kOk;
+  const int nseg = (int)(c.hair.seg.size() / 8);
+  double total = 0.0;
+  std::vector<double> len(nseg);
+  for (int i = 0; i < nseg; ++i) {
+    const float* q = &c.hair.seg[8 * (size_t)i];
+    len[i] = std::sqrt((double)(q[4] - q[0]) * (q[4] - q[0]) + (double)(q[5] - q[1]) * (q[5] - q[1]) +
+                       (double)(q[6] - q[2]) * (q[6] - q[2]));
+    total += len[i];
+  }
+  const double target_samples = 1000000.0;
+  std::vector<int> count(nseg);
+  size_t np = 0;
+  for (int i = 0; i < nseg; ++i) {
+    count[i] = std::max(1, (int)(len[i] / total * target_samples));
+    np += (size_t)count[i];
+  }
+  std::vector<float> pts(3 * np);
+  uint64_t st = seed * 0x2545F4914F6CDD1Dull + 3;
+  size_t k = 0;
+  for (int i = 0; i < nseg; ++i) {
+    const float* q = &c.hair.seg[8 * (size_t)i];
+    for (int j = 0; j < count[i]; ++j, ++k) {
+      st ^= st << 13;
+      st ^= st >> 7;
+      st ^= st << 17;
+      const float u = (float)((st >> 40) * (1.0 / 16777216.0));
+      pts[3 * k] = q[0] + u * (q[4] - q[0]);
+      pts[3 * k + 1] = q[1] + u * (q[5] - q[1]);
+      pts[3 * k + 2] = q[2] + u * (q[6] - q[2]);
+    }
+  }
+  HM_TRY(dev_alloc((void**)&c.scene_points, sizeof(float) * pts.size(), err));
+  HM_HIP(hipMemcpy(c.scene_points, pts.data(), sizeof(float) * pts.size(), hipMemcpyHostToDevice), err);
+  c.num_scene_points = (int)np;
+  return kOk;
+}
+
 int msnn_setup(Context& c, int BETA, std::string& err) {
   if (!c.scene.has_hair) {
     err = "HairMSNN needs hair geometry";
@@ -203,58 +242,33 @@ int msnn_setup(Context& c, int BETA, std::string& err) {
     HM_TRY(dev_alloc((void**)&m->avg[k], sizeof(float4) * (size_t)npx, err));
     HM_HIP(hipMemsetAsync(m->avg[k], 0, sizeof(float4) * (size_t)npx, c.stream), err);
   }
-  // Scene samples on the hair (fetchSceneSamples, render_hair_msnn.cu:34-97):
-  // about one million points, per segment in proportion to its length.
-  const int nseg = (int)(c.hair.seg.size() / 8);
-  double total = 0.0;
-  std::vector<double> len(nseg);
-  for (int i = 0; i < nseg; ++i) {
-    const float* q = &c.hair.seg[8 * (size_t)i];
-    len[i] = std::sqrt((double)(q[4] - q[0]) * (q[4] - q[0]) + (double)(q[5] - q[1]) * (q[5] - q[1]) +
-                       (double)(q[6] - q[2]) * (q[6] - q[2]));
-    total += len[i];
-  }
-  const int target_samples = 1000000;
-  m->points_host.clear();
-  for (int i = 0; i < nseg; ++i) {
-    int k = (int)(len[i] / total * target_samples);
-    if (k == 0) k = 1;
-    const float* q = &c.hair.seg[8 * (size_t)i];
-    for (int j = 0; j < k; ++j) {
-      const float u = (float)((m->next_u64() >> 40) * (1.0 / 16777216.0));
-      m->points_host.push_back(q[0] + u * (q[4] - q[0]));
-      m->points_host.push_back(q[1] + u * (q[5] - q[1]));
-      m->points_host.push_back(q[2] + u * (q[6] - q[2]));
-    }
-  }
-  m->num_points = (int)(m->points_host.size() / 3);
-  m->order.resize(std::max(m->num_points, m->n_train));
+  HM_TRY(ensure_scene_points(c, s.msnn_seed, err));
   HM_TRY(c.ensure_path_buffers(true, err, m->n_train));
   return kOk;
 }
 
-// Random subset of the scene samples for one training-data pass (the
-// reference shuffles its sample indices and takes the first records).
-int upload_gen_points(Context& c, MsnnState& m, std::string& err) {
-  std::vector<float> pts(3 * (size_t)m.n_train);
-  std::vector<int>& ord = m.order;
-  if ((int)ord.size() != m.num_points) ord.resize(m.num_points);
-  std::iota(ord.begin(), ord.begin() + m.num_points, 0);
-  const int take = std::min(m.n_train, m.num_points);
-  for (int i = 0; i < take; ++i) {
-    const int j = i + (int)(m.next_u64() % (uint64_t)(m.num_points - i));
-    std::swap(ord[i], ord[j]);
-  }
-  for (int i = 0; i < m.n_train; ++i) {
-    const int p = ord[i % take];
-    pts[3 * (size_t)i] = m.points_host[3 * (size_t)p];
-    pts[3 * (size_t)i + 1] = m.points_host[3 * (size_t)p + 1];
-    pts[3 * (size_t)i + 2] = m.points_host[3 * (size_t)p + 2];
-  }
-  HM_HIP(hipMemcpyAsync(m.gen_points, pts.data(), sizeof(float) * pts.size(), hipMemcpyHostToDevice, c.stream), err);
-  // the copy reads pts before this function returns
-  HM_HIP(hipStreamSynchronize(c.stream), err);
-  return kOk;
+// Training-data targets for one warm-up pass: n uniform draws (with
+// replacement) from the scene samples, hashed from (seed, i). The reference
+// shuffles its sample indices and takes the first records
+// (render_hair_msnn.cu:640-660); over ~3M samples the two differ only in the
+// rare repeated draw.
+__global__ void k_sample_points(const float* pts, int num_points, uint64_t seed, int n, float* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  const int p = (int)(z % (uint64_t)num_points);
+  out[3 * (size_t)i] = pts[3 * (size_t)p];
+  out[3 * (size_t)i + 1] = pts[3 * (size_t)p + 1];
+  out[3 * (size_t)i + 2] = pts[3 * (size_t)p + 2];
+}
+
+int sample_gen_points(Context& c, MsnnState& m, std::string& err) {
+  hipLaunchKernelGGL(k_sample_points, dim3((m.n_train + 255) / 256), dim3(256), 0, c.stream, c.scene_points,
+                     c.num_scene_points, m.next_u64(), m.n_train, m.gen_points);
+  return check_launch("msnn sample points", err);
 }
 
 PassConfig msnn_pass_config(const Context& c, const MsnnState& m) {
@@ -279,7 +293,7 @@ int train_step(Context& c, MsnnState& m, std::string& err, float* loss_out) {
 // one training step (render_hair_msnn.cu:633-665).
 int msnn_warmup(Context& c, MsnnState& m, int steps, std::string& err) {
   for (int k = 0; k < steps; ++k) {
-    HM_TRY(upload_gen_points(c, m, err));
+    HM_TRY(sample_gen_points(c, m, err));
     PassConfig cfg = msnn_pass_config(c, m);
     cfg.camera_mode = 1;
     cfg.n_slots = m.n_train;
@@ -382,11 +396,17 @@ int hmsnn_msnn_begin(hmsnn_ctx* ctx, int BETA) {
   if (!ctx) return fail(HMSNN_EINVAL, "NULL context");
   std::string err;
   Context& c = ctx->c;
+  const auto t0 = std::chrono::steady_clock::now();
   int rc = msnn_setup(c, BETA, err);
+  if (!rc && hipStreamSynchronize(c.stream) != hipSuccess) rc = HMSNN_EDEVICE, err = "setup failed";
+  const auto t1 = std::chrono::steady_clock::now();
   if (!rc) rc = msnn_warmup(c, *c.msnn, c.scene.msnn_warmup_steps, err);
   if (!rc) {
     hipError_t e = hipStreamSynchronize(c.stream);
     if (e != hipSuccess) return fail(HMSNN_EDEVICE, hipGetErrorString(e));
+    const auto t2 = std::chrono::steady_clock::now();
+    c.msnn->setup_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    c.msnn->warmup_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
   }
   return rc ? fail(rc, err) : HMSNN_OK;
 }
@@ -406,6 +426,8 @@ int hmsnn_msnn_frames(hmsnn_ctx* ctx, int frames, hmsnn_msnn_stats* stats) {
   if (stats) {
     local.loss = c.msnn->loss_last;
     local.train_steps = c.msnn->train_steps;
+    local.setup_ms = c.msnn->setup_ms;
+    local.warmup_ms = c.msnn->warmup_ms;
     *stats = local;
   }
   return HMSNN_OK;

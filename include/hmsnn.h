@@ -190,9 +190,10 @@ int hmsnn_trace_rays_dev(hmsnn_ctx* ctx, const float* rays_dev, int n, int any_h
  *      cuda/hair_msnn.cu:17-357) ------------------------------------------ */
 
 typedef struct hmsnn_msnn_stats {
-  double gbuffer_ms, train_ms, infer_ms, render_ms, total_ms;
+  double gbuffer_ms, train_ms, infer_ms, render_ms, total_ms; /* summed over the call's frames */
   double loss;
   int frames, train_steps;
+  double setup_ms, warmup_ms; /* hmsnn_msnn_begin: scene samples + buffers, warm-up training */
 } hmsnn_msnn_stats;
 
 /* Sets BETA exactly as the CLI argument (kernel beta = BETA - 1,
