@@ -9,10 +9,10 @@ traced to termination with its next-event-estimation and MIS shadow rays
 state are resident in HBM before the timed region. One JSON line is printed.
 
 Roofline: the dominant kernel is the closest-hit BVH traversal (extend).
-Algorithmic bytes per launch = nodes visited x 64 B + segments tested x 32 B
-+ 48 B per ray (32 B ray in, 16 B hit out), counted by the same traversal
-compiled with counters; divided by the extend kernel's mean launch time from
-HIP events on the context's stream.
+Algorithmic bytes per launch = 4-wide nodes visited x 128 B + segments
+tested x 32 B + 48 B per ray (32 B ray in, 16 B hit out), counted by the
+same traversal compiled with counters; divided by the extend kernel's mean
+launch time from HIP events on the context's stream.
 
 CPU baseline: the C++ oracle (FP32 twin of the reference integrator, OpenMP)
 on a bounded crop of the same scene, run on this machine's host cores.
@@ -34,7 +34,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "paths/s @1024² 3.2M segs; HairMSNN β=1 vs plain PT time-to-equal-RMSE"
 HBM_PEAK_GBS = 8000.0
 SCENE = ROOT / "hairmsnn_amd" / "scenes" / "c1_curly.json"
-NODE_BYTES, SEG_BYTES, RAY_BYTES = 64, 32, 48
+NODE_BYTES, SEG_BYTES, RAY_BYTES = 128, 32, 48  # 4-wide node, segment record, ray in + hit out
 
 
 def log(msg: str) -> None:

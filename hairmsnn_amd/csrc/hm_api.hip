@@ -150,6 +150,8 @@ int Context::ensure_path_buffers(bool msnn_mode, std::string& err, int min_slots
   HM_TRY(A((void**)&pb.queue[0], sizeof(int) * n));
   HM_TRY(A((void**)&pb.queue[1], sizeof(int) * n));
   HM_TRY(A((void**)&pb.shadow_queue, sizeof(int) * n));
+  HM_TRY(A((void**)&pb.shadow_rays, sizeof(int) * 2 * n));
+  HM_TRY(A((void**)&pb.sh_vis, 2 * n));
   HM_TRY(A((void**)&pb.counts, sizeof(int) * 16));
   if (msnn_mode) {
     HM_TRY(A((void**)&pb.beta_short, f4));
@@ -158,7 +160,7 @@ int Context::ensure_path_buffers(bool msnn_mode, std::string& err, int min_slots
   }
   pb_msnn = msnn_mode;
   // traversal stack spill: levels beyond the LDS part, one column per thread
-  spill_levels = std::max(1, bvh.max_depth - kStackLds + 2);
+  spill_levels = std::max(1, bvh.stack_bound() - kStackLds + 1);
   pb.spill_stride = trace_blocks * kTraceBlock;
   pb.spill_levels = spill_levels;
   HM_TRY(A((void**)&pb.spill, sizeof(int) * (size_t)spill_levels * pb.spill_stride));
@@ -259,7 +261,7 @@ int Context::init(std::string& err) {
   // ---- view ----
   SceneView& v = view;
   v = SceneView{};
-  v.bvh.nodes = bvh.nodes;
+  v.bvh.nodes = bvh.nodes4;
   v.bvh.segs = bvh.segs;
   v.bvh.num_segs = bvh.num_segs;
   v.seg_by_id = seg_dev;
@@ -430,8 +432,8 @@ int hmsnn_get_info(const hmsnn_ctx* ctx, hmsnn_info* info) {
   fill_scene_info(c.scene, info);
   info->num_segments = c.bvh.num_segs;
   info->num_strands = c.hair.num_strands;
-  info->bvh_nodes = c.bvh.num_nodes;
-  info->bvh_depth = c.bvh.max_depth;
+  info->bvh_nodes = c.bvh.num_nodes4;
+  info->bvh_depth = c.bvh.depth4;
   info->env_width = c.env.width;
   info->env_height = c.env.height;
   info->scene_scale = c.scene_scale;
@@ -544,7 +546,7 @@ int hmsnn_pt_reset(hmsnn_ctx* ctx) {
 static int run_pt_passes(Context& c, int passes, std::string& err) {
   HM_TRY(c.ensure_path_buffers(false, err));
   SceneView v = c.view;
-  v.bvh.nodes = c.bvh.nodes;
+  v.bvh.nodes = c.bvh.nodes4;
   c.pb.spill_stride = c.trace_blocks * kTraceBlock;
   const int n = c.scene.width * c.scene.height;
   for (int k = 0; k < passes; ++k) {

@@ -1,13 +1,14 @@
 // Device-side BVH view and traversal (closest hit and any hit).
 //
-// Layout in HBM: binary nodes of 64 bytes that hold both children's boxes
-// (so one node fetch decides both children), and 32-byte segment records in
-// Morton order. Child index >= 0 is an internal node, < 0 is ~leaf where leaf
-// is the position in the Morton-ordered segment array.
+// Layout in HBM: the Karras LBVH is built binary (64-byte nodes holding both
+// children's boxes) and then collapsed into 4-wide nodes of 128 bytes
+// (BvhNode4) that traversal reads; segments are 32-byte records in Morton
+// order. A child index >= 0 is an inner node, < 0 is ~leaf where leaf is the
+// position in the Morton-ordered segment array.
 //
 // Traversal is one ray per lane (wave64). The stack's top kStackLds entries
 // live in LDS, one column per lane (conflict-free), deeper entries spill to a
-// per-thread global area sized from the tree depth at build time.
+// per-thread global area sized from the 4-wide tree's depth at build time.
 #pragma once
 
 #include "hm_common.h"
@@ -22,9 +23,21 @@ struct BvhNode {
   int4 d;    // child0, child1, unused, unused
 };
 
+// 4-wide node (128 bytes): child boxes as structure-of-arrays, so one node
+// fetch is eight 16-byte loads that decide all four children. child[k] >= 0
+// is an inner node, < 0 is ~segment position (leaf), kEmptyChild is an
+// unused slot (its box is never tested).
+struct BvhNode4 {
+  float4 lox, loy, loz;
+  float4 hix, hiy, hiz;
+  int4 child;
+  int4 pad;
+};
+constexpr int kEmptyChild = 0x7fffffff;
+
 struct BvhView {
-  const BvhNode* nodes;
-  const float4* segs;  // 2 float4 per segment: (p0, r), (p1, id bits)
+  const BvhNode4* nodes;  // 4-wide tree, root at 0
+  const float4* segs;     // 2 float4 per segment: (p0, r), (p1, id bits)
   int num_segs;
 };
 
@@ -35,6 +48,13 @@ struct SphereView {
 
 constexpr int kStackLds = 16;
 constexpr int kTraceBlock = 256;
+
+// Launch bounds of the traversal kernels: 256 threads, and a floor of
+// HM_TRACE_MIN_WAVES waves per SIMD (the register budget follows from it).
+#ifndef HM_TRACE_MIN_WAVES
+#define HM_TRACE_MIN_WAVES 8
+#endif
+#define HM_TRACE_BOUNDS __launch_bounds__(kTraceBlock, HM_TRACE_MIN_WAVES)
 
 struct TraceCounters {
   uint32_t nodes;
@@ -95,16 +115,42 @@ struct TraceStack {
 };
 
 // Traversal: the current entry is an inner node (>= 0) or a leaf (~index <
-// 0); each iteration either visits one node (nearer hit child next, the other
-// pushed, leaves included) or tests one segment. Keeping the two steps apart
-// bounds the per-iteration code a wave executes to one node visit plus one
-// segment test (testing leaf children inside the node visit costs up to two
-// segment tests per iteration under divergence). The cap bounds the loop for
-// a malformed tree only: a valid tree visits each node and leaf at most once.
+// 0); each iteration either visits one 4-wide node (nearest hit child next,
+// the other hit children pushed far to near, leaves included) or tests one
+// segment. Keeping the two steps apart bounds the code a wave executes per
+// iteration to one node visit plus one segment test. The cap bounds the loop
+// for a malformed tree only: a valid tree visits each node and leaf at most
+// once per ray.
 constexpr int kNone = 0x7fffffff;
+
+// Entry distances of the four child boxes against [0, tmax]; +inf for a miss
+// or an empty slot (so that sorting puts them last).
+HM_D void node4_hits(const BvhNode4& n, const RayInv& r, float tmax, float t[4]) {
+  const float lx[4] = {n.lox.x, n.lox.y, n.lox.z, n.lox.w}, ly[4] = {n.loy.x, n.loy.y, n.loy.z, n.loy.w};
+  const float lz[4] = {n.loz.x, n.loz.y, n.loz.z, n.loz.w}, hx[4] = {n.hix.x, n.hix.y, n.hix.z, n.hix.w};
+  const float hy[4] = {n.hiy.x, n.hiy.y, n.hiy.z, n.hiy.w}, hz[4] = {n.hiz.x, n.hiz.y, n.hiz.z, n.hiz.w};
+  const int ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float e = box_enter(r, lx[k], ly[k], lz[k], hx[k], hy[k], hz[k], tmax);
+    t[k] = (ch[k] != kEmptyChild && e <= tmax) ? e : __builtin_inff();
+  }
+}
+
+HM_D void cswap(float& ta, int& ca, float& tb, int& cb) {
+  const bool s = tb < ta;
+  const float t = s ? tb : ta;
+  tb = s ? ta : tb;
+  ta = t;
+  const int c = s ? cb : ca;
+  cb = s ? ca : cb;
+  ca = c;
+}
+
+// Closest hit over hair segments: minimum (t, id) lexicographically.
 template <bool kCount>
 HM_D void trace_closest(const BvhView& bvh, const RayInv& r, TraceStack& st, float& best_t, int& best_id,
-                            float& best_u, TraceCounters& cnt) {
+                        float& best_u, TraceCounters& cnt) {
   if (bvh.num_segs <= 0) return;
   st.sp = 0;
   int cur = 0;
@@ -125,28 +171,28 @@ HM_D void trace_closest(const BvhView& bvh, const RayInv& r, TraceStack& st, flo
       }
       cur = st.sp ? st.pop() : kNone;
     } else {
-      const BvhNode n = bvh.nodes[cur];
+      const BvhNode4 n = bvh.nodes[cur];
       if (kCount) cnt.nodes++;
-      const float t0 = box_enter(r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, best_t);
-      const float t1 = box_enter(r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, best_t);
-      const bool h0 = t0 <= best_t, h1 = t1 <= best_t;
-      const bool sw = h0 && h1 && t1 < t0;
-      const int ca = sw ? n.d.y : n.d.x, cb = sw ? n.d.x : n.d.y;
-      const bool ha = sw ? h1 : h0, hb = sw ? h0 : h1;
-      if (ha && hb) {
-        st.push(cb);
-        cur = ca;
-      } else if (ha) {
-        cur = ca;
-      } else if (hb) {
-        cur = cb;
-      } else {
-        cur = st.sp ? st.pop() : kNone;
-      }
+      float t[4];
+      node4_hits(n, r, best_t, t);
+      int c[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
+      // sorting network for four: nearest first, misses (+inf) last
+      cswap(t[0], c[0], t[1], c[1]);
+      cswap(t[2], c[2], t[3], c[3]);
+      cswap(t[0], c[0], t[2], c[2]);
+      cswap(t[1], c[1], t[3], c[3]);
+      cswap(t[1], c[1], t[2], c[2]);
+      const float inf = __builtin_inff();
+      if (t[3] < inf) st.push(c[3]);
+      if (t[2] < inf) st.push(c[2]);
+      if (t[1] < inf) st.push(c[1]);
+      cur = t[0] < inf ? c[0] : (st.sp ? st.pop() : kNone);
     }
   }
 }
 
+// Any hit with t > 0 over hair segments (binary visibility); children are
+// taken in slot order.
 template <bool kCount>
 HM_D bool trace_any(const BvhView& bvh, const RayInv& r, TraceStack& st, TraceCounters& cnt) {
   if (bvh.num_segs <= 0) return false;
@@ -164,24 +210,20 @@ HM_D bool trace_any(const BvhView& bvh, const RayInv& r, TraceStack& st, TraceCo
         return true;
       cur = st.sp ? st.pop() : kNone;
     } else {
-      const BvhNode n = bvh.nodes[cur];
+      const BvhNode4 n = bvh.nodes[cur];
       if (kCount) cnt.nodes++;
-      const float t0 = box_enter(r, n.a.x, n.a.y, n.a.z, n.a.w, n.b.x, n.b.y, tinf);
-      const float t1 = box_enter(r, n.b.z, n.b.w, n.c.x, n.c.y, n.c.z, n.c.w, tinf);
-      const bool h0 = t0 < tinf, h1 = t1 < tinf;
-      const bool sw = h0 && h1 && t1 < t0;
-      const int ca = sw ? n.d.y : n.d.x, cb = sw ? n.d.x : n.d.y;
-      const bool ha = sw ? h1 : h0, hb = sw ? h0 : h1;
-      if (ha && hb) {
-        st.push(cb);
-        cur = ca;
-      } else if (ha) {
-        cur = ca;
-      } else if (hb) {
-        cur = cb;
-      } else {
-        cur = st.sp ? st.pop() : kNone;
+      float t[4];
+      node4_hits(n, r, tinf, t);
+      const int c[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
+      int next = kNone;
+#pragma unroll
+      for (int k = 3; k >= 0; --k) {
+        if (t[k] < tinf) {
+          if (next != kNone) st.push(next);
+          next = c[k];
+        }
       }
+      cur = next != kNone ? next : (st.sp ? st.pop() : kNone);
     }
   }
   return false;

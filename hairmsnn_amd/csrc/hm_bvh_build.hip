@@ -232,7 +232,122 @@ __global__ void k_emit_single(const float* seg_in, BvhNode* nodes, float4* segs_
   segs_out[1] = make_float4(s[4], s[5], s[6], __int_as_float(0));
 }
 
+// ---- collapse to 4-wide nodes ----------------------------------------------
+
+struct CollapseArgs {
+  const BvhNode* bin;
+  BvhNode4* out;
+  const int2* frontier;  // (binary inner node, 4-wide index)
+  int nfront;
+  int2* next;
+  int* next_count;
+  int* node_count;
+};
+
+__device__ inline float half_area(float lx, float ly, float lz, float hx, float hy, float hz) {
+  const float dx = hx - lx, dy = hy - ly, dz = hz - lz;
+  return dx * dy + dy * dz + dz * dx;
+}
+
+// One 4-wide node per frontier entry: start from the binary node's two
+// children and open the inner child of largest surface area until there are
+// four (or only leaves remain).
+__global__ void k_collapse(CollapseArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.nfront) return;
+  const int2 f = a.frontier[i];
+  float lo[4][3], hi[4][3];
+  int ch[4];
+  auto load2 = [&](int b, int s0, int s1) {
+    const BvhNode n = a.bin[b];
+    lo[s0][0] = n.a.x; lo[s0][1] = n.a.y; lo[s0][2] = n.a.z;
+    hi[s0][0] = n.a.w; hi[s0][1] = n.b.x; hi[s0][2] = n.b.y;
+    lo[s1][0] = n.b.z; lo[s1][1] = n.b.w; lo[s1][2] = n.c.x;
+    hi[s1][0] = n.c.y; hi[s1][1] = n.c.z; hi[s1][2] = n.c.w;
+    ch[s0] = n.d.x;
+    ch[s1] = n.d.y;
+  };
+  load2(f.x, 0, 1);
+  int cnt = 2;
+  while (cnt < 4) {
+    int best = -1;
+    float ba = -1.f;
+    for (int k = 0; k < cnt; ++k) {
+      if (ch[k] < 0) continue;
+      const float ar = half_area(lo[k][0], lo[k][1], lo[k][2], hi[k][0], hi[k][1], hi[k][2]);
+      if (ar > ba) {
+        ba = ar;
+        best = k;
+      }
+    }
+    if (best < 0) break;
+    load2(ch[best], best, cnt);
+    ++cnt;
+  }
+  BvhNode4 o;
+  float* lx = &o.lox.x; float* ly = &o.loy.x; float* lz = &o.loz.x;
+  float* hx = &o.hix.x; float* hy = &o.hiy.x; float* hz = &o.hiz.x;
+  int* oc = &o.child.x;
+  for (int k = 0; k < 4; ++k) {
+    if (k < cnt) {
+      lx[k] = lo[k][0]; ly[k] = lo[k][1]; lz[k] = lo[k][2];
+      hx[k] = hi[k][0]; hy[k] = hi[k][1]; hz[k] = hi[k][2];
+      if (ch[k] >= 0) {
+        const int j = atomicAdd(a.node_count, 1);
+        a.next[atomicAdd(a.next_count, 1)] = make_int2(ch[k], j);
+        oc[k] = j;
+      } else {
+        oc[k] = ch[k];
+      }
+    } else {
+      lx[k] = ly[k] = lz[k] = 0.f;
+      hx[k] = hy[k] = hz[k] = 0.f;
+      oc[k] = kEmptyChild;
+    }
+  }
+  o.pad = make_int4(0, 0, 0, 0);
+  a.out[f.y] = o;
+}
+
 }  // namespace
+
+// Collapses the binary tree into 4-wide nodes, one frontier level per
+// launch (the node count per level comes back to the host, so the loop
+// knows when the tree is done).
+static int bvh_collapse(DeviceBvh& out, hipStream_t stream, std::string& err) {
+  const int cap = std::max(1, out.num_nodes);  // a 4-wide tree has fewer nodes than the binary one
+  HM_TRY(dev_alloc((void**)&out.nodes4, sizeof(BvhNode4) * (size_t)cap, err));
+  DeviceArena arena;
+  int2 *fa = nullptr, *fb = nullptr;
+  int* counters = nullptr;
+  HM_TRY(arena.alloc((void**)&fa, sizeof(int2) * (size_t)cap, err));
+  HM_TRY(arena.alloc((void**)&fb, sizeof(int2) * (size_t)cap, err));
+  HM_TRY(arena.alloc((void**)&counters, sizeof(int) * 2, err));
+  const int2 root = make_int2(0, 0);
+  const int init[2] = {1, 0};  // node count (root allocated), next-frontier count
+  HM_HIP(hipMemcpyAsync(fa, &root, sizeof(root), hipMemcpyHostToDevice, stream), err);
+  HM_HIP(hipMemcpyAsync(counters, init, sizeof(init), hipMemcpyHostToDevice, stream), err);
+  int nfront = 1, levels = 0;
+  while (nfront > 0) {
+    HM_HIP(hipMemsetAsync(counters + 1, 0, sizeof(int), stream), err);
+    CollapseArgs a{out.nodes, out.nodes4, fa, nfront, fb, counters + 1, counters};
+    hipLaunchKernelGGL(k_collapse, dim3((nfront + 255) / 256), dim3(256), 0, stream, a);
+    HM_TRY(check_launch("bvh collapse", err));
+    int h[2];
+    HM_HIP(hipMemcpyAsync(h, counters, sizeof(h), hipMemcpyDeviceToHost, stream), err);
+    HM_HIP(hipStreamSynchronize(stream), err);
+    if (h[0] > cap) {
+      err = "bvh collapse: node count exceeds the binary tree's";
+      return kEDevice;
+    }
+    out.num_nodes4 = h[0];
+    nfront = h[1];
+    std::swap(fa, fb);
+    ++levels;
+  }
+  out.depth4 = levels;
+  return kOk;
+}
 
 // Builds the BVH from device segments (8 floats each). All scratch is
 // released before returning; the result owns nodes and segment records.
@@ -250,7 +365,8 @@ int bvh_build(const float* seg_dev, int n, DeviceBvh& out, hipStream_t stream, s
   if (n == 1) {
     hipLaunchKernelGGL(k_emit_single, dim3(1), dim3(1), 0, stream, seg_dev, out.nodes, out.segs);
     out.max_depth = 1;
-    return check_launch("bvh single", err);
+    HM_TRY(check_launch("bvh single", err));
+    return bvh_collapse(out, stream, err);
   }
   DeviceArena arena;
   BuildBuffers b{};
@@ -298,7 +414,7 @@ int bvh_build(const float* seg_dev, int n, DeviceBvh& out, hipStream_t stream, s
   HM_HIP(hipMemcpyAsync(&depth, b.max_depth, sizeof(int), hipMemcpyDeviceToHost, stream), err);
   HM_HIP(hipStreamSynchronize(stream), err);
   out.max_depth = depth;
-  return 0;
+  return bvh_collapse(out, stream, err);
 }
 
 }  // namespace hm

@@ -67,8 +67,10 @@ struct PathBuffers {
   float4* sh_beta;        // path throughput at the vertex
   float4* sh_beta_short;  // HairMSNN training paths only
   int* queue[2];
-  int* shadow_queue;
-  int* counts;  // [0], [1] continuation queues, [2] shadow queue, [3..] stats
+  int* shadow_queue;  // slots with pending direct light
+  int* shadow_rays;   // pending shadow rays: slot * 2 + (0: light sample, 1: BSDF sample)
+  uint8_t* sh_vis;    // visibility per (slot, ray), same indexing as shadow_rays
+  int* counts;  // [0], [1] continuation queues, [2] shadow slots, [3] shadow rays
   int* spill;   // traversal stack spill
   int spill_stride;
   int spill_levels;

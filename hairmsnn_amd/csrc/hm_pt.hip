@@ -60,10 +60,11 @@ __device__ __forceinline__ void flush_counters(const TraceCounters& c, unsigned 
 
 }  // namespace
 
-__global__ void k_reset_counts(int* counts, int a, int b) {
+__global__ void k_reset_counts(int* counts, int a, int b, int c) {
   if (threadIdx.x == 0) {
     counts[a] = 0;
     counts[b] = 0;
+    counts[c] = 0;
   }
 }
 
@@ -120,7 +121,7 @@ __global__ void k_camera_points(SceneView sc, PathBuffers pb, int accum_id, int 
 }
 
 template <bool kCount>
-__global__ void __launch_bounds__(kTraceBlock) k_extend(SceneView sc, PathBuffers pb, int cur,
+__global__ void HM_TRACE_BOUNDS k_extend(SceneView sc, PathBuffers pb, int cur,
                                                         unsigned long long* stats) {
   __shared__ int lds_stack[kStackLds * kTraceBlock];
   TraceStack st = make_stack(lds_stack, pb);
@@ -150,7 +151,7 @@ __global__ void __launch_bounds__(256) k_shade(SceneView sc, PathBuffers pb, Sha
   const int stride = gridDim.x * blockDim.x;
   const int nloop = (n + stride - 1) / stride * stride;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
-    bool cont = false, shadow = false;
+    bool cont = false, shadow = false, ray1 = false, ray2 = false;
     int slot = -1;
     if (i < n) {
       slot = q[i];
@@ -184,6 +185,8 @@ __global__ void __launch_bounds__(256) k_shade(SceneView sc, PathBuffers pb, Sha
           direct_lighting(sc, h, rng, dl);
           if (dl.has1 || dl.has2) {
             shadow = true;
+            ray1 = dl.has1 != 0;
+            ray2 = dl.has2 != 0;
             pb.sh_o1[slot] = make_float4(dl.o1.x, dl.o1.y, dl.o1.z, dl.has1 ? 1.f : 0.f);
             pb.sh_d1[slot] = make_float4(dl.d1.x, dl.d1.y, dl.d1.z, 0.f);
             pb.sh_c1[slot] = make_float4(dl.c1.x, dl.c1.y, dl.c1.z, 0.f);
@@ -241,37 +244,49 @@ __global__ void __launch_bounds__(256) k_shade(SceneView sc, PathBuffers pb, Sha
     if (cont) pb.queue[nxt][qpos] = slot;
     const int spos = wave_append(&pb.counts[2], shadow);
     if (shadow) pb.shadow_queue[spos] = slot;
+    // one entry per shadow ray (slot * 2 + which), so k_shadow traces one ray per lane
+    const int r1 = wave_append(&pb.counts[3], ray1);
+    if (ray1) pb.shadow_rays[r1] = 2 * slot;
+    const int r2 = wave_append(&pb.counts[3], ray2);
+    if (ray2) pb.shadow_rays[r2] = 2 * slot + 1;
   }
 }
 
+// Any-hit test of each pending shadow ray, one ray per lane; writes the
+// visibility byte of (slot, which).
 template <bool kCount>
-__global__ void __launch_bounds__(kTraceBlock) k_shadow(SceneView sc, PathBuffers pb, int msnn,
-                                                        unsigned long long* stats) {
+__global__ void HM_TRACE_BOUNDS k_shadow(SceneView sc, PathBuffers pb, unsigned long long* stats) {
   __shared__ int lds_stack[kStackLds * kTraceBlock];
   TraceStack st = make_stack(lds_stack, pb);
   TraceCounters cnt{0, 0, 0};
+  const int n = pb.counts[3];
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int e = pb.shadow_rays[i];
+    const int slot = e >> 1;
+    const float4 o = (e & 1) ? pb.sh_o2[slot] : pb.sh_o1[slot];
+    const float4 d = (e & 1) ? pb.sh_d2[slot] : pb.sh_d1[slot];
+    if (kCount) cnt.rays++;
+    pb.sh_vis[e] = occluded<kCount>(sc, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), st, cnt) ? 0 : 1;
+  }
+  if (kCount) flush_counters(cnt, &stats[2], &stats[3], &stats[5]);
+}
+
+// Adds each path's visible direct light (directLighting's sum with its NaN
+// scrub, optix_common.cuh:279-309) times the throughput at the vertex.
+__global__ void k_shadow_resolve(PathBuffers pb, int msnn) {
   const int n = pb.counts[2];
   const int stride = gridDim.x * blockDim.x;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const int slot = pb.shadow_queue[i];
     V3 sum = v3(0.f);
-    const float4 o1 = pb.sh_o1[slot];
-    if (o1.w != 0.f) {
-      if (kCount) cnt.rays++;
-      const float4 d1 = pb.sh_d1[slot];
-      if (!occluded<kCount>(sc, v3(o1.x, o1.y, o1.z), v3(d1.x, d1.y, d1.z), st, cnt)) {
-        const float4 c = pb.sh_c1[slot];
-        sum = sum + v3(c.x, c.y, c.z);
-      }
+    if (pb.sh_o1[slot].w != 0.f && pb.sh_vis[2 * slot]) {
+      const float4 c = pb.sh_c1[slot];
+      sum = sum + v3(c.x, c.y, c.z);
     }
-    const float4 o2 = pb.sh_o2[slot];
-    if (o2.w != 0.f) {
-      if (kCount) cnt.rays++;
-      const float4 d2 = pb.sh_d2[slot];
-      if (!occluded<kCount>(sc, v3(o2.x, o2.y, o2.z), v3(d2.x, d2.y, d2.z), st, cnt)) {
-        const float4 c = pb.sh_c2[slot];
-        sum = sum + v3(c.x, c.y, c.z);
-      }
+    if (pb.sh_o2[slot].w != 0.f && pb.sh_vis[2 * slot + 1]) {
+      const float4 c = pb.sh_c2[slot];
+      sum = sum + v3(c.x, c.y, c.z);
     }
     if (any_nan(sum)) sum = v3(0.f);  // optix_common.cuh:305-306
     const float4 b = pb.sh_beta[slot];
@@ -289,7 +304,6 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(SceneView sc, PathBuffer
       pb.L_short[slot] = Ls;
     }
   }
-  if (kCount) flush_counters(cnt, &stats[2], &stats[3], &stats[5]);
 }
 
 // Running sum and mean per pixel, NaN scrubbed (utils.cuh:13-35).
@@ -310,7 +324,7 @@ __global__ void k_accumulate(const float4* L, float4* accum, float4* average, in
 
 // Closest or any hit for an explicit ray list (8 floats per ray).
 template <bool kCount>
-__global__ void __launch_bounds__(kTraceBlock) k_trace_rays(SceneView sc, const float4* rays, int n, int any_hit,
+__global__ void HM_TRACE_BOUNDS k_trace_rays(SceneView sc, const float4* rays, int n, int any_hit,
                                                             int* prim_out, float* t_out, float* u_out, int* spill,
                                                             int spill_stride, int spill_levels,
                                                             unsigned long long* stats) {
@@ -373,7 +387,7 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
   int cur = 0;
   for (int depth = 0; depth <= cfg.max_vertex; ++depth) {
     const int nxt = cur ^ 1;
-    hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, stream, pb.counts, nxt, 2);
+    hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, stream, pb.counts, nxt, 2, 3);
     tic(0);
     if (count)
       hipLaunchKernelGGL(k_extend<true>, dim3(cfg.trace_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cur,
@@ -395,11 +409,10 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
     toc();
     tic(2);
     if (count)
-      hipLaunchKernelGGL(k_shadow<true>, dim3(cfg.trace_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cfg.msnn,
-                         cfg.stats);
+      hipLaunchKernelGGL(k_shadow<true>, dim3(cfg.trace_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cfg.stats);
     else
-      hipLaunchKernelGGL(k_shadow<false>, dim3(cfg.trace_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cfg.msnn,
-                         cfg.stats);
+      hipLaunchKernelGGL(k_shadow<false>, dim3(cfg.trace_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cfg.stats);
+    hipLaunchKernelGGL(k_shadow_resolve, dim3(cfg.shade_blocks), dim3(256), 0, stream, pb, cfg.msnn);
     toc();
     HM_TRY(check_launch("wavefront vertex", err));
     cur = nxt;

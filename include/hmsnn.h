@@ -100,7 +100,7 @@ int hmsnn_destroy(hmsnn_ctx* ctx);
 typedef struct hmsnn_info {
   int width, height, spp, path_v1, path_v2;
   int num_segments, num_strands;
-  int bvh_nodes, bvh_depth;
+  int bvh_nodes, bvh_depth; /* 4-wide tree that traversal reads: node count, inner-node levels */
   int env_width, env_height;
   int num_dir_lights, has_env, has_sphere, mis, env_pdf;
   float scene_scale;
