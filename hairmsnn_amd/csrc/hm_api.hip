@@ -98,6 +98,14 @@ static void fill_scene_info(const Scene& s, hmsnn_info* info) {
 
 Context::~Context() {
   if (stream) (void)hipStreamSynchronize(stream);
+  for (Flight& f : extra) {
+    if (f.stream) (void)hipStreamSynchronize(f.stream);
+    for (void* p : f.allocs) dev_free(p);
+    f.allocs.clear();
+    if (f.stream) (void)hipStreamDestroy(f.stream);
+    f.stream = nullptr;
+  }
+  if (acc_event) (void)hipEventDestroy(acc_event);
   msnn_release(msnn);
   msnn = nullptr;
   free_path_buffers();
@@ -119,15 +127,14 @@ void Context::free_path_buffers() {
   accum = average = nullptr;
 }
 
-int Context::ensure_path_buffers(bool msnn_mode, std::string& err, int min_slots) {
-  const int want = std::max(scene.width * scene.height, min_slots);
-  if (pb.ray_o && (!msnn_mode || pb_msnn) && pb_slots >= want) return kOk;
-  free_path_buffers();
-  const size_t n = (size_t)want;
-  pb_slots = want;
+// Allocates one set of path buffers for n slots (recording every allocation
+// in allocs) and clears its counters on stream st.
+static int alloc_path_buffers(PathBuffers& pb, std::vector<void*>& allocs, size_t n, bool msnn_mode,
+                              int spill_levels, int spill_stride, hipStream_t st, std::string& err) {
+  pb = PathBuffers{};
   auto A = [&](void** p, size_t bytes) -> int {
     HM_TRY(dev_alloc(p, bytes, err));
-    pb_allocs.push_back(*p);
+    allocs.push_back(*p);
     return kOk;
   };
   const size_t f4 = sizeof(float4) * n;
@@ -158,18 +165,46 @@ int Context::ensure_path_buffers(bool msnn_mode, std::string& err, int min_slots
     HM_TRY(A((void**)&pb.L_short, f4));
     HM_TRY(A((void**)&pb.sh_beta_short, f4));
   }
-  pb_msnn = msnn_mode;
   // traversal stack spill: levels beyond the LDS part, one column per thread
-  spill_levels = std::max(1, bvh.stack_bound() - kStackLds + 1);
-  pb.spill_stride = trace_blocks * kTraceBlock;
+  pb.spill_stride = spill_stride;
   pb.spill_levels = spill_levels;
-  HM_TRY(A((void**)&pb.spill, sizeof(int) * (size_t)spill_levels * pb.spill_stride));
+  HM_TRY(A((void**)&pb.spill, sizeof(int) * (size_t)spill_levels * spill_stride));
+  HM_HIP(hipMemsetAsync(pb.counts, 0, sizeof(int) * 16, st), err);
+  return kOk;
+}
+
+int Context::ensure_path_buffers(bool msnn_mode, std::string& err, int min_slots) {
+  const int want = std::max(scene.width * scene.height, min_slots);
+  if (pb.ray_o && (!msnn_mode || pb_msnn) && pb_slots >= want) return kOk;
+  free_path_buffers();
+  const size_t n = (size_t)want;
+  pb_slots = want;
+  spill_levels = std::max(1, bvh.stack_bound() - kStackLds + 1);
+  HM_TRY(alloc_path_buffers(pb, pb_allocs, n, msnn_mode, spill_levels, trace_blocks * kTraceBlock, stream, err));
+  pb_msnn = msnn_mode;
+  const size_t f4 = sizeof(float4) * n;
   HM_TRY(dev_alloc((void**)&accum, f4, err));
   HM_TRY(dev_alloc((void**)&average, f4, err));
-  HM_HIP(hipMemsetAsync(pb.counts, 0, sizeof(int) * 16, stream), err);
   HM_HIP(hipMemsetAsync(accum, 0, f4, stream), err);
   HM_HIP(hipMemsetAsync(average, 0, f4, stream), err);
   accum_id = 0;
+  return kOk;
+}
+
+// Path-tracing buffers for flights 1.. (flight 0 is pb).
+int Context::ensure_flights(std::string& err) {
+  if (!acc_event) HM_HIP(hipEventCreateWithFlags(&acc_event, hipEventDisableTiming), err);
+  const int want = scene.width * scene.height;
+  for (int f = 1; f < num_flights; ++f) {
+    Flight& fl = extra[f - 1];
+    if (!fl.stream) HM_HIP(hipStreamCreateWithFlags(&fl.stream, hipStreamNonBlocking), err);
+    if (fl.slots >= want) continue;
+    for (void* p : fl.allocs) dev_free(p);
+    fl.allocs.clear();
+    HM_TRY(alloc_path_buffers(fl.pb, fl.allocs, (size_t)want, false, std::max(1, bvh.stack_bound() - kStackLds + 1),
+                              trace_blocks * kTraceBlock, fl.stream, err));
+    fl.slots = want;
+  }
   return kOk;
 }
 
@@ -543,19 +578,44 @@ int hmsnn_pt_reset(hmsnn_ctx* ctx) {
   return HMSNN_OK;
 }
 
-static int run_pt_passes(Context& c, int passes, std::string& err) {
+// Runs passes round-robin over the flights. Every flight first waits for
+// the work already queued on the context stream; accumulations are chained
+// through acc_event so they happen in pass order (the image is the one a
+// single stream makes), and the context stream waits for the last one.
+static int run_pt_passes(Context& c, int passes, std::string& err, PassTimer* timer = nullptr) {
   HM_TRY(c.ensure_path_buffers(false, err));
+  HM_TRY(c.ensure_flights(err));
   SceneView v = c.view;
   v.bvh.nodes = c.bvh.nodes4;
-  c.pb.spill_stride = c.trace_blocks * kTraceBlock;
   const int n = c.scene.width * c.scene.height;
-  for (int k = 0; k < passes; ++k) {
+  const int K = std::max(1, std::min(c.num_flights, Context::kMaxFlights));
+  hipEvent_t start = nullptr;
+  HM_HIP(hipEventCreateWithFlags(&start, hipEventDisableTiming), err);
+  HM_HIP(hipEventRecord(start, c.stream), err);
+  for (int f = 1; f < K; ++f) HM_HIP(hipStreamWaitEvent(c.flight_stream(f), start, 0), err);
+  int rc = kOk;
+  for (int k = 0; k < passes && rc == kOk; ++k) {
+    const int f = c.accum_id % K;
+    PathBuffers& pb = c.flight_pb(f);
+    hipStream_t st = c.flight_stream(f);
     PassConfig cfg = c.pt_pass_config();
-    HM_TRY(pt_run_pass(v, c.pb, cfg, c.stream, err));
-    HM_TRY(pt_accumulate(c.pb.L, c.accum, c.average, n, c.accum_id, c.stream, err));
+    cfg.timer = timer;
+    rc = pt_run_pass(v, pb, cfg, st, err);
+    if (rc) break;
+    if (c.acc_pending && hipStreamWaitEvent(st, c.acc_event, 0) != hipSuccess) {
+      rc = kEDevice;
+      err = "stream wait failed";
+      break;
+    }
+    rc = pt_accumulate(pb.L, c.accum, c.average, n, c.accum_id, st, err);
+    if (rc) break;
+    (void)hipEventRecord(c.acc_event, st);
+    c.acc_pending = true;
     c.accum_id++;
   }
-  return kOk;
+  if (c.acc_pending) (void)hipStreamWaitEvent(c.stream, c.acc_event, 0);
+  (void)hipEventDestroy(start);
+  return rc;
 }
 
 int hmsnn_pt_passes(hmsnn_ctx* ctx, int passes) {
@@ -590,9 +650,11 @@ int hmsnn_get_image(hmsnn_ctx* ctx, int which, float* rgba) {
 int hmsnn_get_last_pass(hmsnn_ctx* ctx, float* rgba) {
   if (!ctx || !rgba) return fail(HMSNN_EINVAL, "hmsnn_get_last_pass: bad arguments");
   Context& c = ctx->c;
-  if (!c.pb.L) return fail(HMSNN_EINVAL, "nothing rendered yet");
+  if (!c.pb.L || c.accum_id == 0) return fail(HMSNN_EINVAL, "nothing rendered yet");
+  const int K = std::max(1, std::min(c.num_flights, Context::kMaxFlights));
+  const PathBuffers& last = c.flight_pb((c.accum_id - 1) % K);
   const size_t bytes = sizeof(float4) * (size_t)c.scene.width * c.scene.height;
-  hipError_t e = hipMemcpyAsync(rgba, c.pb.L, bytes, hipMemcpyDeviceToHost, c.stream);
+  hipError_t e = hipMemcpyAsync(rgba, last.L, bytes, hipMemcpyDeviceToHost, c.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
   return e == hipSuccess ? HMSNN_OK : fail(HMSNN_EDEVICE, std::string("get last pass: ") + hipGetErrorString(e));
 }
@@ -653,14 +715,7 @@ int hmsnn_pt_time_passes(hmsnn_ctx* ctx, int passes, double out[5]) {
   (void)hipEventCreate(&t0);
   (void)hipEventCreate(&t1);
   (void)hipEventRecord(t0, c.stream);
-  const int n = c.scene.width * c.scene.height;
-  for (int p = 0; p < passes && rc == 0; ++p) {
-    PassConfig cfg = c.pt_pass_config();
-    cfg.timer = &tm;
-    rc = pt_run_pass(c.view, c.pb, cfg, c.stream, err);
-    if (rc == 0) rc = pt_accumulate(c.pb.L, c.accum, c.average, n, c.accum_id, c.stream, err);
-    c.accum_id++;
-  }
+  rc = run_pt_passes(c, passes, err, &tm);
   (void)hipEventRecord(t1, c.stream);
   hipError_t e = hipEventSynchronize(t1);
   double sums[3] = {0, 0, 0};

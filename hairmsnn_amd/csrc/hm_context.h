@@ -17,6 +17,16 @@ struct MlpState;  // radiance-cache network (hm_mlp.hip)
 struct MsnnState;  // HairMSNN frame state (hm_msnn.hip)
 void msnn_release(MsnnState* m);
 
+// An extra set of path buffers with its own stream, so that independent
+// passes overlap on the GPU (the last depths of a pass launch kernels whose
+// duration is set by their slowest ray, not by their ray count).
+struct Flight {
+  hipStream_t stream = nullptr;
+  PathBuffers pb{};
+  std::vector<void*> allocs;
+  int slots = 0;
+};
+
 struct Context {
   Scene scene;
   hipStream_t stream = nullptr;
@@ -52,6 +62,15 @@ struct Context {
   unsigned long long* stats_dev = nullptr;
 
   MsnnState* msnn = nullptr;
+  // path-tracing passes in flight at once: flight 0 is (pb, stream)
+  static constexpr int kMaxFlights = 4;
+  int num_flights = 3;
+  Flight extra[kMaxFlights - 1];
+  hipEvent_t acc_event = nullptr;  // last accumulate issued (any flight)
+  bool acc_pending = false;
+  int ensure_flights(std::string& err);
+  PathBuffers& flight_pb(int f) { return f == 0 ? pb : extra[f - 1].pb; }
+  hipStream_t flight_stream(int f) { return f == 0 ? stream : extra[f - 1].stream; }
   // HairMSNN scene samples on the hair (3 floats each), made once per scene
   float* scene_points = nullptr;
   int num_scene_points = 0;

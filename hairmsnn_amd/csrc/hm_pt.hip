@@ -12,6 +12,8 @@
 // read the live count from device memory, so a pass needs no host round trip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "hm_pt.h"
 #include "hm_pt_kernels.h"
 
@@ -384,16 +386,22 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
       tm->used++;
     }
   };
+  // Grids sized to the pass's paths (queues only shrink with depth), so a
+  // small training-data pass does not dispatch the full-frame grid per depth.
+  const int n0 = cfg.camera_mode == 1 ? cfg.n_slots : sc.width * sc.height;
+  const int ext_blocks = std::min(cfg.trace_blocks, std::max(1, (n0 + kTraceBlock - 1) / kTraceBlock));
+  const int sh_blocks = std::min(cfg.trace_blocks, std::max(1, (2 * n0 + kTraceBlock - 1) / kTraceBlock));
+  const int shade_blocks = std::min(cfg.shade_blocks, std::max(1, (n0 + 255) / 256));
   int cur = 0;
   for (int depth = 0; depth <= cfg.max_vertex; ++depth) {
     const int nxt = cur ^ 1;
     hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, stream, pb.counts, nxt, 2, 3);
     tic(0);
     if (count)
-      hipLaunchKernelGGL(k_extend<true>, dim3(cfg.trace_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cur,
+      hipLaunchKernelGGL(k_extend<true>, dim3(ext_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cur,
                          cfg.stats);
     else
-      hipLaunchKernelGGL(k_extend<false>, dim3(cfg.trace_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cur,
+      hipLaunchKernelGGL(k_extend<false>, dim3(ext_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cur,
                          cfg.stats);
     toc();
     ShadeParams sp;
@@ -405,14 +413,14 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
     sp.msnn = cfg.msnn;
     sp.cur = cur;
     tic(1);
-    hipLaunchKernelGGL(k_shade, dim3(cfg.shade_blocks), dim3(256), 0, stream, sc, pb, sp, cfg.gb);
+    hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(256), 0, stream, sc, pb, sp, cfg.gb);
     toc();
     tic(2);
     if (count)
-      hipLaunchKernelGGL(k_shadow<true>, dim3(cfg.trace_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cfg.stats);
+      hipLaunchKernelGGL(k_shadow<true>, dim3(sh_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cfg.stats);
     else
-      hipLaunchKernelGGL(k_shadow<false>, dim3(cfg.trace_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cfg.stats);
-    hipLaunchKernelGGL(k_shadow_resolve, dim3(cfg.shade_blocks), dim3(256), 0, stream, pb, cfg.msnn);
+      hipLaunchKernelGGL(k_shadow<false>, dim3(sh_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cfg.stats);
+    hipLaunchKernelGGL(k_shadow_resolve, dim3(shade_blocks), dim3(256), 0, stream, pb, cfg.msnn);
     toc();
     HM_TRY(check_launch("wavefront vertex", err));
     cur = nxt;
