@@ -153,6 +153,7 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-msnn", action="store_true", help="skip the HairMSNN vs PT time-to-equal-RMSE leg")
     ap.add_argument("--ref-spp", type=int, default=1024, help="passes of the RMSE reference image")
+    ap.add_argument("--pass-concurrency", type=int, default=3, help="path-tracing passes in flight in the timed region")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="instead of the benchmark, run 8 device-to-device copies of this many MiB (the known-size "
                          "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -195,6 +196,7 @@ def main() -> None:
     stats = sc.trace_stats(1)
     log(f"trace stats (1 pass): {stats}")
     # warmup
+    sc.set_pass_concurrency(args.pass_concurrency)
     sc.pt_reset()
     sc.pt_passes(args.warmup)
     # timed region: K passes, synchronised on both sides
@@ -206,9 +208,12 @@ def main() -> None:
     value = paths_per_step * args.steps / wall
     log(f"{args.steps} passes in {wall:.3f} s -> {value / 1e6:.2f} M paths/s")
 
-    # per-kernel timing with HIP events on the context's stream
-    kt = sc.time_passes(max(2, min(8, args.steps)))
+    # per-kernel timing with HIP events on the stream each kernel runs on, with
+    # passes back to back so every launch runs alone (its roofline)
+    sc.set_pass_concurrency(1)
     passes_t = max(2, min(8, args.steps))
+    kt = sc.time_passes(passes_t)
+    sc.set_pass_concurrency(args.pass_concurrency)
     launches = max(kt["extend_launches"], 1)
     extend_ms = kt["extend_ms"] / launches
     bytes_pass = (stats["closest_nodes"] * NODE_BYTES + stats["closest_segs"] * SEG_BYTES

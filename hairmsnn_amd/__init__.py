@@ -244,6 +244,10 @@ class Scene:
         """Offsets the pass index behind every pixel's random stream (0 = the reference's)."""
         check(self.lib.hmsnn_set_seed_offset(self.h, int(offset)))
 
+    def set_pass_concurrency(self, passes_in_flight: int) -> None:
+        """Path-tracing passes in flight at once (1..4); the image does not depend on it."""
+        check(self.lib.hmsnn_set_pass_concurrency(self.h, int(passes_in_flight)))
+
     # ---- path tracing ----
     def pt_reset(self) -> None:
         check(self.lib.hmsnn_pt_reset(self.h))

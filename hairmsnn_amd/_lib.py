@@ -81,6 +81,7 @@ _SIGS = {
     "hmsnn_export_env": (_I, [_P, _P, _P, _P, _P, _P]),
     "hmsnn_set_strand_sigma_a": (_I, [_P, _P, _I]),
     "hmsnn_set_seed_offset": (_I, [_P, _I]),
+    "hmsnn_set_pass_concurrency": (_I, [_P, _I]),
     "hmsnn_pt_reset": (_I, [_P]),
     "hmsnn_pt_passes": (_I, [_P, _I]),
     "hmsnn_ctx_synchronize": (_I, [_P]),

@@ -129,7 +129,7 @@ void Context::free_path_buffers() {
 
 // Allocates one set of path buffers for n slots (recording every allocation
 // in allocs) and clears its counters on stream st.
-static int alloc_path_buffers(PathBuffers& pb, std::vector<void*>& allocs, size_t n, bool msnn_mode,
+int alloc_path_buffers(PathBuffers& pb, std::vector<void*>& allocs, size_t n, bool msnn_mode,
                               int spill_levels, int spill_stride, hipStream_t st, std::string& err) {
   pb = PathBuffers{};
   auto A = [&](void** p, size_t bytes) -> int {
@@ -563,6 +563,15 @@ int hmsnn_set_strand_sigma_a(hmsnn_ctx* ctx, const float* sigma, int num_strands
   if (e != hipSuccess) return fail(HMSNN_EDEVICE, hipGetErrorString(e));
   (void)hipStreamSynchronize(c.stream);
   c.view.strand_sigma = c.strand_sigma_dev;
+  return HMSNN_OK;
+}
+
+int hmsnn_set_pass_concurrency(hmsnn_ctx* ctx, int passes_in_flight) {
+  if (!ctx || passes_in_flight < 1 || passes_in_flight > Context::kMaxFlights)
+    return fail(HMSNN_EINVAL, "hmsnn_set_pass_concurrency: expects 1..4");
+  Context& c = ctx->c;
+  (void)hipStreamSynchronize(c.stream);  // every flight's work is behind the context stream
+  c.num_flights = passes_in_flight;
   return HMSNN_OK;
 }
 

@@ -27,6 +27,11 @@ struct Flight {
   int slots = 0;
 };
 
+// One set of path buffers for n slots (msnn_mode adds the training-path
+// arrays); every allocation is appended to allocs, counters cleared on st.
+int alloc_path_buffers(PathBuffers& pb, std::vector<void*>& allocs, size_t n, bool msnn_mode, int spill_levels,
+                       int spill_stride, hipStream_t st, std::string& err);
+
 struct Context {
   Scene scene;
   hipStream_t stream = nullptr;

@@ -26,6 +26,41 @@
 
 namespace hm {
 
+// Per-frame state of one pipeline slot: its path buffers and G-buffer /
+// training arrays, and the stream its path-tracing passes run on. Consecutive
+// frames (and warm-up steps) use consecutive slots, so their passes overlap;
+// training, inference and compositing run in frame order on the context
+// stream.
+struct MsnnSlot {
+  PathBuffers pb{};
+  std::vector<void*> allocs;
+  hipStream_t st = nullptr;
+  float* nn_input = nullptr;     // W*H*12
+  uint8_t* hit_flags = nullptr;  // W*H
+  float4* short_color = nullptr; // W*H
+  float* train_input = nullptr;  // n_train*12
+  float* train_target = nullptr; // n_train*3
+  uint8_t* train_hit = nullptr;  // n_train
+  int* train_idx = nullptr;      // n_train
+  float* gen_points = nullptr;   // points of one training-data pass, n_train*3
+  int* idx_host = nullptr;       // pinned staging of train_idx
+  hipEvent_t ev_gb = nullptr;    // the slot's pass and finalize are done
+  hipEvent_t ev_done = nullptr;  // the context stream has consumed the slot
+  hipEvent_t ev_copy = nullptr;  // idx_host has been read
+  bool used = false;
+  void release() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (void* p : allocs) dev_free(p);
+    allocs.clear();
+    if (idx_host) (void)hipHostFree(idx_host);
+    idx_host = nullptr;
+    for (hipEvent_t* e : {&ev_gb, &ev_done, &ev_copy})
+      if (*e) (void)hipEventDestroy(*e), *e = nullptr;
+    if (st) (void)hipStreamDestroy(st);
+    st = nullptr;
+  }
+};
+
 struct MsnnState {
   MlpState* mlp = nullptr;
   int BETA = 1;
@@ -38,32 +73,21 @@ struct MsnnState {
   int train_steps = 0;
   int frame = 0;       // accumulated frames (the reference's accumId)
   int gen_id = 0;      // accumId used by the warm-up passes
+  int next_slot = 0;
   uint64_t rng_state = 0;
   double setup_ms = 0.0, warmup_ms = 0.0;  // host wall clock of msnn_begin's two parts
-  // device buffers
-  float* nn_input = nullptr;     // W*H*12
-  uint8_t* hit_flags = nullptr;  // W*H
-  float4* short_color = nullptr; // W*H
-  float* nn_output = nullptr;    // W*H*3
-  float* train_input = nullptr;  // n_train*12
-  float* train_target = nullptr; // n_train*3
-  uint8_t* train_hit = nullptr;  // n_train
-  int* train_idx = nullptr;      // n_train
-  float* gen_points = nullptr;   // points of one training-data pass, n_train*3
+  static constexpr int kSlots = 3;
+  MsnnSlot slots[kSlots];
+  float* nn_output = nullptr;    // W*H*3 (used in frame order on the context stream)
   float4* acc[3] = {nullptr, nullptr, nullptr};  // short, network, final
   float4* avg[3] = {nullptr, nullptr, nullptr};
   double loss_last = 0.0;
+  hipStream_t ctx_stream = nullptr;
   ~MsnnState() {
+    if (ctx_stream) (void)hipStreamSynchronize(ctx_stream);
+    for (MsnnSlot& sl : slots) sl.release();
     if (mlp) mlp_destroy(mlp);
-    dev_free(nn_input);
-    dev_free(hit_flags);
-    dev_free(short_color);
     dev_free(nn_output);
-    dev_free(train_input);
-    dev_free(train_target);
-    dev_free(train_hit);
-    dev_free(train_idx);
-    dev_free(gen_points);
     for (int k = 0; k < 3; ++k) {
       dev_free(acc[k]);
       dev_free(avg[k]);
@@ -74,6 +98,11 @@ struct MsnnState {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
+  }
+  MsnnSlot& take_slot() {
+    MsnnSlot& sl = slots[next_slot];
+    next_slot = (next_slot + 1) % kSlots;
+    return sl;
   }
 };
 
@@ -228,22 +257,39 @@ int msnn_setup(Context& c, int BETA, std::string& err) {
   m->rng_state = s.msnn_seed * 0x2545F4914F6CDD1Dull + 1;
   const int prec = s.msnn_precision == "bf16" ? HMSNN_BF16 : s.msnn_precision == "fp8" ? HMSNN_FP8 : HMSNN_FP16;
   HM_TRY(mlp_create(prec, s.msnn_seed, &m->mlp, err));
-  HM_TRY(dev_alloc((void**)&m->nn_input, sizeof(float) * 12 * (size_t)npx, err));
-  HM_TRY(dev_alloc((void**)&m->hit_flags, (size_t)npx, err));
-  HM_TRY(dev_alloc((void**)&m->short_color, sizeof(float4) * (size_t)npx, err));
+  m->ctx_stream = c.stream;
+  const int spill_levels = std::max(1, c.bvh.stack_bound() - kStackLds + 1);
+  for (MsnnSlot& sl : m->slots) {
+    HM_HIP(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking), err);
+    for (hipEvent_t* e : {&sl.ev_gb, &sl.ev_done, &sl.ev_copy})
+      HM_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming), err);
+    HM_TRY(alloc_path_buffers(sl.pb, sl.allocs, (size_t)std::max(npx, m->n_train), true, spill_levels,
+                              c.trace_blocks * kTraceBlock, sl.st, err));
+    auto A = [&](void** p, size_t bytes) -> int {
+      HM_TRY(dev_alloc(p, bytes, err));
+      sl.allocs.push_back(*p);
+      return kOk;
+    };
+    HM_TRY(A((void**)&sl.nn_input, sizeof(float) * 12 * (size_t)npx));
+    HM_TRY(A((void**)&sl.hit_flags, (size_t)npx));
+    HM_TRY(A((void**)&sl.short_color, sizeof(float4) * (size_t)npx));
+    HM_TRY(A((void**)&sl.train_input, sizeof(float) * 12 * (size_t)m->n_train));
+    HM_TRY(A((void**)&sl.train_target, sizeof(float) * 3 * (size_t)m->n_train));
+    HM_TRY(A((void**)&sl.train_hit, (size_t)m->n_train));
+    HM_TRY(A((void**)&sl.train_idx, sizeof(int) * (size_t)m->n_train));
+    HM_TRY(A((void**)&sl.gen_points, sizeof(float) * 3 * (size_t)m->n_train));
+    HM_HIP(hipHostMalloc((void**)&sl.idx_host, sizeof(int) * (size_t)m->n_train), err);
+  }
   HM_TRY(dev_alloc((void**)&m->nn_output, sizeof(float) * 3 * (size_t)npx, err));
-  HM_TRY(dev_alloc((void**)&m->train_input, sizeof(float) * 12 * (size_t)m->n_train, err));
-  HM_TRY(dev_alloc((void**)&m->train_target, sizeof(float) * 3 * (size_t)m->n_train, err));
-  HM_TRY(dev_alloc((void**)&m->train_hit, (size_t)m->n_train, err));
-  HM_TRY(dev_alloc((void**)&m->train_idx, sizeof(int) * (size_t)m->n_train, err));
-  HM_TRY(dev_alloc((void**)&m->gen_points, sizeof(float) * 3 * (size_t)m->n_train, err));
   for (int k = 0; k < 3; ++k) {
     HM_TRY(dev_alloc((void**)&m->acc[k], sizeof(float4) * (size_t)npx, err));
     HM_TRY(dev_alloc((void**)&m->avg[k], sizeof(float4) * (size_t)npx, err));
     HM_HIP(hipMemsetAsync(m->avg[k], 0, sizeof(float4) * (size_t)npx, c.stream), err);
   }
+  // the slots' first passes start after the set-up work on the context stream
+  HM_HIP(hipStreamSynchronize(c.stream), err);
+  for (MsnnSlot& sl : m->slots) HM_HIP(hipStreamSynchronize(sl.st), err);
   HM_TRY(ensure_scene_points(c, s.msnn_seed, err));
-  HM_TRY(c.ensure_path_buffers(true, err, m->n_train));
   return kOk;
 }
 
@@ -265,10 +311,18 @@ __global__ void k_sample_points(const float* pts, int num_points, uint64_t seed,
   out[3 * (size_t)i + 2] = pts[3 * (size_t)p + 2];
 }
 
-int sample_gen_points(Context& c, MsnnState& m, std::string& err) {
-  hipLaunchKernelGGL(k_sample_points, dim3((m.n_train + 255) / 256), dim3(256), 0, c.stream, c.scene_points,
-                     c.num_scene_points, m.next_u64(), m.n_train, m.gen_points);
+int sample_gen_points(Context& c, MsnnState& m, MsnnSlot& sl, std::string& err) {
+  hipLaunchKernelGGL(k_sample_points, dim3((m.n_train + 255) / 256), dim3(256), 0, sl.st, c.scene_points,
+                     c.num_scene_points, m.next_u64(), m.n_train, sl.gen_points);
   return check_launch("msnn sample points", err);
+}
+
+// The slot's stream waits until the context stream has consumed the slot's
+// previous contents.
+int acquire_slot(MsnnSlot& sl, std::string& err) {
+  if (sl.used) HM_HIP(hipStreamWaitEvent(sl.st, sl.ev_done, 0), err);
+  sl.used = true;
+  return kOk;
 }
 
 PassConfig msnn_pass_config(const Context& c, const MsnnState& m) {
@@ -281,36 +335,41 @@ PassConfig msnn_pass_config(const Context& c, const MsnnState& m) {
   return cfg;
 }
 
-int train_step(Context& c, MsnnState& m, std::string& err, float* loss_out) {
+int train_step(Context& c, MsnnState& m, const float* input, const float* target, std::string& err,
+               float* loss_out) {
   const float lr = lr_at(m.lr, m.train_steps);
-  HM_TRY(mlp_loss_grad_step(*m.mlp, m.train_input, m.train_target, m.n_train, m.loss, nullptr, loss_out, true, lr,
-                            c.stream, err));
+  HM_TRY(mlp_loss_grad_step(*m.mlp, input, target, m.n_train, m.loss, nullptr, loss_out, true, lr, c.stream, err));
   m.train_steps++;
   return kOk;
 }
 
 // Warm-up: training-data passes toward sampled hair points, each followed by
-// one training step (render_hair_msnn.cu:633-665).
+// one training step (render_hair_msnn.cu:633-665). The passes do not depend
+// on the network, so consecutive ones run on consecutive slots and overlap;
+// the training steps run in order on the context stream.
 int msnn_warmup(Context& c, MsnnState& m, int steps, std::string& err) {
   for (int k = 0; k < steps; ++k) {
-    HM_TRY(sample_gen_points(c, m, err));
+    MsnnSlot& sl = m.take_slot();
+    HM_TRY(acquire_slot(sl, err));
+    HM_TRY(sample_gen_points(c, m, sl, err));
     PassConfig cfg = msnn_pass_config(c, m);
     cfg.camera_mode = 1;
     cfg.n_slots = m.n_train;
     cfg.grid_w = m.grid_w;
-    cfg.points = m.gen_points;
+    cfg.points = sl.gen_points;
     cfg.accum_id = m.gen_id++;
     cfg.gb.enabled = 1;
-    cfg.gb.nn_input = m.train_input;
-    cfg.gb.hit_flags = m.train_hit;
+    cfg.gb.nn_input = sl.train_input;
+    cfg.gb.hit_flags = sl.train_hit;
     cfg.gb.scene_scale = c.scene_scale;
-    HM_TRY(pt_run_pass(c.view, c.pb, cfg, c.stream, err));
-    hipLaunchKernelGGL(k_msnn_finalize, dim3((m.n_train + 255) / 256), dim3(256), 0, c.stream, c.pb, m.n_train, 1, 1,
-                       m.n_train, m.train_input, m.train_hit, m.train_input, m.train_target, (float4*)nullptr);
+    HM_TRY(pt_run_pass(c.view, sl.pb, cfg, sl.st, err));
+    hipLaunchKernelGGL(k_msnn_finalize, dim3((m.n_train + 255) / 256), dim3(256), 0, sl.st, sl.pb, m.n_train, 1, 1,
+                       m.n_train, sl.train_input, sl.train_hit, sl.train_input, sl.train_target, (float4*)nullptr);
     HM_TRY(check_launch("msnn finalize (warm-up)", err));
-    float l = 0.f;
-    HM_TRY(train_step(c, m, err, &l));
-    m.loss_last = l;
+    HM_HIP(hipEventRecord(sl.ev_gb, sl.st), err);
+    HM_HIP(hipStreamWaitEvent(c.stream, sl.ev_gb, 0), err);
+    HM_TRY(train_step(c, m, sl.train_input, sl.train_target, err, nullptr));
+    HM_HIP(hipEventRecord(sl.ev_done, c.stream), err);
   }
   return kOk;
 }
@@ -320,6 +379,8 @@ struct EventTimer {
   EventTimer() {
     for (auto& x : e) (void)hipEventCreate(&x);
   }
+  EventTimer(const EventTimer&) = delete;
+  EventTimer& operator=(const EventTimer&) = delete;
   ~EventTimer() {
     for (auto& x : e) (void)hipEventDestroy(x);
   }
@@ -330,53 +391,53 @@ struct EventTimer {
   }
 };
 
-int msnn_frame(Context& c, MsnnState& m, hmsnn_msnn_stats* st, std::string& err) {
+// One frame: G_BUFFER pass + finalize on the frame's slot, then (in frame
+// order on the context stream) training, inference and the RENDER
+// composite (render_hair_msnn.cu:699-771). tm, when given, receives events:
+// [0], [1] around the slot's pass; [2]..[5] on the context stream.
+int msnn_frame(Context& c, MsnnState& m, EventTimer* tm, std::string& err) {
   const int npx = c.scene.width * c.scene.height;
+  MsnnSlot& sl = m.take_slot();
+  HM_TRY(acquire_slot(sl, err));
   // shuffle of the training-pixel indices (render_hair_msnn.cu:712-714)
-  std::vector<int> idx(m.n_train);
-  std::iota(idx.begin(), idx.end(), 0);
+  HM_HIP(hipEventSynchronize(sl.ev_copy), err);
+  int* idx = sl.idx_host;
+  for (int i = 0; i < m.n_train; ++i) idx[i] = i;
   for (int i = m.n_train - 1; i > 0; --i) std::swap(idx[i], idx[(int)(m.next_u64() % (uint64_t)(i + 1))]);
-  HM_HIP(hipMemcpyAsync(m.train_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, c.stream), err);
-  EventTimer tm;
-  (void)hipEventRecord(tm.e[0], c.stream);
+  HM_HIP(hipMemcpyAsync(sl.train_idx, idx, sizeof(int) * (size_t)m.n_train, hipMemcpyHostToDevice, sl.st), err);
+  HM_HIP(hipEventRecord(sl.ev_copy, sl.st), err);
+  if (tm) (void)hipEventRecord(tm->e[0], sl.st);
   // ---- G_BUFFER ----
   PassConfig cfg = msnn_pass_config(c, m);
   cfg.accum_id = m.frame;
-  cfg.train_idx = m.train_idx;
+  cfg.train_idx = sl.train_idx;
   cfg.every_nth = m.every_nth;
   cfg.n_slots = m.n_train;
   cfg.gb.enabled = 1;
-  cfg.gb.nn_input = m.nn_input;
-  cfg.gb.hit_flags = m.hit_flags;
+  cfg.gb.nn_input = sl.nn_input;
+  cfg.gb.hit_flags = sl.hit_flags;
   cfg.gb.scene_scale = c.scene_scale;
-  HM_TRY(pt_run_pass(c.view, c.pb, cfg, c.stream, err));
-  hipLaunchKernelGGL(k_msnn_finalize, dim3((npx + 255) / 256), dim3(256), 0, c.stream, c.pb, npx, 0, m.every_nth,
-                     m.n_train, m.nn_input, m.hit_flags, m.train_input, m.train_target, m.short_color);
+  HM_TRY(pt_run_pass(c.view, sl.pb, cfg, sl.st, err));
+  hipLaunchKernelGGL(k_msnn_finalize, dim3((npx + 255) / 256), dim3(256), 0, sl.st, sl.pb, npx, 0, m.every_nth,
+                     m.n_train, sl.nn_input, sl.hit_flags, sl.train_input, sl.train_target, sl.short_color);
   HM_TRY(check_launch("msnn finalize", err));
-  (void)hipEventRecord(tm.e[1], c.stream);
+  if (tm) (void)hipEventRecord(tm->e[1], sl.st);
+  HM_HIP(hipEventRecord(sl.ev_gb, sl.st), err);
+  HM_HIP(hipStreamWaitEvent(c.stream, sl.ev_gb, 0), err);
+  if (tm) (void)hipEventRecord(tm->e[2], c.stream);
   // ---- train ----
-  float l = 0.f;
-  if (c.scene.nrc_train) HM_TRY(train_step(c, m, err, nullptr));
-  (void)hipEventRecord(tm.e[2], c.stream);
+  if (c.scene.nrc_train) HM_TRY(train_step(c, m, sl.train_input, sl.train_target, err, nullptr));
+  if (tm) (void)hipEventRecord(tm->e[3], c.stream);
   // ---- infer ----
-  HM_TRY(mlp_forward(*m.mlp, m.nn_input, npx, m.nn_output, c.stream, err));
-  (void)hipEventRecord(tm.e[3], c.stream);
+  HM_TRY(mlp_forward(*m.mlp, sl.nn_input, npx, m.nn_output, c.stream, err));
+  if (tm) (void)hipEventRecord(tm->e[4], c.stream);
   // ---- RENDER ----
-  hipLaunchKernelGGL(k_msnn_composite, dim3((npx + 255) / 256), dim3(256), 0, c.stream, npx, m.hit_flags,
-                     m.short_color, m.nn_output, m.acc[0], m.acc[1], m.acc[2], m.avg[0], m.avg[1], m.avg[2], m.frame);
+  hipLaunchKernelGGL(k_msnn_composite, dim3((npx + 255) / 256), dim3(256), 0, c.stream, npx, sl.hit_flags,
+                     sl.short_color, m.nn_output, m.acc[0], m.acc[1], m.acc[2], m.avg[0], m.avg[1], m.avg[2], m.frame);
   HM_TRY(check_launch("msnn composite", err));
-  (void)hipEventRecord(tm.e[4], c.stream);
+  if (tm) (void)hipEventRecord(tm->e[5], c.stream);
+  HM_HIP(hipEventRecord(sl.ev_done, c.stream), err);
   m.frame++;
-  if (st) {
-    HM_HIP(hipEventSynchronize(tm.e[4]), err);
-    st->gbuffer_ms += EventTimer::ms(tm.e[0], tm.e[1]);
-    st->train_ms += EventTimer::ms(tm.e[1], tm.e[2]);
-    st->infer_ms += EventTimer::ms(tm.e[2], tm.e[3]);
-    st->render_ms += EventTimer::ms(tm.e[3], tm.e[4]);
-    st->total_ms += EventTimer::ms(tm.e[0], tm.e[4]);
-    st->frames += 1;
-  }
-  (void)l;
   return kOk;
 }
 
@@ -416,14 +477,23 @@ int hmsnn_msnn_frames(hmsnn_ctx* ctx, int frames, hmsnn_msnn_stats* stats) {
   Context& c = ctx->c;
   if (!c.msnn) return fail(HMSNN_EINVAL, "call hmsnn_msnn_begin first");
   std::string err;
-  hmsnn_msnn_stats local{};
+  std::vector<EventTimer> timers(stats ? (size_t)frames : 0);
   for (int k = 0; k < frames; ++k) {
-    int rc = msnn_frame(c, *c.msnn, stats ? &local : nullptr, err);
+    int rc = msnn_frame(c, *c.msnn, stats ? &timers[k] : nullptr, err);
     if (rc) return fail(rc, err);
   }
   hipError_t e = hipStreamSynchronize(c.stream);
   if (e != hipSuccess) return fail(HMSNN_EDEVICE, hipGetErrorString(e));
   if (stats) {
+    hmsnn_msnn_stats local{};
+    for (EventTimer& tm : timers) {
+      local.gbuffer_ms += EventTimer::ms(tm.e[0], tm.e[1]);
+      local.train_ms += EventTimer::ms(tm.e[2], tm.e[3]);
+      local.infer_ms += EventTimer::ms(tm.e[3], tm.e[4]);
+      local.render_ms += EventTimer::ms(tm.e[4], tm.e[5]);
+    }
+    if (frames > 0) local.total_ms = EventTimer::ms(timers[0].e[0], timers[frames - 1].e[5]);
+    local.frames = frames;
     local.loss = c.msnn->loss_last;
     local.train_steps = c.msnn->train_steps;
     local.setup_ms = c.msnn->setup_ms;
