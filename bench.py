@@ -153,7 +153,7 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-msnn", action="store_true", help="skip the HairMSNN vs PT time-to-equal-RMSE leg")
     ap.add_argument("--ref-spp", type=int, default=1024, help="passes of the RMSE reference image")
-    ap.add_argument("--pass-concurrency", type=int, default=3, help="path-tracing passes in flight in the timed region")
+    ap.add_argument("--pass-concurrency", type=int, default=4, help="path-tracing passes in flight in the timed region")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="instead of the benchmark, run 8 device-to-device copies of this many MiB (the known-size "
                          "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")

@@ -69,7 +69,7 @@ struct Context {
   MsnnState* msnn = nullptr;
   // path-tracing passes in flight at once: flight 0 is (pb, stream)
   static constexpr int kMaxFlights = 4;
-  int num_flights = 3;
+  int num_flights = 4;
   Flight extra[kMaxFlights - 1];
   hipEvent_t acc_event = nullptr;  // last accumulate issued (any flight)
   bool acc_pending = false;

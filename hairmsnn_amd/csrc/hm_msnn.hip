@@ -61,6 +61,32 @@ struct MsnnSlot {
   }
 };
 
+// Training paths of up to kWarmBatch consecutive frames, traced as one pass
+// (they do not depend on the network). Double-buffered: a batch is filled
+// while the previous one is being consumed.
+struct TrainBatch {
+  PathBuffers pb{};
+  std::vector<void*> allocs;
+  int* idx_dev = nullptr;       // shuffled training indices, frame-major
+  int* idx_host = nullptr;      // pinned staging of idx_dev
+  int* pix = nullptr;           // pixel of each record
+  float* nn_input = nullptr;    // G-buffer record of each training path
+  uint8_t* hit = nullptr;
+  float* train_input = nullptr;
+  float* train_target = nullptr;
+  float4* full = nullptr;       // full-path colour (the training pixel's G-buffer colour)
+  hipEvent_t ev_ready = nullptr, ev_free = nullptr, ev_copy = nullptr;
+  bool used = false;
+  void release() {
+    for (void* p : allocs) dev_free(p);
+    allocs.clear();
+    if (idx_host) (void)hipHostFree(idx_host);
+    idx_host = nullptr;
+    for (hipEvent_t* e : {&ev_ready, &ev_free, &ev_copy})
+      if (*e) (void)hipEventDestroy(*e), *e = nullptr;
+  }
+};
+
 struct MsnnState {
   MlpState* mlp = nullptr;
   int BETA = 1;
@@ -77,7 +103,12 @@ struct MsnnState {
   uint64_t rng_state = 0;
   double setup_ms = 0.0, warmup_ms = 0.0;  // host wall clock of msnn_begin's two parts
   static constexpr int kSlots = 3;
+  // warm-up training-data passes batched per path-tracing pass (the passes do
+  // not depend on the network); slot arrays hold this many batches
+  static constexpr int kWarmBatch = 8;
+  int warm_batch = 1;
   MsnnSlot slots[kSlots];
+  TrainBatch tb[2];
   float* nn_output = nullptr;    // W*H*3 (used in frame order on the context stream)
   float4* acc[3] = {nullptr, nullptr, nullptr};  // short, network, final
   float4* avg[3] = {nullptr, nullptr, nullptr};
@@ -86,6 +117,7 @@ struct MsnnState {
   ~MsnnState() {
     if (ctx_stream) (void)hipStreamSynchronize(ctx_stream);
     for (MsnnSlot& sl : slots) sl.release();
+    for (TrainBatch& b : tb) b.release();
     if (mlp) mlp_destroy(mlp);
     dev_free(nn_output);
     for (int k = 0; k < 3; ++k) {
@@ -183,6 +215,13 @@ __global__ void k_msnn_composite(int n, const uint8_t* hit_flags, const float4* 
   avg_f[i] = make_float4(col.x * inv, col.y * inv, col.z * inv, 1.f);
 }
 
+// A training pixel's G-buffer colour is its full path (the reference keeps
+// the training path's colour there, cuda/hair_msnn.cu:270-279).
+__global__ void k_msnn_merge(int n, const int* pix, const float4* full, float4* short_color) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) short_color[pix[j]] = full[j];
+}
+
 // Learning-rate schedule of the reference's optimizer config
 // (tcnn_hairmsnn.json:5-18): ExponentialDecay from step 4000, x0.33 every 4000.
 float lr_at(float base, int step) {
@@ -259,11 +298,15 @@ int msnn_setup(Context& c, int BETA, std::string& err) {
   HM_TRY(mlp_create(prec, s.msnn_seed, &m->mlp, err));
   m->ctx_stream = c.stream;
   const int spill_levels = std::max(1, c.bvh.stack_bound() - kStackLds + 1);
+  // a warm-up pass holds warm_batch training batches; the slots' path buffers
+  // hold a frame or one such pass
+  m->warm_batch = std::max(1, std::min(MsnnState::kWarmBatch, std::max(npx, m->n_train) / m->n_train));
+  const size_t rec = (size_t)m->warm_batch * m->n_train;
   for (MsnnSlot& sl : m->slots) {
     HM_HIP(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking), err);
     for (hipEvent_t* e : {&sl.ev_gb, &sl.ev_done, &sl.ev_copy})
       HM_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming), err);
-    HM_TRY(alloc_path_buffers(sl.pb, sl.allocs, (size_t)std::max(npx, m->n_train), true, spill_levels,
+    HM_TRY(alloc_path_buffers(sl.pb, sl.allocs, std::max((size_t)npx, rec), true, spill_levels,
                               c.trace_blocks * kTraceBlock, sl.st, err));
     auto A = [&](void** p, size_t bytes) -> int {
       HM_TRY(dev_alloc(p, bytes, err));
@@ -273,12 +316,30 @@ int msnn_setup(Context& c, int BETA, std::string& err) {
     HM_TRY(A((void**)&sl.nn_input, sizeof(float) * 12 * (size_t)npx));
     HM_TRY(A((void**)&sl.hit_flags, (size_t)npx));
     HM_TRY(A((void**)&sl.short_color, sizeof(float4) * (size_t)npx));
-    HM_TRY(A((void**)&sl.train_input, sizeof(float) * 12 * (size_t)m->n_train));
-    HM_TRY(A((void**)&sl.train_target, sizeof(float) * 3 * (size_t)m->n_train));
-    HM_TRY(A((void**)&sl.train_hit, (size_t)m->n_train));
+    HM_TRY(A((void**)&sl.train_input, sizeof(float) * 12 * rec));
+    HM_TRY(A((void**)&sl.train_target, sizeof(float) * 3 * rec));
+    HM_TRY(A((void**)&sl.train_hit, rec));
     HM_TRY(A((void**)&sl.train_idx, sizeof(int) * (size_t)m->n_train));
-    HM_TRY(A((void**)&sl.gen_points, sizeof(float) * 3 * (size_t)m->n_train));
+    HM_TRY(A((void**)&sl.gen_points, sizeof(float) * 3 * rec));
     HM_HIP(hipHostMalloc((void**)&sl.idx_host, sizeof(int) * (size_t)m->n_train), err);
+  }
+  for (TrainBatch& b : m->tb) {
+    for (hipEvent_t* e : {&b.ev_ready, &b.ev_free, &b.ev_copy})
+      HM_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming), err);
+    HM_TRY(alloc_path_buffers(b.pb, b.allocs, rec, true, spill_levels, c.trace_blocks * kTraceBlock, c.stream, err));
+    auto A = [&](void** p, size_t bytes) -> int {
+      HM_TRY(dev_alloc(p, bytes, err));
+      b.allocs.push_back(*p);
+      return kOk;
+    };
+    HM_TRY(A((void**)&b.idx_dev, sizeof(int) * rec));
+    HM_TRY(A((void**)&b.pix, sizeof(int) * rec));
+    HM_TRY(A((void**)&b.nn_input, sizeof(float) * 12 * rec));
+    HM_TRY(A((void**)&b.hit, rec));
+    HM_TRY(A((void**)&b.train_input, sizeof(float) * 12 * rec));
+    HM_TRY(A((void**)&b.train_target, sizeof(float) * 3 * rec));
+    HM_TRY(A((void**)&b.full, sizeof(float4) * rec));
+    HM_HIP(hipHostMalloc((void**)&b.idx_host, sizeof(int) * rec), err);
   }
   HM_TRY(dev_alloc((void**)&m->nn_output, sizeof(float) * 3 * (size_t)npx, err));
   for (int k = 0; k < 3; ++k) {
@@ -311,9 +372,12 @@ __global__ void k_sample_points(const float* pts, int num_points, uint64_t seed,
   out[3 * (size_t)i + 2] = pts[3 * (size_t)p + 2];
 }
 
-int sample_gen_points(Context& c, MsnnState& m, MsnnSlot& sl, std::string& err) {
-  hipLaunchKernelGGL(k_sample_points, dim3((m.n_train + 255) / 256), dim3(256), 0, sl.st, c.scene_points,
-                     c.num_scene_points, m.next_u64(), m.n_train, sl.gen_points);
+// Draws for `batches` consecutive training-data passes, batch b at
+// gen_points + b * n_train (one seed per pass, in pass order).
+int sample_gen_points(Context& c, MsnnState& m, MsnnSlot& sl, int batches, std::string& err) {
+  for (int b = 0; b < batches; ++b)
+    hipLaunchKernelGGL(k_sample_points, dim3((m.n_train + 255) / 256), dim3(256), 0, sl.st, c.scene_points,
+                       c.num_scene_points, m.next_u64(), m.n_train, sl.gen_points + 3 * (size_t)b * m.n_train);
   return check_launch("msnn sample points", err);
 }
 
@@ -348,28 +412,35 @@ int train_step(Context& c, MsnnState& m, const float* input, const float* target
 // on the network, so consecutive ones run on consecutive slots and overlap;
 // the training steps run in order on the context stream.
 int msnn_warmup(Context& c, MsnnState& m, int steps, std::string& err) {
-  for (int k = 0; k < steps; ++k) {
+  for (int k = 0; k < steps;) {
+    const int nb = std::min(m.warm_batch, steps - k);
+    const int n = nb * m.n_train;
     MsnnSlot& sl = m.take_slot();
     HM_TRY(acquire_slot(sl, err));
-    HM_TRY(sample_gen_points(c, m, sl, err));
+    HM_TRY(sample_gen_points(c, m, sl, nb, err));
     PassConfig cfg = msnn_pass_config(c, m);
     cfg.camera_mode = 1;
-    cfg.n_slots = m.n_train;
+    cfg.n_slots = n;
+    cfg.chunk = m.n_train;
     cfg.grid_w = m.grid_w;
     cfg.points = sl.gen_points;
-    cfg.accum_id = m.gen_id++;
+    cfg.accum_id = m.gen_id;
+    m.gen_id += nb;
     cfg.gb.enabled = 1;
     cfg.gb.nn_input = sl.train_input;
     cfg.gb.hit_flags = sl.train_hit;
     cfg.gb.scene_scale = c.scene_scale;
     HM_TRY(pt_run_pass(c.view, sl.pb, cfg, sl.st, err));
-    hipLaunchKernelGGL(k_msnn_finalize, dim3((m.n_train + 255) / 256), dim3(256), 0, sl.st, sl.pb, m.n_train, 1, 1,
-                       m.n_train, sl.train_input, sl.train_hit, sl.train_input, sl.train_target, (float4*)nullptr);
+    hipLaunchKernelGGL(k_msnn_finalize, dim3((n + 255) / 256), dim3(256), 0, sl.st, sl.pb, n, 1, 1, n,
+                       sl.train_input, sl.train_hit, sl.train_input, sl.train_target, (float4*)nullptr);
     HM_TRY(check_launch("msnn finalize (warm-up)", err));
     HM_HIP(hipEventRecord(sl.ev_gb, sl.st), err);
     HM_HIP(hipStreamWaitEvent(c.stream, sl.ev_gb, 0), err);
-    HM_TRY(train_step(c, m, sl.train_input, sl.train_target, err, nullptr));
+    for (int b = 0; b < nb; ++b)
+      HM_TRY(train_step(c, m, sl.train_input + 12 * (size_t)b * m.n_train, sl.train_target + 3 * (size_t)b * m.n_train,
+                        err, nullptr));
     HM_HIP(hipEventRecord(sl.ev_done, c.stream), err);
+    k += nb;
   }
   return kOk;
 }
@@ -391,42 +462,85 @@ struct EventTimer {
   }
 };
 
-// One frame: G_BUFFER pass + finalize on the frame's slot, then (in frame
-// order on the context stream) training, inference and the RENDER
-// composite (render_hair_msnn.cu:699-771). tm, when given, receives events:
-// [0], [1] around the slot's pass; [2]..[5] on the context stream.
+// Training paths of frames f0 .. f0 + warm_batch - 1 as one pass on stream
+// st, into batch buffer (f0 / warm_batch) % 2. The frames' index shuffles
+// (render_hair_msnn.cu:712-714) are drawn here, in frame order.
+int gen_train_batch(Context& c, MsnnState& m, int f0, hipStream_t st, std::string& err) {
+  const int B = m.warm_batch;
+  TrainBatch& b = m.tb[(f0 / B) % 2];
+  if (b.used) HM_HIP(hipStreamWaitEvent(st, b.ev_free, 0), err);
+  b.used = true;
+  HM_HIP(hipEventSynchronize(b.ev_copy), err);
+  for (int f = 0; f < B; ++f) {
+    int* idx = b.idx_host + (size_t)f * m.n_train;
+    for (int i = 0; i < m.n_train; ++i) idx[i] = i;
+    for (int i = m.n_train - 1; i > 0; --i) std::swap(idx[i], idx[(int)(m.next_u64() % (uint64_t)(i + 1))]);
+  }
+  const int n = B * m.n_train;
+  HM_HIP(hipMemcpyAsync(b.idx_dev, b.idx_host, sizeof(int) * (size_t)n, hipMemcpyHostToDevice, st), err);
+  HM_HIP(hipEventRecord(b.ev_copy, st), err);
+  PassConfig cfg = msnn_pass_config(c, m);
+  cfg.camera_mode = 2;
+  cfg.n_slots = n;
+  cfg.chunk = m.n_train;
+  cfg.every_nth = m.every_nth;
+  cfg.train_idx = b.idx_dev;
+  cfg.pix = b.pix;
+  cfg.accum_id = f0;
+  cfg.gb.enabled = 1;
+  cfg.gb.nn_input = b.nn_input;
+  cfg.gb.hit_flags = b.hit;
+  cfg.gb.scene_scale = c.scene_scale;
+  HM_TRY(pt_run_pass(c.view, b.pb, cfg, st, err));
+  hipLaunchKernelGGL(k_msnn_finalize, dim3((n + 255) / 256), dim3(256), 0, st, b.pb, n, 1, 1, n, b.nn_input, b.hit,
+                     b.train_input, b.train_target, b.full);
+  HM_TRY(check_launch("msnn finalize (training batch)", err));
+  HM_HIP(hipEventRecord(b.ev_ready, st), err);
+  return kOk;
+}
+
+// One frame (render_hair_msnn.cu:699-771). On the frame's slot: the G_BUFFER
+// pass over every pixel, truncated at the kernel beta, and its finalize; the
+// frame's training pixels, traced in full by their batch, then overwrite
+// their G-buffer colour. In frame order on the context stream: training on
+// the frame's records, inference, and the RENDER composite. tm, when given,
+// receives events: [0], [1] around the slot's work; [2]..[5] on the context
+// stream.
 int msnn_frame(Context& c, MsnnState& m, EventTimer* tm, std::string& err) {
   const int npx = c.scene.width * c.scene.height;
+  const int B = m.warm_batch;
+  const int f = m.frame, chunk = f % B;
   MsnnSlot& sl = m.take_slot();
   HM_TRY(acquire_slot(sl, err));
-  // shuffle of the training-pixel indices (render_hair_msnn.cu:712-714)
-  HM_HIP(hipEventSynchronize(sl.ev_copy), err);
-  int* idx = sl.idx_host;
-  for (int i = 0; i < m.n_train; ++i) idx[i] = i;
-  for (int i = m.n_train - 1; i > 0; --i) std::swap(idx[i], idx[(int)(m.next_u64() % (uint64_t)(i + 1))]);
-  HM_HIP(hipMemcpyAsync(sl.train_idx, idx, sizeof(int) * (size_t)m.n_train, hipMemcpyHostToDevice, sl.st), err);
-  HM_HIP(hipEventRecord(sl.ev_copy, sl.st), err);
   if (tm) (void)hipEventRecord(tm->e[0], sl.st);
-  // ---- G_BUFFER ----
+  if (chunk == 0) HM_TRY(gen_train_batch(c, m, f, sl.st, err));
+  TrainBatch& b = m.tb[(f / B) % 2];
+  // ---- G_BUFFER (every pixel truncated; no training flags) ----
   PassConfig cfg = msnn_pass_config(c, m);
-  cfg.accum_id = m.frame;
-  cfg.train_idx = sl.train_idx;
+  cfg.accum_id = f;
   cfg.every_nth = m.every_nth;
-  cfg.n_slots = m.n_train;
+  cfg.n_slots = 0;
+  cfg.max_vertex = std::min(cfg.max_vertex, m.kbeta + 1);  // the last vertex a truncated path lights
   cfg.gb.enabled = 1;
   cfg.gb.nn_input = sl.nn_input;
   cfg.gb.hit_flags = sl.hit_flags;
   cfg.gb.scene_scale = c.scene_scale;
   HM_TRY(pt_run_pass(c.view, sl.pb, cfg, sl.st, err));
-  hipLaunchKernelGGL(k_msnn_finalize, dim3((npx + 255) / 256), dim3(256), 0, sl.st, sl.pb, npx, 0, m.every_nth,
-                     m.n_train, sl.nn_input, sl.hit_flags, sl.train_input, sl.train_target, sl.short_color);
+  hipLaunchKernelGGL(k_msnn_finalize, dim3((npx + 255) / 256), dim3(256), 0, sl.st, sl.pb, npx, 0, m.every_nth, 0,
+                     sl.nn_input, sl.hit_flags, (float*)nullptr, (float*)nullptr, sl.short_color);
   HM_TRY(check_launch("msnn finalize", err));
+  HM_HIP(hipStreamWaitEvent(sl.st, b.ev_ready, 0), err);
+  const size_t r0 = (size_t)chunk * m.n_train;
+  hipLaunchKernelGGL(k_msnn_merge, dim3((m.n_train + 255) / 256), dim3(256), 0, sl.st, m.n_train, b.pix + r0,
+                     b.full + r0, sl.short_color);
+  HM_TRY(check_launch("msnn merge", err));
   if (tm) (void)hipEventRecord(tm->e[1], sl.st);
   HM_HIP(hipEventRecord(sl.ev_gb, sl.st), err);
   HM_HIP(hipStreamWaitEvent(c.stream, sl.ev_gb, 0), err);
   if (tm) (void)hipEventRecord(tm->e[2], c.stream);
   // ---- train ----
-  if (c.scene.nrc_train) HM_TRY(train_step(c, m, sl.train_input, sl.train_target, err, nullptr));
+  if (c.scene.nrc_train) HM_TRY(train_step(c, m, b.train_input + 12 * r0, b.train_target + 3 * r0, err, nullptr));
+  if (chunk == B - 1) HM_HIP(hipEventRecord(b.ev_free, c.stream), err);
   if (tm) (void)hipEventRecord(tm->e[3], c.stream);
   // ---- infer ----
   HM_TRY(mlp_forward(*m.mlp, sl.nn_input, npx, m.nn_output, c.stream, err));

@@ -100,15 +100,49 @@ __global__ void k_camera(SceneView sc, PathBuffers pb, int accum_id, int msnn, c
   pb.flags[i] = flags;
 }
 
-// Training-data rays: camera position toward sampled hair points; every slot
-// is a training path (cuda/hair_msnn.cu:222-233). Seeds use the launch index
-// of a grid_w-wide 2D launch.
-__global__ void k_camera_points(SceneView sc, PathBuffers pb, int accum_id, int n, int grid_w, const float* pts) {
+// HairMSNN training pixels of several frames as one pass: slot i holds
+// record j = i % chunk of frame accum_id + i / chunk, whose pixel is
+// j * every_nth + train_idx[i] % every_nth (the pixel k_camera flags,
+// hair_msnn.cu:206-209), traced from that pixel's seed and jitter, so the
+// paths are those the frame's own pass would trace. pix receives the pixels.
+__global__ void k_camera_pixels(SceneView sc, PathBuffers pb, int accum_id, int n, int chunk, int every_nth,
+                                const int* train_idx, int* pix) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) pb.counts[0] = n;
   if (i >= n) return;
-  const int x = i % grid_w, y = i / grid_w;
-  const uint32_t rng = path_seed((uint32_t)(accum_id + 10007), (uint32_t)x, (uint32_t)y, (uint32_t)grid_w);
+  const int c = i / chunk, j = i - c * chunk;
+  const int p = j * every_nth + train_idx[i] % every_nth;
+  pix[i] = p;
+  const int x = p % sc.width, y = p / sc.width;
+  uint32_t rng = path_seed((uint32_t)(accum_id + c + 10007), (uint32_t)x, (uint32_t)y, (uint32_t)sc.width);
+  const float jx = draw(rng), jy = draw(rng);
+  const float sx = ((float)x + jx) / (float)sc.width;
+  const float sy = ((float)y + jy) / (float)sc.height;
+  const V3 d = normalize(sc.cam_d00 + sx * sc.cam_du + sy * sc.cam_dv);
+  pb.ray_o[i] = make_float4(sc.cam_pos.x, sc.cam_pos.y, sc.cam_pos.z, 0.f);
+  pb.ray_d[i] = make_float4(d.x, d.y, d.z, 0.f);
+  pb.beta[i] = make_float4(1.f, 1.f, 1.f, 0.f);
+  pb.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  pb.beta_short[i] = make_float4(1.f, 1.f, 1.f, 0.f);
+  pb.L_short[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  pb.rng[i] = rng;
+  pb.queue[0][i] = i;
+  pb.flags[i] = 1u;
+}
+
+// Training-data rays: camera position toward sampled hair points; every slot
+// is a training path (cuda/hair_msnn.cu:222-233). The slots hold consecutive
+// chunks of `chunk` paths, chunk c being the pass of index accum_id + c; a
+// path's seed uses its launch index within its chunk's grid_w-wide 2D launch,
+// so a batch of chunks reproduces the separate passes exactly.
+__global__ void k_camera_points(SceneView sc, PathBuffers pb, int accum_id, int n, int chunk, int grid_w,
+                                const float* pts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) pb.counts[0] = n;
+  if (i >= n) return;
+  const int c = i / chunk, j = i - c * chunk;
+  const int x = j % grid_w, y = j / grid_w;
+  const uint32_t rng = path_seed((uint32_t)(accum_id + c + 10007), (uint32_t)x, (uint32_t)y, (uint32_t)grid_w);
   const V3 p = v3(pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]);
   const V3 d = normalize(p - sc.cam_pos);
   pb.ray_o[i] = make_float4(sc.cam_pos.x, sc.cam_pos.y, sc.cam_pos.z, 0.f);
@@ -362,10 +396,15 @@ __global__ void HM_TRACE_BOUNDS k_trace_rays(SceneView sc, const float4* rays, i
 
 int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hipStream_t stream,
                 std::string& err) {
-  if (cfg.camera_mode == 1) {
+  if (cfg.camera_mode == 2) {
+    const int n = cfg.n_slots;
+    hipLaunchKernelGGL(k_camera_pixels, dim3((n + 255) / 256), dim3(256), 0, stream, sc, pb,
+                       cfg.accum_id + cfg.seed_offset, n, cfg.chunk > 0 ? cfg.chunk : n, cfg.every_nth, cfg.train_idx,
+                       cfg.pix);
+  } else if (cfg.camera_mode == 1) {
     const int n = cfg.n_slots;
     hipLaunchKernelGGL(k_camera_points, dim3((n + 255) / 256), dim3(256), 0, stream, sc, pb,
-                       cfg.accum_id + cfg.seed_offset, n, cfg.grid_w, cfg.points);
+                       cfg.accum_id + cfg.seed_offset, n, cfg.chunk > 0 ? cfg.chunk : n, cfg.grid_w, cfg.points);
   } else {
     const int n = sc.width * sc.height;
     hipLaunchKernelGGL(k_camera, dim3((n + 255) / 256), dim3(256), 0, stream, sc, pb, cfg.accum_id + cfg.seed_offset,
@@ -388,7 +427,7 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
   };
   // Grids sized to the pass's paths (queues only shrink with depth), so a
   // small training-data pass does not dispatch the full-frame grid per depth.
-  const int n0 = cfg.camera_mode == 1 ? cfg.n_slots : sc.width * sc.height;
+  const int n0 = cfg.camera_mode != 0 ? cfg.n_slots : sc.width * sc.height;
   const int ext_blocks = std::min(cfg.trace_blocks, std::max(1, (n0 + kTraceBlock - 1) / kTraceBlock));
   const int sh_blocks = std::min(cfg.trace_blocks, std::max(1, (2 * n0 + kTraceBlock - 1) / kTraceBlock));
   const int shade_blocks = std::min(cfg.shade_blocks, std::max(1, (n0 + 255) / 256));

@@ -72,6 +72,7 @@ struct PassConfig {
   // [5] any-hit rays
   unsigned long long* stats = nullptr;
   PassTimer* timer = nullptr;
+  int chunk = 0;  // camera_mode 1: paths per pass index (0 = all slots one pass)
   int trace_blocks = 2048;
   int shade_blocks = 2048;
   // camera_mode 0: one jittered camera ray per pixel (cuda/path_tracing.cu:32-42)
@@ -81,6 +82,9 @@ struct PassConfig {
   int n_slots = 0;             // camera_mode 1: number of records
   int grid_w = 0;              // camera_mode 1: launch width used for seeding
   const float* points = nullptr;  // camera_mode 1: 3 floats per record
+  // camera_mode 2: the training pixels of frames accum_id.. (chunk records
+  // each, train_idx per record); their pixel indices are written to pix
+  int* pix = nullptr;
 };
 
 // Runs one full sample pass (camera -> all vertices). Leaves per-pixel

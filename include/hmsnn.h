@@ -153,7 +153,7 @@ int hmsnn_set_strand_sigma_a(hmsnn_ctx* ctx, const float* sigma_host, int num_st
  * Default 0 reproduces the reference's streams. No reference counterpart:
  * it lets a reference image and a test render use independent samples. */
 int hmsnn_set_seed_offset(hmsnn_ctx* ctx, int offset);
-/* Path-tracing passes kept in flight at once (1..4, default 3), each with its
+/* Path-tracing passes kept in flight at once (1..4, default 4), each with its
  * own path buffers and stream; accumulation stays in pass order, so the image
  * does not depend on it. 1 runs passes back to back (per-kernel timings of
  * hmsnn_pt_time_passes are then those of kernels running alone). No
