@@ -41,7 +41,10 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sc, budget_s: float = 15.0) -> dict:
+def cpu_baseline(sc, side: int = 128, spp: int = 64) -> dict:
+    """The FP32 oracle (C++ restatement of pathTrace with a CPU BVH, OpenMP over
+    pixel rows) on the centre side x side crop of the same frame at spp samples
+    per pixel, on all host cores."""
     from tests import oracle_lib as ol
     orc = ol.load_oracle()
     t0 = time.perf_counter()
@@ -49,24 +52,48 @@ def cpu_baseline(sc, budget_s: float = 15.0) -> dict:
     build_s = time.perf_counter() - t0
     threads = int(orc.orc_max_threads())
     W, H = sc.width, sc.height
-    # calibrate on a small crop, then size the sample to the time budget
-    cw = 32
-    x0, y0 = W // 2 - cw // 2, H // 2 - cw // 2
-    t0 = time.perf_counter()
-    osc.render(x0, y0, cw, cw, spp=1, accum_id0=0, prec=1, threads=threads)
-    dt = max(time.perf_counter() - t0, 1e-3)
-    rate = cw * cw / dt
-    side = int(np.clip(np.sqrt(rate * budget_s), 32, 256))
     x0, y0 = W // 2 - side // 2, H // 2 - side // 2
     t0 = time.perf_counter()
-    osc.render(x0, y0, side, side, spp=1, accum_id0=1, prec=1, threads=threads)
+    osc.render(x0, y0, side, side, spp=spp, accum_id0=1, prec=1, threads=threads)
     dt = time.perf_counter() - t0
     return {
-        "value": side * side / dt, "unit": "paths/s", "cores": threads, "kind": "port",
-        "sample": f"centre {side}x{side} crop of the same 1024x1024 frame, 1 spp, FP32 oracle "
+        "value": side * side * spp / dt, "unit": "paths/s", "cores": threads, "kind": "port",
+        "sample": f"centre {side}x{side} crop of the same 1024x1024 frame, {spp} spp, FP32 oracle "
                   f"(C++ restatement of pathTrace, CPU BVH built in {build_s:.1f} s, not timed), "
                   f"{dt:.1f} s of CPU work",
     }
+
+
+def mlp_rates(hm, n: int = 1 << 20, reps: int = 10) -> dict:
+    """Inference TFLOP/s of the radiance-cache network (41,344 FLOP per query)
+    in FP16 and FP8 on n seeded queries, device-resident, wall clock over reps
+    calls after one warm call (the FP8 calibration pass runs in the warm call)."""
+    import ctypes as C
+    lib = hm._lib.load()
+    rng = np.random.default_rng(4)
+    rec = np.zeros((n, 12), np.float32)
+    rec[:, 0:3] = rng.uniform(-0.5, 0.5, (n, 3))
+    for a in (3, 6):
+        v = rng.normal(size=(n, 3))
+        rec[:, a:a + 3] = v / np.linalg.norm(v, axis=1, keepdims=True)
+    rec[:, 9:12] = rng.uniform(0.01, 0.5, (n, 3))
+    din, dout = hm.DeviceBuffer.from_numpy(rec), hm.DeviceBuffer(12 * n)
+    out = {}
+    for name, prec in (("fp16", 0), ("fp8", 2)):
+        h = C.c_void_p()
+        hm._lib.check(lib.hmsnn_mlp_create(prec, 1, C.byref(h)))
+        hm._lib.check(lib.hmsnn_mlp_forward_dev(h, din.ptr, n, dout.ptr, None))
+        hm._lib.check(lib.hmsnn_dev_synchronize())
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            hm._lib.check(lib.hmsnn_mlp_forward_dev(h, din.ptr, n, dout.ptr, None))
+        hm._lib.check(lib.hmsnn_dev_synchronize())
+        dt = (time.perf_counter() - t0) / reps
+        lib.hmsnn_mlp_destroy(h)
+        peak = 2500.0 if name == "fp16" else 5000.0
+        tf = 41344.0 * n / dt / 1e12
+        out[name] = {"queries": n, "ms": dt * 1e3, "TFLOP/s": tf, "peak_TFLOP/s": peak, "frac": tf / peak}
+    return out
 
 
 def msnn_vs_pt(sc, pt_ms_per_pass: float, ref_spp: int, frames: int, beta: int = 1) -> dict:
@@ -152,7 +179,7 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-msnn", action="store_true", help="skip the HairMSNN vs PT time-to-equal-RMSE leg")
-    ap.add_argument("--ref-spp", type=int, default=1024, help="passes of the RMSE reference image")
+    ap.add_argument("--ref-spp", type=int, default=4096, help="passes of the RMSE reference image")
     ap.add_argument("--pass-concurrency", type=int, default=4, help="path-tracing passes in flight in the timed region")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="instead of the benchmark, run 8 device-to-device copies of this many MiB (the known-size "
@@ -260,9 +287,20 @@ def main() -> None:
             "shadow": kt["shadow_ms"] / passes_t, "total": kt["total_ms"] / passes_t,
         },
         "rays_per_pass": {"closest": stats["closest_rays"], "any": stats["any_rays"]},
+        "rays_per_s": {"closest": stats["closest_rays"] * args.steps / wall, "any": stats["any_rays"] * args.steps / wall},
     }
     if not args.no_msnn:
         line["msnn_vs_pt"] = msnn_vs_pt(sc, wall / args.steps * 1e3, args.ref_spp, 64, beta=1)
+        fm = line["msnn_vs_pt"]["msnn_frame_ms"]
+        train_records = int(json.loads(Path(args.scene).read_text()).get("hairmsnn", {}).get("train_records", 16384))
+        line["mlp"] = {
+            "frame_inference_TFLOP/s": 41344.0 * paths_per_step / (fm["infer_ms"] * 1e-3) / 1e12,
+            "frame_training_TFLOP/s": 124032.0 * train_records / (fm["train_ms"] * 1e-3) / 1e12,
+        }
+        try:
+            line["mlp"]["inference"] = mlp_rates(hm)
+        except Exception as e:
+            log(f"mlp timing failed: {e}")
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(sc)

@@ -39,6 +39,9 @@ def test_beta_beyond_depth_reproduces_plain_pass(gpu):
 def test_truncated_pass_matches_oracle(gpu):
     cfg = cfg_small(records=128)  # 1 training pixel in 32
     with gpu.Scene(json_text=cfg) as sc:
+        sc.pt_reset()
+        sc.pt_passes(1)
+        pt_full = sc.last_pass()[..., :3].astype(np.float64)  # every pixel's full path, frame 0's seeds
         sc.msnn_begin(BETA=1)
         sc.msnn_frames(1)
         short = sc.msnn_image(1)[..., :3].astype(np.float64)
@@ -51,6 +54,13 @@ def test_truncated_pass_matches_oracle(gpu):
     frac = (close_short | close_full).mean()
     print(f"truncated-pass agreement {frac:.4f}; training-like pixels {np.mean(close_full & ~close_short):.4f}")
     assert frac >= 0.95
+    # the training pixels (1 in 32, traced in full by their batch pass) carry
+    # exactly the plain pass's colour: among pixels whose full path differs
+    # from the truncated one, about 1/32 must equal it bit for bit
+    cont = np.abs(pt_full - o_short).max(-1) > 1e-3 * np.maximum(np.abs(o_short).max(-1), 1e-2)
+    same = (short == pt_full).all(-1)
+    print(f"pixels equal to the full path among continuing paths: {same[cont].mean():.4f} (1/32 = 0.0312)")
+    assert 0.5 / 32 <= same[cont].mean() <= 2.0 / 32
     # the truncated path is never brighter than the full path on average
     assert o_short.mean() <= o_full.mean() + 1e-6
 
