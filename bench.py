@@ -35,6 +35,9 @@ METRIC = "paths/s @1024² 3.2M segs; HairMSNN β=1 vs plain PT time-to-equal-RMS
 HBM_PEAK_GBS = 8000.0
 SCENE = ROOT / "hairmsnn_amd" / "scenes" / "c1_curly.json"
 NODE_BYTES, SEG_BYTES, RAY_BYTES = 128, 32, 48  # 4-wide node, segment record, ray in + hit out
+# HBM bytes per k_extend launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+# (copy-calibrated); counters cannot be read from inside this process
+TRAFFIC_JSON = ROOT / "profiles" / "pt_c1_1024_hbm_traffic.json"
 
 
 def log(msg: str) -> None:
@@ -253,6 +256,9 @@ def main() -> None:
     log(f"extend: {extend_ms:.3f} ms/launch, {bytes_launch / 1e6:.1f} MB/launch -> {achieved:.0f} GB/s; "
         f"shadow {shadow_gbs:.0f} GB/s; shade {kt['shade_ms'] / passes_t:.2f} ms/pass")
 
+    traffic = None
+    if TRAFFIC_JSON.exists():
+        traffic = json.loads(TRAFFIC_JSON.read_text()).get("traffic_bytes_per_launch")
     line = {
         "metric": METRIC,
         "value": value,
@@ -276,8 +282,9 @@ def main() -> None:
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": "k_extend (closest-hit LBVH traversal)",
+            "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/pt_c1_1024_hbm_traffic.md",
             "bytes_per_launch": bytes_launch, "ms_per_launch": extend_ms,
             "nodes_per_ray": stats["closest_nodes"] / max(stats["closest_rays"], 1),
             "segs_per_ray": stats["closest_segs"] / max(stats["closest_rays"], 1),

@@ -38,6 +38,12 @@ def test_beta_beyond_depth_reproduces_plain_pass(gpu):
 
 def test_truncated_pass_matches_oracle(gpu):
     cfg = cfg_small(records=128)  # 1 training pixel in 32
+    cfg_trunc = json.loads(json.dumps(cfg))
+    cfg_trunc["integrator"]["path_v2"] = 2  # vertices 0 and 1: the kernel-beta-0 truncated path
+    with gpu.Scene(json_text=cfg_trunc) as st:
+        st.pt_reset()
+        st.pt_passes(1)
+        pt_trunc = st.last_pass()[..., :3].astype(np.float64)
     with gpu.Scene(json_text=cfg) as sc:
         sc.pt_reset()
         sc.pt_passes(1)
@@ -54,13 +60,15 @@ def test_truncated_pass_matches_oracle(gpu):
     frac = (close_short | close_full).mean()
     print(f"truncated-pass agreement {frac:.4f}; training-like pixels {np.mean(close_full & ~close_short):.4f}")
     assert frac >= 0.95
-    # the training pixels (1 in 32, traced in full by their batch pass) carry
-    # exactly the plain pass's colour: among pixels whose full path differs
-    # from the truncated one, about 1/32 must equal it bit for bit
-    cont = np.abs(pt_full - o_short).max(-1) > 1e-3 * np.maximum(np.abs(o_short).max(-1), 1e-2)
-    same = (short == pt_full).all(-1)
-    print(f"pixels equal to the full path among continuing paths: {same[cont].mean():.4f} (1/32 = 0.0312)")
-    assert 0.5 / 32 <= same[cont].mean() <= 2.0 / 32
+    # GPU against GPU, bit for bit: every pixel of the short image is its
+    # truncated path, except the training pixels (one per 32, traced in full
+    # by their batch pass), which carry their full path
+    diff = ~(short == pt_trunc).all(-1)
+    full = (short == pt_full).all(-1)
+    print(f"pixels differing from the truncated pass: {diff.sum()} (training records 128), all full paths: "
+          f"{bool(full[diff].all())}")
+    assert full[diff].all()
+    assert 0 < diff.sum() <= 128
     # the truncated path is never brighter than the full path on average
     assert o_short.mean() <= o_full.mean() + 1e-6
 
