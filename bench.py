@@ -99,43 +99,46 @@ def mlp_rates(hm, n: int = 1 << 20, reps: int = 10) -> dict:
     return out
 
 
-def msnn_vs_pt(sc, pt_ms_per_pass: float, ref_spp: int, frames: int, beta: int = 1) -> dict:
-    """HairMSNN at BETA vs plain path tracing: time to reach equal RMSE
-    against a ref_spp path-traced reference rendered on independent random
-    streams. PT: MSE(k) = a / k + b fitted over k = 1..frames passes (b is the
-    reference's own noise), time per pass from the timed region. HairMSNN:
-    warm-up training plus `frames` frames, wall clock; its MSE has b removed.
-    Returns the speed-up T_PT(MSE_MSNN) / T_MSNN and both curves."""
-    W, H = sc.width, sc.height
+def pt_reference(sc, ref_spp: int, frames: int) -> dict:
+    """A ref_spp path-traced reference on independent random streams, and the
+    plain tracer's MSE(k) against it for k = 1..frames passes, fitted as
+    a / k + b (b: the reference's own noise)."""
     sc.set_seed_offset(1 << 22)
     sc.pt_reset()
     t0 = time.perf_counter()
     done = 0
     while done < ref_spp:
-        k = min(64, ref_spp - done)
+        k = min(256, ref_spp - done)
         sc.pt_passes(k)
         done += k
         log(f"reference: {done}/{ref_spp} passes ({time.perf_counter() - t0:.1f} s)")
     ref = sc.image(0)[..., :3].astype(np.float64)
     sc.set_seed_offset(0)
-
-    def mse(img):
-        return float(np.mean((img[..., :3].astype(np.float64) - ref) ** 2))
-
     sc.pt_reset()
     ks, pt_mse = [], []
     for k in range(1, frames + 1):
         sc.pt_passes(1)
         if k & (k - 1) == 0 or k == frames:
             ks.append(k)
-            pt_mse.append(mse(sc.image(0)))
+            pt_mse.append(float(np.mean((sc.image(0)[..., :3].astype(np.float64) - ref) ** 2)))
     ks_a, m_a = np.array(ks, float), np.array(pt_mse)
     A = np.stack([1.0 / ks_a, np.ones_like(ks_a)], 1)
     (a, b), *_ = np.linalg.lstsq(A, m_a, rcond=None)
-    b = max(b, 0.0)
-    # HairMSNN: warm-up training + frames. The scene-sample generation inside
-    # hmsnn_msnn_begin is one-time scene setup (like the BVH build, which the
-    # PT time excludes too) and is reported, not counted.
+    return {"ref": ref, "spp": ref_spp, "a": float(a), "b": float(max(b, 0.0)), "ks": ks, "mse": pt_mse}
+
+
+def msnn_vs_pt(sc, pt_ms_per_pass: float, R: dict, frames: int, beta: int = 1) -> dict:
+    """HairMSNN at BETA vs plain path tracing: the time the plain tracer needs
+    to reach the MSE HairMSNN has after its warm-up and `frames` frames
+    (MSE_PT(k) = a / k + b solved for k, times the timed per-pass cost)
+    over HairMSNN's wall time (warm-up training + frames). The
+    scene-sample generation inside hmsnn_msnn_begin is one-time scene setup,
+    like the BVH build the PT time excludes too, and is reported, not counted."""
+    ref, a, b = R["ref"], R["a"], R["b"]
+
+    def mse(img):
+        return float(np.mean((img[..., :3].astype(np.float64) - ref) ** 2))
+
     sc.synchronize()
     sc.msnn_begin(beta)
     st0 = sc.msnn_frames(0)
@@ -160,19 +163,46 @@ def msnn_vs_pt(sc, pt_ms_per_pass: float, ref_spp: int, frames: int, beta: int =
     k_eq = a / m_final
     t_pt = k_eq * pt_ms_per_pass * 1e-3
     t_ms = ms_t[-1]
-    log(f"msnn vs pt: PT a={a:.4g} b={b:.3g}; MSNN {frames} frames MSE {ms_mse[-1]:.4g} in {t_ms:.2f} s "
+    log(f"msnn vs pt (BETA {beta}): MSNN {frames} frames MSE {ms_mse[-1]:.4g} in {t_ms:.2f} s "
         f"(warm-up {t_warm:.2f} s, setup {t_setup:.2f} s not counted) = PT at {k_eq:.1f} spp, {t_pt:.2f} s "
         f"-> speed-up {t_pt / t_ms:.2f}x; per frame: " +
         ", ".join(f"{k} {v / frames:.2f}" for k, v in parts.items()))
     return {
-        "beta": beta, "frames": frames, "reference_spp": ref_spp,
+        "beta": beta, "frames": frames, "reference_spp": R["spp"],
         "speedup": t_pt / t_ms, "t_msnn_s": t_ms, "t_warmup_s": t_warm, "t_setup_s_not_counted": t_setup,
         "t_pt_equal_rmse_s": t_pt, "msnn_frame_ms": {k: v / frames for k, v in parts.items()},
         "pt_spp_equal_rmse": k_eq, "rmse_msnn": float(np.sqrt(m_final)),
-        "pt_fit": {"a": float(a), "b_reference_noise": float(b)},
-        "pt_curve": {"spp": ks, "mse": pt_mse},
         "msnn_curve": {"frames": fr, "t_s": ms_t, "mse": ms_mse},
     }
+
+
+def adjoint_rate(hm, scene_path: Path, size: int = 512, spp: int = 16) -> dict:
+    """The differentiable-appearance workload: c1 geometry at size^2, spp
+    samples, per-strand sigma_a ~ U(0.01, 0.5) (seed 3) and the global
+    roughness / tilt; gradient of an L2 loss against a target rendered with
+    the scene's own sigma_a. Times the forward render and the path-replay
+    adjoint."""
+    cfg = json.loads(scene_path.read_text())
+    cfg["integrator"].update({"width": size, "height": size, "spp": spp})
+    with hm.Scene(json_text=cfg) as sc:
+        target = sc.render_path_tracing(spp)[..., :3].astype(np.float64)
+        ns = sc.num_strands
+        sig = np.random.default_rng(3).uniform(0.01, 0.5, (ns, 3)).astype(np.float32)
+        sc.set_strand_sigma_a(sig)
+        sc.synchronize()
+        t0 = time.perf_counter()
+        img = sc.render_path_tracing(spp)[..., :3].astype(np.float64)
+        t_fwd = time.perf_counter() - t0
+        dl = 2.0 * (img - target) / img.size
+        t0 = time.perf_counter()
+        gs, gg = sc.adjoint(dl, spp)
+        t_adj = time.perf_counter() - t0
+    paths = size * size * spp
+    log(f"adjoint: {size}x{size}, {spp} spp, {ns} strands: forward {t_fwd:.3f} s, adjoint {t_adj:.3f} s "
+        f"({paths / t_adj / 1e6:.2f} M paths/s)")
+    return {"workload": f"c1 geometry, {size}x{size}, {spp} spp, {ns}x3 per-strand sigma_a + beta_m, beta_n, alpha",
+            "forward_s": t_fwd, "adjoint_s": t_adj, "adjoint_paths_per_s": paths / t_adj,
+            "grad_global": [float(x) for x in gg], "grad_sigma_norm": float(np.linalg.norm(gs))}
 
 
 def main() -> None:
@@ -182,6 +212,7 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-msnn", action="store_true", help="skip the HairMSNN vs PT time-to-equal-RMSE leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (differentiable appearance) leg")
     ap.add_argument("--ref-spp", type=int, default=4096, help="passes of the RMSE reference image")
     ap.add_argument("--pass-concurrency", type=int, default=4, help="path-tracing passes in flight in the timed region")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -297,7 +328,14 @@ def main() -> None:
         "rays_per_s": {"closest": stats["closest_rays"] * args.steps / wall, "any": stats["any_rays"] * args.steps / wall},
     }
     if not args.no_msnn:
-        line["msnn_vs_pt"] = msnn_vs_pt(sc, wall / args.steps * 1e3, args.ref_spp, 64, beta=1)
+        R = pt_reference(sc, args.ref_spp, 64)
+        line["msnn_vs_pt"] = msnn_vs_pt(sc, wall / args.steps * 1e3, R, 64, beta=1)
+        line["msnn_vs_pt"]["pt_fit"] = {"a": R["a"], "b_reference_noise": R["b"]}
+        line["msnn_vs_pt"]["pt_curve"] = {"spp": R["ks"], "mse": R["mse"]}
+        line["msnn_vs_pt"]["beta_sweep"] = {
+            str(bt): {k: v for k, v in msnn_vs_pt(sc, wall / args.steps * 1e3, R, 64, beta=bt).items()
+                      if k in ("speedup", "t_msnn_s", "pt_spp_equal_rmse", "rmse_msnn")}
+            for bt in (2, 5, 10)}
         fm = line["msnn_vs_pt"]["msnn_frame_ms"]
         train_records = int(json.loads(Path(args.scene).read_text()).get("hairmsnn", {}).get("train_records", 16384))
         line["mlp"] = {
@@ -308,6 +346,11 @@ def main() -> None:
             line["mlp"]["inference"] = mlp_rates(hm)
         except Exception as e:
             log(f"mlp timing failed: {e}")
+    if not args.no_adjoint:
+        try:
+            line["adjoint"] = adjoint_rate(hm, Path(args.scene))
+        except Exception as e:
+            log(f"adjoint timing failed: {e}")
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(sc)

@@ -296,7 +296,7 @@ int Context::init(std::string& err) {
   // ---- view ----
   SceneView& v = view;
   v = SceneView{};
-  v.bvh.nodes = bvh.nodes4;
+  v.bvh.nodes = bvh.nodes_w;
   v.bvh.segs = bvh.segs;
   v.bvh.num_segs = bvh.num_segs;
   v.seg_by_id = seg_dev;
@@ -467,8 +467,8 @@ int hmsnn_get_info(const hmsnn_ctx* ctx, hmsnn_info* info) {
   fill_scene_info(c.scene, info);
   info->num_segments = c.bvh.num_segs;
   info->num_strands = c.hair.num_strands;
-  info->bvh_nodes = c.bvh.num_nodes4;
-  info->bvh_depth = c.bvh.depth4;
+  info->bvh_nodes = c.bvh.num_nodes_w;
+  info->bvh_depth = c.bvh.depth_w;
   info->env_width = c.env.width;
   info->env_height = c.env.height;
   info->scene_scale = c.scene_scale;
@@ -595,7 +595,7 @@ static int run_pt_passes(Context& c, int passes, std::string& err, PassTimer* ti
   HM_TRY(c.ensure_path_buffers(false, err));
   HM_TRY(c.ensure_flights(err));
   SceneView v = c.view;
-  v.bvh.nodes = c.bvh.nodes4;
+  v.bvh.nodes = c.bvh.nodes_w;
   const int n = c.scene.width * c.scene.height;
   const int K = std::max(1, std::min(c.num_flights, Context::kMaxFlights));
   hipEvent_t start = nullptr;

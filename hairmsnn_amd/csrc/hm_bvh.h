@@ -1,14 +1,14 @@
 // Device-side BVH view and traversal (closest hit and any hit).
 //
 // Layout in HBM: the Karras LBVH is built binary (64-byte nodes holding both
-// children's boxes) and then collapsed into 4-wide nodes of 128 bytes
-// (BvhNode4) that traversal reads; segments are 32-byte records in Morton
+// children's boxes) and then collapsed into wide nodes (BvhNodeWide, 4 or 8
+// children) that traversal reads; segments are 32-byte records in Morton
 // order. A child index >= 0 is an inner node, < 0 is ~leaf where leaf is the
 // position in the Morton-ordered segment array.
 //
 // Traversal is one ray per lane (wave64). The stack's top kStackLds entries
 // live in LDS, one column per lane (conflict-free), deeper entries spill to a
-// per-thread global area sized from the 4-wide tree's depth at build time.
+// per-thread global area sized from the wide tree's depth at build time.
 #pragma once
 
 #include "hm_common.h"
@@ -23,20 +23,25 @@ struct BvhNode {
   int4 d;    // child0, child1, unused, unused
 };
 
-// 4-wide node (128 bytes): child boxes as structure-of-arrays, so one node
-// fetch is eight 16-byte loads that decide all four children. child[k] >= 0
-// is an inner node, < 0 is ~segment position (leaf), kEmptyChild is an
-// unused slot (its box is never tested).
-struct BvhNode4 {
-  float4 lox, loy, loz;
-  float4 hix, hiy, hiz;
-  int4 child;
-  int4 pad;
+// Wide node of kBvhWidth children (4: 128 bytes, 8: 224 bytes): child boxes
+// as structure-of-arrays, so one node fetch is a run of 16-byte loads that
+// decides all children. child[k] >= 0 is an inner node, < 0 is ~segment
+// position (leaf), kEmptyChild an unused slot (its box is never tested).
+#ifndef HM_BVH_WIDTH
+#define HM_BVH_WIDTH 4
+#endif
+constexpr int kBvhWidth = HM_BVH_WIDTH;
+static_assert(kBvhWidth == 4 || kBvhWidth == 8, "4- or 8-wide nodes");
+struct __attribute__((aligned(16))) BvhNodeWide {
+  float lox[kBvhWidth], loy[kBvhWidth], loz[kBvhWidth];
+  float hix[kBvhWidth], hiy[kBvhWidth], hiz[kBvhWidth];
+  int child[kBvhWidth];
+  int pad[kBvhWidth == 4 ? 4 : 0];  // 4-wide: one 128-byte line; 8-wide: 224 bytes
 };
 constexpr int kEmptyChild = 0x7fffffff;
 
 struct BvhView {
-  const BvhNode4* nodes;  // 4-wide tree, root at 0
+  const BvhNodeWide* nodes;  // wide tree, root at 0
   const float4* segs;     // 2 float4 per segment: (p0, r), (p1, id bits)
   int num_segs;
 };
@@ -115,7 +120,7 @@ struct TraceStack {
 };
 
 // Traversal: the current entry is an inner node (>= 0) or a leaf (~index <
-// 0); each iteration either visits one 4-wide node (nearest hit child next,
+// 0); each iteration either visits one wide node (nearest hit child next,
 // the other hit children pushed far to near, leaves included) or tests one
 // segment. Keeping the two steps apart bounds the code a wave executes per
 // iteration to one node visit plus one segment test. The cap bounds the loop
@@ -123,17 +128,13 @@ struct TraceStack {
 // once per ray.
 constexpr int kNone = 0x7fffffff;
 
-// Entry distances of the four child boxes against [0, tmax]; +inf for a miss
-// or an empty slot (so that sorting puts them last).
-HM_D void node4_hits(const BvhNode4& n, const RayInv& r, float tmax, float t[4]) {
-  const float lx[4] = {n.lox.x, n.lox.y, n.lox.z, n.lox.w}, ly[4] = {n.loy.x, n.loy.y, n.loy.z, n.loy.w};
-  const float lz[4] = {n.loz.x, n.loz.y, n.loz.z, n.loz.w}, hx[4] = {n.hix.x, n.hix.y, n.hix.z, n.hix.w};
-  const float hy[4] = {n.hiy.x, n.hiy.y, n.hiy.z, n.hiy.w}, hz[4] = {n.hiz.x, n.hiz.y, n.hiz.z, n.hiz.w};
-  const int ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
+// Entry distances of the child boxes against [0, tmax]; +inf for a miss or
+// an empty slot (so that sorting puts them last).
+HM_D void node_hits(const BvhNodeWide& n, const RayInv& r, float tmax, float t[kBvhWidth]) {
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const float e = box_enter(r, lx[k], ly[k], lz[k], hx[k], hy[k], hz[k], tmax);
-    t[k] = (ch[k] != kEmptyChild && e <= tmax) ? e : __builtin_inff();
+  for (int k = 0; k < kBvhWidth; ++k) {
+    const float e = box_enter(r, n.lox[k], n.loy[k], n.loz[k], n.hix[k], n.hiy[k], n.hiz[k], tmax);
+    t[k] = (n.child[k] != kEmptyChild && e <= tmax) ? e : __builtin_inff();
   }
 }
 
@@ -171,21 +172,30 @@ HM_D void trace_closest(const BvhView& bvh, const RayInv& r, TraceStack& st, flo
       }
       cur = st.sp ? st.pop() : kNone;
     } else {
-      const BvhNode4 n = bvh.nodes[cur];
+      const BvhNodeWide n = bvh.nodes[cur];
       if (kCount) cnt.nodes++;
-      float t[4];
-      node4_hits(n, r, best_t, t);
-      int c[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
-      // sorting network for four: nearest first, misses (+inf) last
-      cswap(t[0], c[0], t[1], c[1]);
-      cswap(t[2], c[2], t[3], c[3]);
-      cswap(t[0], c[0], t[2], c[2]);
-      cswap(t[1], c[1], t[3], c[3]);
-      cswap(t[1], c[1], t[2], c[2]);
+      float t[kBvhWidth];
+      node_hits(n, r, best_t, t);
+      int c[kBvhWidth];
+#pragma unroll
+      for (int k = 0; k < kBvhWidth; ++k) c[k] = n.child[k];
+      // sorting network (Batcher): nearest first, misses (+inf) last
+      if (kBvhWidth == 4) {
+        cswap(t[0], c[0], t[1], c[1]);
+        cswap(t[2], c[2], t[3], c[3]);
+        cswap(t[0], c[0], t[2], c[2]);
+        cswap(t[1], c[1], t[3], c[3]);
+        cswap(t[1], c[1], t[2], c[2]);
+      } else {
+        constexpr int net[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
+                                    {0, 4}, {1, 5}, {2, 6}, {3, 7}, {2, 4}, {3, 5}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+        for (int q = 0; q < 19; ++q) cswap(t[net[q][0]], c[net[q][0]], t[net[q][1]], c[net[q][1]]);
+      }
       const float inf = __builtin_inff();
-      if (t[3] < inf) st.push(c[3]);
-      if (t[2] < inf) st.push(c[2]);
-      if (t[1] < inf) st.push(c[1]);
+#pragma unroll
+      for (int k = kBvhWidth - 1; k >= 1; --k)
+        if (t[k] < inf) st.push(c[k]);
       cur = t[0] < inf ? c[0] : (st.sp ? st.pop() : kNone);
     }
   }
@@ -210,17 +220,16 @@ HM_D bool trace_any(const BvhView& bvh, const RayInv& r, TraceStack& st, TraceCo
         return true;
       cur = st.sp ? st.pop() : kNone;
     } else {
-      const BvhNode4 n = bvh.nodes[cur];
+      const BvhNodeWide n = bvh.nodes[cur];
       if (kCount) cnt.nodes++;
-      float t[4];
-      node4_hits(n, r, tinf, t);
-      const int c[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
+      float t[kBvhWidth];
+      node_hits(n, r, tinf, t);
       int next = kNone;
 #pragma unroll
-      for (int k = 3; k >= 0; --k) {
+      for (int k = kBvhWidth - 1; k >= 0; --k) {
         if (t[k] < tinf) {
           if (next != kNone) st.push(next);
-          next = c[k];
+          next = n.child[k];
         }
       }
       cur = next != kNone ? next : (st.sp ? st.pop() : kNone);

@@ -232,12 +232,12 @@ __global__ void k_emit_single(const float* seg_in, BvhNode* nodes, float4* segs_
   segs_out[1] = make_float4(s[4], s[5], s[6], __int_as_float(0));
 }
 
-// ---- collapse to 4-wide nodes ----------------------------------------------
+// ---- collapse to wide nodes ------------------------------------------------
 
 struct CollapseArgs {
   const BvhNode* bin;
-  BvhNode4* out;
-  const int2* frontier;  // (binary inner node, 4-wide index)
+  BvhNodeWide* out;
+  const int2* frontier;  // (binary inner node, wide-node index)
   int nfront;
   int2* next;
   int* next_count;
@@ -249,15 +249,15 @@ __device__ inline float half_area(float lx, float ly, float lz, float hx, float 
   return dx * dy + dy * dz + dz * dx;
 }
 
-// One 4-wide node per frontier entry: start from the binary node's two
+// One wide node per frontier entry: start from the binary node's two
 // children and open the inner child of largest surface area until there are
-// four (or only leaves remain).
+// kBvhWidth (or only leaves remain).
 __global__ void k_collapse(CollapseArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.nfront) return;
   const int2 f = a.frontier[i];
-  float lo[4][3], hi[4][3];
-  int ch[4];
+  float lo[kBvhWidth][3], hi[kBvhWidth][3];
+  int ch[kBvhWidth];
   auto load2 = [&](int b, int s0, int s1) {
     const BvhNode n = a.bin[b];
     lo[s0][0] = n.a.x; lo[s0][1] = n.a.y; lo[s0][2] = n.a.z;
@@ -269,7 +269,7 @@ __global__ void k_collapse(CollapseArgs a) {
   };
   load2(f.x, 0, 1);
   int cnt = 2;
-  while (cnt < 4) {
+  while (cnt < kBvhWidth) {
     int best = -1;
     float ba = -1.f;
     for (int k = 0; k < cnt; ++k) {
@@ -284,39 +284,38 @@ __global__ void k_collapse(CollapseArgs a) {
     load2(ch[best], best, cnt);
     ++cnt;
   }
-  BvhNode4 o;
-  float* lx = &o.lox.x; float* ly = &o.loy.x; float* lz = &o.loz.x;
-  float* hx = &o.hix.x; float* hy = &o.hiy.x; float* hz = &o.hiz.x;
-  int* oc = &o.child.x;
-  for (int k = 0; k < 4; ++k) {
+  BvhNodeWide o;
+  for (int k = 0; k < kBvhWidth; ++k) {
     if (k < cnt) {
-      lx[k] = lo[k][0]; ly[k] = lo[k][1]; lz[k] = lo[k][2];
-      hx[k] = hi[k][0]; hy[k] = hi[k][1]; hz[k] = hi[k][2];
+      o.lox[k] = lo[k][0]; o.loy[k] = lo[k][1]; o.loz[k] = lo[k][2];
+      o.hix[k] = hi[k][0]; o.hiy[k] = hi[k][1]; o.hiz[k] = hi[k][2];
       if (ch[k] >= 0) {
         const int j = atomicAdd(a.node_count, 1);
         a.next[atomicAdd(a.next_count, 1)] = make_int2(ch[k], j);
-        oc[k] = j;
+        o.child[k] = j;
       } else {
-        oc[k] = ch[k];
+        o.child[k] = ch[k];
       }
     } else {
-      lx[k] = ly[k] = lz[k] = 0.f;
-      hx[k] = hy[k] = hz[k] = 0.f;
-      oc[k] = kEmptyChild;
+      o.lox[k] = o.loy[k] = o.loz[k] = 0.f;
+      o.hix[k] = o.hiy[k] = o.hiz[k] = 0.f;
+      o.child[k] = kEmptyChild;
     }
   }
-  o.pad = make_int4(0, 0, 0, 0);
+  for (int k = 0; k < (int)(sizeof(o.pad) / sizeof(int)); ++k) o.pad[k] = 0;
   a.out[f.y] = o;
 }
 
 }  // namespace
 
-// Collapses the binary tree into 4-wide nodes, one frontier level per
+int DeviceBvh::stack_bound() const { return (kBvhWidth - 1) * depth_w + 1; }
+
+// Collapses the binary tree into wide nodes, one frontier level per
 // launch (the node count per level comes back to the host, so the loop
 // knows when the tree is done).
 static int bvh_collapse(DeviceBvh& out, hipStream_t stream, std::string& err) {
-  const int cap = std::max(1, out.num_nodes);  // a 4-wide tree has fewer nodes than the binary one
-  HM_TRY(dev_alloc((void**)&out.nodes4, sizeof(BvhNode4) * (size_t)cap, err));
+  const int cap = std::max(1, out.num_nodes);  // a wide tree has fewer nodes than the binary one
+  HM_TRY(dev_alloc((void**)&out.nodes_w, sizeof(BvhNodeWide) * (size_t)cap, err));
   DeviceArena arena;
   int2 *fa = nullptr, *fb = nullptr;
   int* counters = nullptr;
@@ -330,7 +329,7 @@ static int bvh_collapse(DeviceBvh& out, hipStream_t stream, std::string& err) {
   int nfront = 1, levels = 0;
   while (nfront > 0) {
     HM_HIP(hipMemsetAsync(counters + 1, 0, sizeof(int), stream), err);
-    CollapseArgs a{out.nodes, out.nodes4, fa, nfront, fb, counters + 1, counters};
+    CollapseArgs a{out.nodes, out.nodes_w, fa, nfront, fb, counters + 1, counters};
     hipLaunchKernelGGL(k_collapse, dim3((nfront + 255) / 256), dim3(256), 0, stream, a);
     HM_TRY(check_launch("bvh collapse", err));
     int h[2];
@@ -340,12 +339,12 @@ static int bvh_collapse(DeviceBvh& out, hipStream_t stream, std::string& err) {
       err = "bvh collapse: node count exceeds the binary tree's";
       return kEDevice;
     }
-    out.num_nodes4 = h[0];
+    out.num_nodes_w = h[0];
     nfront = h[1];
     std::swap(fa, fb);
     ++levels;
   }
-  out.depth4 = levels;
+  out.depth_w = levels;
   return kOk;
 }
 

@@ -69,27 +69,27 @@ struct DeviceArena {
 };
 
 struct BvhNode;
-struct BvhNode4;
+struct BvhNodeWide;
 
 struct DeviceBvh {
   BvhNode* nodes = nullptr;    // binary tree (build intermediate, kept for tools)
-  BvhNode4* nodes4 = nullptr;  // 4-wide tree that traversal reads
+  BvhNodeWide* nodes_w = nullptr;  // wide tree that traversal reads
   float4* segs = nullptr;      // 2 per segment, Morton order
   int num_segs = 0;
   int num_nodes = 0;   // binary inner nodes
-  int num_nodes4 = 0;  // 4-wide nodes
+  int num_nodes_w = 0;  // wide nodes
   int max_depth = 0;   // binary depth
-  int depth4 = 0;      // 4-wide depth (levels of inner nodes)
-  // stack entries one ray can hold: 3 per 4-wide level plus one
-  int stack_bound() const { return 3 * depth4 + 1; }
+  int depth_w = 0;      // wide depth (levels of inner nodes)
+  // stack entries one ray can hold: width - 1 per wide level plus one
+  int stack_bound() const;
   void release() {
     dev_free(nodes);
-    dev_free(nodes4);
+    dev_free(nodes_w);
     dev_free(segs);
     nodes = nullptr;
-    nodes4 = nullptr;
+    nodes_w = nullptr;
     segs = nullptr;
-    num_segs = num_nodes = num_nodes4 = max_depth = depth4 = 0;
+    num_segs = num_nodes = num_nodes_w = max_depth = depth_w = 0;
   }
   ~DeviceBvh() { release(); }
 };
