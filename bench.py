@@ -214,7 +214,10 @@ def main() -> None:
     ap.add_argument("--no-msnn", action="store_true", help="skip the HairMSNN vs PT time-to-equal-RMSE leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (differentiable appearance) leg")
     ap.add_argument("--ref-spp", type=int, default=4096, help="passes of the RMSE reference image")
-    ap.add_argument("--pass-concurrency", type=int, default=4, help="path-tracing passes in flight in the timed region")
+    ap.add_argument("--schedule", choices=("stream", "passes"), default="stream",
+                    help="path-tracing schedule: stream (a pixel's next pass starts when its path ends) or passes "
+                         "(one wavefront per pass, --pass-concurrency of them in flight)")
+    ap.add_argument("--pass-concurrency", type=int, default=4, help="passes in flight with --schedule passes")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="instead of the benchmark, run 8 device-to-device copies of this many MiB (the known-size "
                          "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -257,6 +260,7 @@ def main() -> None:
     stats = sc.trace_stats(1)
     log(f"trace stats (1 pass): {stats}")
     # warmup
+    sc.set_pt_schedule(args.schedule)
     sc.set_pass_concurrency(args.pass_concurrency)
     sc.pt_reset()
     sc.pt_passes(args.warmup)
@@ -269,10 +273,10 @@ def main() -> None:
     value = paths_per_step * args.steps / wall
     log(f"{args.steps} passes in {wall:.3f} s -> {value / 1e6:.2f} M paths/s")
 
-    # per-kernel timing with HIP events on the stream each kernel runs on, with
-    # passes back to back so every launch runs alone (its roofline)
+    # per-kernel timing with HIP events on the stream each kernel runs on, every
+    # launch running alone (streaming: one stream; passes: one pass in flight)
     sc.set_pass_concurrency(1)
-    passes_t = max(2, min(8, args.steps))
+    passes_t = max(2, min(16, args.steps))
     kt = sc.time_passes(passes_t)
     sc.set_pass_concurrency(args.pass_concurrency)
     launches = max(kt["extend_launches"], 1)
@@ -308,7 +312,7 @@ def main() -> None:
                         "(64 steps = the 64-spp render), 50000 strands x 64 linear segments (3.2M) on a unit "
                         "head sphere, procedural 2048x1024 log-normal env + 3 hot spots + 1 directional "
                         "light, depth 40, MIS, LCG",
-            "width": W, "height": H, "paths_per_step": paths_per_step,
+            "width": W, "height": H, "paths_per_step": paths_per_step, "schedule": args.schedule,
             "segments": int(sc.info["num_segments"]), "bvh_depth": int(sc.info["bvh_depth"]),
         },
         "roofline": {

@@ -20,7 +20,7 @@ from pathlib import Path
 
 import numpy as np
 
-from ._lib import (BsdfQuery, HairParams, HmsnnError, Info, MsnnStats, check, declared_symbols, load,
+from ._lib import (BsdfQuery, HairParams, HmsnnError, Info, MsnnStats, NrcStats, check, declared_symbols, load,
                    LIB_PATH)
 
 __all__ = [
@@ -248,6 +248,11 @@ class Scene:
         """Path-tracing passes in flight at once (1..4); the image does not depend on it."""
         check(self.lib.hmsnn_set_pass_concurrency(self.h, int(passes_in_flight)))
 
+    def set_pt_schedule(self, schedule: str) -> None:
+        """"stream" (default: a pixel's next pass starts when its path ends) or
+        "passes" (one wavefront per pass); the image does not depend on it."""
+        check(self.lib.hmsnn_set_pt_schedule(self.h, {"stream": 0, "passes": 1}[schedule]))
+
     # ---- path tracing ----
     def pt_reset(self) -> None:
         check(self.lib.hmsnn_pt_reset(self.h))
@@ -321,6 +326,66 @@ class Scene:
         check(self.lib.hmsnn_render_hair_msnn(self.h, int(BETA), spp, _ptr(out), C.byref(st)))
         return out
 
+    # ---- neural radiance caching ----
+    _NRC_BUFFERS = {
+        # name: (id, dtype, shape from (n_tpx, every_nth, n_query, n_rec, w*h))
+        "query": (0, np.float32, lambda t, e, q, r, p: (q, 12)),
+        "cache": (1, np.float32, lambda t, e, q, r, p: (q, 3)),
+        "gb_radiance": (2, np.float32, lambda t, e, q, r, p: (p, 4)),
+        "gb_beta": (3, np.float32, lambda t, e, q, r, p: (p, 4)),
+        "gb_hit": (4, np.uint8, lambda t, e, q, r, p: (p,)),
+        "train_input": (5, np.float32, lambda t, e, q, r, p: (r, 12)),
+        "train_target": (6, np.float32, lambda t, e, q, r, p: (r, 3)),
+        "train_idx": (7, np.int32, lambda t, e, q, r, p: (t,)),
+        "train_bounces": (8, np.int32, lambda t, e, q, r, p: (t,)),
+        "train_hit": (9, np.uint8, lambda t, e, q, r, p: (t,)),
+    }
+
+    def nrc_begin(self) -> None:
+        check(self.lib.hmsnn_nrc_begin(self.h))
+
+    def nrc_set_params(self, c: float, all_unbiased: bool = False) -> None:
+        check(self.lib.hmsnn_nrc_set_params(self.h, float(c), int(bool(all_unbiased))))
+
+    def nrc_frames(self, frames: int) -> dict:
+        st = NrcStats()
+        check(self.lib.hmsnn_nrc_frames(self.h, int(frames), C.byref(st)))
+        return {k: getattr(st, k) for k, _ in NrcStats._fields_}
+
+    def nrc_image(self) -> np.ndarray:
+        out = np.empty((self.height, self.width, 4), np.float32)
+        check(self.lib.hmsnn_nrc_get_image(self.h, _ptr(out)))
+        return out
+
+    def nrc_sizes(self) -> dict:
+        v = np.zeros(4, np.int32)
+        check(self.lib.hmsnn_nrc_get_sizes(self.h, _ptr(v)))
+        return {"train_pixels": int(v[0]), "every_nth": int(v[1]), "queries": int(v[2]), "train_records": int(v[3])}
+
+    def nrc_buffer(self, name: str) -> np.ndarray:
+        which, dtype, shape_of = self._NRC_BUFFERS[name]
+        z = self.nrc_sizes()
+        shape = shape_of(z["train_pixels"], z["every_nth"], z["queries"], z["train_records"], self.width * self.height)
+        out = np.empty(shape, dtype)
+        check(self.lib.hmsnn_nrc_get_buffer(self.h, which, _ptr(out), out.nbytes))
+        return out
+
+    def nrc_weights(self) -> np.ndarray:
+        w = np.empty(20672, np.float32)
+        check(self.lib.hmsnn_nrc_get_mlp_weights(self.h, _ptr(w), w.size))
+        return w
+
+    def nrc_set_weights(self, w: np.ndarray) -> None:
+        w = np.ascontiguousarray(w, np.float32)
+        check(self.lib.hmsnn_nrc_set_mlp_weights(self.h, _ptr(w), w.size))
+
+    def render_nrc(self, spp: int | None = None) -> np.ndarray:
+        spp = int(self.info["spp"]) if spp is None else int(spp)
+        out = np.empty((self.height, self.width, 4), np.float32)
+        st = NrcStats()
+        check(self.lib.hmsnn_render_nrc(self.h, spp, _ptr(out), C.byref(st)))
+        return out
+
 
 def render_path_tracing(scene_json: str, spp: int | None = None) -> np.ndarray:
     """Python counterpart of the render_path_tracing executable."""
@@ -332,3 +397,9 @@ def render_hair_msnn(scene_json: str, BETA: int = 1, spp: int | None = None) -> 
     """Python counterpart of the render_hair_msnn executable (kernel beta = BETA - 1)."""
     with Scene(scene_json) as s:
         return s.render_hair_msnn(BETA, spp)
+
+
+def render_nrc(scene_json: str, spp: int | None = None) -> np.ndarray:
+    """Python counterpart of the render_nrc executable."""
+    with Scene(scene_json) as s:
+        return s.render_nrc(spp)

@@ -52,6 +52,14 @@ class MsnnStats(C.Structure):
     ]
 
 
+class NrcStats(C.Structure):
+    _fields_ = [
+        ("gbuffer_ms", C.c_double), ("infer_ms", C.c_double), ("render_ms", C.c_double), ("train_ms", C.c_double),
+        ("total_ms", C.c_double), ("frames", C.c_int), ("train_steps", C.c_int), ("cache_queries", C.c_int),
+        ("train_records_used", C.c_int),
+    ]
+
+
 _P = C.c_void_p
 _I = C.c_int
 _F = C.c_float
@@ -82,6 +90,7 @@ _SIGS = {
     "hmsnn_set_strand_sigma_a": (_I, [_P, _P, _I]),
     "hmsnn_set_seed_offset": (_I, [_P, _I]),
     "hmsnn_set_pass_concurrency": (_I, [_P, _I]),
+    "hmsnn_set_pt_schedule": (_I, [_P, _I]),
     "hmsnn_pt_reset": (_I, [_P]),
     "hmsnn_pt_passes": (_I, [_P, _I]),
     "hmsnn_ctx_synchronize": (_I, [_P]),
@@ -96,6 +105,15 @@ _SIGS = {
     "hmsnn_msnn_frames": (_I, [_P, _I, C.POINTER(MsnnStats)]),
     "hmsnn_msnn_get_image": (_I, [_P, _I, _P]),
     "hmsnn_render_hair_msnn": (_I, [_P, _I, _I, _P, C.POINTER(MsnnStats)]),
+    "hmsnn_nrc_begin": (_I, [_P]),
+    "hmsnn_nrc_set_params": (_I, [_P, _F, _I]),
+    "hmsnn_nrc_frames": (_I, [_P, _I, C.POINTER(NrcStats)]),
+    "hmsnn_nrc_get_image": (_I, [_P, _P]),
+    "hmsnn_nrc_get_sizes": (_I, [_P, _P]),
+    "hmsnn_nrc_get_buffer": (_I, [_P, _I, _P, C.c_size_t]),
+    "hmsnn_nrc_get_mlp_weights": (_I, [_P, _P, _I]),
+    "hmsnn_nrc_set_mlp_weights": (_I, [_P, _P, _I]),
+    "hmsnn_render_nrc": (_I, [_P, _I, _P, C.POINTER(NrcStats)]),
     "hmsnn_mlp_create": (_I, [_I, C.c_uint64, C.POINTER(_P)]),
     "hmsnn_mlp_destroy": (_I, [_P]),
     "hmsnn_mlp_get_weights": (_I, [_P, _P, _I]),

@@ -106,8 +106,13 @@ Context::~Context() {
     f.stream = nullptr;
   }
   if (acc_event) (void)hipEventDestroy(acc_event);
+  for (hipEvent_t& e : live_ev)
+    if (e) (void)hipEventDestroy(e), e = nullptr;
+  if (live_host) (void)hipHostFree(live_host);
   msnn_release(msnn);
   msnn = nullptr;
+  nrc_release(nrc);
+  nrc = nullptr;
   free_path_buffers();
   dev_free(scene_points);
   dev_free(seg_dev);
@@ -160,6 +165,9 @@ int alloc_path_buffers(PathBuffers& pb, std::vector<void*>& allocs, size_t n, bo
   HM_TRY(A((void**)&pb.shadow_rays, sizeof(int) * 2 * n));
   HM_TRY(A((void**)&pb.sh_vis, 2 * n));
   HM_TRY(A((void**)&pb.counts, sizeof(int) * 16));
+  HM_TRY(A((void**)&pb.slot_pass, sizeof(int) * n));
+  HM_TRY(A((void**)&pb.slot_depth, sizeof(int) * n));
+  HM_TRY(A((void**)&pb.done_queue, sizeof(int) * n));
   if (msnn_mode) {
     HM_TRY(A((void**)&pb.beta_short, f4));
     HM_TRY(A((void**)&pb.L_short, f4));
@@ -575,6 +583,15 @@ int hmsnn_set_pass_concurrency(hmsnn_ctx* ctx, int passes_in_flight) {
   return HMSNN_OK;
 }
 
+int hmsnn_set_pt_schedule(hmsnn_ctx* ctx, int schedule) {
+  if (!ctx || (schedule != HMSNN_PT_STREAM && schedule != HMSNN_PT_PASSES))
+    return fail(HMSNN_EINVAL, "hmsnn_set_pt_schedule: expects HMSNN_PT_STREAM or HMSNN_PT_PASSES");
+  Context& c = ctx->c;
+  (void)hipStreamSynchronize(c.stream);
+  c.pt_schedule = schedule;
+  return HMSNN_OK;
+}
+
 int hmsnn_set_seed_offset(hmsnn_ctx* ctx, int offset) {
   if (!ctx || offset < 0) return fail(HMSNN_EINVAL, "hmsnn_set_seed_offset: bad arguments");
   ctx->c.seed_offset = offset;
@@ -593,10 +610,26 @@ int hmsnn_pt_reset(hmsnn_ctx* ctx) {
 // single stream makes), and the context stream waits for the last one.
 static int run_pt_passes(Context& c, int passes, std::string& err, PassTimer* timer = nullptr) {
   HM_TRY(c.ensure_path_buffers(false, err));
-  HM_TRY(c.ensure_flights(err));
   SceneView v = c.view;
   v.bvh.nodes = c.bvh.nodes_w;
   const int n = c.scene.width * c.scene.height;
+  if (c.pt_schedule == 0) {
+    if (passes <= 0) return kOk;
+    if (!c.live_host) {
+      HM_HIP(hipHostMalloc((void**)&c.live_host, 2 * sizeof(int)), err);
+      for (hipEvent_t& e : c.live_ev) HM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming), err);
+    }
+    if (c.acc_pending) HM_HIP(hipStreamWaitEvent(c.stream, c.acc_event, 0), err);
+    c.acc_pending = false;
+    PassConfig cfg = c.pt_pass_config();
+    cfg.timer = timer;
+    HM_TRY(pt_run_stream(v, c.pb, cfg, passes, c.accum, c.average, c.live_host, c.live_ev, c.stream, err));
+    c.accum_id += passes;
+    c.last_streamed = true;
+    return kOk;
+  }
+  c.last_streamed = false;
+  HM_TRY(c.ensure_flights(err));
   const int K = std::max(1, std::min(c.num_flights, Context::kMaxFlights));
   hipEvent_t start = nullptr;
   HM_HIP(hipEventCreateWithFlags(&start, hipEventDisableTiming), err);
@@ -661,7 +694,7 @@ int hmsnn_get_last_pass(hmsnn_ctx* ctx, float* rgba) {
   Context& c = ctx->c;
   if (!c.pb.L || c.accum_id == 0) return fail(HMSNN_EINVAL, "nothing rendered yet");
   const int K = std::max(1, std::min(c.num_flights, Context::kMaxFlights));
-  const PathBuffers& last = c.flight_pb((c.accum_id - 1) % K);
+  const PathBuffers& last = c.last_streamed ? c.pb : c.flight_pb((c.accum_id - 1) % K);
   const size_t bytes = sizeof(float4) * (size_t)c.scene.width * c.scene.height;
   hipError_t e = hipMemcpyAsync(rgba, last.L, bytes, hipMemcpyDeviceToHost, c.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
@@ -712,9 +745,10 @@ int hmsnn_pt_time_passes(hmsnn_ctx* ctx, int passes, double out[5]) {
   std::string err;
   int rc = c.ensure_path_buffers(false, err);
   if (rc) return fail(rc, err);
-  const int per_pass = 3 * (c.scene.path_v2 + 1);
+  // records per vertex iteration: extend, shade, shadow (+ finish when streaming)
+  const int per_pass = 4 * (c.scene.path_v2 + 1);
   PassTimer tm;
-  tm.capacity = per_pass * passes;
+  tm.capacity = per_pass * passes + 4 * 64;
   std::vector<hipEvent_t> ev(2 * (size_t)tm.capacity);
   std::vector<int> kinds((size_t)tm.capacity);
   for (auto& x : ev) (void)hipEventCreate(&x);

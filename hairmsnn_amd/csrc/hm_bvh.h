@@ -148,94 +148,142 @@ HM_D void cswap(float& ta, int& ca, float& tb, int& cb) {
   ca = c;
 }
 
+// Per-lane traversal state, advanced one step (one wide node or one segment)
+// at a time so that a kernel can hand an idle lane a new ray while the rest of
+// its wave is still traversing.
+struct ClosestState {
+  float best_t, best_u;
+  int best_id;
+  int cur;   // current entry: node (>= 0), ~leaf (< 0) or kNone when done
+  int iter;  // steps taken (cap for a malformed tree)
+};
+
+HM_D int trace_cap(const BvhView& bvh) { return 2 * bvh.num_segs + 64; }
+
+// Starts a closest-hit query with (best_t, best_id) preset by the caller
+// (e.g. a hit on the head sphere).
+HM_D void closest_begin(const BvhView& bvh, TraceStack& st, ClosestState& s) {
+  st.sp = 0;
+  s.cur = bvh.num_segs > 0 ? 0 : kNone;
+  s.iter = 0;
+}
+
 // Closest hit over hair segments: minimum (t, id) lexicographically.
+template <bool kCount>
+HM_D void closest_step(const BvhView& bvh, const RayInv& r, TraceStack& st, ClosestState& s, TraceCounters& cnt) {
+  if (s.cur < 0) {
+    const int leaf = ~s.cur;
+    const float4 s0 = bvh.segs[2 * leaf];
+    const float4 s1 = bvh.segs[2 * leaf + 1];
+    if (kCount) cnt.segs++;
+    float t, u;
+    if (intersect_segment(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, s0.x, s0.y, s0.z, s1.x, s1.y, s1.z, s0.w, &t, &u)) {
+      const int id = __float_as_int(s1.w);
+      if (t < s.best_t || (t == s.best_t && id < s.best_id)) {
+        s.best_t = t;
+        s.best_id = id;
+        s.best_u = u;
+      }
+    }
+    s.cur = st.sp ? st.pop() : kNone;
+  } else {
+    const BvhNodeWide n = bvh.nodes[s.cur];
+    if (kCount) cnt.nodes++;
+    float t[kBvhWidth];
+    node_hits(n, r, s.best_t, t);
+    int c[kBvhWidth];
+#pragma unroll
+    for (int k = 0; k < kBvhWidth; ++k) c[k] = n.child[k];
+    // sorting network (Batcher): nearest first, misses (+inf) last
+    if (kBvhWidth == 4) {
+      cswap(t[0], c[0], t[1], c[1]);
+      cswap(t[2], c[2], t[3], c[3]);
+      cswap(t[0], c[0], t[2], c[2]);
+      cswap(t[1], c[1], t[3], c[3]);
+      cswap(t[1], c[1], t[2], c[2]);
+    } else {
+      constexpr int net[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
+                                  {0, 4}, {1, 5}, {2, 6}, {3, 7}, {2, 4}, {3, 5}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+      for (int q = 0; q < 19; ++q) cswap(t[net[q][0]], c[net[q][0]], t[net[q][1]], c[net[q][1]]);
+    }
+    const float inf = __builtin_inff();
+#pragma unroll
+    for (int k = kBvhWidth - 1; k >= 1; --k)
+      if (t[k] < inf) st.push(c[k]);
+    s.cur = t[0] < inf ? c[0] : (st.sp ? st.pop() : kNone);
+  }
+  if (++s.iter >= trace_cap(bvh)) s.cur = kNone;
+}
+
 template <bool kCount>
 HM_D void trace_closest(const BvhView& bvh, const RayInv& r, TraceStack& st, float& best_t, int& best_id,
                         float& best_u, TraceCounters& cnt) {
-  if (bvh.num_segs <= 0) return;
-  st.sp = 0;
-  int cur = 0;
-  for (int iter = 0; iter < 2 * bvh.num_segs + 64 && cur != kNone; ++iter) {
-    if (cur < 0) {
-      const int leaf = ~cur;
-      const float4 s0 = bvh.segs[2 * leaf];
-      const float4 s1 = bvh.segs[2 * leaf + 1];
-      if (kCount) cnt.segs++;
-      float t, u;
-      if (intersect_segment(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, s0.x, s0.y, s0.z, s1.x, s1.y, s1.z, s0.w, &t, &u)) {
-        const int id = __float_as_int(s1.w);
-        if (t < best_t || (t == best_t && id < best_id)) {
-          best_t = t;
-          best_id = id;
-          best_u = u;
-        }
-      }
-      cur = st.sp ? st.pop() : kNone;
-    } else {
-      const BvhNodeWide n = bvh.nodes[cur];
-      if (kCount) cnt.nodes++;
-      float t[kBvhWidth];
-      node_hits(n, r, best_t, t);
-      int c[kBvhWidth];
-#pragma unroll
-      for (int k = 0; k < kBvhWidth; ++k) c[k] = n.child[k];
-      // sorting network (Batcher): nearest first, misses (+inf) last
-      if (kBvhWidth == 4) {
-        cswap(t[0], c[0], t[1], c[1]);
-        cswap(t[2], c[2], t[3], c[3]);
-        cswap(t[0], c[0], t[2], c[2]);
-        cswap(t[1], c[1], t[3], c[3]);
-        cswap(t[1], c[1], t[2], c[2]);
-      } else {
-        constexpr int net[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
-                                    {0, 4}, {1, 5}, {2, 6}, {3, 7}, {2, 4}, {3, 5}, {1, 2}, {3, 4}, {5, 6}};
-#pragma unroll
-        for (int q = 0; q < 19; ++q) cswap(t[net[q][0]], c[net[q][0]], t[net[q][1]], c[net[q][1]]);
-      }
-      const float inf = __builtin_inff();
-#pragma unroll
-      for (int k = kBvhWidth - 1; k >= 1; --k)
-        if (t[k] < inf) st.push(c[k]);
-      cur = t[0] < inf ? c[0] : (st.sp ? st.pop() : kNone);
-    }
-  }
+  ClosestState s;
+  s.best_t = best_t;
+  s.best_id = best_id;
+  s.best_u = best_u;
+  closest_begin(bvh, st, s);
+  while (s.cur != kNone) closest_step<kCount>(bvh, r, st, s, cnt);
+  best_t = s.best_t;
+  best_id = s.best_id;
+  best_u = s.best_u;
 }
 
 // Any hit with t > 0 over hair segments (binary visibility); children are
 // taken in slot order.
+struct AnyState {
+  int cur;
+  int iter;
+  bool hit;
+};
+
+HM_D void any_begin(const BvhView& bvh, TraceStack& st, AnyState& s) {
+  st.sp = 0;
+  s.cur = bvh.num_segs > 0 ? 0 : kNone;
+  s.iter = 0;
+  s.hit = false;
+}
+
+template <bool kCount>
+HM_D void any_step(const BvhView& bvh, const RayInv& r, TraceStack& st, AnyState& s, TraceCounters& cnt) {
+  const float tinf = __builtin_inff();
+  if (s.cur < 0) {
+    const int leaf = ~s.cur;
+    const float4 s0 = bvh.segs[2 * leaf];
+    const float4 s1 = bvh.segs[2 * leaf + 1];
+    if (kCount) cnt.segs++;
+    float t, u;
+    if (intersect_segment(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, s0.x, s0.y, s0.z, s1.x, s1.y, s1.z, s0.w, &t, &u)) {
+      s.hit = true;
+      s.cur = kNone;
+    } else {
+      s.cur = st.sp ? st.pop() : kNone;
+    }
+  } else {
+    const BvhNodeWide n = bvh.nodes[s.cur];
+    if (kCount) cnt.nodes++;
+    float t[kBvhWidth];
+    node_hits(n, r, tinf, t);
+    int next = kNone;
+#pragma unroll
+    for (int k = kBvhWidth - 1; k >= 0; --k) {
+      if (t[k] < tinf) {
+        if (next != kNone) st.push(next);
+        next = n.child[k];
+      }
+    }
+    s.cur = next != kNone ? next : (st.sp ? st.pop() : kNone);
+  }
+  if (++s.iter >= trace_cap(bvh)) s.cur = kNone;
+}
+
 template <bool kCount>
 HM_D bool trace_any(const BvhView& bvh, const RayInv& r, TraceStack& st, TraceCounters& cnt) {
-  if (bvh.num_segs <= 0) return false;
-  const float tinf = __builtin_inff();
-  st.sp = 0;
-  int cur = 0;
-  for (int iter = 0; iter < 2 * bvh.num_segs + 64 && cur != kNone; ++iter) {
-    if (cur < 0) {
-      const int leaf = ~cur;
-      const float4 s0 = bvh.segs[2 * leaf];
-      const float4 s1 = bvh.segs[2 * leaf + 1];
-      if (kCount) cnt.segs++;
-      float t, u;
-      if (intersect_segment(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, s0.x, s0.y, s0.z, s1.x, s1.y, s1.z, s0.w, &t, &u))
-        return true;
-      cur = st.sp ? st.pop() : kNone;
-    } else {
-      const BvhNodeWide n = bvh.nodes[cur];
-      if (kCount) cnt.nodes++;
-      float t[kBvhWidth];
-      node_hits(n, r, tinf, t);
-      int next = kNone;
-#pragma unroll
-      for (int k = kBvhWidth - 1; k >= 0; --k) {
-        if (t[k] < tinf) {
-          if (next != kNone) st.push(next);
-          next = n.child[k];
-        }
-      }
-      cur = next != kNone ? next : (st.sp ? st.pop() : kNone);
-    }
-  }
-  return false;
+  AnyState s;
+  any_begin(bvh, st, s);
+  while (s.cur != kNone) any_step<kCount>(bvh, r, st, s, cnt);
+  return s.hit;
 }
 
 #endif  // __HIPCC__

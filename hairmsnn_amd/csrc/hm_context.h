@@ -16,6 +16,8 @@ namespace hm {
 struct MlpState;  // radiance-cache network (hm_mlp.hip)
 struct MsnnState;  // HairMSNN frame state (hm_msnn.hip)
 void msnn_release(MsnnState* m);
+struct NrcState;  // neural radiance caching frame state (hm_nrc.hip)
+void nrc_release(NrcState* s);
 
 // An extra set of path buffers with its own stream, so that independent
 // passes overlap on the GPU (the last depths of a pass launch kernels whose
@@ -67,12 +69,19 @@ struct Context {
   unsigned long long* stats_dev = nullptr;
 
   MsnnState* msnn = nullptr;
+  NrcState* nrc = nullptr;
   // path-tracing passes in flight at once: flight 0 is (pb, stream)
   static constexpr int kMaxFlights = 4;
   int num_flights = 4;
   Flight extra[kMaxFlights - 1];
   hipEvent_t acc_event = nullptr;  // last accumulate issued (any flight)
   bool acc_pending = false;
+  // plain path tracing schedule: 0 streaming (path regeneration, one set of
+  // path buffers), 1 one wavefront per pass with passes in flight
+  int pt_schedule = 0;
+  bool last_streamed = false;   // the last pt call used the streaming schedule
+  int* live_host = nullptr;     // pinned, 2 ints: live-path count polls
+  hipEvent_t live_ev[2] = {nullptr, nullptr};
   int ensure_flights(std::string& err);
   PathBuffers& flight_pb(int f) { return f == 0 ? pb : extra[f - 1].pb; }
   hipStream_t flight_stream(int f) { return f == 0 ? stream : extra[f - 1].stream; }

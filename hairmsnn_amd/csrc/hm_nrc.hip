@@ -1,0 +1,450 @@
+// Neural radiance caching, the reference's comparator renderer
+// (render_nrc.cu:116-701, cuda/nrc.cu:18-398). Per frame:
+//   shuffle  : the training-pixel indices (one pixel in every_nth)
+//   G_BUFFER : every pixel's path until its spread exceeds the camera
+//              footprint, where it queries the cache; training pixels go on
+//              with a suffix that ends in a second query (or, for every 16th,
+//              runs to the end unbiased) and record every vertex
+//   infer    : the cache at all queries (frame + training suffixes)
+//   RENDER   : pixel = radiance before the query + throughput x cache;
+//              training targets = suffix radiance sums + throughput x cache
+//   train    : one Adam step on the training records
+// The cache network is the fused MLP of hm_mlp.hip (12-float records; NRC
+// fills position / scene scale, wo and the shading normal, channels 9-11 0).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "../../include/hmsnn.h"
+#include "hm_api_util.h"
+#include "hm_context.h"
+#include "hm_mlp.h"
+
+namespace hm {
+
+struct NrcState {
+  MlpState* mlp = nullptr;
+  int n_rec = 0;       // training records per step (numTrainingRecords)
+  int n_tpx = 0;       // training pixels (numTrainingPixels)
+  int every_nth = 1;
+  int frame_size = 0;  // pixels
+  int n_query = 0;     // nnFrameSize: frame + training queries, a multiple of 128
+  float c = 0.01f;
+  int all_unbiased = 0;
+  int loss = HMSNN_LOSS_RELATIVE_L2_LUMINANCE;
+  float lr = 1e-2f;
+  int train_steps = 0;
+  int frame = 0;
+  uint64_t rng_state = 0;
+  std::vector<int> idx;  // current permutation of the training indices
+  std::vector<void*> allocs;
+  int* idx_host = nullptr;  // pinned staging
+  int* train_idx = nullptr;
+  float* query = nullptr;
+  float* cache = nullptr;  // n_query x 3
+  float4* gb_beta = nullptr;
+  float4* gb_rad = nullptr;  // radiance before the query (RENDER pass input), per pixel
+  uint8_t* gb_hit = nullptr;
+  float4* state = nullptr;
+  float4* prev = nullptr;
+  float4* tb[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // pos, wo, n, rad, beta
+  int* tb_bounces = nullptr;
+  uint8_t* tb_hit = nullptr;
+  float* train_in = nullptr;   // n_rec x 12
+  float* train_tgt = nullptr;  // n_rec x 3
+  float4* acc = nullptr;
+  float4* avg = nullptr;
+  int* counts_dev = nullptr;   // [0] frame cache queries, [1] training records with data
+  double loss_last = 0.0;
+  ~NrcState() {
+    for (void* p : allocs) dev_free(p);
+    if (idx_host) (void)hipHostFree(idx_host);
+    if (mlp) mlp_destroy(mlp);
+  }
+  uint64_t next_u64() {
+    uint64_t z = (rng_state += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+};
+
+void nrc_release(NrcState* s) { delete s; }
+
+namespace {
+
+// RENDER pass, pixel side (cuda/nrc.cu:378-392): radiance before the query
+// plus throughput x max(cache, 0), a NaN cache counting as 0; accumulated
+// like writePixel (utils.cuh:13-35, NaN scrub). Also stores the G-buffer
+// radiance for inspection and counts the frame's cache queries.
+__global__ void k_nrc_render(PathBuffers pb, NrcPassArgs nr, const float* cache, float4* gb_rad, float4* acc,
+                             float4* avg, int accum_id, int* counts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = i < nr.frame_size;
+  bool queried = false;
+  if (in) {
+    const float4 l4 = (pb.flags[i] & kNrcSuffix) ? pb.L_short[i] : pb.L[i];
+    const float4 b4 = nr.gb_beta[i];
+    V3 cr = v3(cache[3 * (size_t)i], cache[3 * (size_t)i + 1], cache[3 * (size_t)i + 2]);
+    if (any_nan(cr)) cr = v3(0.f);
+    cr = v3(fmaxf(cr.x, 0.f), fmaxf(cr.y, 0.f), fmaxf(cr.z, 0.f));
+    V3 col = v3(l4.x, l4.y, l4.z) + v3(b4.x, b4.y, b4.z) * cr;
+    gb_rad[i] = make_float4(l4.x, l4.y, l4.z, 0.f);
+    queried = b4.x != 0.f || b4.y != 0.f || b4.z != 0.f;
+    if (any_nan(col)) col = v3(0.f);
+    if (accum_id > 0) {
+      const float4 a = acc[i];
+      col = col + v3(a.x, a.y, a.z);
+    }
+    acc[i] = make_float4(col.x, col.y, col.z, 1.f);
+    const float inv = 1.f / (float)(accum_id + 1);
+    avg[i] = make_float4(col.x * inv, col.y * inv, col.z * inv, 1.f);
+  }
+  const unsigned long long m = __ballot(queried);
+  if (__lane_id() == 0 && m) atomicAdd(&counts[0], __popcll(m));
+}
+
+// RENDER pass, training side (nrcGenerateTrainingData, cuda/nrc.cu:69-133):
+// for training pixel tr (whose camera ray hit) and each recorded vertex b <
+// bounces, target = sum over the later vertices of their direct light times
+// the throughput from b, plus that throughput times the suffix's cache value
+// (0 for unbiased suffixes or suffixes without a query); NaN targets become
+// 0. The sums run in the reference's order. Records beyond `bounces` stay 0.
+__global__ void k_nrc_train_data(NrcPassArgs nr, const int* train_idx, const float* cache, float* train_in,
+                                 float* train_tgt, int n_rec, int* counts) {
+  const int tr = blockIdx.x * blockDim.x + threadIdx.x;
+  int used = 0;
+  if (tr < nr.n_tpx) {
+    const int px = tr * nr.every_nth + train_idx[tr] % nr.every_nth;
+    if (px < nr.frame_size && nr.gb_hit[px]) {
+      const int B = nr.tb_bounces[tr];
+      V3 cr = v3(0.f);
+      if (nr.tb_hit[tr] && tr % 16 != 0 && !nr.all_unbiased) {
+        const size_t qi = (size_t)nr.frame_size + tr;
+        cr = v3(cache[3 * qi], cache[3 * qi + 1], cache[3 * qi + 2]);
+      }
+      const size_t base = (size_t)tr * kNrcMaxBounces;
+      for (int b = 0; b < B && b < kNrcMaxBounces; ++b) {
+        V3 color = v3(0.f), beta = v3(1.f);
+        for (int sub = b; sub < B && sub < kNrcMaxBounces; ++sub) {
+          const float4 r = nr.tb_rad[base + sub], vb = nr.tb_beta[base + sub];
+          color = color + beta * v3(r.x, r.y, r.z);
+          beta = beta * v3(vb.x, vb.y, vb.z);
+        }
+        const size_t rec = base + b;
+        if ((int)rec >= n_rec) break;
+        const float4 p = nr.tb_pos[rec], wo = nr.tb_wo[rec], n = nr.tb_n[rec];
+        float* in = train_in + rec * kNnInputCh;
+        in[0] = p.x; in[1] = p.y; in[2] = p.z;
+        in[3] = wo.x; in[4] = wo.y; in[5] = wo.z;
+        in[6] = n.x; in[7] = n.y; in[8] = n.z;
+        in[9] = 0.f; in[10] = 0.f; in[11] = 0.f;
+        V3 t = color + beta * cr;
+        if (any_nan(t)) t = v3(0.f);
+        train_tgt[3 * rec] = t.x;
+        train_tgt[3 * rec + 1] = t.y;
+        train_tgt[3 * rec + 2] = t.z;
+        ++used;
+      }
+    }
+  }
+  // wave sum of the records written
+  for (int off = 32; off > 0; off >>= 1) used += __shfl_xor(used, off);
+  if (__lane_id() == 0 && used) atomicAdd(&counts[1], used);
+}
+
+struct NrcTimer {
+  hipEvent_t e[5];
+  NrcTimer() {
+    for (auto& x : e) (void)hipEventCreate(&x);
+  }
+  NrcTimer(const NrcTimer&) = delete;
+  NrcTimer& operator=(const NrcTimer&) = delete;
+  ~NrcTimer() {
+    for (auto& x : e) (void)hipEventDestroy(x);
+  }
+  static double ms(hipEvent_t a, hipEvent_t b) {
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, a, b);
+    return t;
+  }
+};
+
+float nrc_lr_at(float base, int step) {
+  if (step < 4000) return base;
+  return base * std::pow(0.33f, (float)((step - 4000) / 4000 + 1));
+}
+
+int nrc_setup(Context& c, std::string& err) {
+  nrc_release(c.nrc);
+  c.nrc = nullptr;
+  NrcState* m = new NrcState();
+  c.nrc = m;
+  const Scene& s = c.scene;
+  const int npx = s.width * s.height;
+  m->frame_size = npx;
+  m->n_rec = std::max(128, s.nrc_train_records / 128 * 128);
+  // render_nrc.cu:130-146, integer divisions as in the reference
+  m->n_tpx = std::max(1, m->n_rec / kNrcMaxBounces);
+  m->every_nth = std::max(1, npx / m->n_tpx);
+  m->n_query = npx + m->n_tpx - m->n_tpx % 128 + 128;
+  m->c = s.nrc_c;
+  m->all_unbiased = s.nrc_all_unbiased ? 1 : 0;
+  m->lr = s.msnn_lr;
+  m->loss = s.msnn_loss == "RelativeL2" ? HMSNN_LOSS_RELATIVE_L2
+            : s.msnn_loss == "L2"        ? HMSNN_LOSS_L2
+                                         : HMSNN_LOSS_RELATIVE_L2_LUMINANCE;
+  m->rng_state = s.msnn_seed * 0x2545F4914F6CDD1Dull + 5;
+  const int prec = s.msnn_precision == "bf16" ? HMSNN_BF16 : s.msnn_precision == "fp8" ? HMSNN_FP8 : HMSNN_FP16;
+  HM_TRY(mlp_create(prec, s.msnn_seed, &m->mlp, err));
+  HM_TRY(c.ensure_path_buffers(true, err));
+  auto A = [&](void** p, size_t bytes) -> int {
+    HM_TRY(dev_alloc(p, bytes, err));
+    m->allocs.push_back(*p);
+    return kOk;
+  };
+  const size_t nv = (size_t)m->n_tpx * kNrcMaxBounces;
+  HM_TRY(A((void**)&m->train_idx, sizeof(int) * (size_t)m->n_tpx));
+  HM_TRY(A((void**)&m->query, sizeof(float) * kNnInputCh * (size_t)m->n_query));
+  HM_TRY(A((void**)&m->cache, sizeof(float) * 3 * (size_t)m->n_query));
+  HM_TRY(A((void**)&m->gb_beta, sizeof(float4) * (size_t)npx));
+  HM_TRY(A((void**)&m->gb_rad, sizeof(float4) * (size_t)npx));
+  HM_TRY(A((void**)&m->gb_hit, (size_t)npx));
+  HM_TRY(A((void**)&m->state, sizeof(float4) * (size_t)npx));
+  HM_TRY(A((void**)&m->prev, sizeof(float4) * (size_t)npx));
+  for (float4*& t : m->tb) HM_TRY(A((void**)&t, sizeof(float4) * nv));
+  HM_TRY(A((void**)&m->tb_bounces, sizeof(int) * (size_t)m->n_tpx));
+  HM_TRY(A((void**)&m->tb_hit, (size_t)m->n_tpx));
+  HM_TRY(A((void**)&m->train_in, sizeof(float) * kNnInputCh * (size_t)m->n_rec));
+  HM_TRY(A((void**)&m->train_tgt, sizeof(float) * 3 * (size_t)m->n_rec));
+  HM_TRY(A((void**)&m->acc, sizeof(float4) * (size_t)npx));
+  HM_TRY(A((void**)&m->avg, sizeof(float4) * (size_t)npx));
+  HM_TRY(A((void**)&m->counts_dev, sizeof(int) * 4));
+  HM_HIP(hipMemsetAsync(m->avg, 0, sizeof(float4) * (size_t)npx, c.stream), err);
+  HM_HIP(hipHostMalloc((void**)&m->idx_host, sizeof(int) * (size_t)m->n_tpx), err);
+  m->idx.resize(m->n_tpx);
+  for (int i = 0; i < m->n_tpx; ++i) m->idx[i] = i;  // thrust::sequence (render_nrc.cu:170-173)
+  return kOk;
+}
+
+NrcPassArgs nrc_args(const Context& c, const NrcState& m) {
+  NrcPassArgs a{};
+  a.enabled = 1;
+  a.all_unbiased = m.all_unbiased;
+  a.c = m.c;
+  a.scene_scale = c.scene_scale;
+  a.frame_size = m.frame_size;
+  a.n_tpx = m.n_tpx;
+  a.every_nth = m.every_nth;
+  a.query = m.query;
+  a.gb_beta = m.gb_beta;
+  a.gb_hit = m.gb_hit;
+  a.state = m.state;
+  a.prev = m.prev;
+  a.tb_pos = m.tb[0];
+  a.tb_wo = m.tb[1];
+  a.tb_n = m.tb[2];
+  a.tb_rad = m.tb[3];
+  a.tb_beta = m.tb[4];
+  a.tb_bounces = m.tb_bounces;
+  a.tb_hit = m.tb_hit;
+  return a;
+}
+
+// One frame (render_nrc.cu:640-701). tm, when given, receives events around
+// G_BUFFER [0..1], inference [1..2], RENDER [2..3] and training [3..4].
+int nrc_frame(Context& c, NrcState& m, NrcTimer* tm, std::string& err) {
+  const int npx = m.frame_size;
+  hipStream_t st = c.stream;
+  if (tm) (void)hipEventRecord(tm->e[0], st);
+  // ---- shuffle (render_nrc.cu:652-655): seeded, applied to the previous order ----
+  HM_HIP(hipStreamSynchronize(st), err);  // idx_host is free again
+  for (int i = m.n_tpx - 1; i > 0; --i) std::swap(m.idx[i], m.idx[(int)(m.next_u64() % (uint64_t)(i + 1))]);
+  std::copy(m.idx.begin(), m.idx.end(), m.idx_host);
+  HM_HIP(hipMemcpyAsync(m.train_idx, m.idx_host, sizeof(int) * (size_t)m.n_tpx, hipMemcpyHostToDevice, st), err);
+  // ---- clear the frame's buffers (owlBufferClear + RESET, render_nrc.cu:684-692) ----
+  const size_t nv = (size_t)m.n_tpx * kNrcMaxBounces;
+  HM_HIP(hipMemsetAsync(m.query, 0, sizeof(float) * kNnInputCh * (size_t)m.n_query, st), err);
+  HM_HIP(hipMemsetAsync(m.gb_beta, 0, sizeof(float4) * (size_t)npx, st), err);
+  HM_HIP(hipMemsetAsync(m.gb_hit, 0, (size_t)npx, st), err);
+  for (float4* t : m.tb) HM_HIP(hipMemsetAsync(t, 0, sizeof(float4) * nv, st), err);
+  HM_HIP(hipMemsetAsync(m.tb_bounces, 0, sizeof(int) * (size_t)m.n_tpx, st), err);
+  HM_HIP(hipMemsetAsync(m.tb_hit, 0, (size_t)m.n_tpx, st), err);
+  HM_HIP(hipMemsetAsync(m.train_in, 0, sizeof(float) * kNnInputCh * (size_t)m.n_rec, st), err);
+  HM_HIP(hipMemsetAsync(m.train_tgt, 0, sizeof(float) * 3 * (size_t)m.n_rec, st), err);
+  HM_HIP(hipMemsetAsync(m.counts_dev, 0, sizeof(int) * 4, st), err);
+  // ---- G_BUFFER ----
+  PassConfig cfg = c.pt_pass_config();
+  cfg.accum_id = m.frame;
+  cfg.dl_zero = 1;  // nrcTracePaths lights every vertex (nrc.cu:166-169)
+  cfg.dl_from = 1;
+  cfg.max_vertex = kNrcMaxBounces - 1;
+  cfg.train_idx = m.train_idx;
+  cfg.every_nth = m.every_nth;
+  cfg.nrc = nrc_args(c, m);
+  HM_TRY(pt_run_pass(c.view, c.pb, cfg, st, err));
+  if (tm) (void)hipEventRecord(tm->e[1], st);
+  // ---- infer ----
+  HM_TRY(mlp_forward(*m.mlp, m.query, m.n_query, m.cache, st, err));
+  if (tm) (void)hipEventRecord(tm->e[2], st);
+  // ---- RENDER ----
+  hipLaunchKernelGGL(k_nrc_render, dim3((npx + 255) / 256), dim3(256), 0, st, c.pb, cfg.nrc, m.cache, m.gb_rad,
+                     m.acc, m.avg, m.frame, m.counts_dev);
+  hipLaunchKernelGGL(k_nrc_train_data, dim3((m.n_tpx + 255) / 256), dim3(256), 0, st, cfg.nrc, m.train_idx,
+                     m.cache, m.train_in, m.train_tgt, m.n_rec, m.counts_dev);
+  HM_TRY(check_launch("nrc render", err));
+  if (tm) (void)hipEventRecord(tm->e[3], st);
+  // ---- train ----
+  if (c.scene.nrc_train) {
+    const float lr = nrc_lr_at(m.lr, m.train_steps);
+    HM_TRY(mlp_loss_grad_step(*m.mlp, m.train_in, m.train_tgt, m.n_rec, m.loss, nullptr, nullptr, true, lr, st,
+                              err));
+    m.train_steps++;
+  }
+  if (tm) (void)hipEventRecord(tm->e[4], st);
+  m.frame++;
+  return kOk;
+}
+
+}  // namespace
+
+}  // namespace hm
+
+using namespace hm;
+
+struct hmsnn_ctx {
+  Context c;
+};
+
+extern "C" {
+
+int hmsnn_nrc_begin(hmsnn_ctx* ctx) {
+  if (!ctx) return fail(HMSNN_EINVAL, "NULL context");
+  std::string err;
+  Context& c = ctx->c;
+  int rc = nrc_setup(c, err);
+  if (!rc && hipStreamSynchronize(c.stream) != hipSuccess) rc = HMSNN_EDEVICE, err = "NRC setup failed";
+  return rc ? fail(rc, err) : HMSNN_OK;
+}
+
+int hmsnn_nrc_set_params(hmsnn_ctx* ctx, float c_spread, int all_unbiased) {
+  if (!ctx || !(c_spread >= 0.f)) return fail(HMSNN_EINVAL, "hmsnn_nrc_set_params: bad arguments");
+  if (!ctx->c.nrc) return fail(HMSNN_EINVAL, "call hmsnn_nrc_begin first");
+  ctx->c.nrc->c = c_spread;
+  ctx->c.nrc->all_unbiased = all_unbiased ? 1 : 0;
+  return HMSNN_OK;
+}
+
+int hmsnn_nrc_frames(hmsnn_ctx* ctx, int frames, hmsnn_nrc_stats* stats) {
+  if (!ctx || frames < 0) return fail(HMSNN_EINVAL, "hmsnn_nrc_frames: bad arguments");
+  Context& c = ctx->c;
+  if (!c.nrc) return fail(HMSNN_EINVAL, "call hmsnn_nrc_begin first");
+  std::string err;
+  std::vector<NrcTimer> timers(stats ? (size_t)frames : 0);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int k = 0; k < frames; ++k) {
+    int rc = nrc_frame(c, *c.nrc, stats ? &timers[k] : nullptr, err);
+    if (rc) return fail(rc, err);
+  }
+  hipError_t e = hipStreamSynchronize(c.stream);
+  if (e != hipSuccess) return fail(HMSNN_EDEVICE, hipGetErrorString(e));
+  const auto t1 = std::chrono::steady_clock::now();
+  if (stats) {
+    hmsnn_nrc_stats local{};
+    for (NrcTimer& tm : timers) {
+      local.gbuffer_ms += NrcTimer::ms(tm.e[0], tm.e[1]);
+      local.infer_ms += NrcTimer::ms(tm.e[1], tm.e[2]);
+      local.render_ms += NrcTimer::ms(tm.e[2], tm.e[3]);
+      local.train_ms += NrcTimer::ms(tm.e[3], tm.e[4]);
+    }
+    local.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    local.frames = frames;
+    local.train_steps = c.nrc->train_steps;
+    int counts[4] = {0, 0, 0, 0};
+    e = hipMemcpy(counts, c.nrc->counts_dev, sizeof(counts), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail(HMSNN_EDEVICE, hipGetErrorString(e));
+    local.cache_queries = counts[0];
+    local.train_records_used = counts[1];
+    *stats = local;
+  }
+  return HMSNN_OK;
+}
+
+int hmsnn_nrc_get_image(hmsnn_ctx* ctx, float* rgba) {
+  if (!ctx || !rgba) return fail(HMSNN_EINVAL, "hmsnn_nrc_get_image: bad arguments");
+  Context& c = ctx->c;
+  if (!c.nrc) return fail(HMSNN_EINVAL, "call hmsnn_nrc_begin first");
+  const size_t bytes = sizeof(float4) * (size_t)c.scene.width * c.scene.height;
+  hipError_t e = hipMemcpyAsync(rgba, c.nrc->avg, bytes, hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  return e == hipSuccess ? HMSNN_OK : fail(HMSNN_EDEVICE, hipGetErrorString(e));
+}
+
+int hmsnn_nrc_get_sizes(hmsnn_ctx* ctx, int sizes[4]) {
+  if (!ctx || !sizes) return fail(HMSNN_EINVAL, "hmsnn_nrc_get_sizes: bad arguments");
+  const NrcState* m = ctx->c.nrc;
+  if (!m) return fail(HMSNN_EINVAL, "call hmsnn_nrc_begin first");
+  sizes[0] = m->n_tpx;
+  sizes[1] = m->every_nth;
+  sizes[2] = m->n_query;
+  sizes[3] = m->n_rec;
+  return HMSNN_OK;
+}
+
+int hmsnn_nrc_get_buffer(hmsnn_ctx* ctx, int which, void* host, size_t bytes) {
+  if (!ctx || !host) return fail(HMSNN_EINVAL, "hmsnn_nrc_get_buffer: bad arguments");
+  const NrcState* m = ctx->c.nrc;
+  if (!m) return fail(HMSNN_EINVAL, "call hmsnn_nrc_begin first");
+  const size_t npx = (size_t)m->frame_size;
+  const void* src = nullptr;
+  size_t need = 0;
+  switch (which) {
+    case HMSNN_NRC_QUERY: src = m->query, need = sizeof(float) * kNnInputCh * (size_t)m->n_query; break;
+    case HMSNN_NRC_CACHE: src = m->cache, need = sizeof(float) * 3 * (size_t)m->n_query; break;
+    case HMSNN_NRC_GB_RADIANCE: src = m->gb_rad, need = sizeof(float4) * npx; break;
+    case HMSNN_NRC_GB_BETA: src = m->gb_beta, need = sizeof(float4) * npx; break;
+    case HMSNN_NRC_GB_HIT: src = m->gb_hit, need = npx; break;
+    case HMSNN_NRC_TRAIN_INPUT: src = m->train_in, need = sizeof(float) * kNnInputCh * (size_t)m->n_rec; break;
+    case HMSNN_NRC_TRAIN_TARGET: src = m->train_tgt, need = sizeof(float) * 3 * (size_t)m->n_rec; break;
+    case HMSNN_NRC_TRAIN_IDX: src = m->train_idx, need = sizeof(int) * (size_t)m->n_tpx; break;
+    case HMSNN_NRC_TRAIN_BOUNCES: src = m->tb_bounces, need = sizeof(int) * (size_t)m->n_tpx; break;
+    case HMSNN_NRC_TRAIN_HIT: src = m->tb_hit, need = (size_t)m->n_tpx; break;
+    default: return fail(HMSNN_EINVAL, "hmsnn_nrc_get_buffer: unknown buffer");
+  }
+  if (bytes != need) return fail(HMSNN_EINVAL, "hmsnn_nrc_get_buffer: size mismatch");
+  hipError_t e = hipMemcpyAsync(host, src, bytes, hipMemcpyDeviceToHost, ctx->c.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->c.stream);
+  return e == hipSuccess ? HMSNN_OK : fail(HMSNN_EDEVICE, hipGetErrorString(e));
+}
+
+int hmsnn_nrc_get_mlp_weights(hmsnn_ctx* ctx, float* w_host, int n) {
+  if (!ctx || !w_host || n != kMlpWeightCount) return fail(HMSNN_EINVAL, "hmsnn_nrc_get_mlp_weights: bad arguments");
+  if (!ctx->c.nrc) return fail(HMSNN_EINVAL, "call hmsnn_nrc_begin first");
+  std::string err;
+  if (hipStreamSynchronize(ctx->c.stream) != hipSuccess) return fail(HMSNN_EDEVICE, "stream failed");
+  int rc = mlp_get_weights(*ctx->c.nrc->mlp, w_host, err);
+  return rc ? fail(rc, err) : HMSNN_OK;
+}
+
+int hmsnn_nrc_set_mlp_weights(hmsnn_ctx* ctx, const float* w_host, int n) {
+  if (!ctx || !w_host || n != kMlpWeightCount) return fail(HMSNN_EINVAL, "hmsnn_nrc_set_mlp_weights: bad arguments");
+  if (!ctx->c.nrc) return fail(HMSNN_EINVAL, "call hmsnn_nrc_begin first");
+  std::string err;
+  if (hipStreamSynchronize(ctx->c.stream) != hipSuccess) return fail(HMSNN_EDEVICE, "stream failed");
+  int rc = mlp_set_weights(*ctx->c.nrc->mlp, w_host, err);
+  return rc ? fail(rc, err) : HMSNN_OK;
+}
+
+int hmsnn_render_nrc(hmsnn_ctx* ctx, int spp, float* rgba, hmsnn_nrc_stats* stats) {
+  if (!ctx || spp <= 0) return fail(HMSNN_EINVAL, "hmsnn_render_nrc: bad arguments");
+  int rc = hmsnn_nrc_begin(ctx);
+  if (rc) return rc;
+  rc = hmsnn_nrc_frames(ctx, spp, stats);
+  if (rc) return rc;
+  return rgba ? hmsnn_nrc_get_image(ctx, rgba) : HMSNN_OK;
+}
+
+}  // extern "C"

@@ -43,6 +43,8 @@ struct SceneView {
   int mis;
 };
 
+constexpr int kDoneCount = 4;  // PathBuffers::counts index of the finished-path queue
+
 // Per-slot path state (structure of arrays in HBM). A slot is one pixel of
 // the current sample pass.
 struct PathBuffers {
@@ -70,7 +72,11 @@ struct PathBuffers {
   int* shadow_queue;  // slots with pending direct light
   int* shadow_rays;   // pending shadow rays: slot * 2 + (0: light sample, 1: BSDF sample)
   uint8_t* sh_vis;    // visibility per (slot, ray), same indexing as shadow_rays
-  int* counts;  // [0], [1] continuation queues, [2] shadow slots, [3] shadow rays
+  // streaming schedule: pass index and vertex depth per slot, finished slots
+  int* slot_pass;
+  int* slot_depth;
+  int* done_queue;
+  int* counts;  // [0], [1] continuation queues, [2] shadow slots, [3] shadow rays, [4] finished paths
   int* spill;   // traversal stack spill
   int spill_stride;
   int spill_levels;

@@ -62,24 +62,11 @@ __device__ __forceinline__ void flush_counters(const TraceCounters& c, unsigned 
 
 }  // namespace
 
-__global__ void k_reset_counts(int* counts, int a, int b, int c) {
-  if (threadIdx.x == 0) {
-    counts[a] = 0;
-    counts[b] = 0;
-    counts[c] = 0;
-  }
-}
-
-// Primary rays for every pixel of pass `accum_id` (path_tracing.cu:29-48).
-// HairMSNN training pixels (hair_msnn.cu:206-209) are flagged here.
-__global__ void k_camera(SceneView sc, PathBuffers pb, int accum_id, int msnn, const int* train_idx,
-                         int every_nth, int n_train) {
-  const int n = sc.width * sc.height;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) pb.counts[0] = n;
-  if (i >= n) return;
+// Camera ray of pixel i in the pass whose seed index is seed_pass
+// (path_tracing.cu:29-48): two jitter draws, then the path state reset.
+__device__ __forceinline__ void camera_path(const SceneView& sc, const PathBuffers& pb, int i, int seed_pass) {
   const int x = i % sc.width, y = i / sc.width;
-  uint32_t rng = path_seed((uint32_t)(accum_id + 10007), (uint32_t)x, (uint32_t)y, (uint32_t)sc.width);
+  uint32_t rng = path_seed((uint32_t)(seed_pass + 10007), (uint32_t)x, (uint32_t)y, (uint32_t)sc.width);
   const float jx = draw(rng), jy = draw(rng);
   const float sx = ((float)x + jx) / (float)sc.width;
   const float sy = ((float)y + jy) / (float)sc.height;
@@ -89,6 +76,28 @@ __global__ void k_camera(SceneView sc, PathBuffers pb, int accum_id, int msnn, c
   pb.beta[i] = make_float4(1.f, 1.f, 1.f, 0.f);
   pb.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
   pb.rng[i] = rng;
+}
+
+__global__ void k_reset_counts(int* counts, int a, int b, int c) {
+  if (threadIdx.x == 0) {
+    counts[a] = 0;
+    counts[b] = 0;
+    counts[c] = 0;
+    counts[kDoneCount] = 0;
+  }
+}
+
+// Primary rays for every pixel of pass `accum_id` (path_tracing.cu:29-48).
+// HairMSNN training pixels (hair_msnn.cu:206-209) are flagged here.
+// NRC (nrc != 0): training pixels by the same rule (cuda/nrc.cu:371-377),
+// every 16th of them, or all with all_unbiased, tracing an unbiased suffix.
+__global__ void k_camera(SceneView sc, PathBuffers pb, int accum_id, int msnn, const int* train_idx,
+                         int every_nth, int n_train, int nrc, int all_unbiased) {
+  const int n = sc.width * sc.height;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) pb.counts[0] = n;
+  if (i >= n) return;
+  camera_path(sc, pb, i, accum_id);
   pb.queue[0][i] = i;
   uint32_t flags = 0;
   if (msnn) {
@@ -96,6 +105,13 @@ __global__ void k_camera(SceneView sc, PathBuffers pb, int accum_id, int msnn, c
     if (tr < n_train && i % every_nth == train_idx[tr] % every_nth) flags |= 1u;
     pb.beta_short[i] = make_float4(1.f, 1.f, 1.f, 0.f);
     pb.L_short[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  if (nrc) {
+    const int tr = i / every_nth;
+    if (tr < n_train && i % every_nth == train_idx[tr] % every_nth) {
+      flags |= kNrcTrain;
+      if (tr % 16 == 0 || all_unbiased) flags |= kNrcUnbiased;
+    }
   }
   pb.flags[i] = flags;
 }
@@ -156,6 +172,168 @@ __global__ void k_camera_points(SceneView sc, PathBuffers pb, int accum_id, int 
   pb.flags[i] = 1u;
 }
 
+// NRC G_BUFFER vertex (nrcTracePaths, cuda/nrc.cu:135-343) in wavefront
+// form. The reference's loop iteration b lights vertex b, samples and traces
+// the continuation, and then decides on the new hit; here shading depth d
+// first finishes iteration d - 1 with the hit just traced (spread, cache
+// query, termination) and then runs the head of iteration d (training
+// record, direct light, continuation). No Russian roulette. The radiance of
+// a training pixel is frozen in L_short when its primary path queries the
+// cache and its suffix goes on.
+__global__ void __launch_bounds__(256) k_shade_nrc(SceneView sc, PathBuffers pb, int depth, int cur,
+                                                   NrcPassArgs nr) {
+  const int nxt = cur ^ 1;
+  const int n = pb.counts[cur];
+  const int* q = pb.queue[cur];
+  const int stride = gridDim.x * blockDim.x;
+  const int nloop = (n + stride - 1) / stride * stride;
+  const float kPiRef = 3.1415926f;  // Pi (utils.cuh:10)
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
+    bool cont = false, shadow = false, ray1 = false, ray2 = false;
+    int slot = -1;
+    if (i < n) {
+      slot = q[i];
+      const int prim = pb.hit_prim[slot];
+      const float4 o4 = pb.ray_o[slot], d4 = pb.ray_d[slot];
+      const V3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
+      uint32_t fl = pb.flags[slot];
+      const bool training = fl & kNrcTrain;
+      const int tr = slot / nr.every_nth;
+      const bool miss = prim == kPrimMiss;
+      Hit h;
+      if (!miss) setup_hit(sc, o, d, pb.hit_t[slot], prim, pb.hit_u[slot], h);
+      bool alive = true;
+      float4 ns = make_float4(0.f, 0.f, nr.c, 0.f);  // a0, spread, c, pdf of the last continuation
+      if (depth == 0) {
+        if (miss || (h.surface && h.wo_local.z < 0.f)) {
+          // G-buffer miss: the escaped ray's Le, or Le = 0 on a back face (nrc.cu:314-322)
+          V3 le = v3(0.f);
+          if (miss && sc.env.enabled) le = env_radiance(sc.env, d);
+          pb.L[slot] = make_float4(le.x, le.y, le.z, 0.f);
+          alive = false;
+        } else {
+          const float l = length(h.p - sc.cam_pos);
+          ns.x = l * l / (4.f * kPiRef) / fabsf(h.wo_local.z);  // camera footprint a0 (nrc.cu:153)
+          nr.gb_hit[slot] = 1;
+        }
+      } else {
+        ns = nr.state[slot];
+        const float4 pv = nr.prev[slot];
+        const V3 prev = v3(pv.x, pv.y, pv.z);
+        if (miss) {
+          // the path left the scene (nrc.cu:209-220)
+          if (training) {
+            nr.tb_bounces[tr] = depth - 1;
+            nr.tb_hit[tr] = 0;
+          }
+          alive = false;
+        } else {
+          const float l = length(h.p - prev);
+          if (training && (fl & kNrcRecalcA0)) {
+            ns.x = l * l / (4.f * kPiRef) / fabsf(h.wo_local.z);  // suffix footprint (nrc.cu:189-192)
+            fl &= ~kNrcRecalcA0;
+          }
+          ns.y += sqrtf(l * l / ns.w / fabsf(h.wo_local.z));  // nrcSpread (utils.cuh:86-90)
+          const bool spread_cond = ns.y * ns.y > ns.z * ns.x;
+          if (spread_cond && !(fl & kNrcSuffix)) {
+            // primary path ends in a cache query at this vertex (nrc.cu:222-271)
+            const float4 b4 = pb.beta[slot];
+            nr.gb_beta[slot] = b4;
+            float* r = nr.query + (size_t)slot * kNnInputCh;
+            const V3 p = h.p / nr.scene_scale;
+            r[0] = p.x; r[1] = p.y; r[2] = p.z;
+            r[3] = h.wo.x; r[4] = h.wo.y; r[5] = h.wo.z;
+            r[6] = h.n.x; r[7] = h.n.y; r[8] = h.n.z;
+            r[9] = 0.f; r[10] = 0.f; r[11] = 0.f;
+            if (training) {
+              fl |= kNrcSuffix | kNrcRecalcA0;
+              ns.y = 0.f;
+              pb.L_short[slot] = pb.L[slot];
+            } else {
+              alive = false;
+            }
+          } else if (spread_cond) {
+            // training suffix ends in a cache query (nrc.cu:273-309)
+            float* r = nr.query + ((size_t)nr.frame_size + tr) * kNnInputCh;
+            const V3 p = h.p / nr.scene_scale;
+            r[0] = p.x; r[1] = p.y; r[2] = p.z;
+            r[3] = h.wo.x; r[4] = h.wo.y; r[5] = h.wo.z;
+            r[6] = h.n.x; r[7] = h.n.y; r[8] = h.n.z;
+            r[9] = 0.f; r[10] = 0.f; r[11] = 0.f;
+            nr.tb_bounces[tr] = depth - 1;
+            nr.tb_hit[tr] = 1;
+            if (fl & kNrcUnbiased) ns.z = 1e30f;  // the unbiased suffix goes on to the end
+            else alive = false;
+          }
+        }
+      }
+      if (alive) {
+        uint32_t rng = pb.rng[slot];
+        const float4 b4 = pb.beta[slot];
+        V3 beta = v3(b4.x, b4.y, b4.z);
+        const int rec = tr * kNrcMaxBounces + depth;
+        if (training) {
+          const V3 p = h.p / nr.scene_scale;
+          nr.tb_pos[rec] = make_float4(p.x, p.y, p.z, 0.f);
+          nr.tb_wo[rec] = make_float4(h.wo.x, h.wo.y, h.wo.z, 0.f);
+          nr.tb_n[rec] = make_float4(h.n.x, h.n.y, h.n.z, 0.f);
+        }
+        DirectOut dl;
+        direct_lighting(sc, h, rng, dl);
+        if (dl.has1 || dl.has2) {
+          shadow = true;
+          ray1 = dl.has1 != 0;
+          ray2 = dl.has2 != 0;
+          pb.sh_o1[slot] = make_float4(dl.o1.x, dl.o1.y, dl.o1.z, dl.has1 ? 1.f : 0.f);
+          pb.sh_d1[slot] = make_float4(dl.d1.x, dl.d1.y, dl.d1.z, 0.f);
+          pb.sh_c1[slot] = make_float4(dl.c1.x, dl.c1.y, dl.c1.z, 0.f);
+          pb.sh_o2[slot] = make_float4(dl.o2.x, dl.o2.y, dl.o2.z, dl.has2 ? 1.f : 0.f);
+          pb.sh_d2[slot] = make_float4(dl.d2.x, dl.d2.y, dl.d2.z, 0.f);
+          pb.sh_c2[slot] = make_float4(dl.c2.x, dl.c2.y, dl.c2.z, 0.f);
+          pb.sh_beta[slot] = make_float4(beta.x, beta.y, beta.z, 0.f);
+        }
+        if (depth == kNrcMaxBounces - 1) {
+          // last bounce (nrc.cu:209-220)
+          if (training) {
+            nr.tb_bounces[tr] = depth;
+            nr.tb_hit[tr] = 0;
+          }
+        } else {
+          // nextPathVertexNRC (nrc.cu:16-67)
+          const BsdfSample bs = sample_bsdf(sc, h, rng);
+          V3 mul = (bs.pdf == 0.f) ? bs.f : bs.f / bs.pdf;
+          if (any_nan(mul)) mul = v3(1.f);
+          beta = beta * mul;
+          if (training) nr.tb_beta[rec] = make_float4(mul.x, mul.y, mul.z, 0.f);
+          ns.w = bs.pdf;
+          nr.prev[slot] = make_float4(h.p.x, h.p.y, h.p.z, 0.f);
+          V3 p = h.p, nd = h.n;
+          if (dot(bs.wi, h.n) < 0.f && !h.surface) {
+            nd = -1.f * h.n;
+            p = p + (2.f * h.radius) * nd;
+          }
+          const V3 no = p + 1e-3f * nd;
+          pb.ray_o[slot] = make_float4(no.x, no.y, no.z, 0.f);
+          pb.ray_d[slot] = make_float4(bs.wi.x, bs.wi.y, bs.wi.z, 0.f);
+          pb.beta[slot] = make_float4(beta.x, beta.y, beta.z, 0.f);
+          cont = true;
+        }
+        pb.rng[slot] = rng;
+        nr.state[slot] = ns;
+      }
+      pb.flags[slot] = fl;
+    }
+    const int qpos = wave_append(&pb.counts[nxt], cont);
+    if (cont) pb.queue[nxt][qpos] = slot;
+    const int spos = wave_append(&pb.counts[2], shadow);
+    if (shadow) pb.shadow_queue[spos] = slot;
+    const int r1 = wave_append(&pb.counts[3], ray1);
+    if (ray1) pb.shadow_rays[r1] = 2 * slot;
+    const int r2 = wave_append(&pb.counts[3], ray2);
+    if (ray2) pb.shadow_rays[r2] = 2 * slot + 1;
+  }
+}
+
 template <bool kCount>
 __global__ void HM_TRACE_BOUNDS k_extend(SceneView sc, PathBuffers pb, int cur,
                                                         unsigned long long* stats) {
@@ -179,6 +357,9 @@ __global__ void HM_TRACE_BOUNDS k_extend(SceneView sc, PathBuffers pb, int cur,
   if (kCount) flush_counters(cnt, &stats[0], &stats[1], &stats[4]);
 }
 
+// kStream: the depth is per slot (slot_depth), and terminated paths are
+// appended to done_queue for k_stream_finish.
+template <bool kStream>
 __global__ void __launch_bounds__(256) k_shade(SceneView sc, PathBuffers pb, ShadeParams sp,
                                                GBufferOut gb) {
   const int nxt = sp.cur ^ 1;
@@ -189,14 +370,16 @@ __global__ void __launch_bounds__(256) k_shade(SceneView sc, PathBuffers pb, Sha
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
     bool cont = false, shadow = false, ray1 = false, ray2 = false;
     int slot = -1;
+    bool done = false;
     if (i < n) {
       slot = q[i];
+      const int depth = kStream ? pb.slot_depth[slot] : sp.depth;
       const int prim = pb.hit_prim[slot];
       const float4 o4 = pb.ray_o[slot], d4 = pb.ray_d[slot];
       const V3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
       const bool training = sp.msnn && (pb.flags[slot] & 1u);
       if (prim == kPrimMiss) {
-        if (sp.depth == 0) {
+        if (depth == 0) {
           // camera ray escaped: the miss program's radiance (optix_common.cuh:604-616)
           V3 le = v3(0.f);
           if (sc.env.enabled) le = env_radiance(sc.env, d);
@@ -206,7 +389,7 @@ __global__ void __launch_bounds__(256) k_shade(SceneView sc, PathBuffers pb, Sha
       } else {
         Hit h;
         setup_hit(sc, o, d, pb.hit_t[slot], prim, pb.hit_u[slot], h);
-        if (sp.depth == 0 && gb.enabled) gbuffer_write_hit(gb, sc, slot, h, prim);
+        if (depth == 0 && gb.enabled) gbuffer_write_hit(gb, sc, slot, h, prim);
         uint32_t rng = pb.rng[slot];
         float4 b4 = pb.beta[slot];
         V3 beta = v3(b4.x, b4.y, b4.z);
@@ -216,7 +399,7 @@ __global__ void __launch_bounds__(256) k_shade(SceneView sc, PathBuffers pb, Sha
           beta_s = v3(s4.x, s4.y, s4.z);
         }
         // ---- direct lighting at this vertex ----
-        if (sp.depth == 0 ? sp.dl_zero != 0 : sp.depth >= sp.dl_from) {
+        if (depth == 0 ? sp.dl_zero != 0 : depth >= sp.dl_from) {
           DirectOut dl;
           direct_lighting(sc, h, rng, dl);
           if (dl.has1 || dl.has2) {
@@ -235,12 +418,12 @@ __global__ void __launch_bounds__(256) k_shade(SceneView sc, PathBuffers pb, Sha
         }
         // ---- Russian roulette (optix_common.cuh:398-405; hair_msnn.cu:110-128, 171-181) ----
         bool alive = true;
-        if (sp.depth >= 1) {
+        if (depth >= 1) {
           const float qq = fmaxf(0.05f, 1.f - luminance(beta));
           const float eps = draw(rng);
           if (training) {
             const float qs = fmaxf(0.05f, 1.f - luminance(beta_s));
-            if (eps < qs || sp.depth > sp.trunc_beta) beta_s = v3(0.f);
+            if (eps < qs || depth > sp.trunc_beta) beta_s = v3(0.f);
             if (eps < qq) {
               alive = false;
             } else {
@@ -248,12 +431,12 @@ __global__ void __launch_bounds__(256) k_shade(SceneView sc, PathBuffers pb, Sha
               if (!is_zero(beta_s)) beta_s = beta_s / (1.f - qs);
             }
           } else {
-            if (eps < qq || (sp.msnn && sp.depth > sp.trunc_beta)) alive = false;
+            if (eps < qq || (sp.msnn && depth > sp.trunc_beta)) alive = false;
             else beta = beta / (1.f - qq);
           }
         }
         // ---- continuation (nextPathVertex, optix_common.cuh:311-360) ----
-        if (alive && sp.depth + 1 <= sp.max_vertex) {
+        if (alive && depth + 1 <= sp.max_vertex) {
           const BsdfSample bs = sample_bsdf(sc, h, rng);
           const V3 wl = bs.wl, wi = bs.wi, f = bs.f;
           const float pdf = bs.pdf;
@@ -275,6 +458,14 @@ __global__ void __launch_bounds__(256) k_shade(SceneView sc, PathBuffers pb, Sha
         }
         pb.rng[slot] = rng;
       }
+      if (kStream) {
+        if (cont) pb.slot_depth[slot] = depth + 1;
+        done = !cont;
+      }
+    }
+    if (kStream) {
+      const int dpos = wave_append(&pb.counts[kDoneCount], done);
+      if (done) pb.done_queue[dpos] = slot;
     }
     const int qpos = wave_append(&pb.counts[nxt], cont);
     if (cont) pb.queue[nxt][qpos] = slot;
@@ -308,9 +499,9 @@ __global__ void HM_TRACE_BOUNDS k_shadow(SceneView sc, PathBuffers pb, unsigned 
   if (kCount) flush_counters(cnt, &stats[2], &stats[3], &stats[5]);
 }
 
-// Adds each path's visible direct light (directLighting's sum with its NaN
-// scrub, optix_common.cuh:279-309) times the throughput at the vertex.
-__global__ void k_shadow_resolve(PathBuffers pb, int msnn) {
+// NRC training pixels also record the vertex's direct light (vertRadiance,
+// cuda/nrc.cu:200).
+__global__ void k_shadow_resolve(PathBuffers pb, int msnn, int depth, NrcPassArgs nr) {
   const int n = pb.counts[2];
   const int stride = gridDim.x * blockDim.x;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -338,6 +529,10 @@ __global__ void k_shadow_resolve(PathBuffers pb, int msnn) {
       Ls.y += bs.y * sum.y;
       Ls.z += bs.z * sum.z;
       pb.L_short[slot] = Ls;
+    }
+    if (nr.enabled && (pb.flags[slot] & kNrcTrain)) {
+      const int rec = slot / nr.every_nth * kNrcMaxBounces + depth;
+      nr.tb_rad[rec] = make_float4(sum.x, sum.y, sum.z, 0.f);
     }
   }
 }
@@ -408,7 +603,8 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
   } else {
     const int n = sc.width * sc.height;
     hipLaunchKernelGGL(k_camera, dim3((n + 255) / 256), dim3(256), 0, stream, sc, pb, cfg.accum_id + cfg.seed_offset,
-                       cfg.msnn, cfg.train_idx, cfg.every_nth, cfg.n_slots);
+                       cfg.msnn, cfg.train_idx, cfg.every_nth, cfg.nrc.enabled ? cfg.nrc.n_tpx : cfg.n_slots,
+                       cfg.nrc.enabled, cfg.nrc.all_unbiased);
   }
   HM_TRY(check_launch("camera", err));
   const bool count = cfg.stats != nullptr;
@@ -432,7 +628,9 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
   const int sh_blocks = std::min(cfg.trace_blocks, std::max(1, (2 * n0 + kTraceBlock - 1) / kTraceBlock));
   const int shade_blocks = std::min(cfg.shade_blocks, std::max(1, (n0 + 255) / 256));
   int cur = 0;
-  for (int depth = 0; depth <= cfg.max_vertex; ++depth) {
+  // depth 0 always runs: a camera miss shows the environment even when
+  // pathTrace itself returns black (path_tracing.cu:50-57)
+  for (int depth = 0; depth <= std::max(0, cfg.max_vertex); ++depth) {
     const int nxt = cur ^ 1;
     hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, stream, pb.counts, nxt, 2, 3);
     tic(0);
@@ -452,17 +650,151 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
     sp.msnn = cfg.msnn;
     sp.cur = cur;
     tic(1);
-    hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(256), 0, stream, sc, pb, sp, cfg.gb);
+    if (cfg.nrc.enabled)
+      hipLaunchKernelGGL(k_shade_nrc, dim3(shade_blocks), dim3(256), 0, stream, sc, pb, depth, cur, cfg.nrc);
+    else
+      hipLaunchKernelGGL(k_shade<false>, dim3(shade_blocks), dim3(256), 0, stream, sc, pb, sp, cfg.gb);
     toc();
     tic(2);
     if (count)
       hipLaunchKernelGGL(k_shadow<true>, dim3(sh_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cfg.stats);
     else
       hipLaunchKernelGGL(k_shadow<false>, dim3(sh_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cfg.stats);
-    hipLaunchKernelGGL(k_shadow_resolve, dim3(shade_blocks), dim3(256), 0, stream, pb, cfg.msnn);
+    hipLaunchKernelGGL(k_shadow_resolve, dim3(shade_blocks), dim3(256), 0, stream, pb, cfg.msnn, depth, cfg.nrc);
     toc();
     HM_TRY(check_launch("wavefront vertex", err));
     cur = nxt;
+  }
+  return kOk;
+}
+
+// ---- streaming schedule (path regeneration) ----------------------------------
+// Every pixel keeps one path in flight; when it ends, k_stream_finish adds
+// its radiance to the pixel's running sum (pass order per pixel, so the image
+// equals the one consecutive passes make) and starts the pixel's next pass in
+// the same slot. Each wavefront launch then carries about one ray per pixel
+// instead of a pass's shrinking tail.
+
+__global__ void k_stream_camera(SceneView sc, PathBuffers pb, int first_pass, int seed_offset) {
+  const int n = sc.width * sc.height;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) pb.counts[0] = n;
+  if (i >= n) return;
+  camera_path(sc, pb, i, first_pass + seed_offset);
+  pb.flags[i] = 0u;
+  pb.slot_pass[i] = first_pass;
+  pb.slot_depth[i] = 0;
+  pb.queue[0][i] = i;
+}
+
+// Accumulates each finished path (writePixel, utils.cuh:13-35) and queues the
+// pixel's next pass on queue nxt while passes remain (pass < end_pass).
+__global__ void k_stream_finish(SceneView sc, PathBuffers pb, int nxt, float4* accum, float4* average,
+                                int end_pass, int seed_offset) {
+  const int n = pb.counts[kDoneCount];
+  const int stride = gridDim.x * blockDim.x;
+  const int nloop = (n + stride - 1) / stride * stride;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
+    bool again = false;
+    int slot = -1;
+    if (i < n) {
+      slot = pb.done_queue[i];
+      const int pass = pb.slot_pass[slot];
+      const float4 l = pb.L[slot];
+      V3 c = v3(l.x, l.y, l.z);
+      if (any_nan(c)) c = v3(0.f);
+      if (pass > 0) {
+        const float4 a = accum[slot];
+        c = c + v3(a.x, a.y, a.z);
+      }
+      accum[slot] = make_float4(c.x, c.y, c.z, 1.f);
+      const float inv = 1.f / (float)(pass + 1);
+      average[slot] = make_float4(c.x * inv, c.y * inv, c.z * inv, 1.f);
+      if (pass + 1 < end_pass) {
+        camera_path(sc, pb, slot, pass + 1 + seed_offset);
+        pb.slot_pass[slot] = pass + 1;
+        pb.slot_depth[slot] = 0;
+        again = true;
+      }
+    }
+    const int q = wave_append(&pb.counts[nxt], again);
+    if (again) pb.queue[nxt][q] = slot;
+  }
+}
+
+int pt_run_stream(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, int passes, float4* accum,
+                  float4* average, int* live_host, hipEvent_t* live_ev, hipStream_t stream, std::string& err) {
+  if (passes <= 0) return kOk;
+  const int n = sc.width * sc.height;
+  hipLaunchKernelGGL(k_stream_camera, dim3((n + 255) / 256), dim3(256), 0, stream, sc, pb, cfg.accum_id,
+                     cfg.seed_offset);
+  HM_TRY(check_launch("stream camera", err));
+  PassTimer* tm = cfg.timer;
+  auto tic = [&](int kind) {
+    if (tm && tm->used < tm->capacity) {
+      tm->kinds[tm->used] = kind;
+      (void)hipEventRecord(tm->events[2 * tm->used], stream);
+    }
+  };
+  auto toc = [&]() {
+    if (tm && tm->used < tm->capacity) {
+      (void)hipEventRecord(tm->events[2 * tm->used + 1], stream);
+      tm->used++;
+    }
+  };
+  const int ext_blocks = std::min(cfg.trace_blocks, std::max(1, (n + kTraceBlock - 1) / kTraceBlock));
+  const int sh_blocks = std::min(cfg.trace_blocks, std::max(1, (2 * n + kTraceBlock - 1) / kTraceBlock));
+  const int shade_blocks = std::min(cfg.shade_blocks, std::max(1, (n + 255) / 256));
+  const int end_pass = cfg.accum_id + passes;
+  // Iterations are issued in even-sized batches (the live queue is back at
+  // index 0 after each); the live count after batch k is read while batch
+  // k + 1 runs, so the loop ends one batch of empty launches late.
+  constexpr int kBatch = 8;
+  const long long max_iters = (long long)passes * std::max(1, cfg.max_vertex + 1) + 4 * kBatch;
+  int cur = 0;
+  long long it = 0;
+  for (int batch = 0;; ++batch) {
+    for (int k = 0; k < kBatch; ++k, ++it) {
+      const int nxt = cur ^ 1;
+      hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, stream, pb.counts, nxt, 2, 3);
+      tic(0);
+      hipLaunchKernelGGL(k_extend<false>, dim3(ext_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cur,
+                         (unsigned long long*)nullptr);
+      toc();
+      ShadeParams sp;
+      sp.depth = 0;
+      sp.dl_zero = cfg.dl_zero;
+      sp.dl_from = cfg.dl_from;
+      sp.max_vertex = cfg.max_vertex;
+      sp.trunc_beta = cfg.trunc_beta;
+      sp.msnn = 0;
+      sp.cur = cur;
+      tic(1);
+      hipLaunchKernelGGL(k_shade<true>, dim3(shade_blocks), dim3(256), 0, stream, sc, pb, sp, GBufferOut{});
+      toc();
+      tic(2);
+      hipLaunchKernelGGL(k_shadow<false>, dim3(sh_blocks), dim3(kTraceBlock), 0, stream, sc, pb,
+                         (unsigned long long*)nullptr);
+      hipLaunchKernelGGL(k_shadow_resolve, dim3(shade_blocks), dim3(256), 0, stream, pb, 0, 0, NrcPassArgs{});
+      toc();
+      tic(3);
+      hipLaunchKernelGGL(k_stream_finish, dim3(shade_blocks), dim3(256), 0, stream, sc, pb, nxt, accum, average,
+                         end_pass, cfg.seed_offset);
+      toc();
+      cur = nxt;
+    }
+    HM_TRY(check_launch("stream iteration", err));
+    const int s = batch & 1;
+    HM_HIP(hipMemcpyAsync(&live_host[s], pb.counts + cur, sizeof(int), hipMemcpyDeviceToHost, stream), err);
+    HM_HIP(hipEventRecord(live_ev[s], stream), err);
+    if (batch > 0) {
+      HM_HIP(hipEventSynchronize(live_ev[s ^ 1]), err);
+      if (live_host[s ^ 1] == 0) break;
+    }
+    if (it > max_iters) {
+      err = "streaming schedule did not drain";
+      return kEDevice;
+    }
   }
   return kOk;
 }

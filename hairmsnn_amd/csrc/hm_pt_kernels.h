@@ -48,6 +48,36 @@ HM_D void gbuffer_write_miss(const GBufferOut& gb, int slot, V3 dir) {
 }
 #endif
 
+// Neural radiance caching G_BUFFER pass (cuda/nrc.cu:135-398): paths end
+// where their spread reaches the camera footprint (NRC sect. 3.4) and query
+// the cache there; training pixels keep tracing a suffix whose vertices
+// become training records. Slots are pixels.
+constexpr int kNrcMaxBounces = 40;  // MAX_BOUNCES (nrc.cuh:13)
+constexpr uint32_t kNrcTrain = 1u, kNrcUnbiased = 2u, kNrcSuffix = 4u, kNrcRecalcA0 = 8u;
+
+struct NrcPassArgs {
+  int enabled;
+  int all_unbiased;
+  float c;             // spread threshold multiplier
+  float scene_scale;
+  int frame_size;      // pixels; suffix cache queries follow at frame_size + training index
+  int n_tpx;           // training pixels
+  int every_nth;       // training pixel of pixel p: index p / every_nth
+  float* query;        // cache-query records, 12 floats each
+  float4* gb_beta;     // per pixel: throughput the cache value is multiplied by (0: no query)
+  uint8_t* gb_hit;     // per pixel: camera ray hit a front-facing surface or hair
+  float4* state;       // per pixel: a0, spread, c, pdf of the last sampled direction
+  float4* prev;        // per pixel: previous vertex
+  // per training pixel x kNrcMaxBounces vertices
+  float4* tb_pos;      // vertex / scene scale
+  float4* tb_wo;
+  float4* tb_n;
+  float4* tb_rad;      // direct light at the vertex
+  float4* tb_beta;     // f * cos / pdf of the sampled continuation
+  int* tb_bounces;
+  uint8_t* tb_hit;     // the suffix ended in a cache query
+};
+
 // Optional per-kernel timing: event pairs recorded around each launch.
 struct PassTimer {
   hipEvent_t* events = nullptr;  // 2 per record
@@ -85,12 +115,20 @@ struct PassConfig {
   // camera_mode 2: the training pixels of frames accum_id.. (chunk records
   // each, train_idx per record); their pixel indices are written to pix
   int* pix = nullptr;
+  NrcPassArgs nrc{};
 };
 
 // Runs one full sample pass (camera -> all vertices). Leaves per-pixel
 // radiance in pb.L (and pb.L_short for training pixels).
 int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hipStream_t stream,
                 std::string& err);
+
+// Runs `passes` passes from cfg.accum_id with the streaming schedule: a
+// pixel's next pass starts in its slot as soon as its path ends, each finished
+// path accumulated into accum / average in pass order. live_host (pinned, 2
+// ints) and live_ev (2 events) poll the live-path count.
+int pt_run_stream(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, int passes, float4* accum,
+                  float4* average, int* live_host, hipEvent_t* live_ev, hipStream_t stream, std::string& err);
 
 // Adds pb.L into the running sum and mean buffers (accumulation id `accum_id`).
 int pt_accumulate(const float4* L, float4* accum, float4* average, int n, int accum_id, hipStream_t stream,

@@ -273,6 +273,13 @@ bool parse_json(const Json& cfg, Scene& scene, std::string& err) {
     if (m.has("precision")) scene.msnn_precision = m.at("precision").str();
     if (m.has("seed")) scene.msnn_seed = (uint64_t)m.at("seed").num();
   }
+  // ---- neural radiance caching extension ----
+  if (cfg.has("nrc") && cfg.at("nrc").is_object()) {
+    const Json& m = cfg.at("nrc");
+    if (m.has("c")) scene.nrc_c = m.at("c").f();
+    if (m.has("all_unbiased")) scene.nrc_all_unbiased = m.at("all_unbiased").b();
+    if (m.has("train_records")) scene.nrc_train_records = m.at("train_records").i();
+  }
   return true;
 }
 

@@ -104,6 +104,11 @@ struct Scene {
   std::string msnn_loss = "RelativeL2Luminance";
   std::string msnn_precision = "fp16";
   uint64_t msnn_seed = 7;
+
+  // neural radiance caching extension block (render_nrc.h:112-134 defaults)
+  float nrc_c = 0.01f;              // spread threshold multiplier
+  bool nrc_all_unbiased = false;
+  int nrc_train_records = 65536;
 };
 
 // Parses a scene file. Returns false and fills `err` like the reference's

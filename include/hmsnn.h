@@ -159,6 +159,14 @@ int hmsnn_set_seed_offset(hmsnn_ctx* ctx, int offset);
  * hmsnn_pt_time_passes are then those of kernels running alone). No
  * reference counterpart: the reference renders one launch per frame. */
 int hmsnn_set_pass_concurrency(hmsnn_ctx* ctx, int passes_in_flight);
+/* Schedule of hmsnn_pt_passes. HMSNN_PT_STREAM (default): every pixel keeps
+ * one path in flight and starts its next pass as soon as the last one ends,
+ * so each wavefront launch carries about one ray per pixel. HMSNN_PT_PASSES:
+ * one wavefront per pass, passes in flight per hmsnn_set_pass_concurrency.
+ * Both accumulate each pixel's passes in pass order: the images are equal.
+ * No reference counterpart (the reference launches one megakernel per pass). */
+typedef enum hmsnn_pt_schedule { HMSNN_PT_STREAM = 0, HMSNN_PT_PASSES = 1 } hmsnn_pt_schedule;
+int hmsnn_set_pt_schedule(hmsnn_ctx* ctx, int schedule);
 
 /* Clears the accumulation (accumId = 0), like cameraChanged does. */
 int hmsnn_pt_reset(hmsnn_ctx* ctx);
@@ -212,6 +220,56 @@ int hmsnn_msnn_frames(hmsnn_ctx* ctx, int frames, hmsnn_msnn_stats* stats);
 int hmsnn_msnn_get_image(hmsnn_ctx* ctx, int which, float* rgba_host);
 /* Reset, warm up, render spp frames, copy the final mean. */
 int hmsnn_render_hair_msnn(hmsnn_ctx* ctx, int BETA, int spp, float* rgba_host, hmsnn_msnn_stats* stats);
+
+/* ---- neural radiance caching (render_nrc: render_nrc.cu:116-701,
+ *      cuda/nrc.cu:18-398) -------------------------------------------------- */
+
+typedef struct hmsnn_nrc_stats {
+  double gbuffer_ms, infer_ms, render_ms, train_ms; /* summed over the call's frames (GPU events) */
+  double total_ms;                                   /* host wall clock of the call */
+  int frames, train_steps;
+  int cache_queries;      /* last frame: pixels whose path ended in a cache query */
+  int train_records_used; /* last frame: training records holding a path vertex */
+} hmsnn_nrc_stats;
+
+/* Creates the cache network and frame buffers (RenderWindowNRC::initialize,
+ * render_nrc.cu:116-180): training records from the scene's "nrc" block
+ * (default 65,536), training pixels = records / 40, one in every_nth pixels
+ * (integer divisions as the reference). */
+int hmsnn_nrc_begin(hmsnn_ctx* ctx);
+/* Spread threshold multiplier c (setBounceThresholdControl,
+ * render_nrc.cu:42-46) and unbiased suffixes for every training pixel
+ * (setAllUnbiasedPaths, render_nrc.cu:13-17). */
+int hmsnn_nrc_set_params(hmsnn_ctx* ctx, float c, int all_unbiased);
+/* Renders `frames` frames (RenderWindowNRC::render, render_nrc.cu:640-701):
+ * shuffle, G_BUFFER, inference, RENDER (composite + training data), one
+ * training step; accumulating. */
+int hmsnn_nrc_frames(hmsnn_ctx* ctx, int frames, hmsnn_nrc_stats* stats);
+/* Mean image (w x h x 4). */
+int hmsnn_nrc_get_image(hmsnn_ctx* ctx, float* rgba_host);
+/* sizes[0] training pixels, [1] every_nth, [2] cache queries per frame
+ * (nnFrameSize), [3] training records. */
+int hmsnn_nrc_get_sizes(hmsnn_ctx* ctx, int sizes[4]);
+/* The last frame's buffers, for inspection and tests; `bytes` must equal the
+ * buffer's size. */
+typedef enum hmsnn_nrc_buffer {
+  HMSNN_NRC_QUERY = 0,          /* float[n_query][12] cache-query records */
+  HMSNN_NRC_CACHE = 1,          /* float[n_query][3] network outputs */
+  HMSNN_NRC_GB_RADIANCE = 2,    /* float4[w*h] radiance before the query */
+  HMSNN_NRC_GB_BETA = 3,        /* float4[w*h] throughput at the query (0: none) */
+  HMSNN_NRC_GB_HIT = 4,         /* uint8[w*h] camera ray hit */
+  HMSNN_NRC_TRAIN_INPUT = 5,    /* float[n_rec][12] */
+  HMSNN_NRC_TRAIN_TARGET = 6,   /* float[n_rec][3] */
+  HMSNN_NRC_TRAIN_IDX = 7,      /* int[n_tpx] shuffled training indices */
+  HMSNN_NRC_TRAIN_BOUNCES = 8,  /* int[n_tpx] recorded vertices per training pixel */
+  HMSNN_NRC_TRAIN_HIT = 9       /* uint8[n_tpx] suffix ended in a cache query */
+} hmsnn_nrc_buffer;
+int hmsnn_nrc_get_buffer(hmsnn_ctx* ctx, int which, void* host, size_t bytes);
+/* The cache network's weights (layout of hmsnn_mlp_get_weights). */
+int hmsnn_nrc_get_mlp_weights(hmsnn_ctx* ctx, float* w_host, int n);
+int hmsnn_nrc_set_mlp_weights(hmsnn_ctx* ctx, const float* w_host, int n);
+/* Begin, render spp frames, copy the mean image. */
+int hmsnn_render_nrc(hmsnn_ctx* ctx, int spp, float* rgba_host, hmsnn_nrc_stats* stats);
 
 /* ---- radiance-cache MLP (replaces TINY_MLP, neural_network.cuh:33-50) -- */
 

@@ -266,6 +266,21 @@ int orc_render(const orc_scene* sc, void* bvh_handle, int mode, int kbeta, int a
   return 0;
 }
 
+// NRC G_BUFFER pass of frame accum_id over the whole image (orc_scene.h
+// nrc_gbuffer); out holds width * height NrcPixelOut records.
+int orc_nrc_gbuffer(const orc_scene* sc, void* bvh_handle, int accum_id, const uint8_t* flags, float c,
+                    float scene_scale, int threads, int prec, NrcPixelOut* out) {
+  if (!sc || !bvh_handle || !flags || !out) return 1;
+  EnvTables env;
+  const SceneDesc s = scene_desc(sc, env);
+  const Bvh& bvh = *(const Bvh*)bvh_handle;
+  if (prec == 0) nrc_gbuffer<double>(s, bvh, accum_id, flags, c, scene_scale, threads, out);
+  else nrc_gbuffer<float>(s, bvh, accum_id, flags, c, scene_scale, threads, out);
+  return 0;
+}
+
+int orc_nrc_record_size(void) { return (int)sizeof(NrcPixelOut); }
+
 // FP64 path-replay adjoint of the full image (orc_scene.h adjoint_render).
 int orc_adjoint(const orc_scene* sc, void* bvh_handle, int spp, const float* dl_dimg, int num_strands, int threads,
                 double fd_h, double* grad_sigma, double* grad_glob, double* grad_px) {

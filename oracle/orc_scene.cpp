@@ -772,6 +772,144 @@ Vec<R> msnn_training(const Ctx<R>& c, Inter<R> si, Lcg& rng, int v2, int kbeta, 
   return col;
 }
 
+// nextPathVertexNRC (cuda/nrc.cu:16-67): nextPathVertex that also returns
+// the sampled direction's pdf.
+template <class R>
+Vec<R> nrc_next_vertex(const Ctx<R>& c, Inter<R>& si, Lcg& rng, R* pdf_out) {
+  R pdf = 1;
+  const Vec<R> f = bsdf_sample(c, si, rng, &pdf);
+  Vec<R> mul = pdf == 0 ? f : f / pdf;
+  if (has_nan(mul)) mul = mk((R)1, (R)1, (R)1);
+  const Vec<R> wi = si.wi;
+  Vec<R> nd = si.n;
+  if (dot(wi, si.n) < 0 && !si.surface) {
+    nd = (R)-1 * si.n;
+    si.p = si.p + ((R)2 * si.radius) * nd;
+  }
+  const Vec<R> o = si.p + (R)1e-3f * nd;
+  Inter<R> nx;
+  nx.wi = wi;
+  trace(c, o, wi, nx);
+  si = nx;
+  *pdf_out = pdf;
+  return mul;
+}
+
+template <class R>
+inline R len2(Vec<R> v) {
+  const R l = std::sqrt(dot(v, v));
+  return l * l;
+}
+
+// nrcTracePaths (cuda/nrc.cu:135-343), one pixel sample from its primary hit.
+template <class R>
+void nrc_trace(const Ctx<R>& c, Inter<R> si, Lcg& rng, Vec<R> cam, bool training, bool unbiased, R cthr,
+               R scale, NrcPixelOut& o) {
+  bool suffix = false, recalc = false;
+  R a0 = len2(si.p - cam) / ((R)4 * Pi<R>()) / std::fabs(si.wo_local.z);
+  Vec<R> prev = mk((R)0, (R)0, (R)0);
+  R prev_pdf = 0, spread = 0;
+  Vec<R> col = mk((R)0, (R)0, (R)0), beta = mk((R)1, (R)1, (R)1);
+  auto put = [](double* d, Vec<R> v) { d[0] = (double)v.x; d[1] = (double)v.y; d[2] = (double)v.z; };
+  auto putf = [](float* d, Vec<R> v) { d[0] = (float)v.x; d[1] = (float)v.y; d[2] = (float)v.z; };
+  for (int b = 0; b < kNrcMaxBounces; ++b) {
+    const Vec<R> dl = direct_lighting(c, si, rng);
+    col = col + beta * dl;
+    const Vec<R> pwo = si.wo, pn = si.n;
+    prev = si.p;
+    const Vec<R> mul = nrc_next_vertex(c, si, rng, &prev_pdf);
+    beta = beta * mul;
+    if (training) {
+      if (recalc) {
+        a0 = len2(si.p - prev) / ((R)4 * Pi<R>()) / std::fabs(si.wo_local.z);
+        recalc = false;
+      }
+      double* r = o.rec + (size_t)b * 15;
+      put(r, prev / scale);
+      put(r + 3, pwo);
+      put(r + 6, pn);
+      put(r + 9, dl);
+      put(r + 12, mul);
+    }
+    spread = spread + std::sqrt(len2(si.p - prev) / prev_pdf / std::fabs(si.wo_local.z));
+    const bool cond = spread * spread > cthr * a0;
+    if (!si.hit || b == kNrcMaxBounces - 1) {
+      if (!suffix) {
+        put(o.rad, col);
+        put(o.beta, mk((R)0, (R)0, (R)0));
+      }
+      o.bounces = b;
+      o.tb_hit = 0;
+      break;
+    } else if (cond && !suffix) {
+      put(o.rad, col);
+      put(o.beta, beta);
+      o.query = 1;
+      putf(o.q, si.p / scale);
+      putf(o.q + 3, si.wo);
+      putf(o.q + 6, si.n);
+      if (training) {
+        suffix = true;
+        recalc = true;
+        spread = 0;
+      } else {
+        break;
+      }
+    } else if (cond && suffix) {
+      putf(o.sq, si.p / scale);
+      putf(o.sq + 3, si.wo);
+      putf(o.sq + 6, si.n);
+      o.bounces = b;
+      o.tb_hit = 1;
+      if (!unbiased) break;
+      cthr = (R)1e30f;
+    }
+  }
+}
+
+template <class R>
+void nrc_gbuffer(const SceneDesc& s, const Bvh& bvh, int accum_id, const uint8_t* flags, float cthr, float scale,
+                 int threads, NrcPixelOut* out) {
+  Ctx<R> c{s, bvh, {}, {}};
+  c.hp = HairParams<R>{(R)s.beta_m, (R)s.beta_n, (R)s.alpha, (R)s.gains[0], (R)s.gains[1], (R)s.gains[2], (R)s.gains[3]};
+  c.hs = setup_hair(c.hp);
+  c.hp0 = c.hp;
+  c.hs0 = c.hs;
+  const Vec<R> cp = mk((R)s.cam[0], (R)s.cam[1], (R)s.cam[2]);
+  const Vec<R> d00 = mk((R)s.cam[3], (R)s.cam[4], (R)s.cam[5]);
+  const Vec<R> du = mk((R)s.cam[6], (R)s.cam[7], (R)s.cam[8]);
+  const Vec<R> dv = mk((R)s.cam[9], (R)s.cam[10], (R)s.cam[11]);
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (int py = 0; py < s.height; ++py) {
+    for (int px = 0; px < s.width; ++px) {
+      const size_t i = (size_t)py * s.width + px;
+      NrcPixelOut& o = out[i];
+      Lcg rng{get_rng((uint32_t)(accum_id + 10007), (uint32_t)px, (uint32_t)py, (uint32_t)s.width)};
+      const float jx = rng.next(), jy = rng.next();
+      const float sx = ((float)px + jx) / (float)s.width, sy = ((float)py + jy) / (float)s.height;
+      const Vec<R> d = nrm(d00 + (R)sx * du + (R)sy * dv);
+      Inter<R> si;
+      si.wi = d;
+      trace(c, cp, d, si);
+      if (!si.hit || (si.surface && si.wo_local.z < 0)) {
+        const Vec<R> le = si.hit ? mk((R)0, (R)0, (R)0) : si.Le;
+        o.rad[0] = (double)le.x;
+        o.rad[1] = (double)le.y;
+        o.rad[2] = (double)le.z;
+        continue;
+      }
+      o.gb_hit = 1;
+      nrc_trace(c, si, rng, cp, (flags[i] & 1) != 0, (flags[i] & 2) != 0, (R)cthr, (R)scale, o);
+    }
+  }
+}
+
+template void nrc_gbuffer<float>(const SceneDesc&, const Bvh&, int, const uint8_t*, float, float, int, NrcPixelOut*);
+template void nrc_gbuffer<double>(const SceneDesc&, const Bvh&, int, const uint8_t*, float, float, int, NrcPixelOut*);
+
 template <class R>
 void render_pixels(const SceneDesc& s, const Bvh& bvh, int mode, int kbeta, int accum_id0, int spp, int x0, int y0,
                    int w, int h, int threads, double* out_full, double* out_short, uint8_t* out_hit) {

@@ -93,6 +93,22 @@ template <class R>
 void render_pixels(const SceneDesc& s, const Bvh& bvh, int mode, int kbeta, int accum_id0, int spp, int x0,
                    int y0, int w, int h, int threads, double* out_full, double* out_short, uint8_t* out_hit);
 
+// Neural radiance caching G_BUFFER pass (cuda/nrc.cu:135-343, 357-370) for
+// one frame: per pixel, the radiance before the cache query, the throughput
+// at it and the query record; for training pixels (flags bit 0; bit 1:
+// unbiased suffix) the per-vertex records and the suffix query.
+constexpr int kNrcMaxBounces = 40;
+struct NrcPixelOut {
+  uint8_t gb_hit = 0, query = 0, tb_hit = 0;
+  int bounces = 0;
+  double rad[3] = {0, 0, 0}, beta[3] = {0, 0, 0};
+  float q[9] = {0}, sq[9] = {0};
+  double rec[kNrcMaxBounces * 15] = {0};  // per vertex: pos / scale, wo, n, direct light, f cos / pdf
+};
+template <class R>
+void nrc_gbuffer(const SceneDesc& s, const Bvh& bvh, int accum_id, const uint8_t* flags, float c, float scene_scale,
+                 int threads, NrcPixelOut* out);
+
 // Path-replay adjoint of pathTrace in FP64 (BSDF derivatives by central
 // differences of step fd_h): per-strand d/d sigma_a, d/d (beta_m, beta_n,
 // alpha) and each pixel's share of the latter, for dL/d(image) dl_dimg

@@ -48,7 +48,19 @@ _SIGS = {
     "orc_render": (_I, [C.POINTER(OrcScene), _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P]),
     "orc_max_threads": (_I, []),
     "orc_adjoint": (_I, [C.POINTER(OrcScene), _P, _I, _P, _I, _I, _D, _P, _P, _P]),
+    "orc_nrc_gbuffer": (_I, [C.POINTER(OrcScene), _P, _I, _P, _F, _F, _I, _I, _P]),
+    "orc_nrc_record_size": (_I, []),
 }
+
+# orc_scene.h NrcPixelOut
+NRC_MAX_BOUNCES = 40
+NRC_PIXEL = np.dtype({
+    "names": ["gb_hit", "query", "tb_hit", "bounces", "rad", "beta", "q", "sq", "rec"],
+    "formats": ["u1", "u1", "u1", "<i4", ("<f8", 3), ("<f8", 3), ("<f4", 9), ("<f4", 9),
+                ("<f8", (NRC_MAX_BOUNCES, 15))],
+    "offsets": [0, 1, 2, 4, 8, 32, 56, 92, 128],
+    "itemsize": 128 + 8 * NRC_MAX_BOUNCES * 15,
+})
 
 _lib = None
 
@@ -175,6 +187,19 @@ class OracleScene:
                                  ptr(full), ptr(short), ptr(hit))
         assert rc == 0
         return full, short, hit
+
+    def nrc_gbuffer(self, accum_id: int, flags: np.ndarray, c: float, scene_scale: float, prec: int = 1,
+                    threads: int = 0) -> np.ndarray:
+        """NRC G_BUFFER pass of one frame (cuda/nrc.cu:135-370): one NRC_PIXEL
+        record per pixel; flags per pixel: bit 0 training, bit 1 unbiased."""
+        assert self.lib.orc_nrc_record_size() == NRC_PIXEL.itemsize
+        W, H = int(self.s.width), int(self.s.height)
+        fl = np.ascontiguousarray(flags, np.uint8).reshape(H * W)
+        out = np.zeros(H * W, NRC_PIXEL)
+        rc = self.lib.orc_nrc_gbuffer(C.byref(self.s), self.bvh, int(accum_id), ptr(fl), float(c), float(scene_scale),
+                                      threads, prec, out.ctypes.data_as(C.c_void_p))
+        assert rc == 0
+        return out
 
     def adjoint(self, dl_dimg: np.ndarray, spp: int, num_strands: int, fd_h: float = 1e-6, threads: int = 0):
         """FP64 path-replay adjoint of the mean of passes 0..spp-1: per-strand

@@ -132,3 +132,24 @@ def test_converges_to_fp64_oracle(gpu):
     gm, om = g.mean(), o.mean()
     se_m = np.sqrt(g.mean((1, 2, 3)).var(ddof=1) / n + o.mean((1, 2, 3)).var(ddof=1) / n)
     assert abs(gm - om) < 4 * se_m
+
+
+@pytest.mark.parametrize("name", ["c0_hairball.json", "c1_curly.json"])
+def test_streaming_schedule_equals_pass_schedule(gpu, name):
+    """The streaming schedule (a pixel's next pass starts in its slot when its
+    path ends) accumulates each pixel's passes in pass order, so its image is
+    the per-pass schedule's bit for bit, across calls and seed offsets."""
+    cfg = scene_cfg(name, 64, 64)
+    imgs, lasts = {}, {}
+    for sched in ("passes", "stream"):
+        with gpu.Scene(json_text=cfg) as sc:
+            sc.set_pt_schedule(sched)
+            sc.set_seed_offset(3)
+            sc.pt_reset()
+            sc.pt_passes(5)
+            sc.pt_passes(3)
+            imgs[sched] = sc.image(0).copy()
+            lasts[sched] = sc.last_pass().copy()
+    np.testing.assert_array_equal(imgs["stream"], imgs["passes"])
+    np.testing.assert_array_equal(lasts["stream"], lasts["passes"])
+    assert np.isfinite(imgs["stream"]).all() and imgs["stream"][..., :3].mean() > 0
