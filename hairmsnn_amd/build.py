@@ -70,7 +70,7 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB.with_suffix(".so.tmp")
-    _run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs)])
+    _run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs), "-lz"])
     os.replace(tmp, LIB)
     BIN.mkdir(exist_ok=True)
     clis = sorted(CSRC.glob("cli_*.cpp"))

@@ -248,16 +248,6 @@ int Context::init(std::string& err) {
           "': OBJ meshes are not supported; use surface.sphere {center, radius}";
     return kEUnsupported;
   }
-  if (scene.has_hair && !scene.hair_procedural) {
-    err = "hair.geometry '" + scene.hair_geometry +
-          "': .hair files are not supported; use hair.procedural {...}";
-    return kEUnsupported;
-  }
-  if (scene.has_env && !scene.env_constant && !scene.env_procedural) {
-    err = "lights.environment.exr '" + scene.env_exr +
-          "': EXR maps are not supported; use environment.constant or environment.procedural";
-    return kEUnsupported;
-  }
   if ((int)scene.dlight_from.size() > kMaxDirLights) {
     err = "at most " + std::to_string(kMaxDirLights) + " directional lights are supported";
     return kEUnsupported;
@@ -277,7 +267,8 @@ int Context::init(std::string& err) {
     sscale = length(bmax - bmin);
   }
   if (scene.has_hair) {
-    generate_hair(scene.hair_gen, hair);
+    if (scene.hair_procedural) generate_hair(scene.hair_gen, hair);
+    else HM_TRY(load_hair_file(scene_path(scene, scene.hair_geometry), scene.hair_tessellation, hair, err));
     bmin = vmin(bmin, hair.bmin);
     bmax = vmax(bmax, hair.bmax);
     sscale = std::max(sscale, length(hair.bmax - hair.bmin));  // render_path_tracing.cu:212-214
@@ -290,7 +281,12 @@ int Context::init(std::string& err) {
 
   // ---- environment ----
   if (scene.has_env) {
-    generate_env(scene, env_host);
+    if (scene.env_constant || scene.env_procedural) {
+      generate_env(scene, env_host);
+    } else {
+      // loadEnvTexture (model.cpp:158-180)
+      HM_TRY(read_exr(scene_path(scene, scene.env_exr), env_host.width, env_host.height, env_host.rgba, err));
+    }
     env.width = env_host.width;
     env.height = env_host.height;
     const size_t bytes = sizeof(float) * env_host.rgba.size();
@@ -511,6 +507,10 @@ int hmsnn_generate_host(const char* text, float* seg_host, size_t seg_cap, int* 
   if (!parse_scene_text(text, ".", s, err)) return fail(HMSNN_EINVAL, err);
   HairGeometry hg;
   if (s.has_hair && s.hair_procedural) generate_hair(s.hair_gen, hg);
+  else if (s.has_hair) {
+    const int rc = load_hair_file(scene_path(s, s.hair_geometry), s.hair_tessellation, hg, err);
+    if (rc) return fail(rc, err);
+  }
   const size_t nseg = hg.seg.size() / 8;
   if (num_segments) *num_segments = (int)nseg;
   if (seg_host) {
@@ -519,6 +519,10 @@ int hmsnn_generate_host(const char* text, float* seg_host, size_t seg_cap, int* 
   }
   EnvImage env;
   if (s.has_env && (s.env_constant || s.env_procedural)) generate_env(s, env);
+  else if (s.has_env && !s.env_exr.empty()) {
+    const int rc = read_exr(scene_path(s, s.env_exr), env.width, env.height, env.rgba, err);
+    if (rc) return fail(rc, err);
+  }
   if (env_w) *env_w = env.width;
   if (env_h) *env_h = env.height;
   if (env_rgba) {
@@ -613,24 +617,40 @@ static int run_pt_passes(Context& c, int passes, std::string& err, PassTimer* ti
   SceneView v = c.view;
   v.bvh.nodes = c.bvh.nodes_w;
   const int n = c.scene.width * c.scene.height;
+  HM_TRY(c.ensure_flights(err));
+  const int K = std::max(1, std::min(c.num_flights, Context::kMaxFlights));
   if (c.pt_schedule == 0) {
     if (passes <= 0) return kOk;
     if (!c.live_host) {
-      HM_HIP(hipHostMalloc((void**)&c.live_host, 2 * sizeof(int)), err);
+      HM_HIP(hipHostMalloc((void**)&c.live_host, 2 * Context::kMaxFlights * sizeof(int)), err);
       for (hipEvent_t& e : c.live_ev) HM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming), err);
     }
+    // every group's stream starts after the work queued so far (including
+    // the other schedule's last accumulation)
     if (c.acc_pending) HM_HIP(hipStreamWaitEvent(c.stream, c.acc_event, 0), err);
     c.acc_pending = false;
+    HM_HIP(hipEventRecord(c.acc_event, c.stream), err);
+    PathBuffers* pbs[Context::kMaxFlights];
+    hipStream_t sts[Context::kMaxFlights];
+    for (int g = 0; g < K; ++g) {
+      pbs[g] = &c.flight_pb(g);
+      sts[g] = c.flight_stream(g);
+      if (g > 0) HM_HIP(hipStreamWaitEvent(sts[g], c.acc_event, 0), err);
+    }
     PassConfig cfg = c.pt_pass_config();
     cfg.timer = timer;
-    HM_TRY(pt_run_stream(v, c.pb, cfg, passes, c.accum, c.average, c.live_host, c.live_ev, c.stream, err));
+    HM_TRY(pt_run_stream(v, pbs, sts, K, cfg, passes, c.accum, c.average, c.live_host, c.live_ev, err));
+    // the context stream waits for every group
+    for (int g = 1; g < K; ++g) {
+      HM_HIP(hipEventRecord(c.acc_event, sts[g]), err);
+      HM_HIP(hipStreamWaitEvent(c.stream, c.acc_event, 0), err);
+    }
     c.accum_id += passes;
     c.last_streamed = true;
+    c.stream_groups = K;
     return kOk;
   }
   c.last_streamed = false;
-  HM_TRY(c.ensure_flights(err));
-  const int K = std::max(1, std::min(c.num_flights, Context::kMaxFlights));
   hipEvent_t start = nullptr;
   HM_HIP(hipEventCreateWithFlags(&start, hipEventDisableTiming), err);
   HM_HIP(hipEventRecord(start, c.stream), err);
@@ -693,11 +713,26 @@ int hmsnn_get_last_pass(hmsnn_ctx* ctx, float* rgba) {
   if (!ctx || !rgba) return fail(HMSNN_EINVAL, "hmsnn_get_last_pass: bad arguments");
   Context& c = ctx->c;
   if (!c.pb.L || c.accum_id == 0) return fail(HMSNN_EINVAL, "nothing rendered yet");
-  const int K = std::max(1, std::min(c.num_flights, Context::kMaxFlights));
-  const PathBuffers& last = c.last_streamed ? c.pb : c.flight_pb((c.accum_id - 1) % K);
-  const size_t bytes = sizeof(float4) * (size_t)c.scene.width * c.scene.height;
-  hipError_t e = hipMemcpyAsync(rgba, last.L, bytes, hipMemcpyDeviceToHost, c.stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  const int npx = c.scene.width * c.scene.height;
+  hipError_t e = hipSuccess;
+  if (c.last_streamed) {
+    // each stream group's slots hold its pixels' last pass
+    const int K = c.stream_groups;
+    std::vector<float4> tmp;
+    for (int g = 0; g < K && e == hipSuccess; ++g) {
+      const int cnt = stream_group_slots(npx, g, K);
+      tmp.resize(cnt);
+      e = hipMemcpyAsync(tmp.data(), c.flight_pb(g).L, sizeof(float4) * cnt, hipMemcpyDeviceToHost, c.stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+      for (int s = 0; s < cnt && e == hipSuccess; ++s)
+        std::memcpy(rgba + 4 * (size_t)stream_group_pixel(s, g, K), &tmp[s], sizeof(float4));
+    }
+  } else {
+    const int K = std::max(1, std::min(c.num_flights, Context::kMaxFlights));
+    const PathBuffers& last = c.flight_pb((c.accum_id - 1) % K);
+    e = hipMemcpyAsync(rgba, last.L, sizeof(float4) * (size_t)npx, hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  }
   return e == hipSuccess ? HMSNN_OK : fail(HMSNN_EDEVICE, std::string("get last pass: ") + hipGetErrorString(e));
 }
 

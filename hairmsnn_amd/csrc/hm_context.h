@@ -80,8 +80,9 @@ struct Context {
   // path buffers), 1 one wavefront per pass with passes in flight
   int pt_schedule = 0;
   bool last_streamed = false;   // the last pt call used the streaming schedule
-  int* live_host = nullptr;     // pinned, 2 ints: live-path count polls
-  hipEvent_t live_ev[2] = {nullptr, nullptr};
+  int stream_groups = 1;        // groups of that call
+  int* live_host = nullptr;     // pinned, 2 per group: live-path count polls
+  hipEvent_t live_ev[2 * kMaxFlights] = {};
   int ensure_flights(std::string& err);
   PathBuffers& flight_pb(int f) { return f == 0 ? pb : extra[f - 1].pb; }
   hipStream_t flight_stream(int f) { return f == 0 ? stream : extra[f - 1].stream; }

@@ -62,10 +62,12 @@ __device__ __forceinline__ void flush_counters(const TraceCounters& c, unsigned 
 
 }  // namespace
 
-// Camera ray of pixel i in the pass whose seed index is seed_pass
-// (path_tracing.cu:29-48): two jitter draws, then the path state reset.
-__device__ __forceinline__ void camera_path(const SceneView& sc, const PathBuffers& pb, int i, int seed_pass) {
-  const int x = i % sc.width, y = i / sc.width;
+// Camera ray of pixel px in the pass whose seed index is seed_pass
+// (path_tracing.cu:29-48), into path slot i: two jitter draws, then the path
+// state reset.
+__device__ __forceinline__ void camera_path(const SceneView& sc, const PathBuffers& pb, int i, int px,
+                                            int seed_pass) {
+  const int x = px % sc.width, y = px / sc.width;
   uint32_t rng = path_seed((uint32_t)(seed_pass + 10007), (uint32_t)x, (uint32_t)y, (uint32_t)sc.width);
   const float jx = draw(rng), jy = draw(rng);
   const float sx = ((float)x + jx) / (float)sc.width;
@@ -97,7 +99,7 @@ __global__ void k_camera(SceneView sc, PathBuffers pb, int accum_id, int msnn, c
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) pb.counts[0] = n;
   if (i >= n) return;
-  camera_path(sc, pb, i, accum_id);
+  camera_path(sc, pb, i, i, accum_id);
   pb.queue[0][i] = i;
   uint32_t flags = 0;
   if (msnn) {
@@ -359,9 +361,12 @@ __global__ void HM_TRACE_BOUNDS k_extend(SceneView sc, PathBuffers pb, int cur,
 
 // kStream: the depth is per slot (slot_depth), and terminated paths are
 // appended to done_queue for k_stream_finish.
+#ifndef HM_SHADE_MIN_WAVES
+#define HM_SHADE_MIN_WAVES 1
+#endif
 template <bool kStream>
-__global__ void __launch_bounds__(256) k_shade(SceneView sc, PathBuffers pb, ShadeParams sp,
-                                               GBufferOut gb) {
+__global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc, PathBuffers pb, ShadeParams sp,
+                                                                   GBufferOut gb) {
   const int nxt = sp.cur ^ 1;
   const int n = pb.counts[sp.cur];
   const int* q = pb.queue[sp.cur];
@@ -673,14 +678,17 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
 // its radiance to the pixel's running sum (pass order per pixel, so the image
 // equals the one consecutive passes make) and starts the pixel's next pass in
 // the same slot. Each wavefront launch then carries about one ray per pixel
-// instead of a pass's shrinking tail.
+// instead of a pass's shrinking tail. The pixels are split into K groups,
+// each with its own path buffers and stream, so that one group's shading
+// overlaps another's traversal: group g owns the 64-pixel chunks g, g + K,
+// g + 2K, ... (balanced work, coherent camera rays within a wave).
 
-__global__ void k_stream_camera(SceneView sc, PathBuffers pb, int first_pass, int seed_offset) {
-  const int n = sc.width * sc.height;
+__global__ void k_stream_camera(SceneView sc, PathBuffers pb, int first_pass, int seed_offset, int g, int K,
+                                int count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) pb.counts[0] = n;
-  if (i >= n) return;
-  camera_path(sc, pb, i, first_pass + seed_offset);
+  if (i == 0) pb.counts[0] = count;
+  if (i >= count) return;
+  camera_path(sc, pb, i, stream_group_pixel(i, g, K), first_pass + seed_offset);
   pb.flags[i] = 0u;
   pb.slot_pass[i] = first_pass;
   pb.slot_depth[i] = 0;
@@ -690,7 +698,7 @@ __global__ void k_stream_camera(SceneView sc, PathBuffers pb, int first_pass, in
 // Accumulates each finished path (writePixel, utils.cuh:13-35) and queues the
 // pixel's next pass on queue nxt while passes remain (pass < end_pass).
 __global__ void k_stream_finish(SceneView sc, PathBuffers pb, int nxt, float4* accum, float4* average,
-                                int end_pass, int seed_offset) {
+                                int end_pass, int seed_offset, int g, int K) {
   const int n = pb.counts[kDoneCount];
   const int stride = gridDim.x * blockDim.x;
   const int nloop = (n + stride - 1) / stride * stride;
@@ -699,19 +707,20 @@ __global__ void k_stream_finish(SceneView sc, PathBuffers pb, int nxt, float4* a
     int slot = -1;
     if (i < n) {
       slot = pb.done_queue[i];
+      const int px = stream_group_pixel(slot, g, K);
       const int pass = pb.slot_pass[slot];
       const float4 l = pb.L[slot];
       V3 c = v3(l.x, l.y, l.z);
       if (any_nan(c)) c = v3(0.f);
       if (pass > 0) {
-        const float4 a = accum[slot];
+        const float4 a = accum[px];
         c = c + v3(a.x, a.y, a.z);
       }
-      accum[slot] = make_float4(c.x, c.y, c.z, 1.f);
+      accum[px] = make_float4(c.x, c.y, c.z, 1.f);
       const float inv = 1.f / (float)(pass + 1);
-      average[slot] = make_float4(c.x * inv, c.y * inv, c.z * inv, 1.f);
+      average[px] = make_float4(c.x * inv, c.y * inv, c.z * inv, 1.f);
       if (pass + 1 < end_pass) {
-        camera_path(sc, pb, slot, pass + 1 + seed_offset);
+        camera_path(sc, pb, slot, px, pass + 1 + seed_offset);
         pb.slot_pass[slot] = pass + 1;
         pb.slot_depth[slot] = 0;
         again = true;
@@ -722,78 +731,103 @@ __global__ void k_stream_finish(SceneView sc, PathBuffers pb, int nxt, float4* a
   }
 }
 
-int pt_run_stream(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, int passes, float4* accum,
-                  float4* average, int* live_host, hipEvent_t* live_ev, hipStream_t stream, std::string& err) {
-  if (passes <= 0) return kOk;
-  const int n = sc.width * sc.height;
-  hipLaunchKernelGGL(k_stream_camera, dim3((n + 255) / 256), dim3(256), 0, stream, sc, pb, cfg.accum_id,
-                     cfg.seed_offset);
-  HM_TRY(check_launch("stream camera", err));
+int stream_group_slots(int npx, int g, int K) {
+  const int chunks = npx / 64;
+  return g < chunks ? 64 * ((chunks - g + K - 1) / K) : 0;
+}
+
+int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_t* streams, int K,
+                  const PassConfig& cfg, int passes, float4* accum, float4* average, int* live_host,
+                  hipEvent_t* live_ev, std::string& err) {
+  const int npx = sc.width * sc.height;
+  if (passes <= 0 || npx % 64 != 0 || K < 1) return passes <= 0 ? kOk : (err = "bad stream groups", kEInval);
   PassTimer* tm = cfg.timer;
+  hipStream_t tst = streams[0];
   auto tic = [&](int kind) {
     if (tm && tm->used < tm->capacity) {
       tm->kinds[tm->used] = kind;
-      (void)hipEventRecord(tm->events[2 * tm->used], stream);
+      (void)hipEventRecord(tm->events[2 * tm->used], tst);
     }
   };
   auto toc = [&]() {
     if (tm && tm->used < tm->capacity) {
-      (void)hipEventRecord(tm->events[2 * tm->used + 1], stream);
+      (void)hipEventRecord(tm->events[2 * tm->used + 1], tst);
       tm->used++;
     }
   };
-  const int ext_blocks = std::min(cfg.trace_blocks, std::max(1, (n + kTraceBlock - 1) / kTraceBlock));
-  const int sh_blocks = std::min(cfg.trace_blocks, std::max(1, (2 * n + kTraceBlock - 1) / kTraceBlock));
-  const int shade_blocks = std::min(cfg.shade_blocks, std::max(1, (n + 255) / 256));
   const int end_pass = cfg.accum_id + passes;
-  // Iterations are issued in even-sized batches (the live queue is back at
-  // index 0 after each); the live count after batch k is read while batch
-  // k + 1 runs, so the loop ends one batch of empty launches late.
-  constexpr int kBatch = 8;
+  constexpr int kBatch = 8;  // even: each group's live queue is back at index 0 after a batch
+  std::vector<int> cur(K, 0), live(K, 0);
+  std::vector<long long> iters(K, 0);
+  for (int g = 0; g < K; ++g) {
+    const int cnt = stream_group_slots(npx, g, K);
+    live[g] = cnt > 0;
+    if (!cnt) continue;
+    hipLaunchKernelGGL(k_stream_camera, dim3((cnt + 255) / 256), dim3(256), 0, streams[g], sc, *pbs[g], cfg.accum_id,
+                       cfg.seed_offset, g, K, cnt);
+  }
+  HM_TRY(check_launch("stream camera", err));
   const long long max_iters = (long long)passes * std::max(1, cfg.max_vertex + 1) + 4 * kBatch;
-  int cur = 0;
-  long long it = 0;
   for (int batch = 0;; ++batch) {
-    for (int k = 0; k < kBatch; ++k, ++it) {
-      const int nxt = cur ^ 1;
-      hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, stream, pb.counts, nxt, 2, 3);
-      tic(0);
-      hipLaunchKernelGGL(k_extend<false>, dim3(ext_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cur,
-                         (unsigned long long*)nullptr);
-      toc();
-      ShadeParams sp;
-      sp.depth = 0;
-      sp.dl_zero = cfg.dl_zero;
-      sp.dl_from = cfg.dl_from;
-      sp.max_vertex = cfg.max_vertex;
-      sp.trunc_beta = cfg.trunc_beta;
-      sp.msnn = 0;
-      sp.cur = cur;
-      tic(1);
-      hipLaunchKernelGGL(k_shade<true>, dim3(shade_blocks), dim3(256), 0, stream, sc, pb, sp, GBufferOut{});
-      toc();
-      tic(2);
-      hipLaunchKernelGGL(k_shadow<false>, dim3(sh_blocks), dim3(kTraceBlock), 0, stream, sc, pb,
-                         (unsigned long long*)nullptr);
-      hipLaunchKernelGGL(k_shadow_resolve, dim3(shade_blocks), dim3(256), 0, stream, pb, 0, 0, NrcPassArgs{});
-      toc();
-      tic(3);
-      hipLaunchKernelGGL(k_stream_finish, dim3(shade_blocks), dim3(256), 0, stream, sc, pb, nxt, accum, average,
-                         end_pass, cfg.seed_offset);
-      toc();
-      cur = nxt;
+    bool any = false;
+    for (int g = 0; g < K; ++g) {
+      if (!live[g]) continue;
+      any = true;
+      PathBuffers& pb = *pbs[g];
+      hipStream_t st = streams[g];
+      const int cnt = stream_group_slots(npx, g, K);
+      const int ext_blocks = std::min(cfg.trace_blocks, std::max(1, (cnt + kTraceBlock - 1) / kTraceBlock));
+      const int sh_blocks = std::min(cfg.trace_blocks, std::max(1, (2 * cnt + kTraceBlock - 1) / kTraceBlock));
+      const int shade_blocks = std::min(cfg.shade_blocks, std::max(1, (cnt + 255) / 256));
+      const bool timed = g == 0 && tm;
+      for (int k = 0; k < kBatch; ++k) {
+        const int nxt = cur[g] ^ 1;
+        hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, st, pb.counts, nxt, 2, 3);
+        if (timed) tic(0);
+        hipLaunchKernelGGL(k_extend<false>, dim3(ext_blocks), dim3(kTraceBlock), 0, st, sc, pb, cur[g],
+                           (unsigned long long*)nullptr);
+        if (timed) toc();
+        ShadeParams sp;
+        sp.depth = 0;
+        sp.dl_zero = cfg.dl_zero;
+        sp.dl_from = cfg.dl_from;
+        sp.max_vertex = cfg.max_vertex;
+        sp.trunc_beta = cfg.trunc_beta;
+        sp.msnn = 0;
+        sp.cur = cur[g];
+        if (timed) tic(1);
+        hipLaunchKernelGGL(k_shade<true>, dim3(shade_blocks), dim3(256), 0, st, sc, pb, sp, GBufferOut{});
+        if (timed) toc();
+        if (timed) tic(2);
+        hipLaunchKernelGGL(k_shadow<false>, dim3(sh_blocks), dim3(kTraceBlock), 0, st, sc, pb,
+                           (unsigned long long*)nullptr);
+        hipLaunchKernelGGL(k_shadow_resolve, dim3(shade_blocks), dim3(256), 0, st, pb, 0, 0, NrcPassArgs{});
+        if (timed) toc();
+        if (timed) tic(3);
+        hipLaunchKernelGGL(k_stream_finish, dim3(shade_blocks), dim3(256), 0, st, sc, pb, nxt, accum, average,
+                           end_pass, cfg.seed_offset, g, K);
+        if (timed) toc();
+        cur[g] = nxt;
+      }
+      iters[g] += kBatch;
+      HM_TRY(check_launch("stream iteration", err));
+      const int s = 2 * g + (batch & 1);
+      HM_HIP(hipMemcpyAsync(&live_host[s], pb.counts + cur[g], sizeof(int), hipMemcpyDeviceToHost, st), err);
+      HM_HIP(hipEventRecord(live_ev[s], st), err);
     }
-    HM_TRY(check_launch("stream iteration", err));
-    const int s = batch & 1;
-    HM_HIP(hipMemcpyAsync(&live_host[s], pb.counts + cur, sizeof(int), hipMemcpyDeviceToHost, stream), err);
-    HM_HIP(hipEventRecord(live_ev[s], stream), err);
+    if (!any) break;
     if (batch > 0) {
-      HM_HIP(hipEventSynchronize(live_ev[s ^ 1]), err);
-      if (live_host[s ^ 1] == 0) break;
-    }
-    if (it > max_iters) {
-      err = "streaming schedule did not drain";
-      return kEDevice;
+      // the previous batch's counts, read while this batch runs
+      for (int g = 0; g < K; ++g) {
+        if (!live[g]) continue;
+        const int s = 2 * g + ((batch - 1) & 1);
+        HM_HIP(hipEventSynchronize(live_ev[s]), err);
+        if (live_host[s] == 0) live[g] = 0;
+        if (iters[g] > max_iters) {
+          err = "streaming schedule did not drain";
+          return kEDevice;
+        }
+      }
     }
   }
   return kOk;

@@ -125,10 +125,16 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
 
 // Runs `passes` passes from cfg.accum_id with the streaming schedule: a
 // pixel's next pass starts in its slot as soon as its path ends, each finished
-// path accumulated into accum / average in pass order. live_host (pinned, 2
-// ints) and live_ev (2 events) poll the live-path count.
-int pt_run_stream(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, int passes, float4* accum,
-                  float4* average, int* live_host, hipEvent_t* live_ev, hipStream_t stream, std::string& err);
+// path accumulated into accum / average in pass order. The pixels are split
+// into K groups (64-pixel chunks dealt round-robin), group g on pbs[g] and
+// streams[g]; live_host (pinned, 2K ints) and live_ev (2K events) poll the
+// live-path counts. A timer records group 0's launches.
+int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_t* streams, int K,
+                  const PassConfig& cfg, int passes, float4* accum, float4* average, int* live_host,
+                  hipEvent_t* live_ev, std::string& err);
+// Slots of stream group g of K over npx pixels, and the pixel of a slot.
+int stream_group_slots(int npx, int g, int K);
+HM_HD int stream_group_pixel(int slot, int g, int K) { return ((slot >> 6) * K + g) * 64 + (slot & 63); }
 
 // Adds pb.L into the running sum and mean buffers (accumulation id `accum_id`).
 int pt_accumulate(const float4* L, float4* accum, float4* average, int n, int accum_id, hipStream_t stream,

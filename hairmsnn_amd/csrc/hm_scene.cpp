@@ -2,10 +2,14 @@
 // the seeded procedural generators for hair, head proxy and environment.
 #include "hm_scene.h"
 
+#include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <sstream>
 
+#include "../../include/hmsnn.h"
 #include "hm_json.h"
 
 namespace hm {
@@ -102,6 +106,7 @@ bool parse_json(const Json& cfg, Scene& scene, std::string& err) {
       scene.hair_geometry = h.at("geometry").str();
       is_hair = true;
     }
+    if (h.has("tessellation")) scene.hair_tessellation = std::max(1, std::min(64, h.at("tessellation").i()));
   }
   scene.has_hair = is_hair;
   if (!is_surface && !is_hair) {
@@ -392,7 +397,151 @@ void generate_hair(const ProceduralHair& g, HairGeometry& out) {
   }
 }
 
+int load_hair_file(const std::string& path, int tess, HairGeometry& out, std::string& err) {
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) {
+    err = "Error: Cannot open hair file! (" + path + ")";
+    return HMSNN_EIO;
+  }
+  std::vector<uint8_t> b;
+  {
+    uint8_t tmp[1 << 16];
+    size_t k;
+    while ((k = std::fread(tmp, 1, sizeof(tmp), f)) > 0) b.insert(b.end(), tmp, tmp + k);
+    std::fclose(f);
+  }
+  // header: signature, hair_count, point_count, arrays, d_segments,
+  // d_thickness, d_transparency, d_color[3], info[88]  (128 bytes)
+  if (b.size() < 128) {
+    err = "Error: Cannot read hair file header! (" + path + ")";
+    return HMSNN_EIO;
+  }
+  if (std::memcmp(b.data(), "HAIR", 4) != 0) {
+    err = "Error: File has wrong signature! (" + path + ")";
+    return HMSNN_EIO;
+  }
+  auto u32 = [&](size_t o) {
+    uint32_t v;
+    std::memcpy(&v, b.data() + o, 4);
+    return v;
+  };
+  auto f32 = [&](size_t o) {
+    float v;
+    std::memcpy(&v, b.data() + o, 4);
+    return v;
+  };
+  const uint32_t hair_count = u32(4), point_count = u32(8), arrays = u32(12), d_segments = u32(16);
+  const float d_thickness = f32(20);
+  constexpr uint32_t kSegments = 1, kPoints = 2, kThickness = 4, kTransparency = 8, kColors = 16;
+  if (!(arrays & kPoints)) {
+    err = "Error: Cannot read hair points! (" + path + ": no point array)";
+    return HMSNN_EIO;
+  }
+  size_t o = 128;
+  const uint16_t* segs = nullptr;
+  if (arrays & kSegments) {
+    if (o + 2 * (size_t)hair_count > b.size()) {
+      err = "Error: Cannot read hair segments! (" + path + ")";
+      return HMSNN_EIO;
+    }
+    segs = reinterpret_cast<const uint16_t*>(b.data() + o);
+    o += 2 * (size_t)hair_count;
+  }
+  if (o + 12 * (size_t)point_count > b.size()) {
+    err = "Error: Cannot read hair points! (" + path + ")";
+    return HMSNN_EIO;
+  }
+  std::vector<float> pts((size_t)point_count * 3);
+  std::memcpy(pts.data(), b.data() + o, 12 * (size_t)point_count);
+  o += 12 * (size_t)point_count;
+  std::vector<float> thick;
+  if (arrays & kThickness) {
+    if (o + 4 * (size_t)point_count > b.size()) {
+      err = "Error: Cannot read hair thickness! (" + path + ")";
+      return HMSNN_EIO;
+    }
+    thick.resize(point_count);
+    std::memcpy(thick.data(), b.data() + o, 4 * (size_t)point_count);
+    o += 4 * (size_t)point_count;
+  }
+  if ((arrays & kTransparency) && o + 4 * (size_t)point_count > b.size()) {
+    err = "Error: Cannot read hair transparency! (" + path + ")";
+    return HMSNN_EIO;
+  }
+  if (arrays & kTransparency) o += 4 * (size_t)point_count;
+  if ((arrays & kColors) && o + 12 * (size_t)point_count > b.size()) {
+    err = "Error: Cannot read hair colors! (" + path + ")";
+    return HMSNN_EIO;
+  }
+  out = HairGeometry{};
+  out.bmin = v3(1e30f);
+  out.bmax = v3(-1e30f);
+  tess = std::max(1, tess);
+  size_t first = 0;
+  std::vector<V3> cp;
+  std::vector<float> cw;
+  for (uint32_t sidx = 0; sidx < hair_count; ++sidx) {
+    const int nseg = segs ? (int)segs[sidx] : (int)d_segments;
+    if (nseg < 1) continue;
+    if (first + (size_t)nseg + 1 > point_count) {
+      err = "Error: Cannot read hair points! (" + path + ": strands need more points than the file has)";
+      return HMSNN_EIO;
+    }
+    cp.clear();
+    cw.clear();
+    auto P = [&](size_t i) { return v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]); };
+    auto Wd = [&](size_t i) { return 0.2f * (thick.empty() ? d_thickness : thick[i]); };
+    // phantom points mirrored past the ends (scene.cpp:47-66)
+    cp.push_back(P(first) + (P(first) - P(first + 1)));
+    cw.push_back(Wd(first));
+    for (int k = 0; k <= nseg; ++k) {
+      cp.push_back(P(first + k));
+      cw.push_back(Wd(first + k));
+      out.bmin = vmin(out.bmin, P(first + k));
+      out.bmax = vmax(out.bmax, P(first + k));
+    }
+    cp.push_back(P(first + nseg) + (P(first + nseg) - P(first + nseg - 1)));
+    cw.push_back(Wd(first + nseg));
+    const int strand = out.num_strands++;
+    for (int k = 0; k < nseg; ++k) {
+      // uniform Catmull-Rom through cp[k+1] .. cp[k+2]
+      const V3 p0 = cp[k], p1 = cp[k + 1], p2 = cp[k + 2], p3 = cp[k + 3];
+      auto curve = [&](float t) {
+        const float t2 = t * t, t3 = t2 * t;
+        return 0.5f * ((2.f * p1) + (p2 - p0) * t + (2.f * p0 - 5.f * p1 + 4.f * p2 - p3) * t2 +
+                       (3.f * p1 - p0 - 3.f * p2 + p3) * t3);
+      };
+      auto width = [&](float t) {
+        const float t2 = t * t, t3 = t2 * t;
+        return 0.5f * ((2.f * cw[k + 1]) + (cw[k + 2] - cw[k]) * t +
+                       (2.f * cw[k] - 5.f * cw[k + 1] + 4.f * cw[k + 2] - cw[k + 3]) * t2 +
+                       (3.f * cw[k + 1] - cw[k] - 3.f * cw[k + 2] + cw[k + 3]) * t3);
+      };
+      V3 a = p1;
+      for (int j = 1; j <= tess; ++j) {
+        const V3 e = j == tess ? p2 : curve((float)j / (float)tess);
+        const float r = std::max(0.f, width(((float)j - 0.5f) / (float)tess));
+        const float rec[8] = {a.x, a.y, a.z, r, e.x, e.y, e.z, (float)strand};
+        out.seg.insert(out.seg.end(), rec, rec + 8);
+        out.strand_of_seg.push_back(strand);
+        a = e;
+      }
+    }
+    first += (size_t)nseg + 1;
+  }
+  if (out.seg.empty()) {
+    err = "Error: hair file " + path + " holds no segments";
+    return HMSNN_EIO;
+  }
+  return HMSNN_OK;
+}
+
 // Procedural / constant environment maps (RGBA32F, row y = theta bin).
+std::string scene_path(const Scene& s, const std::string& p) {
+  if (p.empty() || p[0] == '/' || s.base_dir.empty()) return p;
+  return s.base_dir + "/" + p;
+}
+
 void generate_env(const Scene& s, EnvImage& out) {
   if (s.env_constant) {
     out.width = s.env_constant_w;

@@ -60,6 +60,7 @@ struct Scene {
   bool hair_procedural = false;
   ProceduralHair hair_gen;
   std::string hair_geometry;
+  int hair_tessellation = 4;  // linear pieces per Catmull-Rom segment of a .hair file
 
   // camera (required)
   SceneCamera camera;
@@ -125,11 +126,21 @@ struct HairGeometry {
   V3 bmin{1e30f, 1e30f, 1e30f}, bmax{-1e30f, -1e30f, -1e30f};
 };
 void generate_hair(const ProceduralHair& g, HairGeometry& out);
+// Cem Yuksel .hair file (the format cyHairFile reads; scene.cpp:10-75 builds
+// the curves): each strand's points, with a phantom point mirrored past each
+// end, form uniform Catmull-Rom segments of radius 0.2 x thickness; every
+// segment is split into `tess` linear pieces (sampled on the curve, radius
+// at the piece midpoint).
+int load_hair_file(const std::string& path, int tess, HairGeometry& out, std::string& err);
 
 struct EnvImage {
   int width = 0, height = 0;
   std::vector<float> rgba;  // width*height*4
 };
 void generate_env(const Scene& s, EnvImage& out);
+// Scanline OpenEXR reader (hm_image.cpp): RGBA floats, file row order.
+int read_exr(const std::string& path, int& width, int& height, std::vector<float>& rgba, std::string& err);
+// A scene-relative path (the scene file's directory) or an absolute one.
+std::string scene_path(const Scene& s, const std::string& p);
 
 }  // namespace hm

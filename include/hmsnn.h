@@ -303,6 +303,14 @@ int hmsnn_mlp_train_step_dev(hmsnn_mlp* mlp, const float* in_dev, const float* t
  * (model.cpp:366-383). */
 int hmsnn_save_exr(const char* path, const float* rgba, int width, int height);
 
+/* Reads a scanline OpenEXR image as tinyexr's LoadEXR does for the
+ * reference's environment maps (model.cpp:158-180): RGBA floats in file row
+ * order, alpha 1 when absent, a lone Y channel copied to RGB. Channels HALF,
+ * FLOAT or UINT; compression NONE, RLE, ZIPS or ZIP. hmsnn_exr_size gives the
+ * size, hmsnn_load_exr fills width x height x 4 floats. */
+int hmsnn_exr_size(const char* path, int* width, int* height);
+int hmsnn_load_exr(const char* path, float* rgba_host, int width, int height);
+
 /* ---- differentiable appearance (path-replay adjoint) --------------------- */
 
 /* Sets the global hair roughness and scale tilt (the values parseScene reads,
