@@ -176,6 +176,42 @@ def msnn_vs_pt(sc, pt_ms_per_pass: float, R: dict, frames: int, beta: int = 1) -
     }
 
 
+def nrc_vs_pt(sc, pt_ms_per_pass: float, R: dict, frames: int) -> dict:
+    """Neural radiance caching (the reference's comparator renderer) vs plain
+    path tracing, measured like msnn_vs_pt: NRC trains online from the first
+    frame, so its time is the frames' wall time."""
+    ref, a, b = R["ref"], R["a"], R["b"]
+    sc.synchronize()
+    sc.nrc_begin()
+    fr, mses, ts = [], [], []
+    t_frames, done = 0.0, 0
+    parts = {"gbuffer_ms": 0.0, "infer_ms": 0.0, "render_ms": 0.0, "train_ms": 0.0}
+    queries = 0
+    for f in sorted({1, 2, 4, 8, 16, 32, 64, frames}):
+        if f > frames:
+            continue
+        t0 = time.perf_counter()
+        st = sc.nrc_frames(f - done)
+        t_frames += time.perf_counter() - t0
+        for k in parts:
+            parts[k] += st[k]
+        queries = st["cache_queries"]
+        done = f
+        fr.append(f)
+        ts.append(t_frames)
+        mses.append(float(np.mean((sc.nrc_image()[..., :3].astype(np.float64) - ref) ** 2)))
+    m_final = max(mses[-1] - b, 1e-30)
+    k_eq = a / m_final
+    t_pt = k_eq * pt_ms_per_pass * 1e-3
+    log(f"nrc vs pt: NRC {frames} frames MSE {mses[-1]:.4g} in {ts[-1]:.2f} s = PT at {k_eq:.1f} spp, "
+        f"{t_pt:.2f} s -> speed-up {t_pt / ts[-1]:.2f}x; per frame: " +
+        ", ".join(f"{k} {v / frames:.2f}" for k, v in parts.items()))
+    return {"frames": frames, "speedup": t_pt / ts[-1], "t_nrc_s": ts[-1], "t_pt_equal_rmse_s": t_pt,
+            "pt_spp_equal_rmse": k_eq, "rmse_nrc": float(np.sqrt(m_final)),
+            "nrc_frame_ms": {k: v / frames for k, v in parts.items()}, "cache_queries_last_frame": queries,
+            "nrc_curve": {"frames": fr, "t_s": ts, "mse": mses}}
+
+
 def adjoint_rate(hm, scene_path: Path, size: int = 512, spp: int = 16) -> dict:
     """The differentiable-appearance workload: c1 geometry at size^2, spp
     samples, per-strand sigma_a ~ U(0.01, 0.5) (seed 3) and the global
@@ -340,6 +376,7 @@ def main() -> None:
             str(bt): {k: v for k, v in msnn_vs_pt(sc, wall / args.steps * 1e3, R, 64, beta=bt).items()
                       if k in ("speedup", "t_msnn_s", "pt_spp_equal_rmse", "rmse_msnn")}
             for bt in (2, 5, 10)}
+        line["nrc_vs_pt"] = nrc_vs_pt(sc, wall / args.steps * 1e3, R, 64)
         fm = line["msnn_vs_pt"]["msnn_frame_ms"]
         train_records = int(json.loads(Path(args.scene).read_text()).get("hairmsnn", {}).get("train_records", 16384))
         line["mlp"] = {

@@ -148,6 +148,18 @@ def generate_host(json_text: str | dict) -> tuple[np.ndarray, np.ndarray | None]
     return seg, env
 
 
+def load_obj(path: str) -> dict:
+    """An OBJ head mesh as the library reads it (host only): vertices and
+    normals (T x 3 x 3) and diffuse colours (T x 3) per triangle."""
+    lib = load()
+    n = C.c_int()
+    check(lib.hmsnn_obj_load(str(path).encode(), C.byref(n), None, None, None, 0))
+    v, nn, kd = (np.empty((n.value, 3, 3), np.float32), np.empty((n.value, 3, 3), np.float32),
+                 np.empty((n.value, 3), np.float32))
+    check(lib.hmsnn_obj_load(str(path).encode(), C.byref(n), _ptr(v), _ptr(nn), _ptr(kd), n.value))
+    return {"v": v, "n": nn, "kd": kd}
+
+
 class Scene:
     """A scene context: geometry, BVH, environment tables and render state on the GPU."""
 
@@ -198,6 +210,15 @@ class Scene:
         out = np.empty((int(self.info["num_segments"]), 8), dtype=np.float32)
         check(self.lib.hmsnn_export_segments(self.h, _ptr(out)))
         return out
+
+    def mesh(self) -> dict | None:
+        """The OBJ head mesh: vertices (T x 3 x 3), normals (T x 3 x 3), diffuse (T x 3)."""
+        n = int(self.info["num_triangles"])
+        if n == 0:
+            return None
+        v, nn, kd = np.empty((n, 3, 3), np.float32), np.empty((n, 3, 3), np.float32), np.empty((n, 3), np.float32)
+        check(self.lib.hmsnn_export_mesh(self.h, _ptr(v), _ptr(nn), _ptr(kd)))
+        return {"v": v, "n": nn, "kd": kd}
 
     def env(self) -> dict:
         W, H = int(self.info["env_width"]), int(self.info["env_height"])

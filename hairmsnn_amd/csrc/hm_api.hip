@@ -119,6 +119,9 @@ Context::~Context() {
   dev_free(strand_sigma_dev);
   dev_free(stats_dev);
   bvh.release();
+  mesh_bvh.release();
+  dev_free(mesh_tri);
+  dev_free(mesh_shade);
   env.release();
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -187,7 +190,7 @@ int Context::ensure_path_buffers(bool msnn_mode, std::string& err, int min_slots
   free_path_buffers();
   const size_t n = (size_t)want;
   pb_slots = want;
-  spill_levels = std::max(1, bvh.stack_bound() - kStackLds + 1);
+  spill_levels = stack_levels();
   HM_TRY(alloc_path_buffers(pb, pb_allocs, n, msnn_mode, spill_levels, trace_blocks * kTraceBlock, stream, err));
   pb_msnn = msnn_mode;
   const size_t f4 = sizeof(float4) * n;
@@ -209,11 +212,15 @@ int Context::ensure_flights(std::string& err) {
     if (fl.slots >= want) continue;
     for (void* p : fl.allocs) dev_free(p);
     fl.allocs.clear();
-    HM_TRY(alloc_path_buffers(fl.pb, fl.allocs, (size_t)want, false, std::max(1, bvh.stack_bound() - kStackLds + 1),
+    HM_TRY(alloc_path_buffers(fl.pb, fl.allocs, (size_t)want, false, stack_levels(),
                               trace_blocks * kTraceBlock, fl.stream, err));
     fl.slots = want;
   }
   return kOk;
+}
+
+int Context::stack_levels() const {
+  return std::max(1, std::max(bvh.stack_bound(), mesh_bvh.stack_bound()) - kStackLds + 1);
 }
 
 PassConfig Context::pt_pass_config() const {
@@ -243,11 +250,6 @@ int Context::init(std::string& err) {
   trace_blocks = std::max(64, num_cus * 8);
 
   // ---- geometry ----
-  if (scene.has_surface && !scene.surface_sphere) {
-    err = "surface.geometry '" + scene.surface_geometry +
-          "': OBJ meshes are not supported; use surface.sphere {center, radius}";
-    return kEUnsupported;
-  }
   if ((int)scene.dlight_from.size() > kMaxDirLights) {
     err = "at most " + std::to_string(kMaxDirLights) + " directional lights are supported";
     return kEUnsupported;
@@ -259,12 +261,47 @@ int Context::init(std::string& err) {
   bmin = v3(1e30f);
   bmax = v3(-1e30f);
   float sscale = 0.f;
-  if (scene.has_surface) {
+  if (scene.has_surface && scene.surface_sphere) {
     const V3 c = scene.sphere_center;
     const float r = scene.sphere_radius;
     bmin = vmin(bmin, c - v3(r));
     bmax = vmax(bmax, c + v3(r));
     sscale = length(bmax - bmin);
+  } else if (scene.has_surface) {
+    // loadOBJ (model.cpp:233-358); the scene scale takes the mesh bounds like
+    // the reference's surface scale (render_path_tracing.cu:212-214)
+    HM_TRY(load_obj(scene_path(scene, scene.surface_geometry), mesh, err));
+    bmin = vmin(bmin, mesh.bmin);
+    bmax = vmax(bmax, mesh.bmax);
+    sscale = length(mesh.bmax - mesh.bmin);
+    const int nt = mesh.num();
+    std::vector<float4> tri((size_t)3 * nt), shade((size_t)4 * nt);
+    std::vector<float> boxes((size_t)8 * nt);  // the triangles' boxes as zero-radius segments for the BVH builder
+    for (int k = 0; k < nt; ++k) {
+      V3 lo = v3(1e30f), hi = v3(-1e30f);
+      for (int j = 0; j < 3; ++j) {
+        const float* p = &mesh.v[9 * (size_t)k + 3 * j];
+        const float* q = &mesh.n[9 * (size_t)k + 3 * j];
+        tri[3 * (size_t)k + j] = make_float4(p[0], p[1], p[2], 0.f);
+        shade[4 * (size_t)k + j] = make_float4(q[0], q[1], q[2], 0.f);
+        lo = vmin(lo, v3(p[0], p[1], p[2]));
+        hi = vmax(hi, v3(p[0], p[1], p[2]));
+      }
+      shade[4 * (size_t)k + 3] = make_float4(mesh.kd[3 * k], mesh.kd[3 * k + 1], mesh.kd[3 * k + 2], 0.f);
+      const float rec[8] = {lo.x, lo.y, lo.z, 0.f, hi.x, hi.y, hi.z, (float)k};
+      std::copy(rec, rec + 8, &boxes[8 * (size_t)k]);
+    }
+    HM_TRY(dev_alloc((void**)&mesh_tri, sizeof(float4) * tri.size(), err));
+    HM_TRY(dev_alloc((void**)&mesh_shade, sizeof(float4) * shade.size(), err));
+    HM_HIP(hipMemcpy(mesh_tri, tri.data(), sizeof(float4) * tri.size(), hipMemcpyHostToDevice), err);
+    HM_HIP(hipMemcpy(mesh_shade, shade.data(), sizeof(float4) * shade.size(), hipMemcpyHostToDevice), err);
+    float* boxes_dev = nullptr;
+    HM_TRY(dev_alloc((void**)&boxes_dev, sizeof(float) * boxes.size(), err));
+    HM_HIP(hipMemcpy(boxes_dev, boxes.data(), sizeof(float) * boxes.size(), hipMemcpyHostToDevice), err);
+    const int rc = bvh_build(boxes_dev, nt, mesh_bvh, stream, err);
+    (void)hipStreamSynchronize(stream);
+    dev_free(boxes_dev);
+    HM_TRY(rc);
   }
   if (scene.has_hair) {
     if (scene.hair_procedural) generate_hair(scene.hair_gen, hair);
@@ -305,7 +342,13 @@ int Context::init(std::string& err) {
   v.bvh.num_segs = bvh.num_segs;
   v.seg_by_id = seg_dev;
   v.strand_sigma = nullptr;
-  v.sphere.enabled = scene.has_surface ? 1 : 0;
+  v.sphere.enabled = scene.has_surface && scene.surface_sphere ? 1 : 0;
+  v.mesh.bvh.nodes = mesh_bvh.nodes_w;
+  v.mesh.bvh.segs = mesh_bvh.segs;
+  v.mesh.bvh.num_segs = mesh_bvh.num_segs;
+  v.mesh.tri = mesh_tri;
+  v.mesh.shade = mesh_shade;
+  v.mesh.num = mesh.num();
   v.sphere.cx = scene.sphere_center.x;
   v.sphere.cy = scene.sphere_center.y;
   v.sphere.cz = scene.sphere_center.z;
@@ -323,6 +366,8 @@ int Context::init(std::string& err) {
   v.env.c_pdf = env.c_pdf;
   v.env.c_cdf = env.c_cdf;
   v.env.m_pdf = env.m_pdf;
+  v.env.m_guide = env.m_guide;
+  v.env.c_guide = env.c_guide;
   v.env.m_cdf = env.m_cdf;
   v.num_dlights = (int)scene.dlight_from.size();
   v.num_total_lights = v.num_dlights + (scene.has_env ? 1 : 0);
@@ -471,6 +516,7 @@ int hmsnn_get_info(const hmsnn_ctx* ctx, hmsnn_info* info) {
   fill_scene_info(c.scene, info);
   info->num_segments = c.bvh.num_segs;
   info->num_strands = c.hair.num_strands;
+  info->num_triangles = c.mesh.num();
   info->bvh_nodes = c.bvh.num_nodes_w;
   info->bvh_depth = c.bvh.depth_w;
   info->env_width = c.env.width;
@@ -535,6 +581,15 @@ int hmsnn_generate_host(const char* text, float* seg_host, size_t seg_cap, int* 
 int hmsnn_export_segments(const hmsnn_ctx* ctx, float* seg_host) {
   if (!ctx || !seg_host) return fail(HMSNN_EINVAL, "hmsnn_export_segments: NULL argument");
   std::memcpy(seg_host, ctx->c.hair.seg.data(), sizeof(float) * ctx->c.hair.seg.size());
+  return HMSNN_OK;
+}
+
+int hmsnn_export_mesh(const hmsnn_ctx* ctx, float* vertices, float* normals, float* diffuse) {
+  if (!ctx) return fail(HMSNN_EINVAL, "hmsnn_export_mesh: NULL context");
+  const TriMesh& m = ctx->c.mesh;
+  if (vertices) std::memcpy(vertices, m.v.data(), sizeof(float) * m.v.size());
+  if (normals) std::memcpy(normals, m.n.data(), sizeof(float) * m.n.size());
+  if (diffuse) std::memcpy(diffuse, m.kd.data(), sizeof(float) * m.kd.size());
   return HMSNN_OK;
 }
 

@@ -51,6 +51,16 @@ struct SphereView {
   float cx, cy, cz, r;
 };
 
+// OBJ head mesh: BVH over the triangles' boxes, vertices (3 float4 per
+// triangle) for intersection, and per triangle 3 vertex normals + the diffuse
+// colour (4 float4) for shading.
+struct MeshView {
+  BvhView bvh;
+  const float4* tri;
+  const float4* shade;
+  int num;
+};
+
 constexpr int kStackLds = 16;
 constexpr int kTraceBlock = 256;
 
@@ -168,17 +178,40 @@ HM_D void closest_begin(const BvhView& bvh, TraceStack& st, ClosestState& s) {
   s.iter = 0;
 }
 
-// Closest hit over hair segments: minimum (t, id) lexicographically.
-template <bool kCount>
-HM_D void closest_step(const BvhView& bvh, const RayInv& r, TraceStack& st, ClosestState& s, TraceCounters& cnt) {
-  if (s.cur < 0) {
-    const int leaf = ~s.cur;
+// Leaf tests: a leaf record is two float4s in Morton order whose second w
+// holds the primitive id. Hair: the record is the segment itself. Triangle
+// meshes: the record holds the triangle's box, the vertices are read by id.
+struct SegLeaf {
+  HM_D bool operator()(const BvhView& bvh, const RayInv& r, int leaf, float* t, float* u, int* id) const {
     const float4 s0 = bvh.segs[2 * leaf];
     const float4 s1 = bvh.segs[2 * leaf + 1];
+    *id = __float_as_int(s1.w);
+    return intersect_segment(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, s0.x, s0.y, s0.z, s1.x, s1.y, s1.z, s0.w, t, u);
+  }
+};
+
+struct TriLeaf {
+  const float4* tri;  // 3 float4 per triangle, original order
+  HM_D bool operator()(const BvhView& bvh, const RayInv& r, int leaf, float* t, float* u, int* id) const {
+    const int k = __float_as_int(bvh.segs[2 * leaf + 1].w);
+    *id = k;
+    const float4 a = tri[3 * k], b = tri[3 * k + 1], c = tri[3 * k + 2];
+    float v;
+    return intersect_triangle(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t,
+                              u, &v);
+  }
+};
+
+// Closest hit over the tree's primitives: minimum (t, id) lexicographically.
+template <bool kCount, class Leaf = SegLeaf>
+HM_D void closest_step(const BvhView& bvh, const RayInv& r, TraceStack& st, ClosestState& s, TraceCounters& cnt,
+                       const Leaf& leaf_test = Leaf{}) {
+  if (s.cur < 0) {
+    const int leaf = ~s.cur;
     if (kCount) cnt.segs++;
     float t, u;
-    if (intersect_segment(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, s0.x, s0.y, s0.z, s1.x, s1.y, s1.z, s0.w, &t, &u)) {
-      const int id = __float_as_int(s1.w);
+    int id;
+    if (leaf_test(bvh, r, leaf, &t, &u, &id)) {
       if (t < s.best_t || (t == s.best_t && id < s.best_id)) {
         s.best_t = t;
         s.best_id = id;
@@ -216,15 +249,15 @@ HM_D void closest_step(const BvhView& bvh, const RayInv& r, TraceStack& st, Clos
   if (++s.iter >= trace_cap(bvh)) s.cur = kNone;
 }
 
-template <bool kCount>
+template <bool kCount, class Leaf = SegLeaf>
 HM_D void trace_closest(const BvhView& bvh, const RayInv& r, TraceStack& st, float& best_t, int& best_id,
-                        float& best_u, TraceCounters& cnt) {
+                        float& best_u, TraceCounters& cnt, const Leaf& leaf_test = Leaf{}) {
   ClosestState s;
   s.best_t = best_t;
   s.best_id = best_id;
   s.best_u = best_u;
   closest_begin(bvh, st, s);
-  while (s.cur != kNone) closest_step<kCount>(bvh, r, st, s, cnt);
+  while (s.cur != kNone) closest_step<kCount, Leaf>(bvh, r, st, s, cnt, leaf_test);
   best_t = s.best_t;
   best_id = s.best_id;
   best_u = s.best_u;
@@ -245,16 +278,16 @@ HM_D void any_begin(const BvhView& bvh, TraceStack& st, AnyState& s) {
   s.hit = false;
 }
 
-template <bool kCount>
-HM_D void any_step(const BvhView& bvh, const RayInv& r, TraceStack& st, AnyState& s, TraceCounters& cnt) {
+template <bool kCount, class Leaf = SegLeaf>
+HM_D void any_step(const BvhView& bvh, const RayInv& r, TraceStack& st, AnyState& s, TraceCounters& cnt,
+                   const Leaf& leaf_test = Leaf{}) {
   const float tinf = __builtin_inff();
   if (s.cur < 0) {
     const int leaf = ~s.cur;
-    const float4 s0 = bvh.segs[2 * leaf];
-    const float4 s1 = bvh.segs[2 * leaf + 1];
     if (kCount) cnt.segs++;
     float t, u;
-    if (intersect_segment(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, s0.x, s0.y, s0.z, s1.x, s1.y, s1.z, s0.w, &t, &u)) {
+    int id;
+    if (leaf_test(bvh, r, leaf, &t, &u, &id)) {
       s.hit = true;
       s.cur = kNone;
     } else {
@@ -278,11 +311,12 @@ HM_D void any_step(const BvhView& bvh, const RayInv& r, TraceStack& st, AnyState
   if (++s.iter >= trace_cap(bvh)) s.cur = kNone;
 }
 
-template <bool kCount>
-HM_D bool trace_any(const BvhView& bvh, const RayInv& r, TraceStack& st, TraceCounters& cnt) {
+template <bool kCount, class Leaf = SegLeaf>
+HM_D bool trace_any(const BvhView& bvh, const RayInv& r, TraceStack& st, TraceCounters& cnt,
+                    const Leaf& leaf_test = Leaf{}) {
   AnyState s;
   any_begin(bvh, st, s);
-  while (s.cur != kNone) any_step<kCount>(bvh, r, st, s, cnt);
+  while (s.cur != kNone) any_step<kCount, Leaf>(bvh, r, st, s, cnt, leaf_test);
   return s.hit;
 }
 

@@ -44,6 +44,11 @@ struct Context {
   float* seg_dev = nullptr;  // 8 floats per segment, original order
   float* strand_sigma_dev = nullptr;
   DeviceBvh bvh;
+  // OBJ head mesh (surface.geometry)
+  TriMesh mesh;
+  float4* mesh_tri = nullptr;    // 3 per triangle
+  float4* mesh_shade = nullptr;  // 4 per triangle
+  DeviceBvh mesh_bvh;
   float scene_scale = 1.f;
   V3 bmin{0, 0, 0}, bmax{0, 0, 0};
 
@@ -98,6 +103,8 @@ struct Context {
   int pb_slots = 0;
   void free_path_buffers();
   PassConfig pt_pass_config() const;
+  // global stack-spill levels beyond the LDS part for the deepest tree traversed
+  int stack_levels() const;
 };
 
 }  // namespace hm

@@ -11,6 +11,7 @@
 // each prefix sum is a block-parallel scan in FP64, rounded to FP32 once.
 #include <hip/hip_runtime.h>
 
+#include "hm_env.h"
 #include "hm_internal.h"
 
 namespace hm {
@@ -109,6 +110,29 @@ __global__ void k_env_marginal(int W, int H, const float* c_pdf, float* m_pdf, f
 
 }  // namespace
 
+// Guide entries of `rows` CDFs (row r at cdf + r * stride, count entries
+// searched): guide[r][k] = lower_bound(cdf_r, count, k / kEnvGuideBins)
+// before the max(0, . - 1) step, for k = 0 .. kEnvGuideBins.
+__global__ void k_env_guide(const float* cdf, int stride, int count, int rows, int* guide) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (k > kEnvGuideBins || r >= rows) return;
+  const float u = (float)k / (float)kEnvGuideBins;  // exact: power-of-two bins
+  const float* c = cdf + (size_t)r * stride;
+  int first = 0, n = count;
+  while (n > 0) {
+    const int step = n >> 1;
+    const int mid = first + step;
+    if (c[mid] < u) {
+      first = mid + 1;
+      n -= step + 1;
+    } else {
+      n = step;
+    }
+  }
+  guide[(size_t)r * (kEnvGuideBins + 1) + k] = first;
+}
+
 int env_build_tables(DeviceEnv& env, hipStream_t stream, std::string& err) {
   const int W = env.width, H = env.height;
   if (W <= 0 || H <= 0 || !env.tex) {
@@ -129,6 +153,15 @@ int env_build_tables(DeviceEnv& env, hipStream_t stream, std::string& err) {
   hipLaunchKernelGGL(k_env_marginal, dim3(1), dim3(kEnvBlock), 0, stream, W, H, env.c_pdf, env.m_pdf,
                      env.m_cdf);
   HM_TRY(check_launch("env marginal", err));
+  dev_free(env.m_guide);
+  dev_free(env.c_guide);
+  env.m_guide = env.c_guide = nullptr;
+  HM_TRY(dev_alloc((void**)&env.m_guide, sizeof(int) * (kEnvGuideBins + 1), err));
+  HM_TRY(dev_alloc((void**)&env.c_guide, sizeof(int) * (size_t)(kEnvGuideBins + 1) * H, err));
+  const int gb = (kEnvGuideBins + 1 + 255) / 256;
+  hipLaunchKernelGGL(k_env_guide, dim3(gb, 1), dim3(256), 0, stream, env.m_cdf, 0, H, 1, env.m_guide);
+  hipLaunchKernelGGL(k_env_guide, dim3(gb, H), dim3(256), 0, stream, env.c_cdf, W + 1, W, H, env.c_guide);
+  HM_TRY(check_launch("env guide tables", err));
   return kOk;
 }
 

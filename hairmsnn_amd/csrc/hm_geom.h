@@ -83,6 +83,33 @@ HM_HD bool intersect_segment(float ox, float oy, float oz, float dx, float dy, f
 }
 
 // Ray / sphere (head proxy). Nearest root with t > 0.
+// Ray / triangle (Moller-Trumbore, double-sided): nearest t > 0 with
+// barycentrics (u, v) of vertices 1 and 2, in fixed FP32 operations like the
+// segment test.
+HM_HD bool intersect_triangle(float ox, float oy, float oz, float dx, float dy, float dz, float ax, float ay,
+                              float az, float bx, float by, float bz, float cx, float cy, float cz, float* t_out,
+                              float* u_out, float* v_out) {
+  HM_NOCONTRACT
+  const float e1x = bx - ax, e1y = by - ay, e1z = bz - az;
+  const float e2x = cx - ax, e2y = cy - ay, e2z = cz - az;
+  const float px = dy * e2z - dz * e2y, py = dz * e2x - dx * e2z, pz = dx * e2y - dy * e2x;
+  const float det = e1x * px + e1y * py + e1z * pz;
+  if (!(det != 0.f)) return false;
+  const float inv = 1.f / det;
+  const float sx = ox - ax, sy = oy - ay, sz = oz - az;
+  const float u = (sx * px + sy * py + sz * pz) * inv;
+  if (!(u >= 0.f && u <= 1.f)) return false;
+  const float qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
+  const float v = (dx * qx + dy * qy + dz * qz) * inv;
+  if (!(v >= 0.f && u + v <= 1.f)) return false;
+  const float t = (e2x * qx + e2y * qy + e2z * qz) * inv;
+  if (!(t > 0.f)) return false;
+  *t_out = t;
+  *u_out = u;
+  *v_out = v;
+  return true;
+}
+
 HM_HD bool intersect_sphere(V3 o, V3 d, V3 c, float radius, float* t_out) {
   HM_NOCONTRACT
   const float dd = d.x * d.x + d.y * d.y + d.z * d.z;

@@ -105,14 +105,19 @@ struct DeviceEnv {
   float* c_cdf = nullptr;
   float* m_pdf = nullptr;  // height+1
   float* m_cdf = nullptr;
+  int* m_guide = nullptr;  // kEnvGuideBins + 1
+  int* c_guide = nullptr;  // (kEnvGuideBins + 1) * height
   void release() {
     dev_free(tex);
     dev_free(c_pdf);
     dev_free(c_cdf);
     dev_free(m_pdf);
     dev_free(m_cdf);
+    dev_free(m_guide);
+    dev_free(c_guide);
     tex = nullptr;
     c_pdf = c_cdf = m_pdf = m_cdf = nullptr;
+    m_guide = c_guide = nullptr;
     width = height = 0;
   }
   ~DeviceEnv() { release(); }

@@ -297,7 +297,7 @@ int msnn_setup(Context& c, int BETA, std::string& err) {
   const int prec = s.msnn_precision == "bf16" ? HMSNN_BF16 : s.msnn_precision == "fp8" ? HMSNN_FP8 : HMSNN_FP16;
   HM_TRY(mlp_create(prec, s.msnn_seed, &m->mlp, err));
   m->ctx_stream = c.stream;
-  const int spill_levels = std::max(1, c.bvh.stack_bound() - kStackLds + 1);
+  const int spill_levels = c.stack_levels();
   // a warm-up pass holds warm_batch training batches; the slots' path buffers
   // hold a frame or one such pass
   m->warm_batch = std::max(1, std::min(MsnnState::kWarmBatch, std::max(npx, m->n_train) / m->n_train));

@@ -24,12 +24,14 @@ namespace hm {
 constexpr int kMaxDirLights = 8;
 constexpr int kPrimMiss = -1;
 constexpr int kPrimSphere = -2;
+constexpr int kPrimMesh0 = -3;  // triangle k of the head mesh is prim kPrimMesh0 - k
 
 struct SceneView {
   BvhView bvh;
   const float* seg_by_id;     // 8 floats per segment, original order
   const float* strand_sigma;  // per-strand sigma_a (3 floats) or null
   SphereView sphere;
+  MeshView mesh;
   V3 sphere_albedo;
   float sphere_alpha;  // clamped to [0.01, 1] and squared (optix_common.cuh:551-552)
   EnvView env;
@@ -124,6 +126,28 @@ HM_D void setup_hit(const SceneView& sc, V3 o, V3 d, float t, int prim, float u,
     h.surface = 1;
     h.radius = 0.f;
     h.hw = HairWo{};
+  } else if (prim <= kPrimMesh0) {
+    // triangleMeshCH (optix_common.cuh:527-556): barycentric position and
+    // normal, frame from the normal, the material's diffuse colour, alpha 1
+    // (mesh->alpha = 1, model.cpp:323) clamped and squared
+    const int k = kPrimMesh0 - prim;
+    const float4 a = sc.mesh.tri[3 * k], b = sc.mesh.tri[3 * k + 1], c = sc.mesh.tri[3 * k + 2];
+    float tt = 0.f, bu = 0.f, bv = 0.f;
+    (void)intersect_triangle(o.x, o.y, o.z, d.x, d.y, d.z, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, &tt, &bu,
+                             &bv);
+    const float bw = 1.f - bu - bv;
+    h.p = v3(bw * a.x + bu * b.x + bv * c.x, bw * a.y + bu * b.y + bv * c.y, bw * a.z + bu * b.z + bv * c.z);
+    const float4 n0 = sc.mesh.shade[4 * k], n1 = sc.mesh.shade[4 * k + 1], n2 = sc.mesh.shade[4 * k + 2];
+    const float4 kd = sc.mesh.shade[4 * k + 3];
+    h.n = normalize(v3(bw * n0.x + bu * n1.x + bv * n2.x, bw * n0.y + bu * n1.y + bv * n2.y,
+                       bw * n0.z + bu * n1.z + bv * n2.z));
+    h.fr = basis_from_normal(h.n);
+    h.wo_local = normalize(to_local(h.fr, h.wo));
+    h.surface = 1;
+    h.radius = 0.f;
+    h.hw = HairWo{};
+    h.albedo = v3(kd.x, kd.y, kd.z);
+    h.alpha = 1.f;
   } else {
     const float* s = sc.seg_by_id + 8 * (size_t)prim;
     const V3 p0 = v3(s[0], s[1], s[2]), p1 = v3(s[4], s[5], s[6]);
@@ -292,10 +316,16 @@ HM_D void closest_hit(const SceneView& sc, V3 o, V3 d, TraceStack& st, float& t,
       sphere_hit = true;
     }
   }
-  int id = 0x7fffffff;
   const RayInv r = make_ray(o.x, o.y, o.z, d.x, d.y, d.z);
+  int tri = 0x7fffffff;
+  if (sc.mesh.num > 0) {
+    float bu = 0.f;
+    trace_closest<kCount, TriLeaf>(sc.mesh.bvh, r, st, t, tri, bu, cnt, TriLeaf{sc.mesh.tri});
+  }
+  int id = 0x7fffffff;
   trace_closest<kCount>(sc.bvh, r, st, t, id, u, cnt);
   if (id != 0x7fffffff) prim = id;
+  else if (tri != 0x7fffffff) prim = kPrimMesh0 - tri;
   else if (sphere_hit) prim = kPrimSphere;
 }
 
@@ -306,6 +336,7 @@ HM_D bool occluded(const SceneView& sc, V3 o, V3 d, TraceStack& st, TraceCounter
     if (intersect_sphere(o, d, v3(sc.sphere.cx, sc.sphere.cy, sc.sphere.cz), sc.sphere.r, &ts)) return true;
   }
   const RayInv r = make_ray(o.x, o.y, o.z, d.x, d.y, d.z);
+  if (sc.mesh.num > 0 && trace_any<kCount, TriLeaf>(sc.mesh.bvh, r, st, cnt, TriLeaf{sc.mesh.tri})) return true;
   return trace_any<kCount>(sc.bvh, r, st, cnt);
 }
 

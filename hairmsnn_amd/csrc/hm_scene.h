@@ -133,6 +133,17 @@ void generate_hair(const ProceduralHair& g, HairGeometry& out);
 // at the piece midpoint).
 int load_hair_file(const std::string& path, int tess, HairGeometry& out, std::string& err);
 
+// Triangle mesh of an OBJ head (hm_obj.cpp): per triangle its three
+// vertices, three vertex normals and the material's diffuse colour.
+struct TriMesh {
+  std::vector<float> v;   // 9 floats per triangle
+  std::vector<float> n;   // 9 floats per triangle
+  std::vector<float> kd;  // 3 floats per triangle
+  V3 bmin{1e30f, 1e30f, 1e30f}, bmax{-1e30f, -1e30f, -1e30f};
+  int num() const { return (int)(v.size() / 9); }
+};
+int load_obj(const std::string& path, TriMesh& out, std::string& err);
+
 struct EnvImage {
   int width = 0, height = 0;
   std::vector<float> rgba;  // width*height*4

@@ -114,6 +114,7 @@ typedef struct hmsnn_info {
   float camera[12];       /* pos, dir_00, dir_du, dir_dv (render_path_tracing.cu:767-799) */
   float dir_from[8][3];
   float dir_emit[8][3];
+  int num_triangles;      /* OBJ head mesh (surface.geometry), 0 without one */
 } hmsnn_info;
 int hmsnn_get_info(const hmsnn_ctx* ctx, hmsnn_info* info);
 /* Strings of the scene: which = 0 integrator.image_output, 1
@@ -139,6 +140,14 @@ int hmsnn_generate_host(const char* json_text, float* seg_host, size_t seg_cap, 
  * (env_width + 1) x env_height, marginal pdf/cdf of env_height + 1). Any
  * pointer may be NULL to skip that array. */
 int hmsnn_export_segments(const hmsnn_ctx* ctx, float* seg_host);
+/* The OBJ head mesh as loaded (loadOBJ, model.cpp:233-358): per triangle 3
+ * vertices (9 floats), 3 vertex normals (9 floats) and the diffuse colour (3
+ * floats); NULL skips an array. */
+int hmsnn_export_mesh(const hmsnn_ctx* ctx, float* vertices_host, float* normals_host, float* diffuse_host);
+/* Host-only OBJ reader of the same mesh layout (no GPU needed): call with
+ * NULL arrays for the count, then with arrays of `capacity` triangles. */
+int hmsnn_obj_load(const char* path, int* num_triangles, float* vertices_host, float* normals_host,
+                   float* diffuse_host, int capacity);
 int hmsnn_export_env(const hmsnn_ctx* ctx, float* rgba_host, float* c_pdf, float* c_cdf, float* m_pdf,
                      float* m_cdf);
 /* Per-strand absorption (num_strands x 3); NULL restores the global value. */

@@ -95,6 +95,10 @@ struct orc_scene {
   int frozen;
   float beta_m0, beta_n0, alpha0;
   const float* strand_sigma0;
+  const float* tri_v;
+  const float* tri_n;
+  const float* tri_kd;
+  int ntri;
 };
 
 namespace {
@@ -141,6 +145,10 @@ SceneDesc scene_desc(const orc_scene* sc, EnvTables& env) {
   s.beta_n0 = sc->beta_n0;
   s.alpha0 = sc->alpha0;
   s.strand_sigma0 = sc->strand_sigma0;
+  s.tri_v = sc->tri_v;
+  s.tri_n = sc->tri_n;
+  s.tri_kd = sc->tri_kd;
+  s.ntri = sc->ntri;
   return s;
 }
 }  // namespace
@@ -280,6 +288,20 @@ int orc_nrc_gbuffer(const orc_scene* sc, void* bvh_handle, int accum_id, const u
 }
 
 int orc_nrc_record_size(void) { return (int)sizeof(NrcPixelOut); }
+
+// Ray k against triangle k (9 floats each; rays as 8 floats o, pad, d, pad):
+// hit flag and (t, u, v) per pair.
+void orc_tri_intersect(int n, const float* tris, const float* rays, int* hit, float* t, float* u, float* v) {
+  for (int k = 0; k < n; ++k) {
+    const float* r = rays + 8 * (size_t)k;
+    const float o[3] = {r[0], r[1], r[2]}, d[3] = {r[4], r[5], r[6]};
+    float tt = 0.f, uu = 0.f, vv = 0.f;
+    hit[k] = tri_intersect(o, d, tris + 9 * (size_t)k, &tt, &uu, &vv) ? 1 : 0;
+    t[k] = tt;
+    u[k] = uu;
+    v[k] = vv;
+  }
+}
 
 // FP64 path-replay adjoint of the full image (orc_scene.h adjoint_render).
 int orc_adjoint(const orc_scene* sc, void* bvh_handle, int spp, const float* dl_dimg, int num_strands, int threads,

@@ -131,6 +131,27 @@ bool sphere_intersect(const float o[3], const float d[3], const float c[3], floa
   return false;
 }
 
+bool tri_intersect(const float o[3], const float d[3], const float* v, float* t_out, float* u_out, float* v_out) {
+  const float e1x = v[3] - v[0], e1y = v[4] - v[1], e1z = v[5] - v[2];
+  const float e2x = v[6] - v[0], e2y = v[7] - v[1], e2z = v[8] - v[2];
+  const float px = d[1] * e2z - d[2] * e2y, py = d[2] * e2x - d[0] * e2z, pz = d[0] * e2y - d[1] * e2x;
+  const float det = e1x * px + e1y * py + e1z * pz;
+  if (!(det != 0.f)) return false;
+  const float inv = 1.f / det;
+  const float sx = o[0] - v[0], sy = o[1] - v[1], sz = o[2] - v[2];
+  const float u = (sx * px + sy * py + sz * pz) * inv;
+  if (!(u >= 0.f && u <= 1.f)) return false;
+  const float qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
+  const float w = (d[0] * qx + d[1] * qy + d[2] * qz) * inv;
+  if (!(w >= 0.f && u + w <= 1.f)) return false;
+  const float t = (e2x * qx + e2y * qy + e2z * qz) * inv;
+  if (!(t > 0.f)) return false;
+  *t_out = t;
+  *u_out = u;
+  *v_out = w;
+  return true;
+}
+
 // ---------------------------------------------------------------------------
 // CPU BVH (median split, leaves of up to 4 segments). Boxes are conservative
 // (padded) and pruning keeps every candidate with entry distance <= best t,
@@ -415,10 +436,29 @@ void setup_hit(const Ctx<R>& c, Vec<R> o, Vec<R> d, float t, int prim, float u, 
   si.prim = prim;
   si.wo = (R)-1 * d;
   Vec<R> p = o + (R)t * d;
-  if (prim == -2) {  // head sphere (triangleMeshCH analogue, optix_common.cuh:527-556)
-    const Vec<R> cc = mk((R)s.sphere[0], (R)s.sphere[1], (R)s.sphere[2]);
-    si.n = nrm(p - cc);
-    si.p = cc + (R)s.sphere[3] * si.n;
+  if (prim == -2 || prim <= -3) {  // head sphere or mesh triangle (triangleMeshCH, optix_common.cuh:527-556)
+    if (prim == -2) {
+      const Vec<R> cc = mk((R)s.sphere[0], (R)s.sphere[1], (R)s.sphere[2]);
+      si.n = nrm(p - cc);
+      si.p = cc + (R)s.sphere[3] * si.n;
+      si.color = mk((R)s.sphere_albedo[0], (R)s.sphere_albedo[1], (R)s.sphere_albedo[2]);
+      si.alpha = (R)s.sphere_alpha;
+    } else {
+      // barycentric position and normal (barycentricInterpolate, utils.cuh:92-101)
+      const int k = -3 - prim;
+      const float* tv = s.tri_v + 9 * (size_t)k;
+      const float* tn = s.tri_n + 9 * (size_t)k;
+      const float of[3] = {(float)o.x, (float)o.y, (float)o.z}, df[3] = {(float)d.x, (float)d.y, (float)d.z};
+      float tt = 0.f, bu = 0.f, bv = 0.f;
+      (void)tri_intersect(of, df, tv, &tt, &bu, &bv);
+      const R w0 = (R)(1.f - bu - bv), w1 = (R)bu, w2 = (R)bv;
+      si.p = mk(w0 * (R)tv[0] + w1 * (R)tv[3] + w2 * (R)tv[6], w0 * (R)tv[1] + w1 * (R)tv[4] + w2 * (R)tv[7],
+                w0 * (R)tv[2] + w1 * (R)tv[5] + w2 * (R)tv[8]);
+      si.n = nrm(mk(w0 * (R)tn[0] + w1 * (R)tn[3] + w2 * (R)tn[6], w0 * (R)tn[1] + w1 * (R)tn[4] + w2 * (R)tn[7],
+                    w0 * (R)tn[2] + w1 * (R)tn[5] + w2 * (R)tn[8]));
+      si.color = mk((R)s.tri_kd[3 * k], (R)s.tri_kd[3 * k + 1], (R)s.tri_kd[3 * k + 2]);
+      si.alpha = (R)1;  // mesh->alpha = 1 (model.cpp:323), clamped to [0.01, 1] and squared
+    }
     const Vec<R> n = si.n;
     Vec<R> c1, c2;
     if (n.z < (R)-0.999999f) {  // orthonormalBasis, utils.cuh:232-259
@@ -434,9 +474,7 @@ void setup_hit(const Ctx<R>& c, Vec<R> o, Vec<R> d, float t, int prim, float u, 
     si.to_local[1] = c2;
     si.to_local[2] = n;
     si.wo_local = nrm(apply_mat(si.to_local, si.wo));
-    si.color = mk((R)s.sphere_albedo[0], (R)s.sphere_albedo[1], (R)s.sphere_albedo[2]);
     si.color0 = si.color;
-    si.alpha = (R)s.sphere_alpha;
     si.surface = true;
     si.radius = 0;
     return;
@@ -538,6 +576,10 @@ bool occluded(const Ctx<R>& c, Vec<R> o, Vec<R> d) {
     float t;
     if (sphere_intersect(of, df, c.s.sphere, c.s.sphere[3], &t)) return true;
   }
+  for (int k = 0; k < c.s.ntri; ++k) {
+    float t, u, v;
+    if (tri_intersect(of, df, c.s.tri_v + 9 * (size_t)k, &t, &u, &v)) return true;
+  }
   return c.bvh.any(of, df);
 }
 
@@ -552,6 +594,14 @@ void trace(const Ctx<R>& c, Vec<R> o, Vec<R> d, Inter<R>& si) {
     if (sphere_intersect(of, df, c.s.sphere, c.s.sphere[3], &t)) {
       best = t;
       prim = -2;
+    }
+  }
+  // mesh triangles: minimum (t, triangle) by brute force
+  for (int k = 0; k < c.s.ntri; ++k) {
+    float tt, uu, vv;
+    if (tri_intersect(of, df, c.s.tri_v + 9 * (size_t)k, &tt, &uu, &vv) && (tt < best || (tt == best && prim == -2))) {
+      best = tt;
+      prim = -3 - k;
     }
   }
   float th, uh;

@@ -36,6 +36,9 @@ void env_tables(EnvTables& e, int mode);
 // the root solved around the ray point nearest the segment centre.
 bool seg_intersect(const float o[3], const float d[3], const float* seg, float* t, float* u);
 bool sphere_intersect(const float o[3], const float d[3], const float c[3], float r, float* t);
+// Ray / triangle (Moller-Trumbore, double-sided), nearest t > 0 and the
+// barycentrics (u, v) of vertices 1 and 2; IEEE FP32 like seg_intersect.
+bool tri_intersect(const float o[3], const float d[3], const float* v, float* t, float* u, float* w);
 
 struct Bvh {
   struct Node {
@@ -85,6 +88,12 @@ struct SceneDesc {
   int frozen;
   float beta_m0, beta_n0, alpha0;
   const float* strand_sigma0;
+  // OBJ head mesh: per triangle 3 vertices (9 floats), 3 vertex normals (9)
+  // and the diffuse colour (3); ntri = 0 without a mesh
+  const float* tri_v;
+  const float* tri_n;
+  const float* tri_kd;
+  int ntri;
 };
 
 // mode 0: pathTrace; mode 1: msnnPathTrace truncated at kernel beta;

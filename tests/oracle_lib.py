@@ -30,6 +30,7 @@ class OrcScene(C.Structure):
         ("width", _I), ("height", _I), ("beta_m", _F), ("beta_n", _F), ("alpha", _F), ("gains", _F * 4),
         ("sigma_a", _F * 3), ("strand_sigma", _P), ("mis", _I), ("path_v1", _I), ("path_v2", _I),
         ("frozen", _I), ("beta_m0", _F), ("beta_n0", _F), ("alpha0", _F), ("strand_sigma0", _P),
+        ("tri_v", _P), ("tri_n", _P), ("tri_kd", _P), ("ntri", _I),
     ]
 
 
@@ -50,6 +51,7 @@ _SIGS = {
     "orc_adjoint": (_I, [C.POINTER(OrcScene), _P, _I, _P, _I, _I, _D, _P, _P, _P]),
     "orc_nrc_gbuffer": (_I, [C.POINTER(OrcScene), _P, _I, _P, _F, _F, _I, _I, _P]),
     "orc_nrc_record_size": (_I, []),
+    "orc_tri_intersect": (None, [_I, _P, _P, _P, _P, _P, _P]),
 }
 
 # orc_scene.h NrcPixelOut
@@ -134,7 +136,8 @@ def intersect(seg: np.ndarray, rays: np.ndarray, any_hit=False, use_bvh=True):
 class OracleScene:
     """Scene description for the oracle integrators, filled from a context's exports."""
 
-    def __init__(self, info: dict, seg: np.ndarray, env: dict | None, strand_sigma: np.ndarray | None = None):
+    def __init__(self, info: dict, seg: np.ndarray, env: dict | None, strand_sigma: np.ndarray | None = None,
+                 mesh: dict | None = None):
         self.lib = load_oracle()
         self.seg = np.ascontiguousarray(seg, np.float32)
         self.env = {k: np.ascontiguousarray(v, np.float32) for k, v in env.items()} if env else None
@@ -170,6 +173,10 @@ class OracleScene:
             s.gains[i] = float(info["gain"][i])
         s.strand_sigma = ptr(self.strand_sigma) if self.strand_sigma is not None else None
         s.mis, s.path_v1, s.path_v2 = int(info["mis"]), int(info["path_v1"]), int(info["path_v2"])
+        self.mesh = {k: np.ascontiguousarray(v, np.float32) for k, v in mesh.items()} if mesh else None
+        if self.mesh:
+            s.tri_v, s.tri_n, s.tri_kd = ptr(self.mesh["v"]), ptr(self.mesh["n"]), ptr(self.mesh["kd"])
+            s.ntri = self.mesh["v"].shape[0]
         self.s = s
         self.bvh = self.lib.orc_bvh_create(ptr(self.seg), self.seg.shape[0])
 

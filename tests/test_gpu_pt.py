@@ -20,7 +20,7 @@ def scene_cfg(name="c0_hairball.json", w=64, h=64, **integ):
 
 def oracle_for(sc):
     env = sc.env() if sc.info["has_env"] else None
-    return ol.OracleScene(sc.info, sc.segments(), env)
+    return ol.OracleScene(sc.info, sc.segments(), env, mesh=sc.mesh())
 
 
 def test_env_tables_match_oracle(gpu):
@@ -205,3 +205,27 @@ def test_hair_file_scene_matches_oracle(gpu, tmp_path):
     print(f".hair scene depth 1: per-pixel agreement {agree:.4f}, hair pixels {(hit & 1).mean():.3f}")
     assert (hit & 1).mean() > 0.05
     assert agree >= 0.97
+
+
+@pytest.mark.parametrize("path_v2,need", [(1, 0.97), (40, 0.75)])
+def test_obj_head_scene_matches_oracle(gpu, tmp_path, path_v2, need):
+    """An OBJ head (triangleMeshCH: barycentric position and normal, the
+    material's Kd, GGX with alpha 1) under the C0 hair: one pass against the
+    FP32 oracle (brute-force triangles) per pixel."""
+    from tests.objfile import icosphere, write_obj
+    P, N, F = icosphere(3, radius=0.9)
+    p = tmp_path / "head.obj"
+    write_obj(p, P, N, F, kd=(0.4, 0.3, 0.2))
+    cfg = scene_cfg("c0_hairball.json", 64, 64, path_v2=path_v2)
+    cfg["surface"] = {"geometry": str(p)}
+    with gpu.Scene(json_text=cfg) as sc:
+        assert int(sc.info["num_triangles"]) == len(F) and not sc.info["has_sphere"]
+        sc.pt_reset()
+        sc.pt_passes(1)
+        g = sc.last_pass()[..., :3].astype(np.float64)
+        orc = oracle_for(sc)
+    o, _, hit = orc.render(0, 0, 64, 64, spp=1, accum_id0=0, prec=1)
+    agree = (np.abs(g - o) <= 1e-3 * np.maximum(np.abs(o), 1e-2)).all(-1).mean()
+    print(f"OBJ head, depth {path_v2}: per-pixel agreement {agree:.4f}, head pixels {((hit & 2) > 0).mean():.3f}")
+    assert ((hit & 2) > 0).mean() > 0.05
+    assert agree >= need
