@@ -216,37 +216,63 @@ struct DirectOut {
   float lp1, bp1, lp2, bp2;
 };
 
-// directLighting with MIS (optix_common.cuh:205-309) up to the shadow tests.
-HM_D void direct_lighting(const SceneView& sc, const Hit& h, uint32_t& rng, DirectOut& out) {
-  out.has1 = out.has2 = 0;
-  out.c1 = out.c2 = v3(0.f);
-  V3 p = h.p;  // the push-through below is shared by both samples (reference quirk)
+// The light sample of sampleLights (optix_common.cuh:205-246): the light
+// pick and, for the environment, its importance sample. It uses only the
+// random stream, not the hit, so a shading kernel can take it before the hit
+// setup and overlap the table lookups with the hit's loads.
+struct LightSample {
+  V3 emit, wi;
+  float light_pdf;
+  bool is_delta;
+};
+
+HM_D LightSample sample_light(const SceneView& sc, uint32_t& rng) {
+  LightSample ls;
   const float inv_n = 1.f / (float)sc.num_total_lights;
-  // ---- light sample (sampleLights) ----
   const float rsel = draw(rng);
   int sel = (int)floorf(rsel * (float)sc.num_total_lights);
   // A draw of exactly 1.0 selects index N; without an environment light that
   // index does not exist (undefined in the reference), so it maps to the last
   // directional light.
   if (!sc.env.enabled && sel >= sc.num_dlights) sel = sc.num_dlights - 1;
-  bool is_delta = false;
-  V3 emit, wi;
-  float light_pdf = 1.f * inv_n;
+  ls.is_delta = false;
+  ls.light_pdf = 1.f * inv_n;
   if (sel >= sc.num_dlights) {
     const float ux = draw(rng), uy = draw(rng);
     float env_p;
-    emit = env_sample(sc.env, ux, uy, &wi, &env_p);
-    light_pdf = light_pdf * env_p;
+    ls.emit = env_sample(sc.env, ux, uy, &ls.wi, &env_p);
+    ls.light_pdf = ls.light_pdf * env_p;
   } else {
 #pragma unroll
     for (int k = 0; k < kMaxDirLights; ++k) {
       if (k == sel) {  // unrolled select: no dynamic indexing into the kernel argument
-        emit = sc.dl_emit[k];
-        wi = normalize(sc.dl_from[k]);
+        ls.emit = sc.dl_emit[k];
+        ls.wi = normalize(sc.dl_from[k]);
       }
     }
-    is_delta = true;
+    ls.is_delta = true;
   }
+  return ls;
+}
+
+HM_D void direct_lighting(const SceneView& sc, const Hit& h, const LightSample& ls, uint32_t& rng, DirectOut& out);
+
+// directLighting with MIS (optix_common.cuh:205-309) up to the shadow tests.
+HM_D void direct_lighting(const SceneView& sc, const Hit& h, uint32_t& rng, DirectOut& out) {
+  const LightSample ls = sample_light(sc, rng);
+  direct_lighting(sc, h, ls, rng, out);
+}
+
+// directLighting with MIS (optix_common.cuh:279-309) up to the shadow tests,
+// given the vertex's light sample (drawn first from the same stream).
+HM_D void direct_lighting(const SceneView& sc, const Hit& h, const LightSample& ls, uint32_t& rng, DirectOut& out) {
+  out.has1 = out.has2 = 0;
+  out.c1 = out.c2 = v3(0.f);
+  V3 p = h.p;  // the push-through below is shared by both samples (reference quirk)
+  const float inv_n = 1.f / (float)sc.num_total_lights;
+  const V3 emit = ls.emit, wi = ls.wi;
+  const float light_pdf = ls.light_pdf;
+  const bool is_delta = ls.is_delta;
   {
     const V3 wi_local = normalize(to_local(h.fr, wi));
     const bool lower = dot(wi, h.n) < 0.f;
