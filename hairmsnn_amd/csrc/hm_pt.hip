@@ -392,15 +392,10 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc,
           if (gb.enabled) gbuffer_write_miss(gb, slot, d);
         }
       } else {
-        // The vertex's light sample needs only the random stream: taken before
-        // the hit setup so that its table lookups overlap the hit's loads.
-        uint32_t rng = pb.rng[slot];
-        const bool lit = depth == 0 ? sp.dl_zero != 0 : depth >= sp.dl_from;
-        LightSample ls{};
-        if (lit) ls = sample_light(sc, rng);
         Hit h;
         setup_hit(sc, o, d, pb.hit_t[slot], prim, pb.hit_u[slot], h);
         if (depth == 0 && gb.enabled) gbuffer_write_hit(gb, sc, slot, h, prim);
+        uint32_t rng = pb.rng[slot];
         float4 b4 = pb.beta[slot];
         V3 beta = v3(b4.x, b4.y, b4.z);
         V3 beta_s = v3(0.f);
@@ -409,9 +404,9 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc,
           beta_s = v3(s4.x, s4.y, s4.z);
         }
         // ---- direct lighting at this vertex ----
-        if (lit) {
+        if (depth == 0 ? sp.dl_zero != 0 : depth >= sp.dl_from) {
           DirectOut dl;
-          direct_lighting(sc, h, ls, rng, dl);
+          direct_lighting(sc, h, rng, dl);
           if (dl.has1 || dl.has2) {
             shadow = true;
             ray1 = dl.has1 != 0;
