@@ -298,7 +298,8 @@ int Context::init(std::string& err) {
     float* boxes_dev = nullptr;
     HM_TRY(dev_alloc((void**)&boxes_dev, sizeof(float) * boxes.size(), err));
     HM_HIP(hipMemcpy(boxes_dev, boxes.data(), sizeof(float) * boxes.size(), hipMemcpyHostToDevice), err);
-    const int rc = bvh_build(boxes_dev, nt, mesh_bvh, stream, err);
+    const int rc = bvh_build(boxes_dev, nt, mesh_bvh, stream, err, scene.bvh_builder == "ploc" ? kBvhPloc : kBvhLbvh,
+                             scene.ploc_radius);
     (void)hipStreamSynchronize(stream);
     dev_free(boxes_dev);
     HM_TRY(rc);
@@ -312,7 +313,8 @@ int Context::init(std::string& err) {
     const size_t bytes = sizeof(float) * hair.seg.size();
     HM_TRY(dev_alloc((void**)&seg_dev, bytes, err));
     HM_HIP(hipMemcpyAsync(seg_dev, hair.seg.data(), bytes, hipMemcpyHostToDevice, stream), err);
-    HM_TRY(bvh_build(seg_dev, (int)(hair.seg.size() / 8), bvh, stream, err));
+    HM_TRY(bvh_build(seg_dev, (int)(hair.seg.size() / 8), bvh, stream, err,
+                     scene.bvh_builder == "ploc" ? kBvhPloc : kBvhLbvh, scene.ploc_radius));
   }
   scene_scale = sscale > 0.f ? sscale : 1.f;
 

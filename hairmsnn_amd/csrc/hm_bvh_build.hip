@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "hm_bvh.h"
 #include "hm_internal.h"
@@ -249,6 +250,174 @@ __device__ inline float half_area(float lx, float ly, float lz, float hx, float 
   return dx * dy + dy * dz + dz * dx;
 }
 
+// ---- PLOC: parallel locally-ordered clustering (Meister and Bittner, IEEE
+// TVCG 2018). Starting from the leaves in Morton order, every cluster finds
+// the neighbour within +-radius positions whose merged box has the smallest
+// surface area; mutual nearest neighbours merge into a new inner node, and
+// the surviving clusters are compacted in order. The order (area, index) is
+// total, so the pair that is smallest under it is always mutual and each
+// round merges at least one pair. Inner nodes are numbered from n - 2 down,
+// so the last merge, the root, is node 0. The tree replaces the Karras radix
+// tree (child, node_box, parent links); emission and collapse are shared.
+
+constexpr int kPlocBlock = 256;
+constexpr int kPlocMaxRadius = 64;
+
+__global__ void k_ploc_init(BuildBuffers b, int* cid, float* cbox) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.n) return;
+  cid[i] = ~i;
+  const int orig = b.vals_sorted[i];
+  for (int k = 0; k < 6; ++k) cbox[6 * (size_t)i + k] = b.leaf_box[6 * (size_t)orig + k];
+}
+
+// Nearest neighbour of every cluster among positions [i - R, i + R], boxes
+// staged in LDS per block with an R-wide halo on both sides.
+__global__ void __launch_bounds__(kPlocBlock) k_ploc_nn(const float* cbox, int m, int R, int* nn) {
+  __shared__ float tile[(kPlocBlock + 2 * kPlocMaxRadius) * 6];
+  const int base = blockIdx.x * kPlocBlock - R;
+  const int span = kPlocBlock + 2 * R;
+  for (int t = threadIdx.x; t < span; t += blockDim.x) {
+    const int g = base + t;
+    for (int k = 0; k < 6; ++k) tile[6 * t + k] = (g >= 0 && g < m) ? cbox[6 * (size_t)g + k] : 0.f;
+  }
+  __syncthreads();
+  const int i = blockIdx.x * kPlocBlock + threadIdx.x;
+  if (i >= m) return;
+  const float* bi = &tile[6 * (threadIdx.x + R)];
+  float best = __builtin_inff();
+  int bj = -1;
+  const int j0 = max(0, i - R), j1 = min(m - 1, i + R);
+  for (int j = j0; j <= j1; ++j) {
+    if (j == i) continue;
+    const float* bj_box = &tile[6 * (j - base)];
+    const float a = half_area(fminf(bi[0], bj_box[0]), fminf(bi[1], bj_box[1]), fminf(bi[2], bj_box[2]),
+                              fmaxf(bi[3], bj_box[3]), fmaxf(bi[4], bj_box[4]), fmaxf(bi[5], bj_box[5]));
+    if (a < best || (a == best && j < bj)) {  // ties: the lower index (a total order)
+      best = a;
+      bj = j;
+    }
+  }
+  nn[i] = bj;
+}
+
+__global__ void k_ploc_merge(BuildBuffers b, int m, const int* nn, int* cid, float* cbox, int* valid,
+                             int* next_free) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int j = nn[i];
+  const bool mutual = j >= 0 && nn[j] == i;
+  if (mutual && i < j) {
+    const int node = atomicSub(next_free, 1);
+    if (node < 0) {  // more merges than inner nodes: impossible for a valid run; stay in bounds
+      valid[i] = 1;
+      return;
+    }
+    const int ci = cid[i], cj = cid[j];
+    float u[6];
+    for (int k = 0; k < 3; ++k) {
+      u[k] = fminf(cbox[6 * (size_t)i + k], cbox[6 * (size_t)j + k]);
+      u[3 + k] = fmaxf(cbox[6 * (size_t)i + 3 + k], cbox[6 * (size_t)j + 3 + k]);
+    }
+    b.child[2 * node] = ci;
+    b.child[2 * node + 1] = cj;
+    for (int k = 0; k < 6; ++k) b.node_box[6 * (size_t)node + k] = u[k];
+    if (ci >= 0) b.parent_internal[ci] = node;
+    else b.parent_leaf[~ci] = node;
+    if (cj >= 0) b.parent_internal[cj] = node;
+    else b.parent_leaf[~cj] = node;
+    // the merged cluster takes position i (only this thread touches i and j)
+    cid[i] = node;
+    for (int k = 0; k < 6; ++k) cbox[6 * (size_t)i + k] = u[k];
+    valid[i] = 1;
+  } else {
+    valid[i] = (mutual && i > j) ? 0 : 1;
+  }
+}
+
+__global__ void k_ploc_compact(int m, const int* valid, const int* pos, const int* cid, const float* cbox, int* cid2,
+                               float* cbox2, int* count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  if (valid[i]) {
+    const int p = pos[i];
+    cid2[p] = cid[i];
+    for (int k = 0; k < 6; ++k) cbox2[6 * (size_t)p + k] = cbox[6 * (size_t)i + k];
+  }
+  if (i == m - 1) *count = pos[i] + valid[i];
+}
+
+__global__ void k_ploc_root(BuildBuffers b) { b.parent_internal[0] = -1; }
+
+// Builds the binary tree of b (leaf boxes, Morton order in vals_sorted) by
+// PLOC with search radius R.
+static int ploc_build(BuildBuffers& b, DeviceArena& arena, int R, hipStream_t stream, std::string& err) {
+  const int n = b.n;
+  R = std::max(1, std::min(R, kPlocMaxRadius));
+  int *cid[2], *nn, *valid, *pos, *dev_cnt, *next_free;
+  float* cbox[2];
+  for (int k = 0; k < 2; ++k) {
+    HM_TRY(arena.alloc((void**)&cid[k], sizeof(int) * (size_t)n, err));
+    HM_TRY(arena.alloc((void**)&cbox[k], sizeof(float) * 6 * (size_t)n, err));
+  }
+  HM_TRY(arena.alloc((void**)&nn, sizeof(int) * (size_t)n, err));
+  HM_TRY(arena.alloc((void**)&valid, sizeof(int) * (size_t)n, err));
+  HM_TRY(arena.alloc((void**)&pos, sizeof(int) * (size_t)n, err));
+  HM_TRY(arena.alloc((void**)&dev_cnt, sizeof(int) * 2, err));
+  next_free = dev_cnt + 1;
+  const int top = n - 2;
+  HM_HIP(hipMemcpyAsync(next_free, &top, sizeof(int), hipMemcpyHostToDevice, stream), err);
+  size_t scan_bytes = 0;
+  HM_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, valid, pos, 0, (size_t)n, rocprim::plus<int>(), stream), err);
+  void* scan_tmp = nullptr;
+  HM_TRY(arena.alloc(&scan_tmp, scan_bytes, err));
+  const int gs = (n + kPlocBlock - 1) / kPlocBlock;
+  hipLaunchKernelGGL(k_ploc_init, dim3(gs), dim3(kPlocBlock), 0, stream, b, cid[0], cbox[0]);
+  HM_TRY(check_launch("ploc init", err));
+  int m = n, cur = 0;
+  int* cnt_host = nullptr;
+  HM_HIP(hipHostMalloc((void**)&cnt_host, sizeof(int)), err);
+  int rc = kOk;
+  for (int round = 0; m > 1; ++round) {
+    if (round > n) {
+      err = "PLOC did not converge";
+      rc = kEDevice;
+      break;
+    }
+    const int g = (m + kPlocBlock - 1) / kPlocBlock;
+    hipLaunchKernelGGL(k_ploc_nn, dim3(g), dim3(kPlocBlock), 0, stream, cbox[cur], m, R, nn);
+    hipLaunchKernelGGL(k_ploc_merge, dim3(g), dim3(kPlocBlock), 0, stream, b, m, nn, cid[cur], cbox[cur], valid,
+                       next_free);
+    if (hipSuccess != rocprim::exclusive_scan(scan_tmp, scan_bytes, valid, pos, 0, (size_t)m, rocprim::plus<int>(),
+                                              stream)) {
+      err = "PLOC scan failed";
+      rc = kEDevice;
+      break;
+    }
+    hipLaunchKernelGGL(k_ploc_compact, dim3(g), dim3(kPlocBlock), 0, stream, m, valid, pos, cid[cur], cbox[cur],
+                       cid[cur ^ 1], cbox[cur ^ 1], dev_cnt);
+    rc = check_launch("ploc round", err);
+    if (rc) break;
+    if (hipMemcpyAsync(cnt_host, dev_cnt, sizeof(int), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess) {
+      err = "PLOC count read-back failed";
+      rc = kEDevice;
+      break;
+    }
+    if (*cnt_host >= m) {
+      err = "PLOC made no merge in a round";
+      rc = kEDevice;
+      break;
+    }
+    m = *cnt_host;
+    cur ^= 1;
+  }
+  (void)hipHostFree(cnt_host);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_ploc_root, dim3(1), dim3(1), 0, stream, b);
+  return check_launch("ploc root", err);
+}
+
 // One wide node per frontier entry: start from the binary node's two
 // children and open the inner child of largest surface area until there are
 // kBvhWidth (or only leaves remain).
@@ -350,7 +519,8 @@ static int bvh_collapse(DeviceBvh& out, hipStream_t stream, std::string& err) {
 
 // Builds the BVH from device segments (8 floats each). All scratch is
 // released before returning; the result owns nodes and segment records.
-int bvh_build(const float* seg_dev, int n, DeviceBvh& out, hipStream_t stream, std::string& err) {
+int bvh_build(const float* seg_dev, int n, DeviceBvh& out, hipStream_t stream, std::string& err, int builder,
+              int ploc_radius) {
   out.release();
   out.num_segs = n;
   if (n <= 0) {
@@ -401,10 +571,14 @@ int bvh_build(const float* seg_dev, int n, DeviceBvh& out, hipStream_t stream, s
   HM_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, b.keys, b.keys_sorted, b.vals, b.vals_sorted, n, 0, 30,
                                    stream),
          err);
-  hipLaunchKernelGGL(k_karras, dim3(gs), dim3(bs), 0, stream, b);
-  HM_TRY(check_launch("bvh karras", err));
-  hipLaunchKernelGGL(k_refit, dim3(gs), dim3(bs), 0, stream, b);
-  HM_TRY(check_launch("bvh refit", err));
+  if (builder == kBvhPloc) {
+    HM_TRY(ploc_build(b, arena, ploc_radius, stream, err));
+  } else {
+    hipLaunchKernelGGL(k_karras, dim3(gs), dim3(bs), 0, stream, b);
+    HM_TRY(check_launch("bvh karras", err));
+    hipLaunchKernelGGL(k_refit, dim3(gs), dim3(bs), 0, stream, b);
+    HM_TRY(check_launch("bvh refit", err));
+  }
   hipLaunchKernelGGL(k_emit, dim3(gs), dim3(bs), 0, stream, b, out.nodes, out.segs);
   HM_TRY(check_launch("bvh emit", err));
   hipLaunchKernelGGL(k_depth, dim3(gs), dim3(bs), 0, stream, b);

@@ -94,7 +94,11 @@ struct DeviceBvh {
   ~DeviceBvh() { release(); }
 };
 
-int bvh_build(const float* seg_dev, int n, DeviceBvh& out, hipStream_t stream, std::string& err);
+// Binary tree builders, collapsed into the same wide layout: the Karras
+// radix tree over Morton codes, or PLOC clustering in Morton order.
+constexpr int kBvhLbvh = 0, kBvhPloc = 1;
+int bvh_build(const float* seg_dev, int n, DeviceBvh& out, hipStream_t stream, std::string& err,
+              int builder = kBvhLbvh, int ploc_radius = 16);
 
 // Environment map and its importance-sampling tables (RGBA32F map, FP32
 // tables produced from FP64 prefix sums).

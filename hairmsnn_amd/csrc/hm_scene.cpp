@@ -278,6 +278,16 @@ bool parse_json(const Json& cfg, Scene& scene, std::string& err) {
     if (m.has("precision")) scene.msnn_precision = m.at("precision").str();
     if (m.has("seed")) scene.msnn_seed = (uint64_t)m.at("seed").num();
   }
+  // ---- acceleration structure extension ----
+  if (cfg.has("accel") && cfg.at("accel").is_object()) {
+    const Json& a = cfg.at("accel");
+    if (a.has("builder")) scene.bvh_builder = a.at("builder").str();
+    if (a.has("ploc_radius")) scene.ploc_radius = a.at("ploc_radius").i();
+    if (scene.bvh_builder != "lbvh" && scene.bvh_builder != "ploc") {
+      err = "accel.builder must be \"lbvh\" or \"ploc\"";
+      return false;
+    }
+  }
   // ---- neural radiance caching extension ----
   if (cfg.has("nrc") && cfg.at("nrc").is_object()) {
     const Json& m = cfg.at("nrc");

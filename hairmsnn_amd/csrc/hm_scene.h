@@ -106,6 +106,11 @@ struct Scene {
   std::string msnn_precision = "fp16";
   uint64_t msnn_seed = 7;
 
+  // acceleration structure extension block: "lbvh" (Karras radix tree) or
+  // "ploc" (locally-ordered clustering) binary tree, collapsed 4-wide
+  std::string bvh_builder = "lbvh";
+  int ploc_radius = 16;
+
   // neural radiance caching extension block (render_nrc.h:112-134 defaults)
   float nrc_c = 0.01f;              // spread threshold multiplier
   bool nrc_all_unbiased = false;

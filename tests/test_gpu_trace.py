@@ -82,3 +82,26 @@ def test_empty_and_degenerate_rays(gpu):
         prim, t, u = sc.trace_rays(rays)
     seg = sc.segments() if False else None
     assert prim[0] == -1
+
+
+@pytest.mark.parametrize("name,radius", [("c0_hairball.json", 16), ("c1_curly.json", 8)])
+def test_ploc_tree_bit_exact(gpu, name, radius):
+    """The PLOC-built tree (accel.builder = "ploc") gives the same closest and
+    any hits as the CPU BVH; the tree shape changes only the work done."""
+    cfg = json.loads((SCENES / name).read_text())
+    cfg["integrator"].update({"width": 128, "height": 128})
+    cfg["accel"] = {"builder": "ploc", "ploc_radius": radius}
+    cfg.pop("surface", None)  # hair only: the CPU search below has no head sphere
+    with gpu.Scene(json_text=cfg) as sc:
+        seg = sc.segments()
+        rays = make_rays(seg, 60_000, 9)
+        prim, t, u = sc.trace_rays(rays)
+        occ = sc.trace_rays(rays, any_hit=True)
+        depth = int(sc.info["bvh_depth"])
+    prim_c, t_c, u_c = ol.intersect(seg, rays, use_bvh=True)
+    hit = prim_c >= 0
+    print(f"{name} PLOC r={radius}: wide depth {depth}, hit fraction {hit.mean():.3f}")
+    np.testing.assert_array_equal(prim, prim_c)
+    np.testing.assert_array_equal(t[hit], t_c[hit])
+    occ_c = ol.intersect(seg, rays, any_hit=True, use_bvh=True)
+    np.testing.assert_array_equal(occ, occ_c)
