@@ -25,6 +25,42 @@
 
 namespace hm {
 
+// Per-frame state of one pipeline slot: path buffers, the G_BUFFER arrays
+// and training records of one frame, and the stream its G_BUFFER pass runs
+// on. The pass does not depend on the network, so consecutive frames' passes
+// run on consecutive slots and overlap; inference, RENDER and training run in
+// frame order on the context stream.
+struct NrcSlot {
+  PathBuffers pb{};
+  std::vector<void*> allocs;
+  hipStream_t st = nullptr;
+  int* train_idx = nullptr;
+  int* idx_host = nullptr;  // pinned staging of train_idx
+  float* query = nullptr;   // n_query x 12
+  float4* gb_beta = nullptr;
+  uint8_t* gb_hit = nullptr;
+  float4* state = nullptr;
+  float4* prev = nullptr;
+  float4* tb[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // pos, wo, n, rad, beta
+  int* tb_bounces = nullptr;
+  uint8_t* tb_hit = nullptr;
+  hipEvent_t ev_gb = nullptr;    // the slot's G_BUFFER pass is done
+  hipEvent_t ev_done = nullptr;  // the context stream has consumed the slot
+  hipEvent_t ev_copy = nullptr;  // idx_host has been read
+  bool used = false;
+  void release() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (void* p : allocs) dev_free(p);
+    allocs.clear();
+    if (idx_host) (void)hipHostFree(idx_host);
+    idx_host = nullptr;
+    for (hipEvent_t* e : {&ev_gb, &ev_done, &ev_copy})
+      if (*e) (void)hipEventDestroy(*e), *e = nullptr;
+    if (st) (void)hipStreamDestroy(st);
+    st = nullptr;
+  }
+};
+
 struct NrcState {
   MlpState* mlp = nullptr;
   int n_rec = 0;       // training records per step (numTrainingRecords)
@@ -40,28 +76,23 @@ struct NrcState {
   int frame = 0;
   uint64_t rng_state = 0;
   std::vector<int> idx;  // current permutation of the training indices
+  static constexpr int kSlots = 3;
+  NrcSlot slots[kSlots];
+  int next_slot = 0, last_slot = -1;
+  // shared, used in frame order on the context stream
   std::vector<void*> allocs;
-  int* idx_host = nullptr;  // pinned staging
-  int* train_idx = nullptr;
-  float* query = nullptr;
-  float* cache = nullptr;  // n_query x 3
-  float4* gb_beta = nullptr;
+  float* cache = nullptr;    // n_query x 3
   float4* gb_rad = nullptr;  // radiance before the query (RENDER pass input), per pixel
-  uint8_t* gb_hit = nullptr;
-  float4* state = nullptr;
-  float4* prev = nullptr;
-  float4* tb[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // pos, wo, n, rad, beta
-  int* tb_bounces = nullptr;
-  uint8_t* tb_hit = nullptr;
   float* train_in = nullptr;   // n_rec x 12
   float* train_tgt = nullptr;  // n_rec x 3
   float4* acc = nullptr;
   float4* avg = nullptr;
   int* counts_dev = nullptr;   // [0] frame cache queries, [1] training records with data
-  double loss_last = 0.0;
+  hipStream_t ctx_stream = nullptr;
   ~NrcState() {
+    if (ctx_stream) (void)hipStreamSynchronize(ctx_stream);
+    for (NrcSlot& sl : slots) sl.release();
     for (void* p : allocs) dev_free(p);
-    if (idx_host) (void)hipHostFree(idx_host);
     if (mlp) mlp_destroy(mlp);
   }
   uint64_t next_u64() {
@@ -157,7 +188,7 @@ __global__ void k_nrc_train_data(NrcPassArgs nr, const int* train_idx, const flo
 }
 
 struct NrcTimer {
-  hipEvent_t e[5];
+  hipEvent_t e[6];
   NrcTimer() {
     for (auto& x : e) (void)hipEventCreate(&x);
   }
@@ -200,37 +231,52 @@ int nrc_setup(Context& c, std::string& err) {
   m->rng_state = s.msnn_seed * 0x2545F4914F6CDD1Dull + 5;
   const int prec = s.msnn_precision == "bf16" ? HMSNN_BF16 : s.msnn_precision == "fp8" ? HMSNN_FP8 : HMSNN_FP16;
   HM_TRY(mlp_create(prec, s.msnn_seed, &m->mlp, err));
-  HM_TRY(c.ensure_path_buffers(true, err));
+  m->ctx_stream = c.stream;
   auto A = [&](void** p, size_t bytes) -> int {
     HM_TRY(dev_alloc(p, bytes, err));
     m->allocs.push_back(*p);
     return kOk;
   };
   const size_t nv = (size_t)m->n_tpx * kNrcMaxBounces;
-  HM_TRY(A((void**)&m->train_idx, sizeof(int) * (size_t)m->n_tpx));
-  HM_TRY(A((void**)&m->query, sizeof(float) * kNnInputCh * (size_t)m->n_query));
+  for (NrcSlot& sl : m->slots) {
+    HM_HIP(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking), err);
+    for (hipEvent_t* e : {&sl.ev_gb, &sl.ev_done, &sl.ev_copy})
+      HM_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming), err);
+    HM_TRY(alloc_path_buffers(sl.pb, sl.allocs, (size_t)npx, true, c.stack_levels(), c.trace_blocks * kTraceBlock,
+                              sl.st, err));
+    auto S = [&](void** p, size_t bytes) -> int {
+      HM_TRY(dev_alloc(p, bytes, err));
+      sl.allocs.push_back(*p);
+      return kOk;
+    };
+    HM_TRY(S((void**)&sl.train_idx, sizeof(int) * (size_t)m->n_tpx));
+    HM_TRY(S((void**)&sl.query, sizeof(float) * kNnInputCh * (size_t)m->n_query));
+    HM_TRY(S((void**)&sl.gb_beta, sizeof(float4) * (size_t)npx));
+    HM_TRY(S((void**)&sl.gb_hit, (size_t)npx));
+    HM_TRY(S((void**)&sl.state, sizeof(float4) * (size_t)npx));
+    HM_TRY(S((void**)&sl.prev, sizeof(float4) * (size_t)npx));
+    for (float4*& t : sl.tb) HM_TRY(S((void**)&t, sizeof(float4) * nv));
+    HM_TRY(S((void**)&sl.tb_bounces, sizeof(int) * (size_t)m->n_tpx));
+    HM_TRY(S((void**)&sl.tb_hit, (size_t)m->n_tpx));
+    HM_HIP(hipHostMalloc((void**)&sl.idx_host, sizeof(int) * (size_t)m->n_tpx), err);
+  }
   HM_TRY(A((void**)&m->cache, sizeof(float) * 3 * (size_t)m->n_query));
-  HM_TRY(A((void**)&m->gb_beta, sizeof(float4) * (size_t)npx));
   HM_TRY(A((void**)&m->gb_rad, sizeof(float4) * (size_t)npx));
-  HM_TRY(A((void**)&m->gb_hit, (size_t)npx));
-  HM_TRY(A((void**)&m->state, sizeof(float4) * (size_t)npx));
-  HM_TRY(A((void**)&m->prev, sizeof(float4) * (size_t)npx));
-  for (float4*& t : m->tb) HM_TRY(A((void**)&t, sizeof(float4) * nv));
-  HM_TRY(A((void**)&m->tb_bounces, sizeof(int) * (size_t)m->n_tpx));
-  HM_TRY(A((void**)&m->tb_hit, (size_t)m->n_tpx));
   HM_TRY(A((void**)&m->train_in, sizeof(float) * kNnInputCh * (size_t)m->n_rec));
   HM_TRY(A((void**)&m->train_tgt, sizeof(float) * 3 * (size_t)m->n_rec));
   HM_TRY(A((void**)&m->acc, sizeof(float4) * (size_t)npx));
   HM_TRY(A((void**)&m->avg, sizeof(float4) * (size_t)npx));
   HM_TRY(A((void**)&m->counts_dev, sizeof(int) * 4));
   HM_HIP(hipMemsetAsync(m->avg, 0, sizeof(float4) * (size_t)npx, c.stream), err);
-  HM_HIP(hipHostMalloc((void**)&m->idx_host, sizeof(int) * (size_t)m->n_tpx), err);
   m->idx.resize(m->n_tpx);
   for (int i = 0; i < m->n_tpx; ++i) m->idx[i] = i;  // thrust::sequence (render_nrc.cu:170-173)
+  // the slots' first passes start after the set-up work
+  HM_HIP(hipStreamSynchronize(c.stream), err);
+  for (NrcSlot& sl : m->slots) HM_HIP(hipStreamSynchronize(sl.st), err);
   return kOk;
 }
 
-NrcPassArgs nrc_args(const Context& c, const NrcState& m) {
+NrcPassArgs nrc_args(const Context& c, const NrcState& m, const NrcSlot& sl) {
   NrcPassArgs a{};
   a.enabled = 1;
   a.all_unbiased = m.all_unbiased;
@@ -239,72 +285,85 @@ NrcPassArgs nrc_args(const Context& c, const NrcState& m) {
   a.frame_size = m.frame_size;
   a.n_tpx = m.n_tpx;
   a.every_nth = m.every_nth;
-  a.query = m.query;
-  a.gb_beta = m.gb_beta;
-  a.gb_hit = m.gb_hit;
-  a.state = m.state;
-  a.prev = m.prev;
-  a.tb_pos = m.tb[0];
-  a.tb_wo = m.tb[1];
-  a.tb_n = m.tb[2];
-  a.tb_rad = m.tb[3];
-  a.tb_beta = m.tb[4];
-  a.tb_bounces = m.tb_bounces;
-  a.tb_hit = m.tb_hit;
+  a.query = sl.query;
+  a.gb_beta = sl.gb_beta;
+  a.gb_hit = sl.gb_hit;
+  a.state = sl.state;
+  a.prev = sl.prev;
+  a.tb_pos = sl.tb[0];
+  a.tb_wo = sl.tb[1];
+  a.tb_n = sl.tb[2];
+  a.tb_rad = sl.tb[3];
+  a.tb_beta = sl.tb[4];
+  a.tb_bounces = sl.tb_bounces;
+  a.tb_hit = sl.tb_hit;
   return a;
 }
 
-// One frame (render_nrc.cu:640-701). tm, when given, receives events around
-// G_BUFFER [0..1], inference [1..2], RENDER [2..3] and training [3..4].
+// One frame (render_nrc.cu:640-701): the shuffle and G_BUFFER pass on the
+// frame's slot, then inference, RENDER and training in frame order on the
+// context stream. tm, when given, receives events: [0], [1] around the
+// G_BUFFER pass on the slot's stream; [2] inference start, [3] RENDER start,
+// [4] training start and [5] end on the context stream.
 int nrc_frame(Context& c, NrcState& m, NrcTimer* tm, std::string& err) {
   const int npx = m.frame_size;
-  hipStream_t st = c.stream;
+  NrcSlot& sl = m.slots[m.next_slot];
+  m.last_slot = m.next_slot;
+  m.next_slot = (m.next_slot + 1) % NrcState::kSlots;
+  hipStream_t st = sl.st, cs = c.stream;
+  if (sl.used) HM_HIP(hipStreamWaitEvent(st, sl.ev_done, 0), err);
+  sl.used = true;
   if (tm) (void)hipEventRecord(tm->e[0], st);
   // ---- shuffle (render_nrc.cu:652-655): seeded, applied to the previous order ----
-  HM_HIP(hipStreamSynchronize(st), err);  // idx_host is free again
+  HM_HIP(hipEventSynchronize(sl.ev_copy), err);  // the slot's staging buffer is free again
   for (int i = m.n_tpx - 1; i > 0; --i) std::swap(m.idx[i], m.idx[(int)(m.next_u64() % (uint64_t)(i + 1))]);
-  std::copy(m.idx.begin(), m.idx.end(), m.idx_host);
-  HM_HIP(hipMemcpyAsync(m.train_idx, m.idx_host, sizeof(int) * (size_t)m.n_tpx, hipMemcpyHostToDevice, st), err);
+  std::copy(m.idx.begin(), m.idx.end(), sl.idx_host);
+  HM_HIP(hipMemcpyAsync(sl.train_idx, sl.idx_host, sizeof(int) * (size_t)m.n_tpx, hipMemcpyHostToDevice, st), err);
+  HM_HIP(hipEventRecord(sl.ev_copy, st), err);
   // ---- clear the frame's buffers (owlBufferClear + RESET, render_nrc.cu:684-692) ----
   const size_t nv = (size_t)m.n_tpx * kNrcMaxBounces;
-  HM_HIP(hipMemsetAsync(m.query, 0, sizeof(float) * kNnInputCh * (size_t)m.n_query, st), err);
-  HM_HIP(hipMemsetAsync(m.gb_beta, 0, sizeof(float4) * (size_t)npx, st), err);
-  HM_HIP(hipMemsetAsync(m.gb_hit, 0, (size_t)npx, st), err);
-  for (float4* t : m.tb) HM_HIP(hipMemsetAsync(t, 0, sizeof(float4) * nv, st), err);
-  HM_HIP(hipMemsetAsync(m.tb_bounces, 0, sizeof(int) * (size_t)m.n_tpx, st), err);
-  HM_HIP(hipMemsetAsync(m.tb_hit, 0, (size_t)m.n_tpx, st), err);
-  HM_HIP(hipMemsetAsync(m.train_in, 0, sizeof(float) * kNnInputCh * (size_t)m.n_rec, st), err);
-  HM_HIP(hipMemsetAsync(m.train_tgt, 0, sizeof(float) * 3 * (size_t)m.n_rec, st), err);
-  HM_HIP(hipMemsetAsync(m.counts_dev, 0, sizeof(int) * 4, st), err);
+  HM_HIP(hipMemsetAsync(sl.query, 0, sizeof(float) * kNnInputCh * (size_t)m.n_query, st), err);
+  HM_HIP(hipMemsetAsync(sl.gb_beta, 0, sizeof(float4) * (size_t)npx, st), err);
+  HM_HIP(hipMemsetAsync(sl.gb_hit, 0, (size_t)npx, st), err);
+  for (float4* t : sl.tb) HM_HIP(hipMemsetAsync(t, 0, sizeof(float4) * nv, st), err);
+  HM_HIP(hipMemsetAsync(sl.tb_bounces, 0, sizeof(int) * (size_t)m.n_tpx, st), err);
+  HM_HIP(hipMemsetAsync(sl.tb_hit, 0, (size_t)m.n_tpx, st), err);
   // ---- G_BUFFER ----
   PassConfig cfg = c.pt_pass_config();
   cfg.accum_id = m.frame;
   cfg.dl_zero = 1;  // nrcTracePaths lights every vertex (nrc.cu:166-169)
   cfg.dl_from = 1;
   cfg.max_vertex = kNrcMaxBounces - 1;
-  cfg.train_idx = m.train_idx;
+  cfg.train_idx = sl.train_idx;
   cfg.every_nth = m.every_nth;
-  cfg.nrc = nrc_args(c, m);
-  HM_TRY(pt_run_pass(c.view, c.pb, cfg, st, err));
+  cfg.nrc = nrc_args(c, m, sl);
+  HM_TRY(pt_run_pass(c.view, sl.pb, cfg, st, err));
   if (tm) (void)hipEventRecord(tm->e[1], st);
-  // ---- infer ----
-  HM_TRY(mlp_forward(*m.mlp, m.query, m.n_query, m.cache, st, err));
-  if (tm) (void)hipEventRecord(tm->e[2], st);
+  HM_HIP(hipEventRecord(sl.ev_gb, st), err);
+  HM_HIP(hipStreamWaitEvent(cs, sl.ev_gb, 0), err);
+  // ---- infer (context stream, frame order) ----
+  if (tm) (void)hipEventRecord(tm->e[2], cs);
+  HM_HIP(hipMemsetAsync(m.train_in, 0, sizeof(float) * kNnInputCh * (size_t)m.n_rec, cs), err);
+  HM_HIP(hipMemsetAsync(m.train_tgt, 0, sizeof(float) * 3 * (size_t)m.n_rec, cs), err);
+  HM_HIP(hipMemsetAsync(m.counts_dev, 0, sizeof(int) * 4, cs), err);
+  HM_TRY(mlp_forward(*m.mlp, sl.query, m.n_query, m.cache, cs, err));
+  if (tm) (void)hipEventRecord(tm->e[3], cs);
   // ---- RENDER ----
-  hipLaunchKernelGGL(k_nrc_render, dim3((npx + 255) / 256), dim3(256), 0, st, c.pb, cfg.nrc, m.cache, m.gb_rad,
+  hipLaunchKernelGGL(k_nrc_render, dim3((npx + 255) / 256), dim3(256), 0, cs, sl.pb, cfg.nrc, m.cache, m.gb_rad,
                      m.acc, m.avg, m.frame, m.counts_dev);
-  hipLaunchKernelGGL(k_nrc_train_data, dim3((m.n_tpx + 255) / 256), dim3(256), 0, st, cfg.nrc, m.train_idx,
+  hipLaunchKernelGGL(k_nrc_train_data, dim3((m.n_tpx + 255) / 256), dim3(256), 0, cs, cfg.nrc, sl.train_idx,
                      m.cache, m.train_in, m.train_tgt, m.n_rec, m.counts_dev);
   HM_TRY(check_launch("nrc render", err));
-  if (tm) (void)hipEventRecord(tm->e[3], st);
+  if (tm) (void)hipEventRecord(tm->e[4], cs);
   // ---- train ----
   if (c.scene.nrc_train) {
     const float lr = nrc_lr_at(m.lr, m.train_steps);
-    HM_TRY(mlp_loss_grad_step(*m.mlp, m.train_in, m.train_tgt, m.n_rec, m.loss, nullptr, nullptr, true, lr, st,
+    HM_TRY(mlp_loss_grad_step(*m.mlp, m.train_in, m.train_tgt, m.n_rec, m.loss, nullptr, nullptr, true, lr, cs,
                               err));
     m.train_steps++;
   }
-  if (tm) (void)hipEventRecord(tm->e[4], st);
+  if (tm) (void)hipEventRecord(tm->e[5], cs);
+  HM_HIP(hipEventRecord(sl.ev_done, cs), err);
   m.frame++;
   return kOk;
 }
@@ -356,9 +415,9 @@ int hmsnn_nrc_frames(hmsnn_ctx* ctx, int frames, hmsnn_nrc_stats* stats) {
     hmsnn_nrc_stats local{};
     for (NrcTimer& tm : timers) {
       local.gbuffer_ms += NrcTimer::ms(tm.e[0], tm.e[1]);
-      local.infer_ms += NrcTimer::ms(tm.e[1], tm.e[2]);
-      local.render_ms += NrcTimer::ms(tm.e[2], tm.e[3]);
-      local.train_ms += NrcTimer::ms(tm.e[3], tm.e[4]);
+      local.infer_ms += NrcTimer::ms(tm.e[2], tm.e[3]);
+      local.render_ms += NrcTimer::ms(tm.e[3], tm.e[4]);
+      local.train_ms += NrcTimer::ms(tm.e[4], tm.e[5]);
     }
     local.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     local.frames = frames;
@@ -399,19 +458,21 @@ int hmsnn_nrc_get_buffer(hmsnn_ctx* ctx, int which, void* host, size_t bytes) {
   const NrcState* m = ctx->c.nrc;
   if (!m) return fail(HMSNN_EINVAL, "call hmsnn_nrc_begin first");
   const size_t npx = (size_t)m->frame_size;
+  if (m->last_slot < 0) return fail(HMSNN_EINVAL, "hmsnn_nrc_get_buffer: no frame rendered yet");
+  const NrcSlot& sl = m->slots[m->last_slot];
   const void* src = nullptr;
   size_t need = 0;
   switch (which) {
-    case HMSNN_NRC_QUERY: src = m->query, need = sizeof(float) * kNnInputCh * (size_t)m->n_query; break;
+    case HMSNN_NRC_QUERY: src = sl.query, need = sizeof(float) * kNnInputCh * (size_t)m->n_query; break;
     case HMSNN_NRC_CACHE: src = m->cache, need = sizeof(float) * 3 * (size_t)m->n_query; break;
     case HMSNN_NRC_GB_RADIANCE: src = m->gb_rad, need = sizeof(float4) * npx; break;
-    case HMSNN_NRC_GB_BETA: src = m->gb_beta, need = sizeof(float4) * npx; break;
-    case HMSNN_NRC_GB_HIT: src = m->gb_hit, need = npx; break;
+    case HMSNN_NRC_GB_BETA: src = sl.gb_beta, need = sizeof(float4) * npx; break;
+    case HMSNN_NRC_GB_HIT: src = sl.gb_hit, need = npx; break;
     case HMSNN_NRC_TRAIN_INPUT: src = m->train_in, need = sizeof(float) * kNnInputCh * (size_t)m->n_rec; break;
     case HMSNN_NRC_TRAIN_TARGET: src = m->train_tgt, need = sizeof(float) * 3 * (size_t)m->n_rec; break;
-    case HMSNN_NRC_TRAIN_IDX: src = m->train_idx, need = sizeof(int) * (size_t)m->n_tpx; break;
-    case HMSNN_NRC_TRAIN_BOUNCES: src = m->tb_bounces, need = sizeof(int) * (size_t)m->n_tpx; break;
-    case HMSNN_NRC_TRAIN_HIT: src = m->tb_hit, need = (size_t)m->n_tpx; break;
+    case HMSNN_NRC_TRAIN_IDX: src = sl.train_idx, need = sizeof(int) * (size_t)m->n_tpx; break;
+    case HMSNN_NRC_TRAIN_BOUNCES: src = sl.tb_bounces, need = sizeof(int) * (size_t)m->n_tpx; break;
+    case HMSNN_NRC_TRAIN_HIT: src = sl.tb_hit, need = (size_t)m->n_tpx; break;
     default: return fail(HMSNN_EINVAL, "hmsnn_nrc_get_buffer: unknown buffer");
   }
   if (bytes != need) return fail(HMSNN_EINVAL, "hmsnn_nrc_get_buffer: size mismatch");
