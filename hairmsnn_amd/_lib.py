@@ -123,6 +123,7 @@ _SIGS = {
     "hmsnn_mlp_set_weights": (_I, [_P, _P, _I]),
     "hmsnn_mlp_encode_dev": (_I, [_P, _I, _P, _P]),
     "hmsnn_mlp_forward_dev": (_I, [_P, _P, _I, _P, _P]),
+    "hmsnn_mlp_get_fp8_scales": (_I, [_P, _P]),
     "hmsnn_mlp_backward_dev": (_I, [_P, _P, _P, _I, _I, _P, C.POINTER(_F), _P]),
     "hmsnn_mlp_train_step_dev": (_I, [_P, _P, _P, _I, _I, _F, C.POINTER(_F), _P]),
     "hmsnn_save_exr": (_I, [C.c_char_p, _P, _I, _I]),

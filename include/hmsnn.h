@@ -297,6 +297,12 @@ int hmsnn_mlp_set_weights(hmsnn_mlp* mlp, const float* w_host, int n);
 int hmsnn_mlp_encode_dev(const float* in_dev, int n, float* feat_dev, void* stream);
 /* Inference: in n x 12 FP32, out n x 3 FP32 (TINY_MLP::inference). */
 int hmsnn_mlp_forward_dev(hmsnn_mlp* mlp, const float* in_dev, int n, float* out_dev, void* stream);
+/* FP8 networks only (no reference counterpart: tiny-cuda-nn has no FP8 path):
+ * the 12 power-of-two exponents the last FP8 inference used, [0..5] per layer
+ * weights, [6..11] per layer input; a layer computes
+ * 2^(ew + ex) * sum(e4m3(W / 2^ew) * e4m3(x / 2^ex)). Synchronises the
+ * device. HMSNN_EINVAL before the first FP8 inference after a weight change. */
+int hmsnn_mlp_get_fp8_scales(const hmsnn_mlp* mlp, int* exponents12);
 /* Loss and gradient w.r.t. the weights (FP32, 20672) for one batch; does not
  * update the weights. */
 int hmsnn_mlp_backward_dev(hmsnn_mlp* mlp, const float* in_dev, const float* target_dev, int n, int loss,

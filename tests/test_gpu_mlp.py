@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from tests import oracle_lib as ol
-from tests.test_oracle_mlp import records, weights
+from tests.test_oracle_mlp import fp8_exponents, records, weights
 
 pytestmark = pytest.mark.gpu
 
@@ -95,8 +95,10 @@ def test_forward_matches_oracle(gpu, precision, tol):
 
 def test_fp8_inference_against_fp64_and_fp16(gpu):
     """FP8-E4M3 inference (per-layer power-of-two scales, BF16 calibration)
-    on 2^20 seeded inputs against the FP64 oracle, beside FP16 on the same
-    weights."""
+    on 2^20 seeded inputs: against the numpy E4M3 restatement run on the
+    kernel's own encoded features and exponents, then against the FP64 and
+    FP32 oracles and FP16 on the same weights (normwise relative error
+    <= 2e-2)."""
     n = 1 << 20
     r = records(n, 31)
     w = weights(32)
@@ -111,15 +113,49 @@ def test_fp8_inference_against_fp64_and_fp16(gpu):
             i = np.flatnonzero(bad)[:4]
             print(f"precision {prec}: {bad.sum()} non-finite outputs, e.g. rows {i}: {out[prec][i]} "
                   f"(FP64 {y64[i]}) inputs {r[i]}")
+        if prec == 2:
+            exps = (C.c_int * 12)()
+            gpu._lib.check(m.lib.hmsnn_mlp_get_fp8_scales(m.h, exps))
+            exps = list(exps)
     assert all(np.isfinite(out[p]).all() for p in out)
+    # the kernel against the restatement of the same quantisation
+    ns = 1 << 16
+    lib = gpu._lib.load()
+    di = gpu.DeviceBuffer.from_numpy(r[:ns])
+    df = gpu.DeviceBuffer(4 * 64 * ns)
+    gpu._lib.check(lib.hmsnn_mlp_encode_dev(di.ptr, ns, df.ptr, None))
+    gpu._lib.check(lib.hmsnn_dev_synchronize())
+    feat = df.to_numpy(np.float32, (ns, 64))
+    want = fp8_exponents(w, ol.mlp_encode(r[:ns]))
+    print(f"FP8 exponents {exps}, FP64 calibration gives {want}")
+    assert exps[:6] == want[:6]
+    assert all(abs(a - b) <= 1 for a, b in zip(exps[6:], want[6:]))
+    # E4M3 quantisation with exact sums: agrees except where a layer input
+    # sits at an E4M3 rounding midpoint that the instruction's truncated sums
+    # (next check) move to the other side
+    d_fmt = np.abs(out[2][:ns] - ol.mlp_forward_fp8(w, feat, exps)) / np.abs(out[2][:ns])
+    # the same with the matrix instruction's own accumulation restated
+    nm = 4096
+    y8m = ol.mlp_forward_fp8(w, feat[:nm], exps, mfma=True)
+    d = np.abs(out[2][:nm] - y8m) / np.abs(y8m)
+    print(f"FP8 kernel vs E4M3 restatement: exact sums: median {np.median(d_fmt):.2e}, above 1e-4 "
+          f"{(d_fmt > 1e-4).mean():.4f}; instruction's accumulation: max {d.max():.2e}, "
+          f"above 1e-6 {(d > 1e-6).mean():.5f}")
+    assert np.median(d_fmt) <= 1e-6 and (d_fmt <= 1e-4).mean() >= 0.98
+    assert (d <= 1e-6).mean() >= 0.999  # exp of the same FP32 value, but for the restated block-sum rounding
+    y32 = ol.mlp_forward(w, r, 1)
+
+    def normwise(a, b):
+        return np.linalg.norm(a - b) / np.linalg.norm(b)
     e8 = np.abs(out[2] - y64) / np.abs(y64)
     e16 = np.abs(out[0] - y64) / np.abs(y64)
-    e8_16 = np.abs(out[2] - out[0]) / np.abs(out[0])
-    print(f"FP8 vs FP64: median {np.median(e8):.2e} p99 {np.quantile(e8, 0.99):.2e} max {e8.max():.2e}; "
-          f"FP16 vs FP64: median {np.median(e16):.2e} p99 {np.quantile(e16, 0.99):.2e}; "
-          f"FP8 vs FP16 p99 {np.quantile(e8_16, 0.99):.2e}")
+    print(f"FP8 vs FP64: normwise {normwise(out[2], y64):.2e}, median {np.median(e8):.2e}, "
+          f"p99 {np.quantile(e8, 0.99):.2e}, max {e8.max():.2e}; vs FP32 normwise {normwise(out[2], y32):.2e}; "
+          f"FP16 vs FP64: normwise {normwise(out[0], y64):.2e}, p99 {np.quantile(e16, 0.99):.2e}; "
+          f"FP8 vs FP16 normwise {normwise(out[2], out[0]):.2e}")
+    assert normwise(out[2], y64) <= 2e-2 and normwise(out[2], y32) <= 2e-2
     assert np.median(e8) <= 2e-2
-    assert np.quantile(e8, 0.99) <= 1e-1
+    assert np.quantile(e8, 0.99) <= 5e-2  # single outputs: 3 mantissa bits (test_fp8_format_error_level)
 
 
 @pytest.mark.parametrize("loss", [0, 2])
