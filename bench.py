@@ -241,6 +241,76 @@ def adjoint_rate(hm, scene_path: Path, size: int = 512, spp: int = 16) -> dict:
             "grad_global": [float(x) for x in gg], "grad_sigma_norm": float(np.linalg.norm(gs))}
 
 
+def bsdf_rates(hm, n: int = 1 << 22, reps: int = 20) -> dict:
+    """Batched hair-BSDF streams (hmsnn_hair_bsdf_eval_dev / _sample_dev) on n
+    synthetic queries resident in HBM (wo in the hair frame's y = 0 plane,
+    uniform wi, h, sigma_a; beta_m = beta_n = 0.3, alpha = 2 degrees), wall
+    clock over reps calls after one warm call. Algorithmic bytes per query:
+    eval reads the 40-byte query and writes f (12) + pdf (4); sample also
+    reads 4 uniforms (16) and writes wi (12) + f (12) + pdf (4) + lobe (4)."""
+    import ctypes as C
+    lib = hm._lib.load()
+    rng = np.random.default_rng(5)
+    so = rng.uniform(-0.995, 0.995, n)
+    q = np.zeros((n, 10), np.float32)
+    q[:, 0], q[:, 2] = so, np.sqrt(1 - so * so)
+    v = rng.normal(size=(n, 3))
+    q[:, 3:6] = v / np.linalg.norm(v, axis=1, keepdims=True)
+    q[:, 6] = rng.uniform(-0.999, 0.999, n)
+    q[:, 7:10] = rng.uniform(0.01, 1.0, (n, 3))
+    u = rng.random((n, 4), dtype=np.float32)
+    dq, du = hm.DeviceBuffer.from_numpy(q), hm.DeviceBuffer.from_numpy(u)
+    dwi, df, dp, dl = hm.DeviceBuffer(12 * n), hm.DeviceBuffer(12 * n), hm.DeviceBuffer(4 * n), hm.DeviceBuffer(4 * n)
+    prm = hm._hair_params(0.3, 0.3, float(np.radians(2.0)))
+    calls = {
+        "eval": (lambda: lib.hmsnn_hair_bsdf_eval_dev(C.byref(prm), dq.ptr, n, df.ptr, dp.ptr, None), 40 + 16),
+        "sample": (lambda: lib.hmsnn_hair_bsdf_sample_dev(C.byref(prm), dq.ptr, du.ptr, n, dwi.ptr, df.ptr, dp.ptr,
+                                                          dl.ptr, None), 40 + 16 + 32),
+    }
+    out = {}
+    for name, (fn, nbytes) in calls.items():
+        hm._lib.check(fn())
+        hm._lib.check(lib.hmsnn_dev_synchronize())
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            hm._lib.check(fn())
+        hm._lib.check(lib.hmsnn_dev_synchronize())
+        dt = (time.perf_counter() - t0) / reps
+        gbs = nbytes * n / dt / 1e9
+        out[name] = {"queries": n, "ms": dt * 1e3, "Gqueries/s": n / dt / 1e9, "bytes_per_query": nbytes,
+                     "GB/s": gbs, "peak_GB/s": 8000.0, "frac": gbs / 8000.0}
+    return out
+
+
+def mlp_train_rate(hm, n: int = 1 << 16, reps: int = 20) -> dict:
+    """One FP16 training step (forward, backward, weight-gradient GEMM, Adam)
+    on n seeded records: 124,032 FLOP per record (3 x the forward's 41,344)."""
+    import ctypes as C
+    lib = hm._lib.load()
+    rng = np.random.default_rng(6)
+    rec = np.zeros((n, 12), np.float32)
+    rec[:, 0:3] = rng.uniform(-0.5, 0.5, (n, 3))
+    for a in (3, 6):
+        v = rng.normal(size=(n, 3))
+        rec[:, a:a + 3] = v / np.linalg.norm(v, axis=1, keepdims=True)
+    rec[:, 9:12] = rng.uniform(0.01, 0.5, (n, 3))
+    tgt = rng.uniform(0.1, 2.0, (n, 3)).astype(np.float32)
+    din, dt_ = hm.DeviceBuffer.from_numpy(rec), hm.DeviceBuffer.from_numpy(tgt)
+    h = C.c_void_p()
+    hm._lib.check(lib.hmsnn_mlp_create(0, 1, C.byref(h)))
+    loss = C.c_float()
+    hm._lib.check(lib.hmsnn_mlp_train_step_dev(h, din.ptr, dt_.ptr, n, 0, C.c_float(1e-3), C.byref(loss), None))
+    hm._lib.check(lib.hmsnn_dev_synchronize())
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hm._lib.check(lib.hmsnn_mlp_train_step_dev(h, din.ptr, dt_.ptr, n, 0, C.c_float(1e-3), C.byref(loss), None))
+    hm._lib.check(lib.hmsnn_dev_synchronize())
+    dt = (time.perf_counter() - t0) / reps
+    lib.hmsnn_mlp_destroy(h)
+    tf = 124032.0 * n / dt / 1e12
+    return {"records": n, "ms": dt * 1e3, "TFLOP/s": tf, "peak_TFLOP/s": 2500.0, "frac": tf / 2500.0}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,6 +327,9 @@ def main() -> None:
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="instead of the benchmark, run 8 device-to-device copies of this many MiB (the known-size "
                          "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="instead of the benchmark, time only the batched BSDF streams and the MLP inference and "
+                         "training kernels (a short program for rocprofv3 counter runs)")
     ap.add_argument("--scene", default=str(SCENE))
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -285,6 +358,10 @@ def main() -> None:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     log(f"device: {name}")
+    if args.kernels_only:
+        print(json.dumps({"bsdf": bsdf_rates(hm), "mlp_inference": mlp_rates(hm), "mlp_training": mlp_train_rate(hm)}),
+              flush=True)
+        return
     t0 = time.perf_counter()
     sc = hm.Scene(args.scene)
     log(f"scene ready in {time.perf_counter() - t0:.1f} s: {int(sc.info['num_segments'])} segments, "
@@ -385,8 +462,13 @@ def main() -> None:
         }
         try:
             line["mlp"]["inference"] = mlp_rates(hm)
+            line["mlp"]["training"] = mlp_train_rate(hm)
         except Exception as e:
             log(f"mlp timing failed: {e}")
+    try:
+        line["bsdf"] = bsdf_rates(hm)
+    except Exception as e:
+        log(f"bsdf timing failed: {e}")
     if not args.no_adjoint:
         try:
             line["adjoint"] = adjoint_rate(hm, Path(args.scene))
