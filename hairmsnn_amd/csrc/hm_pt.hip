@@ -361,15 +361,10 @@ __global__ void HM_TRACE_BOUNDS k_extend(SceneView sc, PathBuffers pb, int cur,
 
 // kStream: the depth is per slot (slot_depth), and terminated paths are
 // appended to done_queue for k_stream_finish.
-// kPart: 0 = the whole vertex; 1 = its direct-lighting half (escaped camera
-// rays, G-buffer, light sample and shadow rays; stores the random state after
-// its draws); 2 = its continuation half (roulette and BSDF sample, from that
-// state). 1 then 2 draw the same numbers in the same order as 0, and each
-// half keeps fewer values live, so more waves fit on a SIMD.
 #ifndef HM_SHADE_MIN_WAVES
 #define HM_SHADE_MIN_WAVES 1
 #endif
-template <bool kStream, int kPart>
+template <bool kStream>
 __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc, PathBuffers pb, ShadeParams sp,
                                                                    GBufferOut gb) {
   const int nxt = sp.cur ^ 1;
@@ -389,7 +384,7 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc,
       const V3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
       const bool training = sp.msnn && (pb.flags[slot] & 1u);
       if (prim == kPrimMiss) {
-        if (kPart != 2 && depth == 0) {
+        if (depth == 0) {
           // camera ray escaped: the miss program's radiance (optix_common.cuh:604-616)
           V3 le = v3(0.f);
           if (sc.env.enabled) le = env_radiance(sc.env, d);
@@ -399,7 +394,7 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc,
       } else {
         Hit h;
         setup_hit(sc, o, d, pb.hit_t[slot], prim, pb.hit_u[slot], h);
-        if (kPart != 2 && depth == 0 && gb.enabled) gbuffer_write_hit(gb, sc, slot, h, prim);
+        if (depth == 0 && gb.enabled) gbuffer_write_hit(gb, sc, slot, h, prim);
         uint32_t rng = pb.rng[slot];
         float4 b4 = pb.beta[slot];
         V3 beta = v3(b4.x, b4.y, b4.z);
@@ -409,7 +404,7 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc,
           beta_s = v3(s4.x, s4.y, s4.z);
         }
         // ---- direct lighting at this vertex ----
-        if (kPart != 2 && (depth == 0 ? sp.dl_zero != 0 : depth >= sp.dl_from)) {
+        if (depth == 0 ? sp.dl_zero != 0 : depth >= sp.dl_from) {
           DirectOut dl;
           direct_lighting(sc, h, rng, dl);
           if (dl.has1 || dl.has2) {
@@ -427,8 +422,8 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc,
           }
         }
         // ---- Russian roulette (optix_common.cuh:398-405; hair_msnn.cu:110-128, 171-181) ----
-        bool alive = kPart != 1;
-        if (kPart != 1 && depth >= 1) {
+        bool alive = true;
+        if (depth >= 1) {
           const float qq = fmaxf(0.05f, 1.f - luminance(beta));
           const float eps = draw(rng);
           if (training) {
@@ -468,20 +463,17 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc,
         }
         pb.rng[slot] = rng;
       }
-      if (kStream && kPart != 1) {
+      if (kStream) {
         if (cont) pb.slot_depth[slot] = depth + 1;
         done = !cont;
       }
     }
-    if (kStream && kPart != 1) {
+    if (kStream) {
       const int dpos = wave_append(&pb.counts[kDoneCount], done);
       if (done) pb.done_queue[dpos] = slot;
     }
-    if (kPart != 1) {
-      const int qpos = wave_append(&pb.counts[nxt], cont);
-      if (cont) pb.queue[nxt][qpos] = slot;
-    }
-    if (kPart == 2) continue;
+    const int qpos = wave_append(&pb.counts[nxt], cont);
+    if (cont) pb.queue[nxt][qpos] = slot;
     const int spos = wave_append(&pb.counts[2], shadow);
     if (shadow) pb.shadow_queue[spos] = slot;
     // one entry per shadow ray (slot * 2 + which), so k_shadow traces one ray per lane
@@ -489,20 +481,6 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc,
     if (ray1) pb.shadow_rays[r1] = 2 * slot;
     const int r2 = wave_append(&pb.counts[3], ray2);
     if (ray2) pb.shadow_rays[r2] = 2 * slot + 1;
-  }
-}
-
-#ifndef HM_SHADE_SPLIT
-#define HM_SHADE_SPLIT 0
-#endif
-template <bool kStream>
-void launch_shade(int blocks, hipStream_t st, const SceneView& sc, const PathBuffers& pb, const ShadeParams& sp,
-                  const GBufferOut& gb) {
-  if (HM_SHADE_SPLIT) {
-    hipLaunchKernelGGL((k_shade<kStream, 1>), dim3(blocks), dim3(256), 0, st, sc, pb, sp, gb);
-    hipLaunchKernelGGL((k_shade<kStream, 2>), dim3(blocks), dim3(256), 0, st, sc, pb, sp, gb);
-  } else {
-    hipLaunchKernelGGL((k_shade<kStream, 0>), dim3(blocks), dim3(256), 0, st, sc, pb, sp, gb);
   }
 }
 
@@ -680,7 +658,7 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
     if (cfg.nrc.enabled)
       hipLaunchKernelGGL(k_shade_nrc, dim3(shade_blocks), dim3(256), 0, stream, sc, pb, depth, cur, cfg.nrc);
     else
-      launch_shade<false>(shade_blocks, stream, sc, pb, sp, cfg.gb);
+      hipLaunchKernelGGL(k_shade<false>, dim3(shade_blocks), dim3(256), 0, stream, sc, pb, sp, cfg.gb);
     toc();
     tic(2);
     if (count)
@@ -818,7 +796,7 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
         sp.msnn = 0;
         sp.cur = cur[g];
         if (timed) tic(1);
-        launch_shade<true>(shade_blocks, st, sc, pb, sp, GBufferOut{});
+        hipLaunchKernelGGL(k_shade<true>, dim3(shade_blocks), dim3(256), 0, st, sc, pb, sp, GBufferOut{});
         if (timed) toc();
         if (timed) tic(2);
         hipLaunchKernelGGL(k_shadow<false>, dim3(sh_blocks), dim3(kTraceBlock), 0, st, sc, pb,
