@@ -109,6 +109,8 @@ Context::~Context() {
   for (hipEvent_t& e : live_ev)
     if (e) (void)hipEventDestroy(e), e = nullptr;
   if (live_host) (void)hipHostFree(live_host);
+  dev_free(stream_ring);
+  dev_free(stream_ctl);
   msnn_release(msnn);
   msnn = nullptr;
   nrc_release(nrc);
@@ -169,8 +171,10 @@ int alloc_path_buffers(PathBuffers& pb, std::vector<void*>& allocs, size_t n, bo
   HM_TRY(A((void**)&pb.sh_vis, 2 * n));
   HM_TRY(A((void**)&pb.counts, sizeof(int) * 16));
   HM_TRY(A((void**)&pb.slot_pass, sizeof(int) * n));
+  HM_TRY(A((void**)&pb.slot_px, sizeof(int) * n));
   HM_TRY(A((void**)&pb.slot_depth, sizeof(int) * n));
   HM_TRY(A((void**)&pb.done_queue, sizeof(int) * n));
+  HM_TRY(A((void**)&pb.free_slots, sizeof(int) * n));
   if (msnn_mode) {
     HM_TRY(A((void**)&pb.beta_short, f4));
     HM_TRY(A((void**)&pb.L_short, f4));
@@ -682,6 +686,10 @@ static int run_pt_passes(Context& c, int passes, std::string& err, PassTimer* ti
       HM_HIP(hipHostMalloc((void**)&c.live_host, 2 * Context::kMaxFlights * sizeof(int)), err);
       for (hipEvent_t& e : c.live_ev) HM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming), err);
     }
+    if (!c.stream_ring) {
+      HM_TRY(dev_alloc((void**)&c.stream_ring, sizeof(float4) * kStreamRing * (size_t)n, err));
+      HM_TRY(dev_alloc((void**)&c.stream_ctl, sizeof(long long) * kStreamCtl * Context::kMaxFlights, err));
+    }
     // every group's stream starts after the work queued so far (including
     // the other schedule's last accumulation)
     if (c.acc_pending) HM_HIP(hipStreamWaitEvent(c.stream, c.acc_event, 0), err);
@@ -696,7 +704,9 @@ static int run_pt_passes(Context& c, int passes, std::string& err, PassTimer* ti
     }
     PassConfig cfg = c.pt_pass_config();
     cfg.timer = timer;
-    HM_TRY(pt_run_stream(v, pbs, sts, K, cfg, passes, c.accum, c.average, c.live_host, c.live_ev, err));
+    HM_TRY(pt_run_stream(v, pbs, sts, K, cfg, passes, c.accum, c.average, c.stream_ring, c.stream_ctl, c.live_host,
+                         c.live_ev, err));
+    c.last_ring_entry = (passes - 1) % kStreamRing;
     // the context stream waits for every group
     for (int g = 1; g < K; ++g) {
       HM_HIP(hipEventRecord(c.acc_event, sts[g]), err);
@@ -773,17 +783,10 @@ int hmsnn_get_last_pass(hmsnn_ctx* ctx, float* rgba) {
   const int npx = c.scene.width * c.scene.height;
   hipError_t e = hipSuccess;
   if (c.last_streamed) {
-    // each stream group's slots hold its pixels' last pass
-    const int K = c.stream_groups;
-    std::vector<float4> tmp;
-    for (int g = 0; g < K && e == hipSuccess; ++g) {
-      const int cnt = stream_group_slots(npx, g, K);
-      tmp.resize(cnt);
-      e = hipMemcpyAsync(tmp.data(), c.flight_pb(g).L, sizeof(float4) * cnt, hipMemcpyDeviceToHost, c.stream);
-      if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
-      for (int s = 0; s < cnt && e == hipSuccess; ++s)
-        std::memcpy(rgba + 4 * (size_t)stream_group_pixel(s, g, K), &tmp[s], sizeof(float4));
-    }
+    // the ring entry of the last pass holds every pixel's radiance
+    e = hipMemcpyAsync(rgba, c.stream_ring + (size_t)c.last_ring_entry * npx, sizeof(float4) * (size_t)npx,
+                       hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
   } else {
     const int K = std::max(1, std::min(c.num_flights, Context::kMaxFlights));
     const PathBuffers& last = c.flight_pb((c.accum_id - 1) % K);

@@ -87,6 +87,9 @@ struct Context {
   bool last_streamed = false;   // the last pt call used the streaming schedule
   int stream_groups = 1;        // groups of that call
   int* live_host = nullptr;     // pinned, 2 per group: live-path count polls
+  float4* stream_ring = nullptr;  // streaming schedule: kStreamRing passes of radiance per pixel
+  long long* stream_ctl = nullptr;  // streaming schedule: scheduler counters per group
+  int last_ring_entry = 0;          // ring entry of the last streamed pass
   hipEvent_t live_ev[2 * kMaxFlights] = {};
   int ensure_flights(std::string& err);
   PathBuffers& flight_pb(int f) { return f == 0 ? pb : extra[f - 1].pb; }

@@ -74,10 +74,13 @@ struct PathBuffers {
   int* shadow_queue;  // slots with pending direct light
   int* shadow_rays;   // pending shadow rays: slot * 2 + (0: light sample, 1: BSDF sample)
   uint8_t* sh_vis;    // visibility per (slot, ray), same indexing as shadow_rays
-  // streaming schedule: pass index and vertex depth per slot, finished slots
+  // streaming schedule: per slot the job's pass (relative to the call's
+  // first) and pixel and the vertex depth; finished slots; free slots
   int* slot_pass;
+  int* slot_px;
   int* slot_depth;
   int* done_queue;
+  int* free_slots;
   int* counts;  // [0], [1] continuation queues, [2] shadow slots, [3] shadow rays, [4] finished paths
   int* spill;   // traversal stack spill
   int spill_stride;

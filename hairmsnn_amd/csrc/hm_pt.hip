@@ -33,6 +33,18 @@ __device__ __forceinline__ int wave_append(int* counter, bool pred) {
   return pred ? base + rank : -1;
 }
 
+__device__ __forceinline__ int wave_append_ll(long long* counter, bool pred) {
+  const unsigned long long mask = __ballot(pred);
+  const int lane = __lane_id();
+  const int total = __popcll(mask);
+  long long base = 0;
+  const int leader = __ffsll((long long)mask) - 1;
+  if (total > 0 && lane == leader) base = (long long)atomicAdd((unsigned long long*)counter, (unsigned long long)total);
+  base = __shfl(base, leader < 0 ? 0 : leader);
+  const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+  return pred ? (int)base + rank : -1;
+}
+
 __device__ __forceinline__ TraceStack make_stack(int* lds_base, const PathBuffers& pb) {
   TraceStack st;
   st.lds = lds_base + threadIdx.x;
@@ -683,51 +695,127 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
 // overlaps another's traversal: group g owns the 64-pixel chunks g, g + K,
 // g + 2K, ... (balanced work, coherent camera rays within a wave).
 
-__global__ void k_stream_camera(SceneView sc, PathBuffers pb, int first_pass, int seed_offset, int g, int K,
-                                int count) {
+// Job scheduler of the streaming schedule, one per stream group. A job is
+// (pass, pixel): job j of a call is pass j / cnt of the group's pixel j % cnt,
+// so jobs are handed out in pass order. A slot whose path ends takes the next
+// job, whichever pixel it belongs to, so a pixel's passes can run at the same
+// time in different slots and no slot idles while jobs remain. Each finished
+// job leaves its radiance in a ring of kStreamRing passes (indexed by the
+// global pixel); a pass is added to the image once all its jobs are done, in
+// pass order, so the accumulation (and the image) is the per-pass schedule's.
+// Jobs of pass p start only while p < (accumulated passes) + kStreamRing, so
+// the ring never overwrites a pass that is not accumulated yet.
+// ctl (64-bit, per group): [0] next job, [1] passes accumulated, [2] free
+// slots, [3..5] issue plan (first job, count, first free-stack index), [6..7]
+// accumulation plan (first pass, count), [8 + r] finished jobs of ring entry r.
+__global__ void k_stream_init(PathBuffers pb, long long* ctl, int cnt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) pb.counts[0] = count;
-  if (i >= count) return;
-  camera_path(sc, pb, i, stream_group_pixel(i, g, K), first_pass + seed_offset);
-  pb.flags[i] = 0u;
-  pb.slot_pass[i] = first_pass;
-  pb.slot_depth[i] = 0;
-  pb.queue[0][i] = i;
+  if (i < 8 + kStreamRing) ctl[i] = 0;
+  if (i == 2) ctl[2] = cnt;
+  if (i < cnt) pb.free_slots[i] = i;
 }
 
-// Accumulates each finished path (writePixel, utils.cuh:13-35) and queues the
-// pixel's next pass on queue nxt while passes remain (pass < end_pass).
-__global__ void k_stream_finish(SceneView sc, PathBuffers pb, int nxt, float4* accum, float4* average,
-                                int end_pass, int seed_offset, int g, int K) {
+// Single thread: accumulation plan (passes whose jobs are all done, at most
+// a ring's worth), then the issue plan (free slots x jobs inside the window).
+__global__ void k_stream_plan(long long* ctl, int cnt, int passes) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const long long acc = ctl[1];
+  int k = 0;
+  while (acc + k < passes && k < kStreamRing && ctl[8 + (int)((acc + k) % kStreamRing)] == cnt) {
+    ctl[8 + (int)((acc + k) % kStreamRing)] = 0;
+    ++k;
+  }
+  ctl[6] = acc;
+  ctl[7] = k;
+  ctl[1] = acc + k;
+  const long long limit = (long long)min((long long)passes, acc + k + kStreamRing) * cnt;
+  const long long avail = max(0ll, min(ctl[2], limit - ctl[0]));
+  ctl[3] = ctl[0];
+  ctl[4] = avail;
+  ctl[5] = ctl[2] - avail;
+  ctl[0] += avail;
+  ctl[2] -= avail;
+}
+
+// Adds the planned passes (writePixel, utils.cuh:13-35) to the group's pixels
+// in pass order.
+__global__ void k_stream_accumulate(const long long* ctl, const float4* ring, float4* accum, float4* average,
+                                    int first_pass, int npx, int g, int K, int cnt) {
+  const int k = (int)ctl[7];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k == 0 || i >= cnt) return;
+  const int px = stream_group_pixel(i, g, K);
+  const long long p0 = ctl[6];
+  V3 c = v3(0.f);
+  int pass = 0;
+  for (int j = 0; j < k; ++j) {
+    const long long p = p0 + j;
+    pass = first_pass + (int)p;
+    const float4 l = ring[(size_t)(p % kStreamRing) * npx + px];
+    V3 v = v3(l.x, l.y, l.z);
+    if (any_nan(v)) v = v3(0.f);
+    if (pass > 0) {
+      const float4 a = j == 0 ? accum[px] : make_float4(c.x, c.y, c.z, 1.f);
+      v = v + v3(a.x, a.y, a.z);
+    }
+    c = v;
+  }
+  accum[px] = make_float4(c.x, c.y, c.z, 1.f);
+  const float inv = 1.f / (float)(pass + 1);
+  average[px] = make_float4(c.x * inv, c.y * inv, c.z * inv, 1.f);
+}
+
+// Starts the planned jobs in free slots and queues them on queue nxt.
+__global__ void k_stream_issue(SceneView sc, PathBuffers pb, const long long* ctl, int nxt, int first_pass,
+                               int seed_offset, int g, int K, int cnt) {
+  const long long n = ctl[4], job0 = ctl[3], top = ctl[5];
+  const int stride = gridDim.x * blockDim.x;
+  const long long nloop = (n + stride - 1) / stride * stride;
+  for (long long t = blockIdx.x * blockDim.x + threadIdx.x; t < nloop; t += stride) {
+    const bool go = t < n;
+    int slot = -1;
+    if (go) {
+      slot = pb.free_slots[top + t];
+      const long long job = job0 + t;
+      const int pass = (int)(job / cnt);
+      const int px = stream_group_pixel((int)(job % cnt), g, K);
+      camera_path(sc, pb, slot, px, first_pass + pass + seed_offset);
+      pb.flags[slot] = 0u;
+      pb.slot_pass[slot] = pass;
+      pb.slot_px[slot] = px;
+      pb.slot_depth[slot] = 0;
+    }
+    const int q = wave_append(&pb.counts[nxt], go);
+    if (go) pb.queue[nxt][q] = slot;
+  }
+}
+
+// Finished paths: radiance into the ring, one more finished job for the
+// pass's ring entry, the slot back on the free stack.
+__global__ void k_stream_finish(PathBuffers pb, long long* ctl, float4* ring, int npx) {
   const int n = pb.counts[kDoneCount];
   const int stride = gridDim.x * blockDim.x;
   const int nloop = (n + stride - 1) / stride * stride;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
-    bool again = false;
-    int slot = -1;
-    if (i < n) {
+    const bool done = i < n;
+    int slot = -1, r = -1;
+    if (done) {
       slot = pb.done_queue[i];
-      const int px = stream_group_pixel(slot, g, K);
-      const int pass = pb.slot_pass[slot];
-      const float4 l = pb.L[slot];
-      V3 c = v3(l.x, l.y, l.z);
-      if (any_nan(c)) c = v3(0.f);
-      if (pass > 0) {
-        const float4 a = accum[px];
-        c = c + v3(a.x, a.y, a.z);
-      }
-      accum[px] = make_float4(c.x, c.y, c.z, 1.f);
-      const float inv = 1.f / (float)(pass + 1);
-      average[px] = make_float4(c.x * inv, c.y * inv, c.z * inv, 1.f);
-      if (pass + 1 < end_pass) {
-        camera_path(sc, pb, slot, px, pass + 1 + seed_offset);
-        pb.slot_pass[slot] = pass + 1;
-        pb.slot_depth[slot] = 0;
-        again = true;
-      }
+      r = pb.slot_pass[slot] % kStreamRing;
+      ring[(size_t)r * npx + pb.slot_px[slot]] = pb.L[slot];
     }
-    const int q = wave_append(&pb.counts[nxt], again);
-    if (again) pb.queue[nxt][q] = slot;
+    // one atomic per distinct ring entry in the wave (a wave's jobs span a
+    // few passes at most)
+    unsigned long long left = __ballot(done);
+    while (left) {
+      const int lead = __ffsll((long long)left) - 1;
+      const int r0 = __shfl(r, lead);
+      const unsigned long long same = __ballot(done && r == r0);
+      if (__lane_id() == lead) atomicAdd((unsigned long long*)&ctl[8 + r0], (unsigned long long)__popcll(same));
+      left &= ~same;
+    }
+    const int pos = wave_append_ll(&ctl[2], done);
+    if (done) pb.free_slots[pos] = slot;
   }
 }
 
@@ -737,8 +825,8 @@ int stream_group_slots(int npx, int g, int K) {
 }
 
 int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_t* streams, int K,
-                  const PassConfig& cfg, int passes, float4* accum, float4* average, int* live_host,
-                  hipEvent_t* live_ev, std::string& err) {
+                  const PassConfig& cfg, int passes, float4* accum, float4* average, float4* ring, long long* ctl,
+                  int* live_host, hipEvent_t* live_ev, std::string& err) {
   const int npx = sc.width * sc.height;
   if (passes <= 0 || npx % 64 != 0 || K < 1) return passes <= 0 ? kOk : (err = "bad stream groups", kEInval);
   PassTimer* tm = cfg.timer;
@@ -755,18 +843,23 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
       tm->used++;
     }
   };
-  const int end_pass = cfg.accum_id + passes;
   constexpr int kBatch = 8;  // even: each group's live queue is back at index 0 after a batch
   std::vector<int> cur(K, 0), live(K, 0);
   std::vector<long long> iters(K, 0);
+  // first jobs: every slot free, then one issue round into queue 0
   for (int g = 0; g < K; ++g) {
     const int cnt = stream_group_slots(npx, g, K);
     live[g] = cnt > 0;
     if (!cnt) continue;
-    hipLaunchKernelGGL(k_stream_camera, dim3((cnt + 255) / 256), dim3(256), 0, streams[g], sc, *pbs[g], cfg.accum_id,
+    long long* c = ctl + (size_t)g * kStreamCtl;
+    const int blocks = (cnt + 255) / 256;
+    hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, streams[g], pbs[g]->counts, 0, 2, 3);
+    hipLaunchKernelGGL(k_stream_init, dim3(blocks), dim3(256), 0, streams[g], *pbs[g], c, cnt);
+    hipLaunchKernelGGL(k_stream_plan, dim3(1), dim3(64), 0, streams[g], c, cnt, passes);
+    hipLaunchKernelGGL(k_stream_issue, dim3(blocks), dim3(256), 0, streams[g], sc, *pbs[g], c, 0, cfg.accum_id,
                        cfg.seed_offset, g, K, cnt);
   }
-  HM_TRY(check_launch("stream camera", err));
+  HM_TRY(check_launch("stream start", err));
   const long long max_iters = (long long)passes * std::max(1, cfg.max_vertex + 1) + 4 * kBatch;
   for (int batch = 0;; ++batch) {
     bool any = false;
@@ -804,8 +897,13 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
         hipLaunchKernelGGL(k_shadow_resolve, dim3(shade_blocks), dim3(256), 0, st, pb, 0, 0, NrcPassArgs{});
         if (timed) toc();
         if (timed) tic(3);
-        hipLaunchKernelGGL(k_stream_finish, dim3(shade_blocks), dim3(256), 0, st, sc, pb, nxt, accum, average,
-                           end_pass, cfg.seed_offset, g, K);
+        long long* c = ctl + (size_t)g * kStreamCtl;
+        hipLaunchKernelGGL(k_stream_finish, dim3(shade_blocks), dim3(256), 0, st, pb, c, ring, npx);
+        hipLaunchKernelGGL(k_stream_plan, dim3(1), dim3(64), 0, st, c, cnt, passes);
+        hipLaunchKernelGGL(k_stream_accumulate, dim3((cnt + 255) / 256), dim3(256), 0, st, c, ring, accum, average,
+                           cfg.accum_id, npx, g, K, cnt);
+        hipLaunchKernelGGL(k_stream_issue, dim3(shade_blocks), dim3(256), 0, st, sc, pb, c, nxt, cfg.accum_id,
+                           cfg.seed_offset, g, K, cnt);
         if (timed) toc();
         cur[g] = nxt;
       }

@@ -123,16 +123,25 @@ struct PassConfig {
 int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hipStream_t stream,
                 std::string& err);
 
-// Runs `passes` passes from cfg.accum_id with the streaming schedule: a
-// pixel's next pass starts in its slot as soon as its path ends, each finished
-// path accumulated into accum / average in pass order. The pixels are split
-// into K groups (64-pixel chunks dealt round-robin), group g on pbs[g] and
-// streams[g]; live_host (pinned, 2K ints) and live_ev (2K events) poll the
-// live-path counts. A timer records group 0's launches.
+// Passes the streaming schedule keeps in flight at most (its radiance ring).
+constexpr int kStreamRing = 16;
+// Scheduler counters per stream group (64-bit).
+constexpr int kStreamCtl = 64;
+
+// Runs `passes` passes from cfg.accum_id with the streaming schedule: slots
+// take (pass, pixel) jobs in pass order as their paths end, finished jobs
+// leave their radiance in ring (kStreamRing x npx float4, by pass % ring and
+// pixel) and each pass is accumulated into accum / average, in pass order,
+// once all its jobs are done. The pixels are split into K groups (64-pixel
+// chunks dealt round-robin), group g on pbs[g] and streams[g] with ctl +
+// g * kStreamCtl; live_host (pinned, 2K ints) and live_ev (2K events) poll the
+// live-path counts. A timer records group 0's launches. The last pass's
+// radiance is left in ring entry (passes - 1) % kStreamRing.
 int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_t* streams, int K,
-                  const PassConfig& cfg, int passes, float4* accum, float4* average, int* live_host,
-                  hipEvent_t* live_ev, std::string& err);
-// Slots of stream group g of K over npx pixels, and the pixel of a slot.
+                  const PassConfig& cfg, int passes, float4* accum, float4* average, float4* ring, long long* ctl,
+                  int* live_host, hipEvent_t* live_ev, std::string& err);
+// Pixels of stream group g of K over npx pixels, and the pixel of a group's
+// i-th pixel.
 int stream_group_slots(int npx, int g, int K);
 HM_HD int stream_group_pixel(int slot, int g, int K) { return ((slot >> 6) * K + g) * 64 + (slot & 63); }
 
