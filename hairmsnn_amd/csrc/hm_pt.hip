@@ -738,45 +738,44 @@ __global__ void k_stream_plan(long long* ctl, int cnt, int passes) {
 }
 
 // Adds the planned passes (writePixel, utils.cuh:13-35) to the group's pixels
-// in pass order.
-__global__ void k_stream_accumulate(const long long* ctl, const float4* ring, float4* accum, float4* average,
-                                    int first_pass, int npx, int g, int K, int cnt) {
+// in pass order, and starts the planned jobs in free slots, queued on queue
+// nxt. Threads cover the group's pixels only when passes are to be added.
+__global__ void k_stream_next(SceneView sc, PathBuffers pb, const long long* ctl, const float4* ring, float4* accum,
+                              float4* average, int nxt, int first_pass, int seed_offset, int npx, int g, int K,
+                              int cnt) {
   const int k = (int)ctl[7];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k == 0 || i >= cnt) return;
-  const int px = stream_group_pixel(i, g, K);
   const long long p0 = ctl[6];
-  V3 c = v3(0.f);
-  int pass = 0;
-  for (int j = 0; j < k; ++j) {
-    const long long p = p0 + j;
-    pass = first_pass + (int)p;
-    const float4 l = ring[(size_t)(p % kStreamRing) * npx + px];
-    V3 v = v3(l.x, l.y, l.z);
-    if (any_nan(v)) v = v3(0.f);
-    if (pass > 0) {
-      const float4 a = j == 0 ? accum[px] : make_float4(c.x, c.y, c.z, 1.f);
-      v = v + v3(a.x, a.y, a.z);
-    }
-    c = v;
-  }
-  accum[px] = make_float4(c.x, c.y, c.z, 1.f);
-  const float inv = 1.f / (float)(pass + 1);
-  average[px] = make_float4(c.x * inv, c.y * inv, c.z * inv, 1.f);
-}
-
-// Starts the planned jobs in free slots and queues them on queue nxt.
-__global__ void k_stream_issue(SceneView sc, PathBuffers pb, const long long* ctl, int nxt, int first_pass,
-                               int seed_offset, int g, int K, int cnt) {
-  const long long n = ctl[4], job0 = ctl[3], top = ctl[5];
+  const int n = (int)ctl[4];
+  const long long job0 = ctl[3], top = ctl[5];
+  const int work = max(k > 0 ? cnt : 0, n);
   const int stride = gridDim.x * blockDim.x;
-  const long long nloop = (n + stride - 1) / stride * stride;
-  for (long long t = blockIdx.x * blockDim.x + threadIdx.x; t < nloop; t += stride) {
-    const bool go = t < n;
+  const int nloop = (work + stride - 1) / stride * stride;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
+    if (k > 0 && i < cnt) {
+      const int px = stream_group_pixel(i, g, K);
+      V3 c = v3(0.f);
+      int pass = 0;
+      for (int j = 0; j < k; ++j) {
+        const long long p = p0 + j;
+        pass = first_pass + (int)p;
+        const float4 l = ring[(size_t)(p % kStreamRing) * npx + px];
+        V3 v = v3(l.x, l.y, l.z);
+        if (any_nan(v)) v = v3(0.f);
+        if (pass > 0) {
+          const float4 a = j == 0 ? accum[px] : make_float4(c.x, c.y, c.z, 1.f);
+          v = v + v3(a.x, a.y, a.z);
+        }
+        c = v;
+      }
+      accum[px] = make_float4(c.x, c.y, c.z, 1.f);
+      const float inv = 1.f / (float)(pass + 1);
+      average[px] = make_float4(c.x * inv, c.y * inv, c.z * inv, 1.f);
+    }
+    const bool go = i < n;
     int slot = -1;
     if (go) {
-      slot = pb.free_slots[top + t];
-      const long long job = job0 + t;
+      slot = pb.free_slots[top + i];
+      const long long job = job0 + i;
       const int pass = (int)(job / cnt);
       const int px = stream_group_pixel((int)(job % cnt), g, K);
       camera_path(sc, pb, slot, px, first_pass + pass + seed_offset);
@@ -856,8 +855,8 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
     hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, streams[g], pbs[g]->counts, 0, 2, 3);
     hipLaunchKernelGGL(k_stream_init, dim3(blocks), dim3(256), 0, streams[g], *pbs[g], c, cnt);
     hipLaunchKernelGGL(k_stream_plan, dim3(1), dim3(64), 0, streams[g], c, cnt, passes);
-    hipLaunchKernelGGL(k_stream_issue, dim3(blocks), dim3(256), 0, streams[g], sc, *pbs[g], c, 0, cfg.accum_id,
-                       cfg.seed_offset, g, K, cnt);
+    hipLaunchKernelGGL(k_stream_next, dim3(blocks), dim3(256), 0, streams[g], sc, *pbs[g], c, ring, accum, average,
+                       0, cfg.accum_id, cfg.seed_offset, npx, g, K, cnt);
   }
   HM_TRY(check_launch("stream start", err));
   const long long max_iters = (long long)passes * std::max(1, cfg.max_vertex + 1) + 4 * kBatch;
@@ -900,10 +899,8 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
         long long* c = ctl + (size_t)g * kStreamCtl;
         hipLaunchKernelGGL(k_stream_finish, dim3(shade_blocks), dim3(256), 0, st, pb, c, ring, npx);
         hipLaunchKernelGGL(k_stream_plan, dim3(1), dim3(64), 0, st, c, cnt, passes);
-        hipLaunchKernelGGL(k_stream_accumulate, dim3((cnt + 255) / 256), dim3(256), 0, st, c, ring, accum, average,
-                           cfg.accum_id, npx, g, K, cnt);
-        hipLaunchKernelGGL(k_stream_issue, dim3(shade_blocks), dim3(256), 0, st, sc, pb, c, nxt, cfg.accum_id,
-                           cfg.seed_offset, g, K, cnt);
+        hipLaunchKernelGGL(k_stream_next, dim3(shade_blocks), dim3(256), 0, st, sc, pb, c, ring, accum, average,
+                           nxt, cfg.accum_id, cfg.seed_offset, npx, g, K, cnt);
         if (timed) toc();
         cur[g] = nxt;
       }
