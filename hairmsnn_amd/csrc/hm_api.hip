@@ -175,6 +175,18 @@ int alloc_path_buffers(PathBuffers& pb, std::vector<void*>& allocs, size_t n, bo
   HM_TRY(A((void**)&pb.slot_depth, sizeof(int) * n));
   HM_TRY(A((void**)&pb.done_queue, sizeof(int) * n));
   HM_TRY(A((void**)&pb.free_slots, sizeof(int) * n));
+  HM_TRY(A((void**)&pb.alt.ray_o, f4));
+  HM_TRY(A((void**)&pb.alt.ray_d, f4));
+  HM_TRY(A((void**)&pb.alt.beta, f4));
+  HM_TRY(A((void**)&pb.alt.L, f4));
+  HM_TRY(A((void**)&pb.alt.rng, sizeof(uint32_t) * n));
+  HM_TRY(A((void**)&pb.alt.pass, sizeof(int) * n));
+  HM_TRY(A((void**)&pb.alt.px, sizeof(int) * n));
+  HM_TRY(A((void**)&pb.alt.depth, sizeof(int) * n));
+  HM_TRY(A((void**)&pb.dest, sizeof(int) * n));
+  HM_TRY(A((void**)&pb.done_L, f4));
+  HM_TRY(A((void**)&pb.done_px, sizeof(int) * n));
+  HM_TRY(A((void**)&pb.done_pass, sizeof(int) * n));
   if (msnn_mode) {
     HM_TRY(A((void**)&pb.beta_short, f4));
     HM_TRY(A((void**)&pb.L_short, f4));

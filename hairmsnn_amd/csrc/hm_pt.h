@@ -49,6 +49,19 @@ constexpr int kDoneCount = 4;  // PathBuffers::counts index of the finished-path
 
 // Per-slot path state (structure of arrays in HBM). A slot is one pixel of
 // the current sample pass.
+// Second copy of the per-path state for the streaming schedule, which
+// compacts its wavefront into the other copy every iteration.
+struct PathState {
+  float4* ray_o;
+  float4* ray_d;
+  float4* beta;
+  float4* L;
+  uint32_t* rng;
+  int* pass;
+  int* px;
+  int* depth;
+};
+
 struct PathBuffers {
   float4* ray_o;
   float4* ray_d;
@@ -81,6 +94,14 @@ struct PathBuffers {
   int* slot_depth;
   int* done_queue;
   int* free_slots;
+  // compacted streaming: the other state copy, where each path of this
+  // wavefront went (>= 0: position in the next wavefront, < 0: ~position in
+  // the finished list) and the finished paths' radiance, pixel and pass
+  PathState alt;
+  int* dest;
+  float4* done_L;
+  int* done_px;
+  int* done_pass;
   int* counts;  // [0], [1] continuation queues, [2] shadow slots, [3] shadow rays, [4] finished paths
   int* spill;   // traversal stack spill
   int spill_stride;

@@ -348,7 +348,8 @@ __global__ void __launch_bounds__(256) k_shade_nrc(SceneView sc, PathBuffers pb,
   }
 }
 
-template <bool kCount>
+// kDense: the wavefront is compacted (path i is at index i), no queue.
+template <bool kCount, bool kDense = false>
 __global__ void HM_TRACE_BOUNDS k_extend(SceneView sc, PathBuffers pb, int cur,
                                                         unsigned long long* stats) {
   __shared__ int lds_stack[kStackLds * kTraceBlock];
@@ -358,7 +359,7 @@ __global__ void HM_TRACE_BOUNDS k_extend(SceneView sc, PathBuffers pb, int cur,
   const int* q = pb.queue[cur];
   const int stride = gridDim.x * blockDim.x;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const int slot = q[i];
+    const int slot = kDense ? i : q[i];
     const float4 o = pb.ray_o[slot], d = pb.ray_d[slot];
     float t, u;
     int prim;
@@ -493,6 +494,144 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc,
     if (ray1) pb.shadow_rays[r1] = 2 * slot;
     const int r2 = wave_append(&pb.counts[3], ray2);
     if (ray2) pb.shadow_rays[r2] = 2 * slot + 1;
+  }
+}
+
+// Compacted streaming vertex (plain path tracing only): path i of wavefront
+// pb is shaded as k_shade<true> shades a slot (the same operations in the same
+// order), then a continuing path is written densely into wavefront nb and a
+// finished one into the finished list; pb.dest[i] records where it went, for
+// k_shadow_resolve_dense. Shadow records stay at index i.
+__global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade_dense(SceneView sc, PathBuffers pb,
+                                                                          PathBuffers nb, ShadeParams sp) {
+  const int nxt = sp.cur ^ 1;
+  const int n = pb.counts[sp.cur];
+  const int stride = gridDim.x * blockDim.x;
+  const int nloop = (n + stride - 1) / stride * stride;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
+    bool cont = false, shadow = false, ray1 = false, ray2 = false;
+    const bool live = i < n;
+    float4 L = make_float4(0.f, 0.f, 0.f, 0.f), no4 = L, wi4 = L, beta4 = L;
+    uint32_t rng = 0;
+    int depth = 0;
+    if (live) {
+      depth = pb.slot_depth[i];
+      const int prim = pb.hit_prim[i];
+      const float4 o4 = pb.ray_o[i], d4 = pb.ray_d[i];
+      const V3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
+      L = pb.L[i];
+      if (prim == kPrimMiss) {
+        if (depth == 0) {
+          // camera ray escaped: the miss program's radiance (optix_common.cuh:604-616)
+          V3 le = v3(0.f);
+          if (sc.env.enabled) le = env_radiance(sc.env, d);
+          L = make_float4(le.x, le.y, le.z, 0.f);
+        }
+      } else {
+        Hit h;
+        setup_hit(sc, o, d, pb.hit_t[i], prim, pb.hit_u[i], h);
+        rng = pb.rng[i];
+        const float4 b4 = pb.beta[i];
+        V3 beta = v3(b4.x, b4.y, b4.z);
+        if (depth == 0 ? sp.dl_zero != 0 : depth >= sp.dl_from) {
+          DirectOut dl;
+          direct_lighting(sc, h, rng, dl);
+          if (dl.has1 || dl.has2) {
+            shadow = true;
+            ray1 = dl.has1 != 0;
+            ray2 = dl.has2 != 0;
+            pb.sh_o1[i] = make_float4(dl.o1.x, dl.o1.y, dl.o1.z, dl.has1 ? 1.f : 0.f);
+            pb.sh_d1[i] = make_float4(dl.d1.x, dl.d1.y, dl.d1.z, 0.f);
+            pb.sh_c1[i] = make_float4(dl.c1.x, dl.c1.y, dl.c1.z, 0.f);
+            pb.sh_o2[i] = make_float4(dl.o2.x, dl.o2.y, dl.o2.z, dl.has2 ? 1.f : 0.f);
+            pb.sh_d2[i] = make_float4(dl.d2.x, dl.d2.y, dl.d2.z, 0.f);
+            pb.sh_c2[i] = make_float4(dl.c2.x, dl.c2.y, dl.c2.z, 0.f);
+            pb.sh_beta[i] = make_float4(beta.x, beta.y, beta.z, 0.f);
+          }
+        }
+        // Russian roulette (optix_common.cuh:398-405)
+        bool alive = true;
+        if (depth >= 1) {
+          const float qq = fmaxf(0.05f, 1.f - luminance(beta));
+          const float eps = draw(rng);
+          if (eps < qq) alive = false;
+          else beta = beta / (1.f - qq);
+        }
+        // continuation (nextPathVertex, optix_common.cuh:311-360)
+        if (alive && depth + 1 <= sp.max_vertex) {
+          const BsdfSample bs = sample_bsdf(sc, h, rng);
+          const V3 wi = bs.wi, f = bs.f;
+          const float pdf = bs.pdf;
+          V3 mul = (pdf == 0.f) ? f : f / pdf;
+          if (any_nan(mul)) mul = v3(1.f);
+          beta = beta * mul;
+          V3 p = h.p, nd = h.n;
+          if (dot(wi, h.n) < 0.f && !h.surface) {
+            nd = -1.f * h.n;
+            p = p + (2.f * h.radius) * nd;
+          }
+          const V3 no = p + 1e-3f * nd;
+          no4 = make_float4(no.x, no.y, no.z, 0.f);
+          wi4 = make_float4(wi.x, wi.y, wi.z, 0.f);
+          beta4 = make_float4(beta.x, beta.y, beta.z, 0.f);
+          cont = true;
+        }
+      }
+    }
+    const bool done = live && !cont;
+    const int p = wave_append(&pb.counts[nxt], cont);
+    const int dp = wave_append(&pb.counts[kDoneCount], done);
+    if (cont) {
+      nb.ray_o[p] = no4;
+      nb.ray_d[p] = wi4;
+      nb.beta[p] = beta4;
+      nb.L[p] = L;
+      nb.rng[p] = rng;
+      nb.slot_pass[p] = pb.slot_pass[i];
+      nb.slot_px[p] = pb.slot_px[i];
+      nb.slot_depth[p] = depth + 1;
+      pb.dest[i] = p;
+    }
+    if (done) {
+      pb.done_L[dp] = L;
+      pb.done_px[dp] = pb.slot_px[i];
+      pb.done_pass[dp] = pb.slot_pass[i];
+      pb.dest[i] = ~dp;
+    }
+    const int spos = wave_append(&pb.counts[2], shadow);
+    if (shadow) pb.shadow_queue[spos] = i;
+    const int r1 = wave_append(&pb.counts[3], ray1);
+    if (ray1) pb.shadow_rays[r1] = 2 * i;
+    const int r2 = wave_append(&pb.counts[3], ray2);
+    if (ray2) pb.shadow_rays[r2] = 2 * i + 1;
+  }
+}
+
+// The resolve of k_shadow_resolve for a compacted wavefront: the visible
+// contributions go to the path where k_shade_dense moved it.
+__global__ void k_shadow_resolve_dense(PathBuffers pb, PathBuffers nb) {
+  const int n = pb.counts[2];
+  const int stride = gridDim.x * blockDim.x;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+    const int slot = pb.shadow_queue[j];
+    V3 sum = v3(0.f);
+    if (pb.sh_o1[slot].w != 0.f && pb.sh_vis[2 * slot]) {
+      const float4 c = pb.sh_c1[slot];
+      sum = sum + v3(c.x, c.y, c.z);
+    }
+    if (pb.sh_o2[slot].w != 0.f && pb.sh_vis[2 * slot + 1]) {
+      const float4 c = pb.sh_c2[slot];
+      sum = sum + v3(c.x, c.y, c.z);
+    }
+    if (any_nan(sum)) sum = v3(0.f);  // optix_common.cuh:305-306
+    const float4 b = pb.sh_beta[slot];
+    const int dst = pb.dest[slot];
+    float4* Lp = dst >= 0 ? nb.L + dst : pb.done_L + ~dst;
+    float4 L = *Lp;
+    L.x += b.x * sum.x;
+    L.y += b.y * sum.y;
+    L.z += b.z * sum.z;
+    *Lp = L;
   }
 }
 
@@ -717,8 +856,7 @@ __global__ void k_stream_init(PathBuffers pb, long long* ctl, int cnt) {
 
 // Single thread: accumulation plan (passes whose jobs are all done, at most
 // a ring's worth), then the issue plan (free slots x jobs inside the window).
-__global__ void k_stream_plan(long long* ctl, int cnt, int passes) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+__device__ void stream_plan(long long* ctl, int cnt, int passes) {
   const long long acc = ctl[1];
   int k = 0;
   while (acc + k < passes && k < kStreamRing && ctl[8 + (int)((acc + k) % kStreamRing)] == cnt) {
@@ -737,6 +875,33 @@ __global__ void k_stream_plan(long long* ctl, int cnt, int passes) {
   ctl[2] -= avail;
 }
 
+__global__ void k_stream_plan(long long* ctl, int cnt, int passes) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) stream_plan(ctl, cnt, passes);
+}
+
+// Pixel px's planned passes p0 .. p0 + k - 1 from the ring into the running
+// sum and mean, in pass order (writePixel, utils.cuh:13-35).
+__device__ __forceinline__ void accumulate_planned(const float4* ring, float4* accum, float4* average, long long p0,
+                                                   int k, int first_pass, int npx, int px) {
+  V3 c = v3(0.f);
+  int pass = 0;
+  for (int j = 0; j < k; ++j) {
+    const long long p = p0 + j;
+    pass = first_pass + (int)p;
+    const float4 l = ring[(size_t)(p % kStreamRing) * npx + px];
+    V3 v = v3(l.x, l.y, l.z);
+    if (any_nan(v)) v = v3(0.f);
+    if (pass > 0) {
+      const float4 a = j == 0 ? accum[px] : make_float4(c.x, c.y, c.z, 1.f);
+      v = v + v3(a.x, a.y, a.z);
+    }
+    c = v;
+  }
+  accum[px] = make_float4(c.x, c.y, c.z, 1.f);
+  const float inv = 1.f / (float)(pass + 1);
+  average[px] = make_float4(c.x * inv, c.y * inv, c.z * inv, 1.f);
+}
+
 // Adds the planned passes (writePixel, utils.cuh:13-35) to the group's pixels
 // in pass order, and starts the planned jobs in free slots, queued on queue
 // nxt. Threads cover the group's pixels only when passes are to be added.
@@ -751,26 +916,7 @@ __global__ void k_stream_next(SceneView sc, PathBuffers pb, const long long* ctl
   const int stride = gridDim.x * blockDim.x;
   const int nloop = (work + stride - 1) / stride * stride;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
-    if (k > 0 && i < cnt) {
-      const int px = stream_group_pixel(i, g, K);
-      V3 c = v3(0.f);
-      int pass = 0;
-      for (int j = 0; j < k; ++j) {
-        const long long p = p0 + j;
-        pass = first_pass + (int)p;
-        const float4 l = ring[(size_t)(p % kStreamRing) * npx + px];
-        V3 v = v3(l.x, l.y, l.z);
-        if (any_nan(v)) v = v3(0.f);
-        if (pass > 0) {
-          const float4 a = j == 0 ? accum[px] : make_float4(c.x, c.y, c.z, 1.f);
-          v = v + v3(a.x, a.y, a.z);
-        }
-        c = v;
-      }
-      accum[px] = make_float4(c.x, c.y, c.z, 1.f);
-      const float inv = 1.f / (float)(pass + 1);
-      average[px] = make_float4(c.x * inv, c.y * inv, c.z * inv, 1.f);
-    }
+    if (k > 0 && i < cnt) accumulate_planned(ring, accum, average, p0, k, first_pass, npx, stream_group_pixel(i, g, K));
     const bool go = i < n;
     int slot = -1;
     if (go) {
@@ -816,6 +962,78 @@ __global__ void k_stream_finish(PathBuffers pb, long long* ctl, float4* ring, in
     const int pos = wave_append_ll(&ctl[2], done);
     if (done) pb.free_slots[pos] = slot;
   }
+}
+
+// Compacted streaming: finished paths from the finished list into the ring
+// (as k_stream_finish), the free capacity for the plan, and the new jobs'
+// camera rays appended to wavefront nb after the continuing paths.
+__global__ void k_stream_finish_dense(PathBuffers pb, long long* ctl, float4* ring, int npx) {
+  const int n = pb.counts[kDoneCount];
+  const int stride = gridDim.x * blockDim.x;
+  const int nloop = (n + stride - 1) / stride * stride;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
+    const bool done = i < n;
+    int r = -1;
+    if (done) {
+      r = pb.done_pass[i] % kStreamRing;
+      ring[(size_t)r * npx + pb.done_px[i]] = pb.done_L[i];
+    }
+    unsigned long long left = __ballot(done);
+    while (left) {
+      const int lead = __ffsll((long long)left) - 1;
+      const int r0 = __shfl(r, lead);
+      const unsigned long long same = __ballot(done && r == r0);
+      if (__lane_id() == lead) atomicAdd((unsigned long long*)&ctl[8 + r0], (unsigned long long)__popcll(same));
+      left &= ~same;
+    }
+  }
+}
+
+__global__ void k_stream_plan_dense(long long* ctl, const int* counts, int nxt, int slots, int cnt, int passes) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  ctl[2] = slots - counts[nxt];
+  stream_plan(ctl, cnt, passes);
+}
+
+__global__ void k_stream_next_dense(SceneView sc, PathBuffers nb, const long long* ctl, const float4* ring,
+                                    float4* accum, float4* average, int nxt, int first_pass, int seed_offset, int npx,
+                                    int g, int K, int cnt) {
+  const int k = (int)ctl[7];
+  const long long p0 = ctl[6];
+  const int n = (int)ctl[4];
+  const long long job0 = ctl[3];
+  const int work = max(k > 0 ? cnt : 0, n);
+  const int stride = gridDim.x * blockDim.x;
+  const int nloop = (work + stride - 1) / stride * stride;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
+    if (k > 0 && i < cnt) accumulate_planned(ring, accum, average, p0, k, first_pass, npx, stream_group_pixel(i, g, K));
+    const bool go = i < n;
+    const int q = wave_append(&nb.counts[nxt], go);
+    if (go) {
+      const long long job = job0 + i;
+      const int pass = (int)(job / cnt);
+      const int px = stream_group_pixel((int)(job % cnt), g, K);
+      camera_path(sc, nb, q, px, first_pass + pass + seed_offset);
+      nb.slot_pass[q] = pass;
+      nb.slot_px[q] = px;
+      nb.slot_depth[q] = 0;
+    }
+  }
+}
+
+// The same buffers with the two per-path state copies exchanged.
+static PathBuffers other_state(const PathBuffers& pb) {
+  PathBuffers v = pb;
+  v.ray_o = pb.alt.ray_o;
+  v.ray_d = pb.alt.ray_d;
+  v.beta = pb.alt.beta;
+  v.L = pb.alt.L;
+  v.rng = pb.alt.rng;
+  v.slot_pass = pb.alt.pass;
+  v.slot_px = pb.alt.px;
+  v.slot_depth = pb.alt.depth;
+  v.alt = PathState{pb.ray_o, pb.ray_d, pb.beta, pb.L, pb.rng, pb.slot_pass, pb.slot_px, pb.slot_depth};
+  return v;
 }
 
 int stream_group_slots(int npx, int g, int K) {
@@ -864,9 +1082,10 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
     const int blocks = (std::max(cnt, slots) + 255) / 256;
     hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, streams[g], pbs[g]->counts, 0, 2, 3);
     hipLaunchKernelGGL(k_stream_init, dim3(blocks), dim3(256), 0, streams[g], *pbs[g], c, slots);
-    hipLaunchKernelGGL(k_stream_plan, dim3(1), dim3(64), 0, streams[g], c, cnt, passes);
-    hipLaunchKernelGGL(k_stream_next, dim3(blocks), dim3(256), 0, streams[g], sc, *pbs[g], c, ring, accum, average,
-                       0, cfg.accum_id, cfg.seed_offset, npx, g, K, cnt);
+    hipLaunchKernelGGL(k_stream_plan_dense, dim3(1), dim3(64), 0, streams[g], c, pbs[g]->counts, 0, slots, cnt,
+                       passes);
+    hipLaunchKernelGGL(k_stream_next_dense, dim3(blocks), dim3(256), 0, streams[g], sc, *pbs[g], c, ring, accum,
+                       average, 0, cfg.accum_id, cfg.seed_offset, npx, g, K, cnt);
   }
   HM_TRY(check_launch("stream start", err));
   const long long max_iters = (long long)passes * std::max(1, cfg.max_vertex + 1) + 4 * kBatch;
@@ -876,6 +1095,7 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
       if (!live[g]) continue;
       any = true;
       PathBuffers& pb = *pbs[g];
+      const PathBuffers views[2] = {pb, other_state(pb)};  // state copy 0 / 1 as the current wavefront
       hipStream_t st = streams[g];
       const int cnt = stream_group_slots(npx, g, K);
       const int slots = group_slots(cnt);
@@ -885,9 +1105,11 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
       const bool timed = g == 0 && tm;
       for (int k = 0; k < kBatch; ++k) {
         const int nxt = cur[g] ^ 1;
+        const PathBuffers& A = views[cur[g]];
+        const PathBuffers& B = views[nxt];
         hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, st, pb.counts, nxt, 2, 3);
         if (timed) tic(0);
-        hipLaunchKernelGGL(k_extend<false>, dim3(ext_blocks), dim3(kTraceBlock), 0, st, sc, pb, cur[g],
+        hipLaunchKernelGGL((k_extend<false, true>), dim3(ext_blocks), dim3(kTraceBlock), 0, st, sc, A, cur[g],
                            (unsigned long long*)nullptr);
         if (timed) toc();
         ShadeParams sp;
@@ -899,18 +1121,18 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
         sp.msnn = 0;
         sp.cur = cur[g];
         if (timed) tic(1);
-        hipLaunchKernelGGL(k_shade<true>, dim3(shade_blocks), dim3(256), 0, st, sc, pb, sp, GBufferOut{});
+        hipLaunchKernelGGL(k_shade_dense, dim3(shade_blocks), dim3(256), 0, st, sc, A, B, sp);
         if (timed) toc();
         if (timed) tic(2);
-        hipLaunchKernelGGL(k_shadow<false>, dim3(sh_blocks), dim3(kTraceBlock), 0, st, sc, pb,
+        hipLaunchKernelGGL(k_shadow<false>, dim3(sh_blocks), dim3(kTraceBlock), 0, st, sc, A,
                            (unsigned long long*)nullptr);
-        hipLaunchKernelGGL(k_shadow_resolve, dim3(shade_blocks), dim3(256), 0, st, pb, 0, 0, NrcPassArgs{});
+        hipLaunchKernelGGL(k_shadow_resolve_dense, dim3(shade_blocks), dim3(256), 0, st, A, B);
         if (timed) toc();
         if (timed) tic(3);
         long long* c = ctl + (size_t)g * kStreamCtl;
-        hipLaunchKernelGGL(k_stream_finish, dim3(shade_blocks), dim3(256), 0, st, pb, c, ring, npx);
-        hipLaunchKernelGGL(k_stream_plan, dim3(1), dim3(64), 0, st, c, cnt, passes);
-        hipLaunchKernelGGL(k_stream_next, dim3(shade_blocks), dim3(256), 0, st, sc, pb, c, ring, accum, average,
+        hipLaunchKernelGGL(k_stream_finish_dense, dim3(shade_blocks), dim3(256), 0, st, A, c, ring, npx);
+        hipLaunchKernelGGL(k_stream_plan_dense, dim3(1), dim3(64), 0, st, c, pb.counts, nxt, slots, cnt, passes);
+        hipLaunchKernelGGL(k_stream_next_dense, dim3(shade_blocks), dim3(256), 0, st, sc, B, c, ring, accum, average,
                            nxt, cfg.accum_id, cfg.seed_offset, npx, g, K, cnt);
         if (timed) toc();
         cur[g] = nxt;
