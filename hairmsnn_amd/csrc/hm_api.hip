@@ -173,8 +173,6 @@ int alloc_path_buffers(PathBuffers& pb, std::vector<void*>& allocs, size_t n, bo
   HM_TRY(A((void**)&pb.slot_pass, sizeof(int) * n));
   HM_TRY(A((void**)&pb.slot_px, sizeof(int) * n));
   HM_TRY(A((void**)&pb.slot_depth, sizeof(int) * n));
-  HM_TRY(A((void**)&pb.done_queue, sizeof(int) * n));
-  HM_TRY(A((void**)&pb.free_slots, sizeof(int) * n));
   HM_TRY(A((void**)&pb.alt.ray_o, f4));
   HM_TRY(A((void**)&pb.alt.ray_d, f4));
   HM_TRY(A((void**)&pb.alt.beta, f4));

@@ -87,13 +87,11 @@ struct PathBuffers {
   int* shadow_queue;  // slots with pending direct light
   int* shadow_rays;   // pending shadow rays: slot * 2 + (0: light sample, 1: BSDF sample)
   uint8_t* sh_vis;    // visibility per (slot, ray), same indexing as shadow_rays
-  // streaming schedule: per slot the job's pass (relative to the call's
-  // first) and pixel and the vertex depth; finished slots; free slots
+  // streaming schedule: per path the job's pass (relative to the call's
+  // first) and pixel and the vertex depth
   int* slot_pass;
   int* slot_px;
   int* slot_depth;
-  int* done_queue;
-  int* free_slots;
   // compacted streaming: the other state copy, where each path of this
   // wavefront went (>= 0: position in the next wavefront, < 0: ~position in
   // the finished list) and the finished paths' radiance, pixel and pass

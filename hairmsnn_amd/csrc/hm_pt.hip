@@ -372,12 +372,9 @@ __global__ void HM_TRACE_BOUNDS k_extend(SceneView sc, PathBuffers pb, int cur,
   if (kCount) flush_counters(cnt, &stats[0], &stats[1], &stats[4]);
 }
 
-// kStream: the depth is per slot (slot_depth), and terminated paths are
-// appended to done_queue for k_stream_finish.
 #ifndef HM_SHADE_MIN_WAVES
 #define HM_SHADE_MIN_WAVES 1
 #endif
-template <bool kStream>
 __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc, PathBuffers pb, ShadeParams sp,
                                                                    GBufferOut gb) {
   const int nxt = sp.cur ^ 1;
@@ -388,10 +385,9 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc,
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
     bool cont = false, shadow = false, ray1 = false, ray2 = false;
     int slot = -1;
-    bool done = false;
     if (i < n) {
       slot = q[i];
-      const int depth = kStream ? pb.slot_depth[slot] : sp.depth;
+      const int depth = sp.depth;
       const int prim = pb.hit_prim[slot];
       const float4 o4 = pb.ray_o[slot], d4 = pb.ray_d[slot];
       const V3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
@@ -476,14 +472,6 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc,
         }
         pb.rng[slot] = rng;
       }
-      if (kStream) {
-        if (cont) pb.slot_depth[slot] = depth + 1;
-        done = !cont;
-      }
-    }
-    if (kStream) {
-      const int dpos = wave_append(&pb.counts[kDoneCount], done);
-      if (done) pb.done_queue[dpos] = slot;
     }
     const int qpos = wave_append(&pb.counts[nxt], cont);
     if (cont) pb.queue[nxt][qpos] = slot;
@@ -498,8 +486,8 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc,
 }
 
 // Compacted streaming vertex (plain path tracing only): path i of wavefront
-// pb is shaded as k_shade<true> shades a slot (the same operations in the same
-// order), then a continuing path is written densely into wavefront nb and a
+// pb is shaded as k_shade shades a slot of a plain path-tracing pass (the
+// same operations in the same order), then a continuing path is written densely into wavefront nb and a
 // finished one into the finished list; pb.dest[i] records where it went, for
 // k_shadow_resolve_dense. Shadow records stay at index i.
 __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade_dense(SceneView sc, PathBuffers pb,
@@ -809,7 +797,7 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
     if (cfg.nrc.enabled)
       hipLaunchKernelGGL(k_shade_nrc, dim3(shade_blocks), dim3(256), 0, stream, sc, pb, depth, cur, cfg.nrc);
     else
-      hipLaunchKernelGGL(k_shade<false>, dim3(shade_blocks), dim3(256), 0, stream, sc, pb, sp, cfg.gb);
+      hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(256), 0, stream, sc, pb, sp, cfg.gb);
     toc();
     tic(2);
     if (count)
@@ -847,11 +835,9 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
 // ctl (64-bit, per group): [0] next job, [1] passes accumulated, [2] free
 // slots, [3..5] issue plan (first job, count, first free-stack index), [6..7]
 // accumulation plan (first pass, count), [8 + r] finished jobs of ring entry r.
-__global__ void k_stream_init(PathBuffers pb, long long* ctl, int cnt) {
+__global__ void k_stream_init(long long* ctl) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 8 + kStreamRing) ctl[i] = 0;
-  if (i == 2) ctl[2] = cnt;
-  if (i < cnt) pb.free_slots[i] = i;
+  if (i < kStreamCtl) ctl[i] = 0;
 }
 
 // Single thread: accumulation plan (passes whose jobs are all done, at most
@@ -900,68 +886,6 @@ __device__ __forceinline__ void accumulate_planned(const float4* ring, float4* a
   accum[px] = make_float4(c.x, c.y, c.z, 1.f);
   const float inv = 1.f / (float)(pass + 1);
   average[px] = make_float4(c.x * inv, c.y * inv, c.z * inv, 1.f);
-}
-
-// Adds the planned passes (writePixel, utils.cuh:13-35) to the group's pixels
-// in pass order, and starts the planned jobs in free slots, queued on queue
-// nxt. Threads cover the group's pixels only when passes are to be added.
-__global__ void k_stream_next(SceneView sc, PathBuffers pb, const long long* ctl, const float4* ring, float4* accum,
-                              float4* average, int nxt, int first_pass, int seed_offset, int npx, int g, int K,
-                              int cnt) {
-  const int k = (int)ctl[7];
-  const long long p0 = ctl[6];
-  const int n = (int)ctl[4];
-  const long long job0 = ctl[3], top = ctl[5];
-  const int work = max(k > 0 ? cnt : 0, n);
-  const int stride = gridDim.x * blockDim.x;
-  const int nloop = (work + stride - 1) / stride * stride;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
-    if (k > 0 && i < cnt) accumulate_planned(ring, accum, average, p0, k, first_pass, npx, stream_group_pixel(i, g, K));
-    const bool go = i < n;
-    int slot = -1;
-    if (go) {
-      slot = pb.free_slots[top + i];
-      const long long job = job0 + i;
-      const int pass = (int)(job / cnt);
-      const int px = stream_group_pixel((int)(job % cnt), g, K);
-      camera_path(sc, pb, slot, px, first_pass + pass + seed_offset);
-      pb.flags[slot] = 0u;
-      pb.slot_pass[slot] = pass;
-      pb.slot_px[slot] = px;
-      pb.slot_depth[slot] = 0;
-    }
-    const int q = wave_append(&pb.counts[nxt], go);
-    if (go) pb.queue[nxt][q] = slot;
-  }
-}
-
-// Finished paths: radiance into the ring, one more finished job for the
-// pass's ring entry, the slot back on the free stack.
-__global__ void k_stream_finish(PathBuffers pb, long long* ctl, float4* ring, int npx) {
-  const int n = pb.counts[kDoneCount];
-  const int stride = gridDim.x * blockDim.x;
-  const int nloop = (n + stride - 1) / stride * stride;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
-    const bool done = i < n;
-    int slot = -1, r = -1;
-    if (done) {
-      slot = pb.done_queue[i];
-      r = pb.slot_pass[slot] % kStreamRing;
-      ring[(size_t)r * npx + pb.slot_px[slot]] = pb.L[slot];
-    }
-    // one atomic per distinct ring entry in the wave (a wave's jobs span a
-    // few passes at most)
-    unsigned long long left = __ballot(done);
-    while (left) {
-      const int lead = __ffsll((long long)left) - 1;
-      const int r0 = __shfl(r, lead);
-      const unsigned long long same = __ballot(done && r == r0);
-      if (__lane_id() == lead) atomicAdd((unsigned long long*)&ctl[8 + r0], (unsigned long long)__popcll(same));
-      left &= ~same;
-    }
-    const int pos = wave_append_ll(&ctl[2], done);
-    if (done) pb.free_slots[pos] = slot;
-  }
 }
 
 // Compacted streaming: finished paths from the finished list into the ring
@@ -1081,7 +1005,7 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
     const int slots = group_slots(cnt);
     const int blocks = (std::max(cnt, slots) + 255) / 256;
     hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, streams[g], pbs[g]->counts, 0, 2, 3);
-    hipLaunchKernelGGL(k_stream_init, dim3(blocks), dim3(256), 0, streams[g], *pbs[g], c, slots);
+    hipLaunchKernelGGL(k_stream_init, dim3(1), dim3(kStreamCtl), 0, streams[g], c);
     hipLaunchKernelGGL(k_stream_plan_dense, dim3(1), dim3(64), 0, streams[g], c, pbs[g]->counts, 0, slots, cnt,
                        passes);
     hipLaunchKernelGGL(k_stream_next_dense, dim3(blocks), dim3(256), 0, streams[g], sc, *pbs[g], c, ring, accum,
