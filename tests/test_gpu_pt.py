@@ -134,12 +134,15 @@ def test_converges_to_fp64_oracle(gpu):
     assert abs(gm - om) < 4 * se_m
 
 
-@pytest.mark.parametrize("name,groups", [("c0_hairball.json", 1), ("c0_hairball.json", 3), ("c1_curly.json", 4)])
-def test_streaming_schedule_equals_pass_schedule(gpu, name, groups):
-    """The streaming schedule (a pixel's next pass starts in its slot when its
-    path ends; pixels dealt to `groups` streams in 64-pixel chunks)
-    accumulates each pixel's passes in pass order, so its image is the
-    per-pass schedule's bit for bit, across calls and seed offsets."""
+@pytest.mark.parametrize("name,groups,calls", [("c0_hairball.json", 1, (5, 3)), ("c0_hairball.json", 3, (5, 3)),
+                                               ("c1_curly.json", 4, (5, 3)), ("c0_hairball.json", 4, (40, 19))])
+def test_streaming_schedule_equals_pass_schedule(gpu, name, groups, calls):
+    """The streaming schedule (slots take (pass, pixel) jobs in pass order over
+    a compacted wavefront, two paths in flight per pixel of each of `groups`
+    streams; finished radiance waits in a 16-pass ring until its pass is
+    complete) accumulates each pixel's passes in pass order, so its image is
+    the per-pass schedule's bit for bit, across calls, seed offsets and calls
+    longer than the ring."""
     cfg = scene_cfg(name, 64, 64)
     imgs, lasts = {}, {}
     for sched in ("passes", "stream"):
@@ -148,8 +151,8 @@ def test_streaming_schedule_equals_pass_schedule(gpu, name, groups):
             sc.set_pass_concurrency(groups)
             sc.set_seed_offset(3)
             sc.pt_reset()
-            sc.pt_passes(5)
-            sc.pt_passes(3)
+            for n in calls:
+                sc.pt_passes(n)
             imgs[sched] = sc.image(0).copy()
             lasts[sched] = sc.last_pass().copy()
     np.testing.assert_array_equal(imgs["stream"], imgs["passes"])
