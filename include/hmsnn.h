@@ -162,17 +162,22 @@ int hmsnn_set_strand_sigma_a(hmsnn_ctx* ctx, const float* sigma_host, int num_st
  * Default 0 reproduces the reference's streams. No reference counterpart:
  * it lets a reference image and a test render use independent samples. */
 int hmsnn_set_seed_offset(hmsnn_ctx* ctx, int offset);
-/* Path-tracing passes kept in flight at once (1..4, default 4), each with its
- * own path buffers and stream; accumulation stays in pass order, so the image
- * does not depend on it. 1 runs passes back to back (per-kernel timings of
- * hmsnn_pt_time_passes are then those of kernels running alone). No
- * reference counterpart: the reference renders one launch per frame. */
+/* Concurrency of path tracing (1..4, default 4): with the per-pass schedule
+ * the passes kept in flight at once, with the streaming schedule the stream
+ * groups the pixels are dealt to; each has its own path buffers and stream.
+ * Accumulation stays in pass order, so the image does not depend on it. 1
+ * runs everything on one stream (per-kernel timings of hmsnn_pt_time_passes
+ * are then those of kernels running alone). No reference counterpart: the
+ * reference renders one launch per frame. */
 int hmsnn_set_pass_concurrency(hmsnn_ctx* ctx, int passes_in_flight);
-/* Schedule of hmsnn_pt_passes. HMSNN_PT_STREAM (default): every pixel keeps
- * one path in flight and starts its next pass as soon as the last one ends,
- * so each wavefront launch carries about one ray per pixel. HMSNN_PT_PASSES:
- * one wavefront per pass, passes in flight per hmsnn_set_pass_concurrency.
- * Both accumulate each pixel's passes in pass order: the images are equal.
+/* Schedule of hmsnn_pt_passes. HMSNN_PT_STREAM (default): (pass, pixel)
+ * jobs are handed out in pass order to a compacted wavefront of two paths per
+ * pixel; a finished path's radiance waits in a 16-pass ring until its pass is
+ * complete, so each wavefront launch stays full until the call's last jobs
+ * drain. hmsnn_set_pass_concurrency sets the number of stream groups the
+ * pixels are dealt to. HMSNN_PT_PASSES: one wavefront per pass, passes in
+ * flight per hmsnn_set_pass_concurrency. Both accumulate each pixel's passes
+ * in pass order: the images are equal.
  * No reference counterpart (the reference launches one megakernel per pass). */
 typedef enum hmsnn_pt_schedule { HMSNN_PT_STREAM = 0, HMSNN_PT_PASSES = 1 } hmsnn_pt_schedule;
 int hmsnn_set_pt_schedule(hmsnn_ctx* ctx, int schedule);
