@@ -32,6 +32,7 @@ struct AdjArgs {
   float inv_spp;
   int pass0, npass;
   int dl_zero, dl_from, max_vertex;
+  const float4* ring;  // primal radiance of pass pass0 + p at ring[p * npx + pixel]
   double* g_sigma;  // num_strands * 3
   double* g_glob;   // 3
   float* g_px;      // optional: w*h*3 per-pixel d/d(beta_m, beta_n, alpha)
@@ -72,10 +73,9 @@ HM_D V3 safe_ratio(V3 c, V3 f) {
   return v3(f.x != 0.f ? c.x / f.x : 0.f, f.y != 0.f ? c.y / f.y : 0.f, f.z != 0.f ? c.z / f.z : 0.f);
 }
 
-// One path of the integrator (the order of k_camera / k_shade / k_shadow):
-// kReplay false returns the path's radiance; kReplay true also adds the
-// gradient given the primal radiance L_total.
-template <bool kReplay>
+// One path of the integrator (the order of k_camera / k_shade / k_shadow),
+// replayed from its seed: adds the gradient given the path's primal radiance
+// L_total, and returns the radiance the replay gathered.
 HM_D V3 adjoint_path(const SceneView& sc, const AdjArgs& a, TraceStack& st, int x, int y, int pass, V3 L_total,
                      V3 adj, float* gg) {
   TraceCounters cnt{0, 0, 0};
@@ -103,7 +103,7 @@ HM_D V3 adjoint_path(const SceneView& sc, const AdjArgs& a, TraceStack& st, int 
       if (v2) sum = sum + dl.c2;
       const bool bad = any_nan(sum);
       if (bad) sum = v3(0.f);
-      if (kReplay && hair && !bad) {
+      if (hair && !bad) {
         const V3 sigma = strand_sigma_of(sc, prim);
         const V3 W = adj * beta;  // dLoss / dc per channel
         float r[kAdParams];
@@ -138,7 +138,7 @@ HM_D V3 adjoint_path(const SceneView& sc, const AdjArgs& a, TraceStack& st, int 
     V3 mul = (pdf == 0.f) ? f : f / pdf;
     const bool mul_bad = any_nan(mul);
     if (mul_bad) mul = v3(1.f);
-    if (kReplay && hair && !mul_bad) {
+    if (hair && !mul_bad) {
       const V3 rem = L_total - L;  // radiance gathered after this vertex
       Dual fd[3], pd;
       hair_eval_dual(a.hcd, h.wo_local, wl, h.hw.h, strand_sigma_of(sc, prim), fd, &pd);
@@ -179,10 +179,11 @@ __global__ void __launch_bounds__(kTraceBlock) k_adjoint(SceneView sc, AdjArgs a
     const float4 g4 = a.dl_dimg[pix];
     const V3 adj = a.inv_spp * v3(g4.x, g4.y, g4.z);
     if (adj.x == 0.f && adj.y == 0.f && adj.z == 0.f) continue;
-    const V3 Lt = adjoint_path<false>(sc, a, st, x, y, pass, v3(0.f), adj, gg);
+    const float4 l = a.ring[i];  // i = (pass - pass0) * npx + pix
+    const V3 Lt = v3(l.x, l.y, l.z);
     if (any_nan(Lt)) continue;  // writePixel drops NaN samples (utils.cuh:20-21)
     float gp[3] = {0.f, 0.f, 0.f};
-    adjoint_path<true>(sc, a, st, x, y, pass, Lt, adj, gp);
+    adjoint_path(sc, a, st, x, y, pass, Lt, adj, gp);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       gg[k] += gp[k];
@@ -262,8 +263,6 @@ int hmsnn_adjoint_render(hmsnn_ctx* ctx, int spp, const float* dl_dimg, double* 
   a.hcd = make_hair_consts_ad(c.scene.beta_m, c.scene.beta_n, c.scene.alpha, gains);
   a.dl_dimg = d_adj;
   a.inv_spp = 1.f / (float)spp;
-  a.pass0 = c.seed_offset;
-  a.npass = spp;
   const PassConfig pc = c.pt_pass_config();
   a.dl_zero = pc.dl_zero;
   a.dl_from = pc.dl_from;
@@ -274,8 +273,24 @@ int hmsnn_adjoint_render(hmsnn_ctx* ctx, int spp, const float* dl_dimg, double* 
   a.spill = c.pb.spill;
   a.spill_stride = c.pb.spill_stride;
   a.spill_levels = c.pb.spill_levels;
-  hipLaunchKernelGGL(k_adjoint, dim3(c.trace_blocks), dim3(kTraceBlock), 0, c.stream, c.view, a);
-  rc = check_launch("adjoint", err);
+  // The primal radiance of every path comes from the wavefront path tracer
+  // (the same paths: same seeds, same operations), kStreamRing passes at a
+  // time into the context's radiance ring; the replay kernel then only runs
+  // the adjoint pass over them.
+  float4* tmp_acc = nullptr;
+  rc = dev_alloc((void**)&tmp_acc, 2 * sizeof(float4) * (size_t)npx, err);
+  for (int p0 = 0; !rc && p0 < spp; p0 += kStreamRing) {
+    const int np = std::min(kStreamRing, spp - p0);
+    rc = stream_passes(c, p0, np, tmp_acc, tmp_acc + npx, nullptr, err);
+    if (rc) break;
+    a.ring = c.stream_ring;
+    a.pass0 = c.seed_offset + p0;
+    a.npass = np;
+    hipLaunchKernelGGL(k_adjoint, dim3(c.trace_blocks), dim3(kTraceBlock), 0, c.stream, c.view, a);
+    rc = check_launch("adjoint", err);
+  }
+  (void)hipStreamSynchronize(c.stream);
+  dev_free(tmp_acc);
   if (!rc) {
     double gg[4];
     e = hipMemcpyAsync(gg, d_gg, sizeof(gg), hipMemcpyDeviceToHost, c.stream);

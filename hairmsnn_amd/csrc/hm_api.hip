@@ -679,6 +679,58 @@ int hmsnn_pt_reset(hmsnn_ctx* ctx) {
   return HMSNN_OK;
 }
 
+}  // extern "C"
+
+namespace hm {
+
+// Streaming schedule: passes first_pass .. first_pass + passes - 1 of every
+// pixel accumulated into acc / avg (the context's image, or scratch buffers of
+// a caller that only wants the radiance ring); the context stream waits for
+// every group. The ring then holds the last (up to) 16 passes' radiance.
+int stream_passes(Context& c, int first_pass, int passes, float4* acc, float4* avg, PassTimer* timer,
+                  std::string& err) {
+  HM_TRY(c.ensure_path_buffers(false, err));
+  HM_TRY(c.ensure_flights(err));
+  const int n = c.scene.width * c.scene.height;
+  const int K = std::max(1, std::min(c.num_flights, Context::kMaxFlights));
+  if (passes <= 0) return kOk;
+  if (!c.live_host) {
+    HM_HIP(hipHostMalloc((void**)&c.live_host, 2 * Context::kMaxFlights * sizeof(int)), err);
+    for (hipEvent_t& e : c.live_ev) HM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming), err);
+  }
+  if (!c.stream_ring) {
+    HM_TRY(dev_alloc((void**)&c.stream_ring, sizeof(float4) * kStreamRing * (size_t)n, err));
+    HM_TRY(dev_alloc((void**)&c.stream_ctl, sizeof(long long) * kStreamCtl * Context::kMaxFlights, err));
+  }
+  // every group's stream starts after the work queued so far (including
+  // the other schedule's last accumulation)
+  if (c.acc_pending) HM_HIP(hipStreamWaitEvent(c.stream, c.acc_event, 0), err);
+  c.acc_pending = false;
+  HM_HIP(hipEventRecord(c.acc_event, c.stream), err);
+  PathBuffers* pbs[Context::kMaxFlights];
+  hipStream_t sts[Context::kMaxFlights];
+  for (int g = 0; g < K; ++g) {
+    pbs[g] = &c.flight_pb(g);
+    sts[g] = c.flight_stream(g);
+    if (g > 0) HM_HIP(hipStreamWaitEvent(sts[g], c.acc_event, 0), err);
+  }
+  PassConfig cfg = c.pt_pass_config();
+  cfg.accum_id = first_pass;
+  cfg.timer = timer;
+  HM_TRY(pt_run_stream(c.view, pbs, sts, K, cfg, passes, acc, avg, c.stream_ring, c.stream_ctl, c.live_host,
+                       c.live_ev, err));
+  // the context stream waits for every group
+  for (int g = 1; g < K; ++g) {
+    HM_HIP(hipEventRecord(c.acc_event, sts[g]), err);
+    HM_HIP(hipStreamWaitEvent(c.stream, c.acc_event, 0), err);
+  }
+  return kOk;
+}
+
+}  // namespace hm
+
+extern "C" {
+
 // Runs passes round-robin over the flights. Every flight first waits for
 // the work already queued on the context stream; accumulations are chained
 // through acc_event so they happen in pass order (the image is the one a
@@ -692,36 +744,8 @@ static int run_pt_passes(Context& c, int passes, std::string& err, PassTimer* ti
   const int K = std::max(1, std::min(c.num_flights, Context::kMaxFlights));
   if (c.pt_schedule == 0) {
     if (passes <= 0) return kOk;
-    if (!c.live_host) {
-      HM_HIP(hipHostMalloc((void**)&c.live_host, 2 * Context::kMaxFlights * sizeof(int)), err);
-      for (hipEvent_t& e : c.live_ev) HM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming), err);
-    }
-    if (!c.stream_ring) {
-      HM_TRY(dev_alloc((void**)&c.stream_ring, sizeof(float4) * kStreamRing * (size_t)n, err));
-      HM_TRY(dev_alloc((void**)&c.stream_ctl, sizeof(long long) * kStreamCtl * Context::kMaxFlights, err));
-    }
-    // every group's stream starts after the work queued so far (including
-    // the other schedule's last accumulation)
-    if (c.acc_pending) HM_HIP(hipStreamWaitEvent(c.stream, c.acc_event, 0), err);
-    c.acc_pending = false;
-    HM_HIP(hipEventRecord(c.acc_event, c.stream), err);
-    PathBuffers* pbs[Context::kMaxFlights];
-    hipStream_t sts[Context::kMaxFlights];
-    for (int g = 0; g < K; ++g) {
-      pbs[g] = &c.flight_pb(g);
-      sts[g] = c.flight_stream(g);
-      if (g > 0) HM_HIP(hipStreamWaitEvent(sts[g], c.acc_event, 0), err);
-    }
-    PassConfig cfg = c.pt_pass_config();
-    cfg.timer = timer;
-    HM_TRY(pt_run_stream(v, pbs, sts, K, cfg, passes, c.accum, c.average, c.stream_ring, c.stream_ctl, c.live_host,
-                         c.live_ev, err));
+    HM_TRY(stream_passes(c, c.accum_id, passes, c.accum, c.average, timer, err));
     c.last_ring_entry = (passes - 1) % kStreamRing;
-    // the context stream waits for every group
-    for (int g = 1; g < K; ++g) {
-      HM_HIP(hipEventRecord(c.acc_event, sts[g]), err);
-      HM_HIP(hipStreamWaitEvent(c.stream, c.acc_event, 0), err);
-    }
     c.accum_id += passes;
     c.last_streamed = true;
     c.stream_groups = K;

@@ -110,4 +110,10 @@ struct Context {
   int stack_levels() const;
 };
 
+// Streaming-schedule passes first_pass .. first_pass + passes - 1 into acc /
+// avg; afterwards the context stream is ordered after them and c.stream_ring
+// holds the radiance of the last (up to) kStreamRing passes by pass - first_pass.
+int stream_passes(Context& c, int first_pass, int passes, float4* acc, float4* avg, PassTimer* timer,
+                  std::string& err);
+
 }  // namespace hm
