@@ -812,28 +812,25 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
   return kOk;
 }
 
-// ---- streaming schedule (path regeneration) ----------------------------------
-// Every pixel keeps one path in flight; when it ends, k_stream_finish adds
-// its radiance to the pixel's running sum (pass order per pixel, so the image
-// equals the one consecutive passes make) and starts the pixel's next pass in
-// the same slot. Each wavefront launch then carries about one ray per pixel
-// instead of a pass's shrinking tail. The pixels are split into K groups,
-// each with its own path buffers and stream, so that one group's shading
-// overlaps another's traversal: group g owns the 64-pixel chunks g, g + K,
-// g + 2K, ... (balanced work, coherent camera rays within a wave).
-
-// Job scheduler of the streaming schedule, one per stream group. A job is
-// (pass, pixel): job j of a call is pass j / cnt of the group's pixel j % cnt,
-// so jobs are handed out in pass order. A slot whose path ends takes the next
-// job, whichever pixel it belongs to, so a pixel's passes can run at the same
-// time in different slots and no slot idles while jobs remain. Each finished
-// job leaves its radiance in a ring of kStreamRing passes (indexed by the
-// global pixel); a pass is added to the image once all its jobs are done, in
-// pass order, so the accumulation (and the image) is the per-pass schedule's.
-// Jobs of pass p start only while p < (accumulated passes) + kStreamRing, so
-// the ring never overwrites a pass that is not accumulated yet.
+// ---- streaming schedule ------------------------------------------------------
+// A job queue over a compacted wavefront. A job is (pass, pixel): job j of a
+// call is pass j / cnt of the group's pixel j % cnt, so jobs are handed out in
+// pass order, and a pixel's passes can run at the same time on different
+// paths. Each iteration: extend the current wavefront (path i at index i),
+// shade it (continuing paths written densely into the other copy of the
+// per-path state, finished ones into a finished list), trace and resolve the
+// shadow rays, move the finished radiance into a ring of kStreamRing passes
+// (indexed by the global pixel), plan, then add the passes whose jobs are all
+// done to the image (in pass order, so the accumulation and the image are the
+// per-pass schedule's) and append new jobs' camera rays after the continuing
+// paths, up to the group's path capacity. Jobs of pass p start only while
+// p < (accumulated passes) + kStreamRing, so the ring never overwrites a pass
+// that is not accumulated yet. The pixels are split into K groups, each with
+// its own path buffers and stream, so that one group's shading overlaps
+// another's traversal: group g owns the 64-pixel chunks g, g + K, g + 2K, ...
+// (balanced work, coherent camera rays within a wave).
 // ctl (64-bit, per group): [0] next job, [1] passes accumulated, [2] free
-// slots, [3..5] issue plan (first job, count, first free-stack index), [6..7]
+// path capacity, [3..5] issue plan (first job, count, unused), [6..7]
 // accumulation plan (first pass, count), [8 + r] finished jobs of ring entry r.
 __global__ void k_stream_init(long long* ctl) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -841,7 +838,7 @@ __global__ void k_stream_init(long long* ctl) {
 }
 
 // Single thread: accumulation plan (passes whose jobs are all done, at most
-// a ring's worth), then the issue plan (free slots x jobs inside the window).
+// a ring's worth), then the issue plan (free capacity x jobs inside the window).
 __device__ void stream_plan(long long* ctl, int cnt, int passes) {
   const long long acc = ctl[1];
   int k = 0;
@@ -861,9 +858,6 @@ __device__ void stream_plan(long long* ctl, int cnt, int passes) {
   ctl[2] -= avail;
 }
 
-__global__ void k_stream_plan(long long* ctl, int cnt, int passes) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) stream_plan(ctl, cnt, passes);
-}
 
 // Pixel px's planned passes p0 .. p0 + k - 1 from the ring into the running
 // sum and mean, in pass order (writePixel, utils.cuh:13-35).
@@ -888,9 +882,9 @@ __device__ __forceinline__ void accumulate_planned(const float4* ring, float4* a
   average[px] = make_float4(c.x * inv, c.y * inv, c.z * inv, 1.f);
 }
 
-// Compacted streaming: finished paths from the finished list into the ring
-// (as k_stream_finish), the free capacity for the plan, and the new jobs'
-// camera rays appended to wavefront nb after the continuing paths.
+// Finished paths from the finished list into the ring, counted per ring
+// entry; the free capacity for the plan; the new jobs' camera rays appended
+// to wavefront nb after the continuing paths.
 __global__ void k_stream_finish_dense(PathBuffers pb, long long* ctl, float4* ring, int npx) {
   const int n = pb.counts[kDoneCount];
   const int stride = gridDim.x * blockDim.x;
