@@ -829,35 +829,14 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
 // its own path buffers and stream, so that one group's shading overlaps
 // another's traversal: group g owns the 64-pixel chunks g, g + K, g + 2K, ...
 // (balanced work, coherent camera rays within a wave).
-// ctl (64-bit, per group): [0] next job, [1] passes accumulated, [2] free
-// path capacity, [3..5] issue plan (first job, count, unused), [6..7]
-// accumulation plan (first pass, count), [8 + r] finished jobs of ring entry r.
-__global__ void k_stream_init(long long* ctl) {
+// ctl (64-bit, per group): two copies of the scheduler state, each [0] next
+// job, [1] passes accumulated, [2] free path capacity (set by the finishing
+// kernel), [8 + r] finished jobs of ring entry r.
+__global__ void k_stream_init(long long* ctl, int* counts, int slots) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < kStreamCtl) ctl[i] = 0;
+  if (i < kStreamCtl) ctl[i] = i == 2 ? slots : 0;
+  if (i <= kDoneCount) counts[i] = 0;
 }
-
-// Single thread: accumulation plan (passes whose jobs are all done, at most
-// a ring's worth), then the issue plan (free capacity x jobs inside the window).
-__device__ void stream_plan(long long* ctl, int cnt, int passes) {
-  const long long acc = ctl[1];
-  int k = 0;
-  while (acc + k < passes && k < kStreamRing && ctl[8 + (int)((acc + k) % kStreamRing)] == cnt) {
-    ctl[8 + (int)((acc + k) % kStreamRing)] = 0;
-    ++k;
-  }
-  ctl[6] = acc;
-  ctl[7] = k;
-  ctl[1] = acc + k;
-  const long long limit = (long long)min((long long)passes, acc + k + kStreamRing) * cnt;
-  const long long avail = max(0ll, min(ctl[2], limit - ctl[0]));
-  ctl[3] = ctl[0];
-  ctl[4] = avail;
-  ctl[5] = ctl[2] - avail;
-  ctl[0] += avail;
-  ctl[2] -= avail;
-}
-
 
 // Pixel px's planned passes p0 .. p0 + k - 1 from the ring into the running
 // sum and mean, in pass order (writePixel, utils.cuh:13-35).
@@ -885,8 +864,9 @@ __device__ __forceinline__ void accumulate_planned(const float4* ring, float4* a
 // Finished paths from the finished list into the ring, counted per ring
 // entry; the free capacity for the plan; the new jobs' camera rays appended
 // to wavefront nb after the continuing paths.
-__global__ void k_stream_finish_dense(PathBuffers pb, long long* ctl, float4* ring, int npx) {
+__global__ void k_stream_finish_dense(PathBuffers pb, long long* ctl, float4* ring, int npx, int nxt, int slots) {
   const int n = pb.counts[kDoneCount];
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl[2] = slots - pb.counts[nxt];  // free path capacity
   const int stride = gridDim.x * blockDim.x;
   const int nloop = (n + stride - 1) / stride * stride;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
@@ -907,24 +887,39 @@ __global__ void k_stream_finish_dense(PathBuffers pb, long long* ctl, float4* ri
   }
 }
 
-__global__ void k_stream_plan_dense(long long* ctl, const int* counts, int nxt, int slots, int cnt, int passes) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  ctl[2] = slots - counts[nxt];
-  stream_plan(ctl, cnt, passes);
-}
 
-__global__ void k_stream_next_dense(SceneView sc, PathBuffers nb, const long long* ctl, const float4* ring,
-                                    float4* accum, float4* average, int nxt, int first_pass, int seed_offset, int npx,
-                                    int g, int K, int cnt) {
-  const int k = (int)ctl[7];
-  const long long p0 = ctl[6];
-  const int n = (int)ctl[4];
-  const long long job0 = ctl[3];
+// Plans and runs the next step of a group from scheduler state S (written
+// by the previous call and updated by the finishing kernel in between): every
+// thread derives the same plan from S (passes whose jobs are all done, at
+// most a ring's worth, are accumulated; free capacity x jobs inside the
+// window are issued), one thread writes the state after it to T and clears
+// the iteration's counters. Then the completed passes are added to the image
+// and the new jobs' camera rays appended to wavefront nb.
+__global__ void k_stream_next_dense(SceneView sc, PathBuffers nb, const long long* S, long long* T, const float4* ring,
+                                    float4* accum, float4* average, int cur, int nxt, int first_pass, int seed_offset,
+                                    int npx, int g, int K, int cnt, int passes) {
+  const long long acc = S[1], job0 = S[0];
+  int k = 0;
+  while (acc + k < passes && k < kStreamRing && S[8 + (int)((acc + k) % kStreamRing)] == cnt) ++k;
+  const long long limit = (long long)min((long long)passes, acc + k + kStreamRing) * cnt;
+  const int n = (int)max(0ll, min(S[2], limit - job0));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    T[0] = job0 + n;
+    T[1] = acc + k;
+    for (int r = 0; r < kStreamRing; ++r) {
+      const long long d = (r - acc % kStreamRing + kStreamRing) % kStreamRing;  // ring entry r holds pass acc + d
+      T[8 + r] = d < k ? 0 : S[8 + r];
+    }
+    nb.counts[cur] = 0;
+    nb.counts[2] = 0;
+    nb.counts[3] = 0;
+    nb.counts[kDoneCount] = 0;
+  }
   const int work = max(k > 0 ? cnt : 0, n);
   const int stride = gridDim.x * blockDim.x;
   const int nloop = (work + stride - 1) / stride * stride;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
-    if (k > 0 && i < cnt) accumulate_planned(ring, accum, average, p0, k, first_pass, npx, stream_group_pixel(i, g, K));
+    if (k > 0 && i < cnt) accumulate_planned(ring, accum, average, acc, k, first_pass, npx, stream_group_pixel(i, g, K));
     const bool go = i < n;
     const int q = wave_append(&nb.counts[nxt], go);
     if (go) {
@@ -984,7 +979,7 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
 #define HM_STREAM_SLOT_FACTOR 2
 #endif
   constexpr int kBatch = 8;  // even: each group's live queue is back at index 0 after a batch
-  std::vector<int> cur(K, 0), live(K, 0);
+  std::vector<int> cur(K, 0), live(K, 0), calls(K, 0);
   std::vector<long long> iters(K, 0);
   // paths in flight per group: HM_STREAM_SLOT_FACTOR per pixel of the group,
   // at most the npx slots its path buffers hold (jobs are still the group's
@@ -998,12 +993,10 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
     long long* c = ctl + (size_t)g * kStreamCtl;
     const int slots = group_slots(cnt);
     const int blocks = (std::max(cnt, slots) + 255) / 256;
-    hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, streams[g], pbs[g]->counts, 0, 2, 3);
-    hipLaunchKernelGGL(k_stream_init, dim3(1), dim3(kStreamCtl), 0, streams[g], c);
-    hipLaunchKernelGGL(k_stream_plan_dense, dim3(1), dim3(64), 0, streams[g], c, pbs[g]->counts, 0, slots, cnt,
-                       passes);
-    hipLaunchKernelGGL(k_stream_next_dense, dim3(blocks), dim3(256), 0, streams[g], sc, *pbs[g], c, ring, accum,
-                       average, 0, cfg.accum_id, cfg.seed_offset, npx, g, K, cnt);
+    hipLaunchKernelGGL(k_stream_init, dim3(1), dim3(kStreamCtl), 0, streams[g], c, pbs[g]->counts, slots);
+    hipLaunchKernelGGL(k_stream_next_dense, dim3(blocks), dim3(256), 0, streams[g], sc, *pbs[g], c, c + kStreamHalf,
+                       ring, accum, average, 1, 0, cfg.accum_id, cfg.seed_offset, npx, g, K, cnt, passes);
+    calls[g] = 1;
   }
   HM_TRY(check_launch("stream start", err));
   const long long max_iters = (long long)passes * std::max(1, cfg.max_vertex + 1) + 4 * kBatch;
@@ -1025,7 +1018,6 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
         const int nxt = cur[g] ^ 1;
         const PathBuffers& A = views[cur[g]];
         const PathBuffers& B = views[nxt];
-        hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, st, pb.counts, nxt, 2, 3);
         if (timed) tic(0);
         hipLaunchKernelGGL((k_extend<false, true>), dim3(ext_blocks), dim3(kTraceBlock), 0, st, sc, A, cur[g],
                            (unsigned long long*)nullptr);
@@ -1047,11 +1039,14 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
         hipLaunchKernelGGL(k_shadow_resolve_dense, dim3(shade_blocks), dim3(256), 0, st, A, B);
         if (timed) toc();
         if (timed) tic(3);
+        // scheduler state: next call m reads half m % 2 and writes the other
         long long* c = ctl + (size_t)g * kStreamCtl;
-        hipLaunchKernelGGL(k_stream_finish_dense, dim3(shade_blocks), dim3(256), 0, st, A, c, ring, npx);
-        hipLaunchKernelGGL(k_stream_plan_dense, dim3(1), dim3(64), 0, st, c, pb.counts, nxt, slots, cnt, passes);
-        hipLaunchKernelGGL(k_stream_next_dense, dim3(shade_blocks), dim3(256), 0, st, sc, B, c, ring, accum, average,
-                           nxt, cfg.accum_id, cfg.seed_offset, npx, g, K, cnt);
+        long long* S = c + kStreamHalf * (calls[g] & 1);
+        long long* T = c + kStreamHalf * ((calls[g] + 1) & 1);
+        hipLaunchKernelGGL(k_stream_finish_dense, dim3(shade_blocks), dim3(256), 0, st, A, S, ring, npx, nxt, slots);
+        hipLaunchKernelGGL(k_stream_next_dense, dim3(shade_blocks), dim3(256), 0, st, sc, B, S, T, ring, accum,
+                           average, cur[g], nxt, cfg.accum_id, cfg.seed_offset, npx, g, K, cnt, passes);
+        ++calls[g];
         if (timed) toc();
         cur[g] = nxt;
       }

@@ -127,6 +127,7 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
 constexpr int kStreamRing = 16;
 // Scheduler counters per stream group (64-bit).
 constexpr int kStreamCtl = 64;
+constexpr int kStreamHalf = 32;  // two copies of the scheduler state per group
 
 // Runs `passes` passes from cfg.accum_id with the streaming schedule: slots
 // take (pass, pixel) jobs in pass order as their paths end, finished jobs
