@@ -114,8 +114,12 @@ struct MsnnState {
   float4* avg[3] = {nullptr, nullptr, nullptr};
   double loss_last = 0.0;
   hipStream_t ctx_stream = nullptr;
+  hipStream_t batch_stream = nullptr;  // training batches traced one batch ahead
+  int batches_made = 0;                // training batches generated so far
   ~MsnnState() {
     if (ctx_stream) (void)hipStreamSynchronize(ctx_stream);
+    if (batch_stream) (void)hipStreamSynchronize(batch_stream);
+    if (batch_stream) (void)hipStreamDestroy(batch_stream);
     for (MsnnSlot& sl : slots) sl.release();
     for (TrainBatch& b : tb) b.release();
     if (mlp) mlp_destroy(mlp);
@@ -513,7 +517,20 @@ int msnn_frame(Context& c, MsnnState& m, EventTimer* tm, std::string& err) {
   MsnnSlot& sl = m.take_slot();
   HM_TRY(acquire_slot(sl, err));
   if (tm) (void)hipEventRecord(tm->e[0], sl.st);
-  if (chunk == 0) HM_TRY(gen_train_batch(c, m, f, sl.st, err));
+  if (chunk == 0) {
+    // The frame's batch (unless it was traced ahead), then the next batch on
+    // the batch stream, so that its long tail of late-depth launches overlaps
+    // this batch's frames instead of delaying the next batch's first frame.
+    // Training paths do not depend on the network, and the batches draw their
+    // shuffles in the same order either way.
+    if (m.batches_made <= f / B) {
+      HM_TRY(gen_train_batch(c, m, f, sl.st, err));
+      m.batches_made = f / B + 1;
+    }
+    if (!m.batch_stream) HM_HIP(hipStreamCreateWithFlags(&m.batch_stream, hipStreamNonBlocking), err);
+    HM_TRY(gen_train_batch(c, m, f + B, m.batch_stream, err));
+    m.batches_made = f / B + 2;
+  }
   TrainBatch& b = m.tb[(f / B) % 2];
   // ---- G_BUFFER (every pixel truncated; no training flags) ----
   PassConfig cfg = msnn_pass_config(c, m);
