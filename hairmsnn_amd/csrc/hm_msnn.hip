@@ -102,6 +102,10 @@ struct MsnnState {
   int next_slot = 0;
   uint64_t rng_state = 0;
   double setup_ms = 0.0, warmup_ms = 0.0;  // host wall clock of msnn_begin's two parts
+  // frames in flight: 64 C1 frames took 0.33 s with 3 and 0.41 s with 4 in
+  // the benchmark process; a process running only HairMSNN favoured 4
+  // (0.26 s against 0.32 s): the GPU has 4 hardware queues, and which streams
+  // share one depends on every stream the process created
   static constexpr int kSlots = 3;
   // warm-up training-data passes batched per path-tracing pass (the passes do
   // not depend on the network); slot arrays hold this many batches
