@@ -950,15 +950,21 @@ static PathBuffers other_state(const PathBuffers& pb) {
 }
 
 int stream_group_slots(int npx, int g, int K) {
-  const int chunks = npx / 64;
-  return g < chunks ? 64 * ((chunks - g + K - 1) / K) : 0;
+  // chunks of 64 pixels dealt round-robin; a partial last chunk (scenes hold
+  // a multiple of 128 pixels, so it does not occur from a scene file) is the
+  // last chunk of its group, so the group's pixel indices stay contiguous
+  const int chunks = (npx + 63) / 64;
+  if (g >= chunks) return 0;
+  int n = 64 * ((chunks - g + K - 1) / K);
+  if ((chunks - 1) % K == g) n -= 64 * chunks - npx;
+  return n;
 }
 
 int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_t* streams, int K,
                   const PassConfig& cfg, int passes, float4* accum, float4* average, float4* ring, long long* ctl,
                   int* live_host, hipEvent_t* live_ev, std::string& err) {
   const int npx = sc.width * sc.height;
-  if (passes <= 0 || npx % 64 != 0 || K < 1) return passes <= 0 ? kOk : (err = "bad stream groups", kEInval);
+  if (passes <= 0 || npx <= 0 || K < 1) return passes <= 0 ? kOk : (err = "bad stream groups", kEInval);
   PassTimer* tm = cfg.timer;
   hipStream_t tst = streams[0];
   auto tic = [&](int kind) {

@@ -134,7 +134,8 @@ constexpr int kStreamHalf = 32;  // two copies of the scheduler state per group
 // leave their radiance in ring (kStreamRing x npx float4, by pass % ring and
 // pixel) and each pass is accumulated into accum / average, in pass order,
 // once all its jobs are done. The pixels are split into K groups (64-pixel
-// chunks dealt round-robin), group g on pbs[g] and streams[g] with ctl +
+// chunks dealt round-robin, the last one possibly partial), group g on pbs[g]
+// and streams[g] with ctl +
 // g * kStreamCtl; live_host (pinned, 2K ints) and live_ev (2K events) poll the
 // live-path counts. A timer records group 0's launches. The last pass's
 // radiance is left in ring entry (passes - 1) % kStreamRing.

@@ -134,16 +134,20 @@ def test_converges_to_fp64_oracle(gpu):
     assert abs(gm - om) < 4 * se_m
 
 
-@pytest.mark.parametrize("name,groups,calls", [("c0_hairball.json", 1, (5, 3)), ("c0_hairball.json", 3, (5, 3)),
-                                               ("c1_curly.json", 4, (5, 3)), ("c0_hairball.json", 4, (40, 19))])
-def test_streaming_schedule_equals_pass_schedule(gpu, name, groups, calls):
+@pytest.mark.parametrize("name,groups,calls,size", [("c0_hairball.json", 1, (5, 3), (64, 64)),
+                                                    ("c0_hairball.json", 3, (5, 3), (64, 64)),
+                                                    ("c1_curly.json", 4, (5, 3), (64, 64)),
+                                                    ("c0_hairball.json", 4, (40, 19), (64, 64)),
+                                                    ("c0_hairball.json", 4, (6, 2), (48, 40))])
+def test_streaming_schedule_equals_pass_schedule(gpu, name, groups, calls, size):
     """The streaming schedule (slots take (pass, pixel) jobs in pass order over
     a compacted wavefront, two paths in flight per pixel of each of `groups`
     streams; finished radiance waits in a 16-pass ring until its pass is
     complete) accumulates each pixel's passes in pass order, so its image is
-    the per-pass schedule's bit for bit, across calls, seed offsets and calls
-    longer than the ring."""
-    cfg = scene_cfg(name, 64, 64)
+    the per-pass schedule's bit for bit, across calls, seed offsets, calls
+    longer than the ring and groups of unequal size (48 x 40: 30 chunks of
+    64 pixels over 4 groups)."""
+    cfg = scene_cfg(name, *size)
     imgs, lasts = {}, {}
     for sched in ("passes", "stream"):
         with gpu.Scene(json_text=cfg) as sc:
