@@ -128,6 +128,7 @@ constexpr int kStreamRing = 16;
 // Scheduler counters per stream group (64-bit).
 constexpr int kStreamCtl = 64;
 constexpr int kStreamHalf = 32;  // two copies of the scheduler state per group
+static_assert(8 + kStreamRing <= kStreamHalf && 2 * kStreamHalf <= kStreamCtl, "scheduler state layout");
 
 // Runs `passes` passes from cfg.accum_id with the streaming schedule: slots
 // take (pass, pixel) jobs in pass order as their paths end, finished jobs
