@@ -368,6 +368,29 @@ class Scene:
     def nrc_set_params(self, c: float, all_unbiased: bool = False) -> None:
         check(self.lib.hmsnn_nrc_set_params(self.h, float(c), int(bool(all_unbiased))))
 
+    def msnn_sizes(self) -> dict:
+        v = np.zeros(4, np.int32)
+        check(self.lib.hmsnn_msnn_get_sizes(self.h, _ptr(v)))
+        return {"train_records": int(v[0]), "every_nth": int(v[1]), "frames": int(v[2]), "train_steps": int(v[3])}
+
+    # name -> (which, dtype, shape from (train records, pixels))
+    _MSNN_BUFFERS = {
+        "train_input": (0, np.float32, lambda t, p: (t, 12)),
+        "train_target": (1, np.float32, lambda t, p: (t, 3)),
+        "train_pixel": (2, np.int32, lambda t, p: (t,)),
+        "gb_input": (3, np.float32, lambda t, p: (p, 12)),
+        "gb_hit": (4, np.uint8, lambda t, p: (p,)),
+        "gb_color": (5, np.float32, lambda t, p: (p, 4)),
+        "nn_output": (6, np.float32, lambda t, p: (p, 3)),
+    }
+
+    def msnn_buffer(self, name: str) -> np.ndarray:
+        """One of the last HairMSNN frame's buffers (see _MSNN_BUFFERS)."""
+        which, dtype, shape_of = self._MSNN_BUFFERS[name]
+        out = np.empty(shape_of(self.msnn_sizes()["train_records"], self.width * self.height), dtype)
+        check(self.lib.hmsnn_msnn_get_buffer(self.h, which, _ptr(out), out.nbytes))
+        return out
+
     def nrc_frames(self, frames: int) -> dict:
         st = NrcStats()
         check(self.lib.hmsnn_nrc_frames(self.h, int(frames), C.byref(st)))

@@ -107,6 +107,8 @@ _SIGS = {
     "hmsnn_msnn_begin": (_I, [_P, _I]),
     "hmsnn_msnn_frames": (_I, [_P, _I, C.POINTER(MsnnStats)]),
     "hmsnn_msnn_get_image": (_I, [_P, _I, _P]),
+    "hmsnn_msnn_get_sizes": (_I, [_P, _P]),
+    "hmsnn_msnn_get_buffer": (_I, [_P, _I, _P, C.c_size_t]),
     "hmsnn_render_hair_msnn": (_I, [_P, _I, _I, _P, C.POINTER(MsnnStats)]),
     "hmsnn_nrc_begin": (_I, [_P]),
     "hmsnn_nrc_set_params": (_I, [_P, _F, _I]),

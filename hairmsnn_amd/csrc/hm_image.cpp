@@ -114,8 +114,10 @@ struct ExrReader {
   const uint8_t* p;
   size_t n, pos = 0;
   bool ok = true;
+  // k more bytes at pos; written so that neither side can wrap (a chunk
+  // offset from the file may be anywhere in 0 .. 2^64 - 1)
   bool need(size_t k) {
-    if (pos + k > n) ok = false;
+    if (k > n || pos > n - k) ok = false;
     return ok;
   }
   uint32_t u32() {
@@ -249,8 +251,10 @@ int read_exr(const std::string& path, int& width, int& height, std::vector<float
         chans.push_back(c);
       }
     } else if (name == "compression") {
+      if (size < 1) break;
       comp = r.p[r.pos];
     } else if (name == "dataWindow" && type == "box2i") {
+      if (size < 16) break;
       x0 = r.i32();
       y0 = r.i32();
       x1 = r.i32();
@@ -258,7 +262,7 @@ int read_exr(const std::string& path, int& width, int& height, std::vector<float
     }
     r.pos = end;
   }
-  if (!r.ok || chans.empty() || x1 < x0 || y1 < y0) {
+  if (!r.ok || chans.empty() || comp < 0 || x1 < x0 || y1 < y0) {
     err = path + ": malformed OpenEXR header";
     return HMSNN_EIO;
   }
@@ -268,11 +272,12 @@ int read_exr(const std::string& path, int& width, int& height, std::vector<float
           " is not supported (NONE, RLE, ZIPS and ZIP are)";
     return HMSNN_EUNSUPPORTED;
   }
-  const int W = x1 - x0 + 1, H = y1 - y0 + 1;
-  if ((int64_t)W * H > (int64_t)16384 * 16384) {
+  const int64_t W64 = (int64_t)x1 - x0 + 1, H64 = (int64_t)y1 - y0 + 1;
+  if (W64 > 65536 || H64 > 65536 || W64 * H64 > (int64_t)16384 * 16384) {
     err = path + ": image too large";
     return HMSNN_EINVAL;
   }
+  const int W = (int)W64, H = (int)H64;
   std::vector<size_t> coff(chans.size());
   size_t line_bytes = 0;
   int ci[4] = {-1, -1, -1, -1}, cy = -1;  // R, G, B, A; Y
@@ -300,6 +305,10 @@ int read_exr(const std::string& path, int& width, int& height, std::vector<float
   const int nchunks = (H + lines - 1) / lines;
   std::vector<uint64_t> offs(nchunks);
   for (int k = 0; k < nchunks; ++k) offs[k] = r.u64();
+  if (!r.ok) {
+    err = path + ": truncated OpenEXR offset table";
+    return HMSNN_EIO;
+  }
   std::vector<uint8_t> raw, tmp;
   auto value = [&](const uint8_t* line, int c, int x) -> float {
     const uint8_t* q = line + coff[c];
@@ -312,7 +321,7 @@ int read_exr(const std::string& path, int& width, int& height, std::vector<float
     std::memcpy(&fv, &u, 4);
     return fv;
   };
-  for (int k = 0; k < nchunks && r.ok; ++k) {
+  for (int k = 0; k < nchunks; ++k) {
     r.pos = (size_t)offs[k];
     const int32_t ys = r.i32();
     const uint32_t size = r.u32();

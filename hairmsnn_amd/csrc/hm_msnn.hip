@@ -120,6 +120,7 @@ struct MsnnState {
   hipStream_t ctx_stream = nullptr;
   hipStream_t batch_stream = nullptr;  // training batches traced one batch ahead
   int batches_made = 0;                // training batches generated so far
+  int call_end = 0;                    // frame index one past the current hmsnn_msnn_frames call
   ~MsnnState() {
     if (ctx_stream) (void)hipStreamSynchronize(ctx_stream);
     if (batch_stream) (void)hipStreamSynchronize(batch_stream);
@@ -526,14 +527,18 @@ int msnn_frame(Context& c, MsnnState& m, EventTimer* tm, std::string& err) {
     // the batch stream, so that its long tail of late-depth launches overlaps
     // this batch's frames instead of delaying the next batch's first frame.
     // Training paths do not depend on the network, and the batches draw their
-    // shuffles in the same order either way.
+    // shuffles in the same order either way. A batch is traced ahead only
+    // when its first frame belongs to the current call, so every batch a call
+    // starts is consumed (and waited for) by that call's frames.
     if (m.batches_made <= f / B) {
       HM_TRY(gen_train_batch(c, m, f, sl.st, err));
       m.batches_made = f / B + 1;
     }
-    if (!m.batch_stream) HM_HIP(hipStreamCreateWithFlags(&m.batch_stream, hipStreamNonBlocking), err);
-    HM_TRY(gen_train_batch(c, m, f + B, m.batch_stream, err));
-    m.batches_made = f / B + 2;
+    if (f + B < m.call_end) {
+      if (!m.batch_stream) HM_HIP(hipStreamCreateWithFlags(&m.batch_stream, hipStreamNonBlocking), err);
+      HM_TRY(gen_train_batch(c, m, f + B, m.batch_stream, err));
+      m.batches_made = f / B + 2;
+    }
   }
   TrainBatch& b = m.tb[(f / B) % 2];
   // ---- G_BUFFER (every pixel truncated; no training flags) ----
@@ -613,11 +618,16 @@ int hmsnn_msnn_frames(hmsnn_ctx* ctx, int frames, hmsnn_msnn_stats* stats) {
   if (!c.msnn) return fail(HMSNN_EINVAL, "call hmsnn_msnn_begin first");
   std::string err;
   std::vector<EventTimer> timers(stats ? (size_t)frames : 0);
+  c.msnn->call_end = c.msnn->frame + frames;
   for (int k = 0; k < frames; ++k) {
     int rc = msnn_frame(c, *c.msnn, stats ? &timers[k] : nullptr, err);
     if (rc) return fail(rc, err);
   }
   hipError_t e = hipStreamSynchronize(c.stream);
+  // every batch traced ahead was consumed by a frame of this call, so the
+  // batch stream is idle here; waiting on it keeps that true if the rule
+  // above ever changes
+  if (e == hipSuccess && c.msnn->batch_stream) e = hipStreamSynchronize(c.msnn->batch_stream);
   if (e != hipSuccess) return fail(HMSNN_EDEVICE, hipGetErrorString(e));
   if (stats) {
     hmsnn_msnn_stats local{};
@@ -647,6 +657,46 @@ int hmsnn_msnn_get_image(hmsnn_ctx* ctx, int which, float* rgba) {
   const size_t bytes = sizeof(float4) * (size_t)c.scene.width * c.scene.height;
   hipError_t e = hipMemcpyAsync(rgba, c.msnn->avg[slot], bytes, hipMemcpyDeviceToHost, c.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  return e == hipSuccess ? HMSNN_OK : fail(HMSNN_EDEVICE, hipGetErrorString(e));
+}
+
+int hmsnn_msnn_get_sizes(hmsnn_ctx* ctx, int sizes[4]) {
+  if (!ctx || !sizes) return fail(HMSNN_EINVAL, "hmsnn_msnn_get_sizes: bad arguments");
+  const MsnnState* m = ctx->c.msnn;
+  if (!m) return fail(HMSNN_EINVAL, "call hmsnn_msnn_begin first");
+  sizes[0] = m->n_train;
+  sizes[1] = m->every_nth;
+  sizes[2] = m->frame;
+  sizes[3] = m->train_steps;
+  return HMSNN_OK;
+}
+
+int hmsnn_msnn_get_buffer(hmsnn_ctx* ctx, int which, void* host, size_t bytes) {
+  if (!ctx || !host) return fail(HMSNN_EINVAL, "hmsnn_msnn_get_buffer: bad arguments");
+  Context& c = ctx->c;
+  MsnnState* m = c.msnn;
+  if (!m) return fail(HMSNN_EINVAL, "call hmsnn_msnn_begin first");
+  if (m->frame == 0) return fail(HMSNN_EINVAL, "no frame rendered yet");
+  const size_t npx = (size_t)c.scene.width * c.scene.height, nt = (size_t)m->n_train;
+  const int f = m->frame - 1, B = m->warm_batch;
+  const MsnnSlot& sl = m->slots[(m->next_slot + MsnnState::kSlots - 1) % MsnnState::kSlots];
+  const TrainBatch& b = m->tb[(f / B) % 2];
+  const size_t r0 = (size_t)(f % B) * nt;
+  const void* src = nullptr;
+  size_t want = 0;
+  switch (which) {
+    case HMSNN_MSNN_TRAIN_INPUT: src = b.train_input + 12 * r0, want = sizeof(float) * 12 * nt; break;
+    case HMSNN_MSNN_TRAIN_TARGET: src = b.train_target + 3 * r0, want = sizeof(float) * 3 * nt; break;
+    case HMSNN_MSNN_TRAIN_PIXEL: src = b.pix + r0, want = sizeof(int) * nt; break;
+    case HMSNN_MSNN_GB_INPUT: src = sl.nn_input, want = sizeof(float) * 12 * npx; break;
+    case HMSNN_MSNN_GB_HIT: src = sl.hit_flags, want = npx; break;
+    case HMSNN_MSNN_GB_COLOR: src = sl.short_color, want = sizeof(float4) * npx; break;
+    case HMSNN_MSNN_NN_OUTPUT: src = m->nn_output, want = sizeof(float) * 3 * npx; break;
+    default: return fail(HMSNN_EINVAL, "hmsnn_msnn_get_buffer: unknown buffer");
+  }
+  if (bytes != want) return fail(HMSNN_EINVAL, "hmsnn_msnn_get_buffer: size mismatch");
+  hipError_t e = hipStreamSynchronize(c.stream);
+  if (e == hipSuccess) e = hipMemcpy(host, src, bytes, hipMemcpyDeviceToHost);
   return e == hipSuccess ? HMSNN_OK : fail(HMSNN_EDEVICE, hipGetErrorString(e));
 }
 

@@ -1005,7 +1005,19 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
     calls[g] = 1;
   }
   HM_TRY(check_launch("stream start", err));
-  const long long max_iters = (long long)passes * std::max(1, cfg.max_vertex + 1) + 4 * kBatch;
+  // A path lives at most max_vertex + 1 iterations and every group holds at
+  // least as many slots as pixels (slots >= cnt), so a group drains within
+  // passes x (max_vertex + 1) iterations; the live count is read one batch
+  // late, which adds up to two batches. Past twice that bound the schedule is
+  // reported as stuck.
+  for (int g = 0; g < K; ++g) {
+    const int cnt = stream_group_slots(npx, g, K);
+    if (cnt > 0 && group_slots(cnt) < cnt) {
+      err = "stream group holds fewer slots than pixels";
+      return kEInval;
+    }
+  }
+  const long long max_iters = 2 * ((long long)passes * std::max(1, cfg.max_vertex + 1) + 2 * kBatch);
   for (int batch = 0;; ++batch) {
     bool any = false;
     for (int g = 0; g < K; ++g) {
@@ -1069,7 +1081,10 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
         if (!live[g]) continue;
         const int s = 2 * g + ((batch - 1) & 1);
         HM_HIP(hipEventSynchronize(live_ev[s]), err);
-        if (live_host[s] == 0) live[g] = 0;
+        if (live_host[s] == 0) {
+          live[g] = 0;  // drained: never an error, however late
+          continue;
+        }
         if (iters[g] > max_iters) {
           err = "streaming schedule did not drain";
           return kEDevice;

@@ -492,14 +492,24 @@ int load_hair_file(const std::string& path, int tess, HairGeometry& out, std::st
   std::vector<float> cw;
   for (uint32_t sidx = 0; sidx < hair_count; ++sidx) {
     const int nseg = segs ? (int)segs[sidx] : (int)d_segments;
-    if (nseg < 1) continue;
     if (first + (size_t)nseg + 1 > point_count) {
       err = "Error: Cannot read hair points! (" + path + ": strands need more points than the file has)";
       return HMSNN_EIO;
     }
+    auto P = [&](size_t i) { return v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]); };
+    if (nseg < 1) {
+      // a strand of 0 segments still owns one point (the reference's point
+      // loop consumes segments + 1 points per strand, scene.cpp:24-66) and
+      // that point counts toward the bounds; it contributes no curve but
+      // keeps its strand index, so strand ids stay the file's
+      out.bmin = vmin(out.bmin, P(first));
+      out.bmax = vmax(out.bmax, P(first));
+      out.num_strands++;
+      first += 1;
+      continue;
+    }
     cp.clear();
     cw.clear();
-    auto P = [&](size_t i) { return v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]); };
     auto Wd = [&](size_t i) { return 0.2f * (thick.empty() ? d_thickness : thick[i]); };
     // phantom points mirrored past the ends (scene.cpp:47-66)
     cp.push_back(P(first) + (P(first) - P(first + 1)));

@@ -232,6 +232,23 @@ int hmsnn_msnn_begin(hmsnn_ctx* ctx, int BETA);
 int hmsnn_msnn_frames(hmsnn_ctx* ctx, int frames, hmsnn_msnn_stats* stats);
 /* which = 0 final mean, 1 short-path mean, 2 network mean (w x h x 4). */
 int hmsnn_msnn_get_image(hmsnn_ctx* ctx, int which, float* rgba_host);
+/* sizes[0] training records per frame (nnTrainInput rows,
+ * render_hair_msnn.cu:112-120), [1] every_nth (render_hair_msnn.cu:106),
+ * [2] frames rendered (accumId), [3] training steps taken. */
+int hmsnn_msnn_get_sizes(hmsnn_ctx* ctx, int sizes[4]);
+/* The last frame's buffers, for inspection and tests (the reference's
+ * nnTrainInput / nnTrainOutput / nnFrameInput / GBuffer / nnFrameOutput,
+ * render_hair_msnn.cu:112-145); `bytes` must equal the buffer's size. */
+typedef enum hmsnn_msnn_buffer {
+  HMSNN_MSNN_TRAIN_INPUT = 0,   /* float[n_train][12] training records */
+  HMSNN_MSNN_TRAIN_TARGET = 1,  /* float[n_train][3] full - short, scrubbed (hair_msnn.cu:243-276) */
+  HMSNN_MSNN_TRAIN_PIXEL = 2,   /* int[n_train] pixel of each training record */
+  HMSNN_MSNN_GB_INPUT = 3,      /* float[w*h][12] network input at each pixel's primary hit */
+  HMSNN_MSNN_GB_HIT = 4,        /* uint8[w*h] bit 0 hit, bit 1 surface */
+  HMSNN_MSNN_GB_COLOR = 5,      /* float4[w*h] short-path colour (training pixels: full path) */
+  HMSNN_MSNN_NN_OUTPUT = 6      /* float[w*h][3] network output */
+} hmsnn_msnn_buffer;
+int hmsnn_msnn_get_buffer(hmsnn_ctx* ctx, int which, void* host, size_t bytes);
 /* Reset, warm up, render spp frames, copy the final mean. */
 int hmsnn_render_hair_msnn(hmsnn_ctx* ctx, int BETA, int spp, float* rgba_host, hmsnn_msnn_stats* stats);
 

@@ -289,6 +289,21 @@ int orc_nrc_gbuffer(const orc_scene* sc, void* bvh_handle, int accum_id, const u
 
 int orc_nrc_record_size(void) { return (int)sizeof(NrcPixelOut); }
 
+// HairMSNN training records of frame accum_id at n pixels (orc_scene.h
+// msnn_records); out holds n MsnnRecordOut records.
+int orc_msnn_records(const orc_scene* sc, void* bvh_handle, int accum_id, int kbeta, int n, const int* pixels,
+                     float scene_scale, int threads, int prec, MsnnRecordOut* out) {
+  if (!sc || !bvh_handle || n < 0 || !pixels || !out) return 1;
+  EnvTables env;
+  const SceneDesc s = scene_desc(sc, env);
+  const Bvh& bvh = *(const Bvh*)bvh_handle;
+  if (prec == 0) msnn_records<double>(s, bvh, accum_id, kbeta, n, pixels, scene_scale, threads, out);
+  else msnn_records<float>(s, bvh, accum_id, kbeta, n, pixels, scene_scale, threads, out);
+  return 0;
+}
+
+int orc_msnn_record_size(void) { return (int)sizeof(MsnnRecordOut); }
+
 // Ray k against triangle k (9 floats each; rays as 8 floats o, pad, d, pad):
 // hit flag and (t, u, v) per pair.
 void orc_tri_intersect(int n, const float* tris, const float* rays, int* hit, float* t, float* u, float* v) {

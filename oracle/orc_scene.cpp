@@ -1024,6 +1024,71 @@ void render_pixels(const SceneDesc& s, const Bvh& bvh, int mode, int kbeta, int 
   }
 }
 
+template <class R>
+void msnn_records(const SceneDesc& s, const Bvh& bvh, int accum_id, int kbeta, int n, const int* pixels,
+                  float scene_scale, int threads, MsnnRecordOut* out) {
+  Ctx<R> c{s, bvh, {}, {}};
+  c.hp = HairParams<R>{(R)s.beta_m, (R)s.beta_n, (R)s.alpha, (R)s.gains[0], (R)s.gains[1], (R)s.gains[2], (R)s.gains[3]};
+  c.hs = setup_hair(c.hp);
+  c.hp0 = c.hp;
+  c.hs0 = c.hs;
+  const Vec<R> cp = mk((R)s.cam[0], (R)s.cam[1], (R)s.cam[2]);
+  const Vec<R> d00 = mk((R)s.cam[3], (R)s.cam[4], (R)s.cam[5]);
+  const Vec<R> du = mk((R)s.cam[6], (R)s.cam[7], (R)s.cam[8]);
+  const Vec<R> dv = mk((R)s.cam[9], (R)s.cam[10], (R)s.cam[11]);
+  const int v2 = s.path_v2 - 1;
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+  for (int i = 0; i < n; ++i) {
+    const int px = pixels[i] % s.width, py = pixels[i] / s.width;
+    // camera ray of the G_BUFFER pass (hair_msnn.cu:193-220)
+    Lcg rng{get_rng((uint32_t)(accum_id + 10007), (uint32_t)px, (uint32_t)py, (uint32_t)s.width)};
+    const float jx = rng.next(), jy = rng.next();
+    const float sx = ((float)px + jx) / (float)s.width, sy = ((float)py + jy) / (float)s.height;
+    const Vec<R> d = nrm(d00 + (R)sx * du + (R)sy * dv);
+    Inter<R> si;
+    si.wi = d;
+    trace(c, cp, d, si);
+    MsnnRecordOut& o = out[i];
+    o = MsnnRecordOut{};
+    Vec<R> col = mk((R)0, (R)0, (R)0), sc = col;
+    if (si.hit) {
+      col = msnn_training(c, si, rng, v2, kbeta, sc);
+      o.hit = (uint8_t)(1 | (si.surface ? 2 : 0));
+      const Vec<R> p = si.p / (R)scene_scale;
+      const Vec<R> t = si.to_local[0];
+      const float rec[12] = {(float)p.x, (float)p.y, (float)p.z, (float)si.wo.x, (float)si.wo.y, (float)si.wo.z,
+                             (float)t.x, (float)t.y, (float)t.z, 0.f, 0.f, 0.f};
+      std::copy(rec, rec + 12, o.input);
+      // channels 9-11: the hit's absorption (the reference leaves them unset)
+      const Vec<R> sig = si.surface ? mk((R)s.sigma_a[0], (R)s.sigma_a[1], (R)s.sigma_a[2]) : si.color;
+      o.input[9] = (float)sig.x;
+      o.input[10] = (float)sig.y;
+      o.input[11] = (float)sig.z;
+    } else {
+      o.input[3] = (float)-d.x;
+      o.input[4] = (float)-d.y;
+      o.input[5] = (float)-d.z;
+    }
+    auto has_inf = [](Vec<R> v) { return std::isinf(v.x) || std::isinf(v.y) || std::isinf(v.z); };
+    if (has_nan(col)) col = mk((R)0, (R)0, (R)0);
+    if (has_inf(col)) col = mk((R)1e5f, (R)1e5f, (R)1e5f);
+    if (has_nan(sc)) sc = mk((R)0.01f, (R)0.01f, (R)0.01f);
+    if (has_inf(sc)) sc = mk((R)1e5f, (R)1e5f, (R)1e5f);
+    const R tg[3] = {col.x - sc.x, col.y - sc.y, col.z - sc.z};
+    const R fc[3] = {col.x, col.y, col.z}, scc[3] = {sc.x, sc.y, sc.z};
+    for (int k = 0; k < 3; ++k) {
+      o.target[k] = (double)tg[k];
+      o.full[k] = (double)fc[k];
+      o.shortc[k] = (double)scc[k];
+    }
+  }
+}
+template void msnn_records<float>(const SceneDesc&, const Bvh&, int, int, int, const int*, float, int, MsnnRecordOut*);
+template void msnn_records<double>(const SceneDesc&, const Bvh&, int, int, int, const int*, float, int, MsnnRecordOut*);
+
 // ---------------------------------------------------------------------------
 // Path-replay adjoint in FP64 with the BSDF derivatives taken by central
 // differences of the FP64 hair BSDF (an independent route to the GPU's dual

@@ -102,6 +102,22 @@ template <class R>
 void render_pixels(const SceneDesc& s, const Bvh& bvh, int mode, int kbeta, int accum_id0, int spp, int x0,
                    int y0, int w, int h, int threads, double* out_full, double* out_short, uint8_t* out_hit);
 
+// HairMSNN G_BUFFER training records (cuda/hair_msnn.cu:203-276) of frame
+// accum_id at the given pixels: the 12-float network input at the primary
+// hit (position / scene_scale, wo, tangent, absorption), the target
+// full - short after the reference's scrub (NaN full -> 0, Inf full -> 1e5,
+// NaN short -> 0.01, Inf short -> 1e5; both 0 without a hit), the full and
+// short colours after the scrub, and the hit flags (bit 0 hit, bit 1
+// surface).
+struct MsnnRecordOut {
+  float input[12] = {0};
+  double target[3] = {0, 0, 0}, full[3] = {0, 0, 0}, shortc[3] = {0, 0, 0};
+  uint8_t hit = 0;
+};
+template <class R>
+void msnn_records(const SceneDesc& s, const Bvh& bvh, int accum_id, int kbeta, int n, const int* pixels,
+                  float scene_scale, int threads, MsnnRecordOut* out);
+
 // Neural radiance caching G_BUFFER pass (cuda/nrc.cu:135-343, 357-370) for
 // one frame: per pixel, the radiance before the cache query, the throughput
 // at it and the query record; for training pixels (flags bit 0; bit 1:

@@ -51,6 +51,8 @@ _SIGS = {
     "orc_adjoint": (_I, [C.POINTER(OrcScene), _P, _I, _P, _I, _I, _D, _P, _P, _P]),
     "orc_nrc_gbuffer": (_I, [C.POINTER(OrcScene), _P, _I, _P, _F, _F, _I, _I, _P]),
     "orc_nrc_record_size": (_I, []),
+    "orc_msnn_records": (_I, [C.POINTER(OrcScene), _P, _I, _I, _I, _P, _F, _I, _I, _P]),
+    "orc_msnn_record_size": (_I, []),
     "orc_tri_intersect": (None, [_I, _P, _P, _P, _P, _P, _P]),
 }
 
@@ -62,6 +64,14 @@ NRC_PIXEL = np.dtype({
                 ("<f8", (NRC_MAX_BOUNCES, 15))],
     "offsets": [0, 1, 2, 4, 8, 32, 56, 92, 128],
     "itemsize": 128 + 8 * NRC_MAX_BOUNCES * 15,
+})
+
+# orc_scene.h MsnnRecordOut
+MSNN_RECORD = np.dtype({
+    "names": ["input", "target", "full", "short", "hit"],
+    "formats": [("<f4", 12), ("<f8", 3), ("<f8", 3), ("<f8", 3), "u1"],
+    "offsets": [0, 48, 72, 96, 120],
+    "itemsize": 128,
 })
 
 _lib = None
@@ -205,6 +215,18 @@ class OracleScene:
         out = np.zeros(H * W, NRC_PIXEL)
         rc = self.lib.orc_nrc_gbuffer(C.byref(self.s), self.bvh, int(accum_id), ptr(fl), float(c), float(scene_scale),
                                       threads, prec, out.ctypes.data_as(C.c_void_p))
+        assert rc == 0
+        return out
+
+    def msnn_records(self, accum_id: int, kbeta: int, pixels: np.ndarray, scene_scale: float, prec: int = 1,
+                     threads: int = 0) -> np.ndarray:
+        """HairMSNN G_BUFFER training records (cuda/hair_msnn.cu:203-276) of
+        frame accum_id at the given pixels: one MSNN_RECORD each."""
+        assert self.lib.orc_msnn_record_size() == MSNN_RECORD.itemsize
+        px = np.ascontiguousarray(pixels, np.int32)
+        out = np.zeros(px.size, MSNN_RECORD)
+        rc = self.lib.orc_msnn_records(C.byref(self.s), self.bvh, int(accum_id), int(kbeta), px.size, ptr(px),
+                                       float(scene_scale), threads, prec, out.ctypes.data_as(C.c_void_p))
         assert rc == 0
         return out
 

@@ -146,3 +146,42 @@ def test_unsupported_files_fail_cleanly(tmp_path):
         with pytest.raises(hm.HmsnnError) as e:
             load(tmp_path / name)
         assert e.value.code == code, (name, str(e.value))
+
+
+def _header_len(data: bytes) -> int:
+    """Byte offset of the chunk offset table: after the magic, version and
+    the null-terminated attribute list."""
+    pos = 8
+    while data[pos] != 0:
+        pos = data.index(b"\0", pos) + 1          # name
+        pos = data.index(b"\0", pos) + 1          # type
+        size = struct.unpack_from("<I", data, pos)[0]
+        pos += 4 + size
+    return pos + 1
+
+
+def test_corrupt_files_fail_with_eio(tmp_path):
+    """A file cut inside its offset table, chunk offsets pointing past the end
+    of the file (including values near 2^64, which must not wrap the bounds
+    check) and a zero-size compression attribute are read errors, never a
+    black image."""
+    import hairmsnn_amd as hm
+    planes = {c: np.full((20, 6), 0.5, np.float32) for c in "RGB"}
+    good = tmp_path / "good.exr"
+    write_exr(good, planes, 0, False)
+    data = good.read_bytes()
+    table = _header_len(data)
+    cases = {
+        "cut_table.exr": data[: table + 8 * 7 + 3],
+        "past_end.exr": data[:table] + struct.pack("<Q", len(data) + 100) + data[table + 8:],
+        "wrap.exr": data[:table] + struct.pack("<Q", 2 ** 64 - 2) + data[table + 8:],
+        "zero_comp.exr": data.replace(_attr("compression", "compression", b"\0"),
+                                      _attr("compression", "compression", b"")),
+    }
+    assert cases["zero_comp.exr"] != data
+    for name, blob in cases.items():
+        (tmp_path / name).write_bytes(blob)
+        with pytest.raises(hm.HmsnnError) as e:
+            load(tmp_path / name)
+        assert e.value.code == 2, (name, str(e.value))
+    np.testing.assert_array_equal(load(good)[..., 0], planes["R"])

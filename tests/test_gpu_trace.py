@@ -80,8 +80,19 @@ def test_empty_and_degenerate_rays(gpu):
         rays[:, 0:3] = [[10, 10, 10], [0, 0, 0], [0, -6, 0]]
         rays[:, 4:7] = [[1, 0, 0], [0, 0, 1], [0, 1, 0]]  # away, from inside, through centre
         prim, t, u = sc.trace_rays(rays)
-    seg = sc.segments() if False else None
-    assert prim[0] == -1
+        occ = sc.trace_rays(rays, any_hit=True)
+        seg = sc.segments()
+    # ray 0 points away from the ball and must miss; rays 1 (from the centre,
+    # inside the root sphere, which C0 does not render) and 2 (through the
+    # centre) run radially, parallel to the strands around them: whatever
+    # they hit, it is the brute-force answer, and any hit agrees with it
+    prim_b, t_b, u_b = ol.intersect(seg, rays, use_bvh=False)
+    assert prim[0] == -1 and occ[0] == 0
+    np.testing.assert_array_equal(prim, prim_b)
+    hit = prim_b >= 0
+    np.testing.assert_array_equal(t[hit], t_b[hit])
+    np.testing.assert_array_equal(u[hit], u_b[hit])
+    np.testing.assert_array_equal(occ, hit.astype(occ.dtype))
 
 
 @pytest.mark.parametrize("name,radius", [("c0_hairball.json", 16), ("c1_curly.json", 8)])

@@ -97,3 +97,23 @@ def test_bad_files_fail_cleanly(tmp_path):
         with pytest.raises(hm.HmsnnError) as e:
             segments(scene_with(f, 1))
         assert e.value.code == 2 and msg in str(e.value)
+
+
+def test_zero_segment_strand_keeps_its_point(tmp_path):
+    """A strand with 0 segments owns one point of the file (the reference's
+    point loop consumes segments + 1 points per strand, scene.cpp:24-66): the
+    strands after it must read their own points, and strand ids stay the
+    file's strand indices."""
+    strands = [helix(3, seed=1), helix(0, seed=2), helix(4, seed=3), helix(0, seed=4), helix(2, seed=5)]
+    assert [len(s) - 1 for s in strands] == [3, 0, 4, 0, 2]
+    p = tmp_path / "z.hair"
+    write_hair(p, strands, d_thickness=0.05)
+    seg = segments(scene_with(p, 1))
+    assert len(seg) == 3 + 4 + 2
+    k = 0
+    for sid, s in enumerate(strands):
+        for i in range(len(s) - 1):
+            np.testing.assert_allclose(seg[k, 0:3], s[i], rtol=0, atol=1e-6)
+            np.testing.assert_allclose(seg[k, 4:7], s[i + 1], rtol=0, atol=1e-6)
+            assert seg[k, 7] == sid
+            k += 1
