@@ -352,6 +352,84 @@ double orc_mlp_loss_grad(const float* w, const float* in, const float* target, i
   return l;
 }
 
+// ---- probes of single functions (orc_scene.h probe_*; GGX from orc_math.h)
+// prec 0 = FP64, 1 = FP32 throughout.
+
+void orc_probe_env(const orc_scene* sc, int n, const float* dirs, int prec, double* rgb, double* pdf) {
+  EnvTables env;
+  const SceneDesc s = scene_desc(sc, env);
+  if (prec == 0) probe_env<double>(s, n, dirs, rgb, pdf);
+  else probe_env<float>(s, n, dirs, rgb, pdf);
+}
+
+void orc_probe_env_sample(const orc_scene* sc, int n, const uint32_t* seeds, int prec, double* wi, double* rgb,
+                          double* pdf) {
+  EnvTables env;
+  const SceneDesc s = scene_desc(sc, env);
+  if (prec == 0) probe_env_sample<double>(s, n, seeds, wi, rgb, pdf);
+  else probe_env_sample<float>(s, n, seeds, wi, rgb, pdf);
+}
+
+int orc_probe_hit_size(void) { return kProbeHit; }
+
+void orc_probe_hit(const orc_scene* sc, void* bvh_handle, int n, const float* rays, const float* t, const int* prim,
+                   const float* u, int prec, double* out) {
+  EnvTables env;
+  const SceneDesc s = scene_desc(sc, env);
+  const Bvh& bvh = *(const Bvh*)bvh_handle;
+  if (prec == 0) probe_hit<double>(s, bvh, n, rays, t, prim, u, out);
+  else probe_hit<float>(s, bvh, n, rays, t, prim, u, out);
+}
+
+void orc_probe_direct(const orc_scene* sc, void* bvh_handle, const float* ray, float t, int prim, float u, int n,
+                      const uint32_t* seeds, int threads, int prec, double* mean_rgb) {
+  EnvTables env;
+  const SceneDesc s = scene_desc(sc, env);
+  const Bvh& bvh = *(const Bvh*)bvh_handle;
+  if (prec == 0) probe_direct<double>(s, bvh, ray, t, prim, u, n, seeds, threads, mean_rgb);
+  else probe_direct<float>(s, bvh, ray, t, prim, u, n, seeds, threads, mean_rgb);
+}
+
+// GGX microfacet terms in the local frame (z = normal), alpha as used by the
+// BSDF (already clamped and squared): D(h) per half vector; frostbite_GGX
+// (f cos, RGB) and pdf_GGX of wi at h = normalize(wo + wi) per (wo, wi);
+// sample_GGX per (wo, u1, u2): wi, pdf and frostbite_GGX at wi.
+void orc_ggx_d(int n, double a, const double* h, double* out) {
+  for (int i = 0; i < n; ++i) out[i] = ggxD<double>(a, mk(h[3 * i], h[3 * i + 1], h[3 * i + 2]));
+}
+
+void orc_ggx_eval(int n, double a, const double* wo, const double* wi, const double* albedo, double* f,
+                  double* pdf) {
+  const Vec<double> alb = mk(albedo[0], albedo[1], albedo[2]);
+  for (int i = 0; i < n; ++i) {
+    const Vec<double> o = mk(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]);
+    const Vec<double> w = mk(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]);
+    const Vec<double> ff = frostbite<double>(o, w, alb, a);
+    f[3 * i] = ff.x;
+    f[3 * i + 1] = ff.y;
+    f[3 * i + 2] = ff.z;
+    pdf[i] = ggx_pdf<double>(a, o, nrm(o + w));
+  }
+}
+
+void orc_ggx_sample(int n, double a, const double* wo, const double* u2, const double* albedo, double* wi,
+                    double* pdf, double* f) {
+  const Vec<double> alb = mk(albedo[0], albedo[1], albedo[2]);
+  for (int i = 0; i < n; ++i) {
+    const Vec<double> o = mk(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]);
+    double p = 0;
+    const Vec<double> w = ggx_sample<double>(a, o, u2[2 * i], u2[2 * i + 1], &p);
+    const Vec<double> ff = frostbite<double>(o, w, alb, a);
+    wi[3 * i] = w.x;
+    wi[3 * i + 1] = w.y;
+    wi[3 * i + 2] = w.z;
+    pdf[i] = p;
+    f[3 * i] = ff.x;
+    f[3 * i + 1] = ff.y;
+    f[3 * i + 2] = ff.z;
+  }
+}
+
 int orc_max_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

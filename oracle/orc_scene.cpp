@@ -1277,4 +1277,103 @@ template void render_pixels<float>(const SceneDesc&, const Bvh&, int, int, int, 
 template void render_pixels<double>(const SceneDesc&, const Bvh&, int, int, int, int, int, int, int, int, int, double*,
                                     double*, uint8_t*);
 
+// ---- probes of single integrator functions --------------------------------
+
+template <class R>
+static Ctx<R> probe_ctx(const SceneDesc& s, const Bvh& bvh) {
+  Ctx<R> c{s, bvh, {}, {}};
+  c.hp = HairParams<R>{(R)s.beta_m, (R)s.beta_n, (R)s.alpha, (R)s.gains[0], (R)s.gains[1], (R)s.gains[2], (R)s.gains[3]};
+  c.hs = setup_hair(c.hp);
+  c.hp0 = c.hp;
+  c.hs0 = c.hs;
+  return c;
+}
+
+template <class R>
+static void put3(double* o, Vec<R> v) {
+  o[0] = (double)v.x;
+  o[1] = (double)v.y;
+  o[2] = (double)v.z;
+}
+
+template <class R>
+void probe_env(const SceneDesc& s, int n, const float* dirs, double* rgb, double* pdf) {
+  for (int i = 0; i < n; ++i) {
+    const float* d = dirs + 3 * (size_t)i;
+    const Vec<R> w = mk((R)d[0], (R)d[1], (R)d[2]);
+    put3(rgb + 3 * (size_t)i, env_radiance<R>(s, w));
+    pdf[i] = (double)env_pdf<R>(s, w);
+  }
+}
+
+template <class R>
+void probe_env_sample(const SceneDesc& s, int n, const uint32_t* seeds, double* wi, double* rgb, double* pdf) {
+  for (int i = 0; i < n; ++i) {
+    Lcg rng{seeds[i]};
+    Vec<R> w;
+    R p = 0;
+    const Vec<R> L = env_sample<R>(s, rng, &w, &p);
+    put3(wi + 3 * (size_t)i, w);
+    put3(rgb + 3 * (size_t)i, L);
+    pdf[i] = (double)p;
+  }
+}
+
+template <class R>
+void probe_hit(const SceneDesc& s, const Bvh& bvh, int n, const float* rays, const float* t, const int* prim,
+               const float* u, double* out) {
+  const Ctx<R> c = probe_ctx<R>(s, bvh);
+  for (int i = 0; i < n; ++i) {
+    const float* r = rays + 8 * (size_t)i;
+    Inter<R> si;
+    setup_hit(c, mk((R)r[0], (R)r[1], (R)r[2]), mk((R)r[4], (R)r[5], (R)r[6]), t[i], prim[i], u[i], si);
+    double* o = out + kProbeHit * (size_t)i;
+    put3(o, si.p);
+    put3(o + 3, si.n);
+    put3(o + 6, si.to_local[0]);
+    put3(o + 9, si.to_local[1]);
+    put3(o + 12, si.to_local[2]);
+    put3(o + 15, si.wo_local);
+    o[18] = (double)dot(si.to_local[1], si.n);
+    o[19] = (double)si.radius;
+  }
+}
+
+template <class R>
+void probe_direct(const SceneDesc& s, const Bvh& bvh, const float ray[8], float t, int prim, float u, int n,
+                  const uint32_t* seeds, int threads, double* mean_rgb) {
+  const Ctx<R> c = probe_ctx<R>(s, bvh);
+  Inter<R> si;
+  si.wi = mk((R)ray[4], (R)ray[5], (R)ray[6]);
+  setup_hit(c, mk((R)ray[0], (R)ray[1], (R)ray[2]), si.wi, t, prim, u, si);
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(static) reduction(+ : acc[:6])
+#endif
+  for (int i = 0; i < n; ++i) {
+    Lcg rng{seeds[i]};
+    const Vec<R> L = direct_lighting(c, si, rng);
+    const double l[3] = {(double)L.x, (double)L.y, (double)L.z};
+    for (int k = 0; k < 3; ++k) {
+      acc[k] += l[k];
+      acc[3 + k] += l[k] * l[k];
+    }
+  }
+  for (int k = 0; k < 6; ++k) mean_rgb[k] = acc[k] / (n > 0 ? n : 1);
+}
+
+template void probe_env<double>(const SceneDesc&, int, const float*, double*, double*);
+template void probe_env<float>(const SceneDesc&, int, const float*, double*, double*);
+template void probe_env_sample<double>(const SceneDesc&, int, const uint32_t*, double*, double*, double*);
+template void probe_env_sample<float>(const SceneDesc&, int, const uint32_t*, double*, double*, double*);
+template void probe_hit<double>(const SceneDesc&, const Bvh&, int, const float*, const float*, const int*,
+                                const float*, double*);
+template void probe_hit<float>(const SceneDesc&, const Bvh&, int, const float*, const float*, const int*,
+                               const float*, double*);
+template void probe_direct<double>(const SceneDesc&, const Bvh&, const float*, float, int, float, int,
+                                   const uint32_t*, int, double*);
+template void probe_direct<float>(const SceneDesc&, const Bvh&, const float*, float, int, float, int,
+                                  const uint32_t*, int, double*);
+
 }  // namespace orc

@@ -141,4 +141,23 @@ void nrc_gbuffer(const SceneDesc& s, const Bvh& bvh, int accum_id, const uint8_t
 void adjoint_render(const SceneDesc& s, const Bvh& bvh, int spp, const float* dl_dimg, int num_strands, int threads,
                     double fd_h, double* grad_sigma, double* grad_glob, double* grad_px);
 
+// Probes of single integrator functions, for the oracle's own tests:
+// env radiance and pdf per world direction (optix_common.cuh:10-64); env
+// samples from one LCG per seed (:66-149); the hit setup of a closest hit
+// (hairCH / triangleMeshCH, optix_common.cuh:527-599), kProbeHit doubles per
+// hit: p, n, frame rows X, Y, Z, wo_local, h = dot(Y, n), radius; the mean of
+// directLighting (optix_common.cuh:279-309) over one LCG per seed at one hit
+// (mean_rgb: mean RGB, then mean square RGB).
+constexpr int kProbeHit = 20;
+template <class R>
+void probe_env(const SceneDesc& s, int n, const float* dirs, double* rgb, double* pdf);
+template <class R>
+void probe_env_sample(const SceneDesc& s, int n, const uint32_t* seeds, double* wi, double* rgb, double* pdf);
+template <class R>
+void probe_hit(const SceneDesc& s, const Bvh& bvh, int n, const float* rays, const float* t, const int* prim,
+               const float* u, double* out);
+template <class R>
+void probe_direct(const SceneDesc& s, const Bvh& bvh, const float ray[8], float t, int prim, float u, int n,
+                  const uint32_t* seeds, int threads, double* mean_rgb);
+
 }  // namespace orc

@@ -6,6 +6,7 @@ kernels used only as the checker and as the CPU baseline.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
@@ -13,7 +14,9 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent.parent
 ORC_DIR = ROOT / "oracle"
-ORC_LIB = ORC_DIR / "_build" / "liborc.so"
+# HMSNN_ORACLE_LIB points the tests at another build of the oracle (the
+# mutation tests load deliberately broken copies through it)
+ORC_LIB = Path(os.environ.get("HMSNN_ORACLE_LIB", ORC_DIR / "_build" / "liborc.so"))
 
 _P = C.c_void_p
 _I = C.c_int
@@ -86,6 +89,8 @@ def load_oracle() -> C.CDLL:
     if _lib is not None:
         return _lib
     if not ORC_LIB.exists():
+        if "HMSNN_ORACLE_LIB" in os.environ:
+            raise FileNotFoundError(ORC_LIB)
         build_oracle()
     lib = C.CDLL(str(ORC_LIB))
     for n, (r, a) in _SIGS.items():
@@ -377,3 +382,115 @@ def mlp_forward_fp8(w: np.ndarray, feat: np.ndarray, exps, quantize=e4m3, mfma: 
         z = (mfma_fp8_dot(xq, wq) if mfma else xq @ wq.T) * (sw * sx)
         h = np.maximum(z, 0.0) if l < 5 else z
     return np.exp(h)
+
+
+_SIGS_PROBE = {
+    "orc_probe_env": (None, [C.POINTER(OrcScene), _I, _P, _I, _P, _P]),
+    "orc_probe_env_sample": (None, [C.POINTER(OrcScene), _I, _P, _I, _P, _P, _P]),
+    "orc_probe_hit_size": (_I, []),
+    "orc_probe_hit": (None, [C.POINTER(OrcScene), _P, _I, _P, _P, _P, _P, _I, _P]),
+    "orc_probe_direct": (None, [C.POINTER(OrcScene), _P, _P, _F, _I, _F, _I, _P, _I, _I, _P]),
+    "orc_ggx_d": (None, [_I, _D, _P, _P]),
+    "orc_ggx_eval": (None, [_I, _D, _P, _P, _P, _P, _P]),
+    "orc_ggx_sample": (None, [_I, _D, _P, _P, _P, _P, _P, _P]),
+}
+
+
+def _probe_lib():
+    lib = load_oracle()
+    for n, (r, a) in _SIGS_PROBE.items():
+        f = getattr(lib, n)
+        f.restype = r
+        f.argtypes = a
+    return lib
+
+
+def scene_info(**kw) -> dict:
+    """A scene description for OracleScene without a device context: the
+    fields a context exports, with C0's material and integrator defaults."""
+    info = {
+        "has_sphere": 0, "sphere": [0, 0, 0, 1], "sphere_albedo": [0, 0, 0], "sphere_alpha": 1.0,
+        "sigma_a": [0.06, 0.1, 0.2], "has_env": 0, "env_pdf": 1, "env_scale": 1.0, "env_rot_phi": 0.0,
+        "num_dir_lights": 0, "dir_from": [], "dir_emit": [], "camera": [0.0] * 12, "width": 1, "height": 1,
+        "beta_m": 0.3, "beta_n": 0.3, "alpha": 0.0, "gain": [1, 1, 1, 1], "mis": 1, "path_v1": 1, "path_v2": 40,
+    }
+    info.update(kw)
+    return info
+
+
+def env_with_tables(rgba: np.ndarray) -> dict:
+    t = env_tables(rgba, mode=0)
+    return {"rgba": np.ascontiguousarray(rgba, np.float32), **t}
+
+
+def probe_env(osc: OracleScene, dirs: np.ndarray, prec: int = 0):
+    lib = _probe_lib()
+    d = np.ascontiguousarray(dirs, np.float32)
+    n = d.shape[0]
+    rgb, pdf = np.empty((n, 3)), np.empty(n)
+    lib.orc_probe_env(C.byref(osc.s), n, ptr(d), prec, ptr(rgb), ptr(pdf))
+    return rgb, pdf
+
+
+def probe_env_sample(osc: OracleScene, seeds: np.ndarray, prec: int = 0):
+    lib = _probe_lib()
+    s = np.ascontiguousarray(seeds, np.uint32)
+    n = s.size
+    wi, rgb, pdf = np.empty((n, 3)), np.empty((n, 3)), np.empty(n)
+    lib.orc_probe_env_sample(C.byref(osc.s), n, ptr(s), prec, ptr(wi), ptr(rgb), ptr(pdf))
+    return wi, rgb, pdf
+
+
+def probe_hit(osc: OracleScene, rays: np.ndarray, t: np.ndarray, prim: np.ndarray, u: np.ndarray, prec: int = 0):
+    """Hit setup per closest hit: dict of p, n, X, Y, Z (frame rows),
+    wo_local, h, radius."""
+    lib = _probe_lib()
+    k = lib.orc_probe_hit_size()
+    rays = np.ascontiguousarray(rays, np.float32)
+    t, u = np.ascontiguousarray(t, np.float32), np.ascontiguousarray(u, np.float32)
+    prim = np.ascontiguousarray(prim, np.int32)
+    n = rays.shape[0]
+    out = np.empty((n, k))
+    lib.orc_probe_hit(C.byref(osc.s), osc.bvh, n, ptr(rays), ptr(t), ptr(prim), ptr(u), prec, ptr(out))
+    return {"p": out[:, 0:3], "n": out[:, 3:6], "X": out[:, 6:9], "Y": out[:, 9:12], "Z": out[:, 12:15],
+            "wo_local": out[:, 15:18], "h": out[:, 18], "radius": out[:, 19]}
+
+
+def probe_direct(osc: OracleScene, ray: np.ndarray, t: float, prim: int, u: float, seeds: np.ndarray,
+                 prec: int = 0, threads: int = 0):
+    """Mean and mean square of directLighting at one hit over one LCG per seed."""
+    lib = _probe_lib()
+    r = np.ascontiguousarray(ray, np.float32).reshape(8)
+    s = np.ascontiguousarray(seeds, np.uint32)
+    out = np.zeros(6)
+    lib.orc_probe_direct(C.byref(osc.s), osc.bvh, ptr(r), float(t), int(prim), float(u), s.size, ptr(s), threads,
+                         prec, ptr(out))
+    return out[:3], out[3:]
+
+
+def ggx_d(a: float, h: np.ndarray) -> np.ndarray:
+    lib = _probe_lib()
+    h = np.ascontiguousarray(h, np.float64)
+    out = np.empty(h.shape[0])
+    lib.orc_ggx_d(h.shape[0], a, ptr(h), ptr(out))
+    return out
+
+
+def ggx_eval(a: float, wo: np.ndarray, wi: np.ndarray, albedo=(0.0, 0.0, 0.0)):
+    lib = _probe_lib()
+    wo, wi = np.ascontiguousarray(wo, np.float64), np.ascontiguousarray(wi, np.float64)
+    alb = np.asarray(albedo, np.float64)
+    n = wo.shape[0]
+    f, pdf = np.empty((n, 3)), np.empty(n)
+    lib.orc_ggx_eval(n, a, ptr(wo), ptr(wi), ptr(alb), ptr(f), ptr(pdf))
+    return f, pdf
+
+
+def ggx_sample(a: float, wo: np.ndarray, u2: np.ndarray, albedo=(0.0, 0.0, 0.0)):
+    lib = _probe_lib()
+    wo, u2 = np.ascontiguousarray(wo, np.float64), np.ascontiguousarray(u2, np.float64)
+    alb = np.asarray(albedo, np.float64)
+    n = wo.shape[0]
+    wi, pdf, f = np.empty((n, 3)), np.empty(n), np.empty((n, 3))
+    lib.orc_ggx_sample(n, a, ptr(wo), ptr(u2), ptr(alb), ptr(wi), ptr(pdf), ptr(f))
+    return wi, pdf, f
