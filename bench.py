@@ -160,18 +160,22 @@ def msnn_vs_pt(sc, pt_ms_per_pass: float, R: dict, frames: int, beta: int = 1) -
         ms_t.append(t_warm + t_frames)
         ms_mse.append(mse(sc.msnn_image(0)))
     m_final = max(ms_mse[-1] - b, 1e-30)
+    # what the trained network adds: the same frames' truncated paths alone
+    m_short = max(mse(sc.msnn_image(1)) - b, 1e-30)
     k_eq = a / m_final
     t_pt = k_eq * pt_ms_per_pass * 1e-3
     t_ms = ms_t[-1]
     log(f"msnn vs pt (BETA {beta}): MSNN {frames} frames MSE {ms_mse[-1]:.4g} in {t_ms:.2f} s "
         f"(warm-up {t_warm:.2f} s, setup {t_setup:.2f} s not counted) = PT at {k_eq:.1f} spp, {t_pt:.2f} s "
-        f"-> speed-up {t_pt / t_ms:.2f}x; per frame: " +
+        f"-> speed-up {t_pt / t_ms:.2f}x; RMSE {np.sqrt(m_final):.4f} with the cache, {np.sqrt(m_short):.4f} "
+        f"from the truncated paths alone; per frame: " +
         ", ".join(f"{k} {v / frames:.2f}" for k, v in parts.items()))
     return {
         "beta": beta, "frames": frames, "reference_spp": R["spp"],
         "speedup": t_pt / t_ms, "t_msnn_s": t_ms, "t_warmup_s": t_warm, "t_setup_s_not_counted": t_setup,
         "t_pt_equal_rmse_s": t_pt, "msnn_frame_ms": {k: v / frames for k, v in parts.items()},
         "pt_spp_equal_rmse": k_eq, "rmse_msnn": float(np.sqrt(m_final)),
+        "rmse_short_paths_only": float(np.sqrt(m_short)),
         "msnn_curve": {"frames": fr, "t_s": ms_t, "mse": ms_mse},
     }
 
@@ -451,7 +455,7 @@ def main() -> None:
         line["msnn_vs_pt"]["pt_curve"] = {"spp": R["ks"], "mse": R["mse"]}
         line["msnn_vs_pt"]["beta_sweep"] = {
             str(bt): {k: v for k, v in msnn_vs_pt(sc, wall / args.steps * 1e3, R, 64, beta=bt).items()
-                      if k in ("speedup", "t_msnn_s", "pt_spp_equal_rmse", "rmse_msnn")}
+                      if k in ("speedup", "t_msnn_s", "pt_spp_equal_rmse", "rmse_msnn", "rmse_short_paths_only")}
             for bt in (2, 5, 10)}
         line["nrc_vs_pt"] = nrc_vs_pt(sc, wall / args.steps * 1e3, R, 64)
         fm = line["msnn_vs_pt"]["msnn_frame_ms"]

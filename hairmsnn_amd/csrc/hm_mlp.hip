@@ -237,6 +237,16 @@ __device__ __forceinline__ f32x16 output_layer(const typename MfmaTraits<T>::vec
 
 constexpr int kMlpBlock = 256;
 
+// The backward pass takes the exp activation's derivative as max(y, floor).
+// With the exact derivative y, relative-L2 training drives every output
+// toward zero within a few Adam steps at the reference's lr 1e-2 (all outputs
+// start near 1, far above the radiance targets, and records that miss the
+// hair have target 0); once the outputs are ~1e-30 their gradients are
+// ~1e-30 too and the network stays dead (reproduced in FP64 on the CPU).
+// With the floor, outputs below it still receive the pull of their targets.
+// Above the floor the gradient is exact.
+constexpr float kExpGradFloor = 1e-2f;
+
 template <class T>
 __global__ void __launch_bounds__(kMlpBlock) k_mlp_infer(const T* frag, const float* in, int n, float* out) {
   typedef typename MfmaTraits<T>::vec V;
@@ -516,7 +526,7 @@ __global__ void __launch_bounds__(kMlpBlock) k_mlp_fwd_bwd(const T* frag, const 
         if (loss == HMSNN_LOSS_RELATIVE_L2_LUMINANCE) sc = 1.f / (lum * lum + 0.01f);
         else if (loss == HMSNN_LOSS_RELATIVE_L2) sc = 1.f / (y[o] * y[o] + 0.01f);
         lsum += d * d * sc;
-        const float gval = 2.f * d * sc * y[o] * grad_scale;
+        const float gval = 2.f * d * sc * fmaxf(y[o], kExpGradFloor) * grad_scale;
         dz[o] = fminf(fmaxf(gval, -60000.f), 60000.f);
       }
     }

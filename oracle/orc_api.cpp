@@ -344,10 +344,10 @@ void orc_mlp_forward(const float* w, const float* in, int n, int prec, double* o
 }
 
 double orc_mlp_loss_grad(const float* w, const float* in, const float* target, int n, int loss, int prec,
-                         double* grad) {
-  if (prec == 0) return mlp_loss_grad<double>(w, in, target, n, loss, grad);
+                         double exp_floor, double* grad) {
+  if (prec == 0) return mlp_loss_grad<double>(w, in, target, n, loss, exp_floor, grad);
   std::vector<float> g(kMlpWeights);
-  const double l = mlp_loss_grad<float>(w, in, target, n, loss, g.data());
+  const double l = mlp_loss_grad<float>(w, in, target, n, loss, exp_floor, g.data());
   for (int i = 0; i < kMlpWeights; ++i) grad[i] = g[i];
   return l;
 }

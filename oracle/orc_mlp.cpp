@@ -66,7 +66,8 @@ void mlp_forward(const float* w, const float* in, int n, R* out) {
 }
 
 template <class R>
-double mlp_loss_grad(const float* w, const float* in, const float* target, int n, int loss, R* grad) {
+double mlp_loss_grad(const float* w, const float* in, const float* target, int n, int loss, double exp_floor,
+                     R* grad) {
   std::memset(grad, 0, sizeof(R) * kMlpWeights);
   double total = 0.0;
   const double inv_n = 1.0 / ((double)n * kMlpOut);
@@ -88,7 +89,8 @@ double mlp_loss_grad(const float* w, const float* in, const float* target, int n
         if (loss == 0) scale = (R)1 / (lum * lum + (R)0.01);
         else if (loss == 1) scale = (R)1 / (y[o] * y[o] + (R)0.01);
         part += (double)(d * d * scale) * inv_n;
-        dz[o] = (R)2 * d * scale * (R)inv_n * y[o];  // through exp
+        // through exp, its derivative floored at exp_floor
+        dz[o] = (R)2 * d * scale * (R)inv_n * std::fmax(y[o], (R)exp_floor);
       }
       // output layer
       const float* Wo = w + (size_t)kMlpHidden * kMlpFeat * kMlpFeat;
@@ -131,7 +133,7 @@ template void mlp_encode<double>(const float*, double*);
 template void mlp_encode<float>(const float*, float*);
 template void mlp_forward<double>(const float*, const float*, int, double*);
 template void mlp_forward<float>(const float*, const float*, int, float*);
-template double mlp_loss_grad<double>(const float*, const float*, const float*, int, int, double*);
-template double mlp_loss_grad<float>(const float*, const float*, const float*, int, int, float*);
+template double mlp_loss_grad<double>(const float*, const float*, const float*, int, int, double, double*);
+template double mlp_loss_grad<float>(const float*, const float*, const float*, int, int, double, float*);
 
 }  // namespace orc

@@ -32,8 +32,10 @@ void mlp_encode(const float* rec, R* feat);
 template <class R>
 void mlp_forward(const float* w, const float* in, int n, R* out);
 
-// Loss over the batch and dL/dW (FP64 accumulation).
+// Loss over the batch and dL/dW (FP64 accumulation). The derivative of the
+// exp output activation is taken as max(y, exp_floor) (0: the exact exp).
 template <class R>
-double mlp_loss_grad(const float* w, const float* in, const float* target, int n, int loss, R* grad);
+double mlp_loss_grad(const float* w, const float* in, const float* target, int n, int loss, double exp_floor,
+                     R* grad);
 
 }  // namespace orc

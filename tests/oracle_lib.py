@@ -253,7 +253,7 @@ class OracleScene:
 _SIGS_MLP = {
     "orc_mlp_encode": (None, [_P, _I, _P]),
     "orc_mlp_forward": (None, [_P, _P, _I, _I, _P]),
-    "orc_mlp_loss_grad": (_D, [_P, _P, _P, _I, _I, _I, _P]),
+    "orc_mlp_loss_grad": (_D, [_P, _P, _P, _I, _I, _I, _D, _P]),
 }
 
 
@@ -267,6 +267,8 @@ def _mlp_lib():
 
 
 MLP_WEIGHTS = 5 * 64 * 64 + 3 * 64
+# the library's training takes the exp derivative as max(y, this floor)
+MLP_EXP_GRAD_FLOOR = 1e-2
 
 
 def mlp_encode(inp: np.ndarray) -> np.ndarray:
@@ -284,12 +286,14 @@ def mlp_forward(w: np.ndarray, inp: np.ndarray, prec: int = 0) -> np.ndarray:
     return out
 
 
-def mlp_loss_grad(w: np.ndarray, inp: np.ndarray, target: np.ndarray, loss: int = 0, prec: int = 0):
+def mlp_loss_grad(w: np.ndarray, inp: np.ndarray, target: np.ndarray, loss: int = 0, prec: int = 0,
+                  exp_floor: float = 0.0):
     w = np.ascontiguousarray(w, np.float32)
     inp = np.ascontiguousarray(inp, np.float32)
     target = np.ascontiguousarray(target, np.float32)
     g = np.empty(MLP_WEIGHTS)
-    l = _mlp_lib().orc_mlp_loss_grad(ptr(w), ptr(inp), ptr(target), inp.shape[0], loss, prec, ptr(g))
+    l = _mlp_lib().orc_mlp_loss_grad(ptr(w), ptr(inp), ptr(target), inp.shape[0], loss, prec, float(exp_floor),
+                                     ptr(g))
     return l, g
 
 

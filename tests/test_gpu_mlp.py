@@ -167,12 +167,34 @@ def test_gradient_matches_oracle(gpu, loss):
     m = Mlp(gpu, 0)
     m.set_weights(w)
     lg, g = m.grad(r, t, loss)
-    lo, go = ol.mlp_loss_grad(w, r, t, loss=loss, prec=0)
+    lo, go = ol.mlp_loss_grad(w, r, t, loss=loss, prec=0, exp_floor=ol.MLP_EXP_GRAD_FLOOR)
     assert lg == pytest.approx(lo, rel=1e-2)
     for l in range(6):
         sl = slice(l * 4096, (l + 1) * 4096) if l < 5 else slice(5 * 4096, ol.MLP_WEIGHTS)
         rel = np.linalg.norm(g[sl] - go[sl]) / np.linalg.norm(go[sl])
         print(f"loss {loss} layer {l}: relative gradient error {rel:.2e}")
+        assert rel <= 2e-2
+
+
+def test_gradient_below_exp_floor_matches_oracle(gpu):
+    """Output weights scaled so that about half the outputs fall below the
+    exp derivative's floor: the kernel's gradient is the oracle's floored
+    one (which differs from the exact gradient by ~40 % here)."""
+    n = 1 << 14
+    r = records(n, 24)
+    w = weights(25)
+    w[5 * 4096:] = -np.abs(w[5 * 4096:]) * 20
+    t = np.random.default_rng(26).uniform(0.2, 3.0, (n, 3)).astype(np.float32)
+    m = Mlp(gpu, 0)
+    m.set_weights(w)
+    _, g = m.grad(r, t, 0)
+    _, go = ol.mlp_loss_grad(w, r, t, loss=0, prec=0, exp_floor=ol.MLP_EXP_GRAD_FLOOR)
+    _, gx = ol.mlp_loss_grad(w, r, t, loss=0, prec=0)
+    assert np.linalg.norm(gx - go) > 0.2 * np.linalg.norm(go)
+    for l in range(6):
+        sl = slice(l * 4096, (l + 1) * 4096) if l < 5 else slice(5 * 4096, ol.MLP_WEIGHTS)
+        rel = np.linalg.norm(g[sl] - go[sl]) / np.linalg.norm(go[sl])
+        print(f"layer {l}: relative gradient error {rel:.2e}")
         assert rel <= 2e-2
 
 

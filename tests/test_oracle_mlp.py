@@ -204,3 +204,45 @@ def test_mfma_fp8_dot_known_answers():
     x = ol.e4m3(rng.uniform(1, 2, (16, 64)))
     w = ol.e4m3(rng.uniform(1, 2, (4, 64)))
     np.testing.assert_array_equal(ol.mfma_fp8_dot(x, w), (x @ w.T).astype(np.float32))
+
+
+def test_exp_floor_keeps_training_alive(orc):
+    """Relative-L2 training of the exp-output network at lr 1e-2 with records
+    of zero target (rays that miss) beside small positive radiance targets:
+    with the exact exp derivative every output collapses toward zero within a
+    few Adam steps and never recovers; with the derivative floored at
+    MLP_EXP_GRAD_FLOOR the outputs of the positive records come back to the
+    targets' scale. Above the floor the two gradients are identical."""
+    rng = np.random.default_rng(12)
+    n = 1024
+    r = records(n, 13)
+    hit = rng.random(n) < 0.5
+    r[~hit, 9:12] = 0.0
+    t = np.zeros((n, 3), np.float32)
+    t[hit] = (0.05 * (1.2 + np.sin(4 * r[hit, 0:3]))).astype(np.float32)
+    w0 = weights(14)
+
+    def train(floor, steps=48, lr=1e-2):
+        w = w0.astype(np.float64)
+        m = np.zeros_like(w)
+        v = np.zeros_like(w)
+        for k in range(steps):
+            _, g = ol.mlp_loss_grad(w.astype(np.float32), r, t, loss=0, exp_floor=floor)
+            g = g + 1e-6 * w
+            m = 0.9 * m + 0.1 * g
+            v = 0.99 * v + 0.01 * g * g
+            w -= lr * (m / (1 - 0.9 ** (k + 1))) / (np.sqrt(v / (1 - 0.99 ** (k + 1))) + 1e-15)
+        return ol.mlp_forward(w.astype(np.float32), r, 0)
+
+    dead = train(0.0)
+    alive = train(ol.MLP_EXP_GRAD_FLOOR)
+    print(f"mean output on positive records: exact {dead[hit].mean():.3e}, floored {alive[hit].mean():.3e}, "
+          f"target {t[hit].mean():.3e}")
+    assert dead[hit].mean() < 1e-6
+    assert 0.5 < alive[hit].mean() / t[hit].mean() < 2.0
+    # identical gradients where every output is above the floor
+    y = ol.mlp_forward(w0, r, 0)
+    assert y.min() > ol.MLP_EXP_GRAD_FLOOR
+    _, g0 = ol.mlp_loss_grad(w0, r, t, loss=0)
+    _, g1 = ol.mlp_loss_grad(w0, r, t, loss=0, exp_floor=ol.MLP_EXP_GRAD_FLOOR)
+    np.testing.assert_allclose(g0, g1, rtol=1e-12, atol=1e-15)  # threads sum in any order
