@@ -340,6 +340,10 @@ class Scene:
         check(self.lib.hmsnn_msnn_get_image(self.h, which, _ptr(out)))
         return out
 
+    def msnn_save_weights(self, path: str) -> None:
+        """Writes the HairMSNN radiance cache to a weight file for tcnn.init_weights."""
+        check(self.lib.hmsnn_msnn_save_weights(self.h, str(path).encode()))
+
     def render_hair_msnn(self, BETA: int = 1, spp: int | None = None) -> np.ndarray:
         spp = int(self.info["spp"]) if spp is None else int(spp)
         out = np.empty((self.height, self.width, 4), np.float32)
@@ -418,6 +422,10 @@ class Scene:
         w = np.empty(20672, np.float32)
         check(self.lib.hmsnn_nrc_get_mlp_weights(self.h, _ptr(w), w.size))
         return w
+
+    def nrc_save_weights(self, path: str) -> None:
+        """Writes the NRC cache network to a weight file for tcnn.init_weights."""
+        check(self.lib.hmsnn_nrc_save_weights(self.h, str(path).encode()))
 
     def nrc_set_weights(self, w: np.ndarray) -> None:
         w = np.ascontiguousarray(w, np.float32)

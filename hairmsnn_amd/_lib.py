@@ -13,6 +13,8 @@ _PKG = Path(__file__).resolve().parent
 LIB_PATH = _PKG / "libhmsnn.so"
 
 HMSNN_OK = 0
+HMSNN_EINVAL = 1
+HMSNN_EIO = 2
 STATUS_NAMES = {1: "EINVAL", 2: "EIO", 3: "EDEVICE", 4: "ENOMEM", 5: "EUNSUPPORTED"}
 
 
@@ -107,6 +109,7 @@ _SIGS = {
     "hmsnn_msnn_begin": (_I, [_P, _I]),
     "hmsnn_msnn_frames": (_I, [_P, _I, C.POINTER(MsnnStats)]),
     "hmsnn_msnn_get_image": (_I, [_P, _I, _P]),
+    "hmsnn_msnn_save_weights": (_I, [_P, C.c_char_p]),
     "hmsnn_msnn_get_sizes": (_I, [_P, _P]),
     "hmsnn_msnn_get_buffer": (_I, [_P, _I, _P, C.c_size_t]),
     "hmsnn_render_hair_msnn": (_I, [_P, _I, _I, _P, C.POINTER(MsnnStats)]),
@@ -118,11 +121,14 @@ _SIGS = {
     "hmsnn_nrc_get_buffer": (_I, [_P, _I, _P, C.c_size_t]),
     "hmsnn_nrc_get_mlp_weights": (_I, [_P, _P, _I]),
     "hmsnn_nrc_set_mlp_weights": (_I, [_P, _P, _I]),
+    "hmsnn_nrc_save_weights": (_I, [_P, C.c_char_p]),
     "hmsnn_render_nrc": (_I, [_P, _I, _P, C.POINTER(NrcStats)]),
     "hmsnn_mlp_create": (_I, [_I, C.c_uint64, C.POINTER(_P)]),
     "hmsnn_mlp_destroy": (_I, [_P]),
     "hmsnn_mlp_get_weights": (_I, [_P, _P, _I]),
     "hmsnn_mlp_set_weights": (_I, [_P, _P, _I]),
+    "hmsnn_mlp_save": (_I, [_P, C.c_char_p]),
+    "hmsnn_mlp_load": (_I, [_P, C.c_char_p]),
     "hmsnn_mlp_encode_dev": (_I, [_P, _I, _P, _P]),
     "hmsnn_mlp_forward_dev": (_I, [_P, _P, _I, _P, _P]),
     "hmsnn_mlp_get_fp8_scales": (_I, [_P, _P]),

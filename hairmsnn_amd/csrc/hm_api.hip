@@ -24,6 +24,17 @@ int bsdf_eval_launch(const HairConsts& hc, const void* q, int n, float* f, float
 int bsdf_sample_launch(const HairConsts& hc, const void* q, const float* u4, int n, float* wi, float* f,
                        float* pdf, int* lobe, hipStream_t s, std::string& err);
 
+// Directional lights: (from, 0), (emit, 0) per light; the direction is
+// normalised here with the device's own arithmetic, as sampleLights
+// normalises the stored direction at every use (optix_common.cuh:236).
+__global__ void k_normalize_dlights(float4* dl, int n) {
+  const int i = threadIdx.x;
+  if (i >= n) return;
+  const float4 f = dl[2 * i];
+  const V3 d = normalize(v3(f.x, f.y, f.z));
+  dl[2 * i] = make_float4(d.x, d.y, d.z, 0.f);
+}
+
 static HairConsts consts_from(const hmsnn_hair_params* p) {
   return make_hair_consts(p->beta_m, p->beta_n, p->alpha, p->gain);
 }
@@ -120,6 +131,7 @@ Context::~Context() {
   dev_free(seg_dev);
   dev_free(strand_sigma_dev);
   dev_free(stats_dev);
+  dev_free(dlights_dev);
   bvh.release();
   mesh_bvh.release();
   dev_free(mesh_tri);
@@ -387,10 +399,19 @@ int Context::init(std::string& err) {
   v.env.m_cdf = env.m_cdf;
   v.num_dlights = (int)scene.dlight_from.size();
   v.num_total_lights = v.num_dlights + (scene.has_env ? 1 : 0);
-  for (int i = 0; i < v.num_dlights; ++i) {
-    v.dl_from[i] = scene.dlight_from[i];
-    v.dl_emit[i] = scene.dlight_emit[i];
+  {
+    std::vector<float4> dl(2 * kMaxDirLights, make_float4(0.f, 0.f, 0.f, 0.f));
+    for (int i = 0; i < v.num_dlights; ++i) {
+      const V3 f = scene.dlight_from[i], e = scene.dlight_emit[i];
+      dl[2 * i] = make_float4(f.x, f.y, f.z, 0.f);
+      dl[2 * i + 1] = make_float4(e.x, e.y, e.z, 0.f);
+    }
+    HM_TRY(dev_alloc((void**)&dlights_dev, sizeof(float4) * dl.size(), err));
+    HM_HIP(hipMemcpyAsync(dlights_dev, dl.data(), sizeof(float4) * dl.size(), hipMemcpyHostToDevice, stream), err);
+    hipLaunchKernelGGL(k_normalize_dlights, dim3(1), dim3(kMaxDirLights), 0, stream, dlights_dev, v.num_dlights);
+    HM_HIP(hipGetLastError(), err);
   }
+  v.dlights = dlights_dev;
   v.cam_pos = cam_pos;
   v.cam_d00 = cam_d00;
   v.cam_du = cam_du;

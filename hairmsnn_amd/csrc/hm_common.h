@@ -11,9 +11,16 @@
 #include <hip/hip_runtime.h>
 #define HM_HD __host__ __device__ __forceinline__
 #define HM_D __device__ __forceinline__
+// Large routines called from several places of one kernel: one out-of-line
+// copy keeps the shading kernels' machine code small enough for the
+// instruction cache.
+#define HM_HD_CALL __host__ __device__ __attribute__((noinline))
+#define HM_D_CALL __device__ __attribute__((noinline))
 #else
 #define HM_HD inline
 #define HM_D inline
+#define HM_HD_CALL inline
+#define HM_D_CALL inline
 #endif
 
 namespace hm {
@@ -58,6 +65,50 @@ HM_HD bool is_zero(V3 a) { return a.x == 0.f && a.y == 0.f && a.z == 0.f; }
 HM_HD float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 HM_HD float sqr(float x) { return x * x; }
 HM_HD float safe_sqrt(float x) { return sqrtf(fmaxf(0.f, x)); }
+
+// sin and cos for the shading code's angles (|x| below a few hundred; the
+// largest are azimuths of at most ~8 pi). Cody and Waite's reduction: k =
+// nearest integer to x / (pi/2), r = x - k pi/2 with pi/2 split into three
+// FP32 parts (12 + 12 + 24 bits, so k * part is exact for |k| < 2^11), then
+// the Taylor polynomials of sin (to r^9) and cos (to r^10) on
+// [-pi/4, pi/4], whose truncation errors are below 2e-9; the quadrant k mod 4
+// picks and signs the two. This avoids the large-argument reduction that the
+// library sinf / cosf carry at every call site (most of their code size).
+constexpr float kPio2Hi = 1.5703125f;
+constexpr float kPio2Mid = 4.837512970e-04f;
+constexpr float kPio2Lo = 7.549790126e-08f;
+constexpr float kTwoOverPi = 6.366197467e-01f;
+HM_HD void sincos_small(float x, float* s_out, float* c_out) {
+  const float k = rintf(x * kTwoOverPi);
+  float r = fmaf(-k, kPio2Hi, x);
+  r = fmaf(-k, kPio2Mid, r);
+  r = fmaf(-k, kPio2Lo, r);
+  const float r2 = r * r;
+  float ps = fmaf(r2, 2.755731884e-06f, -1.984127011e-04f);
+  ps = fmaf(r2, ps, 8.333333768e-03f);
+  ps = fmaf(r2, ps, -1.666666716e-01f);
+  const float sn = fmaf(r * r2, ps, r);
+  float pc = fmaf(r2, -2.755731998e-07f, 2.480158764e-05f);
+  pc = fmaf(r2, pc, -1.388888923e-03f);
+  pc = fmaf(r2, pc, 4.166666791e-02f);
+  pc = fmaf(r2, pc, -0.5f);
+  const float cs = fmaf(r2, pc, 1.f);
+  const int q = (int)k & 3;
+  const float s = (q & 1) ? cs : sn;
+  const float c = (q & 1) ? sn : cs;
+  *s_out = (q & 2) ? -s : s;
+  *c_out = ((q + 1) & 2) ? -c : c;
+}
+HM_HD float sin_small(float x) {
+  float s, c;
+  sincos_small(x, &s, &c);
+  return s;
+}
+HM_HD float cos_small(float x) {
+  float s, c;
+  sincos_small(x, &s, &c);
+  return c;
+}
 
 // Rec. 709 luminance (utils.cuh:79-83).
 HM_HD float luminance(V3 c) { return 0.212671f * c.x + 0.715160f * c.y + 0.072169f * c.z; }

@@ -60,6 +60,7 @@ struct Context {
   V3 cam_pos, cam_d00, cam_du, cam_dv;
 
   SceneView view{};
+  float4* dlights_dev = nullptr;  // SceneView::dlights
 
   // path-tracer state
   PathBuffers pb{};

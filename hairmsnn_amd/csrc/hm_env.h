@@ -80,7 +80,7 @@ HM_D float env_pdf(const EnvView& e, V3 wi) {
   const int cw = e.width + 1;
   const float last_u = e.c_pdf[iv * cw + e.width];
   const float last_v = e.m_pdf[e.height];
-  const float denom = (2.f * kPi * kPi * sinf(theta)) * last_u * last_v;
+  const float denom = (2.f * kPi * kPi * sin_small(theta)) * last_u * last_v;
   const float cu = e.c_pdf[iv * cw + iu], cv = e.m_pdf[iv];
   return denom != 0.f ? (cu * cv) / denom : 0.f;
 }
@@ -129,7 +129,9 @@ HM_D V3 env_sample(const EnvView& e, float ux, float uy, V3* wi, float* pdf) {
     const float z = 1.f - 2.f * ux;
     const float r = sqrtf(fmaxf(0.f, 1.f - z * z));
     const float ph = 2.f * kPi * uy;
-    *wi = normalize(v3(r * cosf(ph), r * sinf(ph), z));
+    float sp, cp;
+    sincos_small(ph, &sp, &cp);
+    *wi = normalize(v3(r * cp, r * sp, z));
     *pdf = 1.f / (4.f * kPi);
     return env_radiance(e, *wi);
   }
@@ -149,12 +151,15 @@ HM_D V3 env_sample(const EnvView& e, float ux, float uy, V3* wi, float* pdf) {
   const float last_v = e.m_pdf[e.height];
   const float pu = e.c_pdf[iv * cw + iu], pv = e.m_pdf[iv];
   const float theta = kPi * v;
-  const float sin_t = sinf(theta);
+  float sin_t, cos_t;
+  sincos_small(theta, &sin_t, &cos_t);
   const float denom = (2.f * kPi * kPi * sin_t) * last_u * last_v;
   *pdf = denom != 0.f ? (pu * pv) / denom : 0.f;
   float phi = 2.f * kPi * u - e.rot_phi;
   if (phi < 0.f) phi = kTwoPiEnv + phi;
-  *wi = v3(sin_t * cosf(phi), sin_t * sinf(phi), cosf(theta));
+  float sin_p, cos_p;
+  sincos_small(phi, &sin_p, &cos_p);
+  *wi = v3(sin_t * cos_p, sin_t * sin_p, cos_t);
   return L;
 }
 

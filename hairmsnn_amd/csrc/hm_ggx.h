@@ -55,8 +55,10 @@ HM_HD V3 ggx_sample_vndf(float a, V3 v, float u1, float u2) {
   const V3 t2v = cross(vh, t1v);
   const float r = sqrtf(u1);
   const float phi = 2.f * kPi * u2;
-  const float t1 = r * cosf(phi);
-  float t2 = r * sinf(phi);
+  float sin_p, cos_p;
+  sincos_small(phi, &sin_p, &cos_p);
+  const float t1 = r * cos_p;
+  float t2 = r * sin_p;
   const float s = 0.5f * (1.f + vh.z);
   t2 = (1.f - s) * sqrtf(1.f - t1 * t1) + s * t2;
   const V3 nh = t1 * t1v + t2 * t2v + sqrtf(fmaxf(0.f, 1.f - t1 * t1 - t2 * t2)) * vh;

@@ -116,10 +116,20 @@ HM_HD float bessel_i0_series(float x) {
   return val;
 }
 
+HM_HD float sel4(int p, float a0, float a1, float a2, float a3) {
+  float r = a3;
+  r = p == 2 ? a2 : r;
+  r = p == 1 ? a1 : r;
+  r = p == 0 ? a0 : r;
+  return r;
+}
+
 // Longitudinal lobe M_p for cos/sin of theta_i and of the tilted theta_o
 // (co_t >= 0). Both branches of disney_hair.cuh:77-88.
 HM_HD float hair_mp(const HairConsts& hc, int lobe, float ci, float co_t, float si, float so_t) {
-  const float v = hc.v[lobe], inv_v = hc.inv_v[lobe];
+  // value selects, not indexing: the lobe may be a loop counter
+  const float v = sel4(lobe, hc.v[0], hc.v[1], hc.v[2], hc.v[3]);
+  const float inv_v = sel4(lobe, hc.inv_v[0], hc.inv_v[1], hc.inv_v[2], hc.inv_v[3]);
   const float a = ci * co_t * inv_v;
   if (v <= 0.1f) {
     float expo;
@@ -134,10 +144,10 @@ HM_HD float hair_mp(const HairConsts& hc, int lobe, float ci, float co_t, float 
       const float b = si * so_t * inv_v;
       expo = logf(bessel_i0_series(a)) - b - inv_v;
     }
-    return expf(expo + hc.mp_log_c[lobe]);
+    return expf(expo + sel4(lobe, hc.mp_log_c[0], hc.mp_log_c[1], hc.mp_log_c[2], hc.mp_log_c[3]));
   }
   const float b = si * so_t * inv_v;
-  return expf(-b) * bessel_i0_series(a) * hc.mp_sinh_c[lobe];
+  return expf(-b) * bessel_i0_series(a) * sel4(lobe, hc.mp_sinh_c[0], hc.mp_sinh_c[1], hc.mp_sinh_c[2], hc.mp_sinh_c[3]);
 }
 
 // Trimmed logistic over [-pi, pi] (disney_hair.cuh:116-130).
@@ -162,28 +172,41 @@ HM_HD float hair_np(const HairConsts& hc, float phi, int p, float gamma_o, float
   return hair_trimmed_logistic(hc, dphi);
 }
 
-// Everything at a hit that depends on wo, h and sigma_a only.
+// Everything at a hit that depends on wo, h and sigma_a only. The tilted
+// theta_o of each lobe and the lobe probabilities are recomputed where they
+// are used (a few multiplies) rather than held in registers through the
+// whole shading of a vertex.
 struct HairWo {
   float so, co, phi_o;     // sin/cos theta_o, phi_o
-  float so_t[3], co_t[3];  // tilted theta_o per lobe 0..2 (signed cos)
   float gamma_o, gamma_t, h;
   V3 ap[4];                // attenuation A_p, RGB
-  float ap_pdf[4];         // normalised mean-RGB A_p
+  float inv_ap_sum;        // 1 / sum_p mean-RGB A_p
 };
+
+// sin and cos of theta_o tilted for lobe p: lobe 0 rotates by -2 alpha,
+// lobe 1 by +alpha, lobe 2 by +4 alpha (disney_hair.cuh:228-246); lobe 3 is
+// not tilted. cos is signed (sampling uses the sign, evaluation |cos|).
+HM_HD void hair_tilt(const HairConsts& hc, const HairWo& w, int p, float* so_t, float* co_t) {
+  const float c = sel4(p, hc.cos2k[1], hc.cos2k[0], hc.cos2k[2], 1.f);
+  const float s = sel4(p, -hc.sin2k[1], hc.sin2k[0], hc.sin2k[2], 0.f);
+  *so_t = w.so * c + w.co * s;
+  *co_t = w.co * c - w.so * s;
+}
+
+// Probability of lobe p: its mean-RGB A_p over the sum over the lobes
+// (disney_hair.cuh:142-182).
+HM_HD float hair_ap_pdf(const HairWo& w, int p) {
+  const float x = sel4(p, w.ap[0].x, w.ap[1].x, w.ap[2].x, w.ap[3].x);
+  const float y = sel4(p, w.ap[0].y, w.ap[1].y, w.ap[2].y, w.ap[3].y);
+  const float z = sel4(p, w.ap[0].z, w.ap[1].z, w.ap[2].z, w.ap[3].z);
+  return ((x + y + z) * (1.f / 3.f)) * w.inv_ap_sum;
+}
 
 HM_HD HairWo hair_prepare(const HairConsts& hc, V3 wo, float h, V3 sigma_a) {
   HairWo w;
   w.so = wo.x;
   w.co = safe_sqrt((1.f - w.so) * (1.f + w.so));
   w.phi_o = atan2f(wo.z, wo.y);
-  // Scale tilt: lobe 0 rotates by -2 alpha, lobe 1 by +alpha, lobe 2 by +4 alpha
-  // (disney_hair.cuh:228-246).
-  w.so_t[0] = w.so * hc.cos2k[1] - w.co * hc.sin2k[1];
-  w.co_t[0] = w.co * hc.cos2k[1] + w.so * hc.sin2k[1];
-  w.so_t[1] = w.so * hc.cos2k[0] + w.co * hc.sin2k[0];
-  w.co_t[1] = w.co * hc.cos2k[0] - w.so * hc.sin2k[0];
-  w.so_t[2] = w.so * hc.cos2k[2] + w.co * hc.sin2k[2];
-  w.co_t[2] = w.co * hc.cos2k[2] - w.so * hc.sin2k[2];
   w.h = h;
   w.gamma_o = asinf(h);
   // 1 - x^2 is formed as (1 - x)(1 + x), which keeps its relative accuracy
@@ -205,15 +228,10 @@ HM_HD HairWo hair_prepare(const HairConsts& hc, V3 wo, float h, V3 sigma_a) {
   w.ap[1] = omf2 * T;
   w.ap[2] = (omf2 * F) * (T * T);
   w.ap[3] = w.ap[2] * (F * T) / (v3(1.f) - F * T);
-  float m[4], sum = 0.f;
+  float sum = 0.f;
 #pragma unroll
-  for (int p = 0; p < 4; ++p) {
-    m[p] = (w.ap[p].x + w.ap[p].y + w.ap[p].z) * (1.f / 3.f);
-    sum += m[p];
-  }
-  const float inv_sum = 1.f / sum;
-#pragma unroll
-  for (int p = 0; p < 4; ++p) w.ap_pdf[p] = m[p] * inv_sum;
+  for (int p = 0; p < 4; ++p) sum += (w.ap[p].x + w.ap[p].y + w.ap[p].z) * (1.f / 3.f);
+  w.inv_ap_sum = 1.f / sum;
   return w;
 }
 
@@ -232,18 +250,30 @@ HM_HD FPdf hair_eval_fp(const HairConsts& hc, const HairWo& w, V3 wi) {
   const float phi = atan2f(wi.z, wi.y) - w.phi_o;
   V3 f = v3(0.f);
   float pd = 0.f;
-#pragma unroll
-  for (int p = 0; p < 3; ++p) {
-    const float mp = hair_mp(hc, p, ci, fabsf(w.co_t[p]), si, w.so_t[p]);
-    const float np = hair_np(hc, phi, p, w.gamma_o, w.gamma_t);
-    const float mn = mp * np;
-    f += (hc.gain[p] * mn) * w.ap[p];
-    pd += mn * w.ap_pdf[p];
+  // One copy of the lobe code, run for p = 0..3 (the residual lobe p = 3
+  // has the untilted theta_o and N = 1 / 2 pi): the shading kernels inline
+  // this evaluation at several places, and an unrolled body per lobe made
+  // them larger than the instruction cache.
+#pragma unroll 1
+  for (int p = 0; p < 4; ++p) {
+    float so_t, co_t;
+    hair_tilt(hc, w, p, &so_t, &co_t);
+    const float mp = hair_mp(hc, p, ci, fabsf(co_t), si, so_t);
+    const float np = p < 3 ? hair_np(hc, phi, p, w.gamma_o, w.gamma_t) : 1.f / (2.f * kPi);
+    const float gain = sel4(p, hc.gain[0], hc.gain[1], hc.gain[2], hc.gain[3]);
+    const V3 ap = v3(sel4(p, w.ap[0].x, w.ap[1].x, w.ap[2].x, w.ap[3].x),
+                     sel4(p, w.ap[0].y, w.ap[1].y, w.ap[2].y, w.ap[3].y),
+                     sel4(p, w.ap[0].z, w.ap[1].z, w.ap[2].z, w.ap[3].z));
+    const float ap_pdf = hair_ap_pdf(w, p);
+    if (p < 3) {
+      const float mn = mp * np;
+      f += (gain * mn) * ap;
+      pd += mn * ap_pdf;
+    } else {
+      f += (gain * mp * np) * ap;
+      pd += mp * ap_pdf * np;
+    }
   }
-  const float mp3 = hair_mp(hc, 3, ci, w.co, si, w.so);
-  const float np3 = 1.f / (2.f * kPi);
-  f += (hc.gain[3] * mp3 * np3) * w.ap[3];
-  pd += mp3 * w.ap_pdf[3] * np3;
   return FPdf{f, pd};
 }
 
@@ -251,14 +281,6 @@ HM_HD V3 hair_eval(const HairConsts& hc, const HairWo& w, V3 wi, float* pdf) {
   const FPdf r = hair_eval_fp(hc, w, wi);
   *pdf = r.pdf;
   return r.f;
-}
-
-HM_HD float sel4(int p, float a0, float a1, float a2, float a3) {
-  float r = a3;
-  r = p == 2 ? a2 : r;
-  r = p == 1 ? a1 : r;
-  r = p == 0 ? a0 : r;
-  return r;
 }
 
 // Importance-sample wi from four uniforms (disney_hair.cuh:276-386). Returns
@@ -269,27 +291,29 @@ HM_HD V3 hair_sample_dir(const HairConsts& hc, const HairWo& w, float u0, float 
   // array index is a constant
   int p;
   float e1 = u0;
-  if (e1 < w.ap_pdf[0]) {
+  const float q0 = hair_ap_pdf(w, 0);
+  if (e1 < q0) {
     p = 0;
   } else {
-    e1 -= w.ap_pdf[0];
-    if (e1 < w.ap_pdf[1]) {
+    e1 -= q0;
+    const float q1 = hair_ap_pdf(w, 1);
+    if (e1 < q1) {
       p = 1;
     } else {
-      e1 -= w.ap_pdf[1];
-      p = e1 < w.ap_pdf[2] ? 2 : 3;
+      e1 -= q1;
+      p = e1 < hair_ap_pdf(w, 2) ? 2 : 3;
     }
   }
   // value selects (not lvalue conditionals, not indexing) keep the per-lobe
   // values in registers; sampling uses the signed tilted cosine
-  const float so_t = sel4(p, w.so_t[0], w.so_t[1], w.so_t[2], w.so);
-  const float co_t = sel4(p, w.co_t[0], w.co_t[1], w.co_t[2], w.co);
+  float so_t, co_t;
+  hair_tilt(hc, w, p, &so_t, &co_t);
   const float v = sel4(p, hc.vs[0], hc.vs[1], hc.vs[2], hc.vs[3]);
   const float inv_v = sel4(p, hc.inv_vs[0], hc.inv_vs[1], hc.inv_vs[2], hc.inv_vs[3]);
   const float e2 = fmaxf(u1, 1e-5f);
   const float cos_t = 1.f + v * logf(e2 + (1.f - e2) * expf(-2.f * inv_v));
   const float sin_t = safe_sqrt((1.f - cos_t) * (1.f + cos_t));
-  const float cos_phi = cosf(2.f * kPi * u2);
+  const float cos_phi = cos_small(2.f * kPi * u2);
   const float si = -cos_t * so_t + sin_t * cos_phi * co_t;
   const float ci = safe_sqrt((1.f - si) * (1.f + si));
   float dphi;
@@ -303,7 +327,9 @@ HM_HD V3 hair_sample_dir(const HairConsts& hc, const HairWo& w, float u0, float 
   }
   const float phi_i = w.phi_o + dphi;
   *lobe_out = p;
-  return normalize(v3(si, ci * cosf(phi_i), ci * sinf(phi_i)));
+  float sin_pi, cos_pi;
+  sincos_small(phi_i, &sin_pi, &cos_pi);
+  return normalize(v3(si, ci * cos_pi, ci * sin_pi));
 }
 
 // Hair shading frame at a hit (optix_common.cuh:571-583): rows X = tangent,

@@ -734,6 +734,19 @@ int mlp_get_weights(const MlpState& s, float* w_host, std::string& err) {
   return kOk;
 }
 
+int mlp_save(const MlpState& s, const std::string& path, std::string& err) {
+  std::vector<float> w(kMlpWeightCount);
+  HM_HIP(hipDeviceSynchronize(), err);
+  HM_TRY(mlp_get_weights(s, w.data(), err));
+  return mlp_weights_write(path, w.data(), err);
+}
+
+int mlp_load(MlpState& s, const std::string& path, std::string& err) {
+  std::vector<float> w(kMlpWeightCount);
+  HM_TRY(mlp_weights_read(path, w.data(), err));
+  return mlp_set_weights(s, w.data(), err);
+}
+
 static int mlp_blocks(int n) {
   const int tiles = (n + 31) / 32;
   return std::max(1, std::min(1024, (tiles + 3) / 4));
@@ -903,6 +916,20 @@ int hmsnn_mlp_get_fp8_scales(const hmsnn_mlp* m, int* e) {
       hipMemcpy(e, m->s->scales, sizeof(int) * 12, hipMemcpyDeviceToHost) != hipSuccess)
     return fail(HMSNN_EDEVICE, "hmsnn_mlp_get_fp8_scales: copy failed");
   return HMSNN_OK;
+}
+
+int hmsnn_mlp_save(const hmsnn_mlp* m, const char* path) {
+  if (!m || !path) return fail(HMSNN_EINVAL, "hmsnn_mlp_save: bad arguments");
+  std::string err;
+  int rc = mlp_save(*m->s, path, err);
+  return rc ? fail(rc, err) : HMSNN_OK;
+}
+
+int hmsnn_mlp_load(hmsnn_mlp* m, const char* path) {
+  if (!m || !path) return fail(HMSNN_EINVAL, "hmsnn_mlp_load: bad arguments");
+  std::string err;
+  int rc = mlp_load(*m->s, path, err);
+  return rc ? fail(rc, err) : HMSNN_OK;
 }
 
 int hmsnn_mlp_set_weights(hmsnn_mlp* m, const float* w, int n) {

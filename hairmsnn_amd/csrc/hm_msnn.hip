@@ -305,6 +305,8 @@ int msnn_setup(Context& c, int BETA, std::string& err) {
   m->rng_state = s.msnn_seed * 0x2545F4914F6CDD1Dull + 1;
   const int prec = s.msnn_precision == "bf16" ? HMSNN_BF16 : s.msnn_precision == "fp8" ? HMSNN_FP8 : HMSNN_FP16;
   HM_TRY(mlp_create(prec, s.msnn_seed, &m->mlp, err));
+  // tcnn.init_weights (scene.cpp:330-331 -> render_hair_msnn.cu:109-110)
+  if (!s.nrc_weights.empty()) HM_TRY(mlp_load(*m->mlp, scene_path(s, s.nrc_weights), err));
   m->ctx_stream = c.stream;
   const int spill_levels = c.stack_levels();
   // a warm-up pass holds warm_batch training batches; the slots' path buffers
@@ -646,6 +648,17 @@ int hmsnn_msnn_frames(hmsnn_ctx* ctx, int frames, hmsnn_msnn_stats* stats) {
     *stats = local;
   }
   return HMSNN_OK;
+}
+
+int hmsnn_msnn_save_weights(hmsnn_ctx* ctx, const char* path) {
+  if (!ctx || !path) return fail(HMSNN_EINVAL, "hmsnn_msnn_save_weights: bad arguments");
+  Context& c = ctx->c;
+  if (!c.msnn) return fail(HMSNN_EINVAL, "call hmsnn_msnn_begin first");
+  std::string err;
+  int rc = hipStreamSynchronize(c.stream) == hipSuccess ? kOk : kEDevice;
+  if (rc) err = "hmsnn_msnn_save_weights: device error";
+  if (!rc) rc = mlp_save(*c.msnn->mlp, path, err);
+  return rc ? fail(rc, err) : HMSNN_OK;
 }
 
 int hmsnn_msnn_get_image(hmsnn_ctx* ctx, int which, float* rgba) {

@@ -233,6 +233,8 @@ int nrc_setup(Context& c, std::string& err) {
   m->rng_state = s.msnn_seed * 0x2545F4914F6CDD1Dull + 5;
   const int prec = s.msnn_precision == "bf16" ? HMSNN_BF16 : s.msnn_precision == "fp8" ? HMSNN_FP8 : HMSNN_FP16;
   HM_TRY(mlp_create(prec, s.msnn_seed, &m->mlp, err));
+  // tcnn.init_weights (scene.cpp:330-331 -> render_nrc.cu:149-150)
+  if (!s.nrc_weights.empty()) HM_TRY(mlp_load(*m->mlp, scene_path(s, s.nrc_weights), err));
   m->ctx_stream = c.stream;
   auto A = [&](void** p, size_t bytes) -> int {
     HM_TRY(dev_alloc(p, bytes, err));
@@ -489,6 +491,15 @@ int hmsnn_nrc_get_mlp_weights(hmsnn_ctx* ctx, float* w_host, int n) {
   std::string err;
   if (hipStreamSynchronize(ctx->c.stream) != hipSuccess) return fail(HMSNN_EDEVICE, "stream failed");
   int rc = mlp_get_weights(*ctx->c.nrc->mlp, w_host, err);
+  return rc ? fail(rc, err) : HMSNN_OK;
+}
+
+int hmsnn_nrc_save_weights(hmsnn_ctx* ctx, const char* path) {
+  if (!ctx || !path) return fail(HMSNN_EINVAL, "hmsnn_nrc_save_weights: bad arguments");
+  if (!ctx->c.nrc) return fail(HMSNN_EINVAL, "call hmsnn_nrc_begin first");
+  std::string err;
+  if (hipStreamSynchronize(ctx->c.stream) != hipSuccess) return fail(HMSNN_EDEVICE, "stream failed");
+  int rc = mlp_save(*ctx->c.nrc->mlp, path, err);
   return rc ? fail(rc, err) : HMSNN_OK;
 }
 

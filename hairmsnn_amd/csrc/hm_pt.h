@@ -36,8 +36,10 @@ struct SceneView {
   float sphere_alpha;  // clamped to [0.01, 1] and squared (optix_common.cuh:551-552)
   EnvView env;
   int num_dlights, num_total_lights;
-  V3 dl_from[kMaxDirLights];  // normalised
-  V3 dl_emit[kMaxDirLights];
+  // per directional light: (direction, 0), (emit, 0); the direction is the
+  // parsed (normalised) `from` normalised once more on the device, as
+  // sampleLights does at every use (optix_common.cuh:236)
+  const float4* dlights;
   V3 cam_pos, cam_d00, cam_du, cam_dv;
   int width, height;
   HairConsts hc;
@@ -265,13 +267,9 @@ HM_D LightSample sample_light(const SceneView& sc, uint32_t& rng) {
     ls.emit = env_sample(sc.env, ux, uy, &ls.wi, &env_p);
     ls.light_pdf = ls.light_pdf * env_p;
   } else {
-#pragma unroll
-    for (int k = 0; k < kMaxDirLights; ++k) {
-      if (k == sel) {  // unrolled select: no dynamic indexing into the kernel argument
-        ls.emit = sc.dl_emit[k];
-        ls.wi = normalize(sc.dl_from[k]);
-      }
-    }
+    const float4 f = sc.dlights[2 * sel], e = sc.dlights[2 * sel + 1];
+    ls.wi = v3(f.x, f.y, f.z);
+    ls.emit = v3(e.x, e.y, e.z);
     ls.is_delta = true;
   }
   return ls;
@@ -347,6 +345,150 @@ HM_D void direct_lighting(const SceneView& sc, const Hit& h, const LightSample& 
   out.lp2 = lp;
   out.bp2 = bpdf;
 }
+
+// The lighting and continuation of one path vertex: directLighting with its
+// light sample and MIS BSDF sample (optix_common.cuh:151-309), then Russian
+// roulette and the continuation sample (nextPathVertex, :311-360), with the
+// reference's draw order. The three BSDF evaluations (light direction, MIS
+// sample, continuation) and the two BSDF samples run through one copy of the
+// sampling and evaluation code, a loop over k = 0 (light sample), 1 (MIS BSDF
+// sample), 2 (continuation): inlined three times over, that code made the
+// shading kernels larger than the instruction cache, and the kernels spent
+// most of their time refetching instructions.
+//
+// dl: light this vertex. roulette(eps): when rr, called with the roulette draw
+// before the continuation; returns whether the path goes on (and rescales the
+// caller's throughput). may_continue: a continuation is allowed.
+// light(which, o, d, c): an unoccluded shadow ray (which 0: light sample, 1: MIS
+// BSDF sample) with its contribution before the path throughput; called at
+// most once per which, in order. The continuation is returned: out.cont, the
+// new ray origin and direction, the throughput factor f / pdf (NaN -> 1) and
+// the sampling pdf.
+struct Continuation {
+  int cont;
+  V3 o, wi, mul;
+  float pdf;
+};
+
+template <class Roulette, class LightSink>
+HM_D Continuation shade_vertex(const SceneView& sc, const Hit& h, uint32_t& rng, bool dl, bool rr, bool may_continue,
+                               Roulette roulette, LightSink light) {
+  Continuation out;
+  out.cont = 0;
+  out.o = out.wi = out.mul = v3(0.f);
+  out.pdf = 0.f;
+  LightSample ls;
+  ls.emit = ls.wi = v3(0.f);
+  ls.light_pdf = 1.f;
+  ls.is_delta = true;
+  if (dl) ls = sample_light(sc, rng);
+  const float inv_n = 1.f / (float)sc.num_total_lights;
+  V3 pdl = h.p;  // the push-through is shared by both direct-lighting samples (reference quirk)
+#pragma unroll 1
+  for (int k = 0; k < 3; ++k) {
+    if (k == 0 && !dl) continue;
+    if (k == 1 && (!dl || !sc.mis || ls.is_delta)) continue;
+    if (k == 2) {
+      const bool alive = rr ? roulette(draw(rng)) : true;
+      if (!alive || !may_continue) break;
+    }
+    V3 wl, wi;
+    float spdf = 0.f;
+    if (k == 0) {
+      wi = ls.wi;
+      wl = normalize(to_local(h.fr, wi));
+    } else {
+      // sampleBSDF / nextPathVertex draws: two uniforms, two more on hair
+      // (optix_common.cuh:154-170, :314-329)
+      const float r0 = draw(rng), r1 = draw(rng);
+      if (!h.surface) {
+        const float r2 = draw(rng), r3 = draw(rng);
+        int lobe;
+        wl = hair_sample_dir(sc.hc, h.hw, r0, r1, r2, r3, &lobe);
+      } else {
+        const DirPdf d = surface_sample_dp(h.alpha, h.wo_local, r0, r1);
+        wl = d.dir;
+        spdf = d.pdf;
+      }
+      wi = normalize(to_world(h.fr, wl));
+    }
+    const bool lower = dot(wi, h.n) < 0.f;
+    if (k < 2 && h.surface && lower) continue;  // no light from below a surface
+    V3 f;
+    float pdf;
+    if (!h.surface) {
+      const FPdf e = hair_eval_fp(sc.hc, h.hw, wl);
+      f = e.f;
+      pdf = e.pdf;
+    } else {
+      f = surface_eval(h.wo_local, wl, h.albedo, h.alpha);
+      pdf = k == 0 ? surface_pdf(h.alpha, h.wo_local, normalize(h.wo_local + wl)) : spdf;
+    }
+    if (k < 2) {
+      V3 nd = h.n;
+      if (!h.surface && lower) {
+        nd = -1.f * h.n;
+        pdl = pdl + (2.f * h.radius) * nd;
+      }
+      V3 c;
+      if (k == 0) {
+        const float lp = ls.light_pdf;
+        if (sc.mis && !ls.is_delta) c = f * ls.emit * power_heuristic(lp, pdf) / lp;
+        else c = f * ls.emit / lp;
+      } else {
+        const V3 e = env_radiance(sc.env, wi);
+        const float lp = env_pdf(sc.env, wi) * 1.f * inv_n;
+        c = f * e * power_heuristic(pdf, lp) / pdf;
+      }
+      light(k, pdl + 1e-3f * nd, wi, c);
+    } else {
+      V3 mul = (pdf == 0.f) ? f : f / pdf;
+      if (any_nan(mul)) mul = v3(1.f);
+      V3 p = h.p, nd = h.n;
+      if (lower && !h.surface) {
+        nd = -1.f * h.n;
+        p = p + (2.f * h.radius) * nd;
+      }
+      out.cont = 1;
+      out.o = p + 1e-3f * nd;
+      out.wi = wi;
+      out.mul = mul;
+      out.pdf = pdf;
+    }
+  }
+  return out;
+}
+
+// Writes a vertex's shadow records at index i of the pending-light arrays,
+// in the layout k_shadow and the resolve kernels read: (origin, 1 if the ray
+// exists), direction, contribution per ray, and the throughput.
+struct ShadowSink {
+  float4 *o1, *d1, *c1, *o2, *d2, *c2, *beta;
+  int i;
+  bool has1, has2;
+  HM_D ShadowSink(const PathBuffers& pb, int idx)
+      : o1(pb.sh_o1), d1(pb.sh_d1), c1(pb.sh_c1), o2(pb.sh_o2), d2(pb.sh_d2), c2(pb.sh_c2), beta(pb.sh_beta),
+        i(idx), has1(false), has2(false) {}
+  HM_D void operator()(int which, V3 o, V3 d, V3 c) {
+    float4* so = which ? o2 : o1;
+    float4* sd = which ? d2 : d1;
+    float4* scn = which ? c2 : c1;
+    so[i] = make_float4(o.x, o.y, o.z, 1.f);
+    sd[i] = make_float4(d.x, d.y, d.z, 0.f);
+    scn[i] = make_float4(c.x, c.y, c.z, 0.f);
+    if (which) has2 = true;
+    else has1 = true;
+  }
+  // after the vertex: clears the flag of a ray the vertex did not make when
+  // it made the other, and stores the throughput; returns whether any exists
+  HM_D bool finish(V3 b) {
+    if (!(has1 || has2)) return false;
+    if (!has1) o1[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!has2) o2[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    beta[i] = make_float4(b.x, b.y, b.z, 0.f);
+    return true;
+  }
+};
 
 // Closest hit against the head sphere and the hair BVH. The sphere is tested
 // first so its distance prunes the hair traversal; equal distances go to hair.

@@ -292,43 +292,33 @@ __global__ void __launch_bounds__(256) k_shade_nrc(SceneView sc, PathBuffers pb,
           nr.tb_wo[rec] = make_float4(h.wo.x, h.wo.y, h.wo.z, 0.f);
           nr.tb_n[rec] = make_float4(h.n.x, h.n.y, h.n.z, 0.f);
         }
-        DirectOut dl;
-        direct_lighting(sc, h, rng, dl);
-        if (dl.has1 || dl.has2) {
+        ShadowSink sink(pb, slot);
+        const bool last = depth == kNrcMaxBounces - 1;
+        // direct light, then nextPathVertexNRC (nrc.cu:16-67) unless this is
+        // the last bounce; NRC has no Russian roulette
+        const Continuation c = shade_vertex(sc, h, rng, true, false, !last, [](float) { return true; },
+                                            [&](int w, V3 lo, V3 ld, V3 lc) { sink(w, lo, ld, lc); });
+        if (sink.finish(beta)) {
           shadow = true;
-          ray1 = dl.has1 != 0;
-          ray2 = dl.has2 != 0;
-          pb.sh_o1[slot] = make_float4(dl.o1.x, dl.o1.y, dl.o1.z, dl.has1 ? 1.f : 0.f);
-          pb.sh_d1[slot] = make_float4(dl.d1.x, dl.d1.y, dl.d1.z, 0.f);
-          pb.sh_c1[slot] = make_float4(dl.c1.x, dl.c1.y, dl.c1.z, 0.f);
-          pb.sh_o2[slot] = make_float4(dl.o2.x, dl.o2.y, dl.o2.z, dl.has2 ? 1.f : 0.f);
-          pb.sh_d2[slot] = make_float4(dl.d2.x, dl.d2.y, dl.d2.z, 0.f);
-          pb.sh_c2[slot] = make_float4(dl.c2.x, dl.c2.y, dl.c2.z, 0.f);
-          pb.sh_beta[slot] = make_float4(beta.x, beta.y, beta.z, 0.f);
+          ray1 = sink.has1;
+          ray2 = sink.has2;
         }
-        if (depth == kNrcMaxBounces - 1) {
+        if (last) {
           // last bounce (nrc.cu:209-220)
           if (training) {
             nr.tb_bounces[tr] = depth;
             nr.tb_hit[tr] = 0;
           }
         } else {
-          // nextPathVertexNRC (nrc.cu:16-67)
-          const BsdfSample bs = sample_bsdf(sc, h, rng);
-          V3 mul = (bs.pdf == 0.f) ? bs.f : bs.f / bs.pdf;
-          if (any_nan(mul)) mul = v3(1.f);
+          const V3 mul = c.mul;
           beta = beta * mul;
           if (training) nr.tb_beta[rec] = make_float4(mul.x, mul.y, mul.z, 0.f);
-          ns.w = bs.pdf;
+          ns.w = c.pdf;
           nr.prev[slot] = make_float4(h.p.x, h.p.y, h.p.z, 0.f);
-          V3 p = h.p, nd = h.n;
-          if (dot(bs.wi, h.n) < 0.f && !h.surface) {
-            nd = -1.f * h.n;
-            p = p + (2.f * h.radius) * nd;
-          }
-          const V3 no = p + 1e-3f * nd;
+          const V3 no = c.o;
+          const V3 bs_wi = c.wi;
           pb.ray_o[slot] = make_float4(no.x, no.y, no.z, 0.f);
-          pb.ray_d[slot] = make_float4(bs.wi.x, bs.wi.y, bs.wi.z, 0.f);
+          pb.ray_d[slot] = make_float4(bs_wi.x, bs_wi.y, bs_wi.z, 0.f);
           pb.beta[slot] = make_float4(beta.x, beta.y, beta.z, 0.f);
           cont = true;
         }
@@ -373,7 +363,7 @@ __global__ void HM_TRACE_BOUNDS k_extend(SceneView sc, PathBuffers pb, int cur,
 }
 
 #ifndef HM_SHADE_MIN_WAVES
-#define HM_SHADE_MIN_WAVES 1
+#define HM_SHADE_MIN_WAVES 3
 #endif
 __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc, PathBuffers pb, ShadeParams sp,
                                                                    GBufferOut gb) {
@@ -412,60 +402,38 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc,
           const float4 s4 = pb.beta_short[slot];
           beta_s = v3(s4.x, s4.y, s4.z);
         }
-        // ---- direct lighting at this vertex ----
-        if (depth == 0 ? sp.dl_zero != 0 : depth >= sp.dl_from) {
-          DirectOut dl;
-          direct_lighting(sc, h, rng, dl);
-          if (dl.has1 || dl.has2) {
-            shadow = true;
-            ray1 = dl.has1 != 0;
-            ray2 = dl.has2 != 0;
-            pb.sh_o1[slot] = make_float4(dl.o1.x, dl.o1.y, dl.o1.z, dl.has1 ? 1.f : 0.f);
-            pb.sh_d1[slot] = make_float4(dl.d1.x, dl.d1.y, dl.d1.z, 0.f);
-            pb.sh_c1[slot] = make_float4(dl.c1.x, dl.c1.y, dl.c1.z, 0.f);
-            pb.sh_o2[slot] = make_float4(dl.o2.x, dl.o2.y, dl.o2.z, dl.has2 ? 1.f : 0.f);
-            pb.sh_d2[slot] = make_float4(dl.d2.x, dl.d2.y, dl.d2.z, 0.f);
-            pb.sh_c2[slot] = make_float4(dl.c2.x, dl.c2.y, dl.c2.z, 0.f);
-            pb.sh_beta[slot] = make_float4(beta.x, beta.y, beta.z, 0.f);
-            if (training) pb.sh_beta_short[slot] = make_float4(beta_s.x, beta_s.y, beta_s.z, 0.f);
-          }
-        }
-        // ---- Russian roulette (optix_common.cuh:398-405; hair_msnn.cu:110-128, 171-181) ----
-        bool alive = true;
-        if (depth >= 1) {
+        const V3 beta_dl = beta, beta_s_dl = beta_s;
+        ShadowSink sink(pb, slot);
+        // Russian roulette (optix_common.cuh:398-405; hair_msnn.cu:110-128, 171-181)
+        auto roulette = [&](float eps) {
           const float qq = fmaxf(0.05f, 1.f - luminance(beta));
-          const float eps = draw(rng);
           if (training) {
             const float qs = fmaxf(0.05f, 1.f - luminance(beta_s));
             if (eps < qs || depth > sp.trunc_beta) beta_s = v3(0.f);
-            if (eps < qq) {
-              alive = false;
-            } else {
-              beta = beta / (1.f - qq);
-              if (!is_zero(beta_s)) beta_s = beta_s / (1.f - qs);
-            }
-          } else {
-            if (eps < qq || (sp.msnn && depth > sp.trunc_beta)) alive = false;
-            else beta = beta / (1.f - qq);
+            if (eps < qq) return false;
+            beta = beta / (1.f - qq);
+            if (!is_zero(beta_s)) beta_s = beta_s / (1.f - qs);
+            return true;
           }
+          if (eps < qq || (sp.msnn && depth > sp.trunc_beta)) return false;
+          beta = beta / (1.f - qq);
+          return true;
+        };
+        const Continuation c = shade_vertex(sc, h, rng, depth == 0 ? sp.dl_zero != 0 : depth >= sp.dl_from,
+                                            depth >= 1, depth + 1 <= sp.max_vertex, roulette,
+                                            [&](int w, V3 lo, V3 ld, V3 lc) { sink(w, lo, ld, lc); });
+        if (sink.finish(beta_dl)) {
+          shadow = true;
+          ray1 = sink.has1;
+          ray2 = sink.has2;
+          if (training) pb.sh_beta_short[slot] = make_float4(beta_s_dl.x, beta_s_dl.y, beta_s_dl.z, 0.f);
         }
-        // ---- continuation (nextPathVertex, optix_common.cuh:311-360) ----
-        if (alive && depth + 1 <= sp.max_vertex) {
-          const BsdfSample bs = sample_bsdf(sc, h, rng);
-          const V3 wl = bs.wl, wi = bs.wi, f = bs.f;
-          const float pdf = bs.pdf;
-          V3 mul = (pdf == 0.f) ? f : f / pdf;
-          if (any_nan(mul)) mul = v3(1.f);
-          beta = beta * mul;
-          if (training) beta_s = beta_s * mul;
-          V3 p = h.p, nd = h.n;
-          if (dot(wi, h.n) < 0.f && !h.surface) {
-            nd = -1.f * h.n;
-            p = p + (2.f * h.radius) * nd;
-          }
-          const V3 no = p + 1e-3f * nd;
-          pb.ray_o[slot] = make_float4(no.x, no.y, no.z, 0.f);
-          pb.ray_d[slot] = make_float4(wi.x, wi.y, wi.z, 0.f);
+        // continuation (nextPathVertex, optix_common.cuh:311-360)
+        if (c.cont) {
+          beta = beta * c.mul;
+          if (training) beta_s = beta_s * c.mul;
+          pb.ray_o[slot] = make_float4(c.o.x, c.o.y, c.o.z, 0.f);
+          pb.ray_d[slot] = make_float4(c.wi.x, c.wi.y, c.wi.z, 0.f);
           pb.beta[slot] = make_float4(beta.x, beta.y, beta.z, 0.f);
           if (training) pb.beta_short[slot] = make_float4(beta_s.x, beta_s.y, beta_s.z, 0.f);
           cont = true;
@@ -521,46 +489,27 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade_dense(SceneVi
         rng = pb.rng[i];
         const float4 b4 = pb.beta[i];
         V3 beta = v3(b4.x, b4.y, b4.z);
-        if (depth == 0 ? sp.dl_zero != 0 : depth >= sp.dl_from) {
-          DirectOut dl;
-          direct_lighting(sc, h, rng, dl);
-          if (dl.has1 || dl.has2) {
-            shadow = true;
-            ray1 = dl.has1 != 0;
-            ray2 = dl.has2 != 0;
-            pb.sh_o1[i] = make_float4(dl.o1.x, dl.o1.y, dl.o1.z, dl.has1 ? 1.f : 0.f);
-            pb.sh_d1[i] = make_float4(dl.d1.x, dl.d1.y, dl.d1.z, 0.f);
-            pb.sh_c1[i] = make_float4(dl.c1.x, dl.c1.y, dl.c1.z, 0.f);
-            pb.sh_o2[i] = make_float4(dl.o2.x, dl.o2.y, dl.o2.z, dl.has2 ? 1.f : 0.f);
-            pb.sh_d2[i] = make_float4(dl.d2.x, dl.d2.y, dl.d2.z, 0.f);
-            pb.sh_c2[i] = make_float4(dl.c2.x, dl.c2.y, dl.c2.z, 0.f);
-            pb.sh_beta[i] = make_float4(beta.x, beta.y, beta.z, 0.f);
-          }
-        }
-        // Russian roulette (optix_common.cuh:398-405)
-        bool alive = true;
-        if (depth >= 1) {
+        const V3 beta_dl = beta;
+        ShadowSink sink(pb, i);
+        // Russian roulette (optix_common.cuh:398-405) from the second vertex on
+        auto roulette = [&](float eps) {
           const float qq = fmaxf(0.05f, 1.f - luminance(beta));
-          const float eps = draw(rng);
-          if (eps < qq) alive = false;
-          else beta = beta / (1.f - qq);
+          if (eps < qq) return false;
+          beta = beta / (1.f - qq);
+          return true;
+        };
+        const Continuation c = shade_vertex(sc, h, rng, depth == 0 ? sp.dl_zero != 0 : depth >= sp.dl_from,
+                                            depth >= 1, depth + 1 <= sp.max_vertex, roulette,
+                                            [&](int w, V3 lo, V3 ld, V3 lc) { sink(w, lo, ld, lc); });
+        if (sink.finish(beta_dl)) {
+          shadow = true;
+          ray1 = sink.has1;
+          ray2 = sink.has2;
         }
-        // continuation (nextPathVertex, optix_common.cuh:311-360)
-        if (alive && depth + 1 <= sp.max_vertex) {
-          const BsdfSample bs = sample_bsdf(sc, h, rng);
-          const V3 wi = bs.wi, f = bs.f;
-          const float pdf = bs.pdf;
-          V3 mul = (pdf == 0.f) ? f : f / pdf;
-          if (any_nan(mul)) mul = v3(1.f);
-          beta = beta * mul;
-          V3 p = h.p, nd = h.n;
-          if (dot(wi, h.n) < 0.f && !h.surface) {
-            nd = -1.f * h.n;
-            p = p + (2.f * h.radius) * nd;
-          }
-          const V3 no = p + 1e-3f * nd;
-          no4 = make_float4(no.x, no.y, no.z, 0.f);
-          wi4 = make_float4(wi.x, wi.y, wi.z, 0.f);
+        if (c.cont) {
+          beta = beta * c.mul;
+          no4 = make_float4(c.o.x, c.o.y, c.o.z, 0.f);
+          wi4 = make_float4(c.wi.x, c.wi.y, c.wi.z, 0.f);
           beta4 = make_float4(beta.x, beta.y, beta.z, 0.f);
           cont = true;
         }

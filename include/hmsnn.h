@@ -232,6 +232,10 @@ int hmsnn_msnn_begin(hmsnn_ctx* ctx, int BETA);
 int hmsnn_msnn_frames(hmsnn_ctx* ctx, int frames, hmsnn_msnn_stats* stats);
 /* which = 0 final mean, 1 short-path mean, 2 network mean (w x h x 4). */
 int hmsnn_msnn_get_image(hmsnn_ctx* ctx, int which, float* rgba_host);
+/* Writes the radiance cache of the HairMSNN state to a weight file (the
+ * format hmsnn_mlp_save writes), to be named by tcnn.init_weights
+ * (scene.cpp:330-331 -> render_hair_msnn.cu:109-110). */
+int hmsnn_msnn_save_weights(hmsnn_ctx* ctx, const char* path);
 /* sizes[0] training records per frame (nnTrainInput rows,
  * render_hair_msnn.cu:112-120), [1] every_nth (render_hair_msnn.cu:106),
  * [2] frames rendered (accumId), [3] training steps taken. */
@@ -299,6 +303,9 @@ int hmsnn_nrc_get_buffer(hmsnn_ctx* ctx, int which, void* host, size_t bytes);
 /* The cache network's weights (layout of hmsnn_mlp_get_weights). */
 int hmsnn_nrc_get_mlp_weights(hmsnn_ctx* ctx, float* w_host, int n);
 int hmsnn_nrc_set_mlp_weights(hmsnn_ctx* ctx, const float* w_host, int n);
+/* Writes the cache network to a weight file (the format hmsnn_mlp_save
+ * writes), for the scene's tcnn.init_weights (scene.cpp:330-331). */
+int hmsnn_nrc_save_weights(hmsnn_ctx* ctx, const char* path);
 /* Begin, render spp frames, copy the mean image. */
 int hmsnn_render_nrc(hmsnn_ctx* ctx, int spp, float* rgba_host, hmsnn_nrc_stats* stats);
 
@@ -315,6 +322,17 @@ int hmsnn_mlp_destroy(hmsnn_mlp* mlp);
  * each row-major [out][in]. n = 20672. */
 int hmsnn_mlp_get_weights(const hmsnn_mlp* mlp, float* w_host, int n);
 int hmsnn_mlp_set_weights(hmsnn_mlp* mlp, const float* w_host, int n);
+/* Weight files, the file behind the scene's tcnn.init_weights, which the
+ * reference loads with TINY_MLP::loadWeights (neural_network.cu:23-32,
+ * render_hair_msnn.cu:109-110, render_nrc.cu:149-150). The network is not
+ * tiny-cuda-nn's, so the file is this library's JSON document
+ * ({"format": "hairmsnn_amd.mlp", "version": 1, "architecture": {...},
+ * "params": [20672 numbers in the layout above]}); a tiny-cuda-nn
+ * serialisation is refused with HMSNN_EINVAL. hmsnn_msnn_begin and
+ * hmsnn_nrc_begin load the scene's tcnn.init_weights (relative to the scene
+ * file) into their network before the warm-up. */
+int hmsnn_mlp_save(const hmsnn_mlp* mlp, const char* path);
+int hmsnn_mlp_load(hmsnn_mlp* mlp, const char* path);
 /* Encodes 12-float records into 64 features (n x 64 FP32), for tests. */
 int hmsnn_mlp_encode_dev(const float* in_dev, int n, float* feat_dev, void* stream);
 /* Inference: in n x 12 FP32, out n x 3 FP32 (TINY_MLP::inference). */

@@ -179,3 +179,81 @@ def test_torch_autograd_harness(gpu):
     np.testing.assert_allclose(prm.grad.numpy(), gg, rtol=1e-6)
     np.testing.assert_allclose(sig.grad.numpy(), gs, rtol=1e-6, atol=1e-12)
     assert np.isfinite(gs).all() and np.abs(gg).max() > 0
+
+
+def c1_cfg(w, h, path_v2, env=True):
+    cfg = json.loads((SCENES / "c1_curly.json").read_text())
+    cfg["integrator"].update({"width": w, "height": h, "path_v2": path_v2})
+    if not env:
+        cfg["lights"].pop("environment", None)
+    return cfg
+
+
+def c3_sigma(ns):
+    """The differentiable-appearance configuration's per-strand absorption:
+    a uniform (0.01, 0.5) draw per strand and channel, seed 3."""
+    return np.random.default_rng(3).uniform(0.01, 0.5, (ns, 3)).astype(np.float32)
+
+
+def test_c3_on_c1_geometry_matches_fp64_oracle(gpu):
+    """C1's geometry (50,000 strands x 64 segments) with 50,000 x 3 per-strand
+    sigma_a, the procedural environment and the directional light, direct
+    lighting: per-pixel d/d(beta_m, beta_n, alpha) and per-strand d/d sigma_a
+    against the FP64 oracle adjoint on the same paths."""
+    w = h = 64
+    spp = 2
+    with gpu.Scene(json_text=c1_cfg(w, h, 1)) as sc:
+        ns = sc.num_strands
+        assert ns == 50_000
+        sig = c3_sigma(ns)
+        sc.set_strand_sigma_a(sig)
+        wts = loss_weights(w, h)
+        gs, gg, gp = sc.adjoint(wts, spp, per_pixel=True)
+        osc = ol.OracleScene(sc.info, sc.segments(), sc.env(), strand_sigma=sig, mesh=sc.mesh())
+    ogs, ogg, ogp = osc.adjoint(wts, spp, ns)
+    scale = np.abs(ogp).max(axis=(0, 1))
+    close = np.abs(gp - ogp) <= 2e-3 * np.abs(ogp) + 1e-4 * scale
+    frac = close.all(-1).mean()
+    print(f"C1 direct: GPU {gg} oracle {ogg}; pixels agreeing {frac:.4f}")
+    assert frac >= 0.95
+    seen = np.abs(ogs).max(1) > 0
+    ssc = np.abs(ogs).max()
+    sclose = (np.abs(gs - ogs) <= 5e-3 * np.abs(ogs) + 1e-4 * ssc).all(-1)[seen].mean()
+    corr = np.corrcoef(gs[seen].ravel(), ogs[seen].ravel())[0, 1]
+    print(f"strands seen {seen.sum()}, agreeing {sclose:.4f}, correlation {corr:.6f}")
+    assert seen.sum() > 500 and sclose >= 0.9 and corr > 0.99
+
+
+def test_c3_on_c1_geometry_central_differences(gpu):
+    """C1 geometry lit by the directional light only (the render is then
+    sampling-free in the parameters at the first vertex): 64 random
+    per-strand sigma_a entries among the strands the image sees, central
+    differences with h = 1e-3 against the adjoint."""
+    w = h = 64
+    spp = 2
+    with gpu.Scene(json_text=c1_cfg(w, h, 1, env=False)) as sc:
+        ns = sc.num_strands
+        sig = c3_sigma(ns)
+        sc.set_strand_sigma_a(sig)
+        wts = loss_weights(w, h)
+        gs, gg, _ = sc.adjoint(wts, spp, per_pixel=True)
+        rng = np.random.default_rng(7)
+        seen = np.flatnonzero(np.abs(gs).max(1) > 1e-3 * np.abs(gs).max())
+        assert seen.size > 100
+        picks = [(int(rng.choice(seen)), int(rng.integers(3))) for _ in range(64)]
+        bad = 0
+        for st, ch in picks:
+            lp, lm = sig.copy(), sig.copy()
+            lp[st, ch] += 1e-3
+            lm[st, ch] -= 1e-3
+            sc.set_strand_sigma_a(lp)
+            a = render_loss(sc, wts, spp)
+            sc.set_strand_sigma_a(lm)
+            b = render_loss(sc, wts, spp)
+            fd = (a - b) / (float(lp[st, ch]) - float(lm[st, ch]))
+            if abs(fd - gs[st, ch]) > 1e-2 * abs(gs[st, ch]) + 1e-4 * np.abs(gs).max():
+                bad += 1
+                print(f"strand {st} ch {ch}: adjoint {gs[st, ch]:.6e} fd {fd:.6e}")
+        sc.set_strand_sigma_a(sig)
+    print(f"64 central differences: {64 - bad} agree")
+    assert bad <= 2

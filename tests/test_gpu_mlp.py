@@ -158,13 +158,15 @@ def test_fp8_inference_against_fp64_and_fp16(gpu):
     assert np.quantile(e8, 0.99) <= 5e-2  # single outputs: 3 mantissa bits (test_fp8_format_error_level)
 
 
-@pytest.mark.parametrize("loss", [0, 2])
-def test_gradient_matches_oracle(gpu, loss):
+@pytest.mark.parametrize("loss,precision", [(0, 0), (2, 0), (0, 1), (2, 1)])
+def test_gradient_matches_oracle(gpu, loss, precision):
+    """FP16 (precision 0) and BF16 (1) training gradients against the FP64
+    oracle, normwise per layer within 2e-2."""
     n = 1 << 14
     r = records(n, 24)
     w = weights(25)
     t = np.random.default_rng(26).uniform(0.2, 3.0, (n, 3)).astype(np.float32)
-    m = Mlp(gpu, 0)
+    m = Mlp(gpu, precision)
     m.set_weights(w)
     lg, g = m.grad(r, t, loss)
     lo, go = ol.mlp_loss_grad(w, r, t, loss=loss, prec=0, exp_floor=ol.MLP_EXP_GRAD_FLOOR)
@@ -172,7 +174,7 @@ def test_gradient_matches_oracle(gpu, loss):
     for l in range(6):
         sl = slice(l * 4096, (l + 1) * 4096) if l < 5 else slice(5 * 4096, ol.MLP_WEIGHTS)
         rel = np.linalg.norm(g[sl] - go[sl]) / np.linalg.norm(go[sl])
-        print(f"loss {loss} layer {l}: relative gradient error {rel:.2e}")
+        print(f"loss {loss} precision {precision} layer {l}: relative gradient error {rel:.2e}")
         assert rel <= 2e-2
 
 
@@ -220,3 +222,47 @@ def test_adam_step_and_training_reduces_loss(gpu):
         losses.append(m.step(r, t, 2, 5e-3))
     print(f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
     assert losses[-1] < 0.2 * losses[0]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_gradient_batch_sizes_2e14_to_2e18(gpu, precision):
+    """The batch sweep 2^14 .. 2^18: the gradient of a batch is the
+    record-weighted mean of its parts' gradients (a size-independent
+    property, so the largest batches need no CPU oracle); 2^14 is checked
+    against the oracle above."""
+    w = weights(31)
+    m = Mlp(gpu, precision)
+    m.set_weights(w)
+    for lg in range(14, 19):
+        n = 1 << lg
+        r = records(n, 40 + lg)
+        t = np.random.default_rng(50 + lg).uniform(0.2, 3.0, (n, 3)).astype(np.float32)
+        _, g = m.grad(r, t, 2)
+        q = n // 4
+        parts = [m.grad(r[k * q:(k + 1) * q], t[k * q:(k + 1) * q], 2)[1].astype(np.float64) for k in range(4)]
+        gm = np.mean(parts, 0)
+        rel = np.linalg.norm(g - gm) / np.linalg.norm(gm)
+        print(f"precision {precision} batch 2^{lg}: |g - mean of quarters| / |g| = {rel:.2e}")
+        assert np.isfinite(g).all() and rel < 1e-3
+
+
+def test_weight_file_round_trip(gpu, tmp_path):
+    """hmsnn_mlp_save / hmsnn_mlp_load: the weights come back bit for bit
+    (9 significant digits round-trip FP32); a tiny-cuda-nn serialisation
+    or a missing file is refused with a message."""
+    w = weights(41)
+    a = Mlp(gpu, 0)
+    a.set_weights(w)
+    path = tmp_path / "cache.json"
+    gpu._lib.check(a.lib.hmsnn_mlp_save(a.h, str(path).encode()))
+    import json
+    doc = json.loads(path.read_text())
+    assert doc["format"] == "hairmsnn_amd.mlp" and doc["version"] == 1 and len(doc["params"]) == ol.MLP_WEIGHTS
+    b = Mlp(gpu, 1)
+    gpu._lib.check(b.lib.hmsnn_mlp_load(b.h, str(path).encode()))
+    np.testing.assert_array_equal(b.get_weights(), w)
+    (tmp_path / "tcnn.json").write_text(json.dumps({"loss": {}, "optimizer": {}, "snapshot": {"params_binary": []}}))
+    assert b.lib.hmsnn_mlp_load(b.h, str(tmp_path / "tcnn.json").encode()) == gpu._lib.HMSNN_EINVAL
+    assert b"tiny-cuda-nn" in b.lib.hmsnn_last_error()
+    assert b.lib.hmsnn_mlp_load(b.h, str(tmp_path / "missing.json").encode()) == gpu._lib.HMSNN_EIO
+    np.testing.assert_array_equal(b.get_weights(), w)  # a failed load leaves the weights alone

@@ -213,3 +213,50 @@ def test_trained_cache_gains_over_method_without_it(gpu):
           f"plain PT {r_pt:.4f}; wall time HairMSNN {t_ms:.3f} s (incl. warm-up) vs PT {t_pt:.3f} s")
     assert r_final < 0.8 * r_short
     assert r_final < r_pt
+
+
+def test_beta_zero_equals_beta_one(gpu):
+    """Kernel beta = BETA - 1 and the break test `bounces > beta` after the
+    second vertex's light make BETA 0 and BETA 1 the same truncation
+    (render_hair_msnn.cu:1170, hair_msnn.cu:166-179): identical frames."""
+    cfg = cfg_small(records=1024, warmup=4)
+    imgs = []
+    for beta in (0, 1):
+        with gpu.Scene(json_text=cfg) as sc:
+            sc.msnn_begin(BETA=beta)
+            sc.msnn_frames(3)
+            imgs.append((sc.msnn_image(0), sc.msnn_image(1)))
+    np.testing.assert_array_equal(imgs[0][1], imgs[1][1])
+    np.testing.assert_array_equal(imgs[0][0], imgs[1][0])
+
+
+def test_init_weights_from_scene(gpu, tmp_path):
+    """tcnn.init_weights (scene.cpp:330-331 -> render_hair_msnn.cu:109-110):
+    a network saved from one HairMSNN run is loaded by the next scene that
+    names it, before its warm-up; with no warm-up and no training the second
+    run's cache is that network exactly, and its frames match a run whose
+    network was set the same way."""
+    cfg = cfg_small(records=1024, warmup=8)
+    path = tmp_path / "cache.json"
+    with gpu.Scene(json_text=cfg) as sc:
+        sc.msnn_begin(BETA=1)
+        sc.msnn_frames(2)
+        sc.msnn_save_weights(str(path))
+    import json
+    saved = json.loads(path.read_text())["params"]
+    cfg2 = json.loads(json.dumps(cfg))
+    cfg2["hairmsnn"]["warmup_steps"] = 0
+    cfg2["tcnn"] = {"config": "unused.json", "init_train": False, "init_weights": str(path)}
+    with gpu.Scene(json_text=cfg2) as sc:
+        sc.msnn_begin(BETA=1)
+        sc.msnn_frames(1)
+        sc.msnn_save_weights(str(tmp_path / "again.json"))
+        net = sc.msnn_image(2)
+    again = json.loads((tmp_path / "again.json").read_text())["params"]
+    assert again == saved
+    assert np.isfinite(net).all()
+    cfg3 = json.loads(json.dumps(cfg2))
+    cfg3["tcnn"]["init_weights"] = str(tmp_path / "missing.json")
+    with gpu.Scene(json_text=cfg3) as sc:
+        with pytest.raises(gpu.HmsnnError):
+            sc.msnn_begin(BETA=1)

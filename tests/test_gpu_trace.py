@@ -116,3 +116,28 @@ def test_ploc_tree_bit_exact(gpu, name, radius):
     np.testing.assert_array_equal(t[hit], t_c[hit])
     occ_c = ol.intersect(seg, rays, any_hit=True, use_bvh=True)
     np.testing.assert_array_equal(occ, occ_c)
+
+
+def test_default_lbvh_full_c1_bit_exact(gpu):
+    """The default (radix-tree) builder on the full C1 geometry, 50,000
+    strands x 64 segments = 3.2M segments: closest and any hits bit-exact
+    against the CPU BVH."""
+    cfg = json.loads((SCENES / "c1_curly.json").read_text())
+    cfg["integrator"].update({"width": 128, "height": 128})
+    cfg.pop("surface", None)  # hair only: the CPU search below has no head sphere
+    cfg.pop("accel", None)
+    with gpu.Scene(json_text=cfg) as sc:
+        seg = sc.segments()
+        assert seg.shape[0] == 3_200_000
+        rays = make_rays(seg, 60_000, 21)
+        prim, t, u = sc.trace_rays(rays)
+        occ = sc.trace_rays(rays, any_hit=True)
+    prim_c, t_c, u_c = ol.intersect(seg, rays, use_bvh=True)
+    hit = prim_c >= 0
+    print(f"C1 LBVH: hit fraction {hit.mean():.3f}")
+    assert 0.2 < hit.mean() < 0.99
+    np.testing.assert_array_equal(prim, prim_c)
+    np.testing.assert_array_equal(t[hit], t_c[hit])
+    np.testing.assert_array_equal(u[hit], u_c[hit])
+    occ_c = ol.intersect(seg, rays, any_hit=True, use_bvh=True)
+    np.testing.assert_array_equal(occ, occ_c)
