@@ -780,6 +780,36 @@ int mlp_forward(MlpState& s, const float* in, int n, float* out, hipStream_t st,
   return check_launch("mlp infer", err);
 }
 
+// Clears n 32-bit words (a kernel, not hipMemsetAsync, so a caller can
+// capture the launch in a graph).
+__global__ void k_zero_words(uint32_t* p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+static int zero_words(void* p, size_t bytes, hipStream_t st, std::string& err) {
+  const size_t n = bytes / 4;
+  if (n == 0) return kOk;
+  const int blocks = (int)std::min<size_t>(1024, (n + 255) / 256);
+  hipLaunchKernelGGL(k_zero_words, dim3(blocks), dim3(256), 0, st, (uint32_t*)p, n);
+  return check_launch("mlp clear", err);
+}
+
+constexpr int kTrainChunk = 2048;  // records per dW partial
+
+// Scratch of one training step over npad padded records: activations H and
+// pre-activation gradients dZ (6 x 64 x npad 16-bit values each) and the dW
+// partial sums of kTrainChunk-record chunks.
+static size_t train_scratch_bytes(int npad, size_t esize, size_t* h_bytes, size_t* p_bytes) {
+  const int nchunks = (npad + kTrainChunk - 1) / kTrainChunk;
+  *h_bytes = esize * 6 * 64 * (size_t)npad;
+  *p_bytes = sizeof(float) * 6 * 4096 * (size_t)nchunks;
+  return 2 * *h_bytes + *p_bytes;
+}
+
+static int mlp_step_on(MlpState& s, void* H, void* dZ, float* partial, bool kernel_clears, const float* in,
+                       const float* target, int n, int loss, float* grad_out, float* loss_out, bool adam, float lr,
+                       hipStream_t st, std::string& err);
+
 // Forward + backward + weight-gradient GEMM + reduce (+ Adam when lr > 0).
 int mlp_loss_grad_step(MlpState& s, const float* in, const float* target, int n, int loss, float* grad_out,
                        float* loss_out, bool adam, float lr, hipStream_t st, std::string& err) {
@@ -787,9 +817,10 @@ int mlp_loss_grad_step(MlpState& s, const float* in, const float* target, int n,
     err = "empty training batch";
     return kEInval;
   }
-  const int npad = (n + 31) / 32 * 32;
-  const int chunk = 2048;
-  const int nchunks = (npad + chunk - 1) / chunk;
+  // padded to whole 128-record blocks: every wave of k_mlp_fwd_bwd's last
+  // block writes its 32-record tile into the feature-major arrays
+  const int npad = (n + 127) / 128 * 128;
+  const int nchunks = (npad + kTrainChunk - 1) / kTrainChunk;
   if (npad > s.cap) {
     dev_free(s.H);
     dev_free(s.dZ);
@@ -804,10 +835,27 @@ int mlp_loss_grad_step(MlpState& s, const float* in, const float* target, int n,
     HM_HIP(hipMemsetAsync(s.dZ, 0, s.esize() * 6 * 64 * (size_t)npad, st), err);
     s.cap = npad;
   }
+  return mlp_step_on(s, s.H, s.dZ, s.partial, false, in, target, n, loss, grad_out, loss_out, adam, lr, st, err);
+}
+
+static int mlp_step_on(MlpState& s, void* H, void* dZ, float* partial, bool kernel_clears, const float* in,
+                       const float* target, int n, int loss, float* grad_out, float* loss_out, bool adam, float lr,
+                       hipStream_t st, std::string& err) {
+  // padded to whole 128-record blocks: every wave of k_mlp_fwd_bwd's last
+  // block writes its 32-record tile into the feature-major arrays
+  const int npad = (n + 127) / 128 * 128;
+  const int chunk = kTrainChunk;
+  const int nchunks = (npad + chunk - 1) / chunk;
   // the layouts below index with the current npad: clear dZ_5 rows 3..63
-  HM_HIP(hipMemsetAsync((char*)s.dZ + s.esize() * ((size_t)5 * 64 + 3) * npad, 0, s.esize() * 61 * (size_t)npad, st),
-         err);
-  HM_HIP(hipMemsetAsync(s.loss_sum, 0, sizeof(float), st), err);
+  void* dz5 = (char*)dZ + s.esize() * ((size_t)5 * 64 + 3) * npad;
+  const size_t dz5_bytes = s.esize() * 61 * (size_t)npad;
+  if (kernel_clears) {
+    HM_TRY(zero_words(dz5, dz5_bytes, st, err));
+    HM_TRY(zero_words(s.loss_sum, sizeof(float), st, err));
+  } else {
+    HM_HIP(hipMemsetAsync(dz5, 0, dz5_bytes, st), err);
+    HM_HIP(hipMemsetAsync(s.loss_sum, 0, sizeof(float), st), err);
+  }
   const float grad_scale = 1.f / 64.f;
   const size_t smem = sizeof(f16x8) * kFragAll;  // 88 KB of weight fragments
   static bool attr_set = false;
@@ -820,19 +868,19 @@ int mlp_loss_grad_step(MlpState& s, const float* in, const float* target, int n,
            err);
     attr_set = true;
   }
-  const int blocks = npad / 128 + (npad % 128 ? 1 : 0);
+  const int blocks = npad / 128;
   if (s.bf16()) {
     hipLaunchKernelGGL(k_mlp_fwd_bwd<__bf16>, dim3(blocks), dim3(kMlpBlock), smem, st, (const __bf16*)s.frag, in,
-                       target, n, npad, loss, grad_scale, (__bf16*)s.H, (__bf16*)s.dZ, s.loss_sum);
+                       target, n, npad, loss, grad_scale, (__bf16*)H, (__bf16*)dZ, s.loss_sum);
     HM_TRY(check_launch("mlp fwd/bwd", err));
-    hipLaunchKernelGGL(k_mlp_dw<__bf16>, dim3(nchunks, 6), dim3(kMlpBlock), 0, st, (const __bf16*)s.H,
-                       (const __bf16*)s.dZ, npad, chunk, s.partial);
+    hipLaunchKernelGGL(k_mlp_dw<__bf16>, dim3(nchunks, 6), dim3(kMlpBlock), 0, st, (const __bf16*)H,
+                       (const __bf16*)dZ, npad, chunk, partial);
   } else {
     hipLaunchKernelGGL(k_mlp_fwd_bwd<_Float16>, dim3(blocks), dim3(kMlpBlock), smem, st, (const _Float16*)s.frag,
-                       in, target, n, npad, loss, grad_scale, (_Float16*)s.H, (_Float16*)s.dZ, s.loss_sum);
+                       in, target, n, npad, loss, grad_scale, (_Float16*)H, (_Float16*)dZ, s.loss_sum);
     HM_TRY(check_launch("mlp fwd/bwd", err));
-    hipLaunchKernelGGL(k_mlp_dw<_Float16>, dim3(nchunks, 6), dim3(kMlpBlock), 0, st, (const _Float16*)s.H,
-                       (const _Float16*)s.dZ, npad, chunk, s.partial);
+    hipLaunchKernelGGL(k_mlp_dw<_Float16>, dim3(nchunks, 6), dim3(kMlpBlock), 0, st, (const _Float16*)H,
+                       (const _Float16*)dZ, npad, chunk, partial);
   }
   HM_TRY(check_launch("mlp dW", err));
   const float inv_scale = 1.f / (grad_scale * 3.f * (float)n);
@@ -842,7 +890,7 @@ int mlp_loss_grad_step(MlpState& s, const float* in, const float* target, int n,
     bc1 = 1.f - std::pow(0.9f, (float)s.step);
     bc2 = 1.f - std::pow(0.99f, (float)s.step);
   }
-  hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((kMlpWeightCount + 255) / 256), dim3(256), 0, st, s.partial, nchunks,
+  hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((kMlpWeightCount + 255) / 256), dim3(256), 0, st, partial, nchunks,
                      inv_scale, grad_out, s.w, s.m, s.v, adam ? 1 : 0, lr, 0.9f, 0.99f, 1e-15f, 1e-6f, bc1, bc2);
   HM_TRY(check_launch("mlp reduce/adam", err));
   if (adam) HM_TRY(mlp_refresh(s, st, err));

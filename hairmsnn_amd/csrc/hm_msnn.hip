@@ -548,7 +548,10 @@ int msnn_frame(Context& c, MsnnState& m, EventTimer* tm, std::string& err) {
   cfg.accum_id = f;
   cfg.every_nth = m.every_nth;
   cfg.n_slots = 0;
-  cfg.max_vertex = std::min(cfg.max_vertex, m.kbeta + 1);  // the last vertex a truncated path lights
+  // the last vertex a truncated path lights: the break test `bounces > beta`
+  // runs from the second vertex on, so kernel beta -1 (BETA 0) truncates like
+  // beta 0 (hair_msnn.cu:166-179)
+  cfg.max_vertex = std::min(cfg.max_vertex, std::max(m.kbeta, 0) + 1);
   cfg.gb.enabled = 1;
   cfg.gb.nn_input = sl.nn_input;
   cfg.gb.hit_flags = sl.hit_flags;

@@ -155,8 +155,9 @@ def test_training_records_match_oracle(gpu):
     close = (np.abs(tgt[both] - ot) <= 1e-3 * np.maximum(np.abs(of), 1e-2)).all(-1)
     print(f"training targets agreeing with the oracle: {close.mean():.3f}")
     # full-depth paths: FP32 rounding that flips one sampling decision makes a
-    # path diverge (the per-pixel bound at 40 vertices elsewhere)
-    assert close.mean() >= 0.85
+    # path diverge; a target agrees only if both its full path (about 0.9 of
+    # C0 pixels at 40 vertices) and its short path agree
+    assert close.mean() >= 0.80
     # the target is full - short: never negative in expectation
     assert tgt[both].mean() >= -1e-6 and ot.mean() >= -1e-6
 
@@ -204,14 +205,23 @@ def test_trained_cache_gains_over_method_without_it(gpu):
         t_ms = time.perf_counter() - t0
         final = sc.msnn_image(0)[..., :3].astype(np.float64)
         short = sc.msnn_image(1)[..., :3].astype(np.float64)
+        hit = sc.msnn_buffer("gb_hit").reshape(final.shape[:2])
 
     def rmse(a):
         return float(np.sqrt(np.mean((a - ref) ** 2)))
 
     r_final, r_short, r_pt = rmse(final), rmse(short), rmse(pt)
+    # the truncated paths miss the light beyond the second vertex: their
+    # image is darker than the reference; the cache restores most of it
+    hair = (hit & 1).astype(bool) & ~(hit & 2).astype(bool)
+    b_short = short[hair].mean() / ref[hair].mean() - 1
+    b_final = final[hair].mean() / ref[hair].mean() - 1
     print(f"RMSE vs 1024 spp after {frames} frames: cache {r_final:.4f}, short paths alone {r_short:.4f}, "
-          f"plain PT {r_pt:.4f}; wall time HairMSNN {t_ms:.3f} s (incl. warm-up) vs PT {t_pt:.3f} s")
-    assert r_final < 0.8 * r_short
+          f"plain PT {r_pt:.4f}; relative bias: short {b_short:+.4f}, with the cache {b_final:+.4f}; "
+          f"wall time HairMSNN {t_ms:.3f} s (incl. warm-up) vs PT {t_pt:.3f} s")
+    assert b_short < -0.05
+    assert abs(b_final) < 0.5 * abs(b_short)
+    assert r_final < r_short
     assert r_final < r_pt
 
 
@@ -260,3 +270,34 @@ def test_init_weights_from_scene(gpu, tmp_path):
     with gpu.Scene(json_text=cfg3) as sc:
         with pytest.raises(gpu.HmsnnError):
             sc.msnn_begin(BETA=1)
+
+
+def test_training_pixel_shuffles(gpu):
+    """The per-frame shuffle of trainIdxs (render_hair_msnn.cu:712-714) picks
+    pixel i * every_nth + trainIdxs[i] % every_nth as training record i
+    (hair_msnn.cu:206-209). trainIdxs is a permutation of 0..n-1, so over a
+    frame every residue mod every_nth occurs exactly n / every_nth times;
+    consecutive frames draw different permutations; the same seed draws the
+    same sequence in a second context."""
+    cfg = cfg_small(records=1024, warmup=2)
+
+    def residues(frames):
+        out = []
+        with gpu.Scene(json_text=cfg) as sc:
+            sc.msnn_begin(BETA=1)
+            for _ in range(frames):
+                sc.msnn_frames(1)
+                z = sc.msnn_sizes()
+                pix = sc.msnn_buffer("train_pixel")
+                n, nth = z["train_records"], z["every_nth"]
+                np.testing.assert_array_equal(pix // nth, np.arange(n))
+                out.append(pix % nth)
+        return out, n, nth
+
+    a, n, nth = residues(4)
+    for r in a:
+        np.testing.assert_array_equal(np.bincount(r, minlength=nth), np.full(nth, n // nth))
+    assert all((a[k] != a[k + 1]).mean() > 0.5 for k in range(3))
+    b, _, _ = residues(4)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
