@@ -134,6 +134,7 @@ _SIGS = {
     "hmsnn_mlp_get_fp8_scales": (_I, [_P, _P]),
     "hmsnn_mlp_backward_dev": (_I, [_P, _P, _P, _I, _I, _P, C.POINTER(_F), _P]),
     "hmsnn_mlp_train_step_dev": (_I, [_P, _P, _P, _I, _I, _F, C.POINTER(_F), _P]),
+    "hmsnn_mlp_train_step_dev_noalloc": (_I, [_P, _P, _P, _I, _I, _F, _P, _P]),
     "hmsnn_save_exr": (_I, [C.c_char_p, _P, _I, _I]),
     "hmsnn_exr_size": (_I, [C.c_char_p, C.POINTER(_I), C.POINTER(_I)]),
     "hmsnn_load_exr": (_I, [C.c_char_p, _P, _I, _I]),

@@ -36,6 +36,8 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/hmsnn.h"
@@ -807,8 +809,8 @@ static size_t train_scratch_bytes(int npad, size_t esize, size_t* h_bytes, size_
 }
 
 static int mlp_step_on(MlpState& s, void* H, void* dZ, float* partial, bool kernel_clears, const float* in,
-                       const float* target, int n, int loss, float* grad_out, float* loss_out, bool adam, float lr,
-                       hipStream_t st, std::string& err);
+                       const float* target, int n, int loss, float* grad_out, float* loss_out, float* loss_dev,
+                       bool adam, float lr, hipStream_t st, std::string& err);
 
 // Forward + backward + weight-gradient GEMM + reduce (+ Adam when lr > 0).
 int mlp_loss_grad_step(MlpState& s, const float* in, const float* target, int n, int loss, float* grad_out,
@@ -835,12 +837,18 @@ int mlp_loss_grad_step(MlpState& s, const float* in, const float* target, int n,
     HM_HIP(hipMemsetAsync(s.dZ, 0, s.esize() * 6 * 64 * (size_t)npad, st), err);
     s.cap = npad;
   }
-  return mlp_step_on(s, s.H, s.dZ, s.partial, false, in, target, n, loss, grad_out, loss_out, adam, lr, st, err);
+  return mlp_step_on(s, s.H, s.dZ, s.partial, false, in, target, n, loss, grad_out, loss_out, nullptr, adam, lr, st,
+                     err);
+}
+
+// The batch loss into device memory: the summed loss over 3 n outputs.
+__global__ void k_loss_mean(const float* loss_sum, int n, float* out) {
+  if (threadIdx.x == 0) *out = *loss_sum / (3.f * (float)n);
 }
 
 static int mlp_step_on(MlpState& s, void* H, void* dZ, float* partial, bool kernel_clears, const float* in,
-                       const float* target, int n, int loss, float* grad_out, float* loss_out, bool adam, float lr,
-                       hipStream_t st, std::string& err) {
+                       const float* target, int n, int loss, float* grad_out, float* loss_out, float* loss_dev,
+                       bool adam, float lr, hipStream_t st, std::string& err) {
   // padded to whole 128-record blocks: every wave of k_mlp_fwd_bwd's last
   // block writes its 32-record tile into the feature-major arrays
   const int npad = (n + 127) / 128 * 128;
@@ -894,6 +902,10 @@ static int mlp_step_on(MlpState& s, void* H, void* dZ, float* partial, bool kern
                      inv_scale, grad_out, s.w, s.m, s.v, adam ? 1 : 0, lr, 0.9f, 0.99f, 1e-15f, 1e-6f, bc1, bc2);
   HM_TRY(check_launch("mlp reduce/adam", err));
   if (adam) HM_TRY(mlp_refresh(s, st, err));
+  if (loss_dev) {
+    hipLaunchKernelGGL(k_loss_mean, dim3(1), dim3(64), 0, st, s.loss_sum, n, loss_dev);
+    HM_TRY(check_launch("mlp loss", err));
+  }
   if (loss_out) {
     float h = 0.f;
     HM_HIP(hipMemcpyAsync(&h, s.loss_sum, sizeof(float), hipMemcpyDeviceToHost, st), err);
@@ -901,6 +913,63 @@ static int mlp_step_on(MlpState& s, void* H, void* dZ, float* partial, bool kern
     *loss_out = h / (3.f * (float)n);
   }
   return kOk;
+}
+
+// ---- a training step that allocates nothing --------------------------------
+// Callers that cannot allocate device memory (code under graph capture, tight
+// serving loops) get the training step's scratch (activations, pre-activation
+// gradients, dW partial sums) from this file-scope device array, sized for
+// batches of up to kNoAllocMaxRecords records (26 MB, held by every GPU the
+// library runs on until the process ends). Every call on a GPU uses that
+// GPU's copy and leaves its last contents, so calls that overlap on one GPU
+// race; each call clears the parts that must start at zero with a kernel.
+constexpr int kNoAllocMaxRecords = 16384;
+constexpr size_t kNoAllocHBytes = (size_t)2 * 6 * 64 * kNoAllocMaxRecords;  // 16-bit values
+constexpr size_t kNoAllocPartialBytes = sizeof(float) * 6 * 4096 * (kNoAllocMaxRecords / kTrainChunk);
+constexpr size_t kNoAllocScratchBytes = 2 * kNoAllocHBytes + kNoAllocPartialBytes;
+static_assert(kNoAllocHBytes % 256 == 0, "scratch parts stay 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// The device address of `scratch` on the current device, looked up once per
+// device with that device current (each GPU holds its own copy of the array
+// at its own address; host code uses this address, never the array's name).
+constexpr int kMaxDevices = 64;
+static void* g_scratch_addr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+static int scratch_address(void** out, std::string& err) {
+  int dev = 0;
+  HM_HIP(hipGetDevice(&dev), err);
+  if (dev < 0 || dev >= kMaxDevices) {
+    err = "device index out of range for the no-allocation scratch";
+    return kEDevice;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return kEDevice;
+    }
+    g_scratch_addr[dev] = p;
+  }
+  *out = g_scratch_addr[dev];
+  return kOk;
+}
+
+static int mlp_train_step_noalloc(MlpState& s, const float* in, const float* target, int n, int loss, float lr,
+                                  float* loss_dev, hipStream_t st, std::string& err) {
+  if (n <= 0 || n > kNoAllocMaxRecords) {
+    err = "hmsnn_mlp_train_step_dev_noalloc: batch of " + std::to_string(n) + " records; the scratch holds 1.." +
+          std::to_string(kNoAllocMaxRecords);
+    return kEInval;
+  }
+  void* base = nullptr;
+  HM_TRY(scratch_address(&base, err));
+  unsigned char* b = (unsigned char*)base;
+  return mlp_step_on(s, b, b + kNoAllocHBytes, (float*)(b + 2 * kNoAllocHBytes), true, in, target, n, loss,
+                     nullptr, nullptr, loss_dev, true, lr, st, err);
 }
 
 // Encoded features (FP32) for tests.
@@ -1007,6 +1076,15 @@ int hmsnn_mlp_backward_dev(hmsnn_mlp* m, const float* in, const float* target, i
     return fail(HMSNN_EINVAL, "hmsnn_mlp_backward_dev: bad arguments");
   std::string err;
   int rc = mlp_loss_grad_step(*m->s, in, target, n, loss, grad, loss_out, false, 0.f, (hipStream_t)stream, err);
+  return rc ? fail(rc, err) : HMSNN_OK;
+}
+
+int hmsnn_mlp_train_step_dev_noalloc(hmsnn_mlp* m, const float* in, const float* target, int n, int loss, float lr,
+                                     float* loss_dev, void* stream) {
+  if (!m || !in || !target || loss < 0 || loss > 2 || !(lr > 0.f))
+    return fail(HMSNN_EINVAL, "hmsnn_mlp_train_step_dev_noalloc: bad arguments");
+  std::string err;
+  int rc = mlp_train_step_noalloc(*m->s, in, target, n, loss, lr, loss_dev, (hipStream_t)stream, err);
   return rc ? fail(rc, err) : HMSNN_OK;
 }
 
