@@ -181,7 +181,7 @@ int alloc_path_buffers(PathBuffers& pb, std::vector<void*>& allocs, size_t n, bo
   HM_TRY(A((void**)&pb.shadow_queue, sizeof(int) * n));
   HM_TRY(A((void**)&pb.shadow_rays, sizeof(int) * 2 * n));
   HM_TRY(A((void**)&pb.sh_vis, 2 * n));
-  HM_TRY(A((void**)&pb.counts, sizeof(int) * 16));
+  HM_TRY(A((void**)&pb.counts, sizeof(int) * kCountInts));
   HM_TRY(A((void**)&pb.slot_pass, sizeof(int) * n));
   HM_TRY(A((void**)&pb.slot_px, sizeof(int) * n));
   HM_TRY(A((void**)&pb.slot_depth, sizeof(int) * n));
@@ -206,7 +206,7 @@ int alloc_path_buffers(PathBuffers& pb, std::vector<void*>& allocs, size_t n, bo
   pb.spill_stride = spill_stride;
   pb.spill_levels = spill_levels;
   HM_TRY(A((void**)&pb.spill, sizeof(int) * (size_t)spill_levels * spill_stride));
-  HM_HIP(hipMemsetAsync(pb.counts, 0, sizeof(int) * 16, st), err);
+  HM_HIP(hipMemsetAsync(pb.counts, 0, sizeof(int) * kCountInts, st), err);
   return kOk;
 }
 

@@ -48,6 +48,13 @@ struct SceneView {
 };
 
 constexpr int kDoneCount = 4;  // PathBuffers::counts index of the finished-path queue
+// PathBuffers::counts holds each queue length in a 256-byte line of its own:
+// counts[kCountStride * k] is the length of queue k (0 and 1 the two
+// wavefronts, 2 the shadow records, 3 the shadow rays, kDoneCount the
+// finished paths), so the appends to different queues do not contend for
+// one line.
+constexpr int kCountStride = 64;
+constexpr int kCountInts = kCountStride * (kDoneCount + 1);
 
 // Per-slot path state (structure of arrays in HBM). A slot is one pixel of
 // the current sample pass.

@@ -33,16 +33,50 @@ __device__ __forceinline__ int wave_append(int* counter, bool pred) {
   return pred ? base + rank : -1;
 }
 
-__device__ __forceinline__ int wave_append_ll(long long* counter, bool pred) {
-  const unsigned long long mask = __ballot(pred);
-  const int lane = __lane_id();
-  const int total = __popcll(mask);
-  long long base = 0;
-  const int leader = __ffsll((long long)mask) - 1;
-  if (total > 0 && lane == leader) base = (long long)atomicAdd((unsigned long long*)counter, (unsigned long long)total);
-  base = __shfl(base, leader < 0 ? 0 : leader);
-  const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-  return pred ? (int)base + rank : -1;
+// Queue positions of one vertex after a shading step.
+struct Appended {
+  int next, done, shadow, ray1, ray2;
+};
+
+// The appends of a 256-thread block (four waves) to the queues a shading
+// step fills, with one atomic per queue and block: each wave's counts go to
+// LDS, thread q adds queue q's block total to its counter and hands each wave
+// its base. The shadow-ray queue takes a wave's ray1 entries, then its ray2
+// entries. Every thread of the block calls this (the grid-stride loops here
+// run the same number of iterations in every thread of a block); a null
+// counter must receive no entries.
+__device__ __forceinline__ Appended block_append(int* next_q, int* done_q, int* shadow_q, int* ray_q, bool next,
+                                                 bool done, bool shadow, bool ray1, bool ray2) {
+  __shared__ int tot[4][4], base[4][4];
+  const unsigned long long mn = __ballot(next), md = __ballot(done), ms = __ballot(shadow), m1 = __ballot(ray1),
+                           m2 = __ballot(ray2);
+  const int w = threadIdx.x >> 6, lane = __lane_id();
+  if (lane == 0) {
+    tot[w][0] = __popcll(mn);
+    tot[w][1] = __popcll(md);
+    tot[w][2] = __popcll(ms);
+    tot[w][3] = __popcll(m1) + __popcll(m2);
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int q = threadIdx.x;
+    const int sum = tot[0][q] + tot[1][q] + tot[2][q] + tot[3][q];
+    int* c = q == 0 ? next_q : q == 1 ? done_q : q == 2 ? shadow_q : ray_q;
+    int b = sum > 0 ? atomicAdd(c, sum) : 0;
+    for (int k = 0; k < 4; ++k) {
+      base[k][q] = b;
+      b += tot[k][q];
+    }
+  }
+  __syncthreads();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  Appended a;
+  a.next = next ? base[w][0] + __popcll(mn & below) : -1;
+  a.done = done ? base[w][1] + __popcll(md & below) : -1;
+  a.shadow = shadow ? base[w][2] + __popcll(ms & below) : -1;
+  a.ray1 = ray1 ? base[w][3] + __popcll(m1 & below) : -1;
+  a.ray2 = ray2 ? base[w][3] + __popcll(m1) + __popcll(m2 & below) : -1;
+  return a;
 }
 
 __device__ __forceinline__ TraceStack make_stack(int* lds_base, const PathBuffers& pb) {
@@ -94,10 +128,10 @@ __device__ __forceinline__ void camera_path(const SceneView& sc, const PathBuffe
 
 __global__ void k_reset_counts(int* counts, int a, int b, int c) {
   if (threadIdx.x == 0) {
-    counts[a] = 0;
-    counts[b] = 0;
-    counts[c] = 0;
-    counts[kDoneCount] = 0;
+    counts[kCountStride * a] = 0;
+    counts[kCountStride * b] = 0;
+    counts[kCountStride * c] = 0;
+    counts[kCountStride * kDoneCount] = 0;
   }
 }
 
@@ -109,7 +143,7 @@ __global__ void k_camera(SceneView sc, PathBuffers pb, int accum_id, int msnn, c
                          int every_nth, int n_train, int nrc, int all_unbiased) {
   const int n = sc.width * sc.height;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) pb.counts[0] = n;
+  if (i == 0) pb.counts[kCountStride * 0] = n;
   if (i >= n) return;
   camera_path(sc, pb, i, i, accum_id);
   pb.queue[0][i] = i;
@@ -138,7 +172,7 @@ __global__ void k_camera(SceneView sc, PathBuffers pb, int accum_id, int msnn, c
 __global__ void k_camera_pixels(SceneView sc, PathBuffers pb, int accum_id, int n, int chunk, int every_nth,
                                 const int* train_idx, int* pix) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) pb.counts[0] = n;
+  if (i == 0) pb.counts[kCountStride * 0] = n;
   if (i >= n) return;
   const int c = i / chunk, j = i - c * chunk;
   const int p = j * every_nth + train_idx[i] % every_nth;
@@ -168,7 +202,7 @@ __global__ void k_camera_pixels(SceneView sc, PathBuffers pb, int accum_id, int 
 __global__ void k_camera_points(SceneView sc, PathBuffers pb, int accum_id, int n, int chunk, int grid_w,
                                 const float* pts) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) pb.counts[0] = n;
+  if (i == 0) pb.counts[kCountStride * 0] = n;
   if (i >= n) return;
   const int c = i / chunk, j = i - c * chunk;
   const int x = j % grid_w, y = j / grid_w;
@@ -197,7 +231,7 @@ __global__ void k_camera_points(SceneView sc, PathBuffers pb, int accum_id, int 
 __global__ void __launch_bounds__(256) k_shade_nrc(SceneView sc, PathBuffers pb, int depth, int cur,
                                                    NrcPassArgs nr) {
   const int nxt = cur ^ 1;
-  const int n = pb.counts[cur];
+  const int n = pb.counts[kCountStride * cur];
   const int* q = pb.queue[cur];
   const int stride = gridDim.x * blockDim.x;
   const int nloop = (n + stride - 1) / stride * stride;
@@ -327,14 +361,12 @@ __global__ void __launch_bounds__(256) k_shade_nrc(SceneView sc, PathBuffers pb,
       }
       pb.flags[slot] = fl;
     }
-    const int qpos = wave_append(&pb.counts[nxt], cont);
-    if (cont) pb.queue[nxt][qpos] = slot;
-    const int spos = wave_append(&pb.counts[2], shadow);
-    if (shadow) pb.shadow_queue[spos] = slot;
-    const int r1 = wave_append(&pb.counts[3], ray1);
-    if (ray1) pb.shadow_rays[r1] = 2 * slot;
-    const int r2 = wave_append(&pb.counts[3], ray2);
-    if (ray2) pb.shadow_rays[r2] = 2 * slot + 1;
+    const Appended a = block_append(&pb.counts[kCountStride * nxt], nullptr, &pb.counts[kCountStride * 2],
+                                    &pb.counts[kCountStride * 3], cont, false, shadow, ray1, ray2);
+    if (cont) pb.queue[nxt][a.next] = slot;
+    if (shadow) pb.shadow_queue[a.shadow] = slot;
+    if (ray1) pb.shadow_rays[a.ray1] = 2 * slot;
+    if (ray2) pb.shadow_rays[a.ray2] = 2 * slot + 1;
   }
 }
 
@@ -345,7 +377,7 @@ __global__ void HM_TRACE_BOUNDS k_extend(SceneView sc, PathBuffers pb, int cur,
   __shared__ int lds_stack[kStackLds * kTraceBlock];
   TraceStack st = make_stack(lds_stack, pb);
   TraceCounters cnt{0, 0, 0};
-  const int n = pb.counts[cur];
+  const int n = pb.counts[kCountStride * cur];
   const int* q = pb.queue[cur];
   const int stride = gridDim.x * blockDim.x;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -368,7 +400,7 @@ __global__ void HM_TRACE_BOUNDS k_extend(SceneView sc, PathBuffers pb, int cur,
 __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc, PathBuffers pb, ShadeParams sp,
                                                                    GBufferOut gb) {
   const int nxt = sp.cur ^ 1;
-  const int n = pb.counts[sp.cur];
+  const int n = pb.counts[kCountStride * (sp.cur)];
   const int* q = pb.queue[sp.cur];
   const int stride = gridDim.x * blockDim.x;
   const int nloop = (n + stride - 1) / stride * stride;
@@ -441,15 +473,13 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc,
         pb.rng[slot] = rng;
       }
     }
-    const int qpos = wave_append(&pb.counts[nxt], cont);
-    if (cont) pb.queue[nxt][qpos] = slot;
-    const int spos = wave_append(&pb.counts[2], shadow);
-    if (shadow) pb.shadow_queue[spos] = slot;
+    const Appended a = block_append(&pb.counts[kCountStride * nxt], nullptr, &pb.counts[kCountStride * 2],
+                                    &pb.counts[kCountStride * 3], cont, false, shadow, ray1, ray2);
+    if (cont) pb.queue[nxt][a.next] = slot;
+    if (shadow) pb.shadow_queue[a.shadow] = slot;
     // one entry per shadow ray (slot * 2 + which), so k_shadow traces one ray per lane
-    const int r1 = wave_append(&pb.counts[3], ray1);
-    if (ray1) pb.shadow_rays[r1] = 2 * slot;
-    const int r2 = wave_append(&pb.counts[3], ray2);
-    if (ray2) pb.shadow_rays[r2] = 2 * slot + 1;
+    if (ray1) pb.shadow_rays[a.ray1] = 2 * slot;
+    if (ray2) pb.shadow_rays[a.ray2] = 2 * slot + 1;
   }
 }
 
@@ -461,7 +491,7 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc,
 __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade_dense(SceneView sc, PathBuffers pb,
                                                                           PathBuffers nb, ShadeParams sp) {
   const int nxt = sp.cur ^ 1;
-  const int n = pb.counts[sp.cur];
+  const int n = pb.counts[kCountStride * (sp.cur)];
   const int stride = gridDim.x * blockDim.x;
   const int nloop = (n + stride - 1) / stride * stride;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
@@ -516,8 +546,10 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade_dense(SceneVi
       }
     }
     const bool done = live && !cont;
-    const int p = wave_append(&pb.counts[nxt], cont);
-    const int dp = wave_append(&pb.counts[kDoneCount], done);
+    const Appended a = block_append(&pb.counts[kCountStride * nxt], &pb.counts[kCountStride * kDoneCount],
+                                    &pb.counts[kCountStride * 2], &pb.counts[kCountStride * 3], cont, done, shadow,
+                                    ray1, ray2);
+    const int p = a.next, dp = a.done;
     if (cont) {
       nb.ray_o[p] = no4;
       nb.ray_d[p] = wi4;
@@ -535,19 +567,16 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade_dense(SceneVi
       pb.done_pass[dp] = pb.slot_pass[i];
       pb.dest[i] = ~dp;
     }
-    const int spos = wave_append(&pb.counts[2], shadow);
-    if (shadow) pb.shadow_queue[spos] = i;
-    const int r1 = wave_append(&pb.counts[3], ray1);
-    if (ray1) pb.shadow_rays[r1] = 2 * i;
-    const int r2 = wave_append(&pb.counts[3], ray2);
-    if (ray2) pb.shadow_rays[r2] = 2 * i + 1;
+    if (shadow) pb.shadow_queue[a.shadow] = i;
+    if (ray1) pb.shadow_rays[a.ray1] = 2 * i;
+    if (ray2) pb.shadow_rays[a.ray2] = 2 * i + 1;
   }
 }
 
 // The resolve of k_shadow_resolve for a compacted wavefront: the visible
 // contributions go to the path where k_shade_dense moved it.
 __global__ void k_shadow_resolve_dense(PathBuffers pb, PathBuffers nb) {
-  const int n = pb.counts[2];
+  const int n = pb.counts[kCountStride * 2];
   const int stride = gridDim.x * blockDim.x;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
     const int slot = pb.shadow_queue[j];
@@ -579,7 +608,7 @@ __global__ void HM_TRACE_BOUNDS k_shadow(SceneView sc, PathBuffers pb, unsigned 
   __shared__ int lds_stack[kStackLds * kTraceBlock];
   TraceStack st = make_stack(lds_stack, pb);
   TraceCounters cnt{0, 0, 0};
-  const int n = pb.counts[3];
+  const int n = pb.counts[kCountStride * 3];
   const int stride = gridDim.x * blockDim.x;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const int e = pb.shadow_rays[i];
@@ -595,7 +624,7 @@ __global__ void HM_TRACE_BOUNDS k_shadow(SceneView sc, PathBuffers pb, unsigned 
 // NRC training pixels also record the vertex's direct light (vertRadiance,
 // cuda/nrc.cu:200).
 __global__ void k_shadow_resolve(PathBuffers pb, int msnn, int depth, NrcPassArgs nr) {
-  const int n = pb.counts[2];
+  const int n = pb.counts[kCountStride * 2];
   const int stride = gridDim.x * blockDim.x;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const int slot = pb.shadow_queue[i];
@@ -784,7 +813,7 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
 __global__ void k_stream_init(long long* ctl, int* counts, int slots) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < kStreamCtl) ctl[i] = i == 2 ? slots : 0;
-  if (i <= kDoneCount) counts[i] = 0;
+  if (i <= kDoneCount) counts[kCountStride * i] = 0;
 }
 
 // Pixel px's planned passes p0 .. p0 + k - 1 from the ring into the running
@@ -814,8 +843,8 @@ __device__ __forceinline__ void accumulate_planned(const float4* ring, float4* a
 // entry; the free capacity for the plan; the new jobs' camera rays appended
 // to wavefront nb after the continuing paths.
 __global__ void k_stream_finish_dense(PathBuffers pb, long long* ctl, float4* ring, int npx, int nxt, int slots) {
-  const int n = pb.counts[kDoneCount];
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctl[2] = slots - pb.counts[nxt];  // free path capacity
+  const int n = pb.counts[kCountStride * kDoneCount];
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl[2] = slots - pb.counts[kCountStride * nxt];  // free path capacity
   const int stride = gridDim.x * blockDim.x;
   const int nloop = (n + stride - 1) / stride * stride;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
@@ -859,10 +888,10 @@ __global__ void k_stream_next_dense(SceneView sc, PathBuffers nb, const long lon
       const long long d = (r - acc % kStreamRing + kStreamRing) % kStreamRing;  // ring entry r holds pass acc + d
       T[8 + r] = d < k ? 0 : S[8 + r];
     }
-    nb.counts[cur] = 0;
-    nb.counts[2] = 0;
-    nb.counts[3] = 0;
-    nb.counts[kDoneCount] = 0;
+    nb.counts[kCountStride * cur] = 0;
+    nb.counts[kCountStride * 2] = 0;
+    nb.counts[kCountStride * 3] = 0;
+    nb.counts[kCountStride * kDoneCount] = 0;
   }
   const int work = max(k > 0 ? cnt : 0, n);
   const int stride = gridDim.x * blockDim.x;
@@ -870,7 +899,7 @@ __global__ void k_stream_next_dense(SceneView sc, PathBuffers nb, const long lon
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
     if (k > 0 && i < cnt) accumulate_planned(ring, accum, average, acc, k, first_pass, npx, stream_group_pixel(i, g, K));
     const bool go = i < n;
-    const int q = wave_append(&nb.counts[nxt], go);
+    const int q = wave_append(&nb.counts[kCountStride * nxt], go);
     if (go) {
       const long long job = job0 + i;
       const int pass = (int)(job / cnt);
@@ -1020,7 +1049,8 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
       iters[g] += kBatch;
       HM_TRY(check_launch("stream iteration", err));
       const int s = 2 * g + (batch & 1);
-      HM_HIP(hipMemcpyAsync(&live_host[s], pb.counts + cur[g], sizeof(int), hipMemcpyDeviceToHost, st), err);
+      HM_HIP(hipMemcpyAsync(&live_host[s], pb.counts + kCountStride * cur[g], sizeof(int), hipMemcpyDeviceToHost, st),
+             err);
       HM_HIP(hipEventRecord(live_ev[s], st), err);
     }
     if (!any) break;
