@@ -8,11 +8,16 @@ traced to termination with its next-event-estimation and MIS shadow rays
 (max depth 40, Russian roulette). The scene, BVH, environment tables and path
 state are resident in HBM before the timed region. One JSON line is printed.
 
-Roofline: the dominant kernel is the closest-hit BVH traversal (extend).
-Algorithmic bytes per launch = 4-wide nodes visited x 128 B + segments
-tested x 32 B + 48 B per ray (32 B ray in, 16 B hit out), counted by the
-same traversal compiled with counters; divided by the extend kernel's mean
-launch time from HIP events on the context's stream.
+Roofline: every pass kernel is timed alone with HIP events on its stream
+(extend: closest-hit traversal; shade: hit setup, light and BSDF samples,
+compaction; shadow: any-hit traversal), and the line's `roofline` is the
+one with the most time per pass. Algorithmic bytes: traversal = 4-wide nodes
+visited x 128 B + segments tested x 32 B + ray in / hit out (48 B closest,
+36 B any-hit), counted by the same traversal compiled with counters; shade =
+per vertex 56 B of path state read, per continuing vertex 60 B more read
+(hit, RNG, throughput, segment) and 92 B moved into the next wavefront, per
+finished path 36 B, per shadow ray 52 B written. HBM traffic per launch comes
+from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in profiles/.
 
 CPU baseline: the C++ oracle (FP32 twin of the reference integrator, OpenMP)
 on a bounded crop of the same scene, run on this machine's host cores.
@@ -35,9 +40,15 @@ METRIC = "paths/s @1024² 3.2M segs; HairMSNN β=1 vs plain PT time-to-equal-RMS
 HBM_PEAK_GBS = 8000.0
 SCENE = ROOT / "hairmsnn_amd" / "scenes" / "c1_curly.json"
 NODE_BYTES, SEG_BYTES, RAY_BYTES = 128, 32, 48  # 4-wide node, segment record, ray in + hit out
-# HBM bytes per k_extend launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-# (copy-calibrated); counters cannot be read from inside this process
+ANY_RAY_BYTES = 36  # shadow ray in (32 B) + visibility out
+# shade: path state read per vertex, more per continuing vertex, moved per
+# continuing vertex, per finished path, per shadow ray
+SHADE_VERTEX_BYTES, SHADE_HIT_BYTES, SHADE_MOVE_BYTES, SHADE_DONE_BYTES, SHADE_SHADOW_BYTES = 56, 60, 92, 36, 52
+# HBM bytes per launch of each pass kernel from the rocprofv3 FETCH_SIZE /
+# WRITE_SIZE runs (copy-calibrated); counters cannot be read from inside this
+# process
 TRAFFIC_JSON = ROOT / "profiles" / "pt_c1_1024_hbm_traffic.json"
+PASS_KERNELS = {"extend": "k_extend<false, true>", "shade": "k_shade_dense", "shadow": "k_shadow<false>"}
 
 
 def log(msg: str) -> None:
@@ -102,7 +113,7 @@ def mlp_rates(hm, n: int = 1 << 20, reps: int = 10) -> dict:
 def pt_reference(sc, ref_spp: int, frames: int) -> dict:
     """A ref_spp path-traced reference on independent random streams, and the
     plain tracer's MSE(k) against it for k = 1..frames passes, fitted as
-    a / k + b (b: the reference's own noise)."""
+    a / k + b with b = a / ref_spp (the reference's own noise)."""
     sc.set_seed_offset(1 << 22)
     sc.pt_reset()
     t0 = time.perf_counter()
@@ -121,16 +132,19 @@ def pt_reference(sc, ref_spp: int, frames: int) -> dict:
         if k & (k - 1) == 0 or k == frames:
             ks.append(k)
             pt_mse.append(float(np.mean((sc.image(0)[..., :3].astype(np.float64) - ref) ** 2)))
-    ks_a, m_a = np.array(ks, float), np.array(pt_mse)
-    A = np.stack([1.0 / ks_a, np.ones_like(ks_a)], 1)
-    (a, b), *_ = np.linalg.lstsq(A, m_a, rcond=None)
-    return {"ref": ref, "spp": ref_spp, "a": float(a), "b": float(max(b, 0.0)), "ks": ks, "mse": pt_mse}
+    # the reference is the same unbiased estimator at ref_spp passes on
+    # independent streams, so MSE_PT(k) = a (1 / k + 1 / ref_spp): one
+    # parameter, fitted in relative terms so that no single k dominates
+    x, m_a = 1.0 / np.array(ks, float) + 1.0 / ref_spp, np.array(pt_mse)
+    a = float(np.sum(x / m_a) / np.sum((x / m_a) ** 2))
+    return {"ref": ref, "spp": ref_spp, "a": a, "b": a / ref_spp, "ks": ks, "mse": pt_mse}
 
 
 def msnn_vs_pt(sc, pt_ms_per_pass: float, R: dict, frames: int, beta: int = 1) -> dict:
     """HairMSNN at BETA vs plain path tracing: the time the plain tracer needs
     to reach the MSE HairMSNN has after its warm-up and `frames` frames
-    (MSE_PT(k) = a / k + b solved for k, times the timed per-pass cost)
+    (MSE_PT(k) = a / k + b solved for k, times the timed per-pass cost;
+    b, the reference's own noise, is taken off both sides)
     over HairMSNN's wall time (warm-up training + frames). The
     scene-sample generation inside hmsnn_msnn_begin is one-time scene setup,
     like the BVH build the PT time excludes too, and is reported, not counted."""
@@ -337,8 +351,8 @@ def main() -> None:
     ap.add_argument("--scene", default=str(SCENE))
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the path tracer shards by image tiles; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: this build has no multi-GPU path (the path tracer would shard "
+              "a frame by image tiles, one process per GPU). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_mib > 0:
@@ -397,20 +411,33 @@ def main() -> None:
     kt = sc.time_passes(passes_t)
     sc.set_pass_concurrency(args.pass_concurrency)
     launches = max(kt["extend_launches"], 1)
-    extend_ms = kt["extend_ms"] / launches
-    bytes_pass = (stats["closest_nodes"] * NODE_BYTES + stats["closest_segs"] * SEG_BYTES
-                  + stats["closest_rays"] * RAY_BYTES)
     launches_pass = launches / passes_t
-    bytes_launch = bytes_pass / launches_pass
-    achieved = bytes_launch / (extend_ms * 1e-3) / 1e9
-    any_bytes = (stats["any_nodes"] * NODE_BYTES + stats["any_segs"] * SEG_BYTES + stats["any_rays"] * 36)
-    shadow_gbs = any_bytes / (kt["shadow_ms"] / passes_t * 1e-3) / 1e9
-    log(f"extend: {extend_ms:.3f} ms/launch, {bytes_launch / 1e6:.1f} MB/launch -> {achieved:.0f} GB/s; "
-        f"shadow {shadow_gbs:.0f} GB/s; shade {kt['shade_ms'] / passes_t:.2f} ms/pass")
-
-    traffic = None
+    V, P, S = stats["closest_rays"], float(paths_per_step), stats["any_rays"]
+    alg_pass = {
+        "extend": stats["closest_nodes"] * NODE_BYTES + stats["closest_segs"] * SEG_BYTES + V * RAY_BYTES,
+        "shade": (V * SHADE_VERTEX_BYTES + max(V - P, 0.0) * (SHADE_HIT_BYTES + SHADE_MOVE_BYTES)
+                  + P * SHADE_DONE_BYTES + S * SHADE_SHADOW_BYTES),
+        "shadow": stats["any_nodes"] * NODE_BYTES + stats["any_segs"] * SEG_BYTES + S * ANY_RAY_BYTES,
+    }
+    traffic_k = {}
     if TRAFFIC_JSON.exists():
-        traffic = json.loads(TRAFFIC_JSON.read_text()).get("traffic_bytes_per_launch")
+        traffic_k = json.loads(TRAFFIC_JSON.read_text()).get("kernels", {})
+    per_kernel = {}
+    for k, kname in PASS_KERNELS.items():
+        ms_pass = kt[f"{k}_ms"] / passes_t
+        ms_launch = ms_pass / launches_pass
+        b_launch = alg_pass[k] / launches_pass
+        ach = b_launch / (ms_launch * 1e-3) / 1e9
+        tr = traffic_k.get(kname, {}).get("traffic_bytes_per_launch")
+        per_kernel[k] = {"kernel": kname, "ms_per_pass": ms_pass, "ms_per_launch": ms_launch,
+                         "bytes_per_launch": b_launch, "achieved": ach, "frac": ach / HBM_PEAK_GBS, "traffic": tr,
+                         "traffic_GBps": tr / (ms_launch * 1e-3) / 1e9 if tr else None}
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["ms_per_pass"])
+    for k, v in per_kernel.items():
+        log(f"{k}: {v['ms_per_pass']:.2f} ms/pass, {v['ms_per_launch']:.3f} ms/launch, "
+            f"{v['bytes_per_launch'] / 1e6:.1f} MB/launch -> {v['achieved']:.0f} GB/s ({100 * v['frac']:.1f}% of HBM)")
+    log(f"dominant kernel by time: {dom}")
+    d = per_kernel[dom]
     line = {
         "metric": METRIC,
         "value": value,
@@ -433,13 +460,13 @@ def main() -> None:
             "segments": int(sc.info["num_segments"]), "bvh_depth": int(sc.info["bvh_depth"]),
         },
         "roofline": {
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": "k_extend (closest-hit LBVH traversal)",
+            "bound": "hbm", "achieved": d["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": d["frac"], "traffic": d["traffic"], "kernel": d["kernel"],
             "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/pt_c1_1024_hbm_traffic.md",
-            "bytes_per_launch": bytes_launch, "ms_per_launch": extend_ms,
+            "bytes_per_launch": d["bytes_per_launch"], "ms_per_launch": d["ms_per_launch"],
             "nodes_per_ray": stats["closest_nodes"] / max(stats["closest_rays"], 1),
             "segs_per_ray": stats["closest_segs"] / max(stats["closest_rays"], 1),
+            "pass_kernels": per_kernel,
         },
         "kernels_ms_per_pass": {
             "extend": kt["extend_ms"] / passes_t, "shade": kt["shade_ms"] / passes_t,

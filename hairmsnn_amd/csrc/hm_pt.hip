@@ -8,8 +8,11 @@
 //              continuation sample; survivors are compacted into the next queue
 //     shadow : any-hit tests for the pending light samples, then accumulate
 //   accumulate: running sum and mean per pixel                   (utils.cuh:13-35)
-// Queues are compacted with one atomic per wave (ballot + popcount). Kernels
-// read the live count from device memory, so a pass needs no host round trip.
+// Queues are compacted with ballots and popcounts and one atomic per block and
+// queue (shading) or per wave (the rest), each queue length in a cache line of
+// its own. Kernels read the live count from device memory, so a pass needs no
+// host round trip.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -77,6 +80,21 @@ __device__ __forceinline__ Appended block_append(int* next_q, int* done_q, int* 
   a.ray1 = ray1 ? base[w][3] + __popcll(m1 & below) : -1;
   a.ray2 = ray2 ? base[w][3] + __popcll(m1) + __popcll(m2 & below) : -1;
   return a;
+}
+
+// Launches a kernel; with a timer that has room, as record `kind` whose
+// events are taken from the dispatch itself (the kernel's start and end, no
+// marker packets around it).
+template <typename K, typename... Args>
+void launch_timed(PassTimer* tm, int kind, K kernel, dim3 grid, dim3 block, hipStream_t st, Args... args) {
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (tm && tm->used < tm->capacity) {
+    tm->kinds[tm->used] = kind;
+    t0 = tm->events[2 * tm->used];
+    t1 = tm->events[2 * tm->used + 1];
+    tm->used++;
+  }
+  hipExtLaunchKernelGGL(kernel, grid, block, 0, st, t0, t1, 0, args...);
 }
 
 __device__ __forceinline__ TraceStack make_stack(int* lds_base, const PathBuffers& pb) {
@@ -731,18 +749,7 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
   HM_TRY(check_launch("camera", err));
   const bool count = cfg.stats != nullptr;
   PassTimer* tm = cfg.timer;
-  auto tic = [&](int kind) {
-    if (tm && tm->used < tm->capacity) {
-      tm->kinds[tm->used] = kind;
-      (void)hipEventRecord(tm->events[2 * tm->used], stream);
-    }
-  };
-  auto toc = [&]() {
-    if (tm && tm->used < tm->capacity) {
-      (void)hipEventRecord(tm->events[2 * tm->used + 1], stream);
-      tm->used++;
-    }
-  };
+
   // Grids sized to the pass's paths (queues only shrink with depth), so a
   // small training-data pass does not dispatch the full-frame grid per depth.
   const int n0 = cfg.camera_mode != 0 ? cfg.n_slots : sc.width * sc.height;
@@ -755,14 +762,10 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
   for (int depth = 0; depth <= std::max(0, cfg.max_vertex); ++depth) {
     const int nxt = cur ^ 1;
     hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, stream, pb.counts, nxt, 2, 3);
-    tic(0);
     if (count)
-      hipLaunchKernelGGL(k_extend<true>, dim3(ext_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cur,
-                         cfg.stats);
+      launch_timed(tm, 0, k_extend<true>, dim3(ext_blocks), dim3(kTraceBlock), stream, sc, pb, cur, cfg.stats);
     else
-      hipLaunchKernelGGL(k_extend<false>, dim3(ext_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cur,
-                         cfg.stats);
-    toc();
+      launch_timed(tm, 0, k_extend<false>, dim3(ext_blocks), dim3(kTraceBlock), stream, sc, pb, cur, cfg.stats);
     ShadeParams sp;
     sp.depth = depth;
     sp.dl_zero = cfg.dl_zero;
@@ -771,19 +774,15 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
     sp.trunc_beta = cfg.trunc_beta;
     sp.msnn = cfg.msnn;
     sp.cur = cur;
-    tic(1);
     if (cfg.nrc.enabled)
-      hipLaunchKernelGGL(k_shade_nrc, dim3(shade_blocks), dim3(256), 0, stream, sc, pb, depth, cur, cfg.nrc);
+      launch_timed(tm, 1, k_shade_nrc, dim3(shade_blocks), dim3(256), stream, sc, pb, depth, cur, cfg.nrc);
     else
-      hipLaunchKernelGGL(k_shade, dim3(shade_blocks), dim3(256), 0, stream, sc, pb, sp, cfg.gb);
-    toc();
-    tic(2);
+      launch_timed(tm, 1, k_shade, dim3(shade_blocks), dim3(256), stream, sc, pb, sp, cfg.gb);
     if (count)
-      hipLaunchKernelGGL(k_shadow<true>, dim3(sh_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cfg.stats);
+      launch_timed(tm, 2, k_shadow<true>, dim3(sh_blocks), dim3(kTraceBlock), stream, sc, pb, cfg.stats);
     else
-      hipLaunchKernelGGL(k_shadow<false>, dim3(sh_blocks), dim3(kTraceBlock), 0, stream, sc, pb, cfg.stats);
-    hipLaunchKernelGGL(k_shadow_resolve, dim3(shade_blocks), dim3(256), 0, stream, pb, cfg.msnn, depth, cfg.nrc);
-    toc();
+      launch_timed(tm, 2, k_shadow<false>, dim3(sh_blocks), dim3(kTraceBlock), stream, sc, pb, cfg.stats);
+    launch_timed(tm, 3, k_shadow_resolve, dim3(shade_blocks), dim3(256), stream, pb, cfg.msnn, depth, cfg.nrc);
     HM_TRY(check_launch("wavefront vertex", err));
     cur = nxt;
   }
@@ -944,19 +943,7 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
   const int npx = sc.width * sc.height;
   if (passes <= 0 || npx <= 0 || K < 1) return passes <= 0 ? kOk : (err = "bad stream groups", kEInval);
   PassTimer* tm = cfg.timer;
-  hipStream_t tst = streams[0];
-  auto tic = [&](int kind) {
-    if (tm && tm->used < tm->capacity) {
-      tm->kinds[tm->used] = kind;
-      (void)hipEventRecord(tm->events[2 * tm->used], tst);
-    }
-  };
-  auto toc = [&]() {
-    if (tm && tm->used < tm->capacity) {
-      (void)hipEventRecord(tm->events[2 * tm->used + 1], tst);
-      tm->used++;
-    }
-  };
+
 // Two paths in flight per pixel of a group (four groups: 2M paths at 1024²)
 // measured 3% faster than one; four were 4% slower (a longer final drain).
 #ifndef HM_STREAM_SLOT_FACTOR
@@ -1014,10 +1001,9 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
         const int nxt = cur[g] ^ 1;
         const PathBuffers& A = views[cur[g]];
         const PathBuffers& B = views[nxt];
-        if (timed) tic(0);
-        hipLaunchKernelGGL((k_extend<false, true>), dim3(ext_blocks), dim3(kTraceBlock), 0, st, sc, A, cur[g],
-                           (unsigned long long*)nullptr);
-        if (timed) toc();
+        PassTimer* t = timed ? tm : nullptr;
+        launch_timed(t, 0, k_extend<false, true>, dim3(ext_blocks), dim3(kTraceBlock), st, sc, A, cur[g],
+                     (unsigned long long*)nullptr);
         ShadeParams sp;
         sp.depth = 0;
         sp.dl_zero = cfg.dl_zero;
@@ -1026,24 +1012,18 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
         sp.trunc_beta = cfg.trunc_beta;
         sp.msnn = 0;
         sp.cur = cur[g];
-        if (timed) tic(1);
-        hipLaunchKernelGGL(k_shade_dense, dim3(shade_blocks), dim3(256), 0, st, sc, A, B, sp);
-        if (timed) toc();
-        if (timed) tic(2);
-        hipLaunchKernelGGL(k_shadow<false>, dim3(sh_blocks), dim3(kTraceBlock), 0, st, sc, A,
-                           (unsigned long long*)nullptr);
-        hipLaunchKernelGGL(k_shadow_resolve_dense, dim3(shade_blocks), dim3(256), 0, st, A, B);
-        if (timed) toc();
-        if (timed) tic(3);
+        launch_timed(t, 1, k_shade_dense, dim3(shade_blocks), dim3(256), st, sc, A, B, sp);
+        launch_timed(t, 2, k_shadow<false>, dim3(sh_blocks), dim3(kTraceBlock), st, sc, A,
+                     (unsigned long long*)nullptr);
+        launch_timed(t, 3, k_shadow_resolve_dense, dim3(shade_blocks), dim3(256), st, A, B);
         // scheduler state: next call m reads half m % 2 and writes the other
         long long* c = ctl + (size_t)g * kStreamCtl;
         long long* S = c + kStreamHalf * (calls[g] & 1);
         long long* T = c + kStreamHalf * ((calls[g] + 1) & 1);
-        hipLaunchKernelGGL(k_stream_finish_dense, dim3(shade_blocks), dim3(256), 0, st, A, S, ring, npx, nxt, slots);
-        hipLaunchKernelGGL(k_stream_next_dense, dim3(shade_blocks), dim3(256), 0, st, sc, B, S, T, ring, accum,
-                           average, cur[g], nxt, cfg.accum_id, cfg.seed_offset, npx, g, K, cnt, passes);
+        launch_timed(t, 3, k_stream_finish_dense, dim3(shade_blocks), dim3(256), st, A, S, ring, npx, nxt, slots);
+        launch_timed(t, 3, k_stream_next_dense, dim3(shade_blocks), dim3(256), st, sc, B, S, T, ring, accum,
+                     average, cur[g], nxt, cfg.accum_id, cfg.seed_offset, npx, g, K, cnt, passes);
         ++calls[g];
-        if (timed) toc();
         cur[g] = nxt;
       }
       iters[g] += kBatch;
