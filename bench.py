@@ -484,6 +484,14 @@ def main() -> None:
             str(bt): {k: v for k, v in msnn_vs_pt(sc, wall / args.steps * 1e3, R, 64, beta=bt).items()
                       if k in ("speedup", "t_msnn_s", "pt_spp_equal_rmse", "rmse_msnn", "rmse_short_paths_only")}
             for bt in (2, 5, 10)}
+        # the same comparison with the radiance cache trained in BF16 and
+        # queried in FP8-E4M3, against the same reference image
+        cfg8 = json.loads(Path(args.scene).read_text())
+        cfg8.setdefault("hairmsnn", {})["precision"] = "fp8"
+        with hm.Scene(json_text=cfg8, base_dir=str(Path(args.scene).resolve().parent)) as sc8:
+            line["msnn_vs_pt"]["fp8"] = {
+                k: v for k, v in msnn_vs_pt(sc8, wall / args.steps * 1e3, R, 64, beta=1).items()
+                if k in ("speedup", "t_msnn_s", "pt_spp_equal_rmse", "rmse_msnn", "rmse_short_paths_only")}
         line["nrc_vs_pt"] = nrc_vs_pt(sc, wall / args.steps * 1e3, R, 64)
         fm = line["msnn_vs_pt"]["msnn_frame_ms"]
         train_records = int(json.loads(Path(args.scene).read_text()).get("hairmsnn", {}).get("train_records", 16384))

@@ -148,6 +148,21 @@ def generate_host(json_text: str | dict) -> tuple[np.ndarray, np.ndarray | None]
     return seg, env
 
 
+def generate_curves_host(json_text: str | dict) -> np.ndarray:
+    """A .hair scene's round Catmull-Rom segments (n x 20: four control
+    points x, y, z, radius; strand; end flags; 0; 0) on the host."""
+    if isinstance(json_text, dict):
+        json_text = json.dumps(json_text)
+    lib = load()
+    n = C.c_int()
+    t = json_text.encode()
+    check(lib.hmsnn_generate_curves_host(t, None, 0, C.byref(n)))
+    crv = np.empty((n.value, 20), np.float32)
+    if n.value:
+        check(lib.hmsnn_generate_curves_host(t, _ptr(crv), crv.size, C.byref(n)))
+    return crv
+
+
 def load_obj(path: str) -> dict:
     """An OBJ head mesh as the library reads it (host only): vertices and
     normals (T x 3 x 3) and diffuse colours (T x 3) per triangle."""
@@ -209,6 +224,15 @@ class Scene:
     def segments(self) -> np.ndarray:
         out = np.empty((int(self.info["num_segments"]), 8), dtype=np.float32)
         check(self.lib.hmsnn_export_segments(self.h, _ptr(out)))
+        return out
+
+    def curves(self) -> np.ndarray | None:
+        """Round Catmull-Rom segments (n x 20), None when the hair is linear."""
+        n = int(self.info["num_curves"])
+        if n == 0:
+            return None
+        out = np.empty((n, 20), dtype=np.float32)
+        check(self.lib.hmsnn_export_curves(self.h, _ptr(out)))
         return out
 
     def mesh(self) -> dict | None:

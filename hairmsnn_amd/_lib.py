@@ -43,7 +43,7 @@ class Info(C.Structure):
         ("alpha", C.c_float), ("gain", C.c_float * 4), ("env_scale", C.c_float), ("env_rot_phi", C.c_float),
         ("sphere", C.c_float * 4), ("sphere_albedo", C.c_float * 3), ("sphere_alpha", C.c_float),
         ("camera", C.c_float * 12), ("dir_from", (C.c_float * 3) * 8), ("dir_emit", (C.c_float * 3) * 8),
-        ("num_triangles", C.c_int),
+        ("num_triangles", C.c_int), ("num_curves", C.c_int),
     ]
 
 
@@ -89,6 +89,8 @@ _SIGS = {
     "hmsnn_destroy": (_I, [_P]),
     "hmsnn_get_info": (_I, [_P, C.POINTER(Info)]),
     "hmsnn_export_segments": (_I, [_P, _P]),
+    "hmsnn_export_curves": (_I, [_P, _P]),
+    "hmsnn_generate_curves_host": (_I, [C.c_char_p, _P, C.c_size_t, C.POINTER(_I)]),
     "hmsnn_export_mesh": (_I, [_P, _P, _P, _P]),
     "hmsnn_obj_load": (_I, [C.c_char_p, C.POINTER(_I), _P, _P, _P, _I]),
     "hmsnn_export_env": (_I, [_P, _P, _P, _P, _P, _P]),

@@ -86,7 +86,10 @@ HM_D V3 adjoint_path(const SceneView& sc, const AdjArgs& a, TraceStack& st, int 
   V3 o = sc.cam_pos;
   float t, u;
   int prim;
-  closest_hit<false>(sc, o, d, st, t, prim, u, cnt);
+  if (sc.curves)
+    closest_hit<false, true>(sc, o, d, st, t, prim, u, cnt);
+  else
+    closest_hit<false>(sc, o, d, st, t, prim, u, cnt);
   if (prim == kPrimMiss) return sc.env.enabled ? env_radiance(sc.env, d) : v3(0.f);
   V3 beta = v3(1.f), L = v3(0.f);
   for (int depth = 0; depth <= a.max_vertex; ++depth) {
@@ -96,8 +99,10 @@ HM_D V3 adjoint_path(const SceneView& sc, const AdjArgs& a, TraceStack& st, int 
     if (depth == 0 ? a.dl_zero != 0 : depth >= a.dl_from) {
       DirectOut dl;
       direct_lighting(sc, h, rng, dl);
-      const bool v1 = dl.has1 && !occluded<false>(sc, dl.o1, dl.d1, st, cnt);
-      const bool v2 = dl.has2 && !occluded<false>(sc, dl.o2, dl.d2, st, cnt);
+      const bool v1 = dl.has1 && !(sc.curves ? occluded<false, true>(sc, dl.o1, dl.d1, st, cnt)
+                                             : occluded<false>(sc, dl.o1, dl.d1, st, cnt));
+      const bool v2 = dl.has2 && !(sc.curves ? occluded<false, true>(sc, dl.o2, dl.d2, st, cnt)
+                                             : occluded<false>(sc, dl.o2, dl.d2, st, cnt));
       V3 sum = v3(0.f);
       if (v1) sum = sum + dl.c1;
       if (v2) sum = sum + dl.c2;
@@ -155,7 +160,10 @@ HM_D V3 adjoint_path(const SceneView& sc, const AdjArgs& a, TraceStack& st, int 
     }
     o = p + 1e-3f * nd;
     d = wi;
-    closest_hit<false>(sc, o, d, st, t, prim, u, cnt);
+    if (sc.curves)
+      closest_hit<false, true>(sc, o, d, st, t, prim, u, cnt);
+    else
+      closest_hit<false>(sc, o, d, st, t, prim, u, cnt);
     if (prim == kPrimMiss) break;
   }
   return L;

@@ -129,6 +129,7 @@ Context::~Context() {
   free_path_buffers();
   dev_free(scene_points);
   dev_free(seg_dev);
+  dev_free(crv_dev);
   dev_free(strand_sigma_dev);
   dev_free(stats_dev);
   dev_free(dlights_dev);
@@ -332,15 +333,54 @@ int Context::init(std::string& err) {
   }
   if (scene.has_hair) {
     if (scene.hair_procedural) generate_hair(scene.hair_gen, hair);
-    else HM_TRY(load_hair_file(scene_path(scene, scene.hair_geometry), scene.hair_tessellation, hair, err));
+    else
+      HM_TRY(load_hair_file(scene_path(scene, scene.hair_geometry), scene.hair_tessellation, scene.hair_curves, hair,
+                            err));
     bmin = vmin(bmin, hair.bmin);
     bmax = vmax(bmax, hair.bmax);
     sscale = std::max(sscale, length(hair.bmax - hair.bmin));  // render_path_tracing.cu:212-214
     const size_t bytes = sizeof(float) * hair.seg.size();
     HM_TRY(dev_alloc((void**)&seg_dev, bytes, err));
     HM_HIP(hipMemcpyAsync(seg_dev, hair.seg.data(), bytes, hipMemcpyHostToDevice, stream), err);
-    HM_TRY(bvh_build(seg_dev, (int)(hair.seg.size() / 8), bvh, stream, err,
-                     scene.bvh_builder == "ploc" ? kBvhPloc : kBvhLbvh, scene.ploc_radius));
+    const int builder = scene.bvh_builder == "ploc" ? kBvhPloc : kBvhLbvh;
+    if (hair.crv.empty()) {
+      HM_TRY(bvh_build(seg_dev, (int)(hair.seg.size() / 8), bvh, stream, err, builder, scene.ploc_radius));
+    } else {
+      // round curves: the BVH over each segment's box (the Bezier hull of
+      // the Catmull-Rom segment, padded by the radius bound), stored as
+      // zero-radius segment records for the builder; the curves themselves
+      // as 5 float4 per segment
+      const int nc = (int)(hair.crv.size() / kCurveFloats);
+      std::vector<float> boxes((size_t)8 * nc);
+      for (int k = 0; k < nc; ++k) {
+        const float* q = &hair.crv[(size_t)kCurveFloats * k];
+        const float rmax = curve_radius_bound(q[3], q[7], q[11], q[15]);
+        float rec[8];
+        for (int a = 0; a < 3; ++a) {
+          const float b0 = q[4 + a], b3 = q[8 + a];
+          const float b1 = b0 + (q[8 + a] - q[a]) * (1.f / 6.f), b2 = b3 - (q[12 + a] - q[4 + a]) * (1.f / 6.f);
+          const float lo = std::min(std::min(b0, b1), std::min(b2, b3));
+          const float hi = std::max(std::max(b0, b1), std::max(b2, b3));
+          const float pad = 1.01f * rmax + 1e-5f * (std::fabs(lo) + std::fabs(hi)) + 1e-7f;
+          rec[a] = lo - pad;
+          rec[4 + a] = hi + pad;
+        }
+        rec[3] = 0.f;
+        rec[7] = (float)k;
+        std::copy(rec, rec + 8, &boxes[8 * (size_t)k]);
+      }
+      HM_TRY(dev_alloc((void**)&crv_dev, sizeof(float) * hair.crv.size(), err));
+      HM_HIP(hipMemcpyAsync(crv_dev, hair.crv.data(), sizeof(float) * hair.crv.size(), hipMemcpyHostToDevice,
+                            stream),
+             err);
+      float* boxes_dev = nullptr;
+      HM_TRY(dev_alloc((void**)&boxes_dev, sizeof(float) * boxes.size(), err));
+      HM_HIP(hipMemcpy(boxes_dev, boxes.data(), sizeof(float) * boxes.size(), hipMemcpyHostToDevice), err);
+      const int rc = bvh_build(boxes_dev, nc, bvh, stream, err, builder, scene.ploc_radius);
+      (void)hipStreamSynchronize(stream);
+      dev_free(boxes_dev);
+      HM_TRY(rc);
+    }
   }
   scene_scale = sscale > 0.f ? sscale : 1.f;
 
@@ -369,6 +409,8 @@ int Context::init(std::string& err) {
   v.bvh.segs = bvh.segs;
   v.bvh.num_segs = bvh.num_segs;
   v.seg_by_id = seg_dev;
+  v.curves = reinterpret_cast<const float4*>(crv_dev);
+  v.curve_caps = 1;  // render_path_tracing's forceCaps (render_path_tracing.cu:223)
   v.strand_sigma = nullptr;
   v.sphere.enabled = scene.has_surface && scene.surface_sphere ? 1 : 0;
   v.mesh.bvh.nodes = mesh_bvh.nodes_w;
@@ -552,6 +594,7 @@ int hmsnn_get_info(const hmsnn_ctx* ctx, hmsnn_info* info) {
   std::memset(info, 0, sizeof(*info));
   fill_scene_info(c.scene, info);
   info->num_segments = c.bvh.num_segs;
+  info->num_curves = (int)(c.hair.crv.size() / kCurveFloats);
   info->num_strands = c.hair.num_strands;
   info->num_triangles = c.mesh.num();
   info->bvh_nodes = c.bvh.num_nodes_w;
@@ -591,7 +634,7 @@ int hmsnn_generate_host(const char* text, float* seg_host, size_t seg_cap, int* 
   HairGeometry hg;
   if (s.has_hair && s.hair_procedural) generate_hair(s.hair_gen, hg);
   else if (s.has_hair) {
-    const int rc = load_hair_file(scene_path(s, s.hair_geometry), s.hair_tessellation, hg, err);
+    const int rc = load_hair_file(scene_path(s, s.hair_geometry), s.hair_tessellation, s.hair_curves, hg, err);
     if (rc) return fail(rc, err);
   }
   const size_t nseg = hg.seg.size() / 8;
@@ -612,6 +655,30 @@ int hmsnn_generate_host(const char* text, float* seg_host, size_t seg_cap, int* 
     if (env_cap < env.rgba.size()) return fail(HMSNN_EINVAL, "environment buffer too small");
     std::memcpy(env_rgba, env.rgba.data(), sizeof(float) * env.rgba.size());
   }
+  return HMSNN_OK;
+}
+
+int hmsnn_generate_curves_host(const char* text, float* crv_host, size_t crv_cap, int* num_curves) {
+  if (!text) return fail(HMSNN_EINVAL, "hmsnn_generate_curves_host: NULL json");
+  Scene s;
+  std::string err;
+  if (!parse_scene_text(text, ".", s, err)) return fail(HMSNN_EINVAL, err);
+  HairGeometry hg;
+  if (s.has_hair && !s.hair_procedural) {
+    const int rc = load_hair_file(scene_path(s, s.hair_geometry), s.hair_tessellation, s.hair_curves, hg, err);
+    if (rc) return fail(rc, err);
+  }
+  if (num_curves) *num_curves = (int)(hg.crv.size() / kCurveFloats);
+  if (crv_host) {
+    if (crv_cap < hg.crv.size()) return fail(HMSNN_EINVAL, "curve buffer too small");
+    std::memcpy(crv_host, hg.crv.data(), sizeof(float) * hg.crv.size());
+  }
+  return HMSNN_OK;
+}
+
+int hmsnn_export_curves(const hmsnn_ctx* ctx, float* crv_host) {
+  if (!ctx || !crv_host) return fail(HMSNN_EINVAL, "hmsnn_export_curves: NULL argument");
+  std::memcpy(crv_host, ctx->c.hair.crv.data(), sizeof(float) * ctx->c.hair.crv.size());
   return HMSNN_OK;
 }
 

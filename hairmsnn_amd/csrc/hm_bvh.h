@@ -12,6 +12,7 @@
 #pragma once
 
 #include "hm_common.h"
+#include "hm_curve.h"
 #include "hm_geom.h"
 
 namespace hm {
@@ -70,6 +71,12 @@ constexpr int kTraceBlock = 256;
 #define HM_TRACE_MIN_WAVES 8
 #endif
 #define HM_TRACE_BOUNDS __launch_bounds__(kTraceBlock, HM_TRACE_MIN_WAVES)
+// Traversal kernels over round curves: the curve iteration needs more
+// registers than the segment test, fewer waves.
+#ifndef HM_CURVE_MIN_WAVES
+#define HM_CURVE_MIN_WAVES 4
+#endif
+#define HM_TRACE_BOUNDS_T(curves) __launch_bounds__(kTraceBlock, (curves) ? HM_CURVE_MIN_WAVES : HM_TRACE_MIN_WAVES)
 
 struct TraceCounters {
   uint32_t nodes;
@@ -199,6 +206,21 @@ struct TriLeaf {
     float v;
     return intersect_triangle(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, t,
                               u, &v);
+  }
+};
+
+// Round Catmull-Rom hair: the record holds the segment's box, the control
+// points (5 float4 per segment: q0..q3, then strand and end flags) are read
+// by id; caps: end caps on (the flags then say which ends have one).
+struct CurveLeaf {
+  const float4* crv;
+  int caps;
+  HM_D bool operator()(const BvhView& bvh, const RayInv& r, int leaf, float* t, float* u, int* id) const {
+    const int k = __float_as_int(bvh.segs[2 * leaf + 1].w);
+    *id = k;
+    const float4* q = crv + 5 * (size_t)k;
+    const int flags = caps ? (int)q[4].y : 0;
+    return intersect_curve(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, q[0], q[1], q[2], q[3], flags, t, u);
   }
 };
 

@@ -42,6 +42,7 @@ struct Context {
   // geometry
   HairGeometry hair;
   float* seg_dev = nullptr;  // 8 floats per segment, original order
+  float* crv_dev = nullptr;  // round curves: kCurveFloats per segment
   float* strand_sigma_dev = nullptr;
   DeviceBvh bvh;
   // OBJ head mesh (surface.geometry)
@@ -60,6 +61,14 @@ struct Context {
   V3 cam_pos, cam_d00, cam_du, cam_dv;
 
   SceneView view{};
+  // The view HairMSNN and NRC trace with: their round curves have no end
+  // caps (owlCurvesSetDegree(…, 3, false), render_hair_msnn.cu:346,
+  // render_nrc.cu:409), plain path tracing's have (render_path_tracing.cu:223).
+  SceneView view_no_caps() const {
+    SceneView v = view;
+    v.curve_caps = 0;
+    return v;
+  }
   float4* dlights_dev = nullptr;  // SceneView::dlights
 
   // path-tracer state

@@ -263,13 +263,29 @@ int ensure_scene_points(Context& c, uint64_t seed, std::string& err) {
   std::vector<float> pts(3 * np);
   uint64_t st = seed * 0x2545F4914F6CDD1Dull + 3;
   size_t k = 0;
+  const bool curves = !c.hair.crv.empty();
   for (int i = 0; i < nseg; ++i) {
     const float* q = &c.hair.seg[8 * (size_t)i];
+    CurveSeg cs{};
+    if (curves) {
+      // on the round curve itself (interpolator.position3(eps1),
+      // render_hair_msnn.cu:51-64); its chord length sets the count
+      const float* w = &c.hair.crv[(size_t)kCurveFloats * i];
+      cs = make_curve(make_float4(w[0], w[1], w[2], w[3]), make_float4(w[4], w[5], w[6], w[7]),
+                      make_float4(w[8], w[9], w[10], w[11]), make_float4(w[12], w[13], w[14], w[15]));
+    }
     for (int j = 0; j < count[i]; ++j, ++k) {
       st ^= st << 13;
       st ^= st >> 7;
       st ^= st << 17;
       const float u = (float)((st >> 40) * (1.0 / 16777216.0));
+      if (curves) {
+        const V3 p = curve_pos(cs, u);
+        pts[3 * k] = p.x;
+        pts[3 * k + 1] = p.y;
+        pts[3 * k + 2] = p.z;
+        continue;
+      }
       pts[3 * k] = q[0] + u * (q[4] - q[0]);
       pts[3 * k + 1] = q[1] + u * (q[5] - q[1]);
       pts[3 * k + 2] = q[2] + u * (q[6] - q[2]);
@@ -441,7 +457,7 @@ int msnn_warmup(Context& c, MsnnState& m, int steps, std::string& err) {
     cfg.gb.nn_input = sl.train_input;
     cfg.gb.hit_flags = sl.train_hit;
     cfg.gb.scene_scale = c.scene_scale;
-    HM_TRY(pt_run_pass(c.view, sl.pb, cfg, sl.st, err));
+    HM_TRY(pt_run_pass(c.view_no_caps(), sl.pb, cfg, sl.st, err));
     hipLaunchKernelGGL(k_msnn_finalize, dim3((n + 255) / 256), dim3(256), 0, sl.st, sl.pb, n, 1, 1, n,
                        sl.train_input, sl.train_hit, sl.train_input, sl.train_target, (float4*)nullptr);
     HM_TRY(check_launch("msnn finalize (warm-up)", err));
@@ -502,7 +518,7 @@ int gen_train_batch(Context& c, MsnnState& m, int f0, hipStream_t st, std::strin
   cfg.gb.nn_input = b.nn_input;
   cfg.gb.hit_flags = b.hit;
   cfg.gb.scene_scale = c.scene_scale;
-  HM_TRY(pt_run_pass(c.view, b.pb, cfg, st, err));
+  HM_TRY(pt_run_pass(c.view_no_caps(), b.pb, cfg, st, err));
   hipLaunchKernelGGL(k_msnn_finalize, dim3((n + 255) / 256), dim3(256), 0, st, b.pb, n, 1, 1, n, b.nn_input, b.hit,
                      b.train_input, b.train_target, b.full);
   HM_TRY(check_launch("msnn finalize (training batch)", err));
@@ -556,7 +572,7 @@ int msnn_frame(Context& c, MsnnState& m, EventTimer* tm, std::string& err) {
   cfg.gb.nn_input = sl.nn_input;
   cfg.gb.hit_flags = sl.hit_flags;
   cfg.gb.scene_scale = c.scene_scale;
-  HM_TRY(pt_run_pass(c.view, sl.pb, cfg, sl.st, err));
+  HM_TRY(pt_run_pass(c.view_no_caps(), sl.pb, cfg, sl.st, err));
   hipLaunchKernelGGL(k_msnn_finalize, dim3((npx + 255) / 256), dim3(256), 0, sl.st, sl.pb, npx, 0, m.every_nth, 0,
                      sl.nn_input, sl.hit_flags, (float*)nullptr, (float*)nullptr, sl.short_color);
   HM_TRY(check_launch("msnn finalize", err));

@@ -341,7 +341,7 @@ int nrc_frame(Context& c, NrcState& m, NrcTimer* tm, std::string& err) {
   cfg.train_idx = sl.train_idx;
   cfg.every_nth = m.every_nth;
   cfg.nrc = nrc_args(c, m, sl);
-  HM_TRY(pt_run_pass(c.view, sl.pb, cfg, st, err));
+  HM_TRY(pt_run_pass(c.view_no_caps(), sl.pb, cfg, st, err));
   if (tm) (void)hipEventRecord(tm->e[1], st);
   HM_HIP(hipEventRecord(sl.ev_gb, st), err);
   HM_HIP(hipStreamWaitEvent(cs, sl.ev_gb, 0), err);

@@ -29,6 +29,13 @@ constexpr int kPrimMesh0 = -3;  // triangle k of the head mesh is prim kPrimMesh
 struct SceneView {
   BvhView bvh;
   const float* seg_by_id;     // 8 floats per segment, original order
+  // round Catmull-Rom hair (null for linear segments): 5 float4 per segment,
+  // the four control points (x, y, z, radius), then (strand, end flags);
+  // the BVH leaves hold their boxes. curve_caps: flat caps at strand ends
+  // (on for plain path tracing, off for HairMSNN and NRC, as the reference's
+  // forceCaps flags)
+  const float4* curves;
+  int curve_caps;
   const float* strand_sigma;  // per-strand sigma_a (3 floats) or null
   SphereView sphere;
   MeshView mesh;
@@ -181,12 +188,25 @@ HM_D void setup_hit(const SceneView& sc, V3 o, V3 d, float t, int prim, float u,
     h.alpha = 1.f;
   } else {
     const float* s = sc.seg_by_id + 8 * (size_t)prim;
-    const V3 p0 = v3(s[0], s[1], s[2]), p1 = v3(s[4], s[5], s[6]);
-    const SegHitGeom g = segment_hit_geometry(p0, p1, s[3], u, p);
-    h.p = g.p;
-    h.n = g.n;
-    h.radius = s[3];
-    h.fr = hair_frame(g.tangent, h.wo);
+    V3 tangent;
+    if (sc.curves) {
+      // hairCH on a round Catmull-Rom curve (computeCurveIntersection,
+      // curve_utils.cuh:218-241)
+      const float4* q = sc.curves + 5 * (size_t)prim;
+      const CurveHitGeom g = curve_hit_geometry(make_curve(q[0], q[1], q[2], q[3]), u, p);
+      h.p = g.p;
+      h.n = g.n;
+      h.radius = g.radius;
+      tangent = g.tangent;
+    } else {
+      const V3 p0 = v3(s[0], s[1], s[2]), p1 = v3(s[4], s[5], s[6]);
+      const SegHitGeom g = segment_hit_geometry(p0, p1, s[3], u, p);
+      h.p = g.p;
+      h.n = g.n;
+      h.radius = s[3];
+      tangent = g.tangent;
+    }
+    h.fr = hair_frame(tangent, h.wo);
     h.wo_local = normalize(to_local(h.fr, h.wo));
     h.surface = 0;
     V3 sig = sc.sigma_a;
@@ -194,7 +214,7 @@ HM_D void setup_hit(const SceneView& sc, V3 o, V3 d, float t, int prim, float u,
       const int strand = (int)s[7];
       sig = v3(sc.strand_sigma[3 * strand], sc.strand_sigma[3 * strand + 1], sc.strand_sigma[3 * strand + 2]);
     }
-    const float hh = dot(h.fr.r1, g.n);
+    const float hh = dot(h.fr.r1, h.n);
     h.hw = hair_prepare(sc.hc, h.wo_local, hh, sig);
   }
 }
@@ -499,7 +519,9 @@ struct ShadowSink {
 
 // Closest hit against the head sphere and the hair BVH. The sphere is tested
 // first so its distance prunes the hair traversal; equal distances go to hair.
-template <bool kCount>
+// kCurves: the hair BVH holds round Catmull-Rom segments (sc.curves), a
+// separate instantiation so the linear-segment traversal keeps its registers.
+template <bool kCount, bool kCurves = false>
 HM_D void closest_hit(const SceneView& sc, V3 o, V3 d, TraceStack& st, float& t, int& prim, float& u,
                       TraceCounters& cnt) {
   t = __builtin_inff();
@@ -520,13 +542,16 @@ HM_D void closest_hit(const SceneView& sc, V3 o, V3 d, TraceStack& st, float& t,
     trace_closest<kCount, TriLeaf>(sc.mesh.bvh, r, st, t, tri, bu, cnt, TriLeaf{sc.mesh.tri});
   }
   int id = 0x7fffffff;
-  trace_closest<kCount>(sc.bvh, r, st, t, id, u, cnt);
+  if constexpr (kCurves)
+    trace_closest<kCount, CurveLeaf>(sc.bvh, r, st, t, id, u, cnt, CurveLeaf{sc.curves, sc.curve_caps});
+  else
+    trace_closest<kCount>(sc.bvh, r, st, t, id, u, cnt);
   if (id != 0x7fffffff) prim = id;
   else if (tri != 0x7fffffff) prim = kPrimMesh0 - tri;
   else if (sphere_hit) prim = kPrimSphere;
 }
 
-template <bool kCount>
+template <bool kCount, bool kCurves = false>
 HM_D bool occluded(const SceneView& sc, V3 o, V3 d, TraceStack& st, TraceCounters& cnt) {
   if (sc.sphere.enabled) {
     float ts;
@@ -534,6 +559,7 @@ HM_D bool occluded(const SceneView& sc, V3 o, V3 d, TraceStack& st, TraceCounter
   }
   const RayInv r = make_ray(o.x, o.y, o.z, d.x, d.y, d.z);
   if (sc.mesh.num > 0 && trace_any<kCount, TriLeaf>(sc.mesh.bvh, r, st, cnt, TriLeaf{sc.mesh.tri})) return true;
+  if constexpr (kCurves) return trace_any<kCount, CurveLeaf>(sc.bvh, r, st, cnt, CurveLeaf{sc.curves, sc.curve_caps});
   return trace_any<kCount>(sc.bvh, r, st, cnt);
 }
 

@@ -389,8 +389,8 @@ __global__ void __launch_bounds__(256) k_shade_nrc(SceneView sc, PathBuffers pb,
 }
 
 // kDense: the wavefront is compacted (path i is at index i), no queue.
-template <bool kCount, bool kDense = false>
-__global__ void HM_TRACE_BOUNDS k_extend(SceneView sc, PathBuffers pb, int cur,
+template <bool kCount, bool kDense = false, bool kCurves = false>
+__global__ void HM_TRACE_BOUNDS_T(kCurves) k_extend(SceneView sc, PathBuffers pb, int cur,
                                                         unsigned long long* stats) {
   __shared__ int lds_stack[kStackLds * kTraceBlock];
   TraceStack st = make_stack(lds_stack, pb);
@@ -404,7 +404,7 @@ __global__ void HM_TRACE_BOUNDS k_extend(SceneView sc, PathBuffers pb, int cur,
     float t, u;
     int prim;
     if (kCount) cnt.rays++;
-    closest_hit<kCount>(sc, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), st, t, prim, u, cnt);
+    closest_hit<kCount, kCurves>(sc, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), st, t, prim, u, cnt);
     pb.hit_t[slot] = t;
     pb.hit_prim[slot] = prim;
     pb.hit_u[slot] = u;
@@ -621,8 +621,8 @@ __global__ void k_shadow_resolve_dense(PathBuffers pb, PathBuffers nb) {
 
 // Any-hit test of each pending shadow ray, one ray per lane; writes the
 // visibility byte of (slot, which).
-template <bool kCount>
-__global__ void HM_TRACE_BOUNDS k_shadow(SceneView sc, PathBuffers pb, unsigned long long* stats) {
+template <bool kCount, bool kCurves = false>
+__global__ void HM_TRACE_BOUNDS_T(kCurves) k_shadow(SceneView sc, PathBuffers pb, unsigned long long* stats) {
   __shared__ int lds_stack[kStackLds * kTraceBlock];
   TraceStack st = make_stack(lds_stack, pb);
   TraceCounters cnt{0, 0, 0};
@@ -634,7 +634,7 @@ __global__ void HM_TRACE_BOUNDS k_shadow(SceneView sc, PathBuffers pb, unsigned 
     const float4 o = (e & 1) ? pb.sh_o2[slot] : pb.sh_o1[slot];
     const float4 d = (e & 1) ? pb.sh_d2[slot] : pb.sh_d1[slot];
     if (kCount) cnt.rays++;
-    pb.sh_vis[e] = occluded<kCount>(sc, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), st, cnt) ? 0 : 1;
+    pb.sh_vis[e] = occluded<kCount, kCurves>(sc, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), st, cnt) ? 0 : 1;
   }
   if (kCount) flush_counters(cnt, &stats[2], &stats[3], &stats[5]);
 }
@@ -694,8 +694,8 @@ __global__ void k_accumulate(const float4* L, float4* accum, float4* average, in
 }
 
 // Closest or any hit for an explicit ray list (8 floats per ray).
-template <bool kCount>
-__global__ void HM_TRACE_BOUNDS k_trace_rays(SceneView sc, const float4* rays, int n, int any_hit,
+template <bool kCount, bool kCurves = false>
+__global__ void HM_TRACE_BOUNDS_T(kCurves) k_trace_rays(SceneView sc, const float4* rays, int n, int any_hit,
                                                             int* prim_out, float* t_out, float* u_out, int* spill,
                                                             int spill_stride, int spill_levels,
                                                             unsigned long long* stats) {
@@ -712,12 +712,12 @@ __global__ void HM_TRACE_BOUNDS k_trace_rays(SceneView sc, const float4* rays, i
     const float4 o = rays[2 * (size_t)i], d = rays[2 * (size_t)i + 1];
     if (kCount) cnt.rays++;
     if (any_hit) {
-      const bool occ = occluded<kCount>(sc, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), st, cnt);
+      const bool occ = occluded<kCount, kCurves>(sc, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), st, cnt);
       prim_out[i] = occ ? 1 : 0;
     } else {
       float t, u;
       int prim;
-      closest_hit<kCount>(sc, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), st, t, prim, u, cnt);
+      closest_hit<kCount, kCurves>(sc, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), st, t, prim, u, cnt);
       prim_out[i] = prim;
       if (t_out) t_out[i] = t;
       if (u_out) u_out[i] = u;
@@ -748,6 +748,7 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
   }
   HM_TRY(check_launch("camera", err));
   const bool count = cfg.stats != nullptr;
+  const bool curves = sc.curves != nullptr;
   PassTimer* tm = cfg.timer;
 
   // Grids sized to the pass's paths (queues only shrink with depth), so a
@@ -762,10 +763,15 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
   for (int depth = 0; depth <= std::max(0, cfg.max_vertex); ++depth) {
     const int nxt = cur ^ 1;
     hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, stream, pb.counts, nxt, 2, 3);
-    if (count)
-      launch_timed(tm, 0, k_extend<true>, dim3(ext_blocks), dim3(kTraceBlock), stream, sc, pb, cur, cfg.stats);
+    const dim3 eg(ext_blocks), tb(kTraceBlock);
+    if (curves && count)
+      launch_timed(tm, 0, k_extend<true, false, true>, eg, tb, stream, sc, pb, cur, cfg.stats);
+    else if (curves)
+      launch_timed(tm, 0, k_extend<false, false, true>, eg, tb, stream, sc, pb, cur, cfg.stats);
+    else if (count)
+      launch_timed(tm, 0, k_extend<true>, eg, tb, stream, sc, pb, cur, cfg.stats);
     else
-      launch_timed(tm, 0, k_extend<false>, dim3(ext_blocks), dim3(kTraceBlock), stream, sc, pb, cur, cfg.stats);
+      launch_timed(tm, 0, k_extend<false>, eg, tb, stream, sc, pb, cur, cfg.stats);
     ShadeParams sp;
     sp.depth = depth;
     sp.dl_zero = cfg.dl_zero;
@@ -778,10 +784,15 @@ int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hip
       launch_timed(tm, 1, k_shade_nrc, dim3(shade_blocks), dim3(256), stream, sc, pb, depth, cur, cfg.nrc);
     else
       launch_timed(tm, 1, k_shade, dim3(shade_blocks), dim3(256), stream, sc, pb, sp, cfg.gb);
-    if (count)
-      launch_timed(tm, 2, k_shadow<true>, dim3(sh_blocks), dim3(kTraceBlock), stream, sc, pb, cfg.stats);
+    const dim3 sg(sh_blocks);
+    if (curves && count)
+      launch_timed(tm, 2, k_shadow<true, true>, sg, tb, stream, sc, pb, cfg.stats);
+    else if (curves)
+      launch_timed(tm, 2, k_shadow<false, true>, sg, tb, stream, sc, pb, cfg.stats);
+    else if (count)
+      launch_timed(tm, 2, k_shadow<true>, sg, tb, stream, sc, pb, cfg.stats);
     else
-      launch_timed(tm, 2, k_shadow<false>, dim3(sh_blocks), dim3(kTraceBlock), stream, sc, pb, cfg.stats);
+      launch_timed(tm, 2, k_shadow<false>, sg, tb, stream, sc, pb, cfg.stats);
     launch_timed(tm, 3, k_shadow_resolve, dim3(shade_blocks), dim3(256), stream, pb, cfg.msnn, depth, cfg.nrc);
     HM_TRY(check_launch("wavefront vertex", err));
     cur = nxt;
@@ -1002,8 +1013,12 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
         const PathBuffers& A = views[cur[g]];
         const PathBuffers& B = views[nxt];
         PassTimer* t = timed ? tm : nullptr;
-        launch_timed(t, 0, k_extend<false, true>, dim3(ext_blocks), dim3(kTraceBlock), st, sc, A, cur[g],
-                     (unsigned long long*)nullptr);
+        if (sc.curves)
+          launch_timed(t, 0, k_extend<false, true, true>, dim3(ext_blocks), dim3(kTraceBlock), st, sc, A, cur[g],
+                       (unsigned long long*)nullptr);
+        else
+          launch_timed(t, 0, k_extend<false, true>, dim3(ext_blocks), dim3(kTraceBlock), st, sc, A, cur[g],
+                       (unsigned long long*)nullptr);
         ShadeParams sp;
         sp.depth = 0;
         sp.dl_zero = cfg.dl_zero;
@@ -1013,8 +1028,12 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
         sp.msnn = 0;
         sp.cur = cur[g];
         launch_timed(t, 1, k_shade_dense, dim3(shade_blocks), dim3(256), st, sc, A, B, sp);
-        launch_timed(t, 2, k_shadow<false>, dim3(sh_blocks), dim3(kTraceBlock), st, sc, A,
-                     (unsigned long long*)nullptr);
+        if (sc.curves)
+          launch_timed(t, 2, k_shadow<false, true>, dim3(sh_blocks), dim3(kTraceBlock), st, sc, A,
+                       (unsigned long long*)nullptr);
+        else
+          launch_timed(t, 2, k_shadow<false>, dim3(sh_blocks), dim3(kTraceBlock), st, sc, A,
+                       (unsigned long long*)nullptr);
         launch_timed(t, 3, k_shadow_resolve_dense, dim3(shade_blocks), dim3(256), st, A, B);
         // scheduler state: next call m reads half m % 2 and writes the other
         long long* c = ctl + (size_t)g * kStreamCtl;
@@ -1064,14 +1083,20 @@ int trace_rays(const SceneView& sc, const float* rays, int n, int any_hit, int* 
                int* spill, int spill_stride, int spill_levels, int blocks, unsigned long long* stats,
                hipStream_t stream, std::string& err) {
   if (n <= 0) return kOk;
-  if (stats)
-    hipLaunchKernelGGL(k_trace_rays<true>, dim3(blocks), dim3(kTraceBlock), 0, stream, sc,
-                       reinterpret_cast<const float4*>(rays), n, any_hit, prim, t, u, spill, spill_stride, spill_levels,
-                       stats);
+  const float4* r4 = reinterpret_cast<const float4*>(rays);
+  const dim3 g(blocks), b(kTraceBlock);
+  if (sc.curves && stats)
+    launch_timed(nullptr, 0, k_trace_rays<true, true>, g, b, stream, sc, r4, n, any_hit, prim, t, u, spill,
+                 spill_stride, spill_levels, stats);
+  else if (sc.curves)
+    launch_timed(nullptr, 0, k_trace_rays<false, true>, g, b, stream, sc, r4, n, any_hit, prim, t, u, spill,
+                 spill_stride, spill_levels, stats);
+  else if (stats)
+    launch_timed(nullptr, 0, k_trace_rays<true>, g, b, stream, sc, r4, n, any_hit, prim, t, u, spill, spill_stride,
+                 spill_levels, stats);
   else
-    hipLaunchKernelGGL(k_trace_rays<false>, dim3(blocks), dim3(kTraceBlock), 0, stream, sc,
-                       reinterpret_cast<const float4*>(rays), n, any_hit, prim, t, u, spill, spill_stride, spill_levels,
-                       stats);
+    launch_timed(nullptr, 0, k_trace_rays<false>, g, b, stream, sc, r4, n, any_hit, prim, t, u, spill, spill_stride,
+                 spill_levels, stats);
   return check_launch("trace rays", err);
 }
 

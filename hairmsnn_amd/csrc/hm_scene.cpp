@@ -107,6 +107,14 @@ bool parse_json(const Json& cfg, Scene& scene, std::string& err) {
       is_hair = true;
     }
     if (h.has("tessellation")) scene.hair_tessellation = std::max(1, std::min(64, h.at("tessellation").i()));
+    if (h.has("primitive") && h.at("primitive").is_string()) {
+      const std::string prim = h.at("primitive").str();
+      if (prim != "curve" && prim != "linear") {
+        err = "hair.primitive must be \"curve\" or \"linear\"";
+        return false;
+      }
+      scene.hair_curves = prim == "curve";
+    }
   }
   scene.has_hair = is_hair;
   if (!is_surface && !is_hair) {
@@ -407,7 +415,7 @@ void generate_hair(const ProceduralHair& g, HairGeometry& out) {
   }
 }
 
-int load_hair_file(const std::string& path, int tess, HairGeometry& out, std::string& err) {
+int load_hair_file(const std::string& path, int tess, bool curves, HairGeometry& out, std::string& err) {
   FILE* f = std::fopen(path.c_str(), "rb");
   if (!f) {
     err = "Error: Cannot open hair file! (" + path + ")";
@@ -537,6 +545,19 @@ int load_hair_file(const std::string& path, int tess, HairGeometry& out, std::st
                        (2.f * cw[k] - 5.f * cw[k + 1] + 4.f * cw[k + 2] - cw[k + 3]) * t2 +
                        (3.f * cw[k + 1] - cw[k] - 3.f * cw[k + 2] + cw[k + 3]) * t3);
       };
+      if (curves) {
+        // the segment itself (render_path_tracing.cu:236-244 hands the
+        // control points and widths to the curve builder), and its chord
+        const int flags = (k == 0 ? 1 : 0) | (k == nseg - 1 ? 2 : 0);
+        const float rec[kCurveFloats] = {p0.x, p0.y, p0.z, cw[k],     p1.x,          p1.y,          p1.z,
+                                         cw[k + 1], p2.x, p2.y, p2.z, cw[k + 2],     p3.x,          p3.y,
+                                         p3.z, cw[k + 3], (float)strand, (float)flags, 0.f,           0.f};
+        out.crv.insert(out.crv.end(), rec, rec + kCurveFloats);
+        const float chord[8] = {p1.x, p1.y, p1.z, std::max(0.f, width(0.5f)), p2.x, p2.y, p2.z, (float)strand};
+        out.seg.insert(out.seg.end(), chord, chord + 8);
+        out.strand_of_seg.push_back(strand);
+        continue;
+      }
       V3 a = p1;
       for (int j = 1; j <= tess; ++j) {
         const V3 e = j == tess ? p2 : curve((float)j / (float)tess);

@@ -60,7 +60,11 @@ struct Scene {
   bool hair_procedural = false;
   ProceduralHair hair_gen;
   std::string hair_geometry;
-  int hair_tessellation = 4;  // linear pieces per Catmull-Rom segment of a .hair file
+  int hair_tessellation = 4;  // linear pieces per Catmull-Rom segment of a .hair file ("linear")
+  // .hair strands as round cubic Catmull-Rom curves (the reference's
+  // primitive; hair.primitive "curve", the default) or as hair_tessellation
+  // linear pieces per curve segment ("linear")
+  bool hair_curves = true;
 
   // camera (required)
   SceneCamera camera;
@@ -125,7 +129,14 @@ bool parse_scene_text(const std::string& text, const std::string& base_dir, Scen
 
 // Host-side generated content.
 struct HairGeometry {
-  std::vector<float> seg;  // 8 floats per segment: p0 xyz, r, p1 xyz, strand
+  // 8 floats per primitive: p0 xyz, r, p1 xyz, strand. Linear pieces, or for
+  // curves each segment's chord (its two interpolated points, the radius at
+  // u = 0.5), which per-strand lookups and point sampling use.
+  std::vector<float> seg;
+  // curves only: kCurveFloats per segment: the four Catmull-Rom control
+  // points (x, y, z, radius), then strand, end flags (1: first segment of its
+  // strand, 2: last), 0, 0
+  std::vector<float> crv;
   std::vector<int32_t> strand_of_seg;
   int num_strands = 0;
   V3 bmin{1e30f, 1e30f, 1e30f}, bmax{-1e30f, -1e30f, -1e30f};
@@ -136,7 +147,9 @@ void generate_hair(const ProceduralHair& g, HairGeometry& out);
 // end, form uniform Catmull-Rom segments of radius 0.2 x thickness; every
 // segment is split into `tess` linear pieces (sampled on the curve, radius
 // at the piece midpoint).
-int load_hair_file(const std::string& path, int tess, HairGeometry& out, std::string& err);
+// With curves, every segment is kept as one round Catmull-Rom curve instead.
+constexpr int kCurveFloats = 20;
+int load_hair_file(const std::string& path, int tess, bool curves, HairGeometry& out, std::string& err);
 
 // Triangle mesh of an OBJ head (hm_obj.cpp): per triangle its three
 // vertices, three vertex normals and the material's diffuse colour.

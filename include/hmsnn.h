@@ -115,6 +115,7 @@ typedef struct hmsnn_info {
   float dir_from[8][3];
   float dir_emit[8][3];
   int num_triangles;      /* OBJ head mesh (surface.geometry), 0 without one */
+  int num_curves;         /* round Catmull-Rom hair segments (.hair, primitive "curve"), 0 otherwise */
 } hmsnn_info;
 int hmsnn_get_info(const hmsnn_ctx* ctx, hmsnn_info* info);
 /* Strings of the scene: which = 0 integrator.image_output, 1
@@ -140,6 +141,16 @@ int hmsnn_generate_host(const char* json_text, float* seg_host, size_t seg_cap, 
  * (env_width + 1) x env_height, marginal pdf/cdf of env_height + 1). Any
  * pointer may be NULL to skip that array. */
 int hmsnn_export_segments(const hmsnn_ctx* ctx, float* seg_host);
+/* Round Catmull-Rom hair curves (.hair strands with hair.primitive "curve",
+ * the default; the reference's owlCurvesSetControlPoints /
+ * owlCurvesSetSegmentIndices input, render_path_tracing.cu:236-244), 20
+ * floats per segment: four control points (x, y, z, radius), strand id, end
+ * flags (1: first segment of its strand, 2: last), 0, 0. With curves,
+ * hmsnn_export_segments gives each segment's chord. */
+int hmsnn_export_curves(const hmsnn_ctx* ctx, float* crv_host);
+/* The same curves from the scene file on the host only (no GPU); capacity
+ * in floats, *num_curves reports the count (NULL buffer to query). */
+int hmsnn_generate_curves_host(const char* json_text, float* crv_host, size_t crv_cap, int* num_curves);
 /* The OBJ head mesh as loaded (loadOBJ, model.cpp:233-358): per triangle 3
  * vertices (9 floats), 3 vertex normals (9 floats) and the diffuse colour (3
  * floats); NULL skips an array. */

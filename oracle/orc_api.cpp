@@ -99,6 +99,7 @@ struct orc_scene {
   const float* tri_n;
   const float* tri_kd;
   int ntri;
+  const float* crv;
 };
 
 namespace {
@@ -106,6 +107,7 @@ SceneDesc scene_desc(const orc_scene* sc, EnvTables& env) {
   SceneDesc s{};
   s.seg = sc->seg;
   s.nseg = sc->nseg;
+  s.crv = sc->crv;
   s.has_sphere = sc->has_sphere;
   std::memcpy(s.sphere, sc->sphere, sizeof(s.sphere));
   std::memcpy(s.sphere_albedo, sc->sphere_albedo, sizeof(s.sphere_albedo));
@@ -257,6 +259,82 @@ void* orc_bvh_create(const float* seg, int n) {
 }
 
 void orc_bvh_destroy(void* b) { delete (Bvh*)b; }
+
+// A BVH over round Catmull-Rom segments (kCrvFloats each).
+void* orc_bvh_create_curves(const float* crv, int n) {
+  Bvh* b = new Bvh();
+  b->build_curves(crv, n);
+  return b;
+}
+
+int orc_curve_floats() { return kCrvFloats; }
+
+void orc_curve_eval(const float* q, int n, const double* u, int prec, double* out) {
+  for (int i = 0; i < n; ++i) crv_eval(q, u[i], prec, out + 8 * (size_t)i);
+}
+
+// Closest (or any) hit of each ray over round curves: brute force
+// (use_bvh = 0, minimum (t, id)) or the CPU BVH; caps: end caps tested.
+// Rays: 8 floats (o, pad, d, pad).
+void orc_curve_intersect(int ncrv, const float* crv, int nray, const float* rays, int any_hit, int use_bvh, int caps,
+                         int* prim, float* t, float* u) {
+  Bvh bvh;
+  if (use_bvh) bvh.build_curves(crv, ncrv);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < nray; ++i) {
+    const float* r = rays + 8 * (size_t)i;
+    const float o[3] = {r[0], r[1], r[2]}, d[3] = {r[4], r[5], r[6]};
+    float bt = 0.f, bu = 0.f;
+    int bid = -1;
+    if (use_bvh) {
+      if (any_hit) {
+        prim[i] = bvh.any(o, d, caps) ? 1 : 0;
+        continue;
+      }
+      if (!bvh.closest(o, d, &bt, &bid, &bu, caps)) bid = -1;
+    } else {
+      for (int k = 0; k < ncrv; ++k) {
+        const float* q = crv + (size_t)kCrvFloats * k;
+        float tt, uu;
+        if (crv_intersect(o, d, q, caps ? (int)q[17] : 0, &tt, &uu) && (bid < 0 || tt < bt)) {
+          bt = tt;
+          bu = uu;
+          bid = k;
+        }
+      }
+      if (any_hit) {
+        prim[i] = bid >= 0 ? 1 : 0;
+        continue;
+      }
+    }
+    prim[i] = bid;
+    if (t) t[i] = bid >= 0 ? bt : 0.f;
+    if (u) u[i] = bid >= 0 ? bu : 0.f;
+  }
+}
+
+// One ray against one curve segment each: the FP32 iteration (t32, u32,
+// hit32) and the FP64 search (t64, u64, hit64, min_clear). flags: caps.
+void orc_curve_pairs(int n, const float* crv, const double* rays, int flags, float* t32, float* u32, uint8_t* hit32,
+                     double* t64, double* u64, uint8_t* hit64, double* min_clear) {
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < n; ++i) {
+    const float* q = crv + (size_t)kCrvFloats * i;
+    const double* r = rays + 6 * (size_t)i;
+    const float of[3] = {(float)r[0], (float)r[1], (float)r[2]}, df[3] = {(float)r[3], (float)r[4], (float)r[5]};
+    float tf = 0.f, uf = 0.f;
+    hit32[i] = crv_intersect(of, df, q, flags, &tf, &uf) ? 1 : 0;
+    t32[i] = tf;
+    u32[i] = uf;
+    // the FP64 search on the FP32-rounded ray, so both see the same ray
+    const double od[3] = {of[0], of[1], of[2]}, dd[3] = {df[0], df[1], df[2]};
+    double td = 0, ud = 0, mc = 0;
+    hit64[i] = crv_exact(od, dd, q, flags, &td, &ud, &mc) ? 1 : 0;
+    t64[i] = td;
+    u64[i] = ud;
+    min_clear[i] = mc;
+  }
+}
 
 // Renders a crop of the image. mode 0: pathTrace, 1: msnnPathTrace with
 // kernel beta kbeta, 2: msnnTrainingPath. prec 0 FP64, 1 FP32. Outputs the

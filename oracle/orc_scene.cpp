@@ -218,6 +218,30 @@ void Bvh::build(const float* s, int n) {
   }
 }
 
+void Bvh::build_curves(const float* c, int n) {
+  boxes.assign((size_t)8 * n, 0.f);
+  for (int k = 0; k < n; ++k) {
+    float lo[3], hi[3];
+    crv_box(c + (size_t)kCrvFloats * k, lo, hi);
+    float* b = &boxes[8 * (size_t)k];
+    for (int a = 0; a < 3; ++a) {
+      b[a] = lo[a];
+      b[4 + a] = hi[a];
+    }
+    b[7] = (float)k;
+  }
+  build(boxes.data(), n);
+  crv = c;
+}
+
+bool Bvh::leaf(int id, const float o[3], const float d[3], int caps, float* t, float* u) const {
+  if (crv) {
+    const float* q = crv + (size_t)kCrvFloats * id;
+    return crv_intersect(o, d, q, caps ? (int)q[17] : 0, t, u);
+  }
+  return seg_intersect(o, d, seg + 8 * (size_t)id, t, u);
+}
+
 static inline bool box_hit(const Bvh::Node& n, const double o[3], const double inv[3], double tmax, double* tn) {
   double t0 = 0.0, t1 = tmax;
   for (int k = 0; k < 3; ++k) {
@@ -230,7 +254,7 @@ static inline bool box_hit(const Bvh::Node& n, const double o[3], const double i
   return t0 <= t1 * (1.0 + 1e-9) + 1e-12;
 }
 
-bool Bvh::closest(const float o[3], const float d[3], float* t, int* id, float* u) const {
+bool Bvh::closest(const float o[3], const float d[3], float* t, int* id, float* u, int caps) const {
   if (nseg == 0) return false;
   double od[3] = {o[0], o[1], o[2]}, inv[3];
   for (int k = 0; k < 3; ++k) inv[k] = 1.0 / (d[k] != 0.f ? (double)d[k] : 1e-300);
@@ -247,7 +271,7 @@ bool Bvh::closest(const float o[3], const float d[3], float* t, int* id, float* 
       for (int j = n.first; j < n.first + n.count; ++j) {
         const int sid = idx[j];
         float tt, uu;
-        if (seg_intersect(o, d, seg + 8 * (size_t)sid, &tt, &uu)) {
+        if (leaf(sid, o, d, caps, &tt, &uu)) {
           if (tt < best_t || (tt == best_t && sid < best_id)) {
             best_t = tt;
             best_id = sid;
@@ -267,7 +291,7 @@ bool Bvh::closest(const float o[3], const float d[3], float* t, int* id, float* 
   return true;
 }
 
-bool Bvh::any(const float o[3], const float d[3]) const {
+bool Bvh::any(const float o[3], const float d[3], int caps) const {
   if (nseg == 0) return false;
   double od[3] = {o[0], o[1], o[2]}, inv[3];
   for (int k = 0; k < 3; ++k) inv[k] = 1.0 / (d[k] != 0.f ? (double)d[k] : 1e-300);
@@ -281,7 +305,7 @@ bool Bvh::any(const float o[3], const float d[3]) const {
     if (n.count > 0) {
       for (int j = n.first; j < n.first + n.count; ++j) {
         float tt, uu;
-        if (seg_intersect(o, d, seg + 8 * (size_t)idx[j], &tt, &uu)) return true;
+        if (leaf(idx[j], o, d, caps, &tt, &uu)) return true;
       }
     } else {
       stack[sp++] = n.left;
@@ -315,6 +339,7 @@ struct Ctx {
   bool frozen = false;  // sampling uses hp0 / hs0 / color0
   HairParams<R> hp0;
   HairShading<R> hs0;
+  int caps = 1;  // round curves' end caps: plain path tracing only (render_path_tracing.cu:223)
 };
 
 template <class R> inline Vec<R> apply_mat(const Vec<R> m[3], Vec<R> v) { return mk(dot(m[0], v), dot(m[1], v), dot(m[2], v)); }
@@ -479,27 +504,67 @@ void setup_hit(const Ctx<R>& c, Vec<R> o, Vec<R> d, float t, int prim, float u, 
     si.radius = 0;
     return;
   }
-  // hairCH + computeCurveIntersection, linear branch (optix_common.cuh:558-599,
-  // curve_utils.cuh:242-255, surfaceNormal :159-197)
+  // hairCH + computeCurveIntersection (optix_common.cuh:558-599,
+  // curve_utils.cuh:218-255, surfaceNormal :159-197): linear branch, or the
+  // cubic Catmull-Rom branch for round curves
   const float* q = s.seg + 8 * (size_t)prim;
-  const Vec<R> p0 = mk((R)q[0], (R)q[1], (R)q[2]), p1 = mk((R)q[4], (R)q[5], (R)q[6]);
-  const Vec<R> dv = p1 - p0;
-  const R r = (R)q[3];
-  Vec<R> n;
-  if (u == 0.f) {
-    n = nrm((R)-1 * dv);
-  } else if (u == 1.f) {
-    n = nrm(dv);
+  Vec<R> n, X;
+  R r;
+  if (s.crv) {
+    const float* w = s.crv + (size_t)kCrvFloats * prim;
+    R A[4][4];  // power basis per coordinate (x, y, z, radius): a, b, c, d
+    for (int k = 0; k < 4; ++k) {
+      const R q0 = (R)w[k], q1 = (R)w[4 + k], q2 = (R)w[8 + k], q3 = (R)w[12 + k];
+      A[k][0] = (-q0 + (R)3 * q1 - (R)3 * q2 + q3) / (R)2;
+      A[k][1] = ((R)2 * q0 - (R)5 * q1 + (R)4 * q2 - q3) / (R)2;
+      A[k][2] = (q2 - q0) / (R)2;
+      A[k][3] = q1;
+    }
+    const R uu = (R)u;
+    auto P = [&](int k, R x) { return ((A[k][0] * x + A[k][1]) * x + A[k][2]) * x + A[k][3]; };
+    auto V = [&](int k, R x) {  // velocity4 with the triple-knot guard (curve_utils.cuh:95-101)
+      if (x == (R)0) x = (R)0.000001f;
+      if (x == (R)1) x = (R)0.999999f;
+      return ((R)3 * A[k][0] * x + (R)2 * A[k][1]) * x + A[k][2];
+    };
+    auto Acc = [&](int k, R x) { return (R)6 * A[k][0] * x + (R)2 * A[k][1]; };
+    const Vec<R> cp = mk(P(0, uu), P(1, uu), P(2, uu));
+    const Vec<R> dv = mk(V(0, uu), V(1, uu), V(2, uu));
+    r = P(3, uu);
+    if (u == 0.f) {
+      n = nrm((R)-1 * dv);
+    } else if (u == 1.f) {
+      n = nrm(dv);
+    } else {
+      const R dr = V(3, uu);
+      R dd = dot(dv, dv);
+      Vec<R> o1 = p - cp;
+      o1 = o1 - (dot(o1, dv) / dd) * dv;
+      o1 = (r / len(o1)) * o1;
+      p = cp + o1;
+      dd = dd - dot(mk(Acc(0, uu), Acc(1, uu), Acc(2, uu)), o1);
+      n = nrm(dd * o1 - (dr * r) * dv);
+    }
+    X = nrm(dv);
   } else {
-    const Vec<R> pa = p0 + (R)u * dv;
-    const R dd = dot(dv, dv);
-    Vec<R> o1 = p - pa;
-    o1 = o1 - (dot(o1, dv) / dd) * dv;
-    o1 = (r / len(o1)) * o1;
-    p = pa + o1;
-    n = nrm(dd * o1);
+    const Vec<R> p0 = mk((R)q[0], (R)q[1], (R)q[2]), p1 = mk((R)q[4], (R)q[5], (R)q[6]);
+    const Vec<R> dv = p1 - p0;
+    r = (R)q[3];
+    if (u == 0.f) {
+      n = nrm((R)-1 * dv);
+    } else if (u == 1.f) {
+      n = nrm(dv);
+    } else {
+      const Vec<R> pa = p0 + (R)u * dv;
+      const R dd = dot(dv, dv);
+      Vec<R> o1 = p - pa;
+      o1 = o1 - (dot(o1, dv) / dd) * dv;
+      o1 = (r / len(o1)) * o1;
+      p = pa + o1;
+      n = nrm(dd * o1);
+    }
+    X = nrm(dv);
   }
-  const Vec<R> X = nrm(dv);
   const Vec<R> Y = nrm(cross(si.wo, X));
   const Vec<R> Z = nrm(cross(X, Y));
   si.p = p;
@@ -580,7 +645,7 @@ bool occluded(const Ctx<R>& c, Vec<R> o, Vec<R> d) {
     float t, u, v;
     if (tri_intersect(of, df, c.s.tri_v + 9 * (size_t)k, &t, &u, &v)) return true;
   }
-  return c.bvh.any(of, df);
+  return c.bvh.any(of, df, c.caps);
 }
 
 template <class R>
@@ -606,7 +671,7 @@ void trace(const Ctx<R>& c, Vec<R> o, Vec<R> d, Inter<R>& si) {
   }
   float th, uh;
   int id;
-  if (c.bvh.closest(of, df, &th, &id, &uh) && th <= best) {
+  if (c.bvh.closest(of, df, &th, &id, &uh, c.caps) && th <= best) {
     best = th;
     prim = id;
     u = uh;
@@ -921,6 +986,7 @@ template <class R>
 void nrc_gbuffer(const SceneDesc& s, const Bvh& bvh, int accum_id, const uint8_t* flags, float cthr, float scale,
                  int threads, NrcPixelOut* out) {
   Ctx<R> c{s, bvh, {}, {}};
+  c.caps = 0;  // NRC: no end caps (render_nrc.cu:409)
   c.hp = HairParams<R>{(R)s.beta_m, (R)s.beta_n, (R)s.alpha, (R)s.gains[0], (R)s.gains[1], (R)s.gains[2], (R)s.gains[3]};
   c.hs = setup_hair(c.hp);
   c.hp0 = c.hp;
@@ -964,6 +1030,7 @@ template <class R>
 void render_pixels(const SceneDesc& s, const Bvh& bvh, int mode, int kbeta, int accum_id0, int spp, int x0, int y0,
                    int w, int h, int threads, double* out_full, double* out_short, uint8_t* out_hit) {
   Ctx<R> c{s, bvh, {}, {}};
+  c.caps = mode == 0 ? 1 : 0;  // HairMSNN modes have no end caps (render_hair_msnn.cu:346)
   c.hp = HairParams<R>{(R)s.beta_m, (R)s.beta_n, (R)s.alpha, (R)s.gains[0], (R)s.gains[1], (R)s.gains[2], (R)s.gains[3]};
   c.hs = setup_hair(c.hp);
   c.frozen = s.frozen != 0 && mode == 0;
@@ -1028,6 +1095,7 @@ template <class R>
 void msnn_records(const SceneDesc& s, const Bvh& bvh, int accum_id, int kbeta, int n, const int* pixels,
                   float scene_scale, int threads, MsnnRecordOut* out) {
   Ctx<R> c{s, bvh, {}, {}};
+  c.caps = 0;  // HairMSNN: no end caps (render_hair_msnn.cu:346)
   c.hp = HairParams<R>{(R)s.beta_m, (R)s.beta_n, (R)s.alpha, (R)s.gains[0], (R)s.gains[1], (R)s.gains[2], (R)s.gains[3]};
   c.hs = setup_hair(c.hp);
   c.hp0 = c.hp;
@@ -1282,6 +1350,7 @@ template void render_pixels<double>(const SceneDesc&, const Bvh&, int, int, int,
 template <class R>
 static Ctx<R> probe_ctx(const SceneDesc& s, const Bvh& bvh) {
   Ctx<R> c{s, bvh, {}, {}};
+  c.caps = 1;
   c.hp = HairParams<R>{(R)s.beta_m, (R)s.beta_n, (R)s.alpha, (R)s.gains[0], (R)s.gains[1], (R)s.gains[2], (R)s.gains[3]};
   c.hs = setup_hair(c.hp);
   c.hp0 = c.hp;

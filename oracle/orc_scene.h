@@ -40,6 +40,23 @@ bool sphere_intersect(const float o[3], const float d[3], const float c[3], floa
 // barycentrics (u, v) of vertices 1 and 2; IEEE FP32 like seg_intersect.
 bool tri_intersect(const float o[3], const float d[3], const float* v, float* t, float* u, float* w);
 
+// Round Catmull-Rom hair segments (orc_curve.cpp), kCrvFloats per segment:
+// four control points (x, y, z, radius), strand, end flags (1: first
+// segment of its strand, 2: last), 0, 0. crv_intersect: the GPU build's
+// iteration in the same FP32 operations (nearest t > 0; u = 0 / 1 on a cap;
+// flags: caps to test). crv_exact: FP64 first crossing of the swept-circle
+// surface along the ray (min_clear: the smallest |distance - radius| seen,
+// which tells grazing rays apart). crv_box: the box the BVH bounds it by.
+constexpr int kCrvFloats = 20;
+float crv_radius_bound(const float* q);
+void crv_box(const float* q, float lo[3], float hi[3]);
+// Position (x, y, z, radius) and its derivative at u: prec 1 the FP32
+// evaluator the iteration uses, 0 the FP64 one of the brute-force search.
+void crv_eval(const float* q, double u, int prec, double out[8]);
+bool crv_intersect(const float o[3], const float d[3], const float* q, int flags, float* t, float* u);
+bool crv_exact(const double o[3], const double d[3], const float* q, int flags, double* t, double* u,
+               double* min_clear);
+
 struct Bvh {
   struct Node {
     float lo[3], hi[3];
@@ -48,18 +65,23 @@ struct Bvh {
   };
   std::vector<Node> nodes;
   std::vector<int> idx;
-  const float* seg = nullptr;  // 8 floats per segment
+  const float* seg = nullptr;  // 8 floats per segment (curves: their boxes)
   int nseg = 0;
+  const float* crv = nullptr;  // round curves, or null for segments
+  std::vector<float> boxes;
   void build(const float* seg, int n);
-  // minimum (t, id) over all segments, like a brute-force scan
-  bool closest(const float o[3], const float d[3], float* t, int* id, float* u) const;
-  bool any(const float o[3], const float d[3]) const;
+  void build_curves(const float* crv, int n);
+  // minimum (t, id) over all primitives, like a brute-force scan; caps: the
+  // curves' end caps are tested (plain path tracing) or not (HairMSNN, NRC)
+  bool closest(const float o[3], const float d[3], float* t, int* id, float* u, int caps = 1) const;
+  bool any(const float o[3], const float d[3], int caps = 1) const;
+  bool leaf(int id, const float o[3], const float d[3], int caps, float* t, float* u) const;
 };
 
 // ---- scene and integrators --------------------------------------------------
 struct SceneDesc {
   // geometry
-  const float* seg;  // 8 floats per segment
+  const float* seg;  // 8 floats per segment (with curves: each one's chord)
   int nseg;
   int has_sphere;
   float sphere[4];
@@ -94,6 +116,8 @@ struct SceneDesc {
   const float* tri_n;
   const float* tri_kd;
   int ntri;
+  // round Catmull-Rom hair (kCrvFloats per segment, same ids as seg), or null
+  const float* crv;
 };
 
 // mode 0: pathTrace; mode 1: msnnPathTrace truncated at kernel beta;

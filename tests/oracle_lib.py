@@ -33,7 +33,7 @@ class OrcScene(C.Structure):
         ("width", _I), ("height", _I), ("beta_m", _F), ("beta_n", _F), ("alpha", _F), ("gains", _F * 4),
         ("sigma_a", _F * 3), ("strand_sigma", _P), ("mis", _I), ("path_v1", _I), ("path_v2", _I),
         ("frozen", _I), ("beta_m0", _F), ("beta_n0", _F), ("alpha0", _F), ("strand_sigma0", _P),
-        ("tri_v", _P), ("tri_n", _P), ("tri_kd", _P), ("ntri", _I),
+        ("tri_v", _P), ("tri_n", _P), ("tri_kd", _P), ("ntri", _I), ("crv", _P),
     ]
 
 
@@ -49,6 +49,11 @@ _SIGS = {
     "orc_intersect": (None, [_I, _P, _I, _P, _I, _I, _P, _P, _P]),
     "orc_bvh_create": (_P, [_P, _I]),
     "orc_bvh_destroy": (None, [_P]),
+    "orc_bvh_create_curves": (_P, [_P, _I]),
+    "orc_curve_floats": (_I, []),
+    "orc_curve_eval": (None, [_P, _I, _P, _I, _P]),
+    "orc_curve_intersect": (None, [_I, _P, _I, _P, _I, _I, _I, _P, _P, _P]),
+    "orc_curve_pairs": (None, [_I, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P]),
     "orc_render": (_I, [C.POINTER(OrcScene), _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P]),
     "orc_max_threads": (_I, []),
     "orc_adjoint": (_I, [C.POINTER(OrcScene), _P, _I, _P, _I, _I, _D, _P, _P, _P]),
@@ -152,9 +157,10 @@ class OracleScene:
     """Scene description for the oracle integrators, filled from a context's exports."""
 
     def __init__(self, info: dict, seg: np.ndarray, env: dict | None, strand_sigma: np.ndarray | None = None,
-                 mesh: dict | None = None):
+                 mesh: dict | None = None, curves: np.ndarray | None = None):
         self.lib = load_oracle()
         self.seg = np.ascontiguousarray(seg, np.float32)
+        self.crv = None if curves is None else np.ascontiguousarray(curves, np.float32)
         self.env = {k: np.ascontiguousarray(v, np.float32) for k, v in env.items()} if env else None
         self.strand_sigma = None if strand_sigma is None else np.ascontiguousarray(strand_sigma, np.float32)
         s = OrcScene()
@@ -192,8 +198,13 @@ class OracleScene:
         if self.mesh:
             s.tri_v, s.tri_n, s.tri_kd = ptr(self.mesh["v"]), ptr(self.mesh["n"]), ptr(self.mesh["kd"])
             s.ntri = self.mesh["v"].shape[0]
+        if self.crv is not None:
+            s.crv = ptr(self.crv)
         self.s = s
-        self.bvh = self.lib.orc_bvh_create(ptr(self.seg), self.seg.shape[0])
+        if self.crv is not None:
+            self.bvh = self.lib.orc_bvh_create_curves(ptr(self.crv), self.crv.shape[0])
+        else:
+            self.bvh = self.lib.orc_bvh_create(ptr(self.seg), self.seg.shape[0])
 
     def __del__(self):
         try:
@@ -498,3 +509,48 @@ def ggx_sample(a: float, wo: np.ndarray, u2: np.ndarray, albedo=(0.0, 0.0, 0.0))
     wi, pdf, f = np.empty((n, 3)), np.empty(n), np.empty((n, 3))
     lib.orc_ggx_sample(n, a, ptr(wo), ptr(u2), ptr(alb), ptr(wi), ptr(pdf), ptr(f))
     return wi, pdf, f
+
+
+# ---- round Catmull-Rom curves ------------------------------------------------
+def curve_intersect(crv: np.ndarray, rays: np.ndarray, any_hit: bool = False, use_bvh: bool = True,
+                    caps: bool = True):
+    """Closest (or any) hit of each ray (n x 8: o, pad, d, pad) over round
+    curves (m x 20) with the oracle's FP32 iteration: prim (-1: miss), t, u."""
+    lib = load_oracle()
+    crv = np.ascontiguousarray(crv, np.float32)
+    rays = np.ascontiguousarray(rays, np.float32)
+    n = rays.shape[0]
+    prim = np.empty(n, np.int32)
+    t = np.zeros(n, np.float32)
+    u = np.zeros(n, np.float32)
+    lib.orc_curve_intersect(crv.shape[0], ptr(crv), n, ptr(rays), int(any_hit), int(use_bvh), int(caps), ptr(prim),
+                            ptr(t), ptr(u))
+    return prim, t, u
+
+
+def curve_pairs(crv: np.ndarray, rays: np.ndarray, flags: int = 0) -> dict:
+    """Ray i (n x 6: o, d) against curve i (n x 20): the FP32 iteration and
+    the FP64 surface search, with the FP64 search's smallest |clearance|."""
+    lib = load_oracle()
+    crv = np.ascontiguousarray(crv, np.float32)
+    rays = np.ascontiguousarray(rays, np.float64)
+    n = rays.shape[0]
+    out = {"t32": np.zeros(n, np.float32), "u32": np.zeros(n, np.float32), "hit32": np.zeros(n, np.uint8),
+           "t64": np.zeros(n), "u64": np.zeros(n), "hit64": np.zeros(n, np.uint8), "clear": np.zeros(n)}
+    lib.orc_curve_pairs(n, ptr(crv), ptr(rays), int(flags), ptr(out["t32"]), ptr(out["u32"]), ptr(out["hit32"]),
+                        ptr(out["t64"]), ptr(out["u64"]), ptr(out["hit64"]), ptr(out["clear"]))
+    out["hit32"] = out["hit32"].astype(bool)
+    out["hit64"] = out["hit64"].astype(bool)
+    return out
+
+
+def curve_eval(q: np.ndarray, u, prec: int = 0) -> np.ndarray:
+    """Position (x, y, z, r) and derivative (x', y', z', r') of one segment
+    (20 floats) at parameters u: prec 1 the FP32 evaluator of the iteration,
+    0 the FP64 one of the search."""
+    lib = load_oracle()
+    q = np.ascontiguousarray(q, np.float32)
+    u = np.ascontiguousarray(np.atleast_1d(u), np.float64)
+    out = np.zeros((u.size, 8))
+    lib.orc_curve_eval(ptr(q), u.size, ptr(u), int(prec), ptr(out))
+    return out

@@ -199,8 +199,8 @@ def test_hair_file_scene_matches_oracle(gpu, tmp_path):
     p = tmp_path / "h.hair"
     write_hair(p, strands, d_thickness=0.03)
     cfg = scene_cfg("c0_hairball.json", 64, 64, path_v2=1)
-    cfg["hair"] = {"geometry": str(p), "tessellation": 3, "sigma_a": [0.06, 0.1, 0.2], "beta_m": 0.3,
-                   "beta_n": 0.3, "alpha": 2.0}
+    cfg["hair"] = {"geometry": str(p), "primitive": "linear", "tessellation": 3, "sigma_a": [0.06, 0.1, 0.2],
+                   "beta_m": 0.3, "beta_n": 0.3, "alpha": 2.0}
     with gpu.Scene(json_text=cfg) as sc:
         assert int(sc.info["num_strands"]) == 400
         sc.pt_reset()

@@ -1,6 +1,8 @@
 """Cem Yuksel .hair input (scene.cpp:10-75): strands become uniform
 Catmull-Rom curves through their points, with a phantom point mirrored past
-each end and radius 0.2 x thickness, split here into linear pieces."""
+each end and radius 0.2 x thickness; as round curves (hair.primitive
+"curve", the default) the segments keep their control points, as "linear"
+they are split into linear pieces."""
 import json
 from pathlib import Path
 
@@ -21,8 +23,8 @@ def helix(n, r=0.1, seed=0):
 
 def scene_with(path, tess):
     cfg = json.loads((SCENES / "c0_hairball.json").read_text())
-    cfg["hair"] = {"geometry": str(path), "tessellation": tess, "sigma_a": [0.06, 0.1, 0.2], "beta_m": 0.3,
-                   "beta_n": 0.3, "alpha": 2.0}
+    cfg["hair"] = {"geometry": str(path), "primitive": "linear", "tessellation": tess, "sigma_a": [0.06, 0.1, 0.2],
+                   "beta_m": 0.3, "beta_n": 0.3, "alpha": 2.0}
     return cfg
 
 
@@ -73,6 +75,46 @@ def test_tessellated_pieces_lie_on_the_catmull_rom_curve(tmp_path):
         # the curve interpolates the strand's points
         starts = seg[k - tess * (len(s) - 1):k:tess, 0:3]
         np.testing.assert_allclose(starts, s[:-1], atol=1e-6)
+
+
+def test_round_curves_keep_the_control_points(tmp_path):
+    """hair.primitive "curve" (the default): one record per Catmull-Rom
+    segment with its four control points (phantoms mirrored past the strand
+    ends, render_path_tracing.cu:236-244 / scene.cpp:47-66), radii 0.2 x
+    thickness, the strand id and the end flags; the segment records are the
+    chords between consecutive strand points."""
+    import hairmsnn_amd as hm
+    strands = [helix(9, seed=1), helix(1, seed=2), helix(4, seed=3)]
+    thick = [np.linspace(0.04, 0.08, len(s)) for s in strands]
+    p = tmp_path / "c.hair"
+    write_hair(p, strands, thickness=thick)
+    cfg = scene_with(p, 4)
+    del cfg["hair"]["primitive"]
+    crv = hm.generate_curves_host(cfg).astype(np.float64)
+    seg = segments(cfg).astype(np.float64)
+    assert len(crv) == len(seg) == sum(len(s) - 1 for s in strands)
+    k = 0
+    for sid, (s, th) in enumerate(zip(strands, thick)):
+        s = np.asarray(s, np.float32).astype(np.float64)
+        cp = np.concatenate([[2 * s[0] - s[1]], s, [2 * s[-1] - s[-2]]])
+        w = 0.2 * np.concatenate([[th[0]], th, [th[-1]]]).astype(np.float32)
+        n = len(s) - 1
+        for i in range(n):
+            q = crv[k, :16].reshape(4, 4)
+            np.testing.assert_allclose(q[:, :3], cp[i:i + 4], atol=1e-6)
+            np.testing.assert_allclose(q[:, 3], w[i:i + 4], rtol=1e-6)
+            assert crv[k, 16] == sid
+            assert crv[k, 17] == (1 if i == 0 else 0) + (2 if i == n - 1 else 0)
+            np.testing.assert_allclose(seg[k, 0:3], s[i], atol=1e-6)
+            np.testing.assert_allclose(seg[k, 4:7], s[i + 1], atol=1e-6)
+            assert seg[k, 7] == sid
+            k += 1
+    # "linear" keeps the pieces and no curves
+    cfg["hair"]["primitive"] = "linear"
+    assert hm.generate_curves_host(cfg).shape[0] == 0
+    cfg["hair"]["primitive"] = "cubic"
+    with pytest.raises(hm.HmsnnError):
+        hm.generate_curves_host(cfg)
 
 
 def test_default_segment_count(tmp_path):

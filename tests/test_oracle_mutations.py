@@ -36,7 +36,20 @@ MUTANTS = [
     ("hair frame Y = X x wo", "orc_scene.cpp",
      "const Vec<R> Y = nrm(cross(si.wo, X));", "const Vec<R> Y = nrm(cross(X, si.wo));", 0, "hair_hit_geometry"),
     ("hit point not moved onto the fibre surface", "orc_scene.cpp",
-     "o1 = (r / len(o1)) * o1;", "o1 = ((R)0.999 * r / len(o1)) * o1;", 0, "hair_hit_geometry"),
+     "o1 = (r / len(o1)) * o1;", "o1 = ((R)0.999 * r / len(o1)) * o1;", 1, "hair_hit_geometry"),
+    ("curve: Catmull-Rom u^2 coefficient with -4 q1 (FP32)", "orc_curve.cpp",
+     "k.b = ((2.f * q0 - 5.f * q1) + (4.f * q2 - q3)) * 0.5f;",
+     "k.b = ((2.f * q0 - 4.f * q1) + (4.f * q2 - q3)) * 0.5f;", 0, "catmull_rom"),
+    ("curve: Catmull-Rom tangent (q2 - q1) / 2 (FP64)", "orc_curve.cpp",
+     "0.5 * (q2 - q0), q1};", "0.5 * (q2 - q1), q1};", 0, "catmull_rom"),
+    ("curve: closest approach to the chord with its sign flipped", "orc_curve.cpp",
+     "(dd * ew - de * dw) / den", "(de * dw - dd * ew) / den", 0, "hair_like_strands"),
+    ("curve: secant step away from the root", "orc_curve.cpp",
+     "du = -f * (u - up) / (f - fp);", "du = f * (u - up) / (f - fp);", 0, "hair_like_strands or cylinder"),
+    ("curve: normal without the radius derivative's sign", "orc_scene.cpp",
+     "n = nrm(dd * o1 - (dr * r) * dv);", "n = nrm(dd * o1 + (dr * r) * dv);", 0, "hit_frame"),
+    ("curve: hit point not moved onto the swept circle", "orc_scene.cpp",
+     "o1 = (r / len(o1)) * o1;", "o1 = ((R)0.999 * r / len(o1)) * o1;", 0, "hit_frame"),
     ("R lobe tilted by +2 alpha", "orc_math.h",
      "sOp = sinThetaO * hs.cos2k[1] - cosThetaO * hs.sin2k[1];",
      "sOp = sinThetaO * hs.cos2k[1] + cosThetaO * hs.sin2k[1];", 0, "lobe_tilts"),
@@ -75,7 +88,8 @@ def _build(tmp: Path, i: int, mutant) -> Path:
 
 def _run_pins(lib: Path, kexpr: str | None) -> subprocess.CompletedProcess:
     env = dict(os.environ, HMSNN_ORACLE_LIB=str(lib), OMP_NUM_THREADS="2")
-    cmd = [sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider", str(ROOT / "tests" / "test_oracle_pins.py")]
+    cmd = [sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider", str(ROOT / "tests" / "test_oracle_pins.py"),
+           str(ROOT / "tests" / "test_oracle_curves.py")]
     if kexpr:
         cmd += ["-k", kexpr]
     return subprocess.run(cmd, cwd=ROOT, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
