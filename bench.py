@@ -48,7 +48,12 @@ SHADE_VERTEX_BYTES, SHADE_HIT_BYTES, SHADE_MOVE_BYTES, SHADE_DONE_BYTES, SHADE_S
 # WRITE_SIZE runs (copy-calibrated); counters cannot be read from inside this
 # process
 TRAFFIC_JSON = ROOT / "profiles" / "pt_c1_1024_hbm_traffic.json"
-PASS_KERNELS = {"extend": "k_extend<false, true>", "shade": "k_shade_dense", "shadow": "k_shadow<false>"}
+PASS_KERNELS = {"extend": "k_extend<false, true, false>", "shade": "k_shade_dense", "shadow": "k_shadow<false, false>"}
+# FP32 VALU operations per launch (rocprofv3 SQ_INSTS_VALU_FLOPS_FP32 x 64
+# lanes) of the pass kernels and of the BSDF streams, for the VALU FLOP rates
+FLOPS_JSON = ROOT / "profiles" / "pt_c1_1024_flops.json"
+KERNEL_FLOPS_JSON = ROOT / "profiles" / "bsdf_mlp_flops.json"
+VALU_FP32_PEAK_TFLOPS = 157.3
 
 
 def log(msg: str) -> None:
@@ -70,12 +75,53 @@ def cpu_baseline(sc, side: int = 128, spp: int = 64) -> dict:
     t0 = time.perf_counter()
     osc.render(x0, y0, side, side, spp=spp, accum_id0=1, prec=1, threads=threads)
     dt = time.perf_counter() - t0
-    return {
+    out = {
         "value": side * side * spp / dt, "unit": "paths/s", "cores": threads, "kind": "port",
         "sample": f"centre {side}x{side} crop of the same 1024x1024 frame, {spp} spp, FP32 oracle "
                   f"(C++ restatement of pathTrace, CPU BVH built in {build_s:.1f} s, not timed), "
                   f"{dt:.1f} s of CPU work",
     }
+    out["c0_full"] = cpu_c0_full(threads)
+    out["mlp_fwd_bwd"] = cpu_mlp(1 << 16)
+    return out
+
+
+def cpu_c0_full(threads: int, spp: int = 16) -> dict:
+    """The FP32 oracle on the whole C0 frame (the hair-ball scene at 256x256,
+    spp passes), on the same host cores."""
+    import hairmsnn_amd as hm
+    from tests import oracle_lib as ol
+    cfg = json.loads((ROOT / "hairmsnn_amd" / "scenes" / "c0_hairball.json").read_text())
+    with hm.Scene(json_text=cfg) as sc:
+        osc = ol.OracleScene(sc.info, sc.segments(), sc.env() if sc.info["has_env"] else None, mesh=sc.mesh())
+        W, H = sc.width, sc.height
+    t0 = time.perf_counter()
+    osc.render(0, 0, W, H, spp=spp, accum_id0=0, prec=1, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"paths_per_s": W * H * spp / dt, "paths": W * H * spp, "s": dt,
+            "sample": f"c0_hairball {W}x{H} x {spp} spp, FP32 oracle, {threads} threads"}
+
+
+def cpu_mlp(n: int) -> dict:
+    """The oracle network's forward + backward (loss and weight gradient) on n
+    seeded records in FP32 and FP64 (124,032 FLOP per record)."""
+    from tests import oracle_lib as ol
+    rng = np.random.default_rng(8)
+    rec = np.zeros((n, 12), np.float32)
+    rec[:, 0:3] = rng.uniform(-0.5, 0.5, (n, 3))
+    for a in (3, 6):
+        v = rng.normal(size=(n, 3))
+        rec[:, a:a + 3] = v / np.linalg.norm(v, axis=1, keepdims=True)
+    rec[:, 9:12] = rng.uniform(0.01, 0.5, (n, 3))
+    tgt = rng.uniform(0.1, 2.0, (n, 3)).astype(np.float32)
+    w = rng.uniform(-0.2, 0.2, ol.MLP_WEIGHTS).astype(np.float32)
+    out = {"records": n}
+    for name, prec in (("fp32", 1), ("fp64", 0)):
+        t0 = time.perf_counter()
+        ol.mlp_loss_grad(w, rec, tgt, loss=0, prec=prec, exp_floor=ol.MLP_EXP_GRAD_FLOOR)
+        dt = time.perf_counter() - t0
+        out[name] = {"s": dt, "records_per_s": n / dt, "GFLOP/s": 124032.0 * n / dt / 1e9}
+    return out
 
 
 def mlp_rates(hm, n: int = 1 << 20, reps: int = 10) -> dict:
@@ -279,6 +325,7 @@ def bsdf_rates(hm, n: int = 1 << 22, reps: int = 20) -> dict:
     u = rng.random((n, 4), dtype=np.float32)
     dq, du = hm.DeviceBuffer.from_numpy(q), hm.DeviceBuffer.from_numpy(u)
     dwi, df, dp, dl = hm.DeviceBuffer(12 * n), hm.DeviceBuffer(12 * n), hm.DeviceBuffer(4 * n), hm.DeviceBuffer(4 * n)
+    kernel_flops = json.loads(KERNEL_FLOPS_JSON.read_text()).get("kernels", {}) if KERNEL_FLOPS_JSON.exists() else {}
     prm = hm._hair_params(0.3, 0.3, float(np.radians(2.0)))
     calls = {
         "eval": (lambda: lib.hmsnn_hair_bsdf_eval_dev(C.byref(prm), dq.ptr, n, df.ptr, dp.ptr, None), 40 + 16),
@@ -297,6 +344,11 @@ def bsdf_rates(hm, n: int = 1 << 22, reps: int = 20) -> dict:
         gbs = nbytes * n / dt / 1e9
         out[name] = {"queries": n, "ms": dt * 1e3, "Gqueries/s": n / dt / 1e9, "bytes_per_query": nbytes,
                      "GB/s": gbs, "peak_GB/s": 8000.0, "frac": gbs / 8000.0}
+        fl = kernel_flops.get(f"k_bsdf_{name}", {}).get("valu_flops_per_launch")
+        if fl:
+            out[name]["valu_flops_per_query"] = fl / n
+            out[name]["valu_TFLOPs"] = fl / dt / 1e12
+            out[name]["valu_frac"] = fl / dt / 1e12 / VALU_FP32_PEAK_TFLOPS
     return out
 
 
@@ -422,6 +474,9 @@ def main() -> None:
     traffic_k = {}
     if TRAFFIC_JSON.exists():
         traffic_k = json.loads(TRAFFIC_JSON.read_text()).get("kernels", {})
+    flops_k = {}
+    if FLOPS_JSON.exists():
+        flops_k = json.loads(FLOPS_JSON.read_text()).get("kernels", {})
     per_kernel = {}
     for k, kname in PASS_KERNELS.items():
         ms_pass = kt[f"{k}_ms"] / passes_t
@@ -429,9 +484,12 @@ def main() -> None:
         b_launch = alg_pass[k] / launches_pass
         ach = b_launch / (ms_launch * 1e-3) / 1e9
         tr = traffic_k.get(kname, {}).get("traffic_bytes_per_launch")
+        fl = flops_k.get(kname, {}).get("valu_flops_per_launch")
         per_kernel[k] = {"kernel": kname, "ms_per_pass": ms_pass, "ms_per_launch": ms_launch,
                          "bytes_per_launch": b_launch, "achieved": ach, "frac": ach / HBM_PEAK_GBS, "traffic": tr,
-                         "traffic_GBps": tr / (ms_launch * 1e-3) / 1e9 if tr else None}
+                         "traffic_GBps": tr / (ms_launch * 1e-3) / 1e9 if tr else None,
+                         "valu_TFLOPs": fl / (ms_launch * 1e-3) / 1e12 if fl else None,
+                         "valu_frac": fl / (ms_launch * 1e-3) / 1e12 / VALU_FP32_PEAK_TFLOPS if fl else None}
     dom = max(per_kernel, key=lambda k: per_kernel[k]["ms_per_pass"])
     for k, v in per_kernel.items():
         log(f"{k}: {v['ms_per_pass']:.2f} ms/pass, {v['ms_per_launch']:.3f} ms/launch, "

@@ -267,6 +267,10 @@ PassConfig Context::pt_pass_config() const {
   return c;
 }
 
+static int bvh_builder_id(const std::string& name) {
+  return name == "ploc" ? kBvhPloc : name == "sah" ? kBvhSah : kBvhLbvh;
+}
+
 int Context::init(std::string& err) {
   HM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), err);
   int dev = 0;
@@ -325,7 +329,7 @@ int Context::init(std::string& err) {
     float* boxes_dev = nullptr;
     HM_TRY(dev_alloc((void**)&boxes_dev, sizeof(float) * boxes.size(), err));
     HM_HIP(hipMemcpy(boxes_dev, boxes.data(), sizeof(float) * boxes.size(), hipMemcpyHostToDevice), err);
-    const int rc = bvh_build(boxes_dev, nt, mesh_bvh, stream, err, scene.bvh_builder == "ploc" ? kBvhPloc : kBvhLbvh,
+    const int rc = bvh_build(boxes_dev, nt, mesh_bvh, stream, err, bvh_builder_id(scene.bvh_builder),
                              scene.ploc_radius);
     (void)hipStreamSynchronize(stream);
     dev_free(boxes_dev);
@@ -342,7 +346,7 @@ int Context::init(std::string& err) {
     const size_t bytes = sizeof(float) * hair.seg.size();
     HM_TRY(dev_alloc((void**)&seg_dev, bytes, err));
     HM_HIP(hipMemcpyAsync(seg_dev, hair.seg.data(), bytes, hipMemcpyHostToDevice, stream), err);
-    const int builder = scene.bvh_builder == "ploc" ? kBvhPloc : kBvhLbvh;
+    const int builder = bvh_builder_id(scene.bvh_builder);
     if (hair.crv.empty()) {
       HM_TRY(bvh_build(seg_dev, (int)(hair.seg.size() / 8), bvh, stream, err, builder, scene.ploc_radius));
     } else {

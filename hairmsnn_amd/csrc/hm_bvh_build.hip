@@ -9,6 +9,8 @@
 // (render_path_tracing.cu:236-256, render_hair_msnn.cu:370-413).
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
@@ -560,6 +562,34 @@ int bvh_build(const float* seg_dev, int n, DeviceBvh& out, hipStream_t stream, s
   const int bs = 256, gs = (n + bs - 1) / bs;
   hipLaunchKernelGGL(k_leaf_boxes, dim3(gs), dim3(bs), 0, stream, b);
   HM_TRY(check_launch("bvh leaf boxes", err));
+  if (builder == kBvhSah) {
+    // the tree on the host from the padded leaf boxes, then the same refit,
+    // emission and collapse as the radix tree
+    std::vector<float> boxes((size_t)6 * n);
+    std::vector<int> child((size_t)2 * (n - 1)), order(n), pin(n - 1), pleaf(n);
+    HM_HIP(hipMemcpyAsync(boxes.data(), b.leaf_box, sizeof(float) * boxes.size(), hipMemcpyDeviceToHost, stream),
+           err);
+    HM_HIP(hipStreamSynchronize(stream), err);
+    sah_build(boxes.data(), n, child.data(), order.data(), pin.data(), pleaf.data());
+    HM_HIP(hipMemcpyAsync(b.child, child.data(), sizeof(int) * child.size(), hipMemcpyHostToDevice, stream), err);
+    HM_HIP(hipMemcpyAsync(b.vals_sorted, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, stream),
+           err);
+    HM_HIP(hipMemcpyAsync(b.parent_internal, pin.data(), sizeof(int) * pin.size(), hipMemcpyHostToDevice, stream),
+           err);
+    HM_HIP(hipMemcpyAsync(b.parent_leaf, pleaf.data(), sizeof(int) * pleaf.size(), hipMemcpyHostToDevice, stream),
+           err);
+    hipLaunchKernelGGL(k_refit, dim3(gs), dim3(bs), 0, stream, b);
+    HM_TRY(check_launch("bvh refit", err));
+    hipLaunchKernelGGL(k_emit, dim3(gs), dim3(bs), 0, stream, b, out.nodes, out.segs);
+    HM_TRY(check_launch("bvh emit", err));
+    hipLaunchKernelGGL(k_depth, dim3(gs), dim3(bs), 0, stream, b);
+    HM_TRY(check_launch("bvh depth", err));
+    int depth = 0;
+    HM_HIP(hipMemcpyAsync(&depth, b.max_depth, sizeof(int), hipMemcpyDeviceToHost, stream), err);
+    HM_HIP(hipStreamSynchronize(stream), err);
+    out.max_depth = depth;
+    return bvh_collapse(out, stream, err);
+  }
   hipLaunchKernelGGL(k_morton, dim3(gs), dim3(bs), 0, stream, b);
   HM_TRY(check_launch("bvh morton", err));
   size_t tmp_bytes = 0;

@@ -96,7 +96,12 @@ struct DeviceBvh {
 
 // Binary tree builders, collapsed into the same wide layout: the Karras
 // radix tree over Morton codes, or PLOC clustering in Morton order.
-constexpr int kBvhLbvh = 0, kBvhPloc = 1;
+constexpr int kBvhLbvh = 0, kBvhPloc = 1, kBvhSah = 2;
+// Binned SAH binary tree over n boxes (6 floats each: lo xyz, hi xyz), built
+// on the host (hm_bvh_sah.cpp): child[2 (n - 1)] (>= 0 an inner node, ~p the
+// leaf at position p; root 0), order[n] (leaf position -> box index),
+// parent_inner[n - 1] (-1 at the root), parent_leaf[n].
+void sah_build(const float* boxes, int n, int* child, int* order, int* parent_inner, int* parent_leaf);
 int bvh_build(const float* seg_dev, int n, DeviceBvh& out, hipStream_t stream, std::string& err,
               int builder = kBvhLbvh, int ploc_radius = 16);
 

@@ -291,8 +291,8 @@ bool parse_json(const Json& cfg, Scene& scene, std::string& err) {
     const Json& a = cfg.at("accel");
     if (a.has("builder")) scene.bvh_builder = a.at("builder").str();
     if (a.has("ploc_radius")) scene.ploc_radius = a.at("ploc_radius").i();
-    if (scene.bvh_builder != "lbvh" && scene.bvh_builder != "ploc") {
-      err = "accel.builder must be \"lbvh\" or \"ploc\"";
+    if (scene.bvh_builder != "lbvh" && scene.bvh_builder != "ploc" && scene.bvh_builder != "sah") {
+      err = "accel.builder must be \"lbvh\", \"ploc\" or \"sah\"";
       return false;
     }
   }

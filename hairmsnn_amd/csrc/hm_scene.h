@@ -112,7 +112,7 @@ struct Scene {
 
   // acceleration structure extension block: "lbvh" (Karras radix tree) or
   // "ploc" (locally-ordered clustering) binary tree, collapsed 4-wide
-  std::string bvh_builder = "lbvh";
+  std::string bvh_builder = "sah";  // accel.builder: "sah" (host, binned SAH), "lbvh", "ploc"
   int ploc_radius = 16;
 
   // neural radiance caching extension block (render_nrc.h:112-134 defaults)

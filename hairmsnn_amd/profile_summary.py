@@ -4,6 +4,7 @@ and, for HBM traffic, the per-kernel JSON that bench.py reads.
     python -m hairmsnn_amd.profile_summary trace RUN.db
     python -m hairmsnn_amd.profile_summary calls RUN.db KERNEL [KERNEL ...]
     python -m hairmsnn_amd.profile_summary pmc RUN.db
+    python -m hairmsnn_amd.profile_summary flops RUN.db [RUN.db ...] --json out.json
     python -m hairmsnn_amd.profile_summary traffic --fetch F.db --write W.db \
         --copy-fetch CF.db --copy-write CW.db --copy-mib 1024 --json out.json
 
@@ -14,7 +15,11 @@ followed by a full-size one), with the mean duration per call. pmc: the mean per
 of every counter collected (summed over a dispatch's instances). traffic:
 FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by what the same
 counters read for the device-to-device copy kernel of --copy-mib MiB (which
-reads and writes that many bytes), giving HBM bytes per launch.
+reads and writes that many bytes), giving HBM bytes per launch. flops:
+FP32 VALU operations per launch (SQ_INSTS_VALU_FLOPS_FP32 and its
+transcendental counterpart count per wave instruction, so x 64 lanes) and
+matrix-core operations (SQ_INSTS_VALU_MFMA_MOPS_* x 512), with the rate over
+each launch's mean duration.
 """
 from __future__ import annotations
 
@@ -23,15 +28,57 @@ import collections
 import json
 import re
 import sqlite3
+import subprocess
 import sys
 from pathlib import Path
 
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
 
 
+_DEMANGLED: dict[str, str] = {}
+
+
+def _nested_name(name: str) -> str:
+    """The function name and a template argument of a mangled nested name
+    (_ZN<len><id>...I<arg>E...), for names c++filt leaves alone (_Float16
+    arguments)."""
+    if not name.startswith("_ZN"):
+        return name
+    i, ids = 3, []
+    while i < len(name) and name[i].isdigit():
+        j = i
+        while j < len(name) and name[j].isdigit():
+            j += 1
+        n = int(name[i:j])
+        ids.append(name[j:j + n])
+        i = j + n
+    if not ids:
+        return name
+    args = {"DF16_": "_Float16", "DF16b": "__bf16", "f": "float", "d": "double"}
+    if name.startswith("I", i):
+        for code, arg in args.items():
+            if name.startswith(code, i + 1):
+                return f"{ids[-1]}<{arg}>"
+    return ids[-1]
+
+
+def _demangle(name: str) -> str:
+    if not name.startswith("_Z"):
+        return name
+    if name not in _DEMANGLED:
+        out = name
+        try:
+            r = subprocess.run(["c++filt", name], stdout=subprocess.PIPE, text=True, timeout=10)
+            out = r.stdout.strip() or name
+        except (OSError, subprocess.SubprocessError):
+            pass
+        _DEMANGLED[name] = out if not out.startswith("_Z") else _nested_name(out)
+    return _DEMANGLED[name]
+
+
 def short_name(name: str) -> str:
     """Kernel name without its argument list, return type and namespace."""
-    n = name.strip().replace("(anonymous namespace)::", "")
+    n = _demangle(name.strip()).replace("(anonymous namespace)::", "")
     if n.startswith("void "):
         n = n[5:]
     depth, cut = 0, len(n)
@@ -130,6 +177,29 @@ def cmd_pmc(a) -> None:
         print(f"| {k} | {int(v['_launches'])} | {v['_us']:.1f} | {vals} |")
 
 
+def cmd_flops(a) -> None:
+    per: dict[str, dict[str, float]] = {}
+    for db in a.dbs:
+        for k, v in counters(db).items():
+            per.setdefault(k, {}).update(v)
+    print("| kernel | launches | avg us | VALU FP32 GFLOP / launch | VALU TFLOP/s | MFMA GFLOP / launch | MFMA TFLOP/s |")
+    print("|---|---:|---:|---:|---:|---:|---:|")
+    out = {}
+    for k, v in sorted(per.items(), key=lambda kv: -kv[1]["_us"] * kv[1]["_launches"]):
+        valu = 64.0 * (v.get("SQ_INSTS_VALU_FLOPS_FP32", 0.0) + v.get("SQ_INSTS_VALU_FLOPS_FP32_TRANS", 0.0))
+        mfma = 512.0 * sum(x for c, x in v.items() if c.startswith("SQ_INSTS_VALU_MFMA_MOPS_"))
+        if valu == 0 and mfma == 0:
+            continue
+        us = v["_us"]
+        out[k] = {"launches": int(v["_launches"]), "avg_us": us, "valu_flops_per_launch": valu,
+                  "mfma_flops_per_launch": mfma}
+        if len(out) <= a.top:
+            print(f"| {k} | {int(v['_launches'])} | {us:.1f} | {valu / 1e9:.3f} | {valu / (us * 1e-6) / 1e12:.1f} | "
+                  f"{mfma / 1e9:.3f} | {mfma / (us * 1e-6) / 1e12:.1f} |")
+    if a.json:
+        Path(a.json).write_text(json.dumps({"workload": a.workload, "kernels": out}, indent=1) + "\n")
+
+
 def cmd_traffic(a) -> None:
     fetch, write = counters(a.fetch), counters(a.write)
     cf, cw = counters(a.copy_fetch), counters(a.copy_write)
@@ -175,6 +245,11 @@ def main(argv=None) -> None:
     p = sub.add_parser("pmc")
     p.add_argument("db")
     p.add_argument("--top", type=int, default=12)
+    fl = sub.add_parser("flops")
+    fl.add_argument("dbs", nargs="+")
+    fl.add_argument("--json")
+    fl.add_argument("--workload", default="")
+    fl.add_argument("--top", type=int, default=12)
     f = sub.add_parser("traffic")
     for n in ("--fetch", "--write", "--copy-fetch", "--copy-write"):
         f.add_argument(n, required=True)
@@ -183,7 +258,7 @@ def main(argv=None) -> None:
     f.add_argument("--workload", default="")
     f.add_argument("--top", type=int, default=12)
     a = ap.parse_args(argv)
-    {"trace": cmd_trace, "calls": cmd_calls, "pmc": cmd_pmc, "traffic": cmd_traffic}[a.cmd](a)
+    {"trace": cmd_trace, "calls": cmd_calls, "pmc": cmd_pmc, "flops": cmd_flops, "traffic": cmd_traffic}[a.cmd](a)
 
 
 if __name__ == "__main__":

@@ -1,5 +1,6 @@
-"""GPU LBVH traversal against CPU intersection: closest-hit segment ids and
-any-hit bits must be bit-exact (ties resolve to the lower segment id)."""
+"""GPU BVH traversal against CPU intersection: closest-hit segment ids and
+any-hit bits must be bit-exact (ties resolve to the lower segment id), for
+every tree builder (binned SAH, the default; radix tree; PLOC)."""
 import json
 from pathlib import Path
 
@@ -118,14 +119,42 @@ def test_ploc_tree_bit_exact(gpu, name, radius):
     np.testing.assert_array_equal(occ, occ_c)
 
 
-def test_default_lbvh_full_c1_bit_exact(gpu):
-    """The default (radix-tree) builder on the full C1 geometry, 50,000
+@pytest.mark.parametrize("name", ["c0_hairball.json", "c1_curly.json"])
+def test_sah_tree_bit_exact(gpu, name):
+    """The binned-SAH tree built on the host (accel.builder = "sah") gives the
+    same closest and any hits as the CPU BVH."""
+    cfg = json.loads((SCENES / name).read_text())
+    cfg["integrator"].update({"width": 128, "height": 128})
+    cfg["accel"] = {"builder": "sah"}
+    cfg.pop("surface", None)
+    if "procedural" in cfg["hair"]:
+        cfg["hair"]["procedural"]["strands"] = min(cfg["hair"]["procedural"]["strands"], 5000)
+    with gpu.Scene(json_text=cfg) as sc:
+        seg = sc.segments()
+        rays = make_rays(seg, 60_000, 11)
+        prim, t, u = sc.trace_rays(rays)
+        occ = sc.trace_rays(rays, any_hit=True)
+        depth = int(sc.info["bvh_depth"])
+    prim_c, t_c, u_c = ol.intersect(seg, rays, use_bvh=True)
+    hit = prim_c >= 0
+    print(f"{name} SAH: wide depth {depth}, hit fraction {hit.mean():.3f}")
+    assert hit.mean() > 0.1
+    np.testing.assert_array_equal(prim, prim_c)
+    np.testing.assert_array_equal(t[hit], t_c[hit])
+    np.testing.assert_array_equal(u[hit], u_c[hit])
+    occ_c = ol.intersect(seg, rays, any_hit=True, use_bvh=True)
+    np.testing.assert_array_equal(occ, occ_c)
+
+
+@pytest.mark.parametrize("builder", ["lbvh", "sah"])
+def test_full_c1_bit_exact(gpu, builder):
+    """The radix-tree and SAH builders on the full C1 geometry, 50,000
     strands x 64 segments = 3.2M segments: closest and any hits bit-exact
     against the CPU BVH."""
     cfg = json.loads((SCENES / "c1_curly.json").read_text())
     cfg["integrator"].update({"width": 128, "height": 128})
     cfg.pop("surface", None)  # hair only: the CPU search below has no head sphere
-    cfg.pop("accel", None)
+    cfg["accel"] = {"builder": builder}
     with gpu.Scene(json_text=cfg) as sc:
         seg = sc.segments()
         assert seg.shape[0] == 3_200_000
@@ -134,7 +163,7 @@ def test_default_lbvh_full_c1_bit_exact(gpu):
         occ = sc.trace_rays(rays, any_hit=True)
     prim_c, t_c, u_c = ol.intersect(seg, rays, use_bvh=True)
     hit = prim_c >= 0
-    print(f"C1 LBVH: hit fraction {hit.mean():.3f}")
+    print(f"C1 {builder}: hit fraction {hit.mean():.3f}")
     assert 0.2 < hit.mean() < 0.99
     np.testing.assert_array_equal(prim, prim_c)
     np.testing.assert_array_equal(t[hit], t_c[hit])
