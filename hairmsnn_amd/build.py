@@ -60,11 +60,15 @@ def _run(cmd: list[str]) -> None:
 # the surrounding kernel looks like. (Explicit fmaf calls are kept.)
 NO_CONTRACT = {"hm_pt.hip", "hm_adjoint.hip"}
 
+# The MLP kernels keep matrix-core accumulators in ordinary VGPRs, so the
+# ReLU/convert tails read them without copies out of the accumulator file.
+MLP_FLAGS = {"hm_mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+
 
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.name + ".o")
     if src.suffix == ".hip":
-        extra = ["-ffp-contract=off"] if src.name in NO_CONTRACT else []
+        extra = (["-ffp-contract=off"] if src.name in NO_CONTRACT else []) + MLP_FLAGS.get(src.name, [])
         cmd = [hipcc(), *HIP_FLAGS, *extra, "-x", "hip", "-c", str(src), "-o", str(obj)]
     else:
         cmd = [hipcc(), *HIP_FLAGS, "-c", str(src), "-o", str(obj)]

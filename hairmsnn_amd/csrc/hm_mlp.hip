@@ -38,6 +38,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/hmsnn.h"
@@ -104,12 +105,14 @@ constexpr int kFrag8Hidden = 5 * 2 * 64;
 constexpr int kFrag8Total = kFrag8Hidden + 64;
 constexpr float kE4M3Max = 448.f;
 
-// Power-of-two exponent e with amax / 2^e < 448 (0 for an all-zero layer).
+// Power-of-two exponent e with amax / 2^e < 448 (0 for an all-zero layer),
+// limited to [-63, 63] so that the difference of two exponents still fits the
+// matrix instruction's biased 8-bit scale (127 + e in 0..254).
 __host__ __device__ inline int fp8_exponent(float amax) {
   if (!(amax > 0.f) || !(amax < 3.0e38f)) return 0;
   int e;
   (void)frexpf(amax / kE4M3Max, &e);
-  return e < -100 ? -100 : (e > 100 ? 100 : e);
+  return e < -63 ? -63 : (e > 63 ? 63 : e);
 }
 
 __device__ __forceinline__ float fp8_clamp(float x) { return fminf(fmaxf(x, -kE4M3Max), kE4M3Max); }
@@ -118,6 +121,14 @@ __device__ __forceinline__ float fp8_clamp(float x) { return fminf(fmaxf(x, -kE4
 __device__ __forceinline__ int fp8_pack4(float a, float b, float c, float d) {
   const int w = __builtin_amdgcn_cvt_pk_fp8_f32(fp8_clamp(a), fp8_clamp(b), 0, false);
   return __builtin_amdgcn_cvt_pk_fp8_f32(fp8_clamp(c), fp8_clamp(d), w, true);
+}
+
+// Four non-negative-clamped floats -> four E4M3 bytes: med3(x, 0, 448).
+__device__ __forceinline__ int fp8_pack4_relu(float a, float b, float c, float d) {
+  const int w = __builtin_amdgcn_cvt_pk_fp8_f32(__builtin_amdgcn_fmed3f(a, 0.f, kE4M3Max),
+                                                __builtin_amdgcn_fmed3f(b, 0.f, kE4M3Max), 0, false);
+  return __builtin_amdgcn_cvt_pk_fp8_f32(__builtin_amdgcn_fmed3f(c, 0.f, kE4M3Max),
+                                         __builtin_amdgcn_fmed3f(d, 0.f, kE4M3Max), w, true);
 }
 
 // Builds the permuted fragments from the FP32 master weights.
@@ -189,6 +200,42 @@ __device__ __forceinline__ Encoded encode(const float* rec) {
   return e;
 }
 
+// encode() for a wave whose lane halves hold the same 32 records (lane and
+// lane ^ 32): each half evaluates the spherical angles of one of the two
+// directions (acosf and atan2f are the costly part) and takes the other's
+// from its partner lane. The features are the same values encode() gives.
+// Every lane of the wave must call it.
+__device__ __forceinline__ Encoded encode_wave(const float* rec, int g) {
+#pragma clang fp contract(off)
+  Encoded e;
+  const float kPiF = 3.14159265358979f;
+  for (int c = 0; c < 3; ++c) {
+    float s, co;
+    sincospif(rec[c], &s, &co);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      e.f[12 * c + 2 * k] = s;
+      e.f[12 * c + 2 * k + 1] = co;
+      const float s2 = 2.f * s * co, c2 = co * co - s * s;  // double angle
+      s = s2;
+      co = c2;
+    }
+  }
+  const float x = g ? rec[6] : rec[3], y = g ? rec[7] : rec[4], z = g ? rec[8] : rec[5];
+  const float th = acosf(fminf(1.f, fmaxf(-1.f, z))) / kPiF;
+  const float ph = atan2f(y, x) / (2.f * kPiF) + 0.5f;
+  const float th_o = __shfl_xor(th, 32), ph_o = __shfl_xor(ph, 32);
+  e.f[36] = g ? th_o : th;
+  e.f[37] = g ? ph_o : ph;
+  e.f[38] = g ? th : th_o;
+  e.f[39] = g ? ph : ph_o;
+  e.f[40] = rec[9];
+  e.f[41] = rec[10];
+  e.f[42] = rec[11];
+  e.f[43] = 1.f;
+  return e;
+}
+
 template <class T>
 __device__ __forceinline__ typename MfmaTraits<T>::vec encoded_operand(const Encoded& e, int s, int g) {
   typename MfmaTraits<T>::vec b;
@@ -209,6 +256,33 @@ __device__ __forceinline__ void stage_fragments(const T* frag, typename MfmaTrai
   __syncthreads();
 }
 
+// ReLU of the two accumulator blocks into the next layer's operands. FP16:
+// convert in pairs, then a packed max with 0 (max(cvt(x), 0) == cvt(max(x, 0))
+// up to the sign of a zero, which no later sum sees); the conversions'
+// results need no NaN canonicalisation before the max, unlike fmaxf on the
+// matrix-core accumulators. BF16: in FP32.
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+template <class T>
+__device__ __forceinline__ void relu_into(const f32x16 acc[2], typename MfmaTraits<T>::vec B[4]) {
+  if constexpr (std::is_same<T, _Float16>::value) {
+    const f16x2 zero = {(_Float16)0.f, (_Float16)0.f};
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int v = 0; v < 8; v += 2) {
+        f16x2 h = {(_Float16)acc[s >> 1][8 * (s & 1) + v], (_Float16)acc[s >> 1][8 * (s & 1) + v + 1]};
+        h = __builtin_elementwise_max(h, zero);
+        B[s][v] = h[0];
+        B[s][v + 1] = h[1];
+      }
+  } else {
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int v = 0; v < 8; ++v) B[s][v] = MfmaTraits<T>::cvt(fmaxf(acc[s >> 1][8 * (s & 1) + v], 0.f));
+  }
+}
+
 // One hidden layer: acc[ob] = W_l (rows 32 ob..) x B, then ReLU into B.
 template <class T>
 __device__ __forceinline__ void hidden_layer(const typename MfmaTraits<T>::vec* lds, int l, int lane,
@@ -222,10 +296,7 @@ __device__ __forceinline__ void hidden_layer(const typename MfmaTraits<T>::vec* 
 #pragma unroll
     for (int ob = 0; ob < 2; ++ob) acc[ob] = M::mfma(lds[((l * 2 + ob) * 4 + s) * 64 + lane], B[s], acc[ob]);
   }
-#pragma unroll
-  for (int s = 0; s < 4; ++s)
-#pragma unroll
-    for (int v = 0; v < 8; ++v) B[s][v] = M::cvt(fmaxf(acc[s >> 1][8 * (s & 1) + v], 0.f));
+  relu_into<T>(acc, B);
 }
 
 template <class T>
@@ -238,6 +309,8 @@ __device__ __forceinline__ f32x16 output_layer(const typename MfmaTraits<T>::vec
 }
 
 constexpr int kMlpBlock = 256;
+// Inference blocks: eight waves share one LDS copy of the weight fragments.
+constexpr int kInferBlock = 512;
 
 // The backward pass takes the exp activation's derivative as max(y, floor).
 // With the exact derivative y, relative-L2 training drives every output
@@ -250,14 +323,14 @@ constexpr int kMlpBlock = 256;
 constexpr float kExpGradFloor = 1e-2f;
 
 template <class T>
-__global__ void __launch_bounds__(kMlpBlock) k_mlp_infer(const T* frag, const float* in, int n, float* out) {
+__global__ void __launch_bounds__(kInferBlock) k_mlp_infer(const T* frag, const float* in, int n, float* out) {
   typedef typename MfmaTraits<T>::vec V;
   __shared__ V lds[kFragFwdTotal];
   stage_fragments<T>(frag, lds, kFragFwdTotal);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 5, col = lane & 31;
-  const int waves = gridDim.x * (kMlpBlock / 64);
-  for (int tile = blockIdx.x * (kMlpBlock / 64) + wave; tile * 32 < n; tile += waves) {
+  const int waves = gridDim.x * (kInferBlock / 64);
+  for (int tile = blockIdx.x * (kInferBlock / 64) + wave; tile * 32 < n; tile += waves) {
     const int q = tile * 32 + col;
     float rec[12];
     if (q < n) {
@@ -269,7 +342,7 @@ __global__ void __launch_bounds__(kMlpBlock) k_mlp_infer(const T* frag, const fl
       rec[5] = 1.f;
       rec[8] = 1.f;
     }
-    const Encoded e = encode(rec);
+    const Encoded e = encode_wave(rec, g);
     V B[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) B[s] = encoded_operand<T>(e, s, g);
@@ -328,22 +401,22 @@ __global__ void k_mlp_permute_fp8(const float* w, const int* scales, i32x8* frag
 // Calibration: amax of every layer's input (the encoding, then each hidden
 // ReLU output) over the valid queries, from the BF16 / FP16 network.
 template <class T>
-__global__ void __launch_bounds__(kMlpBlock) k_mlp_act_amax(const T* frag, const float* in, int n, unsigned* amax) {
+__global__ void __launch_bounds__(kInferBlock) k_mlp_act_amax(const T* frag, const float* in, int n, unsigned* amax) {
   typedef typename MfmaTraits<T>::vec V;
   __shared__ V lds[kFragFwdTotal];
   stage_fragments<T>(frag, lds, kFragFwdTotal);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 5, col = lane & 31;
-  const int waves = gridDim.x * (kMlpBlock / 64);
+  const int waves = gridDim.x * (kInferBlock / 64);
   float mx[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  for (int tile = blockIdx.x * (kMlpBlock / 64) + wave; tile * 32 < n; tile += waves) {
+  for (int tile = blockIdx.x * (kInferBlock / 64) + wave; tile * 32 < n; tile += waves) {
     const int q = tile * 32 + col;
     const bool valid = q < n;
     float rec[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) rec[k] = valid ? in[(size_t)q * 12 + k] : 0.f;
     if (!valid) rec[5] = rec[8] = 1.f;
-    const Encoded e = encode(rec);
+    const Encoded e = encode_wave(rec, g);
     if (valid)
 #pragma unroll
       for (int k = 0; k < 44; ++k) mx[0] = fmaxf(mx[0], fabsf(e.f[k]));
@@ -391,15 +464,15 @@ __device__ __forceinline__ f32x16 mfma_fp8(const i32x8& a, const i32x8& b, f32x1
 }
 
 // scales: [0..5] weight exponents, [6..11] layer-input exponents.
-__global__ void __launch_bounds__(kMlpBlock) k_mlp_infer_fp8(const i32x8* frag, const int* scales, const float* in,
+__global__ void __launch_bounds__(kInferBlock) k_mlp_infer_fp8(const i32x8* frag, const int* scales, const float* in,
                                                              int n, float* out) {
   __shared__ i32x8 lds[kFrag8Total];
   for (int i = threadIdx.x; i < kFrag8Total; i += blockDim.x) lds[i] = frag[i];
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 5, col = lane & 31;
-  const int waves = gridDim.x * (kMlpBlock / 64);
-  for (int tile = blockIdx.x * (kMlpBlock / 64) + wave; tile * 32 < n; tile += waves) {
+  const int waves = gridDim.x * (kInferBlock / 64);
+  for (int tile = blockIdx.x * (kInferBlock / 64) + wave; tile * 32 < n; tile += waves) {
     const int q = tile * 32 + col;
     float rec[12];
     if (q < n) {
@@ -411,7 +484,7 @@ __global__ void __launch_bounds__(kMlpBlock) k_mlp_infer_fp8(const i32x8* frag, 
       rec[5] = 1.f;
       rec[8] = 1.f;
     }
-    const Encoded e = encode(rec);
+    const Encoded e = encode_wave(rec, g);
     i32x8 B;
     {
       const float sc = ldexpf(1.f, -scales[6]);
@@ -427,18 +500,18 @@ __global__ void __launch_bounds__(kMlpBlock) k_mlp_infer_fp8(const i32x8* frag, 
     }
 #pragma unroll 1
     for (int l = 0; l < 5; ++l) {
-      const int ew = scales[l], ex = scales[6 + l];
+      // The next layer's input scale 2^-scales[7 + l] rides on the weight
+      // exponent of this layer's matrix instruction (a power of two, so the
+      // accumulator is the scaled sum exactly); ReLU and the clamp to the
+      // E4M3 range are then one med3 per value.
+      const int ew = scales[l] - scales[7 + l], ex = scales[6 + l];
       const f32x16 a0 = mfma_fp8(lds[(l * 2 + 0) * 64 + lane], B, f32x16{}, ew, ex);
       const f32x16 a1 = mfma_fp8(lds[(l * 2 + 1) * 64 + lane], B, f32x16{}, ew, ex);
-      const float sc = ldexpf(1.f, -scales[7 + l]);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) B[k] = fp8_pack4_relu(a0[4 * k], a0[4 * k + 1], a0[4 * k + 2], a0[4 * k + 3]);
 #pragma unroll
       for (int k = 0; k < 4; ++k)
-        B[k] = fp8_pack4(fmaxf(a0[4 * k], 0.f) * sc, fmaxf(a0[4 * k + 1], 0.f) * sc, fmaxf(a0[4 * k + 2], 0.f) * sc,
-                         fmaxf(a0[4 * k + 3], 0.f) * sc);
-#pragma unroll
-      for (int k = 0; k < 4; ++k)
-        B[4 + k] = fp8_pack4(fmaxf(a1[4 * k], 0.f) * sc, fmaxf(a1[4 * k + 1], 0.f) * sc,
-                             fmaxf(a1[4 * k + 2], 0.f) * sc, fmaxf(a1[4 * k + 3], 0.f) * sc);
+        B[4 + k] = fp8_pack4_relu(a1[4 * k], a1[4 * k + 1], a1[4 * k + 2], a1[4 * k + 3]);
     }
     f32x16 z = mfma_fp8(lds[kFrag8Hidden + lane], B, f32x16{}, scales[5], scales[11]);
     // Keep the matrix instruction (and the copies that assemble its operands)
@@ -492,7 +565,7 @@ __global__ void __launch_bounds__(kMlpBlock) k_mlp_fwd_bwd(const T* frag, const 
       rec[5] = 1.f;
       rec[8] = 1.f;
     }
-    const Encoded e = encode(rec);
+    const Encoded e = encode_wave(rec, g);
     V B[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) B[s] = encoded_operand<T>(e, s, g);
@@ -749,9 +822,11 @@ int mlp_load(MlpState& s, const std::string& path, std::string& err) {
   return mlp_set_weights(s, w.data(), err);
 }
 
+// Grid of a grid-stride inference kernel: one 32-query tile per wave, at
+// most 1024 blocks.
 static int mlp_blocks(int n) {
-  const int tiles = (n + 31) / 32;
-  return std::max(1, std::min(1024, (tiles + 3) / 4));
+  const int tiles = (n + 31) / 32, per_block = kInferBlock / 64;
+  return std::max(1, std::min(1024, (tiles + per_block - 1) / per_block));
 }
 
 // Queries of the batch used to calibrate the FP8 activation scales.
@@ -763,21 +838,21 @@ int mlp_forward(MlpState& s, const float* in, int n, float* out, hipStream_t st,
     if (s.calib_dirty) {
       const int nc = std::min(n, kFp8CalibQueries);
       HM_HIP(hipMemsetAsync(s.amax, 0, sizeof(unsigned) * 6, st), err);
-      hipLaunchKernelGGL(k_mlp_act_amax<__bf16>, dim3(mlp_blocks(nc)), dim3(kMlpBlock), 0, st, (const __bf16*)s.frag,
+      hipLaunchKernelGGL(k_mlp_act_amax<__bf16>, dim3(mlp_blocks(nc)), dim3(kInferBlock), 0, st, (const __bf16*)s.frag,
                          in, nc, s.amax);
       hipLaunchKernelGGL(k_mlp_act_scales, dim3(1), dim3(64), 0, st, s.amax, s.scales);
       HM_TRY(check_launch("mlp fp8 calibration", err));
       s.calib_dirty = false;
     }
-    hipLaunchKernelGGL(k_mlp_infer_fp8, dim3(mlp_blocks(n)), dim3(kMlpBlock), 0, st, (const i32x8*)s.frag8, s.scales,
+    hipLaunchKernelGGL(k_mlp_infer_fp8, dim3(mlp_blocks(n)), dim3(kInferBlock), 0, st, (const i32x8*)s.frag8, s.scales,
                        in, n, out);
     return check_launch("mlp infer fp8", err);
   }
   if (s.precision == HMSNN_BF16)
-    hipLaunchKernelGGL(k_mlp_infer<__bf16>, dim3(mlp_blocks(n)), dim3(kMlpBlock), 0, st, (const __bf16*)s.frag, in,
+    hipLaunchKernelGGL(k_mlp_infer<__bf16>, dim3(mlp_blocks(n)), dim3(kInferBlock), 0, st, (const __bf16*)s.frag, in,
                        n, out);
   else
-    hipLaunchKernelGGL(k_mlp_infer<_Float16>, dim3(mlp_blocks(n)), dim3(kMlpBlock), 0, st,
+    hipLaunchKernelGGL(k_mlp_infer<_Float16>, dim3(mlp_blocks(n)), dim3(kInferBlock), 0, st,
                        (const _Float16*)s.frag, in, n, out);
   return check_launch("mlp infer", err);
 }

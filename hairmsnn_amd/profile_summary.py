@@ -233,6 +233,28 @@ def cmd_traffic(a) -> None:
                                             "write_scale": write_scale, "kernels": out}, indent=1) + "\n")
 
 
+def cmd_mfma(a) -> None:
+    """Per kernel: launches and mean duration from a kernel trace, HBM bytes
+    from FETCH_SIZE / WRITE_SIZE runs (FETCH_SIZE times --fetch-scale, the
+    copy calibration), matrix-core FLOP per launch and the fraction of cycles
+    the matrix cores were busy. rocprofv3 sums GRBM_GUI_ACTIVE over the 8 XCDs
+    and SQ_VALU_MFMA_BUSY_CYCLES over all SIMDs, so busy = MFMA busy cycles /
+    (GRBM_GUI_ACTIVE / 8 x SIMDs)."""
+    kt = kernel_times(a.trace)
+    fetch, write, mf = counters(a.fetch), counters(a.write), counters(a.mfma)
+    print(f"| kernel | calls | avg us | read MB (FETCH_SIZE x {a.fetch_scale:.2f}) | written MB | "
+          "MFMA FLOP (MOPS x 512) | MFMA busy % |")
+    print("|---|---:|---:|---:|---:|---:|---:|")
+    for k, (n, ns) in sorted(kt.items(), key=lambda kv: -kv[1][1])[: a.top]:
+        rd = fetch.get(k, {}).get("FETCH_SIZE", float("nan")) * 1024.0 * a.fetch_scale / 1e6
+        wr = write.get(k, {}).get("WRITE_SIZE", float("nan")) * 1024.0 / 1e6
+        m = mf.get(k, {})
+        flop = 512.0 * sum(x for c, x in m.items() if c.startswith("SQ_INSTS_VALU_MFMA_MOPS_"))
+        gui = m.get("GRBM_GUI_ACTIVE", 0.0)
+        busy = 100.0 * m.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0) / (gui / 8.0 * a.simds) if gui else float("nan")
+        print(f"| {k} | {n} | {ns / n / 1e3:.1f} | {rd:.1f} | {wr:.1f} | {flop / 1e9:.2f} G | {busy:.1f} |")
+
+
 def main(argv=None) -> None:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -250,6 +272,12 @@ def main(argv=None) -> None:
     fl.add_argument("--json")
     fl.add_argument("--workload", default="")
     fl.add_argument("--top", type=int, default=12)
+    mm = sub.add_parser("mfma")
+    for n in ("--trace", "--fetch", "--write", "--mfma"):
+        mm.add_argument(n, required=True)
+    mm.add_argument("--fetch-scale", type=float, default=2.0)
+    mm.add_argument("--simds", type=int, default=1024)
+    mm.add_argument("--top", type=int, default=16)
     f = sub.add_parser("traffic")
     for n in ("--fetch", "--write", "--copy-fetch", "--copy-write"):
         f.add_argument(n, required=True)
@@ -258,7 +286,8 @@ def main(argv=None) -> None:
     f.add_argument("--workload", default="")
     f.add_argument("--top", type=int, default=12)
     a = ap.parse_args(argv)
-    {"trace": cmd_trace, "calls": cmd_calls, "pmc": cmd_pmc, "flops": cmd_flops, "traffic": cmd_traffic}[a.cmd](a)
+    {"trace": cmd_trace, "calls": cmd_calls, "pmc": cmd_pmc, "flops": cmd_flops, "traffic": cmd_traffic,
+     "mfma": cmd_mfma}[a.cmd](a)
 
 
 if __name__ == "__main__":

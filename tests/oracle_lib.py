@@ -337,10 +337,11 @@ def e4m3_near_midpoint(x: np.ndarray, tol: float) -> np.ndarray:
 
 
 def fp8_exponent(amax: float) -> int:
-    """Power-of-two exponent e with amax / 2^e < 448 (0 for an all-zero layer)."""
+    """Power-of-two exponent e with amax / 2^e < 448 (0 for an all-zero layer),
+    limited to [-63, 63]."""
     if not (0.0 < amax < 3.0e38):
         return 0
-    return int(np.frexp(np.float32(amax) / np.float32(E4M3_MAX))[1])
+    return int(np.clip(np.frexp(np.float32(amax) / np.float32(E4M3_MAX))[1], -63, 63))
 
 
 # The FP8 kernel's register layout: byte 8 s + v of lane group g (of the 32

@@ -126,6 +126,7 @@ def test_e4m3_rounding_matches_torch():
     np.testing.assert_array_equal(ol.e4m3(x), want)
     assert ol.e4m3(np.array([1e4, -1e4, 460.0])).tolist() == [448.0, -448.0, 448.0]  # saturation
     assert ol.fp8_exponent(0.0) == 0 and ol.fp8_exponent(447.0) == 0 and ol.fp8_exponent(448.0) == 1
+    assert ol.fp8_exponent(1e-30) == -63 and ol.fp8_exponent(1e30) == 63
 
 
 def test_fp8_restatement_reduces_to_oracle():
