@@ -198,10 +198,14 @@ int alloc_path_buffers(PathBuffers& pb, std::vector<void*>& allocs, size_t n, bo
   HM_TRY(A((void**)&pb.done_L, f4));
   HM_TRY(A((void**)&pb.done_px, sizeof(int) * n));
   HM_TRY(A((void**)&pb.done_pass, sizeof(int) * n));
+  HM_TRY(A((void**)&pb.job_ctl, sizeof(long long) * kJobCtl));
   if (msnn_mode) {
     HM_TRY(A((void**)&pb.beta_short, f4));
     HM_TRY(A((void**)&pb.L_short, f4));
     HM_TRY(A((void**)&pb.sh_beta_short, f4));
+    HM_TRY(A((void**)&pb.alt.beta_short, f4));
+    HM_TRY(A((void**)&pb.alt.L_short, f4));
+    HM_TRY(A((void**)&pb.done_Ls, f4));
   }
   // traversal stack spill: levels beyond the LDS part, one column per thread
   pb.spill_stride = spill_stride;

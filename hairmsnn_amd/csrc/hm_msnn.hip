@@ -121,12 +121,18 @@ struct MsnnState {
   hipStream_t batch_stream = nullptr;  // training batches traced one batch ahead
   int batches_made = 0;                // training batches generated so far
   int call_end = 0;                    // frame index one past the current hmsnn_msnn_frames call
+  int slot_paths = 0;                      // paths each slot's path buffers hold
+  long long* job_status = nullptr;         // pinned status words of warm-up job runs
+  hipEvent_t job_ev[2] = {nullptr, nullptr};
   ~MsnnState() {
     if (ctx_stream) (void)hipStreamSynchronize(ctx_stream);
     if (batch_stream) (void)hipStreamSynchronize(batch_stream);
     if (batch_stream) (void)hipStreamDestroy(batch_stream);
     for (MsnnSlot& sl : slots) sl.release();
     for (TrainBatch& b : tb) b.release();
+    if (job_status) (void)hipHostFree(job_status);
+    for (hipEvent_t& e : job_ev)
+      if (e) (void)hipEventDestroy(e);
     if (mlp) mlp_destroy(mlp);
     dev_free(nn_output);
     for (int k = 0; k < 3; ++k) {
@@ -155,21 +161,21 @@ namespace {
 // (cuda/hair_msnn.cu:243-311). train_slot < 0: pixel pass, records of the
 // flagged pixels go to index pixel / every_nth; otherwise every slot is a
 // record.
-__global__ void k_msnn_finalize(PathBuffers pb, int n, int all_slots, int every_nth, int n_train,
-                                const float* rec_in, const uint8_t* hit_in, float* train_input, float* train_target,
-                                float4* short_color) {
+__global__ void k_msnn_finalize(const float4* Lf, const float4* Lsh, const uint32_t* flags, int n, int all_slots,
+                                int every_nth, int n_train, const float* rec_in, const uint8_t* hit_in,
+                                float* train_input, float* train_target, float4* short_color) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float4 L = pb.L[i];
+  const float4 L = Lf[i];
   V3 color = v3(L.x, L.y, L.z);
-  const bool training = all_slots || (pb.flags[i] & 1u);
+  const bool training = all_slots || (flags[i] & 1u);
   if (training) {
     const int tr = all_slots ? i : i / every_nth;
     const bool hit = hit_in[i] & 1u;
     V3 full = v3(0.f), sh = v3(0.f);
     if (hit) {
       full = color;
-      const float4 S = pb.L_short[i];
+      const float4 S = Lsh[i];
       sh = v3(S.x, S.y, S.z);
     }
     if (any_nan(full)) full = v3(0.f);
@@ -329,6 +335,9 @@ int msnn_setup(Context& c, int BETA, std::string& err) {
   // hold a frame or one such pass
   m->warm_batch = std::max(1, std::min(MsnnState::kWarmBatch, std::max(npx, m->n_train) / m->n_train));
   const size_t rec = (size_t)m->warm_batch * m->n_train;
+  m->slot_paths = (int)std::max((size_t)npx, rec);
+  HM_HIP(hipHostMalloc((void**)&m->job_status, sizeof(long long) * 4), err);
+  for (hipEvent_t& e : m->job_ev) HM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming), err);
   for (MsnnSlot& sl : m->slots) {
     HM_HIP(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking), err);
     for (hipEvent_t* e : {&sl.ev_gb, &sl.ev_done, &sl.ev_copy})
@@ -401,10 +410,10 @@ __global__ void k_sample_points(const float* pts, int num_points, uint64_t seed,
 
 // Draws for `batches` consecutive training-data passes, batch b at
 // gen_points + b * n_train (one seed per pass, in pass order).
-int sample_gen_points(Context& c, MsnnState& m, MsnnSlot& sl, int batches, std::string& err) {
+int sample_gen_points(Context& c, MsnnState& m, hipStream_t st, float* dst, int batches, std::string& err) {
   for (int b = 0; b < batches; ++b)
-    hipLaunchKernelGGL(k_sample_points, dim3((m.n_train + 255) / 256), dim3(256), 0, sl.st, c.scene_points,
-                       c.num_scene_points, m.next_u64(), m.n_train, sl.gen_points + 3 * (size_t)b * m.n_train);
+    hipLaunchKernelGGL(k_sample_points, dim3((m.n_train + 255) / 256), dim3(256), 0, st, c.scene_points,
+                       c.num_scene_points, m.next_u64(), m.n_train, dst + 3 * (size_t)b * m.n_train);
   return check_launch("msnn sample points", err);
 }
 
@@ -437,14 +446,15 @@ int train_step(Context& c, MsnnState& m, const float* input, const float* target
 // Warm-up: training-data passes toward sampled hair points, each followed by
 // one training step (render_hair_msnn.cu:633-665). The passes do not depend
 // on the network, so consecutive ones run on consecutive slots and overlap;
-// the training steps run in order on the context stream.
-int msnn_warmup(Context& c, MsnnState& m, int steps, std::string& err) {
+// the training steps run in order on the context stream. This is the
+// per-pass schedule; msnn_warmup_jobs traces the same records.
+int msnn_warmup_passes(Context& c, MsnnState& m, int steps, std::string& err) {
   for (int k = 0; k < steps;) {
     const int nb = std::min(m.warm_batch, steps - k);
     const int n = nb * m.n_train;
     MsnnSlot& sl = m.take_slot();
     HM_TRY(acquire_slot(sl, err));
-    HM_TRY(sample_gen_points(c, m, sl, nb, err));
+    HM_TRY(sample_gen_points(c, m, sl.st, sl.gen_points, nb, err));
     PassConfig cfg = msnn_pass_config(c, m);
     cfg.camera_mode = 1;
     cfg.n_slots = n;
@@ -458,8 +468,9 @@ int msnn_warmup(Context& c, MsnnState& m, int steps, std::string& err) {
     cfg.gb.hit_flags = sl.train_hit;
     cfg.gb.scene_scale = c.scene_scale;
     HM_TRY(pt_run_pass(c.view_no_caps(), sl.pb, cfg, sl.st, err));
-    hipLaunchKernelGGL(k_msnn_finalize, dim3((n + 255) / 256), dim3(256), 0, sl.st, sl.pb, n, 1, 1, n,
-                       sl.train_input, sl.train_hit, sl.train_input, sl.train_target, (float4*)nullptr);
+    hipLaunchKernelGGL(k_msnn_finalize, dim3((n + 255) / 256), dim3(256), 0, sl.st, sl.pb.L, sl.pb.L_short,
+                       sl.pb.flags, n, 1, 1, n, sl.train_input, sl.train_hit, sl.train_input, sl.train_target,
+                       (float4*)nullptr);
     HM_TRY(check_launch("msnn finalize (warm-up)", err));
     HM_HIP(hipEventRecord(sl.ev_gb, sl.st), err);
     HM_HIP(hipStreamWaitEvent(c.stream, sl.ev_gb, 0), err);
@@ -470,6 +481,101 @@ int msnn_warmup(Context& c, MsnnState& m, int steps, std::string& err) {
     k += nb;
   }
   return kOk;
+}
+
+// Warm-up steps per job run: their training-data records are traced as one
+// job-schedule run (the slot's path buffers refilled with new records as
+// paths finish), then trained on in step order while the next run traces.
+constexpr int kWarmJobSteps = 32;
+
+// The records of every warm-up step, as msnn_warmup_passes makes them (same
+// points, seeds and paths), on the job schedule: runs of up to
+// kWarmJobSteps steps alternate between two record buffers, so a run traces
+// while the previous one's steps train.
+int msnn_warmup_jobs(Context& c, MsnnState& m, int steps, std::string& err) {
+  if (steps <= 0) return kOk;
+  const int per = std::min(steps, kWarmJobSteps);
+  const size_t rec = (size_t)per * m.n_train;
+  struct Buf {
+    float* pts = nullptr;
+    float* input = nullptr;
+    float* target = nullptr;
+    uint8_t* hit = nullptr;
+    float4* L = nullptr;
+    float4* Ls = nullptr;
+    hipEvent_t ev_free = nullptr;
+    bool used = false;
+  } buf[2];
+  std::vector<void*> allocs;
+  auto release = [&]() {
+    (void)hipStreamSynchronize(c.stream);
+    for (void* p : allocs) dev_free(p);
+    for (Buf& b : buf)
+      if (b.ev_free) (void)hipEventDestroy(b.ev_free);
+  };
+  auto body = [&]() -> int {
+    const int nbuf = steps > per ? 2 : 1;
+    for (int k = 0; k < nbuf; ++k) {
+      Buf& b = buf[k];
+      auto A = [&](void** p, size_t bytes) -> int {
+        HM_TRY(dev_alloc(p, bytes, err));
+        allocs.push_back(*p);
+        return kOk;
+      };
+      HM_TRY(A((void**)&b.pts, sizeof(float) * 3 * rec));
+      HM_TRY(A((void**)&b.input, sizeof(float) * 12 * rec));
+      HM_TRY(A((void**)&b.target, sizeof(float) * 3 * rec));
+      HM_TRY(A((void**)&b.hit, rec));
+      HM_TRY(A((void**)&b.L, sizeof(float4) * rec));
+      HM_TRY(A((void**)&b.Ls, sizeof(float4) * rec));
+      HM_HIP(hipEventCreateWithFlags(&b.ev_free, hipEventDisableTiming), err);
+    }
+    const SceneView sc = c.view_no_caps();
+    for (int k = 0, r = 0; k < steps; ++r) {
+      const int nb = std::min(per, steps - k);
+      const int n = nb * m.n_train;
+      Buf& b = buf[r % 2];
+      MsnnSlot& sl = m.take_slot();
+      HM_TRY(acquire_slot(sl, err));
+      if (b.used) HM_HIP(hipStreamWaitEvent(sl.st, b.ev_free, 0), err);
+      b.used = true;
+      HM_TRY(sample_gen_points(c, m, sl.st, b.pts, nb, err));
+      PassConfig cfg = msnn_pass_config(c, m);
+      cfg.camera_mode = 1;
+      cfg.n_slots = n;
+      cfg.chunk = m.n_train;
+      cfg.grid_w = m.grid_w;
+      cfg.points = b.pts;
+      cfg.accum_id = m.gen_id;
+      m.gen_id += nb;
+      cfg.gb.enabled = 1;
+      cfg.gb.nn_input = b.input;
+      cfg.gb.hit_flags = b.hit;
+      cfg.gb.scene_scale = c.scene_scale;
+      JobRun run;
+      HM_TRY(pt_jobs_start(sc, sl.pb, cfg, std::min(n, m.slot_paths), run, sl.st, err));
+      HM_TRY(pt_jobs_drain(sc, sl.pb, cfg, run, b.L, b.Ls, m.job_status, m.job_ev, sl.st, err));
+      hipLaunchKernelGGL(k_msnn_finalize, dim3((n + 255) / 256), dim3(256), 0, sl.st, b.L, b.Ls,
+                         (const uint32_t*)nullptr, n, 1, 1, n, b.input, b.hit, b.input, b.target, (float4*)nullptr);
+      HM_TRY(check_launch("msnn finalize (warm-up)", err));
+      HM_HIP(hipEventRecord(sl.ev_gb, sl.st), err);
+      HM_HIP(hipStreamWaitEvent(c.stream, sl.ev_gb, 0), err);
+      for (int j = 0; j < nb; ++j)
+        HM_TRY(train_step(c, m, b.input + 12 * (size_t)j * m.n_train, b.target + 3 * (size_t)j * m.n_train, err,
+                          nullptr));
+      HM_HIP(hipEventRecord(b.ev_free, c.stream), err);
+      HM_HIP(hipEventRecord(sl.ev_done, c.stream), err);
+      k += nb;
+    }
+    return kOk;
+  };
+  const int rc = body();
+  release();
+  return rc;
+}
+
+int msnn_warmup(Context& c, MsnnState& m, int steps, std::string& err) {
+  return c.pt_schedule == 0 ? msnn_warmup_jobs(c, m, steps, err) : msnn_warmup_passes(c, m, steps, err);
 }
 
 struct EventTimer {
@@ -491,7 +597,12 @@ struct EventTimer {
 
 // Training paths of frames f0 .. f0 + warm_batch - 1 as one pass on stream
 // st, into batch buffer (f0 / warm_batch) % 2. The frames' index shuffles
-// (render_hair_msnn.cu:712-714) are drawn here, in frame order.
+// (render_hair_msnn.cu:712-714) are drawn here, in frame order. The batch
+// runs on the per-pass schedule on either setting of the path-tracing
+// schedule: traced on the job schedule, the frame loop measured slower
+// (64 C1 frames: 3.97 ms per frame with all records in flight, 4.15 with
+// half, 4.45 with a quarter, against 3.75), because the batch's dense
+// kernels overlap the frames' G_BUFFER passes less well.
 int gen_train_batch(Context& c, MsnnState& m, int f0, hipStream_t st, std::string& err) {
   const int B = m.warm_batch;
   TrainBatch& b = m.tb[(f0 / B) % 2];
@@ -519,8 +630,8 @@ int gen_train_batch(Context& c, MsnnState& m, int f0, hipStream_t st, std::strin
   cfg.gb.hit_flags = b.hit;
   cfg.gb.scene_scale = c.scene_scale;
   HM_TRY(pt_run_pass(c.view_no_caps(), b.pb, cfg, st, err));
-  hipLaunchKernelGGL(k_msnn_finalize, dim3((n + 255) / 256), dim3(256), 0, st, b.pb, n, 1, 1, n, b.nn_input, b.hit,
-                     b.train_input, b.train_target, b.full);
+  hipLaunchKernelGGL(k_msnn_finalize, dim3((n + 255) / 256), dim3(256), 0, st, b.pb.L, b.pb.L_short, b.pb.flags, n,
+                     1, 1, n, b.nn_input, b.hit, b.train_input, b.train_target, b.full);
   HM_TRY(check_launch("msnn finalize (training batch)", err));
   HM_HIP(hipEventRecord(b.ev_ready, st), err);
   return kOk;
@@ -573,7 +684,8 @@ int msnn_frame(Context& c, MsnnState& m, EventTimer* tm, std::string& err) {
   cfg.gb.hit_flags = sl.hit_flags;
   cfg.gb.scene_scale = c.scene_scale;
   HM_TRY(pt_run_pass(c.view_no_caps(), sl.pb, cfg, sl.st, err));
-  hipLaunchKernelGGL(k_msnn_finalize, dim3((npx + 255) / 256), dim3(256), 0, sl.st, sl.pb, npx, 0, m.every_nth, 0,
+  hipLaunchKernelGGL(k_msnn_finalize, dim3((npx + 255) / 256), dim3(256), 0, sl.st, sl.pb.L, sl.pb.L_short,
+                     sl.pb.flags, npx, 0, m.every_nth, 0,
                      sl.nn_input, sl.hit_flags, (float*)nullptr, (float*)nullptr, sl.short_color);
   HM_TRY(check_launch("msnn finalize", err));
   HM_HIP(hipStreamWaitEvent(sl.st, b.ev_ready, 0), err);

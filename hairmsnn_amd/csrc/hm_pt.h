@@ -76,7 +76,16 @@ struct PathState {
   int* pass;
   int* px;
   int* depth;
+  float4* beta_short;  // HairMSNN training paths only
+  float4* L_short;
 };
+
+// Scheduler words of the job schedule (pt_jobs_*): two copies of [0] next
+// job, [2] free path capacity, each copy in its own 64-byte line, then at
+// kJobCtlLive the number of iterations that found live paths.
+constexpr int kJobCtlHalf = 8;
+constexpr int kJobCtlLive = 2 * kJobCtlHalf;
+constexpr int kJobCtl = kJobCtlLive + kJobCtlHalf;
 
 struct PathBuffers {
   float4* ray_o;
@@ -114,8 +123,10 @@ struct PathBuffers {
   PathState alt;
   int* dest;
   float4* done_L;
+  float4* done_Ls;  // HairMSNN training paths: the finished paths' short radiance
   int* done_px;
   int* done_pass;
+  long long* job_ctl;  // kJobCtl words
   int* counts;  // [0], [1] continuation queues, [2] shadow slots, [3] shadow rays, [4] finished paths
   int* spill;   // traversal stack spill
   int spill_stride;

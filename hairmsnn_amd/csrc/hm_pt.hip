@@ -182,16 +182,15 @@ __global__ void k_camera(SceneView sc, PathBuffers pb, int accum_id, int msnn, c
   pb.flags[i] = flags;
 }
 
-// HairMSNN training pixels of several frames as one pass: slot i holds
+// HairMSNN training pixels of several frames as one pass: record i is
 // record j = i % chunk of frame accum_id + i / chunk, whose pixel is
 // j * every_nth + train_idx[i] % every_nth (the pixel k_camera flags,
 // hair_msnn.cu:206-209), traced from that pixel's seed and jitter, so the
 // paths are those the frame's own pass would trace. pix receives the pixels.
-__global__ void k_camera_pixels(SceneView sc, PathBuffers pb, int accum_id, int n, int chunk, int every_nth,
-                                const int* train_idx, int* pix) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) pb.counts[kCountStride * 0] = n;
-  if (i >= n) return;
+// The path goes to position q of pb.
+__device__ __forceinline__ void camera_train_pixel(const SceneView& sc, const PathBuffers& pb, int q, int i,
+                                                   int accum_id, int chunk, int every_nth, const int* train_idx,
+                                                   int* pix) {
   const int c = i / chunk, j = i - c * chunk;
   const int p = j * every_nth + train_idx[i] % every_nth;
   pix[i] = p;
@@ -201,39 +200,55 @@ __global__ void k_camera_pixels(SceneView sc, PathBuffers pb, int accum_id, int 
   const float sx = ((float)x + jx) / (float)sc.width;
   const float sy = ((float)y + jy) / (float)sc.height;
   const V3 d = normalize(sc.cam_d00 + sx * sc.cam_du + sy * sc.cam_dv);
-  pb.ray_o[i] = make_float4(sc.cam_pos.x, sc.cam_pos.y, sc.cam_pos.z, 0.f);
-  pb.ray_d[i] = make_float4(d.x, d.y, d.z, 0.f);
-  pb.beta[i] = make_float4(1.f, 1.f, 1.f, 0.f);
-  pb.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-  pb.beta_short[i] = make_float4(1.f, 1.f, 1.f, 0.f);
-  pb.L_short[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-  pb.rng[i] = rng;
-  pb.queue[0][i] = i;
-  pb.flags[i] = 1u;
+  pb.ray_o[q] = make_float4(sc.cam_pos.x, sc.cam_pos.y, sc.cam_pos.z, 0.f);
+  pb.ray_d[q] = make_float4(d.x, d.y, d.z, 0.f);
+  pb.beta[q] = make_float4(1.f, 1.f, 1.f, 0.f);
+  pb.L[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+  pb.beta_short[q] = make_float4(1.f, 1.f, 1.f, 0.f);
+  pb.L_short[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+  pb.rng[q] = rng;
 }
 
-// Training-data rays: camera position toward sampled hair points; every slot
-// is a training path (cuda/hair_msnn.cu:222-233). The slots hold consecutive
-// chunks of `chunk` paths, chunk c being the pass of index accum_id + c; a
-// path's seed uses its launch index within its chunk's grid_w-wide 2D launch,
-// so a batch of chunks reproduces the separate passes exactly.
-__global__ void k_camera_points(SceneView sc, PathBuffers pb, int accum_id, int n, int chunk, int grid_w,
-                                const float* pts) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) pb.counts[kCountStride * 0] = n;
-  if (i >= n) return;
+// Training-data ray i: camera position toward sampled hair point i, a
+// training path (cuda/hair_msnn.cu:222-233). Records come in consecutive
+// chunks of `chunk`, chunk c being the pass of index accum_id + c; a path's
+// seed uses its launch index within its chunk's grid_w-wide 2D launch, so a
+// batch of chunks reproduces the separate passes exactly. The path goes to
+// position q of pb.
+__device__ __forceinline__ void camera_train_point(const SceneView& sc, const PathBuffers& pb, int q, int i,
+                                                   int accum_id, int chunk, int grid_w, const float* pts) {
   const int c = i / chunk, j = i - c * chunk;
   const int x = j % grid_w, y = j / grid_w;
   const uint32_t rng = path_seed((uint32_t)(accum_id + c + 10007), (uint32_t)x, (uint32_t)y, (uint32_t)grid_w);
   const V3 p = v3(pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]);
   const V3 d = normalize(p - sc.cam_pos);
-  pb.ray_o[i] = make_float4(sc.cam_pos.x, sc.cam_pos.y, sc.cam_pos.z, 0.f);
-  pb.ray_d[i] = make_float4(d.x, d.y, d.z, 0.f);
-  pb.beta[i] = make_float4(1.f, 1.f, 1.f, 0.f);
-  pb.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-  pb.beta_short[i] = make_float4(1.f, 1.f, 1.f, 0.f);
-  pb.L_short[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-  pb.rng[i] = rng;
+  pb.ray_o[q] = make_float4(sc.cam_pos.x, sc.cam_pos.y, sc.cam_pos.z, 0.f);
+  pb.ray_d[q] = make_float4(d.x, d.y, d.z, 0.f);
+  pb.beta[q] = make_float4(1.f, 1.f, 1.f, 0.f);
+  pb.L[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+  pb.beta_short[q] = make_float4(1.f, 1.f, 1.f, 0.f);
+  pb.L_short[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+  pb.rng[q] = rng;
+}
+
+// Slot i = training record i (camera_train_pixel), every slot a training path.
+__global__ void k_camera_pixels(SceneView sc, PathBuffers pb, int accum_id, int n, int chunk, int every_nth,
+                                const int* train_idx, int* pix) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) pb.counts[kCountStride * 0] = n;
+  if (i >= n) return;
+  camera_train_pixel(sc, pb, i, i, accum_id, chunk, every_nth, train_idx, pix);
+  pb.queue[0][i] = i;
+  pb.flags[i] = 1u;
+}
+
+// Slot i = training-data ray i (camera_train_point), every slot a training path.
+__global__ void k_camera_points(SceneView sc, PathBuffers pb, int accum_id, int n, int chunk, int grid_w,
+                                const float* pts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) pb.counts[kCountStride * 0] = n;
+  if (i >= n) return;
+  camera_train_point(sc, pb, i, i, accum_id, chunk, grid_w, pts);
   pb.queue[0][i] = i;
   pb.flags[i] = 1u;
 }
@@ -501,13 +516,17 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade(SceneView sc,
   }
 }
 
-// Compacted streaming vertex (plain path tracing only): path i of wavefront
-// pb is shaded as k_shade shades a slot of a plain path-tracing pass (the
-// same operations in the same order), then a continuing path is written densely into wavefront nb and a
-// finished one into the finished list; pb.dest[i] records where it went, for
-// k_shadow_resolve_dense. Shadow records stay at index i.
-__global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade_dense(SceneView sc, PathBuffers pb,
-                                                                          PathBuffers nb, ShadeParams sp) {
+// Compacted wavefront vertex: path i of wavefront pb is shaded as k_shade
+// shades a slot (the same operations in the same order), then a continuing
+// path is written densely into wavefront nb and a finished one into the
+// finished list; pb.dest[i] records where it went, for
+// k_shadow_resolve_dense. Shadow records stay at index i. kMsnn: every path
+// is a HairMSNN training path (full and short throughput, the short one cut
+// after vertex trunc_beta); the primary hit's network input goes to gb at
+// the path's job (slot_px), and the short radiance travels with the path.
+template <bool kMsnn>
+__device__ __forceinline__ void shade_dense(const SceneView& sc, const PathBuffers& pb, const PathBuffers& nb,
+                                            const ShadeParams& sp, const GBufferOut& gb) {
   const int nxt = sp.cur ^ 1;
   const int n = pb.counts[kCountStride * (sp.cur)];
   const int stride = gridDim.x * blockDim.x;
@@ -516,6 +535,7 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade_dense(SceneVi
     bool cont = false, shadow = false, ray1 = false, ray2 = false;
     const bool live = i < n;
     float4 L = make_float4(0.f, 0.f, 0.f, 0.f), no4 = L, wi4 = L, beta4 = L;
+    float4 Ls = L, bs4 = L;
     uint32_t rng = 0;
     int depth = 0;
     if (live) {
@@ -524,27 +544,46 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade_dense(SceneVi
       const float4 o4 = pb.ray_o[i], d4 = pb.ray_d[i];
       const V3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
       L = pb.L[i];
+      if constexpr (kMsnn) Ls = pb.L_short[i];
       if (prim == kPrimMiss) {
         if (depth == 0) {
           // camera ray escaped: the miss program's radiance (optix_common.cuh:604-616)
           V3 le = v3(0.f);
           if (sc.env.enabled) le = env_radiance(sc.env, d);
           L = make_float4(le.x, le.y, le.z, 0.f);
+          if constexpr (kMsnn) gbuffer_write_miss(gb, pb.slot_px[i], d);
         }
       } else {
         Hit h;
         setup_hit(sc, o, d, pb.hit_t[i], prim, pb.hit_u[i], h);
+        if constexpr (kMsnn)
+          if (depth == 0) gbuffer_write_hit(gb, sc, pb.slot_px[i], h, prim);
         rng = pb.rng[i];
         const float4 b4 = pb.beta[i];
         V3 beta = v3(b4.x, b4.y, b4.z);
-        const V3 beta_dl = beta;
+        V3 beta_s = v3(0.f);
+        if constexpr (kMsnn) {
+          const float4 s4 = pb.beta_short[i];
+          beta_s = v3(s4.x, s4.y, s4.z);
+        }
+        const V3 beta_dl = beta, beta_s_dl = beta_s;
         ShadowSink sink(pb, i);
-        // Russian roulette (optix_common.cuh:398-405) from the second vertex on
+        // Russian roulette (optix_common.cuh:398-405; hair_msnn.cu:110-128,
+        // 171-181) from the second vertex on
         auto roulette = [&](float eps) {
           const float qq = fmaxf(0.05f, 1.f - luminance(beta));
-          if (eps < qq) return false;
-          beta = beta / (1.f - qq);
-          return true;
+          if constexpr (kMsnn) {
+            const float qs = fmaxf(0.05f, 1.f - luminance(beta_s));
+            if (eps < qs || depth > sp.trunc_beta) beta_s = v3(0.f);
+            if (eps < qq) return false;
+            beta = beta / (1.f - qq);
+            if (!is_zero(beta_s)) beta_s = beta_s / (1.f - qs);
+            return true;
+          } else {
+            if (eps < qq) return false;
+            beta = beta / (1.f - qq);
+            return true;
+          }
         };
         const Continuation c = shade_vertex(sc, h, rng, depth == 0 ? sp.dl_zero != 0 : depth >= sp.dl_from,
                                             depth >= 1, depth + 1 <= sp.max_vertex, roulette,
@@ -553,12 +592,17 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade_dense(SceneVi
           shadow = true;
           ray1 = sink.has1;
           ray2 = sink.has2;
+          if constexpr (kMsnn) pb.sh_beta_short[i] = make_float4(beta_s_dl.x, beta_s_dl.y, beta_s_dl.z, 0.f);
         }
         if (c.cont) {
           beta = beta * c.mul;
           no4 = make_float4(c.o.x, c.o.y, c.o.z, 0.f);
           wi4 = make_float4(c.wi.x, c.wi.y, c.wi.z, 0.f);
           beta4 = make_float4(beta.x, beta.y, beta.z, 0.f);
+          if constexpr (kMsnn) {
+            beta_s = beta_s * c.mul;
+            bs4 = make_float4(beta_s.x, beta_s.y, beta_s.z, 0.f);
+          }
           cont = true;
         }
       }
@@ -577,12 +621,17 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade_dense(SceneVi
       nb.slot_pass[p] = pb.slot_pass[i];
       nb.slot_px[p] = pb.slot_px[i];
       nb.slot_depth[p] = depth + 1;
+      if constexpr (kMsnn) {
+        nb.beta_short[p] = bs4;
+        nb.L_short[p] = Ls;
+      }
       pb.dest[i] = p;
     }
     if (done) {
       pb.done_L[dp] = L;
       pb.done_px[dp] = pb.slot_px[i];
       pb.done_pass[dp] = pb.slot_pass[i];
+      if constexpr (kMsnn) pb.done_Ls[dp] = Ls;
       pb.dest[i] = ~dp;
     }
     if (shadow) pb.shadow_queue[a.shadow] = i;
@@ -591,9 +640,22 @@ __global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade_dense(SceneVi
   }
 }
 
+__global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade_dense(SceneView sc, PathBuffers pb,
+                                                                          PathBuffers nb, ShadeParams sp) {
+  shade_dense<false>(sc, pb, nb, sp, GBufferOut{});
+}
+
+__global__ void __launch_bounds__(256, HM_SHADE_MIN_WAVES) k_shade_dense_msnn(SceneView sc, PathBuffers pb,
+                                                                               PathBuffers nb, ShadeParams sp,
+                                                                               GBufferOut gb) {
+  shade_dense<true>(sc, pb, nb, sp, gb);
+}
+
 // The resolve of k_shadow_resolve for a compacted wavefront: the visible
-// contributions go to the path where k_shade_dense moved it.
-__global__ void k_shadow_resolve_dense(PathBuffers pb, PathBuffers nb) {
+// contributions go to the path where k_shade_dense moved it (kMsnn: the
+// short radiance too).
+template <bool kMsnn>
+__device__ __forceinline__ void shadow_resolve_dense(const PathBuffers& pb, const PathBuffers& nb) {
   const int n = pb.counts[kCountStride * 2];
   const int stride = gridDim.x * blockDim.x;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
@@ -616,8 +678,21 @@ __global__ void k_shadow_resolve_dense(PathBuffers pb, PathBuffers nb) {
     L.y += b.y * sum.y;
     L.z += b.z * sum.z;
     *Lp = L;
+    if constexpr (kMsnn) {
+      const float4 bs = pb.sh_beta_short[slot];
+      float4* Lsp = dst >= 0 ? nb.L_short + dst : pb.done_Ls + ~dst;
+      float4 Ls = *Lsp;
+      Ls.x += bs.x * sum.x;
+      Ls.y += bs.y * sum.y;
+      Ls.z += bs.z * sum.z;
+      *Lsp = Ls;
+    }
   }
 }
+
+__global__ void k_shadow_resolve_dense(PathBuffers pb, PathBuffers nb) { shadow_resolve_dense<false>(pb, nb); }
+
+__global__ void k_shadow_resolve_dense_msnn(PathBuffers pb, PathBuffers nb) { shadow_resolve_dense<true>(pb, nb); }
 
 // Any-hit test of each pending shadow ray, one ray per lane; writes the
 // visibility byte of (slot, which).
@@ -933,7 +1008,10 @@ static PathBuffers other_state(const PathBuffers& pb) {
   v.slot_pass = pb.alt.pass;
   v.slot_px = pb.alt.px;
   v.slot_depth = pb.alt.depth;
-  v.alt = PathState{pb.ray_o, pb.ray_d, pb.beta, pb.L, pb.rng, pb.slot_pass, pb.slot_px, pb.slot_depth};
+  v.beta_short = pb.alt.beta_short;
+  v.L_short = pb.alt.L_short;
+  v.alt = PathState{pb.ray_o,     pb.ray_d,   pb.beta,       pb.L,         pb.rng,
+                    pb.slot_pass, pb.slot_px, pb.slot_depth, pb.beta_short, pb.L_short};
   return v;
 }
 
@@ -1071,6 +1149,194 @@ int pt_run_stream(const SceneView& sc, PathBuffers* const* pbs, const hipStream_
     }
   }
   return kOk;
+}
+
+// ---- job schedule -----------------------------------------------------------
+// Records of a camera_mode 1 / 2 pass on a compacted wavefront (the path
+// kernels are those of the streaming schedule with HairMSNN training state):
+// each iteration extends, shades, traces and resolves the current
+// wavefront, moves the finished paths' radiance to their records' outputs
+// (k_jobs_finish) and appends new records' camera rays after the continuing
+// paths, up to the capacity (k_jobs_next). ctl holds two copies of the
+// scheduler words; step m reads copy m % 2 and writes the other.
+
+struct JobCamera {
+  int mode;  // 1: rays toward sampled hair points, 2: training pixels
+  int n_jobs, accum_id, chunk, grid_w, every_nth;
+  const float* points;
+  const int* train_idx;
+  int* pix;
+};
+
+__global__ void k_jobs_init(long long* ctl, int* counts, int capacity) {
+  const int i = threadIdx.x;
+  if (i < kJobCtl) ctl[i] = i == 2 ? capacity : 0;
+  if (i <= kDoneCount) counts[kCountStride * i] = 0;
+}
+
+// Finished paths' radiance to their records; the free capacity for the
+// next issue; one more live iteration if the wavefront just shaded (cur)
+// held paths.
+__global__ void k_jobs_finish(PathBuffers pb, long long* S, long long* ctl, int cur, int nxt, int capacity,
+                              float4* out_L, float4* out_Ls) {
+  const int n = pb.counts[kCountStride * kDoneCount];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S[2] = capacity - pb.counts[kCountStride * nxt];
+    if (pb.counts[kCountStride * cur] > 0) ctl[kJobCtlLive] += 1;
+  }
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int job = pb.done_px[i];
+    out_L[job] = pb.done_L[i];
+    out_Ls[job] = pb.done_Ls[i];
+  }
+}
+
+// Issues min(free capacity, records left) new records into wavefront nb
+// after its continuing paths, and clears the counters the next iteration
+// appends to. Every thread derives the same count from S.
+__global__ void k_jobs_next(SceneView sc, PathBuffers nb, const long long* S, long long* T, int cur, int nxt,
+                            JobCamera jc) {
+  const long long job0 = S[0];
+  const int n = (int)max(0ll, min(S[2], (long long)jc.n_jobs - job0));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    T[0] = job0 + n;
+    nb.counts[kCountStride * cur] = 0;
+    nb.counts[kCountStride * 2] = 0;
+    nb.counts[kCountStride * 3] = 0;
+    nb.counts[kCountStride * kDoneCount] = 0;
+  }
+  const int stride = gridDim.x * blockDim.x;
+  const int nloop = (n + stride - 1) / stride * stride;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
+    const bool go = i < n;
+    const int q = wave_append(&nb.counts[kCountStride * nxt], go);
+    if (go) {
+      const int job = (int)(job0 + i);
+      if (jc.mode == 1)
+        camera_train_point(sc, nb, q, job, jc.accum_id, jc.chunk, jc.grid_w, jc.points);
+      else
+        camera_train_pixel(sc, nb, q, job, jc.accum_id, jc.chunk, jc.every_nth, jc.train_idx, jc.pix);
+      nb.slot_pass[q] = 0;
+      nb.slot_px[q] = job;
+      nb.slot_depth[q] = 0;
+    }
+  }
+}
+
+static JobCamera job_camera(const PassConfig& cfg) {
+  JobCamera jc{};
+  jc.mode = cfg.camera_mode;
+  jc.n_jobs = cfg.n_slots;
+  jc.accum_id = cfg.accum_id + cfg.seed_offset;
+  jc.chunk = cfg.chunk > 0 ? cfg.chunk : cfg.n_slots;
+  jc.grid_w = cfg.grid_w;
+  jc.every_nth = cfg.every_nth;
+  jc.points = cfg.points;
+  jc.train_idx = cfg.train_idx;
+  jc.pix = cfg.pix;
+  return jc;
+}
+
+static int job_blocks(int paths, int cap_blocks, int block) {
+  return std::min(cap_blocks, std::max(1, (paths + block - 1) / block));
+}
+
+int pt_jobs_start(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, int capacity, JobRun& run,
+                  hipStream_t stream, std::string& err) {
+  if ((cfg.camera_mode != 1 && cfg.camera_mode != 2) || capacity < 1 || !pb.done_Ls || !pb.alt.L_short) {
+    err = "job schedule: training-path records and HairMSNN path buffers required";
+    return kEInval;
+  }
+  run = JobRun{};
+  run.capacity = capacity;
+  hipLaunchKernelGGL(k_jobs_init, dim3(1), dim3(64), 0, stream, pb.job_ctl, pb.counts, capacity);
+  // first issue into wavefront 0 (state copy 0)
+  hipLaunchKernelGGL(k_jobs_next, dim3(job_blocks(capacity, cfg.shade_blocks, 256)), dim3(256), 0, stream, sc, pb,
+                     pb.job_ctl, pb.job_ctl + kJobCtlHalf, 1, 0, job_camera(cfg));
+  run.calls = 1;
+  run.cur = 0;
+  return check_launch("job schedule start", err);
+}
+
+int pt_jobs_iterate(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, JobRun& run, int iters,
+                    float4* out_L, float4* out_Ls, hipStream_t stream, std::string& err) {
+  const PathBuffers views[2] = {pb, other_state(pb)};
+  const int C = run.capacity;
+  const int ext_blocks = job_blocks(C, cfg.trace_blocks, kTraceBlock);
+  const int sh_blocks = job_blocks(2 * C, cfg.trace_blocks, kTraceBlock);
+  const int shade_blocks = job_blocks(C, cfg.shade_blocks, 256);
+  const JobCamera jc = job_camera(cfg);
+  ShadeParams sp;
+  sp.depth = 0;
+  sp.dl_zero = cfg.dl_zero;
+  sp.dl_from = cfg.dl_from;
+  sp.max_vertex = cfg.max_vertex;
+  sp.trunc_beta = cfg.trunc_beta;
+  sp.msnn = 1;
+  long long* c = pb.job_ctl;
+  for (int k = 0; k < iters; ++k) {
+    const int cur = run.cur, nxt = cur ^ 1;
+    const PathBuffers& A = views[cur];
+    const PathBuffers& B = views[nxt];
+    sp.cur = cur;
+    if (sc.curves)
+      hipLaunchKernelGGL((k_extend<false, true, true>), dim3(ext_blocks), dim3(kTraceBlock), 0, stream, sc, A, cur,
+                         (unsigned long long*)nullptr);
+    else
+      hipLaunchKernelGGL((k_extend<false, true>), dim3(ext_blocks), dim3(kTraceBlock), 0, stream, sc, A, cur,
+                         (unsigned long long*)nullptr);
+    hipLaunchKernelGGL(k_shade_dense_msnn, dim3(shade_blocks), dim3(256), 0, stream, sc, A, B, sp, cfg.gb);
+    if (sc.curves)
+      hipLaunchKernelGGL((k_shadow<false, true>), dim3(sh_blocks), dim3(kTraceBlock), 0, stream, sc, A,
+                         (unsigned long long*)nullptr);
+    else
+      hipLaunchKernelGGL((k_shadow<false>), dim3(sh_blocks), dim3(kTraceBlock), 0, stream, sc, A,
+                         (unsigned long long*)nullptr);
+    hipLaunchKernelGGL(k_shadow_resolve_dense_msnn, dim3(shade_blocks), dim3(256), 0, stream, A, B);
+    long long* S = c + kJobCtlHalf * (run.calls & 1);
+    long long* T = c + kJobCtlHalf * ((run.calls + 1) & 1);
+    hipLaunchKernelGGL(k_jobs_finish, dim3(shade_blocks), dim3(256), 0, stream, A, S, c, cur, nxt, C, out_L, out_Ls);
+    hipLaunchKernelGGL(k_jobs_next, dim3(shade_blocks), dim3(256), 0, stream, sc, B, S, T, cur, nxt, jc);
+    ++run.calls;
+    ++run.iters;
+    run.cur = nxt;
+  }
+  return check_launch("job schedule iteration", err);
+}
+
+int pt_jobs_status(const PathBuffers& pb, const JobRun& run, long long* host, hipEvent_t ev, hipStream_t stream,
+                   std::string& err) {
+  // the wavefront count is an int: it lands in the low half of host[0]
+  // (little-endian), whose high half is cleared here
+  host[0] = 0;
+  HM_HIP(hipMemcpyAsync(host, pb.counts + kCountStride * run.cur, sizeof(int), hipMemcpyDeviceToHost, stream), err);
+  HM_HIP(hipMemcpyAsync(host + 1, pb.job_ctl + kJobCtlLive, sizeof(long long), hipMemcpyDeviceToHost, stream), err);
+  HM_HIP(hipEventRecord(ev, stream), err);
+  return kOk;
+}
+
+int pt_jobs_drain(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, JobRun& run, float4* out_L,
+                  float4* out_Ls, long long* host, const hipEvent_t* ev, hipStream_t stream, std::string& err,
+                  int* live_iters) {
+  constexpr int kBatch = 8;
+  const int bound = pt_jobs_bound(cfg, run.capacity) + 2 * kBatch;
+  for (int b = 0;; ++b) {
+    HM_TRY(pt_jobs_iterate(sc, pb, cfg, run, kBatch, out_L, out_Ls, stream, err));
+    HM_TRY(pt_jobs_status(pb, run, host + 2 * (b & 1), ev[b & 1], stream, err));
+    if (b > 0) {
+      const long long* h = host + 2 * ((b - 1) & 1);
+      HM_HIP(hipEventSynchronize(ev[(b - 1) & 1]), err);
+      if ((int)h[0] == 0) {  // drained one batch ago; this batch found nothing
+        if (live_iters) *live_iters = (int)h[1];
+        return kOk;
+      }
+    }
+    if (run.iters > bound) {
+      err = "job schedule did not drain";
+      return kEDevice;
+    }
+  }
 }
 
 int pt_accumulate(const float4* L, float4* accum, float4* average, int n, int accum_id, hipStream_t stream,

@@ -2,6 +2,7 @@
 // used by the HairMSNN G_BUFFER pass.
 #pragma once
 
+#include <algorithm>
 #include <string>
 
 #include "hm_internal.h"
@@ -122,6 +123,42 @@ struct PassConfig {
 // radiance in pb.L (and pb.L_short for training pixels).
 int pt_run_pass(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, hipStream_t stream,
                 std::string& err);
+
+// Job schedule: the records of a camera_mode 1 or 2 pass (cfg.n_slots
+// HairMSNN training paths) traced on a compacted wavefront that is refilled
+// with new records, in record order, up to `capacity` paths in flight. The
+// caller launches iterations (pt_jobs_iterate) and learns from
+// pt_jobs_status whether the wavefront has drained; pt_jobs_drain does both
+// until it has. Each path's operations are those of pt_run_pass, so the
+// records are the same bits; out_L / out_Ls receive each record's full and
+// short radiance, cfg.gb its network input at the primary hit.
+struct JobRun {
+  int capacity = 0;
+  int cur = 0;           // current wavefront (state copy and count index)
+  long long calls = 0;   // scheduler steps so far
+  int iters = 0;         // iterations launched
+};
+int pt_jobs_start(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, int capacity, JobRun& run,
+                  hipStream_t stream, std::string& err);
+int pt_jobs_iterate(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, JobRun& run, int iters,
+                    float4* out_L, float4* out_Ls, hipStream_t stream, std::string& err);
+// host[0]: paths in the current wavefront, host[1]: iterations that found
+// live paths (pinned memory); ev is recorded after the copies.
+int pt_jobs_status(const PathBuffers& pb, const JobRun& run, long long* host, hipEvent_t ev, hipStream_t stream,
+                   std::string& err);
+// Iterates until the wavefront is empty, reading the status one batch of
+// iterations late (host: 4 pinned words, ev: two events).
+// live_iters (optional) receives the number of iterations that found live
+// paths over the whole run.
+int pt_jobs_drain(const SceneView& sc, PathBuffers& pb, const PassConfig& cfg, JobRun& run, float4* out_L,
+                  float4* out_Ls, long long* host, const hipEvent_t* ev, hipStream_t stream, std::string& err,
+                  int* live_iters = nullptr);
+// Iterations after which the wavefront is certainly empty: every max_vertex
+// + 1 iterations retire all paths that were in flight at their start.
+inline int pt_jobs_bound(const PassConfig& cfg, int capacity) {
+  const long long rounds = ((long long)cfg.n_slots + capacity - 1) / std::max(1, capacity) + 1;
+  return (int)std::min<long long>(1 << 30, rounds * (std::max(0, cfg.max_vertex) + 1));
+}
 
 // Passes the streaming schedule keeps in flight at most (its radiance ring).
 constexpr int kStreamRing = 16;

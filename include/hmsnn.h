@@ -188,7 +188,11 @@ int hmsnn_set_pass_concurrency(hmsnn_ctx* ctx, int passes_in_flight);
  * drain. hmsnn_set_pass_concurrency sets the number of stream groups the
  * pixels are dealt to. HMSNN_PT_PASSES: one wavefront per pass, passes in
  * flight per hmsnn_set_pass_concurrency. Both accumulate each pixel's passes
- * in pass order: the images are equal.
+ * in pass order: the images are equal. The setting also selects how
+ * hmsnn_msnn_begin traces the warm-up's training records: HMSNN_PT_STREAM
+ * on a compacted wavefront refilled with new records as paths finish,
+ * HMSNN_PT_PASSES one wavefront per pass; the records, network and images
+ * are equal.
  * No reference counterpart (the reference launches one megakernel per pass). */
 typedef enum hmsnn_pt_schedule { HMSNN_PT_STREAM = 0, HMSNN_PT_PASSES = 1 } hmsnn_pt_schedule;
 int hmsnn_set_pt_schedule(hmsnn_ctx* ctx, int schedule);

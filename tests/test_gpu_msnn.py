@@ -162,6 +162,32 @@ def test_training_records_match_oracle(gpu):
     assert tgt[both].mean() >= -1e-6 and ot.mean() >= -1e-6
 
 
+@pytest.mark.parametrize("warmup", [6, 40])
+def test_job_schedule_equals_per_pass_schedule(gpu, warmup):
+    """Warm-up records traced on the job schedule (the default: a compacted
+    wavefront refilled with new records as paths finish) are the per-pass
+    schedule's bit for bit, and so are the network they train and the
+    frames' records and images. 40 warm-up steps of 1024 records are two
+    runs (32 + 8 steps) of more records than the slot's 4096 paths, so the
+    wavefront is refilled and the second run traces while the first one's
+    steps train."""
+    cfg = cfg_small(records=1024, warmup=warmup)
+    out = []
+    for sched in ("stream", "passes"):
+        with gpu.Scene(json_text=cfg) as sc:
+            sc.set_pt_schedule(sched)
+            sc.msnn_begin(BETA=2)
+            sc.msnn_frames(10)
+            got = {k: sc.msnn_buffer(k) for k in ("train_input", "train_target", "train_pixel", "nn_output",
+                                                   "gb_color")}
+            got["final"] = sc.msnn_image(0)
+            got["short"] = sc.msnn_image(1)
+            out.append(got)
+    assert np.isfinite(out[0]["final"]).all() and (out[0]["train_target"] != 0).any()
+    for k in out[0]:
+        np.testing.assert_array_equal(out[0][k], out[1][k], err_msg=k)
+
+
 def test_split_calls_equal_one_call(gpu):
     """Frames rendered in several calls, with a per-strand absorption upload
     between them (equal to the global value), give the image of one call bit
