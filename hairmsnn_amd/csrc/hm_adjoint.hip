@@ -169,7 +169,15 @@ HM_D V3 adjoint_path(const SceneView& sc, const AdjArgs& a, TraceStack& st, int 
   return L;
 }
 
-__global__ void __launch_bounds__(kTraceBlock) k_adjoint(SceneView sc, AdjArgs a) {
+// Waves per SIMD the replay kernel is compiled for. Unbounded it takes 256
+// VGPRs + 19 AGPRs (one wave); at 512x512 C1, 16 spp, the replay ran at
+// 27.7 M paths/s with 1 wave, 43.9 with 2, 48.3 with 3 and 46.7 with 4 (the
+// registers beyond the budget spill to scratch: 400, 704 and 864 bytes per
+// lane).
+#ifndef HM_ADJ_MIN_WAVES
+#define HM_ADJ_MIN_WAVES 3
+#endif
+__global__ void __launch_bounds__(kTraceBlock, HM_ADJ_MIN_WAVES) k_adjoint(SceneView sc, AdjArgs a) {
   __shared__ int lds_stack[kStackLds * kTraceBlock];
   TraceStack st;
   st.lds = lds_stack + threadIdx.x;
