@@ -353,8 +353,9 @@ def bsdf_rates(hm, n: int = 1 << 22, reps: int = 20) -> dict:
 
 
 def mlp_train_rate(hm, n: int = 1 << 16, reps: int = 20) -> dict:
-    """One FP16 training step (forward, backward, weight-gradient GEMM, Adam)
-    on n seeded records: 124,032 FLOP per record (3 x the forward's 41,344)."""
+    """One FP16 training step (forward, backward, weight gradient, Adam) on n
+    seeded records: 124,032 FLOP per record (3 x the forward's 41,344), wall
+    clock over reps steps queued back to back after one warm step."""
     import ctypes as C
     lib = hm._lib.load()
     rng = np.random.default_rng(6)
@@ -371,9 +372,10 @@ def mlp_train_rate(hm, n: int = 1 << 16, reps: int = 20) -> dict:
     loss = C.c_float()
     hm._lib.check(lib.hmsnn_mlp_train_step_dev(h, din.ptr, dt_.ptr, n, 0, C.c_float(1e-3), C.byref(loss), None))
     hm._lib.check(lib.hmsnn_dev_synchronize())
+    # steps queued back to back (no loss read-back, which would wait for each)
     t0 = time.perf_counter()
     for _ in range(reps):
-        hm._lib.check(lib.hmsnn_mlp_train_step_dev(h, din.ptr, dt_.ptr, n, 0, C.c_float(1e-3), C.byref(loss), None))
+        hm._lib.check(lib.hmsnn_mlp_train_step_dev(h, din.ptr, dt_.ptr, n, 0, C.c_float(1e-3), None, None))
     hm._lib.check(lib.hmsnn_dev_synchronize())
     dt = (time.perf_counter() - t0) / reps
     lib.hmsnn_mlp_destroy(h)

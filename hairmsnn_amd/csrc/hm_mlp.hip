@@ -13,10 +13,10 @@
 // fragments are pre-permuted on the device after each update and staged in
 // LDS by every block.
 //
-// Training splits into: forward + backward per 32-query tile (activations
-// and pre-activation gradients written feature-major in FP16), a weight-
-// gradient GEMM over K = queries on the MFMA cores, and a reduce + Adam
-// kernel in FP32.
+// Training is two launches: forward, backward and the weight gradient in
+// one kernel (activations stay in registers; each layer's transpose for the
+// weight gradient goes through LDS), then a reduce + Adam kernel in FP32
+// that also writes the new fragments.
 //
 // FP8 inference (E4M3, v_mfma_scale_f32_32x32x64_f8f6f4): one instruction
 // contracts a whole 64-wide layer. Lane (g, n) holds the 32 activations of
@@ -526,37 +526,86 @@ __global__ void __launch_bounds__(kInferBlock) k_mlp_infer_fp8(const i32x8* frag
   }
 }
 
-// Feature-major activation store for layer l: buf[(l * 64 + f) * npad + q].
-// The lane-dependent part (4 g rows and the query) is folded into one
-// pointer; the per-feature offsets are wave-uniform.
+// ---- training ---------------------------------------------------------------
+//
+// One step is two launches. k_mlp_train works on block tiles of 128 records
+// (four waves of 32). Each wave runs the forward pass of its 32 records with
+// the layer inputs H_0..H_5 kept in registers, the loss and the output
+// gradient, then the backward pass layer by layer (dH_l = W_l^T dZ_l on the
+// matrix cores, as in the forward pass). At each layer the block stages
+// dZ_l and H_l feature-major in LDS ([feature][record], the transpose the
+// weight gradient needs: its contraction runs over records), and wave
+// (mb, nb) adds the 32 x 32 tile (rows 32 mb.., columns 32 nb..) of
+// dW_l = dZ_l H_l^T over the block's 128 records to accumulators it keeps
+// for all the block's tiles. Each block writes its dW sum and its loss once.
+// k_mlp_reduce_adam sums the blocks' partials in a fixed order, applies Adam
+// to the FP32 master weights and writes the permuted 16-bit fragments.
+constexpr int kTrainTile = 128;          // records per block tile
+constexpr int kStagePitch = 136;         // 16-bit values per staged feature row (128 records + 8: bank spread)
+constexpr int kTrainMaxBlocks = 256;     // one block per CU of an MI355X; fixed, so sums do not depend on the device
+constexpr size_t kTrainPartialFloats = (size_t)6 * 4096;  // dW partial of one block
+constexpr size_t kTrainSmem = sizeof(f16x8) * kFragAll + 2 * sizeof(_Float16) * 64 * kStagePitch;
+
+__host__ __device__ constexpr int train_blocks(int n) {
+  return (n + kTrainTile - 1) / kTrainTile < kTrainMaxBlocks ? (n + kTrainTile - 1) / kTrainTile : kTrainMaxBlocks;
+}
+
+// Stores a wave's 32 records x 64 features (operand layout) into a staged
+// [feature][record] array: lane (g, col) writes record qt, features
+// feat_of(s, g, v) = feat_of(s, 0, v) + 4 g.
 template <class T>
-__device__ __forceinline__ void store_fm(T* buf, int l, int npad, int q, int g, const typename MfmaTraits<T>::vec B[4]) {
-  T* p = buf + (size_t)(4 * g) * npad + q;
+__device__ __forceinline__ void stage_tile(T* S, int qt, int g, const typename MfmaTraits<T>::vec X[4]) {
+  T* p = S + 4 * g * kStagePitch + qt;
 #pragma unroll
   for (int s = 0; s < 4; ++s)
 #pragma unroll
-    for (int v = 0; v < 8; ++v) p[(size_t)(l * 64 + feat_of(s, 0, v)) * npad] = B[s][v];
+    for (int v = 0; v < 8; ++v) p[feat_of(s, 0, v) * kStagePitch] = X[s][v];
 }
 
-// Forward + backward for 32-query tiles. Writes H_l (inputs of layer l,
-// l = 0..5) and dZ_l (gradients of layer l's pre-activations, scaled by
-// grad_scale) feature-major, and adds the batch loss.
+// acc += dZ[32 mb + r][records] x H[32 nb + c][records]^T over the 128 staged
+// records (K steps of 16). Lane (g, col) reads 8 consecutive records of row
+// col of each operand; rows with a_valid false contribute zeros.
 template <class T>
-__global__ void __launch_bounds__(kMlpBlock) k_mlp_fwd_bwd(const T* frag, const float* in, const float* target, int n,
-                                                           int npad, int loss, float grad_scale, T* H, T* dZ,
-                                                           float* loss_sum) {
+__device__ __forceinline__ f32x16 dw_tile(const T* SdZ, const T* SH, int mb, int nb, int col, int g, bool a_valid,
+                                          f32x16 acc) {
+  typedef typename MfmaTraits<T>::vec V;
+  const V* pa = reinterpret_cast<const V*>(SdZ + (32 * mb + col) * kStagePitch + 8 * g);
+  const V* pb = reinterpret_cast<const V*>(SH + (32 * nb + col) * kStagePitch + 8 * g);
+#pragma unroll
+  for (int k = 0; k < kTrainTile / 16; ++k) {
+    V a = pa[2 * k];
+    if (!a_valid) a = V{};
+    acc = MfmaTraits<T>::mfma(a, pb[2 * k], acc);
+  }
+  return acc;
+}
+
+// partial[(block * 6 + l) * 4096 + o * 64 + i] = sum over the block's records
+// of dZ_l[o] H_l[i] (dZ scaled by grad_scale); loss_part[block] = the block's
+// summed loss.
+template <class T>
+__global__ void __launch_bounds__(kMlpBlock) k_mlp_train(const T* frag, const float* in, const float* target, int n,
+                                                         int loss, float grad_scale, float* partial,
+                                                         float* loss_part) {
   typedef MfmaTraits<T> M;
   typedef typename M::vec V;
   extern __shared__ char smem[];
   V* lds = reinterpret_cast<V*>(smem);
+  T* sdz = reinterpret_cast<T*>(smem + sizeof(V) * kFragAll);
+  T* sh = sdz + 64 * kStagePitch;
+  __shared__ float wave_loss[kMlpBlock / 64];
   stage_fragments<T>(frag, lds, kFragAll);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 5, col = lane & 31;
+  const int mb = wave >> 1, nb = wave & 1;  // this wave's dW tile
+  const int qt = 32 * wave + col;           // record within the block tile
+  f32x16 dw[6];
+#pragma unroll
+  for (int l = 0; l < 6; ++l) dw[l] = f32x16{};
   float lsum = 0.f;
-  // one 32-query tile per wave (the grid covers the padded batch)
-  {
-    const int tile = blockIdx.x * (kMlpBlock / 64) + wave;
-    const int q = tile * 32 + col;
+  const int tiles = (n + kTrainTile - 1) / kTrainTile;
+  for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
+    const int q = t * kTrainTile + qt;
     const bool valid = q < n;
     float rec[12];
 #pragma unroll
@@ -565,38 +614,33 @@ __global__ void __launch_bounds__(kMlpBlock) k_mlp_fwd_bwd(const T* frag, const 
       rec[5] = 1.f;
       rec[8] = 1.f;
     }
-    const Encoded e = encode_wave(rec, g);
-    V B[4];
+    V H[6][4];
+    {
+      const Encoded e = encode_wave(rec, g);
 #pragma unroll
-    for (int s = 0; s < 4; ++s) B[s] = encoded_operand<T>(e, s, g);
-    store_fm<T>(H, 0, npad, q, g, B);
-    // forward; keep one ReLU bit per held feature for the backward pass
-    uint32_t mask[5];
+      for (int s = 0; s < 4; ++s) H[0][s] = encoded_operand<T>(e, s, g);
+    }
 #pragma unroll
     for (int l = 0; l < 5; ++l) {
-      hidden_layer<T>(lds, l, lane, B);
-      uint32_t m = 0;
 #pragma unroll
-      for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int v = 0; v < 8; ++v) m |= (M::back(B[s][v]) > 0.f ? 1u : 0u) << (8 * s + v);
-      mask[l] = m;
-      store_fm<T>(H, l + 1, npad, q, g, B);
+      for (int s = 0; s < 4; ++s) H[l + 1][s] = H[l][s];
+      hidden_layer<T>(lds, l, lane, H[l + 1]);
     }
-    const f32x16 z = output_layer<T>(lds, lane, B);
+    f32x16 z = output_layer<T>(lds, lane, H[5]);
+    pin_wave(z);
     // ---- loss and output gradient (lanes of group 0 own rows 0..2) ----
     float dz[3] = {0.f, 0.f, 0.f};
     if (g == 0 && valid) {
-      float y[3], t[3];
+      float y[3], tg[3];
 #pragma unroll
       for (int o = 0; o < 3; ++o) {
         y[o] = expf(z[o]);
-        t[o] = target[(size_t)q * 3 + o];
+        tg[o] = target[(size_t)q * 3 + o];
       }
       const float lum = 0.212671f * y[0] + 0.715160f * y[1] + 0.072169f * y[2];
 #pragma unroll
       for (int o = 0; o < 3; ++o) {
-        const float d = y[o] - t[o];
+        const float d = y[o] - tg[o];
         float sc = 1.f;
         if (loss == HMSNN_LOSS_RELATIVE_L2_LUMINANCE) sc = 1.f / (lum * lum + 0.01f);
         else if (loss == HMSNN_LOSS_RELATIVE_L2) sc = 1.f / (y[o] * y[o] + 0.01f);
@@ -605,12 +649,7 @@ __global__ void __launch_bounds__(kMlpBlock) k_mlp_fwd_bwd(const T* frag, const 
         dz[o] = fminf(fmaxf(gval, -60000.f), 60000.f);
       }
     }
-    // dZ_5: rows 0..2 from the lanes of group 0
-    if (g == 0) {
-#pragma unroll
-      for (int o = 0; o < 3; ++o) dZ[((size_t)5 * 64 + o) * npad + q] = M::cvt(dz[o]);
-    }
-    // ---- backward: dH_5 = W5^T dz ----
+    // ---- output layer: dH_5 = W5^T dz; dW_5 from the staged dZ_5 rows 0..2 ----
     V Bdz;
 #pragma unroll
     for (int v = 0; v < 8; ++v) Bdz[v] = M::cvt(v < 3 ? dz[v] : 0.f);  // group 1 holds zeros
@@ -618,6 +657,15 @@ __global__ void __launch_bounds__(kMlpBlock) k_mlp_fwd_bwd(const T* frag, const 
 #pragma unroll
     for (int ib = 0; ib < 2; ++ib)
       acc[ib] = M::mfma(lds[kFragFwd + kFragOut + kFragBwd + ib * 64 + lane], Bdz, f32x16{});
+    if (g == 0) {
+#pragma unroll
+      for (int o = 0; o < 3; ++o) sdz[o * kStagePitch + qt] = Bdz[o];
+    }
+    stage_tile<T>(sh, qt, g, H[5]);
+    __syncthreads();
+    if (mb == 0) dw[5] = dw_tile<T>(sdz, sh, 0, nb, col, g, col < 3, dw[5]);
+    __syncthreads();
+    // ---- hidden layers, last to first ----
 #pragma unroll
     for (int l = 4; l >= 0; --l) {
       // dZ_l = dH_{l+1} * [H_{l+1} > 0]
@@ -626,72 +674,106 @@ __global__ void __launch_bounds__(kMlpBlock) k_mlp_fwd_bwd(const T* frag, const 
       for (int s = 0; s < 4; ++s)
 #pragma unroll
         for (int v = 0; v < 8; ++v)
-          D[s][v] = M::cvt(((mask[l] >> (8 * s + v)) & 1u) ? acc[s >> 1][8 * (s & 1) + v] : 0.f);
-      store_fm<T>(dZ, l, npad, q, g, D);
-      if (l == 0) break;
-      // dH_l = W_l^T dZ_l
-      acc[0] = f32x16{};
-      acc[1] = f32x16{};
+          D[s][v] = M::back(H[l + 1][s][v]) > 0.f ? M::cvt(acc[s >> 1][8 * (s & 1) + v]) : M::cvt(0.f);
+      stage_tile<T>(sdz, qt, g, D);
+      stage_tile<T>(sh, qt, g, H[l]);
+      if (l > 0) {
+        // dH_l = W_l^T dZ_l
+        acc[0] = f32x16{};
+        acc[1] = f32x16{};
 #pragma unroll
-      for (int s = 0; s < 4; ++s)
+        for (int s = 0; s < 4; ++s)
 #pragma unroll
-        for (int ib = 0; ib < 2; ++ib)
-          acc[ib] = M::mfma(lds[kFragFwd + kFragOut + ((l * 2 + ib) * 4 + s) * 64 + lane], D[s], acc[ib]);
+          for (int ib = 0; ib < 2; ++ib)
+            acc[ib] = M::mfma(lds[kFragFwd + kFragOut + ((l * 2 + ib) * 4 + s) * 64 + lane], D[s], acc[ib]);
+      }
+      __syncthreads();
+      dw[l] = dw_tile<T>(sdz, sh, mb, nb, col, g, true, dw[l]);
+      __syncthreads();
     }
   }
-  // wave-reduce the loss, one atomic per wave
-  for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
-  if (lane == 0) atomicAdd(loss_sum, lsum);
-}
-
-// dW_l partial over a chunk of queries: 64 x 64 on four waves (2 x 2 blocks of
-// 32 x 32), K = queries in steps of 16. partial[(chunk * 6 + l) * 4096 + o * 64 + i].
-template <class T>
-__global__ void __launch_bounds__(kMlpBlock) k_mlp_dw(const T* H, const T* dZ, int npad, int chunk, float* partial) {
-  typedef MfmaTraits<T> M;
-  typedef typename M::vec V;
-  const int l = blockIdx.y, c = blockIdx.x;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int mb = wave >> 1, nb = wave & 1;
-  const int g = lane >> 5, r = lane & 31;
-  if (l == 5 && mb == 1) return;  // output layer: 3 rows
-  const int q0 = c * chunk, q1 = min(q0 + chunk, npad);
-  const T* A = dZ + ((size_t)l * 64 + 32 * mb + r) * npad;
-  const T* Bp = H + ((size_t)l * 64 + 32 * nb + r) * npad;
-  f32x16 acc = f32x16{};
-  for (int q = q0; q < q1; q += 16) {
-    const V a = *reinterpret_cast<const V*>(A + q + 8 * g);
-    const V b = *reinterpret_cast<const V*>(Bp + q + 8 * g);
-    acc = M::mfma(a, b, acc);
-  }
-  float* out = partial + ((size_t)c * 6 + l) * 4096;
+  // ---- the block's partial sums ----
+  float* out = partial + (size_t)blockIdx.x * kTrainPartialFloats;
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const int i = 32 * mb + 8 * (k / 4) + 4 * g + (k % 4);  // dW row (output feature)
-    const int j = 32 * nb + r;                              // dW column (input feature)
-    out[i * 64 + j] = acc[k];
+  for (int l = 0; l < 6; ++l) {
+    if (l == 5 && mb == 1) continue;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int o = 32 * mb + 8 * (k / 4) + 4 * g + (k % 4);  // dW row (output feature)
+      const int i = 32 * nb + col;                            // dW column (input feature)
+      if (l < 5 || o < 3) out[l * 4096 + o * 64 + i] = dw[l][k];
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
+  if (lane == 0) wave_loss[wave] = lsum;
+  __syncthreads();
+  if (threadIdx.x == 0) loss_part[blockIdx.x] = ((wave_loss[0] + wave_loss[1]) + wave_loss[2]) + wave_loss[3];
+}
+
+// Where weight k = (layer l, row o, column i) of the flat vector sits in the
+// forward and backward fragments (the inverse of k_mlp_permute's indexing);
+// the elements k_mlp_permute sets to zero never hold a weight.
+__device__ __forceinline__ void frag_slots(int k, int* fwd, int* bwd) {
+  const int l = k / 4096, o = (k % 4096) / 64, i = k % 64;
+  if (l < 5) {
+    // forward: row o = 32 ob + m, column i = feat_of(s, g, v)
+    const int s = 2 * (i >> 5) + ((i & 31) >> 4), v = 4 * ((i >> 3) & 1) + (i & 3), gg = (i >> 2) & 1;
+    *fwd = ((((l * 2 + (o >> 5)) * 4 + s) * 64) + 32 * gg + (o & 31)) * 8 + v;
+    // backward (W^T): row feat_of(s, g, v) = o, column 32 ib + m = i
+    const int sb = 2 * (o >> 5) + ((o & 31) >> 4), vb = 4 * ((o >> 3) & 1) + (o & 3), gb = (o >> 2) & 1;
+    *bwd = (kFragFwd + kFragOut + ((l * 2 + (i >> 5)) * 4 + sb) * 64 + 32 * gb + (i & 31)) * 8 + vb;
+  } else {
+    const int s = 2 * (i >> 5) + ((i & 31) >> 4), v = 4 * ((i >> 3) & 1) + (i & 3), gg = (i >> 2) & 1;
+    *fwd = (kFragFwd + s * 64 + 32 * gg + o) * 8 + v;
+    *bwd = (kFragFwd + kFragOut + kFragBwd + (i >> 5) * 64 + (i & 31)) * 8 + o;
   }
 }
 
-// Sums the chunk partials into the gradient of the flat weight vector, then
-// optionally applies one Adam step to the FP32 master weights.
-__global__ void k_mlp_reduce_adam(const float* partial, int nchunks, float inv_scale, float* grad, float* w,
-                                  float* m, float* v, int adam, float lr, float b1, float b2, float eps, float l2,
-                                  float bc1, float bc2) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= kMlpWeightCount) return;
-  const int l = k / 4096, rem = k % 4096;
+// 64 weights per block: wave p sums the partials of blocks p, p + 4, ...;
+// the four sums are added in LDS. Then the gradient (optionally stored) and,
+// with adam, one Adam step on the FP32 master weights and the new 16-bit
+// fragment values (frag, when given). Block 0's first wave also sums the
+// blocks' losses into loss_sum.
+template <class T>
+__global__ void __launch_bounds__(256) k_mlp_reduce_adam(const float* partial, const float* loss_part, int nchunks,
+                                                         float inv_scale, float* grad, float* loss_sum, float* w,
+                                                         float* m, float* v, int adam, float lr, float b1, float b2,
+                                                         float eps, float l2, float bc1, float bc2, T* frag) {
+  __shared__ float red[4][64];
+  const int j = threadIdx.x & 63, p = threadIdx.x >> 6;
+  const int k = blockIdx.x * 64 + j;
+  const bool ok = k < kMlpWeightCount;
   float s = 0.f;
-  for (int c = 0; c < nchunks; ++c) s += partial[((size_t)c * 6 + l) * 4096 + rem];
-  float gk = s * inv_scale;
-  if (grad) grad[k] = gk;
+  if (ok) {
+    const float* src = partial + k;
+#pragma unroll 4
+    for (int c = p; c < nchunks; c += 4) s += src[(size_t)c * kTrainPartialFloats];
+  }
+  red[p][j] = s;
+  if (blockIdx.x == 0 && p == 0 && loss_sum) {
+    float ls = 0.f;
+    for (int c = j; c < nchunks; c += 64) ls += loss_part[c];
+    for (int off = 32; off > 0; off >>= 1) ls += __shfl_xor(ls, off);
+    if (j == 0) *loss_sum = ls;
+  }
+  __syncthreads();
+  if (p != 0 || !ok) return;
+  const float gk0 = (((red[0][j] + red[1][j]) + red[2][j]) + red[3][j]) * inv_scale;
+  if (grad) grad[k] = gk0;
   if (!adam) return;
-  gk += l2 * w[k];
+  const float gk = gk0 + l2 * w[k];
   const float mk = b1 * m[k] + (1.f - b1) * gk;
   const float vk = b2 * v[k] + (1.f - b2) * gk * gk;
   m[k] = mk;
   v[k] = vk;
-  w[k] -= lr * (mk / bc1) / (sqrtf(vk / bc2) + eps);
+  const float wk = w[k] - lr * (mk / bc1) / (sqrtf(vk / bc2) + eps);
+  w[k] = wk;
+  if (frag) {
+    int fi, bi;
+    frag_slots(k, &fi, &bi);
+    frag[fi] = MfmaTraits<T>::cvt(wk);
+    frag[bi] = MfmaTraits<T>::cvt(wk);
+  }
 }
 
 }  // namespace
@@ -703,10 +785,8 @@ struct MlpState {
   float* v = nullptr;
   void* frag = nullptr;
   float* loss_sum = nullptr;
-  void* H = nullptr;
-  void* dZ = nullptr;
-  float* partial = nullptr;
-  int cap = 0;  // padded batch capacity of H / dZ
+  float* partial = nullptr;  // per-block dW partials, then per-block losses
+  int cap = 0;               // blocks the partial buffer holds
   int step = 0;
   hipStream_t stream = nullptr;
   // FP8 inference state
@@ -725,8 +805,6 @@ struct MlpState {
     dev_free(v);
     dev_free(frag);
     dev_free(loss_sum);
-    dev_free(H);
-    dev_free(dZ);
     dev_free(partial);
   }
   size_t esize() const { return 2; }
@@ -857,63 +935,30 @@ int mlp_forward(MlpState& s, const float* in, int n, float* out, hipStream_t st,
   return check_launch("mlp infer", err);
 }
 
-// Clears n 32-bit words (a kernel, not hipMemsetAsync, so a caller can
-// capture the launch in a graph).
-__global__ void k_zero_words(uint32_t* p, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
-}
+// Floats of the partial buffer for a step over n records: each block's dW
+// sum, then each block's loss.
+static size_t train_partial_floats(int n) { return (kTrainPartialFloats + 1) * (size_t)train_blocks(n); }
 
-static int zero_words(void* p, size_t bytes, hipStream_t st, std::string& err) {
-  const size_t n = bytes / 4;
-  if (n == 0) return kOk;
-  const int blocks = (int)std::min<size_t>(1024, (n + 255) / 256);
-  hipLaunchKernelGGL(k_zero_words, dim3(blocks), dim3(256), 0, st, (uint32_t*)p, n);
-  return check_launch("mlp clear", err);
-}
+static int mlp_step_on(MlpState& s, float* partial, const float* in, const float* target, int n, int loss,
+                       float* grad_out, float* loss_out, float* loss_dev, bool adam, float lr, hipStream_t st,
+                       std::string& err);
 
-constexpr int kTrainChunk = 2048;  // records per dW partial
-
-// Scratch of one training step over npad padded records: activations H and
-// pre-activation gradients dZ (6 x 64 x npad 16-bit values each) and the dW
-// partial sums of kTrainChunk-record chunks.
-static size_t train_scratch_bytes(int npad, size_t esize, size_t* h_bytes, size_t* p_bytes) {
-  const int nchunks = (npad + kTrainChunk - 1) / kTrainChunk;
-  *h_bytes = esize * 6 * 64 * (size_t)npad;
-  *p_bytes = sizeof(float) * 6 * 4096 * (size_t)nchunks;
-  return 2 * *h_bytes + *p_bytes;
-}
-
-static int mlp_step_on(MlpState& s, void* H, void* dZ, float* partial, bool kernel_clears, const float* in,
-                       const float* target, int n, int loss, float* grad_out, float* loss_out, float* loss_dev,
-                       bool adam, float lr, hipStream_t st, std::string& err);
-
-// Forward + backward + weight-gradient GEMM + reduce (+ Adam when lr > 0).
+// Forward + backward + weight gradient, reduce (+ Adam when adam is set).
 int mlp_loss_grad_step(MlpState& s, const float* in, const float* target, int n, int loss, float* grad_out,
                        float* loss_out, bool adam, float lr, hipStream_t st, std::string& err) {
   if (n <= 0) {
     err = "empty training batch";
     return kEInval;
   }
-  // padded to whole 128-record blocks: every wave of k_mlp_fwd_bwd's last
-  // block writes its 32-record tile into the feature-major arrays
-  const int npad = (n + 127) / 128 * 128;
-  const int nchunks = (npad + kTrainChunk - 1) / kTrainChunk;
-  if (npad > s.cap) {
-    dev_free(s.H);
-    dev_free(s.dZ);
+  const int blocks = train_blocks(n);
+  if (blocks > s.cap) {
     dev_free(s.partial);
-    s.H = s.dZ = nullptr;
     s.partial = nullptr;
     s.cap = 0;
-    HM_TRY(dev_alloc(&s.H, s.esize() * 6 * 64 * (size_t)npad, err));
-    HM_TRY(dev_alloc(&s.dZ, s.esize() * 6 * 64 * (size_t)npad, err));
-    HM_TRY(dev_alloc((void**)&s.partial, sizeof(float) * 6 * 4096 * (size_t)nchunks, err));
-    // rows 3..63 of dZ_5 are never written: clear once
-    HM_HIP(hipMemsetAsync(s.dZ, 0, s.esize() * 6 * 64 * (size_t)npad, st), err);
-    s.cap = npad;
+    HM_TRY(dev_alloc((void**)&s.partial, sizeof(float) * train_partial_floats(n), err));
+    s.cap = blocks;
   }
-  return mlp_step_on(s, s.H, s.dZ, s.partial, false, in, target, n, loss, grad_out, loss_out, nullptr, adam, lr, st,
-                     err);
+  return mlp_step_on(s, s.partial, in, target, n, loss, grad_out, loss_out, nullptr, adam, lr, st, err);
 }
 
 // The batch loss into device memory: the summed loss over 3 n outputs.
@@ -921,51 +966,21 @@ __global__ void k_loss_mean(const float* loss_sum, int n, float* out) {
   if (threadIdx.x == 0) *out = *loss_sum / (3.f * (float)n);
 }
 
-static int mlp_step_on(MlpState& s, void* H, void* dZ, float* partial, bool kernel_clears, const float* in,
-                       const float* target, int n, int loss, float* grad_out, float* loss_out, float* loss_dev,
-                       bool adam, float lr, hipStream_t st, std::string& err) {
-  // padded to whole 128-record blocks: every wave of k_mlp_fwd_bwd's last
-  // block writes its 32-record tile into the feature-major arrays
-  const int npad = (n + 127) / 128 * 128;
-  const int chunk = kTrainChunk;
-  const int nchunks = (npad + chunk - 1) / chunk;
-  // the layouts below index with the current npad: clear dZ_5 rows 3..63
-  void* dz5 = (char*)dZ + s.esize() * ((size_t)5 * 64 + 3) * npad;
-  const size_t dz5_bytes = s.esize() * 61 * (size_t)npad;
-  if (kernel_clears) {
-    HM_TRY(zero_words(dz5, dz5_bytes, st, err));
-    HM_TRY(zero_words(s.loss_sum, sizeof(float), st, err));
-  } else {
-    HM_HIP(hipMemsetAsync(dz5, 0, dz5_bytes, st), err);
-    HM_HIP(hipMemsetAsync(s.loss_sum, 0, sizeof(float), st), err);
-  }
-  const float grad_scale = 1.f / 64.f;
-  const size_t smem = sizeof(f16x8) * kFragAll;  // 88 KB of weight fragments
+template <class T>
+static int launch_train(MlpState& s, float* partial, const float* in, const float* target, int n, int loss,
+                        float grad_scale, float* grad_out, bool adam, float lr, hipStream_t st, std::string& err) {
   static bool attr_set = false;
   if (!attr_set) {
-    HM_HIP(hipFuncSetAttribute((const void*)k_mlp_fwd_bwd<_Float16>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)smem),
-           err);
-    HM_HIP(hipFuncSetAttribute((const void*)k_mlp_fwd_bwd<__bf16>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)smem),
+    HM_HIP(hipFuncSetAttribute((const void*)k_mlp_train<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kTrainSmem),
            err);
     attr_set = true;
   }
-  const int blocks = npad / 128;
-  if (s.bf16()) {
-    hipLaunchKernelGGL(k_mlp_fwd_bwd<__bf16>, dim3(blocks), dim3(kMlpBlock), smem, st, (const __bf16*)s.frag, in,
-                       target, n, npad, loss, grad_scale, (__bf16*)H, (__bf16*)dZ, s.loss_sum);
-    HM_TRY(check_launch("mlp fwd/bwd", err));
-    hipLaunchKernelGGL(k_mlp_dw<__bf16>, dim3(nchunks, 6), dim3(kMlpBlock), 0, st, (const __bf16*)H,
-                       (const __bf16*)dZ, npad, chunk, partial);
-  } else {
-    hipLaunchKernelGGL(k_mlp_fwd_bwd<_Float16>, dim3(blocks), dim3(kMlpBlock), smem, st, (const _Float16*)s.frag,
-                       in, target, n, npad, loss, grad_scale, (_Float16*)H, (_Float16*)dZ, s.loss_sum);
-    HM_TRY(check_launch("mlp fwd/bwd", err));
-    hipLaunchKernelGGL(k_mlp_dw<_Float16>, dim3(nchunks, 6), dim3(kMlpBlock), 0, st, (const _Float16*)H,
-                       (const _Float16*)dZ, npad, chunk, partial);
-  }
-  HM_TRY(check_launch("mlp dW", err));
+  const int blocks = train_blocks(n);
+  float* loss_part = partial + kTrainPartialFloats * (size_t)blocks;
+  hipLaunchKernelGGL(k_mlp_train<T>, dim3(blocks), dim3(kMlpBlock), kTrainSmem, st, (const T*)s.frag, in, target, n,
+                     loss, grad_scale, partial, loss_part);
+  HM_TRY(check_launch("mlp train", err));
   const float inv_scale = 1.f / (grad_scale * 3.f * (float)n);
   float bc1 = 1.f, bc2 = 1.f;
   if (adam) {
@@ -973,10 +988,31 @@ static int mlp_step_on(MlpState& s, void* H, void* dZ, float* partial, bool kern
     bc1 = 1.f - std::pow(0.9f, (float)s.step);
     bc2 = 1.f - std::pow(0.99f, (float)s.step);
   }
-  hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((kMlpWeightCount + 255) / 256), dim3(256), 0, st, partial, nchunks,
-                     inv_scale, grad_out, s.w, s.m, s.v, adam ? 1 : 0, lr, 0.9f, 0.99f, 1e-15f, 1e-6f, bc1, bc2);
-  HM_TRY(check_launch("mlp reduce/adam", err));
-  if (adam) HM_TRY(mlp_refresh(s, st, err));
+  hipLaunchKernelGGL(k_mlp_reduce_adam<T>, dim3((kMlpWeightCount + 63) / 64), dim3(256), 0, st, partial, loss_part,
+                     blocks, inv_scale, grad_out, s.loss_sum, s.w, s.m, s.v, adam ? 1 : 0, lr, 0.9f, 0.99f, 1e-15f,
+                     1e-6f, bc1, bc2, adam ? (T*)s.frag : (T*)nullptr);
+  return check_launch("mlp reduce/adam", err);
+}
+
+static int mlp_step_on(MlpState& s, float* partial, const float* in, const float* target, int n, int loss,
+                       float* grad_out, float* loss_out, float* loss_dev, bool adam, float lr, hipStream_t st,
+                       std::string& err) {
+  // output gradients are scaled up by 64 (and the sums back down) so that
+  // small gradients stay normal numbers in 16 bits
+  const float grad_scale = 1.f / 64.f;
+  if (s.bf16())
+    HM_TRY(launch_train<__bf16>(s, partial, in, target, n, loss, grad_scale, grad_out, adam, lr, st, err));
+  else
+    HM_TRY(launch_train<_Float16>(s, partial, in, target, n, loss, grad_scale, grad_out, adam, lr, st, err));
+  if (adam && s.fp8()) {
+    // the BF16 fragments were written by the Adam kernel; the FP8 ones need
+    // the new per-layer weight scales first
+    hipLaunchKernelGGL(k_mlp_w_scales, dim3(6), dim3(256), 0, st, s.w, s.scales);
+    hipLaunchKernelGGL(k_mlp_permute_fp8, dim3((kFrag8Total + 127) / 128), dim3(128), 0, st, s.w, s.scales,
+                       (i32x8*)s.frag8);
+    HM_TRY(check_launch("mlp permute fp8", err));
+    s.calib_dirty = true;
+  }
   if (loss_dev) {
     hipLaunchKernelGGL(k_loss_mean, dim3(1), dim3(64), 0, st, s.loss_sum, n, loss_dev);
     HM_TRY(check_launch("mlp loss", err));
@@ -992,17 +1028,14 @@ static int mlp_step_on(MlpState& s, void* H, void* dZ, float* partial, bool kern
 
 // ---- a training step that allocates nothing --------------------------------
 // Callers that cannot allocate device memory (code under graph capture, tight
-// serving loops) get the training step's scratch (activations, pre-activation
-// gradients, dW partial sums) from this file-scope device array, sized for
-// batches of up to kNoAllocMaxRecords records (26 MB, held by every GPU the
-// library runs on until the process ends). Every call on a GPU uses that
+// serving loops) get the training step's scratch (each block's dW partial
+// sums and loss) from this file-scope device array, sized for batches of up to
+// kNoAllocMaxRecords records (128 blocks x 96 KB = 12.6 MB, held by every GPU
+// the library runs on until the process ends). Every call on a GPU uses that
 // GPU's copy and leaves its last contents, so calls that overlap on one GPU
-// race; each call clears the parts that must start at zero with a kernel.
+// race. Nothing in it needs clearing: every block writes all of its partials.
 constexpr int kNoAllocMaxRecords = 16384;
-constexpr size_t kNoAllocHBytes = (size_t)2 * 6 * 64 * kNoAllocMaxRecords;  // 16-bit values
-constexpr size_t kNoAllocPartialBytes = sizeof(float) * 6 * 4096 * (kNoAllocMaxRecords / kTrainChunk);
-constexpr size_t kNoAllocScratchBytes = 2 * kNoAllocHBytes + kNoAllocPartialBytes;
-static_assert(kNoAllocHBytes % 256 == 0, "scratch parts stay 256-byte aligned");
+constexpr size_t kNoAllocScratchBytes = sizeof(float) * (kTrainPartialFloats + 1) * train_blocks(kNoAllocMaxRecords);
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 // The device address of `scratch` on the current device, looked up once per
@@ -1042,9 +1075,7 @@ static int mlp_train_step_noalloc(MlpState& s, const float* in, const float* tar
   }
   void* base = nullptr;
   HM_TRY(scratch_address(&base, err));
-  unsigned char* b = (unsigned char*)base;
-  return mlp_step_on(s, b, b + kNoAllocHBytes, (float*)(b + 2 * kNoAllocHBytes), true, in, target, n, loss,
-                     nullptr, nullptr, loss_dev, true, lr, st, err);
+  return mlp_step_on(s, (float*)base, in, target, n, loss, nullptr, nullptr, loss_dev, true, lr, st, err);
 }
 
 // Encoded features (FP32) for tests.

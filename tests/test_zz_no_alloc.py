@@ -44,8 +44,8 @@ def test_noalloc_train_step_matches_allocating_step(gpu):
             gpu._lib.check(lib.hmsnn_mlp_get_weights(b, C.c_void_p(wb.ctypes.data), wb.size))
             print(f"step {step} (n = {n}): loss {la.value:.6f} / {lb:.6f}, weights moved "
                   f"{np.abs(wa - w).max():.3e}")
-            # the batch loss is a sum of per-wave partials added with float
-            # atomics in whatever order the waves finish: equal to rounding
+            # the same kernels sum the batch loss in a fixed order; the
+            # allocating step divides it on the host, this one on the device
             assert lb == pytest.approx(la.value, rel=1e-5)
             np.testing.assert_array_equal(wb, wa)
             assert not np.array_equal(wa, w)
