@@ -528,51 +528,86 @@ __global__ void __launch_bounds__(kInferBlock) k_mlp_infer_fp8(const i32x8* frag
 
 // ---- training ---------------------------------------------------------------
 //
-// One step is two launches. k_mlp_train works on block tiles of 128 records
-// (four waves of 32). Each wave runs the forward pass of its 32 records with
-// the layer inputs H_0..H_5 kept in registers, the loss and the output
-// gradient, then the backward pass layer by layer (dH_l = W_l^T dZ_l on the
-// matrix cores, as in the forward pass). At each layer the block stages
-// dZ_l and H_l feature-major in LDS ([feature][record], the transpose the
-// weight gradient needs: its contraction runs over records), and wave
-// (mb, nb) adds the 32 x 32 tile (rows 32 mb.., columns 32 nb..) of
-// dW_l = dZ_l H_l^T over the block's 128 records to accumulators it keeps
-// for all the block's tiles. Each block writes its dW sum and its loss once.
-// k_mlp_reduce_adam sums the blocks' partials in a fixed order, applies Adam
-// to the FP32 master weights and writes the permuted 16-bit fragments.
-constexpr int kTrainTile = 128;          // records per block tile
-constexpr int kStagePitch = 136;         // 16-bit values per staged feature row (128 records + 8: bank spread)
-constexpr int kTrainMaxBlocks = 256;     // one block per CU of an MI355X; fixed, so sums do not depend on the device
+// One step is two launches. k_mlp_train works on block tiles of 32 x WAVES
+// records. Each wave runs the forward pass of its 32 records with the layer
+// inputs H_0..H_5 kept in registers, the loss and the output gradient, then
+// the backward pass layer by layer (dH_l = W_l^T dZ_l on the matrix cores, as
+// in the forward pass). At each layer the block stages dZ_l and H_l
+// feature-major in LDS ([feature][record], the transpose the weight gradient
+// needs: its contraction runs over records), and the 32 x 32 tiles of
+// dW_l = dZ_l H_l^T over the block's records are added to accumulators that
+// stay in registers for all the block's tiles. Waves form groups of four;
+// with G = WAVES / 4 groups, group j owns layers 6 j / G .. 6 (j + 1) / G - 1
+// and wave (mb, nb) of a group the tile of rows 32 mb.., columns 32 nb...
+// Each block writes its dW sum and its loss once. k_mlp_reduce_adam sums the
+// blocks' partials in a fixed order, applies Adam to the FP32 master weights
+// and writes the permuted 16-bit fragments.
+//
+// FP16 runs eight waves (two per SIMD: 256 records per tile, 48 dW
+// accumulators per wave); BF16, whose conversions need more registers, four
+// (one per SIMD, 96 accumulators).
+constexpr int kTrainMaxBlocks = 256;  // one block per CU of an MI355X; fixed, so sums do not depend on the device
 constexpr size_t kTrainPartialFloats = (size_t)6 * 4096;  // dW partial of one block
-constexpr size_t kTrainSmem = sizeof(f16x8) * kFragAll + 2 * sizeof(_Float16) * 64 * kStagePitch;
 
-__host__ __device__ constexpr int train_blocks(int n) {
-  return (n + kTrainTile - 1) / kTrainTile < kTrainMaxBlocks ? (n + kTrainTile - 1) / kTrainTile : kTrainMaxBlocks;
+template <int WAVES>
+struct TrainShape {
+  static constexpr int threads = 64 * WAVES;
+  static constexpr int tile = 32 * WAVES;   // records per block tile
+  static constexpr int pitch = tile + 8;    // 16-bit values per staged feature row (+8: bank spread)
+  static constexpr int groups = WAVES / 4;
+  static constexpr int layers = 6 / groups;  // dW layers per group
+  static constexpr size_t smem = sizeof(f16x8) * kFragAll + 2 * sizeof(_Float16) * 64 * pitch;
+  static_assert(smem + 64 <= 160 * 1024, "weight fragments and staging fit the LDS of one CU");
+};
+template <class T>
+struct TrainWaves {
+  static constexpr int value = 4;
+};
+template <>
+struct TrainWaves<_Float16> {
+  static constexpr int value = 8;
+};
+
+__host__ __device__ constexpr int train_blocks(int n, int tile) {
+  return (n + tile - 1) / tile < kTrainMaxBlocks ? (n + tile - 1) / tile : kTrainMaxBlocks;
 }
 
 // Stores a wave's 32 records x 64 features (operand layout) into a staged
-// [feature][record] array: lane (g, col) writes record qt, features
-// feat_of(s, g, v) = feat_of(s, 0, v) + 4 g.
-template <class T>
+// [feature][record] array. Lane (g, col) holds record qt, features
+// feat_of(s, g, v) = feat_of(s, 0, v) + 4 g, two consecutive features per
+// 32-bit word. Lane pairs (qt even, qt + 1) swap words so that each lane
+// writes one word of two consecutive records of one feature: the even lane
+// the first feature of the pair, the odd lane the second. With a row pitch of
+// 4 mod 32 words the 64 lanes' words fall on every LDS bank exactly twice.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+template <class T, int PITCH>
 __device__ __forceinline__ void stage_tile(T* S, int qt, int g, const typename MfmaTraits<T>::vec X[4]) {
-  T* p = S + 4 * g * kStagePitch + qt;
+  static_assert(PITCH % 64 == 8, "staged rows are 4 mod 32 words apart");
+  const bool odd = qt & 1;
+  const unsigned sel = odd ? 0x03020706u : 0x05040100u;  // odd: (partner hi, own hi); even: (own lo, partner lo)
+  unsigned* base = reinterpret_cast<unsigned*>(S + (4 * g + (odd ? 1 : 0)) * PITCH + (qt & ~1));
 #pragma unroll
-  for (int s = 0; s < 4; ++s)
+  for (int s = 0; s < 4; ++s) {
+    const u32x4 w = __builtin_bit_cast(u32x4, X[s]);
 #pragma unroll
-    for (int v = 0; v < 8; ++v) p[feat_of(s, 0, v) * kStagePitch] = X[s][v];
+    for (int d = 0; d < 4; ++d) {
+      const unsigned partner = (unsigned)__builtin_amdgcn_mov_dpp((int)w[d], 0xB1, 0xF, 0xF, false);  // lane ^ 1
+      base[feat_of(s, 0, 2 * d) * (PITCH / 2)] = __builtin_amdgcn_perm(partner, w[d], sel);
+    }
+  }
 }
 
-// acc += dZ[32 mb + r][records] x H[32 nb + c][records]^T over the 128 staged
-// records (K steps of 16). Lane (g, col) reads 8 consecutive records of row
-// col of each operand; rows with a_valid false contribute zeros.
-template <class T>
+// acc += dZ[32 mb + r][records] x H[32 nb + c][records]^T over the TILE
+// staged records (K steps of 16). Lane (g, col) reads 8 consecutive records
+// of row col of each operand; rows with a_valid false contribute zeros.
+template <class T, int TILE, int PITCH>
 __device__ __forceinline__ f32x16 dw_tile(const T* SdZ, const T* SH, int mb, int nb, int col, int g, bool a_valid,
                                           f32x16 acc) {
   typedef typename MfmaTraits<T>::vec V;
-  const V* pa = reinterpret_cast<const V*>(SdZ + (32 * mb + col) * kStagePitch + 8 * g);
-  const V* pb = reinterpret_cast<const V*>(SH + (32 * nb + col) * kStagePitch + 8 * g);
+  const V* pa = reinterpret_cast<const V*>(SdZ + (32 * mb + col) * PITCH + 8 * g);
+  const V* pb = reinterpret_cast<const V*>(SH + (32 * nb + col) * PITCH + 8 * g);
 #pragma unroll
-  for (int k = 0; k < kTrainTile / 16; ++k) {
+  for (int k = 0; k < TILE / 16; ++k) {
     V a = pa[2 * k];
     if (!a_valid) a = V{};
     acc = MfmaTraits<T>::mfma(a, pb[2 * k], acc);
@@ -580,32 +615,139 @@ __device__ __forceinline__ f32x16 dw_tile(const T* SdZ, const T* SH, int mb, int
   return acc;
 }
 
+// Copies 16 bytes per lane from memory into LDS without registers: the
+// wave's 64 elements land at consecutive 16-byte slots from the wave-uniform
+// address lds_dst (global_load_lds_dwordx4). The builtin exists only in the
+// device pass; a kernel template that names it in the host pass loses its
+// launch stub.
+__device__ __forceinline__ void lds_dma16(const void* src, void* lds_dst) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)lds_dst, 16, 0, 0);
+#endif
+}
+
+// One block tile after the encoding (H[0] set): forward, loss, backward and
+// the weight-gradient tiles of the layers this wave's group owns.
+template <class T, int WAVES>
+__device__ __forceinline__ void train_tile(const typename MfmaTraits<T>::vec* lds, T* sdz, T* sh,
+                                           typename MfmaTraits<T>::vec H[6][4], const float* target, int q,
+                                           bool valid, int loss, float grad_scale, int lane, int g, int col, int qt,
+                                           int grp, int mb, int nb, f32x16* dw, float& lsum) {
+  typedef MfmaTraits<T> M;
+  typedef typename M::vec V;
+  typedef TrainShape<WAVES> S;
+  constexpr int P = S::pitch, LPG = S::layers;
+#pragma unroll
+  for (int l = 0; l < 5; ++l) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) H[l + 1][s] = H[l][s];
+    hidden_layer<T>(lds, l, lane, H[l + 1]);
+  }
+  f32x16 z = output_layer<T>(lds, lane, H[5]);
+  pin_wave(z);
+  // ---- loss and output gradient (lanes of group 0 own rows 0..2) ----
+  float dz[3] = {0.f, 0.f, 0.f};
+  if (g == 0 && valid) {
+    float y[3], tg[3];
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+      y[o] = expf(z[o]);
+      tg[o] = target[(size_t)q * 3 + o];
+    }
+    const float lum = 0.212671f * y[0] + 0.715160f * y[1] + 0.072169f * y[2];
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+      const float d = y[o] - tg[o];
+      float sc = 1.f;
+      if (loss == HMSNN_LOSS_RELATIVE_L2_LUMINANCE) sc = 1.f / (lum * lum + 0.01f);
+      else if (loss == HMSNN_LOSS_RELATIVE_L2) sc = 1.f / (y[o] * y[o] + 0.01f);
+      lsum += d * d * sc;
+      const float gval = 2.f * d * sc * fmaxf(y[o], kExpGradFloor) * grad_scale;
+      dz[o] = fminf(fmaxf(gval, -60000.f), 60000.f);
+    }
+  }
+  // ---- output layer: dH_5 = W5^T dz; dW_5 from the staged dZ_5 rows 0..2 ----
+  V Bdz;
+#pragma unroll
+  for (int v = 0; v < 8; ++v) Bdz[v] = M::cvt(v < 3 ? dz[v] : 0.f);  // group 1 holds zeros
+  f32x16 acc[2];
+#pragma unroll
+  for (int ib = 0; ib < 2; ++ib)
+    acc[ib] = M::mfma(lds[kFragFwd + kFragOut + kFragBwd + ib * 64 + lane], Bdz, f32x16{});
+  if (g == 0) {
+#pragma unroll
+    for (int o = 0; o < 3; ++o) sdz[o * P + qt] = Bdz[o];
+  }
+  stage_tile<T, P>(sh, qt, g, H[5]);
+  __syncthreads();
+  if (grp == 5 / LPG && mb == 0) dw[5 % LPG] = dw_tile<T, S::tile, P>(sdz, sh, 0, nb, col, g, col < 3, dw[5 % LPG]);
+  __syncthreads();
+  // ---- hidden layers, last to first ----
+#pragma unroll
+  for (int l = 4; l >= 0; --l) {
+    // dZ_l = dH_{l+1} * [H_{l+1} > 0]
+    V D[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int v = 0; v < 8; ++v)
+        D[s][v] = M::back(H[l + 1][s][v]) > 0.f ? M::cvt(acc[s >> 1][8 * (s & 1) + v]) : M::cvt(0.f);
+    stage_tile<T, P>(sdz, qt, g, D);
+    stage_tile<T, P>(sh, qt, g, H[l]);
+    if (l > 0) {
+      // dH_l = W_l^T dZ_l
+      acc[0] = f32x16{};
+      acc[1] = f32x16{};
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int ib = 0; ib < 2; ++ib)
+          acc[ib] = M::mfma(lds[kFragFwd + kFragOut + ((l * 2 + ib) * 4 + s) * 64 + lane], D[s], acc[ib]);
+    }
+    __syncthreads();
+    if (grp == l / LPG) dw[l % LPG] = dw_tile<T, S::tile, P>(sdz, sh, mb, nb, col, g, true, dw[l % LPG]);
+    __syncthreads();
+  }
+}
+
 // partial[(block * 6 + l) * 4096 + o * 64 + i] = sum over the block's records
 // of dZ_l[o] H_l[i] (dZ scaled by grad_scale); loss_part[block] = the block's
 // summed loss.
-template <class T>
-__global__ void __launch_bounds__(kMlpBlock) k_mlp_train(const T* frag, const float* in, const float* target, int n,
-                                                         int loss, float grad_scale, float* partial,
-                                                         float* loss_part) {
+template <class T, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) k_mlp_train(const T* frag, const float* in, const float* target, int n,
+                                                          int loss, float grad_scale, float* partial,
+                                                          float* loss_part) {
   typedef MfmaTraits<T> M;
   typedef typename M::vec V;
+  typedef TrainShape<WAVES> S;
   extern __shared__ char smem[];
   V* lds = reinterpret_cast<V*>(smem);
   T* sdz = reinterpret_cast<T*>(smem + sizeof(V) * kFragAll);
-  T* sh = sdz + 64 * kStagePitch;
-  __shared__ float wave_loss[kMlpBlock / 64];
-  stage_fragments<T>(frag, lds, kFragAll);
+  T* sh = sdz + 64 * S::pitch;
+  __shared__ float wave_loss[WAVES];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 5, col = lane & 31;
-  const int mb = wave >> 1, nb = wave & 1;  // this wave's dW tile
-  const int qt = 32 * wave + col;           // record within the block tile
-  f32x16 dw[6];
+  const int grp = wave >> 2;                      // owns the dW of layers S::layers grp ..
+  const int mb = (wave >> 1) & 1, nb = wave & 1;  // this wave's dW tile
+  const int qt = 32 * wave + col;                 // record within the block tile
+  const int tiles = (n + S::tile - 1) / S::tile;
+  // The weight fragments go straight from memory into LDS (no registers) and
+  // are in flight while the first tile's records are loaded and encoded.
+  {
+    const V* src = reinterpret_cast<const V*>(frag);
 #pragma unroll
-  for (int l = 0; l < 6; ++l) dw[l] = f32x16{};
+    for (int k = 0; k < (kFragAll + S::threads - 1) / S::threads; ++k) {
+      const int base = 64 * wave + k * S::threads;  // kFragAll is a multiple of 64
+      if (base < kFragAll) lds_dma16(src + base + lane, lds + base);
+    }
+  }
+  f32x16 dw[S::layers];
+#pragma unroll
+  for (int j = 0; j < S::layers; ++j) dw[j] = f32x16{};
   float lsum = 0.f;
-  const int tiles = (n + kTrainTile - 1) / kTrainTile;
+  bool first = true;
   for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
-    const int q = t * kTrainTile + qt;
+    const int q = t * S::tile + qt;
     const bool valid = q < n;
     float rec[12];
 #pragma unroll
@@ -620,94 +762,36 @@ __global__ void __launch_bounds__(kMlpBlock) k_mlp_train(const T* frag, const fl
 #pragma unroll
       for (int s = 0; s < 4; ++s) H[0][s] = encoded_operand<T>(e, s, g);
     }
-#pragma unroll
-    for (int l = 0; l < 5; ++l) {
-#pragma unroll
-      for (int s = 0; s < 4; ++s) H[l + 1][s] = H[l][s];
-      hidden_layer<T>(lds, l, lane, H[l + 1]);
-    }
-    f32x16 z = output_layer<T>(lds, lane, H[5]);
-    pin_wave(z);
-    // ---- loss and output gradient (lanes of group 0 own rows 0..2) ----
-    float dz[3] = {0.f, 0.f, 0.f};
-    if (g == 0 && valid) {
-      float y[3], tg[3];
-#pragma unroll
-      for (int o = 0; o < 3; ++o) {
-        y[o] = expf(z[o]);
-        tg[o] = target[(size_t)q * 3 + o];
-      }
-      const float lum = 0.212671f * y[0] + 0.715160f * y[1] + 0.072169f * y[2];
-#pragma unroll
-      for (int o = 0; o < 3; ++o) {
-        const float d = y[o] - tg[o];
-        float sc = 1.f;
-        if (loss == HMSNN_LOSS_RELATIVE_L2_LUMINANCE) sc = 1.f / (lum * lum + 0.01f);
-        else if (loss == HMSNN_LOSS_RELATIVE_L2) sc = 1.f / (y[o] * y[o] + 0.01f);
-        lsum += d * d * sc;
-        const float gval = 2.f * d * sc * fmaxf(y[o], kExpGradFloor) * grad_scale;
-        dz[o] = fminf(fmaxf(gval, -60000.f), 60000.f);
-      }
-    }
-    // ---- output layer: dH_5 = W5^T dz; dW_5 from the staged dZ_5 rows 0..2 ----
-    V Bdz;
-#pragma unroll
-    for (int v = 0; v < 8; ++v) Bdz[v] = M::cvt(v < 3 ? dz[v] : 0.f);  // group 1 holds zeros
-    f32x16 acc[2];
-#pragma unroll
-    for (int ib = 0; ib < 2; ++ib)
-      acc[ib] = M::mfma(lds[kFragFwd + kFragOut + kFragBwd + ib * 64 + lane], Bdz, f32x16{});
-    if (g == 0) {
-#pragma unroll
-      for (int o = 0; o < 3; ++o) sdz[o * kStagePitch + qt] = Bdz[o];
-    }
-    stage_tile<T>(sh, qt, g, H[5]);
-    __syncthreads();
-    if (mb == 0) dw[5] = dw_tile<T>(sdz, sh, 0, nb, col, g, col < 3, dw[5]);
-    __syncthreads();
-    // ---- hidden layers, last to first ----
-#pragma unroll
-    for (int l = 4; l >= 0; --l) {
-      // dZ_l = dH_{l+1} * [H_{l+1} > 0]
-      V D[4];
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int v = 0; v < 8; ++v)
-          D[s][v] = M::back(H[l + 1][s][v]) > 0.f ? M::cvt(acc[s >> 1][8 * (s & 1) + v]) : M::cvt(0.f);
-      stage_tile<T>(sdz, qt, g, D);
-      stage_tile<T>(sh, qt, g, H[l]);
-      if (l > 0) {
-        // dH_l = W_l^T dZ_l
-        acc[0] = f32x16{};
-        acc[1] = f32x16{};
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-#pragma unroll
-          for (int ib = 0; ib < 2; ++ib)
-            acc[ib] = M::mfma(lds[kFragFwd + kFragOut + ((l * 2 + ib) * 4 + s) * 64 + lane], D[s], acc[ib]);
-      }
+    if (first) {
+      __builtin_amdgcn_s_waitcnt(0);  // the fragment loads have landed in LDS
       __syncthreads();
-      dw[l] = dw_tile<T>(sdz, sh, mb, nb, col, g, true, dw[l]);
-      __syncthreads();
+      first = false;
     }
+    train_tile<T, WAVES>(lds, sdz, sh, H, target, q, valid, loss, grad_scale, lane, g, col, qt, grp, mb, nb, dw,
+                         lsum);
   }
   // ---- the block's partial sums ----
   float* out = partial + (size_t)blockIdx.x * kTrainPartialFloats;
 #pragma unroll
-  for (int l = 0; l < 6; ++l) {
+  for (int j = 0; j < S::layers; ++j) {
+    const int l = S::layers * grp + j;
     if (l == 5 && mb == 1) continue;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const int o = 32 * mb + 8 * (k / 4) + 4 * g + (k % 4);  // dW row (output feature)
       const int i = 32 * nb + col;                            // dW column (input feature)
-      if (l < 5 || o < 3) out[l * 4096 + o * 64 + i] = dw[l][k];
+      if (l < 5 || o < 3) out[l * 4096 + o * 64 + i] = dw[j][k];
     }
   }
   for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
   if (lane == 0) wave_loss[wave] = lsum;
   __syncthreads();
-  if (threadIdx.x == 0) loss_part[blockIdx.x] = ((wave_loss[0] + wave_loss[1]) + wave_loss[2]) + wave_loss[3];
+  if (threadIdx.x == 0) {
+    float b = 0.f;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) b += wave_loss[w];
+    loss_part[blockIdx.x] = b;
+  }
 }
 
 // Where weight k = (layer l, row o, column i) of the flat vector sits in the
@@ -729,17 +813,20 @@ __device__ __forceinline__ void frag_slots(int k, int* fwd, int* bwd) {
   }
 }
 
-// 64 weights per block: wave p sums the partials of blocks p, p + 4, ...;
-// the four sums are added in LDS. Then the gradient (optionally stored) and,
-// with adam, one Adam step on the FP32 master weights and the new 16-bit
-// fragment values (frag, when given). Block 0's first wave also sums the
-// blocks' losses into loss_sum.
+// 64 weights per block of kReduceWaves waves: wave p sums the partials of
+// blocks p, p + kReduceWaves, ...; the sums are added in LDS in wave order.
+// Then the gradient (optionally stored) and, with adam, one Adam step on the
+// FP32 master weights and the new 16-bit fragment values (frag, when given).
+// Block 0's first wave also sums the blocks' losses into loss_sum.
+constexpr int kReduceWaves = 16;
 template <class T>
-__global__ void __launch_bounds__(256) k_mlp_reduce_adam(const float* partial, const float* loss_part, int nchunks,
-                                                         float inv_scale, float* grad, float* loss_sum, float* w,
-                                                         float* m, float* v, int adam, float lr, float b1, float b2,
-                                                         float eps, float l2, float bc1, float bc2, T* frag) {
-  __shared__ float red[4][64];
+__global__ void __launch_bounds__(64 * kReduceWaves) k_mlp_reduce_adam(const float* partial, const float* loss_part,
+                                                                       int nchunks, float inv_scale, float* grad,
+                                                                       float* loss_sum, float* w, float* m, float* v,
+                                                                       int adam, float lr, float b1, float b2,
+                                                                       float eps, float l2, float bc1, float bc2,
+                                                                       T* frag) {
+  __shared__ float red[kReduceWaves][64];
   const int j = threadIdx.x & 63, p = threadIdx.x >> 6;
   const int k = blockIdx.x * 64 + j;
   const bool ok = k < kMlpWeightCount;
@@ -747,7 +834,7 @@ __global__ void __launch_bounds__(256) k_mlp_reduce_adam(const float* partial, c
   if (ok) {
     const float* src = partial + k;
 #pragma unroll 4
-    for (int c = p; c < nchunks; c += 4) s += src[(size_t)c * kTrainPartialFloats];
+    for (int c = p; c < nchunks; c += kReduceWaves) s += src[(size_t)c * kTrainPartialFloats];
   }
   red[p][j] = s;
   if (blockIdx.x == 0 && p == 0 && loss_sum) {
@@ -758,7 +845,10 @@ __global__ void __launch_bounds__(256) k_mlp_reduce_adam(const float* partial, c
   }
   __syncthreads();
   if (p != 0 || !ok) return;
-  const float gk0 = (((red[0][j] + red[1][j]) + red[2][j]) + red[3][j]) * inv_scale;
+  float gsum = 0.f;
+#pragma unroll
+  for (int r = 0; r < kReduceWaves; ++r) gsum += red[r][j];
+  const float gk0 = gsum * inv_scale;
   if (grad) grad[k] = gk0;
   if (!adam) return;
   const float gk = gk0 + l2 * w[k];
@@ -937,7 +1027,10 @@ int mlp_forward(MlpState& s, const float* in, int n, float* out, hipStream_t st,
 
 // Floats of the partial buffer for a step over n records: each block's dW
 // sum, then each block's loss.
-static size_t train_partial_floats(int n) { return (kTrainPartialFloats + 1) * (size_t)train_blocks(n); }
+static size_t train_partial_floats(int n) {
+  // the most blocks either precision's tile gives
+  return (kTrainPartialFloats + 1) * (size_t)train_blocks(n, TrainShape<4>::tile);
+}
 
 static int mlp_step_on(MlpState& s, float* partial, const float* in, const float* target, int n, int loss,
                        float* grad_out, float* loss_out, float* loss_dev, bool adam, float lr, hipStream_t st,
@@ -950,7 +1043,7 @@ int mlp_loss_grad_step(MlpState& s, const float* in, const float* target, int n,
     err = "empty training batch";
     return kEInval;
   }
-  const int blocks = train_blocks(n);
+  const int blocks = train_blocks(n, TrainShape<4>::tile);
   if (blocks > s.cap) {
     dev_free(s.partial);
     s.partial = nullptr;
@@ -966,20 +1059,21 @@ __global__ void k_loss_mean(const float* loss_sum, int n, float* out) {
   if (threadIdx.x == 0) *out = *loss_sum / (3.f * (float)n);
 }
 
-template <class T>
+template <class T, int W>
 static int launch_train(MlpState& s, float* partial, const float* in, const float* target, int n, int loss,
                         float grad_scale, float* grad_out, bool adam, float lr, hipStream_t st, std::string& err) {
+  typedef TrainShape<W> S;
   static bool attr_set = false;
   if (!attr_set) {
-    HM_HIP(hipFuncSetAttribute((const void*)k_mlp_train<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)kTrainSmem),
+    HM_HIP(hipFuncSetAttribute((const void*)k_mlp_train<T, W>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)S::smem),
            err);
     attr_set = true;
   }
-  const int blocks = train_blocks(n);
+  const int blocks = train_blocks(n, S::tile);
   float* loss_part = partial + kTrainPartialFloats * (size_t)blocks;
-  hipLaunchKernelGGL(k_mlp_train<T>, dim3(blocks), dim3(kMlpBlock), kTrainSmem, st, (const T*)s.frag, in, target, n,
-                     loss, grad_scale, partial, loss_part);
+  hipLaunchKernelGGL((k_mlp_train<T, W>), dim3(blocks), dim3(S::threads), S::smem, st, (const T*)s.frag, in, target,
+                     n, loss, grad_scale, partial, loss_part);
   HM_TRY(check_launch("mlp train", err));
   const float inv_scale = 1.f / (grad_scale * 3.f * (float)n);
   float bc1 = 1.f, bc2 = 1.f;
@@ -988,7 +1082,7 @@ static int launch_train(MlpState& s, float* partial, const float* in, const floa
     bc1 = 1.f - std::pow(0.9f, (float)s.step);
     bc2 = 1.f - std::pow(0.99f, (float)s.step);
   }
-  hipLaunchKernelGGL(k_mlp_reduce_adam<T>, dim3((kMlpWeightCount + 63) / 64), dim3(256), 0, st, partial, loss_part,
+  hipLaunchKernelGGL(k_mlp_reduce_adam<T>, dim3((kMlpWeightCount + 63) / 64), dim3(64 * kReduceWaves), 0, st, partial, loss_part,
                      blocks, inv_scale, grad_out, s.loss_sum, s.w, s.m, s.v, adam ? 1 : 0, lr, 0.9f, 0.99f, 1e-15f,
                      1e-6f, bc1, bc2, adam ? (T*)s.frag : (T*)nullptr);
   return check_launch("mlp reduce/adam", err);
@@ -1001,9 +1095,9 @@ static int mlp_step_on(MlpState& s, float* partial, const float* in, const float
   // small gradients stay normal numbers in 16 bits
   const float grad_scale = 1.f / 64.f;
   if (s.bf16())
-    HM_TRY(launch_train<__bf16>(s, partial, in, target, n, loss, grad_scale, grad_out, adam, lr, st, err));
+    HM_TRY((launch_train<__bf16, TrainWaves<__bf16>::value>)(s, partial, in, target, n, loss, grad_scale, grad_out, adam, lr, st, err));
   else
-    HM_TRY(launch_train<_Float16>(s, partial, in, target, n, loss, grad_scale, grad_out, adam, lr, st, err));
+    HM_TRY((launch_train<_Float16, TrainWaves<_Float16>::value>)(s, partial, in, target, n, loss, grad_scale, grad_out, adam, lr, st, err));
   if (adam && s.fp8()) {
     // the BF16 fragments were written by the Adam kernel; the FP8 ones need
     // the new per-layer weight scales first
@@ -1035,7 +1129,8 @@ static int mlp_step_on(MlpState& s, float* partial, const float* in, const float
 // GPU's copy and leaves its last contents, so calls that overlap on one GPU
 // race. Nothing in it needs clearing: every block writes all of its partials.
 constexpr int kNoAllocMaxRecords = 16384;
-constexpr size_t kNoAllocScratchBytes = sizeof(float) * (kTrainPartialFloats + 1) * train_blocks(kNoAllocMaxRecords);
+constexpr size_t kNoAllocScratchBytes =
+    sizeof(float) * (kTrainPartialFloats + 1) * train_blocks(kNoAllocMaxRecords, TrainShape<4>::tile);
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 // The device address of `scratch` on the current device, looked up once per

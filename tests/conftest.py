@@ -13,23 +13,24 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP library")
 
 
-def _gpu_available() -> bool:
-    try:
-        import hairmsnn_amd as hm
-        n, _ = hm.device_info()
-        return n > 0
-    except Exception:
-        return False
-
-
 @pytest.fixture(scope="session")
 def gpu():
-    """Skips a gpu-marked test on machines without a GPU; fails if the library is missing."""
+    """Skips a gpu-marked test on machines without a GPU; fails if the library
+    is missing or does not load (an unresolved symbol must not read as "no
+    GPU")."""
     import hairmsnn_amd as hm
     from hairmsnn_amd._lib import LIB_PATH
     if not LIB_PATH.exists():
         pytest.fail(f"HIP library missing: {LIB_PATH}")
-    if not _gpu_available():
+    try:
+        hm._lib.load()
+    except OSError as e:
+        pytest.fail(f"HIP library does not load: {e}")
+    try:
+        n, _ = hm.device_info()
+    except Exception:
+        n = 0
+    if n <= 0:
         pytest.skip("no GPU visible")
     return hm
 
