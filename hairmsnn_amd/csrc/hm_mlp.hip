@@ -553,10 +553,9 @@ template <int WAVES>
 struct TrainShape {
   static constexpr int threads = 64 * WAVES;
   static constexpr int tile = 32 * WAVES;   // records per block tile
-  static constexpr int pitch = tile + 8;    // 16-bit values per staged feature row (+8: bank spread)
   static constexpr int groups = WAVES / 4;
   static constexpr int layers = 6 / groups;  // dW layers per group
-  static constexpr size_t smem = sizeof(f16x8) * kFragAll + 2 * sizeof(_Float16) * 64 * pitch;
+  static constexpr size_t smem = sizeof(f16x8) * kFragAll + 2 * sizeof(unsigned) * 64 * (tile / 2 + 12);
   static_assert(smem + 64 <= 160 * 1024, "weight fragments and staging fit the LDS of one CU");
 };
 template <class T>
@@ -572,45 +571,72 @@ __host__ __device__ constexpr int train_blocks(int n, int tile) {
   return (n + tile - 1) / tile < kTrainMaxBlocks ? (n + tile - 1) / tile : kTrainMaxBlocks;
 }
 
+// Staged arrays are [feature][record]: row f starts at 32-bit word
+// f * pitch + sw(f), where the 32-bit word j of a row holds records 2 j and
+// 2 j + 1. pitch = TILE / 2 + 12 words (12 mod 32) and sw(f) = 4 (f & 1) +
+// 4 ((f >> 2) & 1) words put rows 0..7 at 8 different 4-bank groups (the
+// weight gradient's operand reads: lane col reads 4 words of row 32 mb + col)
+// and rows f and f + 1 (f even) 16 banks apart (the staging stores: 16 lanes
+// of a half wave to row f, 16 to row f + 1). sw depends only on bits 0 and 2
+// of f, which a staging lane's base row carries, so every store's offset from
+// the lane's base is a constant.
+template <int TILE>
+struct Stage {
+  static constexpr int pitch = TILE / 2 + 12;  // 32-bit words per row
+  static constexpr int words = 64 * pitch;     // one staged array
+};
+__device__ __forceinline__ int stage_sw(int f) { return 4 * (f & 1) + 4 * ((f >> 2) & 1); }
+
 // Stores a wave's 32 records x 64 features (operand layout) into a staged
-// [feature][record] array. Lane (g, col) holds record qt, features
-// feat_of(s, g, v) = feat_of(s, 0, v) + 4 g, two consecutive features per
-// 32-bit word. Lane pairs (qt even, qt + 1) swap words so that each lane
-// writes one word of two consecutive records of one feature: the even lane
-// the first feature of the pair, the odd lane the second. With a row pitch of
-// 4 mod 32 words the 64 lanes' words fall on every LDS bank exactly twice.
+// array. Lane (g, col) holds record qt, features feat_of(s, g, v) =
+// feat_of(s, 0, v) + 4 g, two consecutive features per 32-bit word. Lane
+// pairs (qt even, qt + 1) swap words so that each lane writes one word of two
+// consecutive records of one feature: the even lane the first feature of the
+// pair, the odd lane the second.
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-template <class T, int PITCH>
+template <class T, int TILE>
 __device__ __forceinline__ void stage_tile(T* S, int qt, int g, const typename MfmaTraits<T>::vec X[4]) {
-  static_assert(PITCH % 64 == 8, "staged rows are 4 mod 32 words apart");
+  constexpr int P = Stage<TILE>::pitch;
   const bool odd = qt & 1;
   const unsigned sel = odd ? 0x03020706u : 0x05040100u;  // odd: (partner hi, own hi); even: (own lo, partner lo)
-  unsigned* base = reinterpret_cast<unsigned*>(S + (4 * g + (odd ? 1 : 0)) * PITCH + (qt & ~1));
+  // row f = f0 + c with c = feat_of(s, 0, 2 d) (bits 1, 3-5) and f0 in
+  // {0, 1, 4, 5} (bits 0, 2): sw(f) = sw(f0)
+  const int f0 = 4 * g + (odd ? 1 : 0);
+  unsigned* base = reinterpret_cast<unsigned*>(S) + f0 * P + stage_sw(f0) + (qt >> 1);
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const u32x4 w = __builtin_bit_cast(u32x4, X[s]);
 #pragma unroll
     for (int d = 0; d < 4; ++d) {
       const unsigned partner = (unsigned)__builtin_amdgcn_mov_dpp((int)w[d], 0xB1, 0xF, 0xF, false);  // lane ^ 1
-      base[feat_of(s, 0, 2 * d) * (PITCH / 2)] = __builtin_amdgcn_perm(partner, w[d], sel);
+      base[feat_of(s, 0, 2 * d) * P] = __builtin_amdgcn_perm(partner, w[d], sel);
     }
   }
 }
 
+// One 16-bit value (row f, record r) into a staged array.
+template <class T, int TILE>
+__device__ __forceinline__ void stage_one(T* S, int f, int r, T x) {
+  S[2 * (f * Stage<TILE>::pitch + stage_sw(f)) + r] = x;
+}
+
 // acc += dZ[32 mb + r][records] x H[32 nb + c][records]^T over the TILE
 // staged records (K steps of 16). Lane (g, col) reads 8 consecutive records
-// of row col of each operand; rows with a_valid false contribute zeros.
-template <class T, int TILE, int PITCH>
+// (4 words) of row col of each operand; rows with a_valid false contribute
+// zeros.
+template <class T, int TILE>
 __device__ __forceinline__ f32x16 dw_tile(const T* SdZ, const T* SH, int mb, int nb, int col, int g, bool a_valid,
                                           f32x16 acc) {
   typedef typename MfmaTraits<T>::vec V;
-  const V* pa = reinterpret_cast<const V*>(SdZ + (32 * mb + col) * PITCH + 8 * g);
-  const V* pb = reinterpret_cast<const V*>(SH + (32 * nb + col) * PITCH + 8 * g);
+  constexpr int P = Stage<TILE>::pitch;
+  const int ra = 32 * mb + col, rb = 32 * nb + col;
+  const unsigned* pa = reinterpret_cast<const unsigned*>(SdZ) + ra * P + stage_sw(ra) + 4 * g;
+  const unsigned* pb = reinterpret_cast<const unsigned*>(SH) + rb * P + stage_sw(rb) + 4 * g;
 #pragma unroll
   for (int k = 0; k < TILE / 16; ++k) {
-    V a = pa[2 * k];
+    V a = *reinterpret_cast<const V*>(pa + 8 * k);
     if (!a_valid) a = V{};
-    acc = MfmaTraits<T>::mfma(a, pb[2 * k], acc);
+    acc = MfmaTraits<T>::mfma(a, *reinterpret_cast<const V*>(pb + 8 * k), acc);
   }
   return acc;
 }
@@ -626,6 +652,42 @@ __device__ __forceinline__ void lds_dma16(const void* src, void* lds_dst) {
 #endif
 }
 
+// D = cvt(acc) where H > 0, else 0 (the ReLU derivative applied to dH held
+// in the accumulator layout). H is a ReLU output, so H > 0 exactly when its
+// 16-bit pattern is a positive integer (+0 and -0 give 0 and 0x8000): the
+// mask 0 - min(bits, 1) is all ones for H > 0 and leaves at most a sign bit
+// (a zero of either sign) otherwise. FP16: packed 16-bit integer ops on pairs;
+// BF16 per value.
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+template <class T>
+__device__ __forceinline__ void relu_grad(const f32x16 acc[2], const typename MfmaTraits<T>::vec H[4],
+                                          typename MfmaTraits<T>::vec D[4]) {
+  if constexpr (std::is_same<T, _Float16>::value) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const u32x4 hb = __builtin_bit_cast(u32x4, H[s]);
+      u32x4 out;
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        const f16x2 dh = {(_Float16)acc[s >> 1][8 * (s & 1) + 2 * d], (_Float16)acc[s >> 1][8 * (s & 1) + 2 * d + 1]};
+        // (the element goes through a scalar: this compiler bit-casts an
+        // indexed vector element as element 0)
+        const unsigned hw = hb[d];
+        const s16x2 m = s16x2{0, 0} - __builtin_elementwise_min(__builtin_bit_cast(s16x2, hw), s16x2{1, 1});
+        out[d] = __builtin_bit_cast(unsigned, dh) & __builtin_bit_cast(unsigned, m);
+      }
+      D[s] = __builtin_bit_cast(typename MfmaTraits<T>::vec, out);
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int v = 0; v < 8; ++v)
+        D[s][v] = MfmaTraits<T>::back(H[s][v]) > 0.f ? MfmaTraits<T>::cvt(acc[s >> 1][8 * (s & 1) + v])
+                                                      : MfmaTraits<T>::cvt(0.f);
+  }
+}
+
 // One block tile after the encoding (H[0] set): forward, loss, backward and
 // the weight-gradient tiles of the layers this wave's group owns.
 template <class T, int WAVES>
@@ -636,7 +698,7 @@ __device__ __forceinline__ void train_tile(const typename MfmaTraits<T>::vec* ld
   typedef MfmaTraits<T> M;
   typedef typename M::vec V;
   typedef TrainShape<WAVES> S;
-  constexpr int P = S::pitch, LPG = S::layers;
+  constexpr int TL = S::tile, LPG = S::layers;
 #pragma unroll
   for (int l = 0; l < 5; ++l) {
 #pragma unroll
@@ -676,24 +738,20 @@ __device__ __forceinline__ void train_tile(const typename MfmaTraits<T>::vec* ld
     acc[ib] = M::mfma(lds[kFragFwd + kFragOut + kFragBwd + ib * 64 + lane], Bdz, f32x16{});
   if (g == 0) {
 #pragma unroll
-    for (int o = 0; o < 3; ++o) sdz[o * P + qt] = Bdz[o];
+    for (int o = 0; o < 3; ++o) stage_one<T, TL>(sdz, o, qt, Bdz[o]);
   }
-  stage_tile<T, P>(sh, qt, g, H[5]);
+  stage_tile<T, TL>(sh, qt, g, H[5]);
   __syncthreads();
-  if (grp == 5 / LPG && mb == 0) dw[5 % LPG] = dw_tile<T, S::tile, P>(sdz, sh, 0, nb, col, g, col < 3, dw[5 % LPG]);
+  if (grp == 5 / LPG && mb == 0) dw[5 % LPG] = dw_tile<T, TL>(sdz, sh, 0, nb, col, g, col < 3, dw[5 % LPG]);
   __syncthreads();
   // ---- hidden layers, last to first ----
 #pragma unroll
   for (int l = 4; l >= 0; --l) {
     // dZ_l = dH_{l+1} * [H_{l+1} > 0]
     V D[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-      for (int v = 0; v < 8; ++v)
-        D[s][v] = M::back(H[l + 1][s][v]) > 0.f ? M::cvt(acc[s >> 1][8 * (s & 1) + v]) : M::cvt(0.f);
-    stage_tile<T, P>(sdz, qt, g, D);
-    stage_tile<T, P>(sh, qt, g, H[l]);
+    relu_grad<T>(acc, H[l + 1], D);
+    stage_tile<T, TL>(sdz, qt, g, D);
+    stage_tile<T, TL>(sh, qt, g, H[l]);
     if (l > 0) {
       // dH_l = W_l^T dZ_l
       acc[0] = f32x16{};
@@ -705,7 +763,7 @@ __device__ __forceinline__ void train_tile(const typename MfmaTraits<T>::vec* ld
           acc[ib] = M::mfma(lds[kFragFwd + kFragOut + ((l * 2 + ib) * 4 + s) * 64 + lane], D[s], acc[ib]);
     }
     __syncthreads();
-    if (grp == l / LPG) dw[l % LPG] = dw_tile<T, S::tile, P>(sdz, sh, mb, nb, col, g, true, dw[l % LPG]);
+    if (grp == l / LPG) dw[l % LPG] = dw_tile<T, TL>(sdz, sh, mb, nb, col, g, true, dw[l % LPG]);
     __syncthreads();
   }
 }
@@ -723,7 +781,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_mlp_train(const T* frag, const f
   extern __shared__ char smem[];
   V* lds = reinterpret_cast<V*>(smem);
   T* sdz = reinterpret_cast<T*>(smem + sizeof(V) * kFragAll);
-  T* sh = sdz + 64 * S::pitch;
+  T* sh = sdz + 2 * Stage<S::tile>::words;
   __shared__ float wave_loss[WAVES];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 5, col = lane & 31;
