@@ -334,7 +334,7 @@ int Context::init(std::string& err) {
     HM_TRY(dev_alloc((void**)&boxes_dev, sizeof(float) * boxes.size(), err));
     HM_HIP(hipMemcpy(boxes_dev, boxes.data(), sizeof(float) * boxes.size(), hipMemcpyHostToDevice), err);
     const int rc = bvh_build(boxes_dev, nt, mesh_bvh, stream, err, bvh_builder_id(scene.bvh_builder),
-                             scene.ploc_radius);
+                             scene.ploc_radius, scene.bvh_max_leaf);
     (void)hipStreamSynchronize(stream);
     dev_free(boxes_dev);
     HM_TRY(rc);
@@ -352,7 +352,8 @@ int Context::init(std::string& err) {
     HM_HIP(hipMemcpyAsync(seg_dev, hair.seg.data(), bytes, hipMemcpyHostToDevice, stream), err);
     const int builder = bvh_builder_id(scene.bvh_builder);
     if (hair.crv.empty()) {
-      HM_TRY(bvh_build(seg_dev, (int)(hair.seg.size() / 8), bvh, stream, err, builder, scene.ploc_radius));
+      HM_TRY(bvh_build(seg_dev, (int)(hair.seg.size() / 8), bvh, stream, err, builder, scene.ploc_radius,
+                       scene.bvh_max_leaf));
     } else {
       // round curves: the BVH over each segment's box (the Bezier hull of
       // the Catmull-Rom segment, padded by the radius bound), stored as
@@ -384,7 +385,7 @@ int Context::init(std::string& err) {
       float* boxes_dev = nullptr;
       HM_TRY(dev_alloc((void**)&boxes_dev, sizeof(float) * boxes.size(), err));
       HM_HIP(hipMemcpy(boxes_dev, boxes.data(), sizeof(float) * boxes.size(), hipMemcpyHostToDevice), err);
-      const int rc = bvh_build(boxes_dev, nc, bvh, stream, err, builder, scene.ploc_radius);
+      const int rc = bvh_build(boxes_dev, nc, bvh, stream, err, builder, scene.ploc_radius, scene.bvh_max_leaf);
       (void)hipStreamSynchronize(stream);
       dev_free(boxes_dev);
       HM_TRY(rc);

@@ -3,8 +3,8 @@
 // Layout in HBM: the Karras LBVH is built binary (64-byte nodes holding both
 // children's boxes) and then collapsed into wide nodes (BvhNodeWide, 4 or 8
 // children) that traversal reads; segments are 32-byte records in Morton
-// order. A child index >= 0 is an inner node, < 0 is ~leaf where leaf is the
-// position in the Morton-ordered segment array.
+// order. A child index >= 0 is an inner node, < 0 codes a leaf: a run of
+// consecutive positions of the segment array (leaf_code).
 //
 // Traversal is one ray per lane (wave64). The stack's top kStackLds entries
 // live in LDS, one column per lane (conflict-free), deeper entries spill to a
@@ -40,6 +40,7 @@ struct __attribute__((aligned(16))) BvhNodeWide {
   int pad[kBvhWidth == 4 ? 4 : 0];  // 4-wide: one 128-byte line; 8-wide: 224 bytes
 };
 constexpr int kEmptyChild = 0x7fffffff;
+
 
 struct BvhView {
   const BvhNodeWide* nodes;  // wide tree, root at 0
@@ -229,7 +230,7 @@ template <bool kCount, class Leaf = SegLeaf>
 HM_D void closest_step(const BvhView& bvh, const RayInv& r, TraceStack& st, ClosestState& s, TraceCounters& cnt,
                        const Leaf& leaf_test = Leaf{}) {
   if (s.cur < 0) {
-    const int leaf = ~s.cur;
+    const int code = ~s.cur, leaf = code & kLeafPosMask;
     if (kCount) cnt.segs++;
     float t, u;
     int id;
@@ -240,7 +241,9 @@ HM_D void closest_step(const BvhView& bvh, const RayInv& r, TraceStack& st, Clos
         s.best_u = u;
       }
     }
-    s.cur = st.sp ? st.pop() : kNone;
+    // the rest of a leaf run, one primitive per step
+    const int rest = leaf_rest(code);
+    s.cur = rest ? rest : (st.sp ? st.pop() : kNone);
   } else {
     const BvhNodeWide n = bvh.nodes[s.cur];
     if (kCount) cnt.nodes++;
@@ -305,7 +308,7 @@ HM_D void any_step(const BvhView& bvh, const RayInv& r, TraceStack& st, AnyState
                    const Leaf& leaf_test = Leaf{}) {
   const float tinf = __builtin_inff();
   if (s.cur < 0) {
-    const int leaf = ~s.cur;
+    const int code = ~s.cur, leaf = code & kLeafPosMask;
     if (kCount) cnt.segs++;
     float t, u;
     int id;
@@ -313,7 +316,8 @@ HM_D void any_step(const BvhView& bvh, const RayInv& r, TraceStack& st, AnyState
       s.hit = true;
       s.cur = kNone;
     } else {
-      s.cur = st.sp ? st.pop() : kNone;
+      const int rest = leaf_rest(code);
+      s.cur = rest ? rest : (st.sp ? st.pop() : kNone);
     }
   } else {
     const BvhNodeWide n = bvh.nodes[s.cur];

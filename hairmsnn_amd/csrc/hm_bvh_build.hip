@@ -153,9 +153,19 @@ __global__ void k_karras(BuildBuffers b) {
 
 __device__ __forceinline__ void child_box(const BuildBuffers& b, int c, float box[6]) {
   if (c < 0) {
-    const int leaf = ~c;
-    const int orig = b.vals_sorted[leaf];
-    for (int k = 0; k < 6; ++k) box[k] = b.leaf_box[6 * (size_t)orig + k];
+    // a leaf run: the union of its primitives' boxes
+    const int code = ~c, first = code & kLeafPosMask, count = (code >> kLeafCountShift) + 1;
+    for (int k = 0; k < 3; ++k) {
+      box[k] = 3.0e38f;
+      box[3 + k] = -3.0e38f;
+    }
+    for (int j = 0; j < count; ++j) {
+      const int orig = b.vals_sorted[first + j];
+      for (int k = 0; k < 3; ++k) {
+        box[k] = fminf(box[k], b.leaf_box[6 * (size_t)orig + k]);
+        box[3 + k] = fmaxf(box[3 + k], b.leaf_box[6 * (size_t)orig + 3 + k]);
+      }
+    }
   } else {
     for (int k = 0; k < 6; ++k)
       box[k] = __hip_atomic_load(&b.node_box[6 * (size_t)c + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -522,9 +532,13 @@ static int bvh_collapse(DeviceBvh& out, hipStream_t stream, std::string& err) {
 // Builds the BVH from device segments (8 floats each). All scratch is
 // released before returning; the result owns nodes and segment records.
 int bvh_build(const float* seg_dev, int n, DeviceBvh& out, hipStream_t stream, std::string& err, int builder,
-              int ploc_radius) {
+              int ploc_radius, int max_leaf) {
   out.release();
   out.num_segs = n;
+  if (n > kLeafPosMask + 1) {
+    err = "BVH: at most 16,777,216 primitives";
+    return kEInval;
+  }
   if (n <= 0) {
     out.max_depth = 0;
     return 0;
@@ -570,7 +584,7 @@ int bvh_build(const float* seg_dev, int n, DeviceBvh& out, hipStream_t stream, s
     HM_HIP(hipMemcpyAsync(boxes.data(), b.leaf_box, sizeof(float) * boxes.size(), hipMemcpyDeviceToHost, stream),
            err);
     HM_HIP(hipStreamSynchronize(stream), err);
-    sah_build(boxes.data(), n, child.data(), order.data(), pin.data(), pleaf.data());
+    sah_build(boxes.data(), n, child.data(), order.data(), pin.data(), pleaf.data(), max_leaf);
     HM_HIP(hipMemcpyAsync(b.child, child.data(), sizeof(int) * child.size(), hipMemcpyHostToDevice, stream), err);
     HM_HIP(hipMemcpyAsync(b.vals_sorted, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, stream),
            err);

@@ -3,7 +3,8 @@
 // Top-down: the centroids of a node's primitives are binned (kBins per
 // axis, all three axes), the split plane with the smallest
 // area(left) x count(left) + area(right) x count(right) is taken, and the
-// primitives are partitioned in place; one primitive per leaf. A node whose
+// primitives are partitioned in place; a subtree of at most max_leaf
+// primitives becomes one leaf (a run of consecutive positions). A node whose
 // centroids fall into one bin is split at the median along the longest
 // centroid axis. Subtrees above kTaskMin primitives are built on their own
 // threads. The result is the layout the GPU refit, emit and collapse steps
@@ -54,6 +55,7 @@ struct Builder {
   std::atomic<int> next_node{1};
   std::atomic<int> tasks{0};
   int max_tasks = 1;
+  int max_leaf = 1;
 
   float centroid(int p, int a) const { return 0.5f * (boxes[6 * (size_t)p + a] + boxes[6 * (size_t)p + 3 + a]); }
 
@@ -141,9 +143,10 @@ struct Builder {
     int c[2];
     const int rb[2] = {b, mid}, re[2] = {mid, e};
     for (int s = 0; s < 2; ++s) {
-      if (re[s] - rb[s] == 1) {
-        c[s] = ~rb[s];
+      if (re[s] - rb[s] <= max_leaf) {
+        c[s] = leaf_code(rb[s], re[s] - rb[s]);
         parent_leaf[rb[s]] = node;
+        for (int p = rb[s] + 1; p < re[s]; ++p) parent_leaf[p] = -1;
       } else {
         c[s] = next_node.fetch_add(1);
         parent_inner[c[s]] = node;
@@ -170,10 +173,12 @@ struct Builder {
 
 }  // namespace
 
-void sah_build(const float* boxes, int n, int* child, int* order, int* parent_inner, int* parent_leaf) {
+void sah_build(const float* boxes, int n, int* child, int* order, int* parent_inner, int* parent_leaf,
+               int max_leaf) {
   for (int i = 0; i < n; ++i) order[i] = i;
   if (n < 2) return;
   Builder bd;
+  bd.max_leaf = std::max(1, std::min(max_leaf, kMaxLeafRun));
   bd.boxes = boxes;
   bd.idx = order;
   bd.child = child;

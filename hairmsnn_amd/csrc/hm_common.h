@@ -144,4 +144,17 @@ HM_HD Frame basis_from_normal(V3 n) {
   return Frame{c1, c2, n};
 }
 
+// BVH leaf child code: ~(first | (count - 1) << kLeafCountShift), a run of
+// count consecutive primitive positions from first (a single primitive:
+// ~first). Trees hold at most 2^24 primitives.
+constexpr int kLeafCountShift = 24;
+constexpr int kLeafPosMask = (1 << kLeafCountShift) - 1;
+constexpr int kMaxLeafRun = 8;
+HM_HD int leaf_code(int first, int count) { return ~(first | ((count - 1) << kLeafCountShift)); }
+// The code of the rest of a run after its first position, or 0 (none left).
+HM_HD int leaf_rest(int code) {
+  const int more = code >> kLeafCountShift;
+  return more ? ~(((code & kLeafPosMask) + 1) | ((more - 1) << kLeafCountShift)) : 0;
+}
+
 }  // namespace hm

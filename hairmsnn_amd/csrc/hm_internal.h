@@ -98,12 +98,16 @@ struct DeviceBvh {
 // radix tree over Morton codes, or PLOC clustering in Morton order.
 constexpr int kBvhLbvh = 0, kBvhPloc = 1, kBvhSah = 2;
 // Binned SAH binary tree over n boxes (6 floats each: lo xyz, hi xyz), built
-// on the host (hm_bvh_sah.cpp): child[2 (n - 1)] (>= 0 an inner node, ~p the
-// leaf at position p; root 0), order[n] (leaf position -> box index),
-// parent_inner[n - 1] (-1 at the root), parent_leaf[n].
-void sah_build(const float* boxes, int n, int* child, int* order, int* parent_inner, int* parent_leaf);
+// on the host (hm_bvh_sah.cpp): child[2 (n - 1)] (>= 0 an inner node, < 0 a
+// leaf run coded as leaf_code(first position, count); root 0), order[n] (leaf
+// position -> box index), parent_inner[n - 1] (-1 at the root),
+// parent_leaf[n] (the parent of a run's first position; -1 for the run's
+// other positions). Subtrees of at most max_leaf boxes become one run.
+void sah_build(const float* boxes, int n, int* child, int* order, int* parent_inner, int* parent_leaf,
+               int max_leaf = 1);
+// max_leaf applies to the SAH builder; the others make one-primitive leaves.
 int bvh_build(const float* seg_dev, int n, DeviceBvh& out, hipStream_t stream, std::string& err,
-              int builder = kBvhLbvh, int ploc_radius = 16);
+              int builder = kBvhLbvh, int ploc_radius = 16, int max_leaf = 1);
 
 // Environment map and its importance-sampling tables (RGBA32F map, FP32
 // tables produced from FP64 prefix sums).

@@ -291,6 +291,11 @@ bool parse_json(const Json& cfg, Scene& scene, std::string& err) {
     const Json& a = cfg.at("accel");
     if (a.has("builder")) scene.bvh_builder = a.at("builder").str();
     if (a.has("ploc_radius")) scene.ploc_radius = a.at("ploc_radius").i();
+    if (a.has("max_leaf")) scene.bvh_max_leaf = a.at("max_leaf").i();
+    if (scene.bvh_max_leaf < 1 || scene.bvh_max_leaf > 8) {
+      err = "accel.max_leaf must be 1..8";
+      return false;
+    }
     if (scene.bvh_builder != "lbvh" && scene.bvh_builder != "ploc" && scene.bvh_builder != "sah") {
       err = "accel.builder must be \"lbvh\", \"ploc\" or \"sah\"";
       return false;

@@ -246,6 +246,45 @@ def test_gradient_batch_sizes_2e14_to_2e18(gpu, precision):
         assert np.isfinite(g).all() and rel < 1e-3
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_gradient_ragged_batches(gpu, precision):
+    """Batches that leave tiles and blocks part-filled. Padded records must
+    add nothing: a batch's gradient is the record-weighted mean of its
+    unequal parts' gradients (each record's 16-bit operands are the same
+    wherever it sits; only the FP32 summation order differs) for 257, 1000
+    and 70,001 records (the last: more tiles than blocks, so blocks take a
+    second, partial tile). Against the FP64 oracle, FP16 batches of 1 and 31
+    records (no cancellation between records yet) agree to 2e-3; larger
+    batches of random targets cancel, which magnifies the 16-bit rounding of
+    the per-record gradients (the 2^14-record test above holds 2e-2)."""
+    w = weights(33)
+    m = Mlp(gpu, precision)
+    m.set_weights(w)
+    if precision == 0:
+        for n in (1, 31):
+            r = records(n, 60 + n % 97)
+            t = np.random.default_rng(61 + n % 89).uniform(0.2, 3.0, (n, 3)).astype(np.float32)
+            lg, g = m.grad(r, t, 0)
+            lo, go = ol.mlp_loss_grad(w, r, t, loss=0, prec=0, exp_floor=ol.MLP_EXP_GRAD_FLOOR)
+            assert lg == pytest.approx(lo, rel=1e-3)
+            for l in range(6):
+                sl = slice(l * 4096, (l + 1) * 4096) if l < 5 else slice(5 * 4096, ol.MLP_WEIGHTS)
+                rel = np.linalg.norm(g[sl] - go[sl]) / np.linalg.norm(go[sl])
+                print(f"n {n} layer {l}: relative gradient error {rel:.2e}")
+                assert rel <= 2e-3
+    for n, cuts in ((257, (256,)), (1000, (1, 300, 999)), (70001, (9000, 30001, 52345))):
+        r = records(n, 71 + n % 13)
+        t = np.random.default_rng(72 + n % 11).uniform(0.2, 3.0, (n, 3)).astype(np.float32)
+        _, g = m.grad(r, t, 2)
+        edges = [0, *cuts, n]
+        gm = np.zeros(ol.MLP_WEIGHTS)
+        for a, b in zip(edges[:-1], edges[1:]):
+            gm += (b - a) / n * m.grad(r[a:b], t[a:b], 2)[1].astype(np.float64)
+        rel = np.linalg.norm(g - gm) / np.linalg.norm(gm)
+        print(f"precision {precision} batch {n}: |g - weighted mean of parts| / |g| = {rel:.2e}")
+        assert np.isfinite(g).all() and rel < 1e-3
+
+
 def test_weight_file_round_trip(gpu, tmp_path):
     """hmsnn_mlp_save / hmsnn_mlp_load: the weights come back bit for bit
     (9 significant digits round-trip FP32); a tiny-cuda-nn serialisation

@@ -119,13 +119,15 @@ def test_ploc_tree_bit_exact(gpu, name, radius):
     np.testing.assert_array_equal(occ, occ_c)
 
 
-@pytest.mark.parametrize("name", ["c0_hairball.json", "c1_curly.json"])
-def test_sah_tree_bit_exact(gpu, name):
+@pytest.mark.parametrize("name,max_leaf", [("c0_hairball.json", 1), ("c1_curly.json", 1), ("c0_hairball.json", 3),
+                                           ("c1_curly.json", 8)])
+def test_sah_tree_bit_exact(gpu, name, max_leaf):
     """The binned-SAH tree built on the host (accel.builder = "sah") gives the
-    same closest and any hits as the CPU BVH."""
+    same closest and any hits as the CPU BVH, with one primitive per leaf and
+    with leaves that are runs of up to max_leaf primitives."""
     cfg = json.loads((SCENES / name).read_text())
     cfg["integrator"].update({"width": 128, "height": 128})
-    cfg["accel"] = {"builder": "sah"}
+    cfg["accel"] = {"builder": "sah", "max_leaf": max_leaf}
     cfg.pop("surface", None)
     if "procedural" in cfg["hair"]:
         cfg["hair"]["procedural"]["strands"] = min(cfg["hair"]["procedural"]["strands"], 5000)
@@ -137,7 +139,7 @@ def test_sah_tree_bit_exact(gpu, name):
         depth = int(sc.info["bvh_depth"])
     prim_c, t_c, u_c = ol.intersect(seg, rays, use_bvh=True)
     hit = prim_c >= 0
-    print(f"{name} SAH: wide depth {depth}, hit fraction {hit.mean():.3f}")
+    print(f"{name} SAH, runs of up to {max_leaf}: wide depth {depth}, hit fraction {hit.mean():.3f}")
     assert hit.mean() > 0.1
     np.testing.assert_array_equal(prim, prim_c)
     np.testing.assert_array_equal(t[hit], t_c[hit])
