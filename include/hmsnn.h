@@ -368,11 +368,11 @@ int hmsnn_mlp_train_step_dev(hmsnn_mlp* mlp, const float* in_dev, const float* t
 /* hmsnn_mlp_train_step_dev without allocations (no counterpart in the
  * reference): for callers that cannot allocate device memory (code under
  * graph capture, tight serving or training loops). The same kernels as
- * hmsnn_mlp_train_step_dev, with their scratch (activations, gradients,
- * partial sums) in a static 26 MB device array per GPU instead of buffers
- * allocated on demand: n is limited to 1..16384 records (HMSNN_EINVAL
- * beyond), the parts that must start at zero are cleared by kernels (no
- * memset nodes), and the batch loss is written to loss_dev (device float,
+ * hmsnn_mlp_train_step_dev, with their scratch (per-block weight-gradient
+ * partial sums and losses) in a static 12.6 MB device array per GPU instead
+ * of a buffer allocated on demand: n is limited to 1..16384 records
+ * (HMSNN_EINVAL beyond), nothing needs clearing (every block writes all of
+ * its partials; no memset nodes), and the batch loss is written to loss_dev (device float,
  * optional) instead of being copied to the host. Every call on a GPU shares
  * that GPU's scratch, so calls that overlap on one GPU race. */
 int hmsnn_mlp_train_step_dev_noalloc(hmsnn_mlp* mlp, const float* in_dev, const float* target_dev, int n, int loss,
