@@ -73,109 +73,121 @@ HM_D V3 safe_ratio(V3 c, V3 f) {
   return v3(f.x != 0.f ? c.x / f.x : 0.f, f.y != 0.f ? c.y / f.y : 0.f, f.z != 0.f ? c.z / f.z : 0.f);
 }
 
-// One path of the integrator (the order of k_camera / k_shade / k_shadow),
-// replayed from its seed: adds the gradient given the path's primal radiance
-// L_total, and returns the radiance the replay gathered.
-HM_D V3 adjoint_path(const SceneView& sc, const AdjArgs& a, TraceStack& st, int x, int y, int pass, V3 L_total,
-                     V3 adj, float* gg) {
-  TraceCounters cnt{0, 0, 0};
-  uint32_t rng = path_seed((uint32_t)(pass + 10007), (uint32_t)x, (uint32_t)y, (uint32_t)sc.width);
-  const float jx = draw(rng), jy = draw(rng);
+// A replayed path between vertices: the ray to trace next and what the
+// vertices so far left (the operations and random draws are those of
+// k_camera / k_shade / k_shadow for the path's seed).
+struct AdjPath {
+  uint32_t rng;
+  V3 o, d;      // the ray to trace next
+  V3 beta, L;   // throughput and radiance gathered so far
+  V3 Lt, adj;   // the path's primal radiance and dLoss/dI / spp
+  int depth;    // the vertex the next hit is
+  int pix;
+  float gp[3];  // the path's global gradient
+};
+
+// The camera ray of pixel (x, y) in pass `pass` (path_tracing.cu:29-48).
+HM_D void adj_start(const SceneView& sc, int x, int y, int pass, AdjPath& q) {
+  q.rng = path_seed((uint32_t)(pass + 10007), (uint32_t)x, (uint32_t)y, (uint32_t)sc.width);
+  const float jx = draw(q.rng), jy = draw(q.rng);
   const float sx = ((float)x + jx) / (float)sc.width, sy = ((float)y + jy) / (float)sc.height;
-  V3 d = normalize(sc.cam_d00 + sx * sc.cam_du + sy * sc.cam_dv);
-  V3 o = sc.cam_pos;
-  float t, u;
-  int prim;
-  if (sc.curves)
-    closest_hit<false, true>(sc, o, d, st, t, prim, u, cnt);
-  else
-    closest_hit<false>(sc, o, d, st, t, prim, u, cnt);
-  if (prim == kPrimMiss) return sc.env.enabled ? env_radiance(sc.env, d) : v3(0.f);
-  V3 beta = v3(1.f), L = v3(0.f);
-  for (int depth = 0; depth <= a.max_vertex; ++depth) {
-    Hit h;
-    setup_hit(sc, o, d, t, prim, u, h);
-    const bool hair = !h.surface;
-    if (depth == 0 ? a.dl_zero != 0 : depth >= a.dl_from) {
-      DirectOut dl;
-      direct_lighting(sc, h, rng, dl);
-      const bool v1 = dl.has1 && !(sc.curves ? occluded<false, true>(sc, dl.o1, dl.d1, st, cnt)
-                                             : occluded<false>(sc, dl.o1, dl.d1, st, cnt));
-      const bool v2 = dl.has2 && !(sc.curves ? occluded<false, true>(sc, dl.o2, dl.d2, st, cnt)
-                                             : occluded<false>(sc, dl.o2, dl.d2, st, cnt));
-      V3 sum = v3(0.f);
-      if (v1) sum = sum + dl.c1;
-      if (v2) sum = sum + dl.c2;
-      const bool bad = any_nan(sum);
-      if (bad) sum = v3(0.f);
-      if (hair && !bad) {
-        const V3 sigma = strand_sigma_of(sc, prim);
-        const V3 W = adj * beta;  // dLoss / dc per channel
-        float r[kAdParams];
-        if (v1) {
-          Dual f[3], pd;
-          hair_eval_dual(a.hcd, h.wo_local, dl.wl1, h.hw.h, sigma, f, &pd);
-          const float pb = dl.bp1, pl = dl.lp1;
-#pragma unroll
-          for (int k = 0; k < kAdParams; ++k) r[k] = dl.mis1 ? -2.f * pb * pd.d[k] / (pl * pl + pb * pb) : 0.f;
-          add_grad(sc, a, prim, f, W * safe_ratio(dl.c1, dl.f1), dot(W, dl.c1), r, gg);
-        }
-        if (v2) {
-          Dual f[3], pd;
-          hair_eval_dual(a.hcd, h.wo_local, dl.wl2, h.hw.h, sigma, f, &pd);
-          const float pb = dl.bp2, pl = dl.lp2;
-#pragma unroll
-          for (int k = 0; k < kAdParams; ++k) r[k] = pb > 0.f ? 2.f * pl * pl * pd.d[k] / (pb * (pb * pb + pl * pl)) : 0.f;
-          add_grad(sc, a, prim, f, W * safe_ratio(dl.c2, dl.f2), dot(W, dl.c2), r, gg);
-        }
-      }
-      L = L + beta * sum;
-    }
-    if (depth >= 1) {
-      const float q = fmaxf(0.05f, 1.f - luminance(beta));
-      if (draw(rng) < q) break;
-      beta = beta / (1.f - q);
-    }
-    if (depth + 1 > a.max_vertex) break;
-    const BsdfSample bs = sample_bsdf(sc, h, rng);
-    const V3 wl = bs.wl, wi = bs.wi, f = bs.f;
-    const float pdf = bs.pdf;
-    V3 mul = (pdf == 0.f) ? f : f / pdf;
-    const bool mul_bad = any_nan(mul);
-    if (mul_bad) mul = v3(1.f);
-    if (hair && !mul_bad) {
-      const V3 rem = L_total - L;  // radiance gathered after this vertex
-      Dual fd[3], pd;
-      hair_eval_dual(a.hcd, h.wo_local, wl, h.hw.h, strand_sigma_of(sc, prim), fd, &pd);
-      float r[kAdParams];
-#pragma unroll
-      for (int k = 0; k < kAdParams; ++k) r[k] = 0.f;
-      add_grad(sc, a, prim, fd, adj * safe_ratio(rem, f), 0.f, r, gg);
-    }
-    beta = beta * mul;
-    V3 p = h.p, nd = h.n;
-    if (dot(wi, h.n) < 0.f && hair) {
-      nd = -1.f * h.n;
-      p = p + (2.f * h.radius) * nd;
-    }
-    o = p + 1e-3f * nd;
-    d = wi;
-    if (sc.curves)
-      closest_hit<false, true>(sc, o, d, st, t, prim, u, cnt);
-    else
-      closest_hit<false>(sc, o, d, st, t, prim, u, cnt);
-    if (prim == kPrimMiss) break;
-  }
-  return L;
+  q.d = normalize(sc.cam_d00 + sx * sc.cam_du + sy * sc.cam_dv);
+  q.o = sc.cam_pos;
+  q.beta = v3(1.f);
+  q.L = v3(0.f);
+  q.depth = 0;
+  q.gp[0] = q.gp[1] = q.gp[2] = 0.f;
 }
 
-// Waves per SIMD the replay kernel is compiled for. Unbounded it takes 256
-// VGPRs + 19 AGPRs (one wave); at 512x512 C1, 16 spp, the replay ran at
-// 27.7 M paths/s with 1 wave, 43.9 with 2, 48.3 with 3 and 46.7 with 4 (the
-// registers beyond the budget spill to scratch: 400, 704 and 864 bytes per
-// lane).
+// The vertex the path's ray hit (t, prim, u): its direct lighting with its
+// two shadow rays, the gradient of every factor there that depends on the
+// parameters, Russian roulette and the continuation. Returns whether the
+// path goes on (q then holds the next ray). Given the path's primal
+// radiance Lt, the continuation's gradient uses the radiance still to come,
+// Lt - L.
+HM_D bool adj_vertex(const SceneView& sc, const AdjArgs& a, TraceStack& st, float t, int prim, float u,
+                     AdjPath& q) {
+  TraceCounters cnt{0, 0, 0};
+  const int depth = q.depth;
+  Hit h;
+  setup_hit(sc, q.o, q.d, t, prim, u, h);
+  const bool hair = !h.surface;
+  if (depth == 0 ? a.dl_zero != 0 : depth >= a.dl_from) {
+    DirectOut dl;
+    direct_lighting(sc, h, q.rng, dl);
+    const bool v1 = dl.has1 && !(sc.curves ? occluded<false, true>(sc, dl.o1, dl.d1, st, cnt)
+                                           : occluded<false>(sc, dl.o1, dl.d1, st, cnt));
+    const bool v2 = dl.has2 && !(sc.curves ? occluded<false, true>(sc, dl.o2, dl.d2, st, cnt)
+                                           : occluded<false>(sc, dl.o2, dl.d2, st, cnt));
+    V3 sum = v3(0.f);
+    if (v1) sum = sum + dl.c1;
+    if (v2) sum = sum + dl.c2;
+    const bool bad = any_nan(sum);
+    if (bad) sum = v3(0.f);
+    if (hair && !bad) {
+      const V3 sigma = strand_sigma_of(sc, prim);
+      const V3 W = q.adj * q.beta;  // dLoss / dc per channel
+      float r[kAdParams];
+      if (v1) {
+        Dual f[3], pd;
+        hair_eval_dual(a.hcd, h.wo_local, dl.wl1, h.hw.h, sigma, f, &pd);
+        const float pb = dl.bp1, pl = dl.lp1;
+#pragma unroll
+        for (int k = 0; k < kAdParams; ++k) r[k] = dl.mis1 ? -2.f * pb * pd.d[k] / (pl * pl + pb * pb) : 0.f;
+        add_grad(sc, a, prim, f, W * safe_ratio(dl.c1, dl.f1), dot(W, dl.c1), r, q.gp);
+      }
+      if (v2) {
+        Dual f[3], pd;
+        hair_eval_dual(a.hcd, h.wo_local, dl.wl2, h.hw.h, sigma, f, &pd);
+        const float pb = dl.bp2, pl = dl.lp2;
+#pragma unroll
+        for (int k = 0; k < kAdParams; ++k) r[k] = pb > 0.f ? 2.f * pl * pl * pd.d[k] / (pb * (pb * pb + pl * pl)) : 0.f;
+        add_grad(sc, a, prim, f, W * safe_ratio(dl.c2, dl.f2), dot(W, dl.c2), r, q.gp);
+      }
+    }
+    q.L = q.L + q.beta * sum;
+  }
+  if (depth >= 1) {
+    const float rr = fmaxf(0.05f, 1.f - luminance(q.beta));
+    if (draw(q.rng) < rr) return false;
+    q.beta = q.beta / (1.f - rr);
+  }
+  if (depth + 1 > a.max_vertex) return false;
+  const BsdfSample bs = sample_bsdf(sc, h, q.rng);
+  const V3 wl = bs.wl, wi = bs.wi, f = bs.f;
+  const float pdf = bs.pdf;
+  V3 mul = (pdf == 0.f) ? f : f / pdf;
+  const bool mul_bad = any_nan(mul);
+  if (mul_bad) mul = v3(1.f);
+  if (hair && !mul_bad) {
+    const V3 rem = q.Lt - q.L;  // radiance gathered after this vertex
+    Dual fd[3], pd;
+    hair_eval_dual(a.hcd, h.wo_local, wl, h.hw.h, strand_sigma_of(sc, prim), fd, &pd);
+    float r[kAdParams];
+#pragma unroll
+    for (int k = 0; k < kAdParams; ++k) r[k] = 0.f;
+    add_grad(sc, a, prim, fd, q.adj * safe_ratio(rem, f), 0.f, r, q.gp);
+  }
+  q.beta = q.beta * mul;
+  V3 p = h.p, nd = h.n;
+  if (dot(wi, h.n) < 0.f && hair) {
+    nd = -1.f * h.n;
+    p = p + (2.f * h.radius) * nd;
+  }
+  q.o = p + 1e-3f * nd;
+  q.d = wi;
+  q.depth = depth + 1;
+  return true;
+}
+
+// Waves per SIMD the replay kernel is compiled for. At 512x512 C1, 16 spp,
+// with path regeneration the replay ran at 55.4 M paths/s with 2 waves, 59.3
+// with 3, 62-63 with 4, 54.0 with 5 and 49.8 with 6 (the registers beyond
+// the budget spill to scratch: 784 bytes per lane at 4). Without
+// regeneration (each lane's path run to its end while the wave waited for
+// its longest path) the best was 48.3 with 3 waves.
 #ifndef HM_ADJ_MIN_WAVES
-#define HM_ADJ_MIN_WAVES 3
+#define HM_ADJ_MIN_WAVES 4
 #endif
 __global__ void __launch_bounds__(kTraceBlock, HM_ADJ_MIN_WAVES) k_adjoint(SceneView sc, AdjArgs a) {
   __shared__ int lds_stack[kStackLds * kTraceBlock];
@@ -189,21 +201,47 @@ __global__ void __launch_bounds__(kTraceBlock, HM_ADJ_MIN_WAVES) k_adjoint(Scene
   const long long total = (long long)npx * a.npass;
   float gg[3] = {0.f, 0.f, 0.f};
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-    const int pix = (int)(i % npx), pass = a.pass0 + (int)(i / npx);
-    const int x = pix % sc.width, y = pix / sc.width;
-    const float4 g4 = a.dl_dimg[pix];
-    const V3 adj = a.inv_spp * v3(g4.x, g4.y, g4.z);
-    if (adj.x == 0.f && adj.y == 0.f && adj.z == 0.f) continue;
-    const float4 l = a.ring[i];  // i = (pass - pass0) * npx + pix
-    const V3 Lt = v3(l.x, l.y, l.z);
-    if (any_nan(Lt)) continue;  // writePixel drops NaN samples (utils.cuh:20-21)
-    float gp[3] = {0.f, 0.f, 0.f};
-    adjoint_path(sc, a, st, x, y, pass, Lt, adj, gp);
+  // Path regeneration: each lane replays its paths i, i + stride, ... one
+  // vertex per iteration, and a lane whose path has ended starts its next
+  // one in the same iteration, so the wave's lanes stay busy instead of
+  // waiting for the longest path of the wave.
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x - stride;
+  bool live = false;
+  AdjPath q;
+  for (;;) {
+    if (!live) {
+      for (i += stride; i < total; i += stride) {
+        const int pix = (int)(i % npx), pass = a.pass0 + (int)(i / npx);
+        const float4 g4 = a.dl_dimg[pix];
+        const V3 adj = a.inv_spp * v3(g4.x, g4.y, g4.z);
+        if (adj.x == 0.f && adj.y == 0.f && adj.z == 0.f) continue;
+        const float4 l = a.ring[i];  // i = (pass - pass0) * npx + pix
+        const V3 Lt = v3(l.x, l.y, l.z);
+        if (any_nan(Lt)) continue;  // writePixel drops NaN samples (utils.cuh:20-21)
+        adj_start(sc, pix % sc.width, pix / sc.width, pass, q);
+        q.adj = adj;
+        q.Lt = Lt;
+        q.pix = pix;
+        live = true;
+        break;
+      }
+    }
+    if (!__any(live)) break;
+    if (!live) continue;
+    float t, u;
+    int prim;
+    TraceCounters cnt{0, 0, 0};
+    if (sc.curves)
+      closest_hit<false, true>(sc, q.o, q.d, st, t, prim, u, cnt);
+    else
+      closest_hit<false>(sc, q.o, q.d, st, t, prim, u, cnt);
+    live = prim != kPrimMiss && adj_vertex(sc, a, st, t, prim, u, q);
+    if (!live) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      gg[k] += gp[k];
-      if (a.g_px && gp[k] != 0.f) atomicAdd(&a.g_px[3 * (size_t)pix + k], gp[k]);
+      for (int k = 0; k < 3; ++k) {
+        gg[k] += q.gp[k];
+        if (a.g_px && q.gp[k] != 0.f) atomicAdd(&a.g_px[3 * (size_t)q.pix + k], q.gp[k]);
+      }
     }
   }
 #pragma unroll
