@@ -24,16 +24,29 @@ namespace hm {
 
 namespace {
 
-__device__ __forceinline__ int wave_append(int* counter, bool pred) {
+// Appends to a queue from a 256-thread block: each lane with pred set gets a
+// position. The four waves' counts meet in LDS and one thread makes the
+// block's single atomic (one atomic per wave, every wave of a launch on the
+// same counter, serialised there). Every thread of the block calls it the
+// same number of times.
+__device__ __forceinline__ int block_append1(int* counter, bool pred) {
+  __shared__ int tot[4], base[4];
   const unsigned long long mask = __ballot(pred);
-  const int lane = __lane_id();
-  const int total = __popcll(mask);
-  int base = 0;
-  const int leader = __ffsll((long long)mask) - 1;
-  if (total > 0 && lane == leader) base = atomicAdd(counter, total);
-  base = __shfl(base, leader < 0 ? 0 : leader);
-  const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-  return pred ? base + rank : -1;
+  const int w = threadIdx.x >> 6, lane = __lane_id();
+  if (lane == 0) tot[w] = __popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int sum = tot[0] + tot[1] + tot[2] + tot[3];
+    int b = sum > 0 ? atomicAdd(counter, sum) : 0;
+    for (int k = 0; k < 4; ++k) {
+      base[k] = b;
+      b += tot[k];
+    }
+  }
+  __syncthreads();
+  const int q = pred ? base[w] + __popcll(mask & ((1ull << lane) - 1ull)) : -1;
+  __syncthreads();  // base[] is read before the next call overwrites it
+  return q;
 }
 
 // Queue positions of one vertex after a shading step.
@@ -928,28 +941,24 @@ __device__ __forceinline__ void accumulate_planned(const float4* ring, float4* a
 // entry; the free capacity for the plan; the new jobs' camera rays appended
 // to wavefront nb after the continuing paths.
 __global__ void k_stream_finish_dense(PathBuffers pb, long long* ctl, float4* ring, int npx, int nxt, int slots) {
+  // finished jobs per ring entry, counted in LDS; one global atomic per
+  // block and entry (every block's waves adding to the same few counters
+  // serialised on them)
+  __shared__ int per_entry[kStreamRing];
+  if (threadIdx.x < kStreamRing) per_entry[threadIdx.x] = 0;
+  __syncthreads();
   const int n = pb.counts[kCountStride * kDoneCount];
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl[2] = slots - pb.counts[kCountStride * nxt];  // free path capacity
   const int stride = gridDim.x * blockDim.x;
-  const int nloop = (n + stride - 1) / stride * stride;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
-    const bool done = i < n;
-    int r = -1;
-    if (done) {
-      r = pb.done_pass[i] % kStreamRing;
-      ring[(size_t)r * npx + pb.done_px[i]] = pb.done_L[i];
-    }
-    unsigned long long left = __ballot(done);
-    while (left) {
-      const int lead = __ffsll((long long)left) - 1;
-      const int r0 = __shfl(r, lead);
-      const unsigned long long same = __ballot(done && r == r0);
-      if (__lane_id() == lead) atomicAdd((unsigned long long*)&ctl[8 + r0], (unsigned long long)__popcll(same));
-      left &= ~same;
-    }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int r = pb.done_pass[i] % kStreamRing;
+    ring[(size_t)r * npx + pb.done_px[i]] = pb.done_L[i];
+    atomicAdd(&per_entry[r], 1);
   }
+  __syncthreads();
+  if (threadIdx.x < kStreamRing && per_entry[threadIdx.x] > 0)
+    atomicAdd((unsigned long long*)&ctl[8 + threadIdx.x], (unsigned long long)per_entry[threadIdx.x]);
 }
-
 
 // Plans and runs the next step of a group from scheduler state S (written
 // by the previous call and updated by the finishing kernel in between): every
@@ -984,7 +993,7 @@ __global__ void k_stream_next_dense(SceneView sc, PathBuffers nb, const long lon
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
     if (k > 0 && i < cnt) accumulate_planned(ring, accum, average, acc, k, first_pass, npx, stream_group_pixel(i, g, K));
     const bool go = i < n;
-    const int q = wave_append(&nb.counts[kCountStride * nxt], go);
+    const int q = block_append1(&nb.counts[kCountStride * nxt], go);
     if (go) {
       const long long job = job0 + i;
       const int pass = (int)(job / cnt);
@@ -1210,7 +1219,7 @@ __global__ void k_jobs_next(SceneView sc, PathBuffers nb, const long long* S, lo
   const int nloop = (n + stride - 1) / stride * stride;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nloop; i += stride) {
     const bool go = i < n;
-    const int q = wave_append(&nb.counts[kCountStride * nxt], go);
+    const int q = block_append1(&nb.counts[kCountStride * nxt], go);
     if (go) {
       const int job = (int)(job0 + i);
       if (jc.mode == 1)
