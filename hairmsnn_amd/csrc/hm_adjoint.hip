@@ -244,11 +244,20 @@ __global__ void __launch_bounds__(kTraceBlock, HM_ADJ_MIN_WAVES) k_adjoint(Scene
       }
     }
   }
+  // the global gradients: wave sums, block sums in LDS, one FP64 atomic per
+  // block and parameter
+  __shared__ float part[kTraceBlock / 64][3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     float v = gg[k];
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    if (__lane_id() == 0 && v != 0.f) atomicAdd(&a.g_glob[k], (double)v);
+    if (__lane_id() == 0) part[threadIdx.x >> 6][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double v = 0.0;
+    for (int w = 0; w < kTraceBlock / 64; ++w) v += (double)part[w][threadIdx.x];
+    if (v != 0.0) atomicAdd(&a.g_glob[threadIdx.x], v);
   }
 }
 

@@ -109,6 +109,20 @@ void nrc_release(NrcState* s) { delete s; }
 
 namespace {
 
+// Adds v, summed over a 256-thread block, to *counter with one atomic per
+// block (every wave of a frame-sized launch adding to the one counter
+// serialised there). Every thread of the block calls it.
+__device__ __forceinline__ void block_count(int* counter, int v) {
+  __shared__ int part[4];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if (__lane_id() == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int sum = part[0] + part[1] + part[2] + part[3];
+    if (sum) atomicAdd(counter, sum);
+  }
+}
+
 // RENDER pass, pixel side (cuda/nrc.cu:378-392): radiance before the query
 // plus throughput x max(cache, 0), a NaN cache counting as 0; accumulated
 // like writePixel (utils.cuh:13-35, NaN scrub). Also stores the G-buffer
@@ -136,8 +150,7 @@ __global__ void k_nrc_render(PathBuffers pb, NrcPassArgs nr, const float* cache,
     const float inv = 1.f / (float)(accum_id + 1);
     avg[i] = make_float4(col.x * inv, col.y * inv, col.z * inv, 1.f);
   }
-  const unsigned long long m = __ballot(queried);
-  if (__lane_id() == 0 && m) atomicAdd(&counts[0], __popcll(m));
+  block_count(&counts[0], queried ? 1 : 0);
 }
 
 // RENDER pass, training side (nrcGenerateTrainingData, cuda/nrc.cu:69-133):
@@ -184,9 +197,7 @@ __global__ void k_nrc_train_data(NrcPassArgs nr, const int* train_idx, const flo
       }
     }
   }
-  // wave sum of the records written
-  for (int off = 32; off > 0; off >>= 1) used += __shfl_xor(used, off);
-  if (__lane_id() == 0 && used) atomicAdd(&counts[1], used);
+  block_count(&counts[1], used);  // the records written
 }
 
 struct NrcTimer {
