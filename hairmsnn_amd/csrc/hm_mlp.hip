@@ -123,10 +123,13 @@ __device__ __forceinline__ int fp8_pack4(float a, float b, float c, float d) {
   return __builtin_amdgcn_cvt_pk_fp8_f32(fp8_clamp(c), fp8_clamp(d), w, true);
 }
 
-// Four non-negative-clamped floats -> four E4M3 bytes: med3(x, 0, 448).
-__device__ __forceinline__ int fp8_pack4_relu(float a, float b, float c, float d) {
+// Four non-negative-clamped floats -> four E4M3 bytes: med3(x, 0, 448). The
+// two conversions overwrite both halves of `prev`, whose value does not
+// matter: building the word over the operand it replaces spares the compiler
+// a zeroed copy per word.
+__device__ __forceinline__ int fp8_pack4_relu(int prev, float a, float b, float c, float d) {
   const int w = __builtin_amdgcn_cvt_pk_fp8_f32(__builtin_amdgcn_fmed3f(a, 0.f, kE4M3Max),
-                                                __builtin_amdgcn_fmed3f(b, 0.f, kE4M3Max), 0, false);
+                                                __builtin_amdgcn_fmed3f(b, 0.f, kE4M3Max), prev, false);
   return __builtin_amdgcn_cvt_pk_fp8_f32(__builtin_amdgcn_fmed3f(c, 0.f, kE4M3Max),
                                          __builtin_amdgcn_fmed3f(d, 0.f, kE4M3Max), w, true);
 }
@@ -167,32 +170,41 @@ struct Encoded {
   float f[44];
 };
 
-__device__ __forceinline__ Encoded encode(const float* rec) {
-  // No multiply-add contraction: the features are then the same IEEE
-  // operations wherever this is inlined (the standalone encoder and each
-  // inference kernel), so the standalone encoder's output is exactly what
-  // the FP8 kernel quantises.
+// Frequency features of one coordinate: sin and cos of 2^k pi x for
+// k = 0..5, by the double-angle recurrence from sincospi(x).
+__device__ __forceinline__ void freq_chain(float x, float v[12]) {
 #pragma clang fp contract(off)
+  float s, co;
+  sincospif(x, &s, &co);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    v[2 * k] = s;
+    v[2 * k + 1] = co;
+    const float s2 = 2.f * s * co, c2 = co * co - s * s;  // double angle
+    s = s2;
+    co = c2;
+  }
+}
+
+// Spherical angles of a unit direction, scaled to [0, 1]: theta / pi and
+// phi / (2 pi) + 1/2 (by the reciprocal constants: within an ulp of the
+// quotients).
+__device__ __forceinline__ void sph_angles(float x, float y, float z, float* th, float* ph) {
+#pragma clang fp contract(off)
+  const float kInvPi = 0.318309886183791f, kInv2Pi = 0.159154943091895f;
+  *th = acosf(fminf(1.f, fmaxf(-1.f, z))) * kInvPi;
+  *ph = atan2f(y, x) * kInv2Pi + 0.5f;
+}
+
+__device__ __forceinline__ Encoded encode(const float* rec) {
+  // No multiply-add contraction (in the helpers too): the features are then
+  // the same IEEE operations wherever this is inlined (the standalone encoder
+  // and each inference kernel), so the standalone encoder's output is exactly
+  // what the FP8 kernel quantises.
   Encoded e;
-  const float kPiF = 3.14159265358979f;
-  for (int c = 0; c < 3; ++c) {
-    float s, co;
-    sincospif(rec[c], &s, &co);
+  for (int c = 0; c < 3; ++c) freq_chain(rec[c], &e.f[12 * c]);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      e.f[12 * c + 2 * k] = s;
-      e.f[12 * c + 2 * k + 1] = co;
-      const float s2 = 2.f * s * co, c2 = co * co - s * s;  // double angle
-      s = s2;
-      co = c2;
-    }
-  }
-#pragma unroll
-  for (int v = 0; v < 2; ++v) {
-    const float x = rec[3 + 3 * v], y = rec[4 + 3 * v], z = rec[5 + 3 * v];
-    e.f[36 + 2 * v] = acosf(fminf(1.f, fmaxf(-1.f, z))) / kPiF;
-    e.f[37 + 2 * v] = atan2f(y, x) / (2.f * kPiF) + 0.5f;
-  }
+  for (int v = 0; v < 2; ++v) sph_angles(rec[3 + 3 * v], rec[4 + 3 * v], rec[5 + 3 * v], &e.f[36 + 2 * v], &e.f[37 + 2 * v]);
   e.f[40] = rec[9];
   e.f[41] = rec[10];
   e.f[42] = rec[11];
@@ -201,29 +213,35 @@ __device__ __forceinline__ Encoded encode(const float* rec) {
 }
 
 // encode() for a wave whose lane halves hold the same 32 records (lane and
-// lane ^ 32): each half evaluates the spherical angles of one of the two
-// directions (acosf and atan2f are the costly part) and takes the other's
-// from its partner lane. The features are the same values encode() gives.
-// Every lane of the wave must call it.
+// lane ^ 32), sharing the work between the halves. Lane half g uses features
+// feat_of(s, g, v) only: for the frequency features these are levels k = 0,
+// 1, 4, 5 of coordinate g, levels 2, 3 of coordinate 1 - g, and the levels of
+// coordinate 2 that feat_of picks. So half g runs the recurrence of
+// coordinate g and takes levels 2, 3 of the other from its partner lane; both
+// run coordinate 2. Likewise each half evaluates the spherical angles of one
+// direction (acosf and atan2f are the costly part) and takes the other's
+// from its partner. Entries of f[] that the lane does not use hold other
+// features of the same record (so that a maximum over f[] over both halves
+// is the maximum over the record's features); the ones it uses are the
+// values encode() gives. Every lane of the wave must call it.
 __device__ __forceinline__ Encoded encode_wave(const float* rec, int g) {
-#pragma clang fp contract(off)
   Encoded e;
-  const float kPiF = 3.14159265358979f;
-  for (int c = 0; c < 3; ++c) {
-    float s, co;
-    sincospif(rec[c], &s, &co);
+  float P[12];
+  freq_chain(g ? rec[1] : rec[0], P);
+  freq_chain(rec[2], &e.f[24]);
+  float X[4];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      e.f[12 * c + 2 * k] = s;
-      e.f[12 * c + 2 * k + 1] = co;
-      const float s2 = 2.f * s * co, c2 = co * co - s * s;  // double angle
-      s = s2;
-      co = c2;
-    }
+  for (int k = 0; k < 4; ++k) X[k] = __shfl_xor(P[4 + k], 32);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    // coordinate 0: half 0 owns it, half 1 takes levels 2, 3 from half 0;
+    // coordinate 1 the other way round
+    e.f[k] = e.f[12 + k] = P[k];
+    e.f[4 + k] = e.f[16 + k] = X[k];
+    e.f[8 + k] = e.f[20 + k] = P[8 + k];
   }
-  const float x = g ? rec[6] : rec[3], y = g ? rec[7] : rec[4], z = g ? rec[8] : rec[5];
-  const float th = acosf(fminf(1.f, fmaxf(-1.f, z))) / kPiF;
-  const float ph = atan2f(y, x) / (2.f * kPiF) + 0.5f;
+  float th, ph;
+  sph_angles(g ? rec[6] : rec[3], g ? rec[7] : rec[4], g ? rec[8] : rec[5], &th, &ph);
   const float th_o = __shfl_xor(th, 32), ph_o = __shfl_xor(ph, 32);
   e.f[36] = g ? th_o : th;
   e.f[37] = g ? ph_o : ph;
@@ -459,15 +477,28 @@ __global__ void k_mlp_act_scales(const unsigned* amax, int* scales) {
   if (l < 6) scales[6 + l] = fp8_exponent(2.f * __uint_as_float(amax[l]));
 }
 
-__device__ __forceinline__ f32x16 mfma_fp8(const i32x8& a, const i32x8& b, f32x16 c, int ea, int eb) {
-  return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, 127 + ea, 0, 127 + eb);
+// FP8 x FP8 with E8M0 block scales: e holds the biased exponents (127 + e)
+// of the weight and input operands.
+__device__ __forceinline__ f32x16 mfma_fp8b(const i32x8& a, const i32x8& b, int2 e) {
+  return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, f32x16{}, 0, 0, 0, e.x, 0, e.y);
 }
 
 // scales: [0..5] weight exponents, [6..11] layer-input exponents.
 __global__ void __launch_bounds__(kInferBlock) k_mlp_infer_fp8(const i32x8* frag, const int* scales, const float* in,
                                                              int n, float* out) {
   __shared__ i32x8 lds[kFrag8Total];
+  // Biased block-scale exponents of the six matrix instructions (weight
+  // operand, input operand), read once per block: loaded from global memory
+  // inside the layer loop they would put a memory round trip before every
+  // layer (the compiler cannot hoist them above the output stores).
+  __shared__ int2 lexp[6];
   for (int i = threadIdx.x; i < kFrag8Total; i += blockDim.x) lds[i] = frag[i];
+  if (threadIdx.x < 6) {
+    const int l = threadIdx.x;
+    const int ew = l < 5 ? scales[l] - scales[7 + l] : scales[5];
+    lexp[l] = make_int2(127 + ew, 127 + scales[6 + l]);
+  }
+  const float in_scale = ldexpf(1.f, -scales[6]);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 5, col = lane & 31;
@@ -487,7 +518,7 @@ __global__ void __launch_bounds__(kInferBlock) k_mlp_infer_fp8(const i32x8* frag
     const Encoded e = encode_wave(rec, g);
     i32x8 B;
     {
-      const float sc = ldexpf(1.f, -scales[6]);
+      const float sc = in_scale;
       float x[32];
 #pragma unroll
       for (int b = 0; b < 32; ++b) {
@@ -504,16 +535,17 @@ __global__ void __launch_bounds__(kInferBlock) k_mlp_infer_fp8(const i32x8* frag
       // exponent of this layer's matrix instruction (a power of two, so the
       // accumulator is the scaled sum exactly); ReLU and the clamp to the
       // E4M3 range are then one med3 per value.
-      const int ew = scales[l] - scales[7 + l], ex = scales[6 + l];
-      const f32x16 a0 = mfma_fp8(lds[(l * 2 + 0) * 64 + lane], B, f32x16{}, ew, ex);
-      const f32x16 a1 = mfma_fp8(lds[(l * 2 + 1) * 64 + lane], B, f32x16{}, ew, ex);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) B[k] = fp8_pack4_relu(a0[4 * k], a0[4 * k + 1], a0[4 * k + 2], a0[4 * k + 3]);
+      const int2 ex = lexp[l];
+      const f32x16 a0 = mfma_fp8b(lds[(l * 2 + 0) * 64 + lane], B, ex);
+      const f32x16 a1 = mfma_fp8b(lds[(l * 2 + 1) * 64 + lane], B, ex);
 #pragma unroll
       for (int k = 0; k < 4; ++k)
-        B[4 + k] = fp8_pack4_relu(a1[4 * k], a1[4 * k + 1], a1[4 * k + 2], a1[4 * k + 3]);
+        B[k] = fp8_pack4_relu(B[k], a0[4 * k], a0[4 * k + 1], a0[4 * k + 2], a0[4 * k + 3]);
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        B[4 + k] = fp8_pack4_relu(B[4 + k], a1[4 * k], a1[4 * k + 1], a1[4 * k + 2], a1[4 * k + 3]);
     }
-    f32x16 z = mfma_fp8(lds[kFrag8Hidden + lane], B, f32x16{}, scales[5], scales[11]);
+    f32x16 z = mfma_fp8b(lds[kFrag8Hidden + lane], B, lexp[5]);
     // Keep the matrix instruction (and the copies that assemble its operands)
     // ahead of the lane-divergent store below: it reads all 64 lanes' operands,
     // and sunk into the branch its operand copies would run for half the wave.
