@@ -353,7 +353,7 @@ int Context::init(std::string& err) {
     const int builder = bvh_builder_id(scene.bvh_builder);
     if (hair.crv.empty()) {
       HM_TRY(bvh_build(seg_dev, (int)(hair.seg.size() / 8), bvh, stream, err, builder, scene.ploc_radius,
-                       scene.bvh_max_leaf));
+                       scene.bvh_max_leaf, scene.bvh_split));
     } else {
       // round curves: the BVH over each segment's box (the Bezier hull of
       // the Catmull-Rom segment, padded by the radius bound), stored as
@@ -602,7 +602,7 @@ int hmsnn_get_info(const hmsnn_ctx* ctx, hmsnn_info* info) {
   const Context& c = ctx->c;
   std::memset(info, 0, sizeof(*info));
   fill_scene_info(c.scene, info);
-  info->num_segments = c.bvh.num_segs;
+  info->num_segments = c.bvh.num_prims;
   info->num_curves = (int)(c.hair.crv.size() / kCurveFloats);
   info->num_strands = c.hair.num_strands;
   info->num_triangles = c.mesh.num();

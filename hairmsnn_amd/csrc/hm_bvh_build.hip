@@ -38,7 +38,8 @@ __device__ __forceinline__ float ord2f(int i) { return __int_as_float(i >= 0 ? i
 
 struct BuildBuffers {
   const float* seg_in;  // 8 floats per segment
-  int n;
+  int n;                // leaves: segments x pieces
+  int pieces;           // boxes per segment (leaf i is piece i % pieces of segment i / pieces)
   float* leaf_box;      // 6 floats per segment, padded
   int* cbounds;         // 6 ordered ints: min xyz, max xyz
   unsigned* keys;
@@ -57,11 +58,17 @@ __global__ void k_leaf_boxes(BuildBuffers b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int lmin[3] = {INT_MAX, INT_MAX, INT_MAX}, lmax[3] = {INT_MIN, INT_MIN, INT_MIN};
   if (i < b.n) {
-    const float* s = b.seg_in + 8 * (size_t)i;
+    const int P = b.pieces, piece = i % P;
+    const float* s = b.seg_in + 8 * (size_t)(i / P);
     const float r = s[3];
+    const float t0 = (float)piece / (float)P, t1 = (float)(piece + 1) / (float)P;
     float lo[3], hi[3];
     for (int k = 0; k < 3; ++k) {
-      const float a = s[k], c = s[4 + k];
+      // the piece's end points (the segment's own at its ends); the relative
+      // padding below covers the rounding of the interpolation
+      const float d = s[4 + k] - s[k];
+      const float a = piece == 0 ? s[k] : fmaf(d, t0, s[k]);
+      const float c = piece == P - 1 ? s[4 + k] : fmaf(d, t1, s[k]);
       const float pad = r + 1e-6f * (fabsf(a) + fabsf(c) + r) + 1e-7f;
       lo[k] = fminf(a, c) - pad;
       hi[k] = fmaxf(a, c) + pad;
@@ -218,7 +225,7 @@ __global__ void k_emit(BuildBuffers b, BvhNode* nodes, float4* segs_out) {
     nodes[i] = nd;
   }
   if (i < b.n) {
-    const int orig = b.vals_sorted[i];
+    const int orig = b.vals_sorted[i] / b.pieces;
     const float* s = b.seg_in + 8 * (size_t)orig;
     segs_out[2 * (size_t)i] = make_float4(s[0], s[1], s[2], s[3]);
     segs_out[2 * (size_t)i + 1] = make_float4(s[4], s[5], s[6], __int_as_float(orig));
@@ -532,13 +539,20 @@ static int bvh_collapse(DeviceBvh& out, hipStream_t stream, std::string& err) {
 // Builds the BVH from device segments (8 floats each). All scratch is
 // released before returning; the result owns nodes and segment records.
 int bvh_build(const float* seg_dev, int n, DeviceBvh& out, hipStream_t stream, std::string& err, int builder,
-              int ploc_radius, int max_leaf) {
+              int ploc_radius, int max_leaf, int split) {
   out.release();
-  out.num_segs = n;
-  if (n > kLeafPosMask + 1) {
-    err = "BVH: at most 16,777,216 primitives";
+  out.num_prims = std::max(n, 0);
+  if (split < 1 || split > 4) {
+    err = "BVH: split must be 1..4";
     return kEInval;
   }
+  const int pieces = (builder == kBvhSah && n > 1) ? split : 1;
+  if ((long long)n * pieces > kLeafPosMask + 1) {
+    err = "BVH: at most 16,777,216 leaf boxes (primitives x accel.split)";
+    return kEInval;
+  }
+  n *= pieces;  // from here on: leaves
+  out.num_segs = n;
   if (n <= 0) {
     out.max_depth = 0;
     return 0;
@@ -557,6 +571,7 @@ int bvh_build(const float* seg_dev, int n, DeviceBvh& out, hipStream_t stream, s
   BuildBuffers b{};
   b.seg_in = seg_dev;
   b.n = n;
+  b.pieces = pieces;
   HM_TRY(arena.alloc((void**)&b.leaf_box, sizeof(float) * 6 * (size_t)n, err));
   HM_TRY(arena.alloc((void**)&b.cbounds, sizeof(int) * 8, err));
   HM_TRY(arena.alloc((void**)&b.keys, sizeof(unsigned) * (size_t)n, err));

@@ -74,8 +74,9 @@ struct BvhNodeWide;
 struct DeviceBvh {
   BvhNode* nodes = nullptr;    // binary tree (build intermediate, kept for tools)
   BvhNodeWide* nodes_w = nullptr;  // wide tree that traversal reads
-  float4* segs = nullptr;      // 2 per segment, Morton order
-  int num_segs = 0;
+  float4* segs = nullptr;      // 2 per leaf record (a segment, or a copy of it per piece), in leaf order
+  int num_segs = 0;    // leaf records
+  int num_prims = 0;   // primitives (segments, curves or triangles)
   int num_nodes = 0;   // binary inner nodes
   int num_nodes_w = 0;  // wide nodes
   int max_depth = 0;   // binary depth
@@ -89,7 +90,7 @@ struct DeviceBvh {
     nodes = nullptr;
     nodes_w = nullptr;
     segs = nullptr;
-    num_segs = num_nodes = num_nodes_w = max_depth = depth_w = 0;
+    num_segs = num_prims = num_nodes = num_nodes_w = max_depth = depth_w = 0;
   }
   ~DeviceBvh() { release(); }
 };
@@ -106,8 +107,12 @@ constexpr int kBvhLbvh = 0, kBvhPloc = 1, kBvhSah = 2;
 void sah_build(const float* boxes, int n, int* child, int* order, int* parent_inner, int* parent_leaf,
                int max_leaf = 1);
 // max_leaf applies to the SAH builder; the others make one-primitive leaves.
+// split > 1 (SAH builder, linear segments only) builds the tree over `split`
+// boxes per segment, one per equal piece of its length, each padded by the
+// radius; every leaf of a piece holds the whole segment's record, so a
+// segment can sit in several leaves and hits are those of the unsplit tree.
 int bvh_build(const float* seg_dev, int n, DeviceBvh& out, hipStream_t stream, std::string& err,
-              int builder = kBvhLbvh, int ploc_radius = 16, int max_leaf = 1);
+              int builder = kBvhLbvh, int ploc_radius = 16, int max_leaf = 1, int split = 1);
 
 // Environment map and its importance-sampling tables (RGBA32F map, FP32
 // tables produced from FP64 prefix sums).

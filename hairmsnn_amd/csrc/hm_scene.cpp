@@ -296,6 +296,11 @@ bool parse_json(const Json& cfg, Scene& scene, std::string& err) {
       err = "accel.max_leaf must be 1..8";
       return false;
     }
+    if (a.has("split")) scene.bvh_split = a.at("split").i();
+    if (scene.bvh_split < 1 || scene.bvh_split > 4) {
+      err = "accel.split must be 1..4";
+      return false;
+    }
     if (scene.bvh_builder != "lbvh" && scene.bvh_builder != "ploc" && scene.bvh_builder != "sah") {
       err = "accel.builder must be \"lbvh\", \"ploc\" or \"sah\"";
       return false;

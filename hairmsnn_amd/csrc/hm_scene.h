@@ -115,6 +115,7 @@ struct Scene {
   std::string bvh_builder = "sah";  // accel.builder: "sah" (host, binned SAH), "lbvh", "ploc"
   int ploc_radius = 16;
   int bvh_max_leaf = 1;  // accel.max_leaf: primitives per leaf run of the SAH tree (1..8)
+  int bvh_split = 1;     // accel.split: boxes per linear hair segment in the SAH tree (1..4)
 
   // neural radiance caching extension block (render_nrc.h:112-134 defaults)
   float nrc_c = 0.01f;              // spread threshold multiplier

@@ -153,3 +153,18 @@ def test_procedural_configs_parse():
     for name, segs in [("c0_hairball.json", 2000 * 16), ("c1_curly.json", 50000 * 64)]:
         info = hm.parse_scene((ROOT / "hairmsnn_amd" / "scenes" / name).read_text())
         assert info["num_segments"] == segs
+
+
+@pytest.mark.parametrize("accel,msg", [({"max_leaf": 0}, "max_leaf must be 1..8"), ({"max_leaf": 9}, "max_leaf must be 1..8"),
+                                       ({"split": 0}, "split must be 1..4"), ({"split": 5}, "split must be 1..4"),
+                                       ({"builder": "kd"}, "accel.builder must be")])
+def test_accel_options_out_of_range(accel, msg):
+    """The acceleration-structure extension block rejects what the builders
+    do not support, at parse time."""
+    import hairmsnn_amd as hm
+    cfg = base_cfg()
+    cfg["accel"] = accel
+    with pytest.raises(hm.HmsnnError, match=msg):
+        hm.parse_scene(json.dumps(cfg))
+    cfg["accel"] = {"builder": "sah", "max_leaf": 8, "split": 4}
+    assert hm.parse_scene(json.dumps(cfg))["width"] > 0

@@ -119,15 +119,18 @@ def test_ploc_tree_bit_exact(gpu, name, radius):
     np.testing.assert_array_equal(occ, occ_c)
 
 
-@pytest.mark.parametrize("name,max_leaf", [("c0_hairball.json", 1), ("c1_curly.json", 1), ("c0_hairball.json", 3),
-                                           ("c1_curly.json", 8)])
-def test_sah_tree_bit_exact(gpu, name, max_leaf):
+@pytest.mark.parametrize("name,max_leaf,split", [("c0_hairball.json", 1, 1), ("c1_curly.json", 1, 1),
+                                                 ("c0_hairball.json", 3, 1), ("c1_curly.json", 8, 1),
+                                                 ("c1_curly.json", 1, 2), ("c0_hairball.json", 2, 4)])
+def test_sah_tree_bit_exact(gpu, name, max_leaf, split):
     """The binned-SAH tree built on the host (accel.builder = "sah") gives the
-    same closest and any hits as the CPU BVH, with one primitive per leaf and
-    with leaves that are runs of up to max_leaf primitives."""
+    same closest and any hits as the CPU BVH, with one primitive per leaf,
+    with leaves that are runs of up to max_leaf primitives, and with each
+    segment's box split into `split` pieces along its length (every piece's
+    leaf holding the whole segment)."""
     cfg = json.loads((SCENES / name).read_text())
     cfg["integrator"].update({"width": 128, "height": 128})
-    cfg["accel"] = {"builder": "sah", "max_leaf": max_leaf}
+    cfg["accel"] = {"builder": "sah", "max_leaf": max_leaf, "split": split}
     cfg.pop("surface", None)
     if "procedural" in cfg["hair"]:
         cfg["hair"]["procedural"]["strands"] = min(cfg["hair"]["procedural"]["strands"], 5000)
@@ -137,9 +140,12 @@ def test_sah_tree_bit_exact(gpu, name, max_leaf):
         prim, t, u = sc.trace_rays(rays)
         occ = sc.trace_rays(rays, any_hit=True)
         depth = int(sc.info["bvh_depth"])
+        nseg = int(sc.info["num_segments"])
     prim_c, t_c, u_c = ol.intersect(seg, rays, use_bvh=True)
     hit = prim_c >= 0
-    print(f"{name} SAH, runs of up to {max_leaf}: wide depth {depth}, hit fraction {hit.mean():.3f}")
+    print(f"{name} SAH, runs of up to {max_leaf}, {split} boxes per segment: wide depth {depth}, "
+          f"hit fraction {hit.mean():.3f}")
+    assert seg.shape[0] == nseg
     assert hit.mean() > 0.1
     np.testing.assert_array_equal(prim, prim_c)
     np.testing.assert_array_equal(t[hit], t_c[hit])
