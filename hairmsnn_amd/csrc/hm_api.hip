@@ -282,7 +282,10 @@ int Context::init(std::string& err) {
   hipDeviceProp_t prop;
   HM_HIP(hipGetDeviceProperties(&prop, dev), err);
   num_cus = prop.multiProcessorCount;
-  trace_blocks = std::max(64, num_cus * 8);
+  // traversal blocks resident per CU: 8 (8 waves per SIMD), fewer when the
+  // blocks' LDS stacks do not fit the CU's 160 KB together
+  const int lds_blocks = (160 * 1024) / (kStackLds * kTraceBlock * (int)sizeof(int));
+  trace_blocks = std::max(64, num_cus * std::max(1, std::min(8, lds_blocks)));
 
   // ---- geometry ----
   if ((int)scene.dlight_from.size() > kMaxDirLights) {

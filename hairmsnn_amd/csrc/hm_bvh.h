@@ -63,7 +63,11 @@ struct MeshView {
   int num;
 };
 
-constexpr int kStackLds = 16;
+// Stack entries per lane kept in LDS (4 bytes x 256 lanes each per block).
+#ifndef HM_STACK_LDS
+#define HM_STACK_LDS 16
+#endif
+constexpr int kStackLds = HM_STACK_LDS;
 constexpr int kTraceBlock = 256;
 
 // Launch bounds of the traversal kernels: 256 threads, and a floor of
