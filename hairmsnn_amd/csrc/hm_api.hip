@@ -267,7 +267,7 @@ PassConfig Context::pt_pass_config() const {
     c.dl_from = 1 << 30;
   }
   c.trace_blocks = trace_blocks;
-  c.shade_blocks = trace_blocks;
+  c.shade_blocks = std::max(64, num_cus * 8);
   return c;
 }
 
@@ -283,9 +283,15 @@ int Context::init(std::string& err) {
   HM_HIP(hipGetDeviceProperties(&prop, dev), err);
   num_cus = prop.multiProcessorCount;
   // traversal blocks resident per CU: 8 (8 waves per SIMD), fewer when the
-  // blocks' LDS stacks do not fit the CU's 160 KB together
+  // blocks' LDS stacks do not fit the CU's 160 KB together; the traversal
+  // grid is that many blocks per CU unless HM_TRACE_GRID_PER_CU says more
   const int lds_blocks = (160 * 1024) / (kStackLds * kTraceBlock * (int)sizeof(int));
-  trace_blocks = std::max(64, num_cus * std::max(1, std::min(8, lds_blocks)));
+  const int resident = std::max(1, std::min(8, lds_blocks));
+#ifdef HM_TRACE_GRID_PER_CU
+  trace_blocks = std::max(64, num_cus * HM_TRACE_GRID_PER_CU);
+#else
+  trace_blocks = std::max(64, num_cus * resident);
+#endif
 
   // ---- geometry ----
   if ((int)scene.dlight_from.size() > kMaxDirLights) {
