@@ -104,6 +104,9 @@ typedef int i32x8 __attribute__((ext_vector_type(8)));
 constexpr int kFrag8Hidden = 5 * 2 * 64;
 constexpr int kFrag8Total = kFrag8Hidden + 64;
 constexpr float kE4M3Max = 448.f;
+// Calibration amax counters, one per layer input, each on its own 128-byte
+// line (in unsigned words).
+constexpr int kAmaxStride = 32;
 
 // Power-of-two exponent e with amax / 2^e < 448 (0 for an all-zero layer),
 // limited to [-63, 63] so that the difference of two exponents still fits the
@@ -463,18 +466,28 @@ __global__ void __launch_bounds__(kInferBlock) k_mlp_act_amax(const T* frag, con
       if (valid) mx[l + 1] = fmaxf(mx[l + 1], m);
     }
   }
+  // wave maxima, then the block's in LDS, then one atomic per block and layer
+  // (per-wave atomics on the six counters serialise in one L2 channel)
+  __shared__ float red[kInferBlock / 64][6];
 #pragma unroll
   for (int l = 0; l < 6; ++l) {
     float v = mx[l];
     for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
-    if (lane == 0) atomicMax(&amax[l], __float_as_uint(v));  // non-negative floats order as integers
+    if (lane == 0) red[wave][l] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    float v = 0.f;
+    for (int w = 0; w < kInferBlock / 64; ++w) v = fmaxf(v, red[w][threadIdx.x]);
+    // non-negative floats order as integers
+    atomicMax(&amax[threadIdx.x * kAmaxStride], __float_as_uint(v));
   }
 }
 
 // Layer-input exponents scales[6..11] from the calibration amax, 2x margin.
 __global__ void k_mlp_act_scales(const unsigned* amax, int* scales) {
   const int l = threadIdx.x;
-  if (l < 6) scales[6 + l] = fp8_exponent(2.f * __uint_as_float(amax[l]));
+  if (l < 6) scales[6 + l] = fp8_exponent(2.f * __uint_as_float(amax[l * kAmaxStride]));
 }
 
 // FP8 x FP8 with E8M0 block scales: e holds the biased exponents (127 + e)
@@ -972,7 +985,7 @@ struct MlpState {
   // FP8 inference state
   void* frag8 = nullptr;
   int* scales = nullptr;      // [0..5] weight exponents, [6..11] layer-input exponents
-  unsigned* amax = nullptr;   // calibration amax per layer input
+  unsigned* amax = nullptr;   // calibration amax per layer input (stride kAmaxStride)
   bool calib_dirty = true;    // weights changed since the last calibration
   bool fp8() const { return precision == HMSNN_FP8; }
   bool bf16() const { return precision == HMSNN_BF16 || precision == HMSNN_FP8; }
@@ -1022,7 +1035,7 @@ int mlp_create(int precision, uint64_t seed, MlpState** out, std::string& err) {
   if (!rc) rc = dev_alloc((void**)&s->loss_sum, sizeof(float) * 4, err);
   if (!rc && s->fp8()) rc = dev_alloc(&s->frag8, sizeof(i32x8) * kFrag8Total, err);
   if (!rc && s->fp8()) rc = dev_alloc((void**)&s->scales, sizeof(int) * 12, err);
-  if (!rc && s->fp8()) rc = dev_alloc((void**)&s->amax, sizeof(unsigned) * 6, err);
+  if (!rc && s->fp8()) rc = dev_alloc((void**)&s->amax, sizeof(unsigned) * 6 * kAmaxStride, err);
   if (rc) {
     delete s;
     return rc;
@@ -1095,7 +1108,7 @@ int mlp_forward(MlpState& s, const float* in, int n, float* out, hipStream_t st,
   if (s.fp8()) {
     if (s.calib_dirty) {
       const int nc = std::min(n, kFp8CalibQueries);
-      HM_HIP(hipMemsetAsync(s.amax, 0, sizeof(unsigned) * 6, st), err);
+      HM_HIP(hipMemsetAsync(s.amax, 0, sizeof(unsigned) * 6 * kAmaxStride, st), err);
       hipLaunchKernelGGL(k_mlp_act_amax<__bf16>, dim3(mlp_blocks(nc)), dim3(kInferBlock), 0, st, (const __bf16*)s.frag,
                          in, nc, s.amax);
       hipLaunchKernelGGL(k_mlp_act_scales, dim3(1), dim3(64), 0, st, s.amax, s.scales);
